@@ -1,0 +1,179 @@
+/*
+ * gbi.h -- C ABI of the MI355X (gfx950) integral Generalized-Born engine.
+ *
+ * This is the drop-in boundary for the hot path of ymwangg/OpenMMCustomGBIntegralForce:
+ * the kernel interfaces CalcCharmmGBSWForceKernel and CalcCharmmGBMVForceKernel
+ * (openmmapi/include/openmm/charmmKernels.h:49-129) reduced to plain pointers and sizes.
+ * No exception crosses this boundary; every call returns a gbi_status and
+ * gbi_last_error() describes the last failure of the calling thread.
+ *
+ * Units: nm, kJ/mol, proton charge.  Positions and forces are N x 3 row-major
+ * doubles (the Reference platform's vector<Vec3> layout).  On the device, forces are
+ * accumulated in 64-bit fixed point with scale 2^32 (the OpenMM CUDA force-buffer
+ * convention), so results are bitwise reproducible.
+ *
+ * Threading: a context is driven by one host thread at a time; all device work of a
+ * context is ordered on its HIP stream.
+ */
+#ifndef GBI_H
+#define GBI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GBI_ABI_VERSION 1
+
+typedef enum gbi_status {
+    GBI_OK = 0,
+    GBI_ERR_INVALID_ARGUMENT = 1,
+    GBI_ERR_UNSUPPORTED = 2,       /* e.g. CustomIntegral (type 3) or a cutoff method */
+    GBI_ERR_PARTICLE_COUNT = 3,    /* updateParametersInContext with another N */
+    GBI_ERR_HIP = 4,               /* a HIP runtime call failed */
+    GBI_ERR_OUT_OF_MEMORY = 5,
+    GBI_ERR_GRID_TOO_LARGE = 6,    /* lookup grid longer than 1000 nm on an axis */
+    GBI_ERR_NONFINITE = 7,         /* NaN/Inf in the inputs or the result */
+    GBI_ERR_NO_DEVICE = 8,
+    GBI_ERR_CAPACITY = 9           /* internal: grid buffers grew; the call was redone */
+} gbi_status;
+
+/* Which reference force the context evaluates. */
+typedef enum gbi_model {
+    GBI_MODEL_GBSW = 0, /* CharmmGBSWForce (CharmmReferenceGBSW.cpp) */
+    GBI_MODEL_GBMV = 1  /* CharmmGBMVForce with the canonical GB energy (CharmmReferenceGBMV.cpp) */
+} gbi_model;
+
+/* CharmmGBMVForce::GBIntegralType (CharmmGBMVForce.h:99-104). */
+typedef enum gbi_integral_type {
+    GBI_INTEGRAL_GBSW = 0,
+    GBI_INTEGRAL_GBMV1 = 1,
+    GBI_INTEGRAL_GBMV2 = 2,
+    GBI_INTEGRAL_CUSTOM = 3 /* unimplemented in the reference; rejected */
+} gbi_integral_type;
+
+/* Which atoms a quadrature point sees. */
+typedef enum gbi_lookup_policy {
+    GBI_LOOKUP_REFERENCE = 0, /* the reference lookup grid of the model (default) */
+    GBI_LOOKUP_BRUTE = 1      /* every atom (the reference's computeVolume paths); small N only */
+} gbi_lookup_policy;
+
+/* CalcCharmmGBMVForceKernel::NonbondedMethod (charmmKernels.h:52-56). */
+typedef enum gbi_nonbonded_method {
+    GBI_NO_CUTOFF = 0,
+    GBI_CUTOFF_NON_PERIODIC = 1, /* not yet supported on this platform */
+    GBI_CUTOFF_PERIODIC = 2      /* not yet supported on this platform */
+} gbi_nonbonded_method;
+
+/*
+ * All constants of the path.  gbi_params_default() fills the reference values; every
+ * field may be changed before gbi_create().
+ */
+typedef struct gbi_params {
+    int32_t model;          /* gbi_model */
+    int32_t integral_type;  /* gbi_integral_type (GBMV model; GBSW model uses GBSW) */
+    int32_t lookup_policy;  /* gbi_lookup_policy */
+    int32_t nonbonded_method;
+    double cutoff;          /* nm (ignored for GBI_NO_CUTOFF) */
+    /* quadrature: n_radial Gauss-Legendre points on [r0, r1] x n_lebedev directions */
+    int32_t n_radial;
+    int32_t n_lebedev;      /* 6, 14, 26, 38 or 50 */
+    double r0, r1;
+    /* lookup grid */
+    double grid_length;     /* nm */
+    double buffer;          /* nm */
+    int32_t cell_cap;       /* atoms kept per cell (<= 0: uncapped) */
+    int32_t orders[2];      /* integral orders; the canonical Born map uses {4, 7} */
+    /* GBSW smoothing: R' = rscale (R + roffset), switching half-width sw */
+    double sw, rscale, roffset;
+    /* GBMV-I: gamma0, beta, lambda; GBMV-II: beta, lambda, P1, P2, S0 */
+    double gamma0, beta, lambda, P1, P2, S0;
+    double vguard;          /* GBMV-II: |v|^2 below this means S = +inf (V = 1) */
+    /* Born map B = slope / (alpha0 (1/eta - I4) + alpha1 (1/(4 eta^4) - I7)^(1/4)) + shift */
+    double alpha0, alpha1, eta, slope, shift;
+    /* energy E = sum_i P q_i^2/(2 B_i) + sum_{i<j} P q_i q_j / f_ij */
+    double prefactor;
+    /* performance knobs (0 = automatic) */
+    double tight_grid_length; /* GBSW model: cell size of the exact neighbour grid */
+    int32_t block_volume;     /* threads per workgroup of the volume pass */
+    int32_t block_force;      /* threads per workgroup of the chain-rule pass */
+} gbi_params;
+
+typedef struct gbi_stats {
+    double n_points;      /* quadrature points with a non-empty neighbour list */
+    double n_vol_pairs;   /* point/atom pairs scanned by the volume pass */
+    double n_cells;       /* lookup cells */
+    double n_entries;     /* cell list entries */
+    double n_atoms;
+    double n_quad;        /* quadrature points per atom */
+} gbi_stats;
+
+/* Device view of a context's buffers (for callers that keep data on the GPU). */
+typedef struct gbi_device_view {
+    double* positions;    /* N x 3 doubles */
+    int64_t* forces;      /* 3 x N fixed point (x[N] | y[N] | z[N]), scale 2^32 */
+    double* energy;       /* 1 double */
+    double* born_radii;   /* N doubles */
+    double* integrals;    /* 2 x N doubles (integral-major) */
+    void* stream;         /* hipStream_t of the context */
+} gbi_device_view;
+
+typedef struct gbi_ctx gbi_ctx;
+
+/* Flags of gbi_compute. */
+#define GBI_FORCES 1u
+#define GBI_ENERGY 2u
+
+/* Reference defaults of a model (constructors of GBSWIntegral / GBMVIntegralType{I,II},
+ * CharmmReferenceGBSW.cpp:20-52). */
+gbi_status gbi_params_default(int32_t model, int32_t integral_type, gbi_params* out);
+
+/*
+ * Create a context: replaces CalcCharmmGB{SW,MV}ForceKernel::initialize(System, Force)
+ * (charmmKernels.h:71,111).  charge / radius: N doubles (copied).
+ */
+gbi_status gbi_create(int32_t n_atoms, const double* charge, const double* radius,
+                      const gbi_params* params, gbi_ctx** out);
+
+/*
+ * One evaluation: replaces CalcCharmmGB{SW,MV}ForceKernel::execute(context, includeForces,
+ * includeEnergy) (charmmKernels.h:81,121).  pos: N x 3 host doubles.  Forces are ADDED to
+ * `forces` (N x 3 host doubles, may be NULL when include_forces is 0); the energy is written
+ * to *energy (may be NULL).  Synchronous.
+ */
+gbi_status gbi_execute(gbi_ctx* ctx, const double* pos, int32_t include_forces, int32_t include_energy,
+                       double* energy, double* forces);
+
+/*
+ * Replaces copyParametersToContext (charmmKernels.h:88,128): new charges and radii for the
+ * same N; GBI_ERR_PARTICLE_COUNT when n_atoms differs.
+ */
+gbi_status gbi_update_parameters(gbi_ctx* ctx, int32_t n_atoms, const double* charge, const double* radius);
+
+/* Asynchronous pieces of gbi_execute for device-resident callers. */
+gbi_status gbi_set_positions(gbi_ctx* ctx, const double* pos, int32_t pos_on_device);
+gbi_status gbi_compute(gbi_ctx* ctx, uint32_t flags);   /* enqueue one evaluation */
+gbi_status gbi_synchronize(gbi_ctx* ctx);                /* wait + check device flags */
+gbi_status gbi_get_energy(gbi_ctx* ctx, double* energy);
+gbi_status gbi_get_forces(gbi_ctx* ctx, double* forces, int32_t add);
+gbi_status gbi_get_born_radii(gbi_ctx* ctx, double* born);
+gbi_status gbi_get_integrals(gbi_ctx* ctx, double* integrals); /* 2 x N */
+gbi_status gbi_get_stats(gbi_ctx* ctx, gbi_stats* stats);
+gbi_status gbi_get_device_view(gbi_ctx* ctx, gbi_device_view* view);
+gbi_status gbi_set_stream(gbi_ctx* ctx, void* hip_stream); /* NULL: the context's own stream */
+
+void gbi_destroy(gbi_ctx* ctx);
+const char* gbi_last_error(void);
+int32_t gbi_abi_version(void);
+
+/*
+ * Quadrature tables of the library (for inspection and tests): Q x 5 (x, y, z, r, w),
+ * index = radial * n_lebedev + angular, as GBSWIntegral.cpp:41-56.
+ */
+gbi_status gbi_quadrature(int32_t n_radial, double r0, double r1, int32_t n_lebedev, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GBI_H */

@@ -1,0 +1,241 @@
+"""ctypes binding of the C ABI in ``include/gbi.h`` (``lib/libgbi.so``).
+
+There is no fallback: if the library is missing or cannot be loaded, every entry
+point raises.  Build it with ``python -m openmmcustomgbintegralforce_amd.build``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libgbi.so")
+
+GBI_OK = 0
+STATUS_NAMES = {
+    0: "GBI_OK", 1: "GBI_ERR_INVALID_ARGUMENT", 2: "GBI_ERR_UNSUPPORTED", 3: "GBI_ERR_PARTICLE_COUNT",
+    4: "GBI_ERR_HIP", 5: "GBI_ERR_OUT_OF_MEMORY", 6: "GBI_ERR_GRID_TOO_LARGE", 7: "GBI_ERR_NONFINITE",
+    8: "GBI_ERR_NO_DEVICE", 9: "GBI_ERR_CAPACITY",
+}
+MODEL_GBSW, MODEL_GBMV = 0, 1
+INTEGRAL_GBSW, INTEGRAL_GBMV1, INTEGRAL_GBMV2, INTEGRAL_CUSTOM = 0, 1, 2, 3
+LOOKUP_REFERENCE, LOOKUP_BRUTE = 0, 1
+FORCES, ENERGY = 1, 2
+
+# Every symbol the header declares (checked by the CPU tests).
+EXPORTS = [
+    "gbi_params_default", "gbi_create", "gbi_execute", "gbi_update_parameters", "gbi_set_positions",
+    "gbi_compute", "gbi_synchronize", "gbi_get_energy", "gbi_get_forces", "gbi_get_born_radii",
+    "gbi_get_integrals", "gbi_get_stats", "gbi_get_device_view", "gbi_set_stream", "gbi_destroy",
+    "gbi_last_error", "gbi_abi_version", "gbi_quadrature",
+]
+
+
+class GbiError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("model", ctypes.c_int32),
+        ("integral_type", ctypes.c_int32),
+        ("lookup_policy", ctypes.c_int32),
+        ("nonbonded_method", ctypes.c_int32),
+        ("cutoff", ctypes.c_double),
+        ("n_radial", ctypes.c_int32),
+        ("n_lebedev", ctypes.c_int32),
+        ("r0", ctypes.c_double),
+        ("r1", ctypes.c_double),
+        ("grid_length", ctypes.c_double),
+        ("buffer", ctypes.c_double),
+        ("cell_cap", ctypes.c_int32),
+        ("orders", ctypes.c_int32 * 2),
+        ("sw", ctypes.c_double),
+        ("rscale", ctypes.c_double),
+        ("roffset", ctypes.c_double),
+        ("gamma0", ctypes.c_double),
+        ("beta", ctypes.c_double),
+        ("lambda_", ctypes.c_double),
+        ("P1", ctypes.c_double),
+        ("P2", ctypes.c_double),
+        ("S0", ctypes.c_double),
+        ("vguard", ctypes.c_double),
+        ("alpha0", ctypes.c_double),
+        ("alpha1", ctypes.c_double),
+        ("eta", ctypes.c_double),
+        ("slope", ctypes.c_double),
+        ("shift", ctypes.c_double),
+        ("prefactor", ctypes.c_double),
+        ("tight_grid_length", ctypes.c_double),
+        ("block_volume", ctypes.c_int32),
+        ("block_force", ctypes.c_int32),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("n_points", "n_vol_pairs", "n_cells", "n_entries", "n_atoms", "n_quad")]
+
+
+class DeviceView(ctypes.Structure):
+    _fields_ = [
+        ("positions", ctypes.c_void_p),
+        ("forces", ctypes.c_void_p),
+        ("energy", ctypes.c_void_p),
+        ("born_radii", ctypes.c_void_p),
+        ("integrals", ctypes.c_void_p),
+        ("stream", ctypes.c_void_p),
+    ]
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libgbi.so (raises when it is absent: there is no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing; build it with "
+                           "`python -m openmmcustomgbintegralforce_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    dp = ctypes.POINTER(ctypes.c_double)
+    vp = ctypes.c_void_p
+    L.gbi_params_default.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Params)]
+    L.gbi_create.argtypes = [ctypes.c_int32, dp, dp, ctypes.POINTER(Params), ctypes.POINTER(vp)]
+    L.gbi_execute.argtypes = [vp, dp, ctypes.c_int32, ctypes.c_int32, dp, dp]
+    L.gbi_update_parameters.argtypes = [vp, ctypes.c_int32, dp, dp]
+    L.gbi_set_positions.argtypes = [vp, vp, ctypes.c_int32]
+    L.gbi_compute.argtypes = [vp, ctypes.c_uint32]
+    L.gbi_synchronize.argtypes = [vp]
+    L.gbi_get_energy.argtypes = [vp, dp]
+    L.gbi_get_forces.argtypes = [vp, dp, ctypes.c_int32]
+    L.gbi_get_born_radii.argtypes = [vp, dp]
+    L.gbi_get_integrals.argtypes = [vp, dp]
+    L.gbi_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.gbi_get_device_view.argtypes = [vp, ctypes.POINTER(DeviceView)]
+    L.gbi_set_stream.argtypes = [vp, vp]
+    L.gbi_destroy.argtypes = [vp]
+    L.gbi_destroy.restype = None
+    L.gbi_last_error.restype = ctypes.c_char_p
+    L.gbi_abi_version.restype = ctypes.c_int32
+    L.gbi_quadrature.argtypes = [ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_int32, dp]
+    for name in EXPORTS:
+        fn = getattr(L, name)
+        if fn.restype is ctypes.c_int:  # default -> gbi_status
+            fn.restype = ctypes.c_int32
+    _lib = L
+    return L
+
+
+def _check(status: int) -> None:
+    if status != GBI_OK:
+        msg = load().gbi_last_error()
+        raise GbiError(status, msg.decode() if msg else "")
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def default_params(model: int, integral_type: int = INTEGRAL_GBSW) -> Params:
+    p = Params()
+    _check(load().gbi_params_default(model, integral_type, ctypes.byref(p)))
+    return p
+
+
+def quadrature(n_radial: int = 24, r0: float = 0.05, r1: float = 2.0, n_lebedev: int = 38) -> np.ndarray:
+    out = np.zeros((n_radial * n_lebedev, 5))
+    _check(load().gbi_quadrature(n_radial, r0, r1, n_lebedev, _dp(out)))
+    return out
+
+
+class Engine:
+    """One gbi_ctx: the HIP counterpart of a CalcCharmmGB{SW,MV}ForceKernel instance."""
+
+    def __init__(self, params: Params, charge, radius):
+        self._lib = load()
+        self.charge = np.ascontiguousarray(charge, dtype=np.float64)
+        self.radius = np.ascontiguousarray(radius, dtype=np.float64)
+        self.n = int(self.charge.shape[0])
+        self.params = params
+        handle = ctypes.c_void_p()
+        _check(self._lib.gbi_create(self.n, _dp(self.charge), _dp(self.radius), ctypes.byref(params),
+                                    ctypes.byref(handle)))
+        self._h = handle
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.gbi_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def execute(self, pos, include_forces: bool = True, include_energy: bool = True, forces=None):
+        """(energy, forces) of one evaluation; forces are ADDED to `forces` when given."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.n, 3)
+        out = np.zeros((self.n, 3)) if forces is None else forces
+        e = np.zeros(1)
+        _check(self._lib.gbi_execute(self._h, _dp(pos), int(include_forces), int(include_energy), _dp(e),
+                                     _dp(out)))
+        return float(e[0]), out
+
+    def update_parameters(self, charge, radius) -> None:
+        charge = np.ascontiguousarray(charge, dtype=np.float64)
+        radius = np.ascontiguousarray(radius, dtype=np.float64)
+        _check(self._lib.gbi_update_parameters(self._h, int(charge.shape[0]), _dp(charge), _dp(radius)))
+        self.charge, self.radius = charge, radius
+
+    def set_positions(self, pos, on_device: bool = False) -> None:
+        if on_device:
+            _check(self._lib.gbi_set_positions(self._h, ctypes.c_void_p(int(pos)), 1))
+        else:
+            pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.n, 3)
+            _check(self._lib.gbi_set_positions(self._h, pos.ctypes.data_as(ctypes.c_void_p), 0))
+
+    def compute(self, flags: int = FORCES | ENERGY) -> None:
+        _check(self._lib.gbi_compute(self._h, flags))
+
+    def synchronize(self) -> None:
+        _check(self._lib.gbi_synchronize(self._h))
+
+    def energy(self) -> float:
+        e = np.zeros(1)
+        _check(self._lib.gbi_get_energy(self._h, _dp(e)))
+        return float(e[0])
+
+    def forces(self) -> np.ndarray:
+        f = np.zeros((self.n, 3))
+        _check(self._lib.gbi_get_forces(self._h, _dp(f), 0))
+        return f
+
+    def born_radii(self) -> np.ndarray:
+        b = np.zeros(self.n)
+        _check(self._lib.gbi_get_born_radii(self._h, _dp(b)))
+        return b
+
+    def integrals(self) -> np.ndarray:
+        out = np.zeros((2, self.n))
+        _check(self._lib.gbi_get_integrals(self._h, _dp(out)))
+        return out
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(self._lib.gbi_get_stats(self._h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+    def device_view(self) -> DeviceView:
+        v = DeviceView()
+        _check(self._lib.gbi_get_device_view(self._h, ctypes.byref(v)))
+        return v
+
+    def set_stream(self, stream_ptr: int | None) -> None:
+        _check(self._lib.gbi_set_stream(self._h, ctypes.c_void_p(stream_ptr or 0)))

@@ -1,0 +1,1084 @@
+// gfx950 kernels of the integral Generalized-Born engine.
+//
+// One evaluation (launch_evaluate) runs, on one stream:
+//   clear -> bbox/grid geometry -> cell lists (count, scan, fill, sort+cap)
+//   -> point binning (count, scan, scatter) -> volume pass (per cell, lanes = points)
+//   -> integrals + Born map (per atom) -> STILL pair pass (row blocks x column splits)
+//   -> per-atom chain (dE/dB -> dE/dI, self energy) -> energy reduction
+//   -> chain-rule force pass (per cell, lanes = list atoms) -> owner-force reduction.
+//
+// Determinism: every sum is taken in a fixed order, except the per-(cell, atom) force
+// partials, which are quantised to 2^-24 and summed exactly in FP64 before one int64
+// fixed-point atomic per (cell, atom); integer addition is associative, so forces are
+// bitwise reproducible.
+//
+// Reference counterparts: platforms/cuda/src/kernels/lookupTable.cu (grid),
+// computeIntegral.cu + compute{GBSW,GBMV1,GBMV2}Volume.cu (volume pass),
+// customGBValuePerParticle.cu (Born map), customGBEnergyN2.cu (pair pass),
+// customGBEnergyPerParticle.cu (dE/dB -> dE/dI), reduceIntegralForce.cu +
+// compute{GBSW,GBMV1,GBMV2}Force.cu (chain-rule pass).
+#include "gbi_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+#include <cmath>
+
+namespace gbi {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr int kFatal = FLAG_CELL_OVERFLOW | FLAG_ENTRY_OVERFLOW | FLAG_GRID_TOO_LARGE |
+                       FLAG_NONFINITE_POS | FLAG_POINT_OVERFLOW;
+constexpr double kFix32 = 4294967296.0;  // 2^32
+
+__device__ inline float wave_sum(float v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+__device__ inline double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+__device__ inline double wave_min(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+__device__ inline double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+__device__ inline int wave_scan_incl(int v, int lane) {
+    for (int off = 1; off < kWave; off <<= 1) {
+        int t = __shfl_up(v, off, kWave);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+__device__ inline bool fatal(const Geom* g) { return (g->flags & kFatal) != 0; }
+
+__device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// ---------------------------------------------------------------------------
+// Grid geometry and membership (FP64, no contraction: bitwise the oracle's arithmetic)
+// ---------------------------------------------------------------------------
+
+// Cell of a point; -1 outside.  Faithful: CustomGBIntegral.cpp:310-324.
+__device__ inline int point_cell(const KParams& p, const Geom& G, double px, double py, double pz) {
+#pragma clang fp contract(off)
+    if (p.grid_kind == GRID_BRUTE) return 0;
+    const double pt[3] = {px, py, pz};
+    int idx[3];
+    if (p.grid_kind == GRID_FAITHFUL) {
+        for (int d = 0; d < 3; ++d) {
+            if (pt[d] < G.origin[d] || pt[d] > G.max[d]) return -1;
+            idx[d] = (int)floor((pt[d] - G.origin[d]) / p.g);
+        }
+    } else {
+        for (int d = 0; d < 3; ++d) {
+            double t = (pt[d] - G.origin[d]) * p.inv_g;
+            if (!(t >= 0.0) || t >= (double)G.n[d]) return -1;
+            idx[d] = (int)floor(t);
+            if (idx[d] >= G.n[d]) return -1;
+        }
+    }
+    return (idx[0] * G.n[1] + idx[1]) * G.n[2] + idx[2];
+}
+
+// Calls f(cell) for every cell whose list atom j joins.
+// Faithful: corner within pad_j = R_j + sqrt3/2 g + buffer (CustomGBIntegral.cpp:198-227),
+//           restricted to cells inside the grid.
+// Tight:    distance from x_j to the cell box below reach_j = R'_j + w + margin.
+template <class F>
+__device__ inline void for_each_member_cell(const KParams& p, const Geom& G, int j, F&& f) {
+#pragma clang fp contract(off)
+    const double x[3] = {p.pos[3 * j], p.pos[3 * j + 1], p.pos[3 * j + 2]};
+    const double g = p.g;
+    int b[3], e[3];
+    if (p.grid_kind == GRID_FAITHFUL) {
+        const double pad = (p.radius[j] + p.halfdiag) + p.buffer;
+        for (int d = 0; d < 3; ++d) {
+            b[d] = (int)floor((x[d] - pad - G.origin[d]) / g);
+            e[d] = (int)ceil((x[d] + pad - G.origin[d]) / g);
+            if (b[d] < 0) b[d] = 0;
+            if (e[d] > G.n[d] - 1) e[d] = G.n[d] - 1;
+        }
+        const double pad2 = pad * pad;
+        for (int i = b[0]; i <= e[0]; ++i) {
+            const double dx = (G.origin[0] + i * g) - x[0];
+            const double dx2 = dx * dx;
+            for (int jj = b[1]; jj <= e[1]; ++jj) {
+                const double dy = (G.origin[1] + jj * g) - x[1];
+                const double dxy = dx2 + dy * dy;
+                for (int k = b[2]; k <= e[2]; ++k) {
+                    const double dz = (G.origin[2] + k * g) - x[2];
+                    if (dxy + dz * dz < pad2) f((i * G.n[1] + jj) * G.n[2] + k);
+                }
+            }
+        }
+    } else {
+        const double reach = p.reach[j];
+        for (int d = 0; d < 3; ++d) {
+            b[d] = (int)floor((x[d] - reach - G.origin[d]) * p.inv_g);
+            e[d] = (int)floor((x[d] + reach - G.origin[d]) * p.inv_g);
+            if (b[d] < 0) b[d] = 0;
+            if (e[d] > G.n[d] - 1) e[d] = G.n[d] - 1;
+        }
+        const double r2 = reach * reach;
+        for (int i = b[0]; i <= e[0]; ++i) {
+            const double lo = G.origin[0] + i * g;
+            const double dx = fmax(fmax(lo - x[0], x[0] - (lo + g)), 0.0);
+            for (int jj = b[1]; jj <= e[1]; ++jj) {
+                const double loy = G.origin[1] + jj * g;
+                const double dy = fmax(fmax(loy - x[1], x[1] - (loy + g)), 0.0);
+                const double dxy = dx * dx + dy * dy;
+                for (int k = b[2]; k <= e[2]; ++k) {
+                    const double loz = G.origin[2] + k * g;
+                    const double dz = fmax(fmax(loz - x[2], x[2] - (loz + g)), 0.0);
+                    if (dxy + dz * dz < r2) f((i * G.n[1] + jj) * G.n[2] + k);
+                }
+            }
+        }
+    }
+}
+
+__device__ inline void cell_corner(const KParams& p, const Geom& G, int c, double* corner) {
+#pragma clang fp contract(off)
+    if (p.grid_kind == GRID_BRUTE) {
+        corner[0] = G.origin[0];
+        corner[1] = G.origin[1];
+        corner[2] = G.origin[2];
+        return;
+    }
+    const int nyz = G.n[1] * G.n[2];
+    const int ix = c / nyz;
+    const int rem = c - ix * nyz;
+    const int iy = rem / G.n[2];
+    const int iz = rem - iy * G.n[2];
+    corner[0] = G.origin[0] + ix * p.g;
+    corner[1] = G.origin[1] + iy * p.g;
+    corner[2] = G.origin[2] + iz * p.g;
+}
+
+// ---------------------------------------------------------------------------
+// Small kernels
+// ---------------------------------------------------------------------------
+
+__global__ void k_clear(KParams p) {
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < p.cell_cap; i += stride) {
+        p.cell_count[i] = 0;
+        p.cell_fill[i] = 0;
+        p.cell_pt_count[i] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 4) p.work[threadIdx.x] = 0;
+}
+
+__device__ void bbox_to_geom(const KParams& p, double (*smin)[16], double (*smax)[16], const int* sbad, int nw);
+
+// Bounding box -> grid geometry (one workgroup).
+__global__ void __launch_bounds__(1024) k_bbox(KParams p) {
+    __shared__ double smin[3][16], smax[3][16];
+    __shared__ int sbad[16];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    int bad = 0;
+    for (int a = tid; a < p.N; a += blockDim.x) {
+        for (int d = 0; d < 3; ++d) {
+            const double v = p.pos[3 * a + d];
+            if (!isfinite(v)) bad = 1;
+            mn[d] = fmin(mn[d], v);
+            mx[d] = fmax(mx[d], v);
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        mn[d] = wave_min(mn[d]);
+        mx[d] = wave_max(mx[d]);
+    }
+    const int anybad = __any(bad);
+    if (lane == 0) {
+        for (int d = 0; d < 3; ++d) {
+            smin[d][wid] = mn[d];
+            smax[d][wid] = mx[d];
+        }
+        sbad[wid] = anybad;
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    bbox_to_geom(p, smin, smax, sbad, (blockDim.x + kWave - 1) / kWave);
+}
+
+// Grid geometry from the per-wave bounding boxes (thread 0 of k_bbox).
+__device__ void bbox_to_geom(const KParams& p, double (*smin)[16], double (*smax)[16], const int* sbad, int nw) {
+#pragma clang fp contract(off)
+    double mn[3], mx[3];
+    int isbad = 0;
+    for (int d = 0; d < 3; ++d) {
+        mn[d] = smin[d][0];
+        mx[d] = smax[d][0];
+        for (int w = 1; w < nw; ++w) {
+            mn[d] = fmin(mn[d], smin[d][w]);
+            mx[d] = fmax(mx[d], smax[d][w]);
+        }
+    }
+    for (int w = 0; w < nw; ++w) isbad |= sbad[w];
+    Geom G;
+    G.flags = 0;
+    G.total_entries = 0;
+    G.total_points = 0;
+    G.pad_ = 0;
+    long long ncell = 1;
+    if (isbad) {
+        G.flags |= FLAG_NONFINITE_POS;
+        ncell = 0;
+        for (int d = 0; d < 3; ++d) {
+            G.origin[d] = 0.0;
+            G.max[d] = 0.0;
+            G.n[d] = 0;
+        }
+    } else if (p.grid_kind == GRID_BRUTE) {
+        for (int d = 0; d < 3; ++d) {
+            G.origin[d] = 0.5 * (mn[d] + mx[d]);
+            G.max[d] = DBL_MAX;
+            G.n[d] = 1;
+        }
+    } else if (p.grid_kind == GRID_FAITHFUL) {
+        // CustomGBIntegral.cpp:173-187
+        for (int d = 0; d < 3; ++d) {
+            const double lo = mn[d] - p.gpad;
+            const double hi = mx[d] + p.gpad;
+            const double len = hi - lo;
+            if (len > 1000.0) G.flags |= FLAG_GRID_TOO_LARGE;
+            const int n = len > 1000.0 ? 1 : (int)ceil(len / p.g) + 1;
+            G.n[d] = n;
+            G.origin[d] = lo;
+            G.max[d] = lo + (n - 1) * p.g;
+            ncell *= n;
+        }
+    } else {
+        for (int d = 0; d < 3; ++d) {
+            const double lo = mn[d] - p.gpad;
+            const double hi = mx[d] + p.gpad;
+            const double len = hi - lo;
+            if (len > 1000.0) G.flags |= FLAG_GRID_TOO_LARGE;
+            const int n = len > 1000.0 ? 1 : (int)floor(len * p.inv_g) + 1;
+            G.n[d] = n;
+            G.origin[d] = lo;
+            G.max[d] = lo + n * p.g;
+            ncell *= n;
+        }
+    }
+    if (ncell > p.cell_cap) {
+        G.flags |= FLAG_CELL_OVERFLOW;
+        ncell = 0;
+    }
+    if (G.flags & kFatal) ncell = 0;
+    G.ncell = (int)ncell;
+    *p.geom = G;
+}
+
+__global__ void k_grid_count(KParams p) {
+    const Geom G = *p.geom;
+    if (G.flags & kFatal) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= p.N) return;
+    for_each_member_cell(p, G, j, [&](int c) { atomicAdd(&p.cell_count[c], 1); });
+}
+
+__global__ void k_grid_fill(KParams p) {
+    const Geom G = *p.geom;
+    if (G.flags & kFatal) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= p.N) return;
+    for_each_member_cell(p, G, j, [&](int c) {
+        const int s = atomicAdd(&p.cell_fill[c], 1);
+        const int idx = p.cell_start[c] + s;
+        if (idx < p.entry_cap) p.cell_atoms_tmp[idx] = j;
+    });
+}
+
+// One wave per cell: sort the cell's atoms by index (rank sort) and apply the cap
+// (the reference appends in ascending atom order and stops at the cap,
+// CustomGBIntegral.cpp:219-222).
+__global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
+    __shared__ int s_list[1024];
+    const Geom G = *p.geom;
+    if (G.flags & kFatal) return;
+    const int lane = threadIdx.x;
+    for (int c = blockIdx.x; c < G.ncell; c += gridDim.x) {
+        const int n = p.cell_count[c];
+        if (n == 0) {
+            if (lane == 0) p.cell_count_eff[c] = 0;
+            continue;
+        }
+        const int base = p.cell_start[c];
+        const bool in_lds = n <= 1024;
+        if (in_lds) {
+            for (int k = lane; k < n; k += kWave) s_list[k] = p.cell_atoms_tmp[base + k];
+        }
+        __syncthreads();
+        for (int k = lane; k < n; k += kWave) {
+            const int a = in_lds ? s_list[k] : p.cell_atoms_tmp[base + k];
+            int r = 0;
+            if (in_lds) {
+                for (int m = 0; m < n; ++m) r += s_list[m] < a;
+            } else {
+                for (int m = 0; m < n; ++m) r += p.cell_atoms_tmp[base + m] < a;
+            }
+            p.cell_atoms[base + r] = a;
+        }
+        if (lane == 0) p.cell_count_eff[c] = n < p.cap ? n : p.cap;
+        __syncthreads();
+    }
+}
+
+__global__ void k_grid_brute(KParams p) {
+    const Geom G = *p.geom;
+    if (G.flags & kFatal) return;
+    const int stride = gridDim.x * blockDim.x;
+    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < p.N; a += stride) p.cell_atoms[a] = a;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        p.cell_count[0] = p.N;
+        p.cell_count_eff[0] = p.N;
+        p.cell_start[0] = 0;
+        p.cell_start[1] = p.N;
+        p.geom->total_entries = p.N;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of int arrays (three passes, deterministic)
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int* in, int* out, int n, int* block_sums) {
+    __shared__ int s_wave[kScanThreads / kWave];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const long long base = (long long)blockIdx.x * kScanTile + (long long)tid * kScanItems;
+    int v[kScanItems];
+    int tsum = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        const long long idx = base + k;
+        v[k] = idx < n ? in[idx] : 0;
+        tsum += v[k];
+    }
+    const int incl = wave_scan_incl(tsum, lane);
+    if (lane == kWave - 1) s_wave[wid] = incl;
+    __syncthreads();
+    if (wid == 0) {
+        const int nw = kScanThreads / kWave;
+        int w = lane < nw ? s_wave[lane] : 0;
+        const int wi = wave_scan_incl(w, lane);
+        if (lane < nw) s_wave[lane] = wi - w;
+        if (lane == nw - 1) block_sums[blockIdx.x] = wi;
+    }
+    __syncthreads();
+    int run = s_wave[wid] + incl - tsum;
+    for (int k = 0; k < kScanItems; ++k) {
+        const long long idx = base + k;
+        if (idx < n) out[idx] = run;
+        run += v[k];
+    }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_top(int* block_sums, int nblocks, int* out, int n,
+                                                           int* total_dst, int limit, int* flags, int flag) {
+    __shared__ int s_wave[kScanThreads / kWave];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    int v[kScanItems];
+    int tsum = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        const int idx = tid * kScanItems + k;
+        v[k] = idx < nblocks ? block_sums[idx] : 0;
+        tsum += v[k];
+    }
+    const int incl = wave_scan_incl(tsum, lane);
+    if (lane == kWave - 1) s_wave[wid] = incl;
+    __syncthreads();
+    if (wid == 0) {
+        const int nw = kScanThreads / kWave;
+        int w = lane < nw ? s_wave[lane] : 0;
+        const int wi = wave_scan_incl(w, lane);
+        if (lane < nw) s_wave[lane] = wi - w;
+        if (lane == nw - 1) {
+            out[n] = wi;
+            if (total_dst) *total_dst = wi;
+            if (flags && wi > limit) atomicOr(flags, flag);
+        }
+    }
+    __syncthreads();
+    int run = s_wave[wid] + incl - tsum;
+    for (int k = 0; k < kScanItems; ++k) {
+        const int idx = tid * kScanItems + k;
+        if (idx < nblocks) block_sums[idx] = run;
+        run += v[k];
+    }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_add(int* out, int n, const int* block_sums) {
+    const long long base = (long long)blockIdx.x * kScanTile;
+    const int add = block_sums[blockIdx.x];
+    for (int k = threadIdx.x; k < kScanTile; k += kScanThreads) {
+        const long long idx = base + k;
+        if (idx < n) out[idx] += add;
+    }
+}
+
+hipError_t scan_exclusive(const int* in, int* out, int n, int* tmp, int* total_dst, int limit, int* flags,
+                          int flag, hipStream_t stream) {
+    const int nblocks = (n + kScanTile - 1) / kScanTile;
+    if (nblocks > kScanTile) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nblocks), dim3(kScanThreads), 0, stream, in, out, n, tmp);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, stream, tmp, nblocks, out, n, total_dst, limit,
+                       flags, flag);
+    hipLaunchKernelGGL(k_scan_add, dim3(nblocks), dim3(kScanThreads), 0, stream, out, n, (const int*)tmp);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Point binning
+// ---------------------------------------------------------------------------
+
+// One workgroup per atom: the cell of every quadrature point x_i + off_q (FP64, the
+// reference's r_q = x_i + quad_q, GBSWIntegral.cpp:132-133).  Points outside the grid or
+// in an empty cell get the empty-list volume directly.
+__global__ void __launch_bounds__(256) k_point_bin(KParams p) {
+    const Geom G = *p.geom;
+    if (G.flags & kFatal) return;
+    const int i = blockIdx.x;
+    const double xi = p.pos[3 * i], yi = p.pos[3 * i + 1], zi = p.pos[3 * i + 2];
+    for (int q = threadIdx.x; q < p.Q; q += blockDim.x) {
+        const int pid = i * p.Q + q;
+        double px, py, pz;
+        {
+#pragma clang fp contract(off)
+            px = xi + p.quad_off[3 * q];
+            py = yi + p.quad_off[3 * q + 1];
+            pz = zi + p.quad_off[3 * q + 2];
+        }
+        const int c = point_cell(p, G, px, py, pz);
+        if (c >= 0 && p.cell_count_eff[c] > 0) {
+            const int s = atomicAdd(&p.cell_pt_count[c], 1);
+            p.pt_cell[pid] = c;
+            p.pt_slot[pid] = s;
+        } else {
+            p.pt_cell[pid] = -1;
+            p.vpt[pid] = p.v_empty;
+            p.owner_f[pid] = 0.0f;
+            p.owner_f[p.NQ + pid] = 0.0f;
+            p.owner_f[2 * p.NQ + pid] = 0.0f;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_point_scatter(KParams p) {
+    const Geom G = *p.geom;
+    if (G.flags & kFatal) return;
+    const int i = blockIdx.x;
+    for (int q = threadIdx.x; q < p.Q; q += blockDim.x) {
+        const int pid = i * p.Q + q;
+        const int c = p.pt_cell[pid];
+        if (c >= 0) {
+            const int idx = p.cell_pt_start[c] + p.pt_slot[pid];
+            if (idx < p.NQ) p.cell_pts[idx] = ((uint32_t)i << kQBits) | (uint32_t)q;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Volume pass: one wave per cell (work queue), lanes = quadrature points,
+// the cell's atoms staged in LDS relative to the cell corner.
+// ---------------------------------------------------------------------------
+
+template <int VK>
+__global__ void __launch_bounds__(64) k_volume(KParams p) {
+    __shared__ float s_x[kMaxAtomChunk], s_y[kMaxAtomChunk], s_z[kMaxAtomChunk];
+    __shared__ float s_a[kMaxAtomChunk], s_b[kMaxAtomChunk], s_c[kMaxAtomChunk];
+    const Geom G = *p.geom;
+    if (G.flags & kFatal) return;
+    const int lane = threadIdx.x;
+    for (;;) {
+        int c = 0;
+        if (lane == 0) c = atomicAdd(&p.work[0], 1);
+        c = __shfl(c, 0, kWave);
+        if (c >= G.ncell) break;
+        const int np = p.cell_pt_count[c];
+        const int na = p.cell_count_eff[c];
+        if (np == 0 || na == 0) continue;
+        double corner[3];
+        cell_corner(p, G, c, corner);
+        const int ps = p.cell_pt_start[c];
+        const int as = p.cell_start[c];
+        const bool single = na <= kMaxAtomChunk;
+        auto stage = [&](int ab, int cnt) {
+            for (int k = lane; k < cnt; k += kWave) {
+                const int j = p.cell_atoms[as + ab + k];
+                s_x[k] = (float)(p.pos[3 * j] - corner[0]);
+                s_y[k] = (float)(p.pos[3 * j + 1] - corner[1]);
+                s_z[k] = (float)(p.pos[3 * j + 2] - corner[2]);
+                const float4 ap = p.aparam[j];
+                s_a[k] = ap.x;
+                s_b[k] = ap.y;
+                s_c[k] = ap.z;
+            }
+        };
+        __syncthreads();
+        if (single) stage(0, na);
+        __syncthreads();
+        for (int pb = 0; pb < np; pb += kWave) {
+            const int k = pb + lane;
+            const bool valid = k < np;
+            const uint32_t pk = valid ? p.cell_pts[ps + k] : 0u;
+            const int i = (int)(pk >> kQBits), q = (int)(pk & ((1u << kQBits) - 1));
+            float prx = 0.f, pry = 0.f, prz = 0.f;
+            if (valid) {
+#pragma clang fp contract(off)
+                const double px = p.pos[3 * i] + p.quad_off[3 * q];
+                const double py = p.pos[3 * i + 1] + p.quad_off[3 * q + 1];
+                const double pz = p.pos[3 * i + 2] + p.quad_off[3 * q + 2];
+                prx = (float)(px - corner[0]);
+                pry = (float)(py - corner[1]);
+                prz = (float)(pz - corner[2]);
+            }
+            // per-kind state
+            float V = 1.f;
+            bool done = !valid;            // GBSW: buried or invalid
+            float s1 = 0.f, s2 = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;  // GBMV2 sums / GBMV1 S in s1
+            for (int ab = 0; ab < na; ab += kMaxAtomChunk) {
+                const int cnt = min(kMaxAtomChunk, na - ab);
+                if (!single) {
+                    __syncthreads();
+                    stage(ab, cnt);
+                    __syncthreads();
+                }
+                if (VK == VOL_GBSW) {
+                    for (int m = 0; m < cnt; ++m) {
+                        if (__all(done)) break;
+                        const float dx = s_x[m] - prx, dy = s_y[m] - pry, dz = s_z[m] - prz;
+                        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                        if (!done && d2 < s_c[m]) {
+                            if (d2 <= s_b[m]) {
+                                done = true;
+                                V = 0.f;
+                            } else {
+                                const float x = fast_sqrt(d2) - s_a[m];
+                                const float h = fmaf(x, fmaf(-p.k3 * x, x, p.k1), 0.5f);
+                                if (h <= 0.f) {
+                                    done = true;
+                                    V = 0.f;
+                                } else {
+                                    V *= h;
+                                }
+                            }
+                        }
+                    }
+                } else if (VK == VOL_GBMV2) {
+                    if (valid) {
+                        for (int m = 0; m < cnt; ++m) {
+                            const float dx = s_x[m] - prx, dy = s_y[m] - pry, dz = s_z[m] - prz;
+                            const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                            const float t = fmaf(100.f, d2, s_b[m]);
+                            const float ct = s_a[m] * fast_rcp(t);
+                            const float F = fminf(ct * ct, p.fclamp);
+                            s1 += F;
+                            s2 = fmaf(d2, F * F, s2);
+                            vx = fmaf(F, dx, vx);
+                            vy = fmaf(F, dy, vy);
+                            vz = fmaf(F, dz, vz);
+                        }
+                    }
+                } else {  // GBMV1
+                    if (valid) {
+                        for (int m = 0; m < cnt; ++m) {
+                            const float dx = s_x[m] - prx, dy = s_y[m] - pry, dz = s_z[m] - prz;
+                            const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                            s1 += __expf(s_a[m] * (d2 * d2));
+                        }
+                    }
+                }
+            }
+            if (valid) {
+                const int pid = i * p.Q + q;
+                if (VK == VOL_GBSW) {
+                    p.vpt[pid] = V;
+                } else if (VK == VOL_GBMV2) {
+                    const float vsq = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
+                    const float S = p.S0 * s1 * s2 / vsq;
+                    const bool sat = !(vsq >= p.vguard) || !(S <= p.ssat);
+                    float Vv = 1.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+                    if (!sat) {
+                        const float pre = __expf(p.beta * (S - p.lambda));
+                        const float opre = 1.f + pre;
+                        Vv = 1.f / opre;
+                        const float kk = -p.beta * pre / (opre * opre) * p.S0;
+                        const float ivsq = 1.f / vsq;
+                        c1 = kk * s2 * ivsq;
+                        c2 = 2.f * kk * s1 * ivsq;
+                        c3 = -2.f * kk * s1 * s2 * ivsq * ivsq;
+                    }
+                    p.vpt[pid] = Vv;
+                    p.aux[pid] = c1;
+                    p.aux[p.NQ + pid] = c2;
+                    p.aux[2 * p.NQ + pid] = c3;
+                    p.aux[3 * p.NQ + pid] = vx;
+                    p.aux[4 * p.NQ + pid] = vy;
+                    p.aux[5 * p.NQ + pid] = vz;
+                } else {
+                    const float pre = __expf(p.beta * (s1 - p.lambda));
+                    const float opre = 1.f + pre;
+                    p.vpt[pid] = 1.f / opre;
+                    p.aux[pid] = -p.beta * pre / (opre * opre);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Integrals and Born radii: one wave per atom, fixed-order FP64 sums over q.
+// I_k(i) = sum_q w_q / r_q^(n_k - 2) * (1 - V) [GBSW] or * V [GBMV]
+// (GBSWIntegral.cpp:142-146; GBMVIntegralTypeII.cpp:145-149).
+// B = slope / (a0 (1/eta - I4) + a1 (1/(4 eta^4) - I7)^(1/4)) + shift
+// (CharmmReferenceGBSW.cpp:591-593; dB/dI as :607-609).
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(64) k_integrals(KParams p) {
+    const Geom G = *p.geom;
+    if (G.flags & kFatal) return;
+    const int i = blockIdx.x, lane = threadIdx.x;
+    double a0 = 0.0, a1 = 0.0;
+    const bool gbsw = p.vol_kind == VOL_GBSW;
+    for (int q = lane; q < p.Q; q += kWave) {
+        const double v = (double)p.vpt[i * p.Q + q];
+        const double f = gbsw ? 1.0 - v : v;
+        a0 += p.quad_pw[q] * f;
+        a1 += p.quad_pw[p.Q + q] * f;
+    }
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    if (lane != 0) return;
+    p.integrals[i] = a0;
+    p.integrals[p.N + i] = a1;
+    const double I4 = p.i4 == 0 ? a0 : a1;
+    const double I7 = p.i7 == 0 ? a0 : a1;
+    const double eta4 = p.eta * p.eta * p.eta * p.eta;
+    const double T = 1.0 / (4.0 * eta4) - I7;
+    const double D = p.alpha0 * (1.0 / p.eta - I4) + p.alpha1 * pow(T, 0.25);
+    const double B = p.slope / D + p.shift;
+    p.born[i] = B;
+    p.dbdi[p.i4 * p.N + i] = p.slope * p.alpha0 / (D * D);
+    p.dbdi[p.i7 * p.N + i] = p.slope * 0.25 * p.alpha1 * pow(T, -0.75) / (D * D);
+    if (!isfinite(B) || !(B > 0.0)) atomicOr(&p.geom->flags, FLAG_NONFINITE_RESULT);
+}
+
+// ---------------------------------------------------------------------------
+// STILL pair pass: workgroup = 64 atoms i x one column split; every ordered pair
+// (i, j != i) is evaluated from i's side, so no atomics and a fixed summation order.
+// E_ij = P q_i q_j / f_ij, f = sqrt(r^2 + B_i B_j exp(-r^2/(4 B_i B_j)))
+// (CharmmReferenceGBSW.cpp:248-282).
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(64) k_pairs(KParams p) {
+    __shared__ double s_px[64], s_py[64], s_pz[64];
+    __shared__ float s_q[64], s_B[64];
+    const Geom G = *p.geom;
+    if (G.flags & kFatal) return;
+    const int lane = threadIdx.x;
+    const int i = blockIdx.x * kWave + lane;
+    const int s = blockIdx.y, S = gridDim.y;
+    const int jlo = (int)((long long)s * p.N / S), jhi = (int)((long long)(s + 1) * p.N / S);
+    const bool iv = i < p.N;
+    const double xi = iv ? p.pos[3 * i] : 0.0, yi = iv ? p.pos[3 * i + 1] : 0.0, zi = iv ? p.pos[3 * i + 2] : 0.0;
+    const float qi = iv ? (float)p.charge[i] : 0.f;
+    const float Bi = iv ? (float)p.born[i] : 1.f;
+    const float Pq = (float)p.prefactor * qi;
+    const float q4Bi = 0.25f / Bi;
+    double fx = 0.0, fy = 0.0, fz = 0.0, dedb = 0.0, e = 0.0;
+    for (int jb = jlo; jb < jhi; jb += kWave) {
+        const int jj = jb + lane;
+        __syncthreads();
+        if (jj < jhi) {
+            s_px[lane] = p.pos[3 * jj];
+            s_py[lane] = p.pos[3 * jj + 1];
+            s_pz[lane] = p.pos[3 * jj + 2];
+            s_q[lane] = (float)p.charge[jj];
+            s_B[lane] = (float)p.born[jj];
+        }
+        __syncthreads();
+        const int cnt = min(kWave, jhi - jb);
+        if (iv) {
+            float efs = 0.f;
+            float fxs = 0.f, fys = 0.f, fzs = 0.f, dbs = 0.f;
+            for (int m = 0; m < cnt; ++m) {
+                if (jb + m == i) continue;
+                const float dx = (float)(s_px[m] - xi), dy = (float)(s_py[m] - yi), dz = (float)(s_pz[m] - zi);
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                const float Bj = s_B[m];
+                const float BB = Bi * Bj;
+                const float ex = __expf(-r2 * 0.25f * fast_rcp(BB));
+                const float f2 = fmaf(BB, ex, r2);
+                const float rf = __builtin_amdgcn_rsqf(f2);
+                const float rf3 = rf * rf * rf;
+                const float Pqq = Pq * s_q[m];
+                efs = fmaf(0.5f * Pqq, rf, efs);
+                const float dGdr = -0.25f * Pqq * (4.f - ex) * rf3;
+                fxs = fmaf(dx, dGdr, fxs);
+                fys = fmaf(dy, dGdr, fys);
+                fzs = fmaf(dz, dGdr, fzs);
+                dbs = fmaf(-0.5f * Pqq * ex * rf3, fmaf(r2, q4Bi, Bj), dbs);
+            }
+            // fold the tile's FP32 partials into FP64 accumulators (64 terms per tile)
+            e += efs;
+            fx += fxs;
+            fy += fys;
+            fz += fzs;
+            dedb += dbs;
+        }
+    }
+    if (iv) {
+        double* out = p.pair_part + (size_t)s * 5 * p.N;
+        out[i] = fx;
+        out[p.N + i] = fy;
+        out[2 * p.N + i] = fz;
+        out[3 * p.N + i] = dedb;
+        out[4 * p.N + i] = e;
+    }
+}
+
+// Per atom: pair partials in split order, self term, dE/dI = dE/dB dB/dI
+// (customGBEnergyPerParticle.cu; CharmmReferenceGBSW.cpp:257-273).
+__global__ void k_atom_chain(KParams p) {
+    const Geom G = *p.geom;
+    if (G.flags & kFatal) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.N) return;
+    double fx = 0.0, fy = 0.0, fz = 0.0, dedb = 0.0, e = 0.0;
+    for (int s = 0; s < p.pair_splits; ++s) {
+        const double* in = p.pair_part + (size_t)s * 5 * p.N;
+        fx += in[i];
+        fy += in[p.N + i];
+        fz += in[2 * p.N + i];
+        dedb += in[3 * p.N + i];
+        e += in[4 * p.N + i];
+    }
+    const double q = p.charge[i], B = p.born[i];
+    e += 0.5 * p.prefactor * q * q / B;
+    dedb += -0.5 * p.prefactor * q * q / (B * B);
+    p.atom_energy[i] = e;
+    p.force_fix[i] = (long long)llrint(fx * kFix32);
+    p.force_fix[p.N + i] = (long long)llrint(fy * kFix32);
+    p.force_fix[2 * p.N + i] = (long long)llrint(fz * kFix32);
+    p.dedi[i] = dedb * p.dbdi[i];
+    p.dedi[p.N + i] = dedb * p.dbdi[p.N + i];
+}
+
+__global__ void __launch_bounds__(1024) k_energy(KParams p) {
+    __shared__ double s_w[16];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    if (fatal(p.geom)) {
+        if (tid == 0) *p.energy = 0.0;
+        return;
+    }
+    double acc = 0.0;
+    for (int i = tid; i < p.N; i += blockDim.x) acc += p.atom_energy[i];
+    acc = wave_sum(acc);
+    if (lane == 0) s_w[wid] = acc;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += s_w[w];
+        *p.energy = t;
+        if (!isfinite(t)) atomicOr(&p.geom->flags, FLAG_NONFINITE_RESULT);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Chain-rule force pass: one wave per cell (work queue), lanes = the cell's atoms,
+// loop over the cell's active points broadcast from LDS.  For a point p of owner i
+// with fac = sum_k dE/dI_k(i) w_q / r_q^(n_k - 2):
+//   F_j     += s fac dV_p/dx_j   (j != i; s = +1 GBSW, -1 GBMV)
+//   F_owner -= s fac sum_j dV_p/dx_j
+// (GBSWIntegral.cpp:271-314, GBMVIntegralTypeI.cpp:195-224,
+//  GBMVIntegralTypeII.cpp:221-282; CharmmReferenceGBMV.cpp:376-383).
+// ---------------------------------------------------------------------------
+
+template <int VK>
+__global__ void __launch_bounds__(64) k_force(KParams p) {
+    __shared__ float s_px[kWave], s_py[kWave], s_pz[kWave];
+    __shared__ float s_d0[kWave], s_d1[kWave], s_d2[kWave], s_d3[kWave], s_d4[kWave], s_d5[kWave];
+    __shared__ int s_owner[kWave], s_pid[kWave];
+    const Geom G = *p.geom;
+    if (G.flags & kFatal) return;
+    const int lane = threadIdx.x;
+    for (;;) {
+        int c = 0;
+        if (lane == 0) c = atomicAdd(&p.work[1], 1);
+        c = __shfl(c, 0, kWave);
+        if (c >= G.ncell) break;
+        const int np = p.cell_pt_count[c];
+        const int na = p.cell_count_eff[c];
+        if (np == 0 || na == 0) continue;
+        double corner[3];
+        cell_corner(p, G, c, corner);
+        const int ps = p.cell_pt_start[c];
+        const int as = p.cell_start[c];
+        for (int ab = 0; ab < na; ab += kWave) {
+            const int slot = ab + lane;
+            const bool aval = slot < na;
+            const int j = aval ? p.cell_atoms[as + slot] : -1;
+            float ax = 0.f, ay = 0.f, az = 0.f;
+            float4 ap = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (aval) {
+                ax = (float)(p.pos[3 * j] - corner[0]);
+                ay = (float)(p.pos[3 * j + 1] - corner[1]);
+                az = (float)(p.pos[3 * j + 2] - corner[2]);
+                ap = p.aparam[j];
+            }
+            double accx = 0.0, accy = 0.0, accz = 0.0;
+            for (int pb = 0; pb < np; pb += kWave) {
+                const int k = pb + lane;
+                const bool pvalid = k < np;
+                int i = 0, q = 0, pid = 0;
+                bool active = false;
+                float d0 = 0.f, d1 = 0.f, d2v = 0.f, d3 = 0.f, d4 = 0.f, d5 = 0.f;
+                float prx = 0.f, pry = 0.f, prz = 0.f;
+                if (pvalid) {
+                    const uint32_t pk = p.cell_pts[ps + k];
+                    i = (int)(pk >> kQBits);
+                    q = (int)(pk & ((1u << kQBits) - 1));
+                    pid = i * p.Q + q;
+                    const double fac = p.dedi[i] * p.quad_pw[q] + p.dedi[p.N + i] * p.quad_pw[p.Q + q];
+                    const float facs = p.sign_chain * (float)fac;
+                    if (VK == VOL_GBSW) {
+                        const float V = p.vpt[pid];
+                        active = V > 0.f && V < 1.f;
+                        d0 = facs;
+                        d1 = V;
+                    } else if (VK == VOL_GBMV2) {
+                        const float c1 = p.aux[pid], c2 = p.aux[p.NQ + pid], c3 = p.aux[2 * p.NQ + pid];
+                        active = (c1 != 0.f) || (c2 != 0.f) || (c3 != 0.f);
+                        d0 = facs * c1;
+                        d1 = facs * c2;
+                        d2v = facs * c3;
+                        d3 = p.aux[3 * p.NQ + pid];
+                        d4 = p.aux[4 * p.NQ + pid];
+                        d5 = p.aux[5 * p.NQ + pid];
+                    } else {
+                        const float kk = p.aux[pid];
+                        active = kk != 0.f;
+                        d0 = facs * kk;
+                    }
+                    if (active) {
+#pragma clang fp contract(off)
+                        const double px = p.pos[3 * i] + p.quad_off[3 * q];
+                        const double py = p.pos[3 * i + 1] + p.quad_off[3 * q + 1];
+                        const double pz = p.pos[3 * i + 2] + p.quad_off[3 * q + 2];
+                        prx = (float)(px - corner[0]);
+                        pry = (float)(py - corner[1]);
+                        prz = (float)(pz - corner[2]);
+                    } else if (ab == 0) {
+                        p.owner_f[pid] = 0.f;
+                        p.owner_f[p.NQ + pid] = 0.f;
+                        p.owner_f[2 * p.NQ + pid] = 0.f;
+                    }
+                }
+                const unsigned long long mask = __ballot(active);
+                const int cnt = __popcll(mask);
+                __syncthreads();
+                if (active) {
+                    const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+                    s_px[pos] = prx;
+                    s_py[pos] = pry;
+                    s_pz[pos] = prz;
+                    s_d0[pos] = d0;
+                    s_d1[pos] = d1;
+                    s_d2[pos] = d2v;
+                    s_d3[pos] = d3;
+                    s_d4[pos] = d4;
+                    s_d5[pos] = d5;
+                    s_owner[pos] = i;
+                    s_pid[pos] = pid;
+                }
+                __syncthreads();
+                for (int m = 0; m < cnt; ++m) {
+                    const int owner = s_owner[m];
+                    float cx = 0.f, cy = 0.f, cz = 0.f;
+                    if (aval && j != owner) {
+                        const float dx = ax - s_px[m], dy = ay - s_py[m], dz = az - s_pz[m];
+                        const float dd = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                        if (VK == VOL_GBSW) {
+                            // shell atoms only: (R'-w)^2 < d^2 < (R'+w)^2
+                            if (dd < ap.z && dd > ap.y) {
+                                const float d = fast_sqrt(dd);
+                                const float x = d - ap.x;
+                                const float h = fmaf(x, fmaf(-p.k3 * x, x, p.k1), 0.5f);
+                                if (h > 0.f) {
+                                    const float dh = fmaf(-3.f * p.k3 * x, x, p.k1);
+                                    const float s = s_d0[m] * (s_d1[m] * fast_rcp(h)) * dh * fast_rcp(d);
+                                    cx = s * dx;
+                                    cy = s * dy;
+                                    cz = s * dz;
+                                }
+                            }
+                        } else if (VK == VOL_GBMV2) {
+                            const float t = fmaf(100.f, dd, ap.y);
+                            const float rt = fast_rcp(t);
+                            const float ct = ap.x * rt;
+                            const float F = fminf(ct * ct, p.fclamp);
+                            const float D = -ap.z * rt * rt * rt;
+                            const float vx = s_d3[m], vy = s_d4[m], vz = s_d5[m];
+                            const float vdr = fmaf(vx, dx, fmaf(vy, dy, vz * dz));
+                            const float alpha = fmaf(s_d0[m], D, fmaf(s_d1[m] * F, fmaf(D, dd, F), s_d2[m] * vdr * D));
+                            const float c3F = s_d2[m] * F;
+                            cx = fmaf(alpha, dx, c3F * vx);
+                            cy = fmaf(alpha, dy, c3F * vy);
+                            cz = fmaf(alpha, dz, c3F * vz);
+                        } else {
+                            const float gam = ap.x;
+                            const float s = s_d0[m] * gam * __expf(gam * (dd * dd)) * 4.f * dd;
+                            cx = s * dx;
+                            cy = s * dy;
+                            cz = s * dz;
+                        }
+                    }
+                    accx += (double)rintf(cx * kQuantum);
+                    accy += (double)rintf(cy * kQuantum);
+                    accz += (double)rintf(cz * kQuantum);
+                    const float ox = -wave_sum(cx), oy = -wave_sum(cy), oz = -wave_sum(cz);
+                    if (lane == 0) {
+                        const int pid = s_pid[m];
+                        if (ab == 0) {
+                            p.owner_f[pid] = ox;
+                            p.owner_f[p.NQ + pid] = oy;
+                            p.owner_f[2 * p.NQ + pid] = oz;
+                        } else {
+                            p.owner_f[pid] += ox;
+                            p.owner_f[p.NQ + pid] += oy;
+                            p.owner_f[2 * p.NQ + pid] += oz;
+                        }
+                    }
+                }
+            }
+            if (aval) {
+                // accumulators hold exact integers in units of 2^-24; x 2^8 -> 2^-32 units
+                if (accx != 0.0) atomicAdd((unsigned long long*)&p.force_fix[j], (unsigned long long)(long long)(accx * 256.0));
+                if (accy != 0.0) atomicAdd((unsigned long long*)&p.force_fix[p.N + j], (unsigned long long)(long long)(accy * 256.0));
+                if (accz != 0.0) atomicAdd((unsigned long long*)&p.force_fix[2 * p.N + j], (unsigned long long)(long long)(accz * 256.0));
+            }
+        }
+    }
+}
+
+// Owner forces: one wave per atom, fixed-order FP64 sum over its quadrature points.
+__global__ void __launch_bounds__(64) k_owner_reduce(KParams p) {
+    const Geom G = *p.geom;
+    if (G.flags & kFatal) return;
+    const int i = blockIdx.x, lane = threadIdx.x;
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int q = lane; q < p.Q; q += kWave) {
+        const int pid = i * p.Q + q;
+        a[0] += (double)p.owner_f[pid];
+        a[1] += (double)p.owner_f[p.NQ + pid];
+        a[2] += (double)p.owner_f[2 * p.NQ + pid];
+    }
+    for (int d = 0; d < 3; ++d) a[d] = wave_sum(a[d]);
+    if (lane == 0) {
+        for (int d = 0; d < 3; ++d) p.force_fix[d * p.N + i] += (long long)llrint(a[d] * kFix32);
+    }
+}
+
+__global__ void k_stats(KParams p, double* out) {
+    const Geom G = *p.geom;
+    if (G.flags & kFatal) return;
+    double np = 0.0, pairs = 0.0;
+    const int stride = gridDim.x * blockDim.x;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < G.ncell; c += stride) {
+        const double a = p.cell_pt_count[c];
+        np += a;
+        pairs += a * p.cell_count_eff[c];
+    }
+    np = wave_sum(np);
+    pairs = wave_sum(pairs);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        atomicAdd(&out[0], np);
+        atomicAdd(&out[1], pairs);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        out[2] = G.ncell;
+        out[3] = G.total_entries;
+    }
+}
+
+int g_num_cus = 0;
+
+int persistent_blocks() {
+    if (g_num_cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&g_num_cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            g_num_cus <= 0)
+            g_num_cus = 256;
+    }
+    return g_num_cus * 16;
+}
+
+}  // namespace
+
+hipError_t launch_bbox_only(const KParams& p, hipStream_t stream) {
+    hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, stream, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_evaluate(const KParams& p, unsigned flags, hipStream_t stream) {
+    (void)flags;
+    hipError_t err;
+    const int clear_blocks = (p.cell_cap + 255) / 256;
+    hipLaunchKernelGGL(k_clear, dim3(clear_blocks > 0 ? clear_blocks : 1), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, stream, p);
+    const int atom_blocks = (p.N + 255) / 256;
+    if (p.grid_kind == GRID_BRUTE) {
+        hipLaunchKernelGGL(k_grid_brute, dim3(atom_blocks), dim3(256), 0, stream, p);
+    } else {
+        hipLaunchKernelGGL(k_grid_count, dim3(atom_blocks), dim3(256), 0, stream, p);
+        err = scan_exclusive(p.cell_count, p.cell_start, p.cell_cap, p.scan_tmp, &p.geom->total_entries,
+                             p.entry_cap, &p.geom->flags, FLAG_ENTRY_OVERFLOW, stream);
+        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL(k_grid_fill, dim3(atom_blocks), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL(k_grid_sort, dim3(persistent_blocks()), dim3(64), 0, stream, p);
+    }
+    hipLaunchKernelGGL(k_point_bin, dim3(p.N), dim3(256), 0, stream, p);
+    err = scan_exclusive(p.cell_pt_count, p.cell_pt_start, p.cell_cap, p.scan_tmp, &p.geom->total_points, p.NQ,
+                         &p.geom->flags, FLAG_POINT_OVERFLOW, stream);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_point_scatter, dim3(p.N), dim3(256), 0, stream, p);
+    const int pb = persistent_blocks();
+    switch (p.vol_kind) {
+    case VOL_GBSW: hipLaunchKernelGGL(k_volume<VOL_GBSW>, dim3(pb), dim3(64), 0, stream, p); break;
+    case VOL_GBMV1: hipLaunchKernelGGL(k_volume<VOL_GBMV1>, dim3(pb), dim3(64), 0, stream, p); break;
+    default: hipLaunchKernelGGL(k_volume<VOL_GBMV2>, dim3(pb), dim3(64), 0, stream, p); break;
+    }
+    hipLaunchKernelGGL(k_integrals, dim3(p.N), dim3(64), 0, stream, p);
+    hipLaunchKernelGGL(k_pairs, dim3((p.N + kWave - 1) / kWave, p.pair_splits), dim3(64), 0, stream, p);
+    hipLaunchKernelGGL(k_atom_chain, dim3(atom_blocks), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(k_energy, dim3(1), dim3(1024), 0, stream, p);
+    switch (p.vol_kind) {
+    case VOL_GBSW: hipLaunchKernelGGL(k_force<VOL_GBSW>, dim3(pb), dim3(64), 0, stream, p); break;
+    case VOL_GBMV1: hipLaunchKernelGGL(k_force<VOL_GBMV1>, dim3(pb), dim3(64), 0, stream, p); break;
+    default: hipLaunchKernelGGL(k_force<VOL_GBMV2>, dim3(pb), dim3(64), 0, stream, p); break;
+    }
+    hipLaunchKernelGGL(k_owner_reduce, dim3(p.N), dim3(64), 0, stream, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_stats(const KParams& p, double* out4, hipStream_t stream) {
+    hipLaunchKernelGGL(k_stats, dim3(256), dim3(256), 0, stream, p, out4);
+    return hipGetLastError();
+}
+
+}  // namespace gbi

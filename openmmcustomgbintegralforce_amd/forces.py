@@ -1,0 +1,388 @@
+"""Python mirror of the reference's force API and kernel interface for the GB-integral path.
+
+Classes and method names follow ``openmmapi/include/openmm/CharmmGBSWForce.h``,
+``CharmmGBMVForce.h`` and ``charmmKernels.h``; the HIP kernels behind them are reached
+through the C ABI (``include/gbi.h``).  ``CustomGBIntegralForce`` is an alias of
+``CharmmGBMVForce`` (the "custom GB integral force").
+
+Only the canonical GB forms run on the HIP platform: a ``CharmmGBMVForce`` must define
+integrals of order 4 and 7, the Born radius from them, and the STILL self + pair energy
+(see ``canonical_gbmv_force``).  Other expressions raise ``GbiError(UNSUPPORTED)``.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _native as N
+
+ONE_4PI_EPS0 = 138.935456  # OpenMM 7.x
+
+
+class OpenMMException(RuntimeError):
+    pass
+
+
+# --------------------------------------------------------------------------------------
+# CharmmGBSWForce (CharmmGBSWForce.h:58-222)
+# --------------------------------------------------------------------------------------
+class CharmmGBSWForce:
+    NoCutoff, CutoffNonPeriodic, CutoffPeriodic = 0, 1, 2
+
+    def __init__(self):
+        self._particles = []  # (charge, radius, scalingFactor)
+        self._solvent = 78.3
+        self._solute = 1.0
+        self._sa = 2.25936
+        self._method = self.NoCutoff
+        self._cutoff = 1.0
+
+    def getNumParticles(self) -> int:
+        return len(self._particles)
+
+    def addParticle(self, charge: float, radius: float, scalingFactor: float = 1.0) -> int:
+        self._particles.append((float(charge), float(radius), float(scalingFactor)))
+        return len(self._particles) - 1
+
+    def getParticleParameters(self, index: int):
+        return self._particles[index]
+
+    def setParticleParameters(self, index: int, charge: float, radius: float, scalingFactor: float) -> None:
+        self._particles[index] = (float(charge), float(radius), float(scalingFactor))
+
+    def getSolventDielectric(self) -> float:
+        return self._solvent
+
+    def setSolventDielectric(self, v: float) -> None:
+        self._solvent = float(v)
+
+    def getSoluteDielectric(self) -> float:
+        return self._solute
+
+    def setSoluteDielectric(self, v: float) -> None:
+        self._solute = float(v)
+
+    def getSurfaceAreaEnergy(self) -> float:
+        return self._sa
+
+    def setSurfaceAreaEnergy(self, v: float) -> None:
+        self._sa = float(v)
+
+    def getNonbondedMethod(self) -> int:
+        return self._method
+
+    def setNonbondedMethod(self, m: int) -> None:
+        self._method = int(m)
+
+    def getCutoffDistance(self) -> float:
+        return self._cutoff
+
+    def setCutoffDistance(self, d: float) -> None:
+        self._cutoff = float(d)
+
+    def arrays(self):
+        a = np.array(self._particles, dtype=np.float64).reshape(-1, 3)
+        return a[:, 0].copy(), a[:, 1].copy(), a[:, 2].copy()
+
+
+# --------------------------------------------------------------------------------------
+# CharmmGBMVForce (CharmmGBMVForce.h:63-626)
+# --------------------------------------------------------------------------------------
+class CharmmGBMVForce:
+    NoCutoff, CutoffNonPeriodic, CutoffPeriodic = 0, 1, 2
+    SingleParticle, ParticlePair, ParticlePairNoExclusions = 0, 1, 2
+    GBSWIntegral, GBMVIntegralTypeI, GBMVIntegralTypeII, CustomIntegral = 0, 1, 2, 3
+
+    def __init__(self):
+        self._method = self.NoCutoff
+        self._cutoff = 1.0
+        self._integral_type = self.GBSWIntegral  # CharmmGBMVForce.cpp:51
+        self._param_names = []
+        self._globals = []  # (name, default)
+        self._particles = []
+        self._exclusions = []
+        self._values = []  # (name, expression, type)
+        self._integrals = []  # (name, ints, reals)
+        self._energy = []  # (expression, type)
+        self._param_derivs = []
+
+    # -- bookkeeping (CharmmGBMVForce.h:119-410)
+    def getNumParticles(self):
+        return len(self._particles)
+
+    def getNumExclusions(self):
+        return len(self._exclusions)
+
+    def getNumPerParticleParameters(self):
+        return len(self._param_names)
+
+    def getNumGlobalParameters(self):
+        return len(self._globals)
+
+    def getNumComputedValues(self):
+        return len(self._values)
+
+    def getNumGBIntegrals(self):
+        return len(self._integrals)
+
+    def getNumEnergyTerms(self):
+        return len(self._energy)
+
+    def getNonbondedMethod(self):
+        return self._method
+
+    def setNonbondedMethod(self, m):
+        self._method = int(m)
+
+    def getCutoffDistance(self):
+        return self._cutoff
+
+    def setCutoffDistance(self, d):
+        self._cutoff = float(d)
+
+    def addPerParticleParameter(self, name: str) -> int:
+        self._param_names.append(name)
+        return len(self._param_names) - 1
+
+    def getPerParticleParameterName(self, i):
+        return self._param_names[i]
+
+    def addGlobalParameter(self, name: str, defaultValue: float) -> int:
+        self._globals.append((name, float(defaultValue)))
+        return len(self._globals) - 1
+
+    def getGlobalParameterName(self, i):
+        return self._globals[i][0]
+
+    def getGlobalParameterDefaultValue(self, i):
+        return self._globals[i][1]
+
+    def addEnergyParameterDerivative(self, name: str) -> None:
+        self._param_derivs.append(name)
+
+    def addParticle(self, parameters=()) -> int:
+        self._particles.append([float(v) for v in parameters])
+        return len(self._particles) - 1
+
+    def getParticleParameters(self, i):
+        return list(self._particles[i])
+
+    def setParticleParameters(self, i, parameters):
+        self._particles[i] = [float(v) for v in parameters]
+
+    def addComputedValue(self, name: str, expression: str, type: int) -> int:
+        self._values.append((name, expression, int(type)))
+        return len(self._values) - 1
+
+    def getComputedValueParameters(self, i):
+        return self._values[i]
+
+    def addGBIntegral(self, name: str, parametersInt=(), parametersReal=()) -> int:
+        self._integrals.append((name, [int(v) for v in parametersInt], [float(v) for v in parametersReal]))
+        return len(self._integrals) - 1
+
+    def getGBIntegralParameters(self, i):
+        return self._integrals[i]
+
+    def setGBIntegralType(self, t: int) -> None:
+        self._integral_type = int(t)
+
+    def getGBIntegralType(self) -> int:
+        return self._integral_type
+
+    def addEnergyTerm(self, expression: str, type: int) -> int:
+        self._energy.append((expression, int(type)))
+        return len(self._energy) - 1
+
+    def getEnergyTermParameters(self, i):
+        return self._energy[i]
+
+    def addExclusion(self, p1: int, p2: int) -> int:
+        self._exclusions.append((int(p1), int(p2)))
+        return len(self._exclusions) - 1
+
+
+CustomGBIntegralForce = CharmmGBMVForce
+
+
+def _norm(expr: str) -> str:
+    return "".join(expr.split())
+
+
+# Canonical expressions of the CharmmGBMVForce family recognised by the HIP platform.
+CANONICAL_BORN = ("1/(alpha0*(1/eta-I4)+alpha1*(1/(4*eta^4)-I7)^(1/4))", )
+CANONICAL_SELF = "0.5*P*q^2/B"
+CANONICAL_PAIR = "P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))"
+
+
+def canonical_gbmv_force(charges, radii, integral_type: int, *, prefactor: float | None = None,
+                         lebedev_rule: int | None = None, n_radial: int = 24,
+                         alpha0=-0.180, alpha1=1.817, eta=0.05, slope=1.0, shift=0.0,
+                         solute_dielectric=1.0, solvent_dielectric=80.0) -> CharmmGBMVForce:
+    """A CharmmGBMVForce holding the canonical GB energy the HIP platform implements:
+
+    integrals I4 (order 4) and I7 (order 7); computed value
+    B = slope/(alpha0 (1/eta - I4) + alpha1 (1/(4 eta^4) - I7)^(1/4)) + shift;
+    energy 0.5 P q^2/B (single particle) + P q1 q2/f (pairs, no exclusions),
+    P = -ONE_4PI_EPS0 (1/solute - 1/solvent).
+    ``lebedev_rule`` (parametersInt[1]) and ``n_radial`` (parametersInt[2]) extend the
+    reference, which fixes rule 3 and 24 radial points (GBSWIntegral.cpp:37-40).
+    """
+    f = CharmmGBMVForce()
+    f.addPerParticleParameter("q")
+    f.addPerParticleParameter("radius")
+    P = prefactor if prefactor is not None else -ONE_4PI_EPS0 * (1.0 / solute_dielectric - 1.0 / solvent_dielectric)
+    for name, val in (("alpha0", alpha0), ("alpha1", alpha1), ("eta", eta), ("slope", slope),
+                      ("shift", shift), ("P", P)):
+        f.addGlobalParameter(name, val)
+    ints4 = [4] + ([lebedev_rule, n_radial] if lebedev_rule is not None else [])
+    ints7 = [7] + ([lebedev_rule, n_radial] if lebedev_rule is not None else [])
+    f.addGBIntegral("I4", ints4)
+    f.addGBIntegral("I7", ints7)
+    f.setGBIntegralType(integral_type)
+    f.addComputedValue("B", "slope*" + CANONICAL_BORN[0] + "+shift", CharmmGBMVForce.SingleParticle)
+    f.addEnergyTerm(CANONICAL_SELF, CharmmGBMVForce.SingleParticle)
+    f.addEnergyTerm(CANONICAL_PAIR, CharmmGBMVForce.ParticlePairNoExclusions)
+    for q, r in zip(charges, radii):
+        f.addParticle([q, r])
+    return f
+
+
+_LEBEDEV_RULE_NPTS = {1: 6, 2: 14, 3: 26, 4: 38, 5: 50}
+
+
+def gbmv_params_from_force(force: CharmmGBMVForce) -> tuple:
+    """(gbi Params, charges, radii) of a canonical CharmmGBMVForce; raises otherwise."""
+    t = force.getGBIntegralType()
+    if t == CharmmGBMVForce.CustomIntegral:
+        raise N.GbiError(2, "CustomIntegral (type 3) is not implemented by the reference either")
+    if force.getNonbondedMethod() != CharmmGBMVForce.NoCutoff:
+        raise N.GbiError(2, "only NoCutoff is supported on the HIP platform")
+    names = [force.getPerParticleParameterName(i) for i in range(force.getNumPerParticleParameters())]
+    if "radius" not in names:  # GBSWIntegral.cpp:79-80
+        raise OpenMMException("GBSWIntegral: the per perticle parameter 'radius' must be defined")
+    ints = [force.getGBIntegralParameters(i)[1] for i in range(force.getNumGBIntegrals())]
+    for pi in ints:
+        if len(pi) == 0:
+            raise OpenMMException("GBSWIntegral: the order of integral must be given")
+        if pi[0] < 2:
+            raise OpenMMException("GBSWIntegral: the order of integral must be greater or equal to 2")
+    orders = [pi[0] for pi in ints]
+    if sorted(orders) != [4, 7]:
+        raise N.GbiError(2, "the HIP platform implements the canonical Born map over integrals of order 4 and 7")
+    values = [(n, _norm(e), ty) for n, e, ty in force._values]
+    energies = [(_norm(e), ty) for e, ty in force._energy]
+    canon_values = [("B", _norm("slope*" + CANONICAL_BORN[0] + "+shift"), CharmmGBMVForce.SingleParticle)]
+    canon_energy = [(_norm(CANONICAL_SELF), CharmmGBMVForce.SingleParticle),
+                    (_norm(CANONICAL_PAIR), CharmmGBMVForce.ParticlePairNoExclusions)]
+    if values != canon_values or energies != canon_energy:
+        raise N.GbiError(2, "only the canonical GB expressions run on the HIP platform")
+    g = {force.getGlobalParameterName(i): force.getGlobalParameterDefaultValue(i)
+         for i in range(force.getNumGlobalParameters())}
+    p = N.default_params(N.MODEL_GBMV, t)
+    first = ints[0]
+    if len(first) > 1:
+        p.n_lebedev = _LEBEDEV_RULE_NPTS[first[1]]
+    if len(first) > 2:
+        p.n_radial = first[2]
+    p.orders[0], p.orders[1] = orders[0], orders[1]
+    for k in ("alpha0", "alpha1", "eta", "slope", "shift"):
+        setattr(p, k, g[k])
+    p.prefactor = g["P"]
+    iq, ir = names.index("q"), names.index("radius")
+    parts = np.array([force.getParticleParameters(i) for i in range(force.getNumParticles())], dtype=np.float64)
+    return p, parts[:, iq].copy(), parts[:, ir].copy()
+
+
+# --------------------------------------------------------------------------------------
+# Stand-ins for the OpenMM objects the kernel interface takes
+# --------------------------------------------------------------------------------------
+class System:
+    def __init__(self, n: int):
+        self._n = int(n)
+
+    def getNumParticles(self) -> int:
+        return self._n
+
+
+class ContextImpl:
+    """Holds positions (N x 3 nm) and the force accumulator, as ReferencePlatform::PlatformData."""
+
+    def __init__(self, positions):
+        self.positions = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        self.forces = np.zeros_like(self.positions)
+
+
+# --------------------------------------------------------------------------------------
+# Kernel interface (charmmKernels.h:49-129) implemented on HIP
+# --------------------------------------------------------------------------------------
+class HipCalcCharmmGBSWForceKernel:
+    """CalcCharmmGBSWForceKernel on HIP: initialize / execute / copyParametersToContext."""
+
+    @staticmethod
+    def Name() -> str:
+        return "CalcCharmmGBSWForce"
+
+    def __init__(self, honor_dielectrics: bool = False, **overrides):
+        self._engine = None
+        self._honor = honor_dielectrics
+        self._overrides = overrides
+
+    def initialize(self, system: System, force: CharmmGBSWForce) -> None:
+        if system.getNumParticles() != force.getNumParticles():  # CharmmGBSWForceImpl.cpp:47-48
+            raise OpenMMException("CharmmGBSWForce must have exactly as many particles as the System it belongs to.")
+        if force.getNonbondedMethod() != CharmmGBSWForce.NoCutoff:
+            raise N.GbiError(2, "only NoCutoff is supported on the HIP platform")
+        q, r, _ = force.arrays()
+        p = N.default_params(N.MODEL_GBSW)
+        if self._honor:
+            p.prefactor = -ONE_4PI_EPS0 * (1.0 / force.getSoluteDielectric() - 1.0 / force.getSolventDielectric())
+        for k, v in self._overrides.items():
+            setattr(p, "lambda_" if k == "lambda" else k, v)
+        self._engine = N.Engine(p, q, r)
+
+    def execute(self, context: ContextImpl, includeForces: bool = True, includeEnergy: bool = True) -> float:
+        e, _ = self._engine.execute(context.positions, includeForces, includeEnergy, forces=context.forces)
+        return e if includeEnergy else 0.0
+
+    def copyParametersToContext(self, context: ContextImpl, force: CharmmGBSWForce) -> None:
+        q, r, _ = force.arrays()
+        self._engine.update_parameters(q, r)
+
+    @property
+    def engine(self) -> N.Engine:
+        return self._engine
+
+
+class HipCalcCharmmGBMVForceKernel:
+    """CalcCharmmGBMVForceKernel on HIP for the canonical GB energy."""
+
+    @staticmethod
+    def Name() -> str:
+        return "CalcCharmmGBMVForce"
+
+    def __init__(self, **overrides):
+        self._engine = None
+        self._overrides = overrides
+
+    def initialize(self, system: System, force: CharmmGBMVForce) -> None:
+        if system.getNumParticles() != force.getNumParticles():  # CharmmGBMVForceImpl.cpp:58-59
+            raise OpenMMException("CharmmGBMVForce must have exactly as many particles as the System it belongs to.")
+        p, q, r = gbmv_params_from_force(force)
+        for k, v in self._overrides.items():
+            setattr(p, "lambda_" if k == "lambda" else k, v)
+        self._engine = N.Engine(p, q, r)
+
+    def execute(self, context: ContextImpl, includeForces: bool = True, includeEnergy: bool = True) -> float:
+        e, _ = self._engine.execute(context.positions, includeForces, includeEnergy, forces=context.forces)
+        return e if includeEnergy else 0.0
+
+    def copyParametersToContext(self, context: ContextImpl, force: CharmmGBMVForce) -> None:
+        _, q, r = gbmv_params_from_force(force)
+        self._engine.update_parameters(q, r)
+
+    @property
+    def engine(self) -> N.Engine:
+        return self._engine
+
+
+HipCalcCustomGBIntegralForceKernel = HipCalcCharmmGBMVForceKernel
