@@ -163,6 +163,17 @@ gbi_status gbi_get_stats(gbi_ctx* ctx, gbi_stats* stats);
 gbi_status gbi_get_device_view(gbi_ctx* ctx, gbi_device_view* view);
 gbi_status gbi_set_stream(gbi_ctx* ctx, void* hip_stream); /* NULL: the context's own stream */
 
+/*
+ * Diagnostics (no reference counterpart): HIP events recorded on the context's stream
+ * around each pass of every evaluation while enabled.  Passes: 0 grid build, 1 point
+ * binning, 2 volume, 3 integrals + Born radii, 4 pair energy, 5 dE/dB -> dE/dI + energy,
+ * 6 chain-rule forces, 7 owner-force reduction.
+ */
+#define GBI_PROFILE_PASSES 8
+gbi_status gbi_profile_enable(gbi_ctx* ctx, int32_t enable); /* enabling clears the record */
+gbi_status gbi_profile_read(gbi_ctx* ctx, double* avg_ms, int32_t* launches); /* avg_ms[GBI_PROFILE_PASSES] */
+const char* gbi_profile_pass_name(int32_t pass);
+
 void gbi_destroy(gbi_ctx* ctx);
 const char* gbi_last_error(void);
 int32_t gbi_abi_version(void);

@@ -28,8 +28,10 @@ EXPORTS = [
     "gbi_params_default", "gbi_create", "gbi_execute", "gbi_update_parameters", "gbi_set_positions",
     "gbi_compute", "gbi_synchronize", "gbi_get_energy", "gbi_get_forces", "gbi_get_born_radii",
     "gbi_get_integrals", "gbi_get_stats", "gbi_get_device_view", "gbi_set_stream", "gbi_destroy",
-    "gbi_last_error", "gbi_abi_version", "gbi_quadrature",
+    "gbi_last_error", "gbi_abi_version", "gbi_quadrature", "gbi_profile_enable", "gbi_profile_read",
+    "gbi_profile_pass_name",
 ]
+PROFILE_PASSES = ["grid", "bin", "volume", "integrals", "pairs", "chain", "force", "owner"]
 
 
 class GbiError(RuntimeError):
@@ -124,6 +126,10 @@ def load() -> ctypes.CDLL:
     L.gbi_last_error.restype = ctypes.c_char_p
     L.gbi_abi_version.restype = ctypes.c_int32
     L.gbi_quadrature.argtypes = [ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_int32, dp]
+    L.gbi_profile_enable.argtypes = [vp, ctypes.c_int32]
+    L.gbi_profile_read.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int32)]
+    L.gbi_profile_pass_name.argtypes = [ctypes.c_int32]
+    L.gbi_profile_pass_name.restype = ctypes.c_char_p
     for name in EXPORTS:
         fn = getattr(L, name)
         if fn.restype is ctypes.c_int:  # default -> gbi_status
@@ -236,6 +242,16 @@ class Engine:
         v = DeviceView()
         _check(self._lib.gbi_get_device_view(self._h, ctypes.byref(v)))
         return v
+
+    def profile_enable(self, enable: bool = True) -> None:
+        _check(self._lib.gbi_profile_enable(self._h, int(enable)))
+
+    def profile_read(self) -> tuple:
+        """({pass name: average ms per evaluation}, number of profiled evaluations)."""
+        ms = np.zeros(len(PROFILE_PASSES))
+        n = ctypes.c_int32()
+        _check(self._lib.gbi_profile_read(self._h, _dp(ms), ctypes.byref(n)))
+        return dict(zip(PROFILE_PASSES, ms.tolist())), n.value
 
     def set_stream(self, stream_ptr: int | None) -> None:
         _check(self._lib.gbi_set_stream(self._h, ctypes.c_void_p(stream_ptr or 0)))

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -43,7 +44,7 @@ struct DevBuf {
     size_t n = 0;
     hipError_t ensure(size_t count) {
         if (count <= n && p) return hipSuccess;
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
         hipError_t e = hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
@@ -51,7 +52,7 @@ struct DevBuf {
         return e;
     }
     void release() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
     }
@@ -88,7 +89,20 @@ struct gbi_ctx {
     int pair_splits = 1;
     double gpad = 0.0, halfdiag = 0.0, max_reach = 0.0;
 
+    // diagnostics: kPasses + 1 events per profiled evaluation
+    bool profiling = false;
+    std::vector<std::array<hipEvent_t, gbi::kPasses + 1>> prof_events;
+    size_t prof_used = 0;
+
+    void release_profile() {
+        for (auto& a : prof_events)
+            for (auto& e : a) (void)hipEventDestroy(e);
+        prof_events.clear();
+        prof_used = 0;
+    }
+
     ~gbi_ctx() {
+        release_profile();
         DevBuf<double>* d[] = {&pos, &d_charge, &d_radius, &d_reach, &quad_off, &quad_pw, &integrals, &born,
                                &dbdi, &dedi, &pair_part, &atom_energy, &energy, &stats};
         for (auto* b : d) b->release();
@@ -102,7 +116,7 @@ struct gbi_ctx {
         aux.release();
         owner_f.release();
         force_fix.release();
-        if (own_stream) hipStreamDestroy(own_stream);
+        if (own_stream) (void)hipStreamDestroy(own_stream);
     }
 };
 
@@ -366,7 +380,16 @@ gbi_status run(gbi_ctx& c) {
         if (st != GBI_OK) return st;
     }
     gbi::KParams k = make_kparams(c);
-    HIP_TRY(gbi::launch_evaluate(k, GBI_FORCES | GBI_ENERGY, c.stream));
+    const hipEvent_t* marks = nullptr;
+    if (c.profiling && c.prof_used < 4096) {
+        if (c.prof_used == c.prof_events.size()) {
+            std::array<hipEvent_t, gbi::kPasses + 1> a;
+            for (auto& e : a) HIP_TRY(hipEventCreate(&e));
+            c.prof_events.push_back(a);
+        }
+        marks = c.prof_events[c.prof_used++].data();
+    }
+    HIP_TRY(gbi::launch_evaluate(k, GBI_FORCES | GBI_ENERGY, c.stream, marks));
     c.computed = true;
     return GBI_OK;
 }
@@ -495,7 +518,7 @@ gbi_status gbi_create(int32_t n_atoms, const double* charge, const double* radiu
         c->kind = gbi::GRID_FAITHFUL;
     c->halfdiag = std::sqrt(3.0) / 2.0 * c->P.grid_length;
     c->gpad = 0.03 + c->halfdiag + c->P.buffer;  // CustomGBIntegral.cpp:174-175
-    hipGetDevice(&c->device);
+    (void)hipGetDevice(&c->device);
     auto cleanup = [&](gbi_status s) {
         delete c;
         return s;
@@ -565,7 +588,7 @@ gbi_status gbi_create(int32_t n_atoms, const double* charge, const double* radiu
 
 void gbi_destroy(gbi_ctx* ctx) {
     if (!ctx) return;
-    hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
     delete ctx;
 }
 
@@ -697,6 +720,37 @@ gbi_status gbi_get_stats(gbi_ctx* ctx, gbi_stats* stats) {
     stats->n_atoms = ctx->N;
     stats->n_quad = ctx->Q;
     return GBI_OK;
+}
+
+gbi_status gbi_profile_enable(gbi_ctx* ctx, int32_t enable) {
+    if (!ctx) return fail(GBI_ERR_INVALID_ARGUMENT, "null context");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->profiling = enable != 0;
+    ctx->prof_used = 0;
+    return GBI_OK;
+}
+
+gbi_status gbi_profile_read(gbi_ctx* ctx, double* avg_ms, int32_t* launches) {
+    if (!ctx || !avg_ms) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < gbi::kPasses; ++k) avg_ms[k] = 0.0;
+    for (size_t l = 0; l < ctx->prof_used; ++l) {
+        for (int k = 0; k < gbi::kPasses; ++k) {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, ctx->prof_events[l][k], ctx->prof_events[l][k + 1]));
+            avg_ms[k] += ms;
+        }
+    }
+    if (ctx->prof_used > 0)
+        for (int k = 0; k < gbi::kPasses; ++k) avg_ms[k] /= (double)ctx->prof_used;
+    if (launches) *launches = (int32_t)ctx->prof_used;
+    return GBI_OK;
+}
+
+const char* gbi_profile_pass_name(int32_t pass) {
+    static const char* names[gbi::kPasses] = {"grid", "bin", "volume", "integrals",
+                                              "pairs", "chain", "force", "owner"};
+    return pass >= 0 && pass < gbi::kPasses ? names[pass] : "";
 }
 
 gbi_status gbi_get_device_view(gbi_ctx* ctx, gbi_device_view* view) {

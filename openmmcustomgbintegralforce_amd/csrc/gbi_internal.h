@@ -111,7 +111,9 @@ struct KParams {
 };
 
 // Launchers (gbi_kernels.hip).  All enqueue on `stream`.
-hipError_t launch_evaluate(const KParams& p, unsigned flags, hipStream_t stream);
+// marks: nullptr or kPasses + 1 events recorded before each pass and after the last.
+constexpr int kPasses = 8;
+hipError_t launch_evaluate(const KParams& p, unsigned flags, hipStream_t stream, const hipEvent_t* marks);
 hipError_t launch_bbox_only(const KParams& p, hipStream_t stream);
 hipError_t launch_stats(const KParams& p, double* out4, hipStream_t stream);
 

@@ -1035,9 +1035,13 @@ hipError_t launch_bbox_only(const KParams& p, hipStream_t stream) {
     return hipGetLastError();
 }
 
-hipError_t launch_evaluate(const KParams& p, unsigned flags, hipStream_t stream) {
+hipError_t launch_evaluate(const KParams& p, unsigned flags, hipStream_t stream, const hipEvent_t* marks) {
     (void)flags;
     hipError_t err;
+    auto mark = [&](int k) {
+        if (marks) (void)hipEventRecord(marks[k], stream);
+    };
+    mark(0);
     const int clear_blocks = (p.cell_cap + 255) / 256;
     hipLaunchKernelGGL(k_clear, dim3(clear_blocks > 0 ? clear_blocks : 1), dim3(256), 0, stream, p);
     hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, stream, p);
@@ -1052,27 +1056,35 @@ hipError_t launch_evaluate(const KParams& p, unsigned flags, hipStream_t stream)
         hipLaunchKernelGGL(k_grid_fill, dim3(atom_blocks), dim3(256), 0, stream, p);
         hipLaunchKernelGGL(k_grid_sort, dim3(persistent_blocks()), dim3(64), 0, stream, p);
     }
+    mark(1);
     hipLaunchKernelGGL(k_point_bin, dim3(p.N), dim3(256), 0, stream, p);
     err = scan_exclusive(p.cell_pt_count, p.cell_pt_start, p.cell_cap, p.scan_tmp, &p.geom->total_points, p.NQ,
                          &p.geom->flags, FLAG_POINT_OVERFLOW, stream);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(k_point_scatter, dim3(p.N), dim3(256), 0, stream, p);
     const int pb = persistent_blocks();
+    mark(2);
     switch (p.vol_kind) {
     case VOL_GBSW: hipLaunchKernelGGL(k_volume<VOL_GBSW>, dim3(pb), dim3(64), 0, stream, p); break;
     case VOL_GBMV1: hipLaunchKernelGGL(k_volume<VOL_GBMV1>, dim3(pb), dim3(64), 0, stream, p); break;
     default: hipLaunchKernelGGL(k_volume<VOL_GBMV2>, dim3(pb), dim3(64), 0, stream, p); break;
     }
+    mark(3);
     hipLaunchKernelGGL(k_integrals, dim3(p.N), dim3(64), 0, stream, p);
+    mark(4);
     hipLaunchKernelGGL(k_pairs, dim3((p.N + kWave - 1) / kWave, p.pair_splits), dim3(64), 0, stream, p);
+    mark(5);
     hipLaunchKernelGGL(k_atom_chain, dim3(atom_blocks), dim3(256), 0, stream, p);
     hipLaunchKernelGGL(k_energy, dim3(1), dim3(1024), 0, stream, p);
+    mark(6);
     switch (p.vol_kind) {
     case VOL_GBSW: hipLaunchKernelGGL(k_force<VOL_GBSW>, dim3(pb), dim3(64), 0, stream, p); break;
     case VOL_GBMV1: hipLaunchKernelGGL(k_force<VOL_GBMV1>, dim3(pb), dim3(64), 0, stream, p); break;
     default: hipLaunchKernelGGL(k_force<VOL_GBMV2>, dim3(pb), dim3(64), 0, stream, p); break;
     }
+    mark(7);
     hipLaunchKernelGGL(k_owner_reduce, dim3(p.N), dim3(64), 0, stream, p);
+    mark(8);
     return hipGetLastError();
 }
 

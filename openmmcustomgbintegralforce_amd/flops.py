@@ -1,0 +1,23 @@
+"""Algorithmic FLOP weights per interaction (fixed; never tuned to the kernels).
+
+Convention: add/sub/mul = 1, FMA = 2, sqrt/rsqrt/rcp/div/exp/log = 1 each, counted from
+the reference formulas (GBSWIntegral.cpp:176-182, 295-305; GBMVIntegralTypeI.cpp:164-171,
+205-214; GBMVIntegralTypeII.cpp:174-189, 233-273; CharmmReferenceGBSW.cpp:248-282).
+"""
+C_VOL = {"gbsw": 16, "gbmv1": 12, "gbmv2": 23}   # per point/atom pair of the volume pass
+C_FRC = {"gbsw": 32, "gbmv1": 28, "gbmv2": 60}   # per point/atom pair of the chain-rule pass
+C_QP = 16      # per quadrature point (setup, cell index, accumulation)
+C_PAIR = 40    # per unordered atom pair (STILL energy, dE/dr, dE/dB_i, dE/dB_j, force)
+C_ATOM = 30    # per atom (Born map, self term, dE/dB -> dE/dI)
+
+
+def volume_flops(kind: str, n_vol_pairs: float, n_points: float) -> float:
+    return C_VOL[kind] * n_vol_pairs + C_QP * n_points
+
+
+def force_flops(kind: str, n_frc_pairs: float) -> float:
+    return C_FRC[kind] * n_frc_pairs
+
+
+def pair_flops(n_atoms: int) -> float:
+    return C_PAIR * n_atoms * (n_atoms - 1) / 2.0

@@ -154,6 +154,25 @@ def _apply(p: Params, kw) -> Params:
     return p
 
 
+_SHARED_FIELDS = ["n_radial", "n_lebedev", "r0", "r1", "grid_length", "buffer", "sw", "rscale", "roffset",
+                  "gamma0", "beta", "lambda_", "P1", "P2", "S0", "vguard", "alpha0", "alpha1", "eta", "slope",
+                  "shift", "prefactor"]
+
+
+def params_from_abi(p, nthreads: int = 8, dense: int = 0) -> Params:
+    """Oracle parameters for the model described by a gbi_params structure `p`
+    (read by attribute name: model, integral_type, lookup_policy, cell_cap, orders, ...)."""
+    op = gbsw_standalone_params() if p.model == 0 else gbmv_params(p.integral_type)
+    for f in _SHARED_FIELDS:
+        setattr(op, f, getattr(p, f))
+    op.cap = p.cell_cap
+    op.orders[0], op.orders[1] = p.orders[0], p.orders[1]
+    op.policy = POLICY_BRUTE if p.lookup_policy == 1 else POLICY_GRID
+    op.nthreads = nthreads
+    op.dense = dense
+    return op
+
+
 @dataclass
 class Result:
     energy: float
