@@ -81,12 +81,13 @@ struct gbi_ctx {
     DevBuf<float4> aparam;
     DevBuf<gbi::Geom> geom;
     DevBuf<int> cell_count, cell_fill, cell_start, cell_count_eff, cell_atoms_tmp, cell_atoms, pt_cell, pt_slot,
-        cell_pt_count, cell_pt_start, work, scan_tmp;
-    DevBuf<uint32_t> cell_pts;
-    DevBuf<float> vpt, aux, owner_f;
+        cell_pt_count, cell_pt_start, cell_act_count, worklist, work, scan_tmp;
+    DevBuf<float4> ent_pos, ent_par, prec, frec0, frec1, frec2;
+    DevBuf<float> vpt, owner_f, quad_r2;
     DevBuf<long long> force_fix;
 
     int pair_splits = 1;
+    int perm_stride = 1;
     double gpad = 0.0, halfdiag = 0.0, max_reach = 0.0;
 
     // diagnostics: kPasses + 1 events per profiled evaluation
@@ -109,12 +110,14 @@ struct gbi_ctx {
         aparam.release();
         geom.release();
         DevBuf<int>* ib[] = {&cell_count, &cell_fill, &cell_start, &cell_count_eff, &cell_atoms_tmp, &cell_atoms,
-                             &pt_cell, &pt_slot, &cell_pt_count, &cell_pt_start, &work, &scan_tmp};
+                             &pt_cell, &pt_slot, &cell_pt_count, &cell_pt_start, &cell_act_count, &worklist,
+                             &work, &scan_tmp};
         for (auto* b : ib) b->release();
-        cell_pts.release();
+        DevBuf<float4>* fb[] = {&ent_pos, &ent_par, &prec, &frec0, &frec1, &frec2};
+        for (auto* b : fb) b->release();
         vpt.release();
-        aux.release();
         owner_f.release();
+        quad_r2.release();
         force_fix.release();
         if (own_stream) (void)hipStreamDestroy(own_stream);
     }
@@ -242,6 +245,8 @@ gbi_status ensure_grid_buffers(gbi_ctx& c, long long cells, long long entries) {
         HIP_TRY(c.cell_count_eff.ensure(cells));
         HIP_TRY(c.cell_pt_count.ensure(cells));
         HIP_TRY(c.cell_pt_start.ensure(cells + 1));
+        HIP_TRY(c.cell_act_count.ensure(cells));
+        HIP_TRY(c.worklist.ensure(cells));
         const long long tiles = (cells + 4095) / 4096;
         HIP_TRY(c.scan_tmp.ensure(std::max<long long>(tiles, 1) + 16));
     }
@@ -249,6 +254,8 @@ gbi_status ensure_grid_buffers(gbi_ctx& c, long long cells, long long entries) {
         c.entry_cap = (int)entries;
         HIP_TRY(c.cell_atoms_tmp.ensure(entries));
         HIP_TRY(c.cell_atoms.ensure(entries));
+        HIP_TRY(c.ent_pos.ensure(entries));
+        HIP_TRY(c.ent_par.ensure(entries));
     }
     return GBI_OK;
 }
@@ -314,10 +321,18 @@ gbi::KParams make_kparams(gbi_ctx& c) {
     k.pt_slot = c.pt_slot.p;
     k.cell_pt_count = c.cell_pt_count.p;
     k.cell_pt_start = c.cell_pt_start.p;
-    k.cell_pts = c.cell_pts.p;
+    k.ent_pos = c.ent_pos.p;
+    k.ent_par = c.ent_par.p;
+    k.cell_act_count = c.cell_act_count.p;
+    k.worklist = c.worklist.p;
+    k.prec = c.prec.p;
+    k.frec0 = c.frec0.p;
+    k.frec1 = c.frec1.p;
+    k.frec2 = c.frec2.p;
     k.vpt = c.vpt.p;
-    k.aux = c.aux.p;
     k.owner_f = c.owner_f.p;
+    k.quad_r2 = c.quad_r2.p;
+    k.perm_stride = c.perm_stride;
     k.integrals = c.integrals.p;
     k.born = c.born.p;
     k.dbdi = c.dbdi.p;
@@ -553,9 +568,12 @@ gbi_status gbi_create(int32_t n_atoms, const double* charge, const double* radiu
     CTX_TRY(c->work.ensure(4));
     CTX_TRY(c->pt_cell.ensure(NQ));
     CTX_TRY(c->pt_slot.ensure(NQ));
-    CTX_TRY(c->cell_pts.ensure(NQ));
+    CTX_TRY(c->prec.ensure(NQ));
+    CTX_TRY(c->frec0.ensure(NQ));
+    CTX_TRY(c->frec1.ensure(NQ));
+    CTX_TRY(c->frec2.ensure(c->vol == gbi::VOL_GBMV2 ? NQ : 1));
     CTX_TRY(c->vpt.ensure(NQ));
-    CTX_TRY(c->aux.ensure(c->vol == gbi::VOL_GBSW ? 1 : (c->vol == gbi::VOL_GBMV1 ? NQ : 6 * NQ)));
+    CTX_TRY(c->quad_r2.ensure(Q));
     CTX_TRY(c->owner_f.ensure(3 * NQ));
     CTX_TRY(c->force_fix.ensure(3 * N));
     // pair pass: enough workgroups to fill the device, at least 64 columns per split
@@ -579,6 +597,17 @@ gbi_status gbi_create(int32_t n_atoms, const double* charge, const double* radiu
     }
     CTX_TRY(hipMemcpyAsync(c->quad_off.p, off.data(), sizeof(double) * 3 * Q, hipMemcpyHostToDevice, c->stream));
     CTX_TRY(hipMemcpyAsync(c->quad_pw.p, pw.data(), sizeof(double) * 2 * Q, hipMemcpyHostToDevice, c->stream));
+    std::vector<float> r2(Q);
+    for (size_t q = 0; q < Q; ++q) r2[q] = (float)(table[q][3] * table[q][3]);
+    CTX_TRY(hipMemcpyAsync(c->quad_r2.p, r2.data(), sizeof(float) * Q, hipMemcpyHostToDevice, c->stream));
+    // binning visits atoms with a stride coprime to N so concurrent workgroups hit distant cells
+    c->perm_stride = 1;
+    for (int cand : {7919, 7927, 7933, 7937, 7949})
+        if (n_atoms % cand != 0) {
+            c->perm_stride = cand % n_atoms == 0 ? 1 : cand;
+            break;
+        }
+    CTX_TRY(hipStreamSynchronize(c->stream));
 #undef CTX_TRY
     st = upload_params(*c);
     if (st != GBI_OK) return cleanup(st);

@@ -88,15 +88,23 @@ struct KParams {
     int* cell_count_eff;   // cell_cap
     int* cell_atoms_tmp;   // entry_cap
     int* cell_atoms;       // entry_cap
+    float4* ent_pos;       // entry_cap: atom position relative to the cell corner, .w = atom index bits
+    float4* ent_par;       // entry_cap: per-atom volume parameters (aparam)
     // points
     int* pt_cell;          // NQ
     int* pt_slot;          // NQ
     int* cell_pt_count;    // cell_cap
     int* cell_pt_start;    // cell_cap + 1
-    uint32_t* cell_pts;    // NQ
+    int* cell_act_count;   // cell_cap: points with a nonzero chain-rule factor
+    int* worklist;         // cell_cap: cells holding points
+    float4* prec;          // NQ, cell order: point relative to the cell corner, .w = packed index bits
+    float4* frec0;         // NQ, cell order (active points): copy of prec
+    float4* frec1;         // NQ: GBSW (V, 0, 0, 0) / GBMV1 (dV/dS, ...) / GBMV2 (c1, c2, c3, vx)
+    float4* frec2;         // NQ: GBMV2 (vy, vz, 0, 0)
     float* vpt;            // NQ
-    float* aux;            // 6 x NQ
-    float* owner_f;        // 3 x NQ
+    float* owner_f;        // 3 x NQ: GBSW dV/dp, GBMV owner force
+    const float* quad_r2;  // Q: |offset_q|^2
+    int perm_stride;       // atom visiting order of the binning pass
     // per atom
     double* integrals;     // 2 x N
     double* born;          // N

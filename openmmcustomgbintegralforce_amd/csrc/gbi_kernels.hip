@@ -23,6 +23,7 @@
 
 #include <cfloat>
 #include <climits>
+#include <algorithm>
 #include <cmath>
 
 namespace gbi {
@@ -37,10 +38,6 @@ constexpr int kFatal = FLAG_CELL_OVERFLOW | FLAG_ENTRY_OVERFLOW | FLAG_GRID_TOO_
                        FLAG_NONFINITE_POS | FLAG_POINT_OVERFLOW;
 constexpr double kFix32 = 4294967296.0;  // 2^32
 
-__device__ inline float wave_sum(float v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    return v;
-}
 __device__ inline double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
     return v;
@@ -90,63 +87,6 @@ __device__ inline int point_cell(const KParams& p, const Geom& G, double px, dou
         }
     }
     return (idx[0] * G.n[1] + idx[1]) * G.n[2] + idx[2];
-}
-
-// Calls f(cell) for every cell whose list atom j joins.
-// Faithful: corner within pad_j = R_j + sqrt3/2 g + buffer (CustomGBIntegral.cpp:198-227),
-//           restricted to cells inside the grid.
-// Tight:    distance from x_j to the cell box below reach_j = R'_j + w + margin.
-template <class F>
-__device__ inline void for_each_member_cell(const KParams& p, const Geom& G, int j, F&& f) {
-#pragma clang fp contract(off)
-    const double x[3] = {p.pos[3 * j], p.pos[3 * j + 1], p.pos[3 * j + 2]};
-    const double g = p.g;
-    int b[3], e[3];
-    if (p.grid_kind == GRID_FAITHFUL) {
-        const double pad = (p.radius[j] + p.halfdiag) + p.buffer;
-        for (int d = 0; d < 3; ++d) {
-            b[d] = (int)floor((x[d] - pad - G.origin[d]) / g);
-            e[d] = (int)ceil((x[d] + pad - G.origin[d]) / g);
-            if (b[d] < 0) b[d] = 0;
-            if (e[d] > G.n[d] - 1) e[d] = G.n[d] - 1;
-        }
-        const double pad2 = pad * pad;
-        for (int i = b[0]; i <= e[0]; ++i) {
-            const double dx = (G.origin[0] + i * g) - x[0];
-            const double dx2 = dx * dx;
-            for (int jj = b[1]; jj <= e[1]; ++jj) {
-                const double dy = (G.origin[1] + jj * g) - x[1];
-                const double dxy = dx2 + dy * dy;
-                for (int k = b[2]; k <= e[2]; ++k) {
-                    const double dz = (G.origin[2] + k * g) - x[2];
-                    if (dxy + dz * dz < pad2) f((i * G.n[1] + jj) * G.n[2] + k);
-                }
-            }
-        }
-    } else {
-        const double reach = p.reach[j];
-        for (int d = 0; d < 3; ++d) {
-            b[d] = (int)floor((x[d] - reach - G.origin[d]) * p.inv_g);
-            e[d] = (int)floor((x[d] + reach - G.origin[d]) * p.inv_g);
-            if (b[d] < 0) b[d] = 0;
-            if (e[d] > G.n[d] - 1) e[d] = G.n[d] - 1;
-        }
-        const double r2 = reach * reach;
-        for (int i = b[0]; i <= e[0]; ++i) {
-            const double lo = G.origin[0] + i * g;
-            const double dx = fmax(fmax(lo - x[0], x[0] - (lo + g)), 0.0);
-            for (int jj = b[1]; jj <= e[1]; ++jj) {
-                const double loy = G.origin[1] + jj * g;
-                const double dy = fmax(fmax(loy - x[1], x[1] - (loy + g)), 0.0);
-                const double dxy = dx * dx + dy * dy;
-                for (int k = b[2]; k <= e[2]; ++k) {
-                    const double loz = G.origin[2] + k * g;
-                    const double dz = fmax(fmax(loz - x[2], x[2] - (loz + g)), 0.0);
-                    if (dxy + dz * dz < r2) f((i * G.n[1] + jj) * G.n[2] + k);
-                }
-            }
-        }
-    }
 }
 
 __device__ inline void cell_corner(const KParams& p, const Geom& G, int c, double* corner) {
@@ -284,31 +224,107 @@ __device__ void bbox_to_geom(const KParams& p, double (*smin)[16], double (*smax
     *p.geom = G;
 }
 
-__global__ void k_grid_count(KParams p) {
-    const Geom G = *p.geom;
-    if (G.flags & kFatal) return;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= p.N) return;
-    for_each_member_cell(p, G, j, [&](int c) { atomicAdd(&p.cell_count[c], 1); });
+// Candidate cells of atom j (before the membership test), clamped to the grid.
+struct CandBox {
+    int b[3], e[3];
+    int count;
+};
+
+__device__ inline CandBox candidate_box(const KParams& p, const Geom& G, const double* x, double pad) {
+#pragma clang fp contract(off)
+    CandBox B;
+    for (int d = 0; d < 3; ++d) {
+        if (p.grid_kind == GRID_FAITHFUL) {
+            B.b[d] = (int)floor((x[d] - pad - G.origin[d]) / p.g);
+            B.e[d] = (int)ceil((x[d] + pad - G.origin[d]) / p.g);
+        } else {
+            B.b[d] = (int)floor((x[d] - pad - G.origin[d]) * p.inv_g);
+            B.e[d] = (int)floor((x[d] + pad - G.origin[d]) * p.inv_g);
+        }
+        if (B.b[d] < 0) B.b[d] = 0;
+        if (B.e[d] > G.n[d] - 1) B.e[d] = G.n[d] - 1;
+    }
+    const int nx = B.e[0] - B.b[0] + 1, ny = B.e[1] - B.b[1] + 1, nz = B.e[2] - B.b[2] + 1;
+    B.count = (nx > 0 && ny > 0 && nz > 0) ? nx * ny * nz : 0;
+    return B;
 }
 
-__global__ void k_grid_fill(KParams p) {
-    const Geom G = *p.geom;
-    if (G.flags & kFatal) return;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= p.N) return;
-    for_each_member_cell(p, G, j, [&](int c) {
-        const int s = atomicAdd(&p.cell_fill[c], 1);
-        const int idx = p.cell_start[c] + s;
-        if (idx < p.entry_cap) p.cell_atoms_tmp[idx] = j;
-    });
+// Membership of atom j in cell (i, jj, k).
+// Faithful: corner within pad_j = R_j + sqrt3/2 g + buffer (CustomGBIntegral.cpp:210-223).
+// Tight: distance from x_j to the cell box below reach_j = R'_j + w + margin.
+__device__ inline bool cell_member(const KParams& p, const Geom& G, const double* x, double pad, int i, int jj,
+                                   int k) {
+#pragma clang fp contract(off)
+    const double g = p.g;
+    if (p.grid_kind == GRID_FAITHFUL) {
+        const double dx = (G.origin[0] + i * g) - x[0];
+        const double dy = (G.origin[1] + jj * g) - x[1];
+        const double dz = (G.origin[2] + k * g) - x[2];
+        return dx * dx + dy * dy + dz * dz < pad * pad;
+    }
+    const double lx = G.origin[0] + i * g, ly = G.origin[1] + jj * g, lz = G.origin[2] + k * g;
+    const double dx = fmax(fmax(lx - x[0], x[0] - (lx + g)), 0.0);
+    const double dy = fmax(fmax(ly - x[1], x[1] - (ly + g)), 0.0);
+    const double dz = fmax(fmax(lz - x[2], x[2] - (lz + g)), 0.0);
+    return dx * dx + dy * dy + dz * dz < pad * pad;
 }
 
-// One wave per cell: sort the cell's atoms by index (rank sort) and apply the cap
+__device__ inline double atom_pad(const KParams& p, int j) {
+#pragma clang fp contract(off)
+    return p.grid_kind == GRID_FAITHFUL ? (p.radius[j] + p.halfdiag) + p.buffer : p.reach[j];
+}
+
+// One workgroup per atom, lanes over its candidate cells: count the members of each cell.
+__global__ void __launch_bounds__(256) k_grid_count(KParams p) {
+    const Geom G = *p.geom;
+    if (G.flags & kFatal) return;
+    const int j = blockIdx.x;
+    const double x[3] = {p.pos[3 * j], p.pos[3 * j + 1], p.pos[3 * j + 2]};
+    const double pad = atom_pad(p, j);
+    const CandBox B = candidate_box(p, G, x, pad);
+    const int ny = B.e[1] - B.b[1] + 1, nz = B.e[2] - B.b[2] + 1;
+    for (int t = threadIdx.x; t < B.count; t += blockDim.x) {
+        const int i = B.b[0] + t / (ny * nz);
+        const int r = t - (i - B.b[0]) * (ny * nz);
+        const int jj = B.b[1] + r / nz;
+        const int k = B.b[2] + (r - (jj - B.b[1]) * nz);
+        if (cell_member(p, G, x, pad, i, jj, k)) atomicAdd(&p.cell_count[(i * G.n[1] + jj) * G.n[2] + k], 1);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_grid_fill(KParams p) {
+    const Geom G = *p.geom;
+    if (G.flags & kFatal) return;
+    const int j = blockIdx.x;
+    const double x[3] = {p.pos[3 * j], p.pos[3 * j + 1], p.pos[3 * j + 2]};
+    const double pad = atom_pad(p, j);
+    const CandBox B = candidate_box(p, G, x, pad);
+    const int ny = B.e[1] - B.b[1] + 1, nz = B.e[2] - B.b[2] + 1;
+    for (int t = threadIdx.x; t < B.count; t += blockDim.x) {
+        const int i = B.b[0] + t / (ny * nz);
+        const int r = t - (i - B.b[0]) * (ny * nz);
+        const int jj = B.b[1] + r / nz;
+        const int k = B.b[2] + (r - (jj - B.b[1]) * nz);
+        if (cell_member(p, G, x, pad, i, jj, k)) {
+            const int c = (i * G.n[1] + jj) * G.n[2] + k;
+            const int s = atomicAdd(&p.cell_fill[c], 1);
+            const int idx = p.cell_start[c] + s;
+            if (idx < p.entry_cap) p.cell_atoms_tmp[idx] = j;
+        }
+    }
+}
+
+__device__ inline void write_entry(const KParams& p, int e, int j, const double* corner) {
+    p.ent_pos[e] = make_float4((float)(p.pos[3 * j] - corner[0]), (float)(p.pos[3 * j + 1] - corner[1]),
+                               (float)(p.pos[3 * j + 2] - corner[2]), __int_as_float(j));
+    p.ent_par[e] = p.aparam[j];
+}
+
+// One wave per cell: order the cell's atoms by index (rank sort), keep the first `cap`
 // (the reference appends in ascending atom order and stops at the cap,
-// CustomGBIntegral.cpp:219-222).
+// CustomGBIntegral.cpp:219-222) and write their corner-relative entries.
 __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
-    __shared__ int s_list[1024];
+    __shared__ int s_in[1024], s_out[1024];
     const Geom G = *p.geom;
     if (G.flags & kFatal) return;
     const int lane = threadIdx.x;
@@ -319,22 +335,32 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
             continue;
         }
         const int base = p.cell_start[c];
-        const bool in_lds = n <= 1024;
-        if (in_lds) {
-            for (int k = lane; k < n; k += kWave) s_list[k] = p.cell_atoms_tmp[base + k];
-        }
-        __syncthreads();
-        for (int k = lane; k < n; k += kWave) {
-            const int a = in_lds ? s_list[k] : p.cell_atoms_tmp[base + k];
-            int r = 0;
-            if (in_lds) {
-                for (int m = 0; m < n; ++m) r += s_list[m] < a;
-            } else {
-                for (int m = 0; m < n; ++m) r += p.cell_atoms_tmp[base + m] < a;
+        const int eff = n < p.cap ? n : p.cap;
+        double corner[3];
+        cell_corner(p, G, c, corner);
+        if (n <= 1024) {
+            for (int k = lane; k < n; k += kWave) s_in[k] = p.cell_atoms_tmp[base + k];
+            __syncthreads();
+            for (int k = lane; k < n; k += kWave) {
+                const int a = s_in[k];
+                int r = 0;
+                for (int m = 0; m < n; ++m) r += s_in[m] < a;
+                s_out[r] = a;
             }
-            p.cell_atoms[base + r] = a;
+            __syncthreads();
+            for (int s = lane; s < eff; s += kWave) write_entry(p, base + s, s_out[s], corner);
+        } else {
+            for (int k = lane; k < n; k += kWave) {
+                const int a = p.cell_atoms_tmp[base + k];
+                int r = 0;
+                for (int m = 0; m < n; ++m) r += p.cell_atoms_tmp[base + m] < a;
+                p.cell_atoms[base + r] = a;
+            }
+            __threadfence_block();
+            __syncthreads();
+            for (int s = lane; s < eff; s += kWave) write_entry(p, base + s, p.cell_atoms[base + s], corner);
         }
-        if (lane == 0) p.cell_count_eff[c] = n < p.cap ? n : p.cap;
+        if (lane == 0) p.cell_count_eff[c] = eff;
         __syncthreads();
     }
 }
@@ -343,7 +369,7 @@ __global__ void k_grid_brute(KParams p) {
     const Geom G = *p.geom;
     if (G.flags & kFatal) return;
     const int stride = gridDim.x * blockDim.x;
-    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < p.N; a += stride) p.cell_atoms[a] = a;
+    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < p.N; a += stride) write_entry(p, a, a, G.origin);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         p.cell_count[0] = p.N;
         p.cell_count_eff[0] = p.N;
@@ -445,31 +471,38 @@ hipError_t scan_exclusive(const int* in, int* out, int n, int* tmp, int* total_d
 // Point binning
 // ---------------------------------------------------------------------------
 
-// One workgroup per atom: the cell of every quadrature point x_i + off_q (FP64, the
-// reference's r_q = x_i + quad_q, GBSWIntegral.cpp:132-133).  Points outside the grid or
-// in an empty cell get the empty-list volume directly.
+// One workgroup per atom (visited with a stride coprime to N so concurrent workgroups hit
+// distant cells): the cell of every quadrature point x_i + off_q in FP64 (the reference's
+// r_q = x_i + quad_q, GBSWIntegral.cpp:132-133).  Points outside the grid, in an empty
+// cell, or (GBSW) inside the owner's own burial radius get their volume directly.
 __global__ void __launch_bounds__(256) k_point_bin(KParams p) {
     const Geom G = *p.geom;
     if (G.flags & kFatal) return;
-    const int i = blockIdx.x;
+    const int i = (int)(((long long)blockIdx.x * p.perm_stride) % p.N);
     const double xi = p.pos[3 * i], yi = p.pos[3 * i + 1], zi = p.pos[3 * i + 2];
+    const float own_lo2 = p.aparam[i].y;
     for (int q = threadIdx.x; q < p.Q; q += blockDim.x) {
         const int pid = i * p.Q + q;
-        double px, py, pz;
-        {
+        int c = -1;
+        // only lists that surely hold the owner (exact or brute) may skip it; the reference
+        // GBMV-form grid is under-padded and can miss it (GBSWIntegral.cpp:59-60)
+        const bool owner_buried =
+            p.vol_kind == VOL_GBSW && p.grid_kind != GRID_FAITHFUL && p.quad_r2[q] <= own_lo2;
+        if (!owner_buried) {
 #pragma clang fp contract(off)
-            px = xi + p.quad_off[3 * q];
-            py = yi + p.quad_off[3 * q + 1];
-            pz = zi + p.quad_off[3 * q + 2];
+            const double px = xi + p.quad_off[3 * q];
+            const double py = yi + p.quad_off[3 * q + 1];
+            const double pz = zi + p.quad_off[3 * q + 2];
+            c = point_cell(p, G, px, py, pz);
+            if (c >= 0 && p.cell_count_eff[c] == 0) c = -1;
         }
-        const int c = point_cell(p, G, px, py, pz);
-        if (c >= 0 && p.cell_count_eff[c] > 0) {
+        if (c >= 0) {
             const int s = atomicAdd(&p.cell_pt_count[c], 1);
             p.pt_cell[pid] = c;
             p.pt_slot[pid] = s;
         } else {
             p.pt_cell[pid] = -1;
-            p.vpt[pid] = p.v_empty;
+            p.vpt[pid] = owner_buried ? 0.0f : p.v_empty;
             p.owner_f[pid] = 0.0f;
             p.owner_f[p.NQ + pid] = 0.0f;
             p.owner_f[2 * p.NQ + pid] = 0.0f;
@@ -477,6 +510,8 @@ __global__ void __launch_bounds__(256) k_point_bin(KParams p) {
     }
 }
 
+// Points in cell order: position relative to the cell corner (FP64 difference rounded once
+// to FP32) and the packed (atom, q) index.
 __global__ void __launch_bounds__(256) k_point_scatter(KParams p) {
     const Geom G = *p.geom;
     if (G.flags & kFatal) return;
@@ -484,72 +519,84 @@ __global__ void __launch_bounds__(256) k_point_scatter(KParams p) {
     for (int q = threadIdx.x; q < p.Q; q += blockDim.x) {
         const int pid = i * p.Q + q;
         const int c = p.pt_cell[pid];
-        if (c >= 0) {
-            const int idx = p.cell_pt_start[c] + p.pt_slot[pid];
-            if (idx < p.NQ) p.cell_pts[idx] = ((uint32_t)i << kQBits) | (uint32_t)q;
+        if (c < 0) continue;
+        const int idx = p.cell_pt_start[c] + p.pt_slot[pid];
+        if (idx >= p.NQ) continue;
+        double corner[3];
+        cell_corner(p, G, c, corner);
+        float rx, ry, rz;
+        {
+#pragma clang fp contract(off)
+            rx = (float)((p.pos[3 * i] + p.quad_off[3 * q]) - corner[0]);
+            ry = (float)((p.pos[3 * i + 1] + p.quad_off[3 * q + 1]) - corner[1]);
+            rz = (float)((p.pos[3 * i + 2] + p.quad_off[3 * q + 2]) - corner[2]);
         }
+        p.prec[idx] = make_float4(rx, ry, rz, __uint_as_float(((uint32_t)i << kQBits) | (uint32_t)q));
     }
 }
 
+// Cells holding points (order irrelevant: every later sum is order independent).
+__global__ void k_worklist(KParams p) {
+    const Geom G = *p.geom;
+    if (G.flags & kFatal) return;
+    const int stride = gridDim.x * blockDim.x;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < G.ncell; c += stride)
+        if (p.cell_pt_count[c] > 0) p.worklist[atomicAdd(&p.work[2], 1)] = c;
+}
+
 // ---------------------------------------------------------------------------
-// Volume pass: one wave per cell (work queue), lanes = quadrature points,
-// the cell's atoms staged in LDS relative to the cell corner.
+// Volume pass: one wave per cell, lanes = quadrature points, the cell's atoms staged in
+// LDS (corner-relative FP32).  Writes V per point, the compacted records of the points
+// that carry a chain-rule factor, and (GBSW) the owner gradient dV/dp.
+// GBSW: V = prod_j H_j, early exit on burial (GBSWIntegral.cpp:187-224);
+// GBMV-I: S = sum exp(gamma_j d^4) (GBMVIntegralTypeI.cpp:155-173);
+// GBMV-II: sum1, sum2, v, S = S0 sum1 sum2/|v|^2 (GBMVIntegralTypeII.cpp:163-191).
 // ---------------------------------------------------------------------------
 
 template <int VK>
 __global__ void __launch_bounds__(64) k_volume(KParams p) {
     __shared__ float s_x[kMaxAtomChunk], s_y[kMaxAtomChunk], s_z[kMaxAtomChunk];
     __shared__ float s_a[kMaxAtomChunk], s_b[kMaxAtomChunk], s_c[kMaxAtomChunk];
+    __shared__ int s_j[kMaxAtomChunk];
     const Geom G = *p.geom;
     if (G.flags & kFatal) return;
     const int lane = threadIdx.x;
-    for (;;) {
-        int c = 0;
-        if (lane == 0) c = atomicAdd(&p.work[0], 1);
-        c = __shfl(c, 0, kWave);
-        if (c >= G.ncell) break;
+    const int nwork = p.work[2];
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const int c = p.worklist[w];
         const int np = p.cell_pt_count[c];
         const int na = p.cell_count_eff[c];
-        if (np == 0 || na == 0) continue;
-        double corner[3];
-        cell_corner(p, G, c, corner);
         const int ps = p.cell_pt_start[c];
         const int as = p.cell_start[c];
         const bool single = na <= kMaxAtomChunk;
         auto stage = [&](int ab, int cnt) {
             for (int k = lane; k < cnt; k += kWave) {
-                const int j = p.cell_atoms[as + ab + k];
-                s_x[k] = (float)(p.pos[3 * j] - corner[0]);
-                s_y[k] = (float)(p.pos[3 * j + 1] - corner[1]);
-                s_z[k] = (float)(p.pos[3 * j + 2] - corner[2]);
-                const float4 ap = p.aparam[j];
-                s_a[k] = ap.x;
-                s_b[k] = ap.y;
-                s_c[k] = ap.z;
+                const float4 e = p.ent_pos[as + ab + k];
+                const float4 q = p.ent_par[as + ab + k];
+                s_x[k] = e.x;
+                s_y[k] = e.y;
+                s_z[k] = e.z;
+                s_j[k] = __float_as_int(e.w);
+                s_a[k] = q.x;
+                s_b[k] = q.y;
+                s_c[k] = q.z;
             }
         };
         __syncthreads();
         if (single) stage(0, na);
         __syncthreads();
+        int act = 0;
         for (int pb = 0; pb < np; pb += kWave) {
             const int k = pb + lane;
             const bool valid = k < np;
-            const uint32_t pk = valid ? p.cell_pts[ps + k] : 0u;
+            const float4 pr = valid ? p.prec[ps + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const uint32_t pk = __float_as_uint(pr.w);
             const int i = (int)(pk >> kQBits), q = (int)(pk & ((1u << kQBits) - 1));
-            float prx = 0.f, pry = 0.f, prz = 0.f;
-            if (valid) {
-#pragma clang fp contract(off)
-                const double px = p.pos[3 * i] + p.quad_off[3 * q];
-                const double py = p.pos[3 * i + 1] + p.quad_off[3 * q + 1];
-                const double pz = p.pos[3 * i + 2] + p.quad_off[3 * q + 2];
-                prx = (float)(px - corner[0]);
-                pry = (float)(py - corner[1]);
-                prz = (float)(pz - corner[2]);
-            }
-            // per-kind state
+            const float prx = pr.x, pry = pr.y, prz = pr.z;
             float V = 1.f;
-            bool done = !valid;            // GBSW: buried or invalid
-            float s1 = 0.f, s2 = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;  // GBMV2 sums / GBMV1 S in s1
+            bool done = !valid;
+            float s1 = 0.f, s2 = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;
+            float lx = 0.f, ly = 0.f, lz = 0.f;
             for (int ab = 0; ab < na; ab += kMaxAtomChunk) {
                 const int cnt = min(kMaxAtomChunk, na - ab);
                 if (!single) {
@@ -567,13 +614,20 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                                 done = true;
                                 V = 0.f;
                             } else {
-                                const float x = fast_sqrt(d2) - s_a[m];
+                                const float d = fast_sqrt(d2);
+                                const float x = d - s_a[m];
                                 const float h = fmaf(x, fmaf(-p.k3 * x, x, p.k1), 0.5f);
                                 if (h <= 0.f) {
                                     done = true;
                                     V = 0.f;
                                 } else {
                                     V *= h;
+                                    if (s_j[m] != i) {
+                                        const float f = fmaf(-3.f * p.k3 * x, x, p.k1) * fast_rcp(h) * fast_rcp(d);
+                                        lx = fmaf(f, dx, lx);
+                                        ly = fmaf(f, dy, ly);
+                                        lz = fmaf(f, dz, lz);
+                                    }
                                 }
                             }
                         }
@@ -593,7 +647,7 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                             vz = fmaf(F, dz, vz);
                         }
                     }
-                } else {  // GBMV1
+                } else {
                     if (valid) {
                         for (int m = 0; m < cnt; ++m) {
                             const float dx = s_x[m] - prx, dy = s_y[m] - pry, dz = s_z[m] - prz;
@@ -603,40 +657,57 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                     }
                 }
             }
+            bool active = false;
+            float4 r1 = make_float4(0.f, 0.f, 0.f, 0.f), r2 = make_float4(0.f, 0.f, 0.f, 0.f);
             if (valid) {
                 const int pid = i * p.Q + q;
+                if (VK != VOL_GBSW) {  // the force pass overwrites the slots of active points
+                    p.owner_f[pid] = 0.f;
+                    p.owner_f[p.NQ + pid] = 0.f;
+                    p.owner_f[2 * p.NQ + pid] = 0.f;
+                }
                 if (VK == VOL_GBSW) {
+                    active = V > 0.f && V < 1.f;
+                    const float gs = active ? -V : 0.f;  // dV/dp = -V sum_{j != owner} (H'/H) (x_j - p)/d
                     p.vpt[pid] = V;
+                    p.owner_f[pid] = gs * lx;
+                    p.owner_f[p.NQ + pid] = gs * ly;
+                    p.owner_f[2 * p.NQ + pid] = gs * lz;
+                    r1.x = V;
                 } else if (VK == VOL_GBMV2) {
                     const float vsq = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
                     const float S = p.S0 * s1 * s2 / vsq;
                     const bool sat = !(vsq >= p.vguard) || !(S <= p.ssat);
-                    float Vv = 1.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+                    float Vv = 1.f;
                     if (!sat) {
                         const float pre = __expf(p.beta * (S - p.lambda));
                         const float opre = 1.f + pre;
                         Vv = 1.f / opre;
                         const float kk = -p.beta * pre / (opre * opre) * p.S0;
                         const float ivsq = 1.f / vsq;
-                        c1 = kk * s2 * ivsq;
-                        c2 = 2.f * kk * s1 * ivsq;
-                        c3 = -2.f * kk * s1 * s2 * ivsq * ivsq;
+                        r1 = make_float4(kk * s2 * ivsq, 2.f * kk * s1 * ivsq, -2.f * kk * s1 * s2 * ivsq * ivsq, vx);
+                        r2 = make_float4(vy, vz, 0.f, 0.f);
+                        active = true;
                     }
                     p.vpt[pid] = Vv;
-                    p.aux[pid] = c1;
-                    p.aux[p.NQ + pid] = c2;
-                    p.aux[2 * p.NQ + pid] = c3;
-                    p.aux[3 * p.NQ + pid] = vx;
-                    p.aux[4 * p.NQ + pid] = vy;
-                    p.aux[5 * p.NQ + pid] = vz;
                 } else {
                     const float pre = __expf(p.beta * (s1 - p.lambda));
                     const float opre = 1.f + pre;
                     p.vpt[pid] = 1.f / opre;
-                    p.aux[pid] = -p.beta * pre / (opre * opre);
+                    r1.x = -p.beta * pre / (opre * opre);
+                    active = r1.x != 0.f;
                 }
             }
+            const unsigned long long mask = __ballot(active);
+            if (active) {
+                const int o = ps + act + __popcll(mask & ((1ull << lane) - 1ull));
+                p.frec0[o] = pr;
+                p.frec1[o] = r1;
+                if (VK == VOL_GBMV2) p.frec2[o] = r2;
+            }
+            act += __popcll(mask);
         }
+        if (lane == 0) p.cell_act_count[c] = act;
     }
 }
 
@@ -799,193 +870,205 @@ __global__ void __launch_bounds__(1024) k_energy(KParams p) {
 }
 
 // ---------------------------------------------------------------------------
-// Chain-rule force pass: one wave per cell (work queue), lanes = the cell's atoms,
-// loop over the cell's active points broadcast from LDS.  For a point p of owner i
-// with fac = sum_k dE/dI_k(i) w_q / r_q^(n_k - 2):
-//   F_j     += s fac dV_p/dx_j   (j != i; s = +1 GBSW, -1 GBMV)
-//   F_owner -= s fac sum_j dV_p/dx_j
-// (GBSWIntegral.cpp:271-314, GBMVIntegralTypeI.cpp:195-224,
-//  GBMVIntegralTypeII.cpp:221-282; CharmmReferenceGBMV.cpp:376-383).
+// Chain-rule force pass: one wave per cell.  Lanes form an 8 x 8 tile: lane = atom group
+// (lane & 7) x point group (lane >> 3); a lane owns 8 atoms of the chunk (registers) and
+// 8 points of each 64-point chunk (LDS).  For a point p of owner i with
+// fac = sum_k dE/dI_k(i) w_q / r_q^(n_k - 2) and s = +1 (GBSW) / -1 (GBMV):
+//   F_j     += s fac dV_p/dx_j                 (j != i)
+//   F_owner -= s fac sum_j dV_p/dx_j           (GBMV here; GBSW uses dV/dp from the volume pass)
+// (GBSWIntegral.cpp:271-314, GBMVIntegralTypeI.cpp:195-224, GBMVIntegralTypeII.cpp:221-282;
+//  CharmmReferenceGBMV.cpp:376-383).  Atom partials are quantised to 2^-24 and summed
+// exactly in FP64, so the per-cell totals do not depend on the point order.
 // ---------------------------------------------------------------------------
+
+// Sum over the 8 lanes 8g..8g+7 (DPP: quad xor 1, quad xor 2, half-row mirror).
+__device__ inline float sum8(float v) {
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));
+    return v;
+}
 
 template <int VK>
 __global__ void __launch_bounds__(64) k_force(KParams p) {
-    __shared__ float s_px[kWave], s_py[kWave], s_pz[kWave];
-    __shared__ float s_d0[kWave], s_d1[kWave], s_d2[kWave], s_d3[kWave], s_d4[kWave], s_d5[kWave];
-    __shared__ int s_owner[kWave], s_pid[kWave];
+    __shared__ float q_x[kWave], q_y[kWave], q_z[kWave];
+    __shared__ float q_0[kWave], q_1[kWave], q_2[kWave], q_3[kWave], q_4[kWave], q_5[kWave];
+    __shared__ int q_own[kWave], q_pid[kWave];
     const Geom G = *p.geom;
     if (G.flags & kFatal) return;
-    const int lane = threadIdx.x;
-    for (;;) {
-        int c = 0;
-        if (lane == 0) c = atomicAdd(&p.work[1], 1);
-        c = __shfl(c, 0, kWave);
-        if (c >= G.ncell) break;
-        const int np = p.cell_pt_count[c];
+    const int lane = threadIdx.x, ag = lane & 7, pg = lane >> 3;
+    const int nwork = p.work[2];
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const int c = p.worklist[w];
+        const int nact = p.cell_act_count[c];
+        if (nact == 0) continue;
         const int na = p.cell_count_eff[c];
-        if (np == 0 || na == 0) continue;
-        double corner[3];
-        cell_corner(p, G, c, corner);
         const int ps = p.cell_pt_start[c];
         const int as = p.cell_start[c];
         for (int ab = 0; ab < na; ab += kWave) {
-            const int slot = ab + lane;
-            const bool aval = slot < na;
-            const int j = aval ? p.cell_atoms[as + slot] : -1;
-            float ax = 0.f, ay = 0.f, az = 0.f;
-            float4 ap = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (aval) {
-                ax = (float)(p.pos[3 * j] - corner[0]);
-                ay = (float)(p.pos[3 * j + 1] - corner[1]);
-                az = (float)(p.pos[3 * j + 2] - corner[2]);
-                ap = p.aparam[j];
+            float ax[8], ay[8], az[8], a0[8], a1[8], a2[8];
+            int aj[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int s = ab + ag + 8 * k;
+                float4 e = make_float4(0.f, 0.f, 0.f, __int_as_float(-1)), q = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (s < na) {
+                    e = p.ent_pos[as + s];
+                    q = p.ent_par[as + s];
+                }
+                ax[k] = e.x;
+                ay[k] = e.y;
+                az[k] = e.z;
+                aj[k] = __float_as_int(e.w);
+                a0[k] = q.x;
+                a1[k] = q.y;
+                a2[k] = q.z;
             }
-            double accx = 0.0, accy = 0.0, accz = 0.0;
-            for (int pb = 0; pb < np; pb += kWave) {
-                const int k = pb + lane;
-                const bool pvalid = k < np;
-                int i = 0, q = 0, pid = 0;
-                bool active = false;
-                float d0 = 0.f, d1 = 0.f, d2v = 0.f, d3 = 0.f, d4 = 0.f, d5 = 0.f;
-                float prx = 0.f, pry = 0.f, prz = 0.f;
-                if (pvalid) {
-                    const uint32_t pk = p.cell_pts[ps + k];
-                    i = (int)(pk >> kQBits);
-                    q = (int)(pk & ((1u << kQBits) - 1));
-                    pid = i * p.Q + q;
+            double acc[8][3];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
+            for (int pb = 0; pb < nact; pb += kWave) {
+                const int cntp = min(kWave, nact - pb);
+                __syncthreads();
+                if (lane < cntp) {
+                    const int o = ps + pb + lane;
+                    const float4 r0 = p.frec0[o];
+                    const float4 r1 = p.frec1[o];
+                    const uint32_t pk = __float_as_uint(r0.w);
+                    const int i = (int)(pk >> kQBits), q = (int)(pk & ((1u << kQBits) - 1));
                     const double fac = p.dedi[i] * p.quad_pw[q] + p.dedi[p.N + i] * p.quad_pw[p.Q + q];
                     const float facs = p.sign_chain * (float)fac;
+                    q_x[lane] = r0.x;
+                    q_y[lane] = r0.y;
+                    q_z[lane] = r0.z;
+                    q_own[lane] = i;
+                    q_pid[lane] = i * p.Q + q;
                     if (VK == VOL_GBSW) {
-                        const float V = p.vpt[pid];
-                        active = V > 0.f && V < 1.f;
-                        d0 = facs;
-                        d1 = V;
+                        q_0[lane] = facs;
+                        q_1[lane] = r1.x;
                     } else if (VK == VOL_GBMV2) {
-                        const float c1 = p.aux[pid], c2 = p.aux[p.NQ + pid], c3 = p.aux[2 * p.NQ + pid];
-                        active = (c1 != 0.f) || (c2 != 0.f) || (c3 != 0.f);
-                        d0 = facs * c1;
-                        d1 = facs * c2;
-                        d2v = facs * c3;
-                        d3 = p.aux[3 * p.NQ + pid];
-                        d4 = p.aux[4 * p.NQ + pid];
-                        d5 = p.aux[5 * p.NQ + pid];
+                        const float4 r2 = p.frec2[o];
+                        q_0[lane] = facs * r1.x;
+                        q_1[lane] = facs * r1.y;
+                        q_2[lane] = facs * r1.z;
+                        q_3[lane] = r1.w;
+                        q_4[lane] = r2.x;
+                        q_5[lane] = r2.y;
                     } else {
-                        const float kk = p.aux[pid];
-                        active = kk != 0.f;
-                        d0 = facs * kk;
-                    }
-                    if (active) {
-#pragma clang fp contract(off)
-                        const double px = p.pos[3 * i] + p.quad_off[3 * q];
-                        const double py = p.pos[3 * i + 1] + p.quad_off[3 * q + 1];
-                        const double pz = p.pos[3 * i + 2] + p.quad_off[3 * q + 2];
-                        prx = (float)(px - corner[0]);
-                        pry = (float)(py - corner[1]);
-                        prz = (float)(pz - corner[2]);
-                    } else if (ab == 0) {
-                        p.owner_f[pid] = 0.f;
-                        p.owner_f[p.NQ + pid] = 0.f;
-                        p.owner_f[2 * p.NQ + pid] = 0.f;
+                        q_0[lane] = facs * r1.x;
                     }
                 }
-                const unsigned long long mask = __ballot(active);
-                const int cnt = __popcll(mask);
                 __syncthreads();
-                if (active) {
-                    const int pos = __popcll(mask & ((1ull << lane) - 1ull));
-                    s_px[pos] = prx;
-                    s_py[pos] = pry;
-                    s_pz[pos] = prz;
-                    s_d0[pos] = d0;
-                    s_d1[pos] = d1;
-                    s_d2[pos] = d2v;
-                    s_d3[pos] = d3;
-                    s_d4[pos] = d4;
-                    s_d5[pos] = d5;
-                    s_owner[pos] = i;
-                    s_pid[pos] = pid;
-                }
-                __syncthreads();
-                for (int m = 0; m < cnt; ++m) {
-                    const int owner = s_owner[m];
-                    float cx = 0.f, cy = 0.f, cz = 0.f;
-                    if (aval && j != owner) {
-                        const float dx = ax - s_px[m], dy = ay - s_py[m], dz = az - s_pz[m];
-                        const float dd = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                        if (VK == VOL_GBSW) {
-                            // shell atoms only: (R'-w)^2 < d^2 < (R'+w)^2
-                            if (dd < ap.z && dd > ap.y) {
-                                const float d = fast_sqrt(dd);
-                                const float x = d - ap.x;
-                                const float h = fmaf(x, fmaf(-p.k3 * x, x, p.k1), 0.5f);
-                                if (h > 0.f) {
-                                    const float dh = fmaf(-3.f * p.k3 * x, x, p.k1);
-                                    const float s = s_d0[m] * (s_d1[m] * fast_rcp(h)) * dh * fast_rcp(d);
-                                    cx = s * dx;
-                                    cy = s * dy;
-                                    cz = s * dz;
+                for (int t = 0; t < 8; ++t) {
+                    const int m = pg + 8 * t;
+                    float ox = 0.f, oy = 0.f, oz = 0.f;
+                    if (m < cntp) {
+                        const float px = q_x[m], py = q_y[m], pz = q_z[m];
+                        const int own = q_own[m];
+                        const float d0 = q_0[m], d1 = q_1[m];
+                        const float d2c = VK == VOL_GBMV2 ? q_2[m] : 0.f;
+                        const float vx = VK == VOL_GBMV2 ? q_3[m] : 0.f;
+                        const float vy = VK == VOL_GBMV2 ? q_4[m] : 0.f;
+                        const float vz = VK == VOL_GBMV2 ? q_5[m] : 0.f;
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            if (aj[k] < 0 || aj[k] == own) continue;
+                            const float dx = ax[k] - px, dy = ay[k] - py, dz = az[k] - pz;
+                            const float dd = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                            float s = 0.f, c3F = 0.f;
+                            if (VK == VOL_GBSW) {
+                                // shell atoms only: (R'-w)^2 < d^2 < (R'+w)^2
+                                if (dd < a2[k] && dd > a1[k]) {
+                                    const float d = fast_sqrt(dd);
+                                    const float x = d - a0[k];
+                                    const float h = fmaf(x, fmaf(-p.k3 * x, x, p.k1), 0.5f);
+                                    if (h > 0.f) {
+                                        const float dh = fmaf(-3.f * p.k3 * x, x, p.k1);
+                                        s = d0 * (d1 * fast_rcp(h)) * dh * fast_rcp(d);
+                                    }
                                 }
+                            } else if (VK == VOL_GBMV2) {
+                                const float tt = fmaf(100.f, dd, a1[k]);
+                                const float rt = fast_rcp(tt);
+                                const float ct = a0[k] * rt;
+                                const float F = fminf(ct * ct, p.fclamp);
+                                const float D = -a2[k] * rt * rt * rt;
+                                const float vdr = fmaf(vx, dx, fmaf(vy, dy, vz * dz));
+                                s = fmaf(d0, D, fmaf(d1 * F, fmaf(D, dd, F), d2c * vdr * D));
+                                c3F = d2c * F;
+                            } else {
+                                const float gam = a0[k];
+                                s = d0 * gam * __expf(gam * (dd * dd)) * 4.f * dd;
                             }
-                        } else if (VK == VOL_GBMV2) {
-                            const float t = fmaf(100.f, dd, ap.y);
-                            const float rt = fast_rcp(t);
-                            const float ct = ap.x * rt;
-                            const float F = fminf(ct * ct, p.fclamp);
-                            const float D = -ap.z * rt * rt * rt;
-                            const float vx = s_d3[m], vy = s_d4[m], vz = s_d5[m];
-                            const float vdr = fmaf(vx, dx, fmaf(vy, dy, vz * dz));
-                            const float alpha = fmaf(s_d0[m], D, fmaf(s_d1[m] * F, fmaf(D, dd, F), s_d2[m] * vdr * D));
-                            const float c3F = s_d2[m] * F;
-                            cx = fmaf(alpha, dx, c3F * vx);
-                            cy = fmaf(alpha, dy, c3F * vy);
-                            cz = fmaf(alpha, dz, c3F * vz);
-                        } else {
-                            const float gam = ap.x;
-                            const float s = s_d0[m] * gam * __expf(gam * (dd * dd)) * 4.f * dd;
-                            cx = s * dx;
-                            cy = s * dy;
-                            cz = s * dz;
+                            const float cx = fmaf(s, dx, c3F * vx);
+                            const float cy = fmaf(s, dy, c3F * vy);
+                            const float cz = fmaf(s, dz, c3F * vz);
+                            acc[k][0] += (double)rintf(cx * kQuantum);
+                            acc[k][1] += (double)rintf(cy * kQuantum);
+                            acc[k][2] += (double)rintf(cz * kQuantum);
+                            ox += cx;
+                            oy += cy;
+                            oz += cz;
                         }
                     }
-                    accx += (double)rintf(cx * kQuantum);
-                    accy += (double)rintf(cy * kQuantum);
-                    accz += (double)rintf(cz * kQuantum);
-                    const float ox = -wave_sum(cx), oy = -wave_sum(cy), oz = -wave_sum(cz);
-                    if (lane == 0) {
-                        const int pid = s_pid[m];
-                        if (ab == 0) {
-                            p.owner_f[pid] = ox;
-                            p.owner_f[p.NQ + pid] = oy;
-                            p.owner_f[2 * p.NQ + pid] = oz;
-                        } else {
-                            p.owner_f[pid] += ox;
-                            p.owner_f[p.NQ + pid] += oy;
-                            p.owner_f[2 * p.NQ + pid] += oz;
+                    if (VK != VOL_GBSW) {
+                        ox = sum8(ox);
+                        oy = sum8(oy);
+                        oz = sum8(oz);
+                        if (ag == 0 && m < cntp) {
+                            const int pid = q_pid[m];
+                            if (ab == 0) {
+                                p.owner_f[pid] = -ox;
+                                p.owner_f[p.NQ + pid] = -oy;
+                                p.owner_f[2 * p.NQ + pid] = -oz;
+                            } else {
+                                p.owner_f[pid] -= ox;
+                                p.owner_f[p.NQ + pid] -= oy;
+                                p.owner_f[2 * p.NQ + pid] -= oz;
+                            }
                         }
                     }
                 }
             }
-            if (aval) {
-                // accumulators hold exact integers in units of 2^-24; x 2^8 -> 2^-32 units
-                if (accx != 0.0) atomicAdd((unsigned long long*)&p.force_fix[j], (unsigned long long)(long long)(accx * 256.0));
-                if (accy != 0.0) atomicAdd((unsigned long long*)&p.force_fix[p.N + j], (unsigned long long)(long long)(accy * 256.0));
-                if (accz != 0.0) atomicAdd((unsigned long long*)&p.force_fix[2 * p.N + j], (unsigned long long)(long long)(accz * 256.0));
+            // exact sums over the point groups (lanes ag + 8 pg)
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                for (int d = 0; d < 3; ++d) {
+                    double v = acc[k][d];
+                    v += __shfl_xor(v, 8, kWave);
+                    v += __shfl_xor(v, 16, kWave);
+                    v += __shfl_xor(v, 32, kWave);
+                    acc[k][d] = v;
+                }
+            if (pg == 0) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (aj[k] < 0) continue;
+                    for (int d = 0; d < 3; ++d)
+                        if (acc[k][d] != 0.0)  // exact integers in units of 2^-24; x 2^8 -> 2^-32 units
+                            atomicAdd((unsigned long long*)&p.force_fix[d * p.N + aj[k]],
+                                      (unsigned long long)(long long)(acc[k][d] * 256.0));
+                }
             }
         }
     }
 }
 
 // Owner forces: one wave per atom, fixed-order FP64 sum over its quadrature points.
+// GBSW: F_i += sum_q fac(i,q) dV/dp(i,q); GBMV: F_i += sum_q owner force(i,q).
 __global__ void __launch_bounds__(64) k_owner_reduce(KParams p) {
     const Geom G = *p.geom;
     if (G.flags & kFatal) return;
     const int i = blockIdx.x, lane = threadIdx.x;
+    const bool gbsw = p.vol_kind == VOL_GBSW;
+    const double e4 = p.dedi[i], e7 = p.dedi[p.N + i];
     double a[3] = {0.0, 0.0, 0.0};
     for (int q = lane; q < p.Q; q += kWave) {
         const int pid = i * p.Q + q;
-        a[0] += (double)p.owner_f[pid];
-        a[1] += (double)p.owner_f[p.NQ + pid];
-        a[2] += (double)p.owner_f[2 * p.NQ + pid];
+        const double wq = gbsw ? (double)p.sign_chain * (e4 * p.quad_pw[q] + e7 * p.quad_pw[p.Q + q]) : 1.0;
+        a[0] += wq * (double)p.owner_f[pid];
+        a[1] += wq * (double)p.owner_f[p.NQ + pid];
+        a[2] += wq * (double)p.owner_f[2 * p.NQ + pid];
     }
     for (int d = 0; d < 3; ++d) a[d] = wave_sum(a[d]);
     if (lane == 0) {
@@ -1025,7 +1108,7 @@ int persistent_blocks() {
             g_num_cus <= 0)
             g_num_cus = 256;
     }
-    return g_num_cus * 16;
+    return g_num_cus * 32;
 }
 
 }  // namespace
@@ -1049,12 +1132,12 @@ hipError_t launch_evaluate(const KParams& p, unsigned flags, hipStream_t stream,
     if (p.grid_kind == GRID_BRUTE) {
         hipLaunchKernelGGL(k_grid_brute, dim3(atom_blocks), dim3(256), 0, stream, p);
     } else {
-        hipLaunchKernelGGL(k_grid_count, dim3(atom_blocks), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL(k_grid_count, dim3(p.N), dim3(256), 0, stream, p);
         err = scan_exclusive(p.cell_count, p.cell_start, p.cell_cap, p.scan_tmp, &p.geom->total_entries,
                              p.entry_cap, &p.geom->flags, FLAG_ENTRY_OVERFLOW, stream);
         if (err != hipSuccess) return err;
-        hipLaunchKernelGGL(k_grid_fill, dim3(atom_blocks), dim3(256), 0, stream, p);
-        hipLaunchKernelGGL(k_grid_sort, dim3(persistent_blocks()), dim3(64), 0, stream, p);
+        hipLaunchKernelGGL(k_grid_fill, dim3(p.N), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL(k_grid_sort, dim3(std::min(p.cell_cap, 8192)), dim3(64), 0, stream, p);
     }
     mark(1);
     hipLaunchKernelGGL(k_point_bin, dim3(p.N), dim3(256), 0, stream, p);
@@ -1062,7 +1145,8 @@ hipError_t launch_evaluate(const KParams& p, unsigned flags, hipStream_t stream,
                          &p.geom->flags, FLAG_POINT_OVERFLOW, stream);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(k_point_scatter, dim3(p.N), dim3(256), 0, stream, p);
-    const int pb = persistent_blocks();
+    hipLaunchKernelGGL(k_worklist, dim3(std::min((p.cell_cap + 255) / 256, 4096)), dim3(256), 0, stream, p);
+    const int pb = std::min(p.cell_cap, persistent_blocks());
     mark(2);
     switch (p.vol_kind) {
     case VOL_GBSW: hipLaunchKernelGGL(k_volume<VOL_GBSW>, dim3(pb), dim3(64), 0, stream, p); break;
