@@ -80,10 +80,11 @@ struct gbi_ctx {
         atom_energy, energy, stats;
     DevBuf<float4> aparam;
     DevBuf<gbi::Geom> geom;
-    DevBuf<int> cell_count, cell_fill, cell_start, cell_count_eff, cell_atoms_tmp, cell_atoms, pt_cell, pt_slot,
-        cell_pt_count, cell_pt_start, cell_act_count, worklist, work, scan_tmp;
-    DevBuf<float4> ent_pos, ent_par, prec, frec0, frec1, frec2;
-    DevBuf<float> vpt, owner_f, quad_r2;
+    DevBuf<int> cell_count, cell_fill, cell_start, cell_count_eff, cell_atoms_tmp, cell_atoms, cell_pt_count,
+        cell_act_count, frec_base, worklist, work, scan_tmp;
+    DevBuf<float4> ent_pos, ent_par, prec, frec0, frec1, frec2, vrec;
+    DevBuf<float> quad_r2;
+    int pt_cap = 0;
     DevBuf<long long> force_fix;
 
     int pair_splits = 1;
@@ -110,13 +111,10 @@ struct gbi_ctx {
         aparam.release();
         geom.release();
         DevBuf<int>* ib[] = {&cell_count, &cell_fill, &cell_start, &cell_count_eff, &cell_atoms_tmp, &cell_atoms,
-                             &pt_cell, &pt_slot, &cell_pt_count, &cell_pt_start, &cell_act_count, &worklist,
-                             &work, &scan_tmp};
+                             &cell_pt_count, &cell_act_count, &frec_base, &worklist, &work, &scan_tmp};
         for (auto* b : ib) b->release();
-        DevBuf<float4>* fb[] = {&ent_pos, &ent_par, &prec, &frec0, &frec1, &frec2};
+        DevBuf<float4>* fb[] = {&ent_pos, &ent_par, &prec, &frec0, &frec1, &frec2, &vrec};
         for (auto* b : fb) b->release();
-        vpt.release();
-        owner_f.release();
         quad_r2.release();
         force_fix.release();
         if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -244,12 +242,13 @@ gbi_status ensure_grid_buffers(gbi_ctx& c, long long cells, long long entries) {
         HIP_TRY(c.cell_start.ensure(cells + 1));
         HIP_TRY(c.cell_count_eff.ensure(cells));
         HIP_TRY(c.cell_pt_count.ensure(cells));
-        HIP_TRY(c.cell_pt_start.ensure(cells + 1));
         HIP_TRY(c.cell_act_count.ensure(cells));
+        HIP_TRY(c.frec_base.ensure(cells + 1));
         HIP_TRY(c.worklist.ensure(cells));
         const long long tiles = (cells + 4095) / 4096;
         HIP_TRY(c.scan_tmp.ensure(std::max<long long>(tiles, 1) + 16));
     }
+    HIP_TRY(c.prec.ensure((size_t)c.cell_cap * c.pt_cap));
     if (entries > c.entry_cap) {
         c.entry_cap = (int)entries;
         HIP_TRY(c.cell_atoms_tmp.ensure(entries));
@@ -317,20 +316,18 @@ gbi::KParams make_kparams(gbi_ctx& c) {
     k.cell_count_eff = c.cell_count_eff.p;
     k.cell_atoms_tmp = c.cell_atoms_tmp.p;
     k.cell_atoms = c.cell_atoms.p;
-    k.pt_cell = c.pt_cell.p;
-    k.pt_slot = c.pt_slot.p;
     k.cell_pt_count = c.cell_pt_count.p;
-    k.cell_pt_start = c.cell_pt_start.p;
     k.ent_pos = c.ent_pos.p;
     k.ent_par = c.ent_par.p;
     k.cell_act_count = c.cell_act_count.p;
+    k.frec_base = c.frec_base.p;
+    k.pt_cap = c.pt_cap;
     k.worklist = c.worklist.p;
     k.prec = c.prec.p;
     k.frec0 = c.frec0.p;
     k.frec1 = c.frec1.p;
     k.frec2 = c.frec2.p;
-    k.vpt = c.vpt.p;
-    k.owner_f = c.owner_f.p;
+    k.vrec = c.vrec.p;
     k.quad_r2 = c.quad_r2.p;
     k.perm_stride = c.perm_stride;
     k.integrals = c.integrals.p;
@@ -375,6 +372,7 @@ gbi_status check_flags(gbi_ctx& c, bool& retry) {
                     "CharmmGBMVForce: CustomGBIntegral lookup table dimension is too large, check atom positions!");
     if (G.flags & (gbi::FLAG_CELL_OVERFLOW | gbi::FLAG_ENTRY_OVERFLOW | gbi::FLAG_POINT_OVERFLOW)) {
         long long cells = (long long)c.cell_cap, entries = c.entry_cap;
+        if (G.flags & gbi::FLAG_POINT_OVERFLOW) c.pt_cap = std::min(2 * c.pt_cap, c.N * c.Q);
         if (G.flags & gbi::FLAG_CELL_OVERFLOW) cells = (long long)(c.cell_cap * 2ll);
         if (G.flags & gbi::FLAG_ENTRY_OVERFLOW) entries = std::max<long long>(2ll * c.entry_cap, (long long)G.total_entries * 5 / 4);
         st = size_from_device(c);
@@ -532,6 +530,12 @@ gbi_status gbi_create(int32_t n_atoms, const double* charge, const double* radiu
     else
         c->kind = gbi::GRID_FAITHFUL;
     c->halfdiag = std::sqrt(3.0) / 2.0 * c->P.grid_length;
+    {
+        const double gc = c->kind == gbi::GRID_FAITHFUL ? c->P.grid_length : c->P.tight_grid_length;
+        const double est = 1.6 * 100.0 * c->Q * gc * gc * gc + 64.0;
+        c->pt_cap = c->kind == gbi::GRID_BRUTE ? n_atoms * c->Q : std::min((int)est, n_atoms * c->Q);
+        c->pt_cap = std::max(c->pt_cap, 1);
+    }
     c->gpad = 0.03 + c->halfdiag + c->P.buffer;  // CustomGBIntegral.cpp:174-175
     (void)hipGetDevice(&c->device);
     auto cleanup = [&](gbi_status s) {
@@ -566,15 +570,11 @@ gbi_status gbi_create(int32_t n_atoms, const double* charge, const double* radiu
     CTX_TRY(c->stats.ensure(4));
     CTX_TRY(c->geom.ensure(1));
     CTX_TRY(c->work.ensure(4));
-    CTX_TRY(c->pt_cell.ensure(NQ));
-    CTX_TRY(c->pt_slot.ensure(NQ));
-    CTX_TRY(c->prec.ensure(NQ));
     CTX_TRY(c->frec0.ensure(NQ));
     CTX_TRY(c->frec1.ensure(NQ));
     CTX_TRY(c->frec2.ensure(c->vol == gbi::VOL_GBMV2 ? NQ : 1));
-    CTX_TRY(c->vpt.ensure(NQ));
+    CTX_TRY(c->vrec.ensure(NQ));
     CTX_TRY(c->quad_r2.ensure(Q));
-    CTX_TRY(c->owner_f.ensure(3 * NQ));
     CTX_TRY(c->force_fix.ensure(3 * N));
     // pair pass: enough workgroups to fill the device, at least 64 columns per split
     const int rows = (int)((N + 63) / 64);

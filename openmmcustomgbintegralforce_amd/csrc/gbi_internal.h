@@ -91,18 +91,16 @@ struct KParams {
     float4* ent_pos;       // entry_cap: atom position relative to the cell corner, .w = atom index bits
     float4* ent_par;       // entry_cap: per-atom volume parameters (aparam)
     // points
-    int* pt_cell;          // NQ
-    int* pt_slot;          // NQ
+    int pt_cap;            // point slots per cell
     int* cell_pt_count;    // cell_cap
-    int* cell_pt_start;    // cell_cap + 1
     int* cell_act_count;   // cell_cap: points with a nonzero chain-rule factor
+    int* frec_base;        // cell_cap + 1: first record of the cell's active points (scan of counts)
     int* worklist;         // cell_cap: cells holding points
-    float4* prec;          // NQ, cell order: point relative to the cell corner, .w = packed index bits
-    float4* frec0;         // NQ, cell order (active points): copy of prec
-    float4* frec1;         // NQ: GBSW (V, 0, 0, 0) / GBMV1 (dV/dS, ...) / GBMV2 (c1, c2, c3, vx)
-    float4* frec2;         // NQ: GBMV2 (vy, vz, 0, 0)
-    float* vpt;            // NQ
-    float* owner_f;        // 3 x NQ: GBSW dV/dp, GBMV owner force
+    float4* prec;          // cell_cap x pt_cap: point relative to the cell corner, .w = packed index bits
+    float4* frec0;         // NQ (active points, grouped by cell): copy of prec
+    float4* frec1;         // NQ: GBSW (V, 0, 0, 0) / GBMV1 (dV/dS, V, 0, 0) / GBMV2 (c1, c2, c3, vx)
+    float4* frec2;         // NQ: GBMV2 (vy, vz, V, 0)
+    float4* vrec;          // NQ by (atom, q): (V, owner vector) -- GBSW dV/dp, GBMV owner force
     const float* quad_r2;  // Q: |offset_q|^2
     int perm_stride;       // atom visiting order of the binning pass
     // per atom

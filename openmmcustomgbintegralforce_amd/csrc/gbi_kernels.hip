@@ -473,8 +473,16 @@ hipError_t scan_exclusive(const int* in, int* out, int n, int* tmp, int* total_d
 
 // One workgroup per atom (visited with a stride coprime to N so concurrent workgroups hit
 // distant cells): the cell of every quadrature point x_i + off_q in FP64 (the reference's
-// r_q = x_i + quad_q, GBSWIntegral.cpp:132-133).  Points outside the grid, in an empty
-// cell, or (GBSW) inside the owner's own burial radius get their volume directly.
+// r_q = x_i + quad_q, GBSWIntegral.cpp:132-133), and its corner-relative FP32 record in the
+// cell's slots.  Points outside the grid, in an empty cell, or (GBSW) inside the owner's own
+// burial radius get their volume directly.
+__device__ inline void quad_point(const KParams& p, double xi, double yi, double zi, int q, double* pt) {
+#pragma clang fp contract(off)
+    pt[0] = xi + p.quad_off[3 * q];
+    pt[1] = yi + p.quad_off[3 * q + 1];
+    pt[2] = zi + p.quad_off[3 * q + 2];
+}
+
 __global__ void __launch_bounds__(256) k_point_bin(KParams p) {
     const Geom G = *p.geom;
     if (G.flags & kFatal) return;
@@ -484,54 +492,30 @@ __global__ void __launch_bounds__(256) k_point_bin(KParams p) {
     for (int q = threadIdx.x; q < p.Q; q += blockDim.x) {
         const int pid = i * p.Q + q;
         int c = -1;
+        double pt[3] = {0.0, 0.0, 0.0};
         // only lists that surely hold the owner (exact or brute) may skip it; the reference
         // GBMV-form grid is under-padded and can miss it (GBSWIntegral.cpp:59-60)
         const bool owner_buried =
             p.vol_kind == VOL_GBSW && p.grid_kind != GRID_FAITHFUL && p.quad_r2[q] <= own_lo2;
         if (!owner_buried) {
-#pragma clang fp contract(off)
-            const double px = xi + p.quad_off[3 * q];
-            const double py = yi + p.quad_off[3 * q + 1];
-            const double pz = zi + p.quad_off[3 * q + 2];
-            c = point_cell(p, G, px, py, pz);
+            quad_point(p, xi, yi, zi, q, pt);
+            c = point_cell(p, G, pt[0], pt[1], pt[2]);
             if (c >= 0 && p.cell_count_eff[c] == 0) c = -1;
         }
         if (c >= 0) {
             const int s = atomicAdd(&p.cell_pt_count[c], 1);
-            p.pt_cell[pid] = c;
-            p.pt_slot[pid] = s;
+            if (s < p.pt_cap) {
+                double corner[3];
+                cell_corner(p, G, c, corner);
+                p.prec[(size_t)c * p.pt_cap + s] =
+                    make_float4((float)(pt[0] - corner[0]), (float)(pt[1] - corner[1]), (float)(pt[2] - corner[2]),
+                                __uint_as_float(((uint32_t)i << kQBits) | (uint32_t)q));
+            } else {
+                atomicOr(&p.geom->flags, FLAG_POINT_OVERFLOW);
+            }
         } else {
-            p.pt_cell[pid] = -1;
-            p.vpt[pid] = owner_buried ? 0.0f : p.v_empty;
-            p.owner_f[pid] = 0.0f;
-            p.owner_f[p.NQ + pid] = 0.0f;
-            p.owner_f[2 * p.NQ + pid] = 0.0f;
+            p.vrec[pid] = make_float4(owner_buried ? 0.0f : p.v_empty, 0.f, 0.f, 0.f);
         }
-    }
-}
-
-// Points in cell order: position relative to the cell corner (FP64 difference rounded once
-// to FP32) and the packed (atom, q) index.
-__global__ void __launch_bounds__(256) k_point_scatter(KParams p) {
-    const Geom G = *p.geom;
-    if (G.flags & kFatal) return;
-    const int i = blockIdx.x;
-    for (int q = threadIdx.x; q < p.Q; q += blockDim.x) {
-        const int pid = i * p.Q + q;
-        const int c = p.pt_cell[pid];
-        if (c < 0) continue;
-        const int idx = p.cell_pt_start[c] + p.pt_slot[pid];
-        if (idx >= p.NQ) continue;
-        double corner[3];
-        cell_corner(p, G, c, corner);
-        float rx, ry, rz;
-        {
-#pragma clang fp contract(off)
-            rx = (float)((p.pos[3 * i] + p.quad_off[3 * q]) - corner[0]);
-            ry = (float)((p.pos[3 * i + 1] + p.quad_off[3 * q + 1]) - corner[1]);
-            rz = (float)((p.pos[3 * i + 2] + p.quad_off[3 * q + 2]) - corner[2]);
-        }
-        p.prec[idx] = make_float4(rx, ry, rz, __uint_as_float(((uint32_t)i << kQBits) | (uint32_t)q));
     }
 }
 
@@ -564,11 +548,12 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
     const int nwork = p.work[2];
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         const int c = p.worklist[w];
-        const int np = p.cell_pt_count[c];
+        const int np = min(p.cell_pt_count[c], p.pt_cap);
         const int na = p.cell_count_eff[c];
-        const int ps = p.cell_pt_start[c];
+        const size_t ps = (size_t)c * p.pt_cap;
         const int as = p.cell_start[c];
         const bool single = na <= kMaxAtomChunk;
+        const int fb = p.frec_base[c];  // exclusive scan of the point counts
         auto stage = [&](int ab, int cnt) {
             for (int k = lane; k < cnt; k += kWave) {
                 const float4 e = p.ent_pos[as + ab + k];
@@ -661,18 +646,10 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
             float4 r1 = make_float4(0.f, 0.f, 0.f, 0.f), r2 = make_float4(0.f, 0.f, 0.f, 0.f);
             if (valid) {
                 const int pid = i * p.Q + q;
-                if (VK != VOL_GBSW) {  // the force pass overwrites the slots of active points
-                    p.owner_f[pid] = 0.f;
-                    p.owner_f[p.NQ + pid] = 0.f;
-                    p.owner_f[2 * p.NQ + pid] = 0.f;
-                }
                 if (VK == VOL_GBSW) {
                     active = V > 0.f && V < 1.f;
                     const float gs = active ? -V : 0.f;  // dV/dp = -V sum_{j != owner} (H'/H) (x_j - p)/d
-                    p.vpt[pid] = V;
-                    p.owner_f[pid] = gs * lx;
-                    p.owner_f[p.NQ + pid] = gs * ly;
-                    p.owner_f[2 * p.NQ + pid] = gs * lz;
+                    p.vrec[pid] = make_float4(V, gs * lx, gs * ly, gs * lz);
                     r1.x = V;
                 } else if (VK == VOL_GBMV2) {
                     const float vsq = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
@@ -686,21 +663,23 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                         const float kk = -p.beta * pre / (opre * opre) * p.S0;
                         const float ivsq = 1.f / vsq;
                         r1 = make_float4(kk * s2 * ivsq, 2.f * kk * s1 * ivsq, -2.f * kk * s1 * s2 * ivsq * ivsq, vx);
-                        r2 = make_float4(vy, vz, 0.f, 0.f);
+                        r2 = make_float4(vy, vz, Vv, 0.f);
                         active = true;
                     }
-                    p.vpt[pid] = Vv;
+                    p.vrec[pid] = make_float4(Vv, 0.f, 0.f, 0.f);  // the force pass fills the owner force
                 } else {
                     const float pre = __expf(p.beta * (s1 - p.lambda));
                     const float opre = 1.f + pre;
-                    p.vpt[pid] = 1.f / opre;
+                    const float Vv = 1.f / opre;
+                    p.vrec[pid] = make_float4(Vv, 0.f, 0.f, 0.f);
                     r1.x = -p.beta * pre / (opre * opre);
+                    r1.y = Vv;
                     active = r1.x != 0.f;
                 }
             }
             const unsigned long long mask = __ballot(active);
             if (active) {
-                const int o = ps + act + __popcll(mask & ((1ull << lane) - 1ull));
+                const int o = fb + act + __popcll(mask & ((1ull << lane) - 1ull));
                 p.frec0[o] = pr;
                 p.frec1[o] = r1;
                 if (VK == VOL_GBMV2) p.frec2[o] = r2;
@@ -726,7 +705,7 @@ __global__ void __launch_bounds__(64) k_integrals(KParams p) {
     double a0 = 0.0, a1 = 0.0;
     const bool gbsw = p.vol_kind == VOL_GBSW;
     for (int q = lane; q < p.Q; q += kWave) {
-        const double v = (double)p.vpt[i * p.Q + q];
+        const double v = (double)p.vrec[i * p.Q + q].x;
         const double f = gbsw ? 1.0 - v : v;
         a0 += p.quad_pw[q] * f;
         a1 += p.quad_pw[p.Q + q] * f;
@@ -894,6 +873,7 @@ __global__ void __launch_bounds__(64) k_force(KParams p) {
     __shared__ float q_x[kWave], q_y[kWave], q_z[kWave];
     __shared__ float q_0[kWave], q_1[kWave], q_2[kWave], q_3[kWave], q_4[kWave], q_5[kWave];
     __shared__ int q_own[kWave], q_pid[kWave];
+    __shared__ float q_V[kWave];
     const Geom G = *p.geom;
     if (G.flags & kFatal) return;
     const int lane = threadIdx.x, ag = lane & 7, pg = lane >> 3;
@@ -903,7 +883,7 @@ __global__ void __launch_bounds__(64) k_force(KParams p) {
         const int nact = p.cell_act_count[c];
         if (nact == 0) continue;
         const int na = p.cell_count_eff[c];
-        const int ps = p.cell_pt_start[c];
+        const int ps = p.frec_base[c];
         const int as = p.cell_start[c];
         for (int ab = 0; ab < na; ab += kWave) {
             float ax[8], ay[8], az[8], a0[8], a1[8], a2[8];
@@ -954,8 +934,10 @@ __global__ void __launch_bounds__(64) k_force(KParams p) {
                         q_3[lane] = r1.w;
                         q_4[lane] = r2.x;
                         q_5[lane] = r2.y;
+                        q_V[lane] = r2.z;
                     } else {
                         q_0[lane] = facs * r1.x;
+                        q_V[lane] = r1.y;
                     }
                 }
                 __syncthreads();
@@ -1018,13 +1000,10 @@ __global__ void __launch_bounds__(64) k_force(KParams p) {
                         if (ag == 0 && m < cntp) {
                             const int pid = q_pid[m];
                             if (ab == 0) {
-                                p.owner_f[pid] = -ox;
-                                p.owner_f[p.NQ + pid] = -oy;
-                                p.owner_f[2 * p.NQ + pid] = -oz;
+                                p.vrec[pid] = make_float4(q_V[m], -ox, -oy, -oz);
                             } else {
-                                p.owner_f[pid] -= ox;
-                                p.owner_f[p.NQ + pid] -= oy;
-                                p.owner_f[2 * p.NQ + pid] -= oz;
+                                float4 v = p.vrec[pid];
+                                p.vrec[pid] = make_float4(v.x, v.y - ox, v.z - oy, v.w - oz);
                             }
                         }
                     }
@@ -1066,9 +1045,10 @@ __global__ void __launch_bounds__(64) k_owner_reduce(KParams p) {
     for (int q = lane; q < p.Q; q += kWave) {
         const int pid = i * p.Q + q;
         const double wq = gbsw ? (double)p.sign_chain * (e4 * p.quad_pw[q] + e7 * p.quad_pw[p.Q + q]) : 1.0;
-        a[0] += wq * (double)p.owner_f[pid];
-        a[1] += wq * (double)p.owner_f[p.NQ + pid];
-        a[2] += wq * (double)p.owner_f[2 * p.NQ + pid];
+        const float4 v = p.vrec[pid];
+        a[0] += wq * (double)v.y;
+        a[1] += wq * (double)v.z;
+        a[2] += wq * (double)v.w;
     }
     for (int d = 0; d < 3; ++d) a[d] = wave_sum(a[d]);
     if (lane == 0) {
@@ -1141,10 +1121,10 @@ hipError_t launch_evaluate(const KParams& p, unsigned flags, hipStream_t stream,
     }
     mark(1);
     hipLaunchKernelGGL(k_point_bin, dim3(p.N), dim3(256), 0, stream, p);
-    err = scan_exclusive(p.cell_pt_count, p.cell_pt_start, p.cell_cap, p.scan_tmp, &p.geom->total_points, p.NQ,
+    // records of the active points of cell c start at the exclusive prefix sum of the point counts
+    err = scan_exclusive(p.cell_pt_count, p.frec_base, p.cell_cap, p.scan_tmp, &p.geom->total_points, p.NQ,
                          &p.geom->flags, FLAG_POINT_OVERFLOW, stream);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k_point_scatter, dim3(p.N), dim3(256), 0, stream, p);
     hipLaunchKernelGGL(k_worklist, dim3(std::min((p.cell_cap + 255) / 256, 4096)), dim3(256), 0, stream, p);
     const int pb = std::min(p.cell_cap, persistent_blocks());
     mark(2);
