@@ -14,6 +14,11 @@
  *
  * Threading: a context is driven by one host thread at a time; all device work of a
  * context is ordered on its HIP stream.
+ *
+ * Batches: a context holds S >= 1 independent systems (gbi_batch_create; gbi_create is
+ * the S = 1 case).  Per-atom arrays of a batch are the systems' arrays concatenated in
+ * system order (N = sum of the systems' atom counts); energies are one double per system.
+ * Every pass of an evaluation is one launch over all systems.
  */
 #ifndef GBI_H
 #define GBI_H
@@ -24,7 +29,7 @@
 extern "C" {
 #endif
 
-#define GBI_ABI_VERSION 1
+#define GBI_ABI_VERSION 2
 
 typedef enum gbi_status {
     GBI_OK = 0,
@@ -113,7 +118,7 @@ typedef struct gbi_stats {
 typedef struct gbi_device_view {
     double* positions;    /* N x 3 doubles */
     int64_t* forces;      /* 3 x N fixed point (x[N] | y[N] | z[N]), scale 2^32 */
-    double* energy;       /* 1 double */
+    double* energy;       /* S doubles */
     double* born_radii;   /* N doubles */
     double* integrals;    /* 2 x N doubles (integral-major) */
     void* stream;         /* hipStream_t of the context */
@@ -124,6 +129,39 @@ typedef struct gbi_ctx gbi_ctx;
 /* Flags of gbi_compute. */
 #define GBI_FORCES 1u
 #define GBI_ENERGY 2u
+#define GBI_ADJOINT 4u /* also dE/dR_j, dE/dq_j and dE/dtheta (gbi_get_adjoint) */
+
+/*
+ * Constants theta of a system whose energy derivatives the adjoint returns
+ * (gbi_get_adjoint, GBI_N_THETA doubles per system, zero where the model has no such
+ * constant).  No reference counterpart: the reference hard-codes these constants.
+ */
+typedef enum gbi_theta {
+    GBI_THETA_PREFACTOR = 0, /* P of the STILL energy */
+    GBI_THETA_ALPHA0,        /* Born map */
+    GBI_THETA_ALPHA1,
+    GBI_THETA_ETA,           /* Born map only (the quadrature range r0 stays fixed) */
+    GBI_THETA_SLOPE,
+    GBI_THETA_SHIFT,
+    GBI_THETA_SW,            /* GBSW switching half-width w */
+    GBI_THETA_RSCALE,        /* GBSW R' = rscale (R + roffset) */
+    GBI_THETA_ROFFSET,
+    GBI_THETA_GAMMA0,        /* GBMV-I */
+    GBI_THETA_BETA,          /* GBMV-I / GBMV-II */
+    GBI_THETA_LAMBDA,        /* GBMV-I / GBMV-II */
+    GBI_THETA_P1,            /* GBMV-II */
+    GBI_THETA_P2,
+    GBI_THETA_S0,
+    GBI_N_THETA
+} gbi_theta;
+
+/* Launch-shape knobs (gbi_autotune / gbi_set_tuning). */
+typedef struct gbi_tuning {
+    int32_t blocks_per_cu;    /* workgroups per compute unit of the per-cell passes */
+    int32_t pair_columns;     /* target columns per pair tile (0: automatic) */
+    double tight_grid_length; /* nm, cell of the exact GBSW grid (GBSW model systems) */
+    double ms_per_eval;       /* gbi_autotune: measured time of the chosen shape */
+} gbi_tuning;
 
 /* Reference defaults of a model (constructors of GBSWIntegral / GBMVIntegralType{I,II},
  * CharmmReferenceGBSW.cpp:20-52). */
@@ -137,10 +175,20 @@ gbi_status gbi_create(int32_t n_atoms, const double* charge, const double* radiu
                       const gbi_params* params, gbi_ctx** out);
 
 /*
+ * A batch of n_systems independent systems in one context (no reference counterpart: the
+ * reference evaluates one System per Context).  n_atoms[s] atoms each; charge and radius
+ * hold sum(n_atoms) doubles in system order; params[s] configures system s (models,
+ * integral types and quadratures may differ).
+ */
+gbi_status gbi_batch_create(int32_t n_systems, const int32_t* n_atoms, const double* charge, const double* radius,
+                            const gbi_params* params, gbi_ctx** out);
+gbi_status gbi_get_layout(gbi_ctx* ctx, int32_t* n_systems, int32_t* n_atoms_total);
+
+/*
  * One evaluation: replaces CalcCharmmGB{SW,MV}ForceKernel::execute(context, includeForces,
  * includeEnergy) (charmmKernels.h:81,121).  pos: N x 3 host doubles.  Forces are ADDED to
- * `forces` (N x 3 host doubles, may be NULL when include_forces is 0); the energy is written
- * to *energy (may be NULL).  Synchronous.
+ * `forces` (N x 3 host doubles, may be NULL when include_forces is 0); the energy of each
+ * system is written to energy[0..S-1] (may be NULL).  Synchronous.
  */
 gbi_status gbi_execute(gbi_ctx* ctx, const double* pos, int32_t include_forces, int32_t include_energy,
                        double* energy, double* forces);
@@ -158,10 +206,37 @@ gbi_status gbi_synchronize(gbi_ctx* ctx);                /* wait + check device 
 gbi_status gbi_get_energy(gbi_ctx* ctx, double* energy);
 gbi_status gbi_get_forces(gbi_ctx* ctx, double* forces, int32_t add);
 gbi_status gbi_get_born_radii(gbi_ctx* ctx, double* born);
-gbi_status gbi_get_integrals(gbi_ctx* ctx, double* integrals); /* 2 x N */
+gbi_status gbi_get_integrals(gbi_ctx* ctx, double* integrals); /* 2 x N (integral-major) */
 gbi_status gbi_get_stats(gbi_ctx* ctx, gbi_stats* stats);
 gbi_status gbi_get_device_view(gbi_ctx* ctx, gbi_device_view* view);
 gbi_status gbi_set_stream(gbi_ctx* ctx, void* hip_stream); /* NULL: the context's own stream */
+
+/*
+ * Adjoint of the last GBI_ADJOINT evaluation (no reference counterpart): dE/dR (N),
+ * dE/dq (N) and dE/dtheta (S x GBI_N_THETA, gbi_theta order).  Any pointer may be NULL.
+ * The lookup grids' membership tests are held fixed (the derivative of the evaluated
+ * piece).  gbi_execute_adjoint = set positions + GBI_FORCES|GBI_ENERGY|GBI_ADJOINT + reads.
+ */
+gbi_status gbi_get_adjoint(gbi_ctx* ctx, double* dEdR, double* dEdq, double* dEdtheta);
+gbi_status gbi_execute_adjoint(gbi_ctx* ctx, const double* pos, double* energy, double* forces, double* dEdR,
+                               double* dEdq, double* dEdtheta);
+
+/*
+ * Replay evaluations from a captured hipGraph (one graph launch per evaluation instead of
+ * ~25 kernel launches).  The graph is re-captured when buffers grow.  Not used while
+ * profiling is enabled.
+ */
+gbi_status gbi_set_graph(gbi_ctx* ctx, int32_t enable);
+
+/*
+ * Launch shapes: gbi_autotune times `evals` evaluations of each candidate shape on the
+ * current positions, keeps the fastest and reports it; gbi_set_tuning sets one directly.
+ * Results do not depend on the shape except through the GBSW exact-grid cell length
+ * (FP32 summation order of V inside a cell).
+ */
+gbi_status gbi_autotune(gbi_ctx* ctx, int32_t evals, gbi_tuning* out);
+gbi_status gbi_set_tuning(gbi_ctx* ctx, const gbi_tuning* tuning);
+gbi_status gbi_get_tuning(gbi_ctx* ctx, gbi_tuning* out);
 
 /*
  * Diagnostics (no reference counterpart): HIP events recorded on the context's stream

@@ -21,15 +21,20 @@ STATUS_NAMES = {
 MODEL_GBSW, MODEL_GBMV = 0, 1
 INTEGRAL_GBSW, INTEGRAL_GBMV1, INTEGRAL_GBMV2, INTEGRAL_CUSTOM = 0, 1, 2, 3
 LOOKUP_REFERENCE, LOOKUP_BRUTE = 0, 1
-FORCES, ENERGY = 1, 2
+FORCES, ENERGY, ADJOINT = 1, 2, 4
+
+# dE/dtheta order of gbi_get_adjoint (gbi_theta in gbi.h)
+THETA = ["prefactor", "alpha0", "alpha1", "eta", "slope", "shift", "sw", "rscale", "roffset", "gamma0",
+         "beta", "lambda", "P1", "P2", "S0"]
 
 # Every symbol the header declares (checked by the CPU tests).
 EXPORTS = [
-    "gbi_params_default", "gbi_create", "gbi_execute", "gbi_update_parameters", "gbi_set_positions",
-    "gbi_compute", "gbi_synchronize", "gbi_get_energy", "gbi_get_forces", "gbi_get_born_radii",
-    "gbi_get_integrals", "gbi_get_stats", "gbi_get_device_view", "gbi_set_stream", "gbi_destroy",
-    "gbi_last_error", "gbi_abi_version", "gbi_quadrature", "gbi_profile_enable", "gbi_profile_read",
-    "gbi_profile_pass_name",
+    "gbi_params_default", "gbi_create", "gbi_batch_create", "gbi_get_layout", "gbi_execute",
+    "gbi_update_parameters", "gbi_set_positions", "gbi_compute", "gbi_synchronize", "gbi_get_energy",
+    "gbi_get_forces", "gbi_get_born_radii", "gbi_get_integrals", "gbi_get_stats", "gbi_get_device_view",
+    "gbi_set_stream", "gbi_get_adjoint", "gbi_execute_adjoint", "gbi_set_graph", "gbi_autotune",
+    "gbi_set_tuning", "gbi_get_tuning", "gbi_destroy", "gbi_last_error", "gbi_abi_version", "gbi_quadrature",
+    "gbi_profile_enable", "gbi_profile_read", "gbi_profile_pass_name",
 ]
 PROFILE_PASSES = ["grid", "bin", "volume", "integrals", "pairs", "chain", "force", "owner"]
 
@@ -82,6 +87,15 @@ class Stats(ctypes.Structure):
                 ("n_points", "n_vol_pairs", "n_cells", "n_entries", "n_atoms", "n_quad")]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("blocks_per_cu", ctypes.c_int32),
+        ("pair_columns", ctypes.c_int32),
+        ("tight_grid_length", ctypes.c_double),
+        ("ms_per_eval", ctypes.c_double),
+    ]
+
+
 class DeviceView(ctypes.Structure):
     _fields_ = [
         ("positions", ctypes.c_void_p),
@@ -108,7 +122,16 @@ def load() -> ctypes.CDLL:
     dp = ctypes.POINTER(ctypes.c_double)
     vp = ctypes.c_void_p
     L.gbi_params_default.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Params)]
+    ip = ctypes.POINTER(ctypes.c_int32)
     L.gbi_create.argtypes = [ctypes.c_int32, dp, dp, ctypes.POINTER(Params), ctypes.POINTER(vp)]
+    L.gbi_batch_create.argtypes = [ctypes.c_int32, ip, dp, dp, ctypes.POINTER(Params), ctypes.POINTER(vp)]
+    L.gbi_get_layout.argtypes = [vp, ip, ip]
+    L.gbi_get_adjoint.argtypes = [vp, dp, dp, dp]
+    L.gbi_execute_adjoint.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.gbi_set_graph.argtypes = [vp, ctypes.c_int32]
+    L.gbi_autotune.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(Tuning)]
+    L.gbi_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    L.gbi_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     L.gbi_execute.argtypes = [vp, dp, ctypes.c_int32, ctypes.c_int32, dp, dp]
     L.gbi_update_parameters.argtypes = [vp, ctypes.c_int32, dp, dp]
     L.gbi_set_positions.argtypes = [vp, vp, ctypes.c_int32]
@@ -161,18 +184,38 @@ def quadrature(n_radial: int = 24, r0: float = 0.05, r1: float = 2.0, n_lebedev:
 
 
 class Engine:
-    """One gbi_ctx: the HIP counterpart of a CalcCharmmGB{SW,MV}ForceKernel instance."""
+    """One gbi_ctx: the HIP counterpart of a CalcCharmmGB{SW,MV}ForceKernel instance.
 
-    def __init__(self, params: Params, charge, radius):
+    ``params`` may be a list (one Params per system) with ``n_atoms`` giving each system's
+    atom count: a batch whose per-atom arrays are the systems' arrays concatenated."""
+
+    def __init__(self, params, charge, radius, n_atoms=None):
         self._lib = load()
         self.charge = np.ascontiguousarray(charge, dtype=np.float64)
         self.radius = np.ascontiguousarray(radius, dtype=np.float64)
         self.n = int(self.charge.shape[0])
-        self.params = params
         handle = ctypes.c_void_p()
-        _check(self._lib.gbi_create(self.n, _dp(self.charge), _dp(self.radius), ctypes.byref(params),
-                                    ctypes.byref(handle)))
+        if isinstance(params, Params):
+            self.params = [params]
+            self.n_atoms = np.array([self.n], dtype=np.int32)
+            _check(self._lib.gbi_create(self.n, _dp(self.charge), _dp(self.radius), ctypes.byref(params),
+                                        ctypes.byref(handle)))
+        else:
+            self.params = list(params)
+            self.n_atoms = np.ascontiguousarray(n_atoms, dtype=np.int32)
+            arr = (Params * len(self.params))(*self.params)
+            _check(self._lib.gbi_batch_create(len(self.params),
+                                              self.n_atoms.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                              _dp(self.charge), _dp(self.radius), arr, ctypes.byref(handle)))
+        self.n_systems = len(self.params)
+        self.offsets = np.concatenate([[0], np.cumsum(self.n_atoms)]).astype(np.int64)
         self._h = handle
+
+    @classmethod
+    def batch(cls, params_list, charges, radii):
+        """Batch of systems from per-system parameter, charge and radius lists."""
+        n = [len(c) for c in charges]
+        return cls(params_list, np.concatenate(charges), np.concatenate(radii), n_atoms=n)
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -186,13 +229,35 @@ class Engine:
             pass
 
     def execute(self, pos, include_forces: bool = True, include_energy: bool = True, forces=None):
-        """(energy, forces) of one evaluation; forces are ADDED to `forces` when given."""
+        """(energy, forces) of one evaluation; forces are ADDED to `forces` when given.
+        The energy is a float for one system and an array of S energies for a batch."""
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.n, 3)
         out = np.zeros((self.n, 3)) if forces is None else forces
-        e = np.zeros(1)
+        e = np.zeros(self.n_systems)
         _check(self._lib.gbi_execute(self._h, _dp(pos), int(include_forces), int(include_energy), _dp(e),
                                      _dp(out)))
-        return float(e[0]), out
+        return (float(e[0]) if self.n_systems == 1 else e), out
+
+    def execute_adjoint(self, pos):
+        """dict(energy, forces, dEdR, dEdq, dEdtheta {name: value or per-system array})."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.n, 3)
+        e = np.zeros(self.n_systems)
+        f = np.zeros((self.n, 3))
+        dr = np.zeros(self.n)
+        dq = np.zeros(self.n)
+        th = np.zeros((self.n_systems, len(THETA)))
+        _check(self._lib.gbi_execute_adjoint(self._h, _dp(pos), _dp(e), _dp(f), _dp(dr), _dp(dq), _dp(th)))
+        one = self.n_systems == 1
+        theta = {k: (float(th[0, i]) if one else th[:, i].copy()) for i, k in enumerate(THETA)}
+        return {"energy": float(e[0]) if one else e, "forces": f, "dEdR": dr, "dEdq": dq, "dEdtheta": theta}
+
+    def adjoint(self):
+        """(dE/dR, dE/dq, dE/dtheta [S x 15]) of the last GBI_ADJOINT evaluation."""
+        dr = np.zeros(self.n)
+        dq = np.zeros(self.n)
+        th = np.zeros((self.n_systems, len(THETA)))
+        _check(self._lib.gbi_get_adjoint(self._h, _dp(dr), _dp(dq), _dp(th)))
+        return dr, dq, th
 
     def update_parameters(self, charge, radius) -> None:
         charge = np.ascontiguousarray(charge, dtype=np.float64)
@@ -213,10 +278,10 @@ class Engine:
     def synchronize(self) -> None:
         _check(self._lib.gbi_synchronize(self._h))
 
-    def energy(self) -> float:
-        e = np.zeros(1)
+    def energy(self):
+        e = np.zeros(self.n_systems)
         _check(self._lib.gbi_get_energy(self._h, _dp(e)))
-        return float(e[0])
+        return float(e[0]) if self.n_systems == 1 else e
 
     def forces(self) -> np.ndarray:
         f = np.zeros((self.n, 3))
@@ -242,6 +307,23 @@ class Engine:
         v = DeviceView()
         _check(self._lib.gbi_get_device_view(self._h, ctypes.byref(v)))
         return v
+
+    def set_graph(self, enable: bool = True) -> None:
+        _check(self._lib.gbi_set_graph(self._h, int(enable)))
+
+    def autotune(self, evals: int = 3) -> dict:
+        t = Tuning()
+        _check(self._lib.gbi_autotune(self._h, int(evals), ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in Tuning._fields_}
+
+    def set_tuning(self, blocks_per_cu: int = 32, pair_columns: int = 0, tight_grid_length: float = 0.0) -> None:
+        t = Tuning(blocks_per_cu, pair_columns, tight_grid_length, 0.0)
+        _check(self._lib.gbi_set_tuning(self._h, ctypes.byref(t)))
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(self._lib.gbi_get_tuning(self._h, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in Tuning._fields_}
 
     def profile_enable(self, enable: bool = True) -> None:
         _check(self._lib.gbi_profile_enable(self._h, int(enable)))

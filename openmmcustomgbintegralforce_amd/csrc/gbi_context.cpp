@@ -1,5 +1,6 @@
-// Host side of the C ABI (include/gbi.h): parameter defaults, device buffers, capacity
-// management and the execute sequence.  Replaces the kernel drivers
+// Host side of the C ABI (include/gbi.h): parameter defaults, the batch layout (per-system
+// ranges of atoms, quadrature rows, cells and point slots), device buffers, capacity
+// management, graph capture and the execute sequence.  Replaces the kernel drivers
 // ReferenceCalcCharmmGB{SW,MV}ForceKernel::execute (CharmmReferenceKernels.cpp:337-370,
 // :426-437) and CudaCalcCharmmGBMVForceKernel::execute / CustomGBIntegral::evaluate
 // (CharmmCudaKernels.cpp:845-993, cuda CustomGBIntegral.cpp:625-717).
@@ -9,6 +10,7 @@
 
 #include <algorithm>
 #include <array>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -35,8 +37,16 @@ gbi_status fail(gbi_status st, const std::string& msg) {
                         std::string(#expr) + ": " + hipGetErrorString(e_));                        \
     } while (0)
 
+#define ST_TRY(expr)                      \
+    do {                                  \
+        gbi_status s_ = (expr);           \
+        if (s_ != GBI_OK) return s_;      \
+    } while (0)
+
 constexpr double kOne4PiEps0 = 138.935456;  // OpenMM 7.x SimTKOpenMMRealType.h
 constexpr double kTightMargin = 1e-6;       // nm, guards the exact GBSW lists against rounding
+constexpr long long kMaxCells = 16ll * 1024 * 1024;  // limit of the three-pass scan
+constexpr int kDefaultBlocksPerCU = 32;
 
 template <class T>
 struct DevBuf {
@@ -58,43 +68,69 @@ struct DevBuf {
     }
 };
 
+// Host description of one system of the batch.
+struct SysHost {
+    gbi_params P{};
+    int N = 0, atom_begin = 0, Q = 0, quad_base = 0, pt_base = 0;
+    gbi::GridKind kind = gbi::GRID_FAITHFUL;
+    gbi::VolKind vol = gbi::VOL_GBSW;
+    double g = 0.15, gpad = 0.0, halfdiag = 0.0, max_reach = 0.0;
+    long long cell_cap = 0;
+    int pt_cap = 1;
+    int pair_splits = 1;
+};
+
 }  // namespace
 
 struct gbi_ctx {
-    int N = 0;
-    int Q = 0;
-    gbi_params P{};
-    gbi::GridKind kind = gbi::GRID_FAITHFUL;
-    gbi::VolKind vol = gbi::VOL_GBSW;
+    std::vector<SysHost> sys;
+    int S = 0, N = 0, NQ = 0, quad_rows = 0;
+    std::vector<double> charge, radius;
+    long long cell_total = 0;
+    long long prec_total = 0;
+    int entry_cap = 0;
+    bool kinds[gbi::VOL_KINDS] = {false, false, false};
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     int device = 0;
+    int num_cus = 256;
+    int perm_stride = 1;
     bool have_positions = false;
     bool computed = false;
-
-    std::vector<double> charge, radius;
-    int cell_cap = 0, entry_cap = 0;
+    bool computed_adjoint = false;
+    gbi_tuning tuning{};
 
     // device buffers
+    DevBuf<gbi::SysDesc> d_sys;
+    DevBuf<gbi::SysGeom> geom;
+    DevBuf<int> flags_all, atom_sys, cell_sys;
+    DevBuf<int4> pair_tiles;
+    int n_tiles = 0, max_splits = 1;
     DevBuf<double> pos, d_charge, d_radius, d_reach, quad_off, quad_pw, integrals, born, dbdi, dedi, pair_part,
-        atom_energy, energy, stats;
+        atom_energy, energy, stats, dedq, dedr, theta_atom, theta;
     DevBuf<float4> aparam;
-    DevBuf<gbi::Geom> geom;
     DevBuf<int> cell_count, cell_fill, cell_start, cell_count_eff, cell_atoms_tmp, cell_atoms, cell_pt_count,
-        cell_act_count, frec_base, worklist, work, scan_tmp;
+        cell_act_count, frec_base, worklist, worklist_sys, work, scan_tmp;
     DevBuf<float4> ent_pos, ent_par, prec, frec0, frec1, frec2, vrec;
-    DevBuf<float> quad_r2;
-    int pt_cap = 0;
-    DevBuf<long long> force_fix;
+    DevBuf<float> quad_r2, vsig;
+    DevBuf<long long> force_fix, dedr_fix, vacc_fix;
 
-    int pair_splits = 1;
-    int perm_stride = 1;
-    double gpad = 0.0, halfdiag = 0.0, max_reach = 0.0;
+    // graph replay (index: adjoint evaluation)
+    bool use_graph = false;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
 
     // diagnostics: kPasses + 1 events per profiled evaluation
     bool profiling = false;
     std::vector<std::array<hipEvent_t, gbi::kPasses + 1>> prof_events;
     size_t prof_used = 0;
+
+    void drop_graphs() {
+        for (auto& g : graph)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+    }
 
     void release_profile() {
         for (auto& a : prof_events)
@@ -104,19 +140,27 @@ struct gbi_ctx {
     }
 
     ~gbi_ctx() {
+        drop_graphs();
         release_profile();
-        DevBuf<double>* d[] = {&pos, &d_charge, &d_radius, &d_reach, &quad_off, &quad_pw, &integrals, &born,
-                               &dbdi, &dedi, &pair_part, &atom_energy, &energy, &stats};
+        d_sys.release();
+        geom.release();
+        DevBuf<double>* d[] = {&pos,       &d_charge, &d_radius, &d_reach, &quad_off,   &quad_pw, &integrals,
+                               &born,      &dbdi,     &dedi,     &pair_part, &atom_energy, &energy, &stats,
+                               &dedq,      &dedr,     &theta_atom, &theta};
         for (auto* b : d) b->release();
         aparam.release();
-        geom.release();
-        DevBuf<int>* ib[] = {&cell_count, &cell_fill, &cell_start, &cell_count_eff, &cell_atoms_tmp, &cell_atoms,
-                             &cell_pt_count, &cell_act_count, &frec_base, &worklist, &work, &scan_tmp};
+        pair_tiles.release();
+        DevBuf<int>* ib[] = {&flags_all,      &atom_sys,     &cell_sys,       &cell_count, &cell_fill,
+                             &cell_start,     &cell_count_eff, &cell_atoms_tmp, &cell_atoms, &cell_pt_count,
+                             &cell_act_count, &frec_base,    &worklist,       &worklist_sys, &work, &scan_tmp};
         for (auto* b : ib) b->release();
         DevBuf<float4>* fb[] = {&ent_pos, &ent_par, &prec, &frec0, &frec1, &frec2, &vrec};
         for (auto* b : fb) b->release();
         quad_r2.release();
+        vsig.release();
         force_fix.release();
+        dedr_fix.release();
+        vacc_fix.release();
         if (own_stream) (void)hipStreamDestroy(own_stream);
     }
 };
@@ -139,47 +183,58 @@ gbi_status validate_params(const gbi_params& P, int N) {
         return fail(GBI_ERR_INVALID_ARGUMENT, "n_lebedev must be 6, 14, 26, 38 or 50");
     if (!(P.r0 > 0.0) || !(P.r1 > P.r0)) return fail(GBI_ERR_INVALID_ARGUMENT, "need 0 < r0 < r1");
     if (!(P.grid_length > 0.0)) return fail(GBI_ERR_INVALID_ARGUMENT, "grid_length must be positive");
+    if (P.model == GBI_MODEL_GBSW || P.integral_type == GBI_INTEGRAL_GBSW) {
+        if (!(P.sw > 0.0)) return fail(GBI_ERR_INVALID_ARGUMENT, "the switching half-width must be positive");
+    }
     for (int k = 0; k < 2; ++k)
         if (P.orders[k] < 2)  // GBSWIntegral.cpp:92-95
             return fail(GBI_ERR_INVALID_ARGUMENT, "the order of an integral must be >= 2");
     const bool o47 = (P.orders[0] == 4 && P.orders[1] == 7) || (P.orders[0] == 7 && P.orders[1] == 4);
     if (!o47) return fail(GBI_ERR_UNSUPPORTED, "the canonical Born map needs integrals of order 4 and 7");
-    if ((long long)N * P.n_radial * P.n_lebedev >= (1ll << 31))
-        return fail(GBI_ERR_INVALID_ARGUMENT, "N * Q must be below 2^31");
-    if (N >= (1 << (32 - gbi::kQBits))) return fail(GBI_ERR_INVALID_ARGUMENT, "too many atoms");
     return GBI_OK;
 }
 
-// Per-atom constants (float for the volume functions, double reach for the tight grid).
+gbi_status check_atoms(int n, const double* charge, const double* radius) {
+    for (int j = 0; j < n; ++j) {
+        if (!std::isfinite(charge[j]) || !std::isfinite(radius[j]))
+            return fail(GBI_ERR_NONFINITE, "non-finite charge or radius");
+        if (!(radius[j] > 0.0)) return fail(GBI_ERR_INVALID_ARGUMENT, "radii must be positive");
+    }
+    return GBI_OK;
+}
+
+// Per-atom constants (float for the volume functions, double reach for the exact grid).
 void atom_params(const gbi_ctx& c, std::vector<float4>& ap, std::vector<double>& reach) {
-    const gbi_params& P = c.P;
     ap.resize(c.N);
     reach.resize(c.N);
-    for (int j = 0; j < c.N; ++j) {
-        const double R = c.radius[j];
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (c.vol == gbi::VOL_GBSW) {
-            const double Rp = (R + P.roffset) * P.rscale;
-            const double lo = Rp - P.sw, hi = Rp + P.sw;
-            v.x = (float)Rp;
-            v.y = lo > 0.0 ? (float)(lo * lo) : -1.0f;
-            v.z = (float)(hi * hi);
-            reach[j] = hi + kTightMargin;
-        } else if (c.vol == gbi::VOL_GBMV1) {
-            v.x = (float)(P.gamma0 * std::log(P.lambda) / (R * R * R * R));
-            reach[j] = 0.0;
-        } else {
-            const double C = P.P1 * R * 10.0 + P.P2;
-            v.x = (float)C;
-            v.y = (float)(C - R * R * 100.0);
-            v.z = (float)(400.0 * C * C);
-            reach[j] = 0.0;
+    for (const SysHost& s : c.sys) {
+        const gbi_params& P = s.P;
+        for (int j = s.atom_begin; j < s.atom_begin + s.N; ++j) {
+            const double R = c.radius[j];
+            float4 v = make_float4(0.f, 0.f, 0.f, (float)R);
+            if (s.vol == gbi::VOL_GBSW) {
+                const double Rp = (R + P.roffset) * P.rscale;
+                const double lo = Rp - P.sw, hi = Rp + P.sw;
+                v.x = (float)Rp;
+                v.y = lo > 0.0 ? (float)(lo * lo) : -1.0f;
+                v.z = (float)(hi * hi);
+                reach[j] = hi + kTightMargin;
+            } else if (s.vol == gbi::VOL_GBMV1) {
+                v.x = (float)(P.gamma0 * std::log(P.lambda) / (R * R * R * R));
+                reach[j] = 0.0;
+            } else {
+                const double C = P.P1 * R * 10.0 + P.P2;
+                v.x = (float)C;
+                v.y = (float)(C - R * R * 100.0);
+                v.z = (float)(400.0 * C * C);
+                reach[j] = 0.0;
+            }
+            ap[j] = v;
         }
-        ap[j] = v;
     }
 }
 
-gbi_status upload_params(gbi_ctx& c) {
+gbi_status upload_atoms(gbi_ctx& c) {
     std::vector<float4> ap;
     std::vector<double> reach;
     atom_params(c, ap, reach);
@@ -187,121 +242,224 @@ gbi_status upload_params(gbi_ctx& c) {
     HIP_TRY(hipMemcpyAsync(c.d_reach.p, reach.data(), sizeof(double) * c.N, hipMemcpyHostToDevice, c.stream));
     HIP_TRY(hipMemcpyAsync(c.d_charge.p, c.charge.data(), sizeof(double) * c.N, hipMemcpyHostToDevice, c.stream));
     HIP_TRY(hipMemcpyAsync(c.d_radius.p, c.radius.data(), sizeof(double) * c.N, hipMemcpyHostToDevice, c.stream));
-    c.max_reach = 0.0;
-    for (double r : reach) c.max_reach = std::max(c.max_reach, r);
+    for (SysHost& s : c.sys) {
+        s.max_reach = 0.0;
+        for (int j = s.atom_begin; j < s.atom_begin + s.N; ++j) s.max_reach = std::max(s.max_reach, reach[j]);
+    }
     HIP_TRY(hipStreamSynchronize(c.stream));
     return GBI_OK;
 }
 
-// Grid sizes for the current positions (host copy) with headroom for motion.
-void estimate_caps(const gbi_ctx& c, const std::vector<double>& pos, long long& cells, long long& entries) {
-    if (c.kind == gbi::GRID_BRUTE) {
+// System constants in the device layout.
+gbi::SysDesc make_desc(const gbi_ctx& c, const SysHost& s, long long cell_base, long long prec_base) {
+    gbi::SysDesc d{};
+    const gbi_params& P = s.P;
+    d.grid_kind = s.kind;
+    d.vol_kind = s.vol;
+    d.atom_begin = s.atom_begin;
+    d.atom_end = s.atom_begin + s.N;
+    d.Q = s.Q;
+    d.quad_base = s.quad_base;
+    d.pt_base = s.pt_base;
+    d.cell_base = (int)cell_base;
+    d.cell_cap = (int)s.cell_cap;
+    d.pt_cap = s.pt_cap;
+    d.prec_base = prec_base;
+    d.cap = (P.cell_cap > 0 && s.kind != gbi::GRID_BRUTE) ? P.cell_cap : INT_MAX;
+    d.pair_splits = s.pair_splits;
+    d.i4 = P.orders[0] == 4 ? 0 : 1;
+    d.i7 = 1 - d.i4;
+    d.g = s.g;
+    d.inv_g = 1.0 / s.g;
+    d.gpad = s.kind == gbi::GRID_FAITHFUL ? s.gpad : s.max_reach;
+    d.halfdiag = s.halfdiag;
+    d.buffer = P.buffer;
+    d.k1 = (float)(3.0 / (4.0 * P.sw));
+    d.k3 = (float)(1.0 / (4.0 * P.sw * P.sw * P.sw));
+    d.sw = (float)P.sw;
+    d.inv_sw = P.sw > 0.0 ? (float)(1.0 / P.sw) : 0.f;
+    d.rscale = (float)P.rscale;
+    d.roffset = (float)P.roffset;
+    d.gamma0 = (float)P.gamma0;
+    d.beta = (float)P.beta;
+    d.lambda = (float)P.lambda;
+    d.S0 = (float)P.S0;
+    d.vguard = (float)P.vguard;
+    d.fclamp = 1.0e6f;
+    d.ssat = 6.0f;
+    d.P1 = (float)P.P1;
+    d.v_empty = s.vol == gbi::VOL_GBSW    ? 1.0f
+                : s.vol == gbi::VOL_GBMV2 ? 0.0f
+                                          : (float)(1.0 / (1.0 + std::exp(P.beta * (0.0 - P.lambda))));
+    d.sign_chain = s.vol == gbi::VOL_GBSW ? 1.0f : -1.0f;
+    d.alpha0 = P.alpha0;
+    d.alpha1 = P.alpha1;
+    d.eta = P.eta;
+    d.slope = P.slope;
+    d.shift = P.shift;
+    d.prefactor = P.prefactor;
+    (void)c;
+    return d;
+}
+
+// Cell, slot and worklist buffers for the systems' current capacities; uploads the
+// system descriptors and the cell -> system map.
+gbi_status layout(gbi_ctx& c) {
+    long long cells = 0, slots = 0;
+    for (const SysHost& s : c.sys) {
+        cells += s.cell_cap;
+        slots += s.cell_cap * (long long)s.pt_cap;
+    }
+    if (cells > kMaxCells) return fail(GBI_ERR_GRID_TOO_LARGE, "more than 16M lookup cells in the batch");
+    if (slots * (long long)sizeof(float4) > (200ll << 30))
+        return fail(GBI_ERR_GRID_TOO_LARGE, "point slots of the lookup grids exceed 200 GB");
+    c.drop_graphs();
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    c.cell_total = cells;
+    c.prec_total = slots;
+    HIP_TRY(c.cell_count.ensure(cells));
+    HIP_TRY(c.cell_fill.ensure(cells));
+    HIP_TRY(c.cell_start.ensure(cells + 1));
+    HIP_TRY(c.cell_count_eff.ensure(cells));
+    HIP_TRY(c.cell_pt_count.ensure(cells));
+    HIP_TRY(c.cell_act_count.ensure(cells));
+    HIP_TRY(c.frec_base.ensure(cells + 1));
+    HIP_TRY(c.worklist.ensure(gbi::VOL_KINDS * cells));
+    HIP_TRY(c.worklist_sys.ensure(gbi::VOL_KINDS * cells));
+    HIP_TRY(c.cell_sys.ensure(cells));
+    HIP_TRY(c.scan_tmp.ensure((cells + 4095) / 4096 + 16));
+    HIP_TRY(c.prec.ensure(slots));
+    std::vector<int> cs(cells);
+    std::vector<gbi::SysDesc> desc(c.S);
+    long long cb = 0, pb = 0;
+    for (int s = 0; s < c.S; ++s) {
+        desc[s] = make_desc(c, c.sys[s], cb, pb);
+        std::fill(cs.begin() + cb, cs.begin() + cb + c.sys[s].cell_cap, s);
+        cb += c.sys[s].cell_cap;
+        pb += c.sys[s].cell_cap * (long long)c.sys[s].pt_cap;
+    }
+    HIP_TRY(hipMemcpyAsync(c.cell_sys.p, cs.data(), sizeof(int) * cells, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(hipMemcpyAsync(c.d_sys.p, desc.data(), sizeof(gbi::SysDesc) * c.S, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    return GBI_OK;
+}
+
+// Pair tiles: 64 rows x a column range of the same system.  Enough tiles to fill the
+// device, and at most `pair_columns` columns per tile for load balance across systems.
+gbi_status build_tiles(gbi_ctx& c) {
+    long long rows = 0;
+    for (const SysHost& s : c.sys) rows += (s.N + 63) / 64;
+    const int base = (int)std::max<long long>(1, (2048 + rows - 1) / rows);
+    const int cols = c.tuning.pair_columns > 0 ? c.tuning.pair_columns : 8192;
+    std::vector<int4> tiles;
+    c.max_splits = 1;
+    for (SysHost& s : c.sys) {
+        int splits = std::max(base, (s.N + cols - 1) / cols);
+        splits = std::max(1, std::min(splits, std::min(64, s.N / 64)));
+        s.pair_splits = splits;
+        c.max_splits = std::max(c.max_splits, splits);
+        for (int r = 0; r < s.N; r += 64)
+            for (int k = 0; k < splits; ++k) {
+                const int lo = s.atom_begin + (int)((long long)k * s.N / splits);
+                const int hi = s.atom_begin + (int)((long long)(k + 1) * s.N / splits);
+                tiles.push_back(make_int4(s.atom_begin + r, lo, hi, k));
+            }
+    }
+    c.n_tiles = (int)tiles.size();
+    c.drop_graphs();
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    HIP_TRY(c.pair_tiles.ensure(tiles.size()));
+    HIP_TRY(c.pair_part.ensure((size_t)c.max_splits * 6 * c.N));
+    HIP_TRY(hipMemcpyAsync(c.pair_tiles.p, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice,
+                           c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    return GBI_OK;
+}
+
+// Cell count and list entries of one system for the given positions, with headroom.
+void estimate_caps(const gbi_ctx& c, const SysHost& s, const double* pos, long long& cells, long long& entries) {
+    if (s.kind == gbi::GRID_BRUTE) {
         cells = 1;
-        entries = c.N;
+        entries = s.N;
         return;
     }
     double mn[3], mx[3];
     for (int d = 0; d < 3; ++d) mn[d] = mx[d] = pos[d];
-    for (int a = 0; a < c.N; ++a)
+    for (int a = 0; a < s.N; ++a)
         for (int d = 0; d < 3; ++d) {
             mn[d] = std::min(mn[d], pos[3 * a + d]);
             mx[d] = std::max(mx[d], pos[3 * a + d]);
         }
-    const double g = c.P.grid_length > 0 && c.kind == gbi::GRID_FAITHFUL ? c.P.grid_length
-                                                                         : c.P.tight_grid_length;
+    const double g = s.g;
     // headroom: 0.5 nm per side plus 10 %
-    const double gridpad = c.kind == gbi::GRID_FAITHFUL ? c.gpad : c.max_reach;
+    const double gridpad = s.kind == gbi::GRID_FAITHFUL ? s.gpad : s.max_reach;
     cells = 1;
     for (int d = 0; d < 3; ++d) {
         const double len = (mx[d] - mn[d]) * 1.1 + 2.0 * gridpad + 1.0;
         cells *= (long long)std::ceil(len / g) + 2;
     }
     // entries: box of candidate cells per atom (an upper bound of the members)
-    double per_atom;
-    if (c.kind == gbi::GRID_FAITHFUL) {
+    double side;
+    if (s.kind == gbi::GRID_FAITHFUL) {
         double maxR = 0.0;
-        for (double r : c.radius) maxR = std::max(maxR, r);
-        const double pad = maxR + c.halfdiag + c.P.buffer;
-        const double side = std::floor(2.0 * pad / g) + 3.0;
-        per_atom = side * side * side;
+        for (int j = s.atom_begin; j < s.atom_begin + s.N; ++j) maxR = std::max(maxR, c.radius[j]);
+        side = std::floor(2.0 * (maxR + s.halfdiag + s.P.buffer) / g) + 3.0;
     } else {
-        const double side = std::floor(2.0 * c.max_reach / g) + 3.0;
-        per_atom = side * side * side;
+        side = std::floor(2.0 * s.max_reach / g) + 3.0;
     }
-    entries = (long long)(per_atom * c.N * 1.05) + 1024;
+    entries = (long long)(side * side * side * s.N * 1.05) + 1024;
 }
 
-gbi_status ensure_grid_buffers(gbi_ctx& c, long long cells, long long entries) {
-    if (cells > (1ll << 30) || entries > (1ll << 31) - 1)
-        return fail(GBI_ERR_GRID_TOO_LARGE, "lookup grid too large for the device buffers");
-    if (cells > 16ll * 1024 * 1024 * 4) return fail(GBI_ERR_GRID_TOO_LARGE, "more than 64M lookup cells");
-    if (cells > c.cell_cap) {
-        c.cell_cap = (int)cells;
-        HIP_TRY(c.cell_count.ensure(cells));
-        HIP_TRY(c.cell_fill.ensure(cells));
-        HIP_TRY(c.cell_start.ensure(cells + 1));
-        HIP_TRY(c.cell_count_eff.ensure(cells));
-        HIP_TRY(c.cell_pt_count.ensure(cells));
-        HIP_TRY(c.cell_act_count.ensure(cells));
-        HIP_TRY(c.frec_base.ensure(cells + 1));
-        HIP_TRY(c.worklist.ensure(cells));
-        const long long tiles = (cells + 4095) / 4096;
-        HIP_TRY(c.scan_tmp.ensure(std::max<long long>(tiles, 1) + 16));
+// Grow the systems' capacities for the device positions (first call, new radii, overflow).
+gbi_status size_from_device(gbi_ctx& c, long long min_entries) {
+    std::vector<double> host(3 * (size_t)c.N);
+    HIP_TRY(hipMemcpyAsync(host.data(), c.pos.p, sizeof(double) * 3 * c.N, hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    for (double v : host)
+        if (!std::isfinite(v)) return fail(GBI_ERR_NONFINITE, "non-finite atom position");
+    bool changed = c.cell_total == 0;
+    long long entries = 0;
+    for (SysHost& s : c.sys) {
+        long long cells, ent;
+        estimate_caps(c, s, host.data() + 3 * (size_t)s.atom_begin, cells, ent);
+        entries += ent;
+        if (cells > s.cell_cap) {
+            s.cell_cap = cells;
+            changed = true;
+        }
     }
-    HIP_TRY(c.prec.ensure((size_t)c.cell_cap * c.pt_cap));
+    entries = std::max(entries, min_entries);
+    if (entries > INT_MAX - 1) return fail(GBI_ERR_GRID_TOO_LARGE, "lookup grid too large for the device buffers");
     if (entries > c.entry_cap) {
+        c.drop_graphs();
+        HIP_TRY(hipStreamSynchronize(c.stream));
         c.entry_cap = (int)entries;
         HIP_TRY(c.cell_atoms_tmp.ensure(entries));
         HIP_TRY(c.cell_atoms.ensure(entries));
         HIP_TRY(c.ent_pos.ensure(entries));
         HIP_TRY(c.ent_par.ensure(entries));
     }
+    if (changed) return layout(c);
     return GBI_OK;
 }
 
-gbi::KParams make_kparams(gbi_ctx& c) {
+gbi::KParams make_kparams(gbi_ctx& c, bool adjoint) {
     gbi::KParams k{};
-    const gbi_params& P = c.P;
+    k.S = c.S;
     k.N = c.N;
-    k.Q = c.Q;
-    k.NQ = c.N * c.Q;
-    k.cell_cap = c.cell_cap;
+    k.NQ = c.NQ;
+    k.cell_total = (int)c.cell_total;
     k.entry_cap = c.entry_cap;
-    k.grid_kind = c.kind;
-    k.vol_kind = c.vol;
-    k.cap = P.cell_cap > 0 ? P.cell_cap : INT32_MAX;
-    k.n_lebedev = P.n_lebedev;
-    k.g = c.kind == gbi::GRID_FAITHFUL ? P.grid_length : P.tight_grid_length;
-    k.inv_g = 1.0 / k.g;
-    k.gpad = c.kind == gbi::GRID_FAITHFUL ? c.gpad : c.max_reach;
-    k.halfdiag = c.halfdiag;
-    k.buffer = P.buffer;
-    k.tight_margin = kTightMargin;
-    k.sw_d = P.sw;
-    k.sw = (float)P.sw;
-    k.k1 = (float)(3.0 / (4.0 * P.sw));
-    k.k3 = (float)(1.0 / (4.0 * P.sw * P.sw * P.sw));
-    k.beta = (float)P.beta;
-    k.lambda = (float)P.lambda;
-    k.S0 = (float)P.S0;
-    k.vguard = (float)P.vguard;
-    k.fclamp = 1.0e6f;
-    k.ssat = 6.0f;
-    k.v_empty = c.vol == gbi::VOL_GBSW ? 1.0f
-                : c.vol == gbi::VOL_GBMV2 ? 0.0f
-                                          : (float)(1.0 / (1.0 + std::exp(P.beta * (0.0 - P.lambda))));
-    k.sign_chain = c.vol == gbi::VOL_GBSW ? 1.0f : -1.0f;
-    k.alpha0 = P.alpha0;
-    k.alpha1 = P.alpha1;
-    k.eta = P.eta;
-    k.slope = P.slope;
-    k.shift = P.shift;
-    k.prefactor = P.prefactor;
-    k.orders[0] = P.orders[0];
-    k.orders[1] = P.orders[1];
-    k.i4 = P.orders[0] == 4 ? 0 : 1;
-    k.i7 = 1 - k.i4;
-    k.pair_splits = c.pair_splits;
+    k.n_tiles = c.n_tiles;
+    k.max_splits = c.max_splits;
+    k.adjoint = adjoint ? 1 : 0;
+    k.perm_stride = c.perm_stride;
+    k.persist_blocks = c.num_cus * std::max(1, c.tuning.blocks_per_cu);
+    k.sys = c.d_sys.p;
+    k.geom = c.geom.p;
+    k.flags_all = c.flags_all.p;
+    k.atom_sys = c.atom_sys.p;
+    k.cell_sys = c.cell_sys.p;
     k.pos = c.pos.p;
     k.charge = c.d_charge.p;
     k.radius = c.d_radius.p;
@@ -309,33 +467,40 @@ gbi::KParams make_kparams(gbi_ctx& c) {
     k.aparam = c.aparam.p;
     k.quad_off = c.quad_off.p;
     k.quad_pw = c.quad_pw.p;
-    k.geom = c.geom.p;
+    k.quad_r2 = c.quad_r2.p;
+    k.quad_rows = c.quad_rows;
+    k.pair_tiles = c.pair_tiles.p;
     k.cell_count = c.cell_count.p;
     k.cell_fill = c.cell_fill.p;
     k.cell_start = c.cell_start.p;
     k.cell_count_eff = c.cell_count_eff.p;
     k.cell_atoms_tmp = c.cell_atoms_tmp.p;
     k.cell_atoms = c.cell_atoms.p;
-    k.cell_pt_count = c.cell_pt_count.p;
     k.ent_pos = c.ent_pos.p;
     k.ent_par = c.ent_par.p;
+    k.cell_pt_count = c.cell_pt_count.p;
     k.cell_act_count = c.cell_act_count.p;
     k.frec_base = c.frec_base.p;
-    k.pt_cap = c.pt_cap;
     k.worklist = c.worklist.p;
+    k.worklist_sys = c.worklist_sys.p;
     k.prec = c.prec.p;
     k.frec0 = c.frec0.p;
     k.frec1 = c.frec1.p;
     k.frec2 = c.frec2.p;
     k.vrec = c.vrec.p;
-    k.quad_r2 = c.quad_r2.p;
-    k.perm_stride = c.perm_stride;
+    k.vsig = c.vsig.p;
     k.integrals = c.integrals.p;
     k.born = c.born.p;
     k.dbdi = c.dbdi.p;
     k.dedi = c.dedi.p;
     k.pair_part = c.pair_part.p;
     k.atom_energy = c.atom_energy.p;
+    k.dedq = c.dedq.p;
+    k.dedr_fix = c.dedr_fix.p;
+    k.dedr = c.dedr.p;
+    k.vacc_fix = c.vacc_fix.p;
+    k.theta_atom = c.theta_atom.p;
+    k.theta = c.theta.p;
     k.force_fix = c.force_fix.p;
     k.energy = c.energy.p;
     k.work = c.work.p;
@@ -343,80 +508,265 @@ gbi::KParams make_kparams(gbi_ctx& c) {
     return k;
 }
 
-gbi_status read_geom(gbi_ctx& c, gbi::Geom& G) {
-    HIP_TRY(hipMemcpyAsync(&G, c.geom.p, sizeof(G), hipMemcpyDeviceToHost, c.stream));
+gbi_status ensure_adjoint_buffers(gbi_ctx& c) {
+    if (c.theta.p) return GBI_OK;
+    c.drop_graphs();
     HIP_TRY(hipStreamSynchronize(c.stream));
+    HIP_TRY(c.vsig.ensure(c.NQ));
+    HIP_TRY(c.dedq.ensure(c.N));
+    HIP_TRY(c.dedr.ensure(c.N));
+    HIP_TRY(c.dedr_fix.ensure(c.N));
+    HIP_TRY(c.vacc_fix.ensure((size_t)c.S * gbi::VA_COUNT));
+    HIP_TRY(c.theta_atom.ensure((size_t)gbi::TH_COUNT * c.N));
+    HIP_TRY(c.theta.ensure((size_t)c.S * gbi::TH_COUNT));
     return GBI_OK;
 }
 
-// Size the grid buffers from the device's own bounding box (first call or after overflow).
-gbi_status size_from_device(gbi_ctx& c) {
-    std::vector<double> host(3 * (size_t)c.N);
-    HIP_TRY(hipMemcpyAsync(host.data(), c.pos.p, sizeof(double) * 3 * c.N, hipMemcpyDeviceToHost, c.stream));
-    HIP_TRY(hipStreamSynchronize(c.stream));
-    for (double v : host)
-        if (!std::isfinite(v)) return fail(GBI_ERR_NONFINITE, "non-finite atom position");
-    long long cells, entries;
-    estimate_caps(c, host, cells, entries);
-    return ensure_grid_buffers(c, std::max<long long>(cells, c.cell_cap), std::max<long long>(entries, c.entry_cap));
+gbi_status run(gbi_ctx& c, uint32_t flags) {
+    if (!c.have_positions) return fail(GBI_ERR_INVALID_ARGUMENT, "positions were never set");
+    const bool adjoint = (flags & GBI_ADJOINT) != 0;
+    if (c.cell_total == 0) ST_TRY(size_from_device(c, 0));
+    if (adjoint) ST_TRY(ensure_adjoint_buffers(c));
+    gbi::KParams k = make_kparams(c, adjoint);
+    if (c.use_graph && !c.profiling) {
+        hipGraphExec_t& ge = c.graph[adjoint ? 1 : 0];
+        if (!ge) {
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+            const hipError_t le = gbi::launch_evaluate(k, c.kinds, c.stream, nullptr);
+            const hipError_t ce = hipStreamEndCapture(c.stream, &g);
+            HIP_TRY(le);
+            HIP_TRY(ce);
+            const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) ge = nullptr;
+            HIP_TRY(ie);
+        }
+        HIP_TRY(hipGraphLaunch(ge, c.stream));
+    } else {
+        const hipEvent_t* marks = nullptr;
+        if (c.profiling && c.prof_used < 4096) {
+            if (c.prof_used == c.prof_events.size()) {
+                std::array<hipEvent_t, gbi::kPasses + 1> a;
+                for (auto& e : a) HIP_TRY(hipEventCreate(&e));
+                c.prof_events.push_back(a);
+            }
+            marks = c.prof_events[c.prof_used++].data();
+        }
+        HIP_TRY(gbi::launch_evaluate(k, c.kinds, c.stream, marks));
+    }
+    c.computed = true;
+    c.computed_adjoint = adjoint;
+    return GBI_OK;
 }
 
+// Device flags of the last evaluation: errors, or capacity growth and a redo.
 gbi_status check_flags(gbi_ctx& c, bool& retry) {
     retry = false;
-    gbi::Geom G;
-    gbi_status st = read_geom(c, G);
-    if (st != GBI_OK) return st;
-    if (G.flags & gbi::FLAG_NONFINITE_POS) return fail(GBI_ERR_NONFINITE, "non-finite atom position");
-    if (G.flags & gbi::FLAG_GRID_TOO_LARGE)
+    std::vector<gbi::SysGeom> G(c.S);
+    int all = 0, total_entries = 0;
+    HIP_TRY(hipMemcpyAsync(G.data(), c.geom.p, sizeof(gbi::SysGeom) * c.S, hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(hipMemcpyAsync(&all, c.flags_all.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(hipMemcpyAsync(&total_entries, c.cell_start.p + c.cell_total, sizeof(int), hipMemcpyDeviceToHost,
+                           c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    for (int s = 0; s < c.S; ++s) all |= G[s].flags;
+    if (all & gbi::FLAG_NONFINITE_POS) return fail(GBI_ERR_NONFINITE, "non-finite atom position");
+    if (all & gbi::FLAG_GRID_TOO_LARGE)
         return fail(GBI_ERR_GRID_TOO_LARGE,
                     "CharmmGBMVForce: CustomGBIntegral lookup table dimension is too large, check atom positions!");
-    if (G.flags & (gbi::FLAG_CELL_OVERFLOW | gbi::FLAG_ENTRY_OVERFLOW | gbi::FLAG_POINT_OVERFLOW)) {
-        long long cells = (long long)c.cell_cap, entries = c.entry_cap;
-        if (G.flags & gbi::FLAG_POINT_OVERFLOW) c.pt_cap = std::min(2 * c.pt_cap, c.N * c.Q);
-        if (G.flags & gbi::FLAG_CELL_OVERFLOW) cells = (long long)(c.cell_cap * 2ll);
-        if (G.flags & gbi::FLAG_ENTRY_OVERFLOW) entries = std::max<long long>(2ll * c.entry_cap, (long long)G.total_entries * 5 / 4);
-        st = size_from_device(c);
-        if (st != GBI_OK) return st;
-        st = ensure_grid_buffers(c, cells, entries);
-        if (st != GBI_OK) return st;
+    if (all & (gbi::FLAG_CELL_OVERFLOW | gbi::FLAG_ENTRY_OVERFLOW | gbi::FLAG_POINT_OVERFLOW)) {
+        bool grew = false;
+        for (int s = 0; s < c.S; ++s) {
+            SysHost& h = c.sys[s];
+            if (G[s].flags & gbi::FLAG_CELL_OVERFLOW) {
+                h.cell_cap *= 2;
+                grew = true;
+            }
+            if (G[s].flags & gbi::FLAG_POINT_OVERFLOW) {
+                h.pt_cap = (int)std::min<long long>(2ll * h.pt_cap, (long long)h.N * h.Q);
+                grew = true;
+            }
+        }
+        long long min_entries = 0;
+        if (all & gbi::FLAG_ENTRY_OVERFLOW)
+            min_entries = std::max<long long>(2ll * c.entry_cap, (long long)total_entries * 5 / 4);
+        if (grew) ST_TRY(layout(c));
+        ST_TRY(size_from_device(c, min_entries));
         retry = true;
         return GBI_OK;
     }
-    if (G.flags & gbi::FLAG_NONFINITE_RESULT) return fail(GBI_ERR_NONFINITE, "non-finite Born radius or energy");
+    if (all & gbi::FLAG_NONFINITE_RESULT) return fail(GBI_ERR_NONFINITE, "non-finite Born radius or energy");
     return GBI_OK;
 }
 
-gbi_status run(gbi_ctx& c) {
-    if (!c.have_positions) return fail(GBI_ERR_INVALID_ARGUMENT, "positions were never set");
-    if (c.cell_cap == 0) {
-        gbi_status st = size_from_device(c);
-        if (st != GBI_OK) return st;
-    }
-    gbi::KParams k = make_kparams(c);
-    const hipEvent_t* marks = nullptr;
-    if (c.profiling && c.prof_used < 4096) {
-        if (c.prof_used == c.prof_events.size()) {
-            std::array<hipEvent_t, gbi::kPasses + 1> a;
-            for (auto& e : a) HIP_TRY(hipEventCreate(&e));
-            c.prof_events.push_back(a);
-        }
-        marks = c.prof_events[c.prof_used++].data();
-    }
-    HIP_TRY(gbi::launch_evaluate(k, GBI_FORCES | GBI_ENERGY, c.stream, marks));
-    c.computed = true;
-    return GBI_OK;
-}
-
-gbi_status run_checked(gbi_ctx& c) {
-    for (int attempt = 0; attempt < 4; ++attempt) {
-        gbi_status st = run(c);
-        if (st != GBI_OK) return st;
+gbi_status run_checked(gbi_ctx& c, uint32_t flags) {
+    for (int attempt = 0; attempt < 6; ++attempt) {
+        ST_TRY(run(c, flags));
         bool retry = false;
-        st = check_flags(c, retry);
-        if (st != GBI_OK) return st;
+        ST_TRY(check_flags(c, retry));
         if (!retry) return GBI_OK;
     }
     return fail(GBI_ERR_CAPACITY, "lookup grid kept outgrowing its buffers");
+}
+
+gbi_status apply_tight_grid(gbi_ctx& c, double g) {
+    bool changed = false;
+    for (SysHost& s : c.sys)
+        if (s.kind == gbi::GRID_TIGHT && s.g != g) {
+            s.g = g;
+            s.cell_cap = 0;
+            const double est = 1.6 * 100.0 * s.Q * g * g * g + 64.0;
+            s.pt_cap = std::max(1, (int)std::min<double>(est, (double)s.N * s.Q));
+            changed = true;
+        }
+    if (!changed) return GBI_OK;
+    c.cell_total = 0;  // re-sized from the positions on the next evaluation
+    c.drop_graphs();
+    return GBI_OK;
+}
+
+gbi_status create_batch(int S, const int32_t* n_atoms, const double* charge, const double* radius,
+                        const gbi_params* params, gbi_ctx** out) {
+    if (!out || !n_atoms || !charge || !radius || !params) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (S <= 0) return fail(GBI_ERR_INVALID_ARGUMENT, "n_systems must be positive");
+    long long Ntot = 0, NQtot = 0;
+    for (int s = 0; s < S; ++s) {
+        ST_TRY(validate_params(params[s], n_atoms[s]));
+        Ntot += n_atoms[s];
+        NQtot += (long long)n_atoms[s] * params[s].n_radial * params[s].n_lebedev;
+    }
+    if (Ntot >= gbi::kMaxAtoms) return fail(GBI_ERR_INVALID_ARGUMENT, "too many atoms (at most 2^21 - 1)");
+    if (NQtot >= (1ll << 31)) return fail(GBI_ERR_INVALID_ARGUMENT, "N * Q must be below 2^31");
+    ST_TRY(check_atoms((int)Ntot, charge, radius));
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GBI_ERR_NO_DEVICE, "no HIP device");
+    auto* c = new gbi_ctx();
+    auto cleanup = [&](gbi_status st) {
+        delete c;
+        return st;
+    };
+#define CTX_TRY(expr)                      \
+    do {                                   \
+        gbi_status s_ = (expr);            \
+        if (s_ != GBI_OK) return cleanup(s_); \
+    } while (0)
+#define CTX_HIP(expr)                                                                                \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            return cleanup(fail(e_ == hipErrorOutOfMemory ? GBI_ERR_OUT_OF_MEMORY : GBI_ERR_HIP,     \
+                                std::string(#expr) + ": " + hipGetErrorString(e_)));                 \
+    } while (0)
+    c->S = S;
+    c->N = (int)Ntot;
+    c->NQ = (int)NQtot;
+    c->charge.assign(charge, charge + Ntot);
+    c->radius.assign(radius, radius + Ntot);
+    c->tuning.blocks_per_cu = kDefaultBlocksPerCU;
+    c->tuning.pair_columns = 0;
+    c->tuning.tight_grid_length = 0.0;
+    int atom = 0, qrow = 0, pt = 0;
+    for (int s = 0; s < S; ++s) {
+        SysHost h;
+        h.P = params[s];
+        if (h.P.model == GBI_MODEL_GBSW) h.P.integral_type = GBI_INTEGRAL_GBSW;
+        if (h.P.tight_grid_length <= 0.0) h.P.tight_grid_length = 0.15;
+        h.N = n_atoms[s];
+        h.atom_begin = atom;
+        h.Q = h.P.n_radial * h.P.n_lebedev;
+        h.quad_base = qrow;
+        h.pt_base = pt;
+        h.vol = (gbi::VolKind)h.P.integral_type;
+        if (h.P.lookup_policy == GBI_LOOKUP_BRUTE)
+            h.kind = gbi::GRID_BRUTE;
+        else if (h.P.model == GBI_MODEL_GBSW)
+            h.kind = gbi::GRID_TIGHT;  // the standalone lists are complete, so exact lists give the same V
+        else
+            h.kind = gbi::GRID_FAITHFUL;
+        h.g = h.kind == gbi::GRID_FAITHFUL ? h.P.grid_length : h.P.tight_grid_length;
+        h.halfdiag = std::sqrt(3.0) / 2.0 * h.P.grid_length;
+        h.gpad = 0.03 + h.halfdiag + h.P.buffer;  // CustomGBIntegral.cpp:174-175
+        const double est = 1.6 * 100.0 * h.Q * h.g * h.g * h.g + 64.0;
+        h.pt_cap = h.kind == gbi::GRID_BRUTE ? h.N * h.Q : (int)std::min<double>(est, (double)h.N * h.Q);
+        h.pt_cap = std::max(h.pt_cap, 1);
+        c->kinds[h.vol] = true;
+        atom += h.N;
+        qrow += h.Q;
+        pt += h.N * h.Q;
+        c->sys.push_back(h);
+    }
+    c->quad_rows = qrow;
+    (void)hipGetDevice(&c->device);
+    if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+        c->num_cus <= 0)
+        c->num_cus = 256;
+    CTX_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+    c->stream = c->own_stream;
+    const size_t N = c->N, NQ = c->NQ;
+    CTX_HIP(c->d_sys.ensure(S));
+    CTX_HIP(c->geom.ensure(S));
+    CTX_HIP(c->flags_all.ensure(1));
+    CTX_HIP(c->atom_sys.ensure(N));
+    CTX_HIP(c->pos.ensure(3 * N));
+    CTX_HIP(c->d_charge.ensure(N));
+    CTX_HIP(c->d_radius.ensure(N));
+    CTX_HIP(c->d_reach.ensure(N));
+    CTX_HIP(c->aparam.ensure(N));
+    CTX_HIP(c->quad_off.ensure(3 * (size_t)qrow));
+    CTX_HIP(c->quad_pw.ensure(2 * (size_t)qrow));
+    CTX_HIP(c->quad_r2.ensure(qrow));
+    CTX_HIP(c->integrals.ensure(2 * N));
+    CTX_HIP(c->born.ensure(N));
+    CTX_HIP(c->dbdi.ensure(2 * N));
+    CTX_HIP(c->dedi.ensure(2 * N));
+    CTX_HIP(c->atom_energy.ensure(N));
+    CTX_HIP(c->energy.ensure(S));
+    CTX_HIP(c->stats.ensure(4));
+    CTX_HIP(c->work.ensure(8));
+    CTX_HIP(c->frec0.ensure(NQ));
+    CTX_HIP(c->frec1.ensure(NQ));
+    CTX_HIP(c->frec2.ensure(c->kinds[gbi::VOL_GBMV2] ? NQ : 1));
+    CTX_HIP(c->vrec.ensure(NQ));
+    CTX_HIP(c->force_fix.ensure(3 * N));
+    // quadrature tables: offsets and w_q / r_q^(order-2) per integral, one block of rows per system
+    std::vector<double> off(3 * (size_t)qrow), pw(2 * (size_t)qrow);
+    std::vector<float> r2(qrow);
+    for (const SysHost& h : c->sys) {
+        std::vector<std::array<double, 5>> table;
+        try {
+            table = gbi::quadrature_table(h.P.n_radial, h.P.r0, h.P.r1, h.P.n_lebedev);
+        } catch (const std::exception& e) {
+            return cleanup(fail(GBI_ERR_INVALID_ARGUMENT, e.what()));
+        }
+        for (int q = 0; q < h.Q; ++q) {
+            const size_t row = (size_t)h.quad_base + q;
+            for (int d = 0; d < 3; ++d) off[3 * row + d] = table[q][d];
+            for (int k = 0; k < 2; ++k) pw[k * (size_t)qrow + row] = table[q][4] / std::pow(table[q][3], h.P.orders[k] - 2);
+            r2[row] = (float)(table[q][3] * table[q][3]);
+        }
+    }
+    std::vector<int> as(N);
+    for (int s = 0; s < S; ++s) std::fill(as.begin() + c->sys[s].atom_begin, as.begin() + c->sys[s].atom_begin + c->sys[s].N, s);
+    CTX_HIP(hipMemcpyAsync(c->quad_off.p, off.data(), sizeof(double) * off.size(), hipMemcpyHostToDevice, c->stream));
+    CTX_HIP(hipMemcpyAsync(c->quad_pw.p, pw.data(), sizeof(double) * pw.size(), hipMemcpyHostToDevice, c->stream));
+    CTX_HIP(hipMemcpyAsync(c->quad_r2.p, r2.data(), sizeof(float) * r2.size(), hipMemcpyHostToDevice, c->stream));
+    CTX_HIP(hipMemcpyAsync(c->atom_sys.p, as.data(), sizeof(int) * N, hipMemcpyHostToDevice, c->stream));
+    // binning visits atoms with a stride coprime to N so concurrent workgroups hit distant cells
+    c->perm_stride = 1;
+    for (int cand : {7919, 7927, 7933, 7937, 7949})
+        if (c->N % cand != 0) {
+            c->perm_stride = cand % c->N == 0 ? 1 : cand;
+            break;
+        }
+    CTX_HIP(hipStreamSynchronize(c->stream));
+    CTX_TRY(upload_atoms(*c));
+    CTX_TRY(build_tiles(*c));
+#undef CTX_TRY
+#undef CTX_HIP
+    *out = c;
+    return GBI_OK;
 }
 
 }  // namespace
@@ -503,115 +853,18 @@ gbi_status gbi_quadrature(int32_t n_radial, double r0, double r1, int32_t n_lebe
 
 gbi_status gbi_create(int32_t n_atoms, const double* charge, const double* radius, const gbi_params* params,
                       gbi_ctx** out) {
-    if (!out || !charge || !radius || !params) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
-    *out = nullptr;
-    gbi_status st = validate_params(*params, n_atoms);
-    if (st != GBI_OK) return st;
-    for (int j = 0; j < n_atoms; ++j) {
-        if (!std::isfinite(charge[j]) || !std::isfinite(radius[j]))
-            return fail(GBI_ERR_NONFINITE, "non-finite charge or radius");
-        if (!(radius[j] > 0.0)) return fail(GBI_ERR_INVALID_ARGUMENT, "radii must be positive");
-    }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GBI_ERR_NO_DEVICE, "no HIP device");
-    auto* c = new gbi_ctx();
-    c->N = n_atoms;
-    c->P = *params;
-    if (c->P.model == GBI_MODEL_GBSW) c->P.integral_type = GBI_INTEGRAL_GBSW;
-    if (c->P.tight_grid_length <= 0.0) c->P.tight_grid_length = 0.15;
-    c->Q = c->P.n_radial * c->P.n_lebedev;
-    c->charge.assign(charge, charge + n_atoms);
-    c->radius.assign(radius, radius + n_atoms);
-    c->vol = (gbi::VolKind)c->P.integral_type;
-    if (c->P.lookup_policy == GBI_LOOKUP_BRUTE)
-        c->kind = gbi::GRID_BRUTE;
-    else if (c->P.model == GBI_MODEL_GBSW)
-        c->kind = gbi::GRID_TIGHT;  // the standalone lists are complete, so exact lists give the same V
-    else
-        c->kind = gbi::GRID_FAITHFUL;
-    c->halfdiag = std::sqrt(3.0) / 2.0 * c->P.grid_length;
-    {
-        const double gc = c->kind == gbi::GRID_FAITHFUL ? c->P.grid_length : c->P.tight_grid_length;
-        const double est = 1.6 * 100.0 * c->Q * gc * gc * gc + 64.0;
-        c->pt_cap = c->kind == gbi::GRID_BRUTE ? n_atoms * c->Q : std::min((int)est, n_atoms * c->Q);
-        c->pt_cap = std::max(c->pt_cap, 1);
-    }
-    c->gpad = 0.03 + c->halfdiag + c->P.buffer;  // CustomGBIntegral.cpp:174-175
-    (void)hipGetDevice(&c->device);
-    auto cleanup = [&](gbi_status s) {
-        delete c;
-        return s;
-    };
-#define CTX_TRY(expr)                                                                                \
-    do {                                                                                             \
-        hipError_t e_ = (expr);                                                                      \
-        if (e_ != hipSuccess) {                                                                      \
-            fail(e_ == hipErrorOutOfMemory ? GBI_ERR_OUT_OF_MEMORY : GBI_ERR_HIP,                    \
-                 std::string(#expr) + ": " + hipGetErrorString(e_));                                 \
-            return cleanup(e_ == hipErrorOutOfMemory ? GBI_ERR_OUT_OF_MEMORY : GBI_ERR_HIP);          \
-        }                                                                                            \
-    } while (0)
-    CTX_TRY(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
-    c->stream = c->own_stream;
-    const size_t N = n_atoms, Q = c->Q, NQ = N * Q;
-    CTX_TRY(c->pos.ensure(3 * N));
-    CTX_TRY(c->d_charge.ensure(N));
-    CTX_TRY(c->d_radius.ensure(N));
-    CTX_TRY(c->d_reach.ensure(N));
-    CTX_TRY(c->aparam.ensure(N));
-    CTX_TRY(c->quad_off.ensure(3 * Q));
-    CTX_TRY(c->quad_pw.ensure(2 * Q));
-    CTX_TRY(c->integrals.ensure(2 * N));
-    CTX_TRY(c->born.ensure(N));
-    CTX_TRY(c->dbdi.ensure(2 * N));
-    CTX_TRY(c->dedi.ensure(2 * N));
-    CTX_TRY(c->atom_energy.ensure(N));
-    CTX_TRY(c->energy.ensure(1));
-    CTX_TRY(c->stats.ensure(4));
-    CTX_TRY(c->geom.ensure(1));
-    CTX_TRY(c->work.ensure(4));
-    CTX_TRY(c->frec0.ensure(NQ));
-    CTX_TRY(c->frec1.ensure(NQ));
-    CTX_TRY(c->frec2.ensure(c->vol == gbi::VOL_GBMV2 ? NQ : 1));
-    CTX_TRY(c->vrec.ensure(NQ));
-    CTX_TRY(c->quad_r2.ensure(Q));
-    CTX_TRY(c->force_fix.ensure(3 * N));
-    // pair pass: enough workgroups to fill the device, at least 64 columns per split
-    const int rows = (int)((N + 63) / 64);
-    int splits = (2048 + rows - 1) / rows;
-    splits = std::max(1, std::min(splits, std::min(64, (int)(N / 64))));
-    c->pair_splits = std::max(1, splits);
-    CTX_TRY(c->pair_part.ensure((size_t)c->pair_splits * 5 * N));
-    // quadrature tables: offsets and w_q / r_q^(order-2) per integral
-    std::vector<std::array<double, 5>> table;
-    try {
-        table = gbi::quadrature_table(c->P.n_radial, c->P.r0, c->P.r1, c->P.n_lebedev);
-    } catch (const std::exception& e) {
-        fail(GBI_ERR_INVALID_ARGUMENT, e.what());
-        return cleanup(GBI_ERR_INVALID_ARGUMENT);
-    }
-    std::vector<double> off(3 * Q), pw(2 * Q);
-    for (size_t q = 0; q < Q; ++q) {
-        for (int d = 0; d < 3; ++d) off[3 * q + d] = table[q][d];
-        for (int k = 0; k < 2; ++k) pw[k * Q + q] = table[q][4] / std::pow(table[q][3], c->P.orders[k] - 2);
-    }
-    CTX_TRY(hipMemcpyAsync(c->quad_off.p, off.data(), sizeof(double) * 3 * Q, hipMemcpyHostToDevice, c->stream));
-    CTX_TRY(hipMemcpyAsync(c->quad_pw.p, pw.data(), sizeof(double) * 2 * Q, hipMemcpyHostToDevice, c->stream));
-    std::vector<float> r2(Q);
-    for (size_t q = 0; q < Q; ++q) r2[q] = (float)(table[q][3] * table[q][3]);
-    CTX_TRY(hipMemcpyAsync(c->quad_r2.p, r2.data(), sizeof(float) * Q, hipMemcpyHostToDevice, c->stream));
-    // binning visits atoms with a stride coprime to N so concurrent workgroups hit distant cells
-    c->perm_stride = 1;
-    for (int cand : {7919, 7927, 7933, 7937, 7949})
-        if (n_atoms % cand != 0) {
-            c->perm_stride = cand % n_atoms == 0 ? 1 : cand;
-            break;
-        }
-    CTX_TRY(hipStreamSynchronize(c->stream));
-#undef CTX_TRY
-    st = upload_params(*c);
-    if (st != GBI_OK) return cleanup(st);
-    *out = c;
+    return create_batch(1, &n_atoms, charge, radius, params, out);
+}
+
+gbi_status gbi_batch_create(int32_t n_systems, const int32_t* n_atoms, const double* charge, const double* radius,
+                            const gbi_params* params, gbi_ctx** out) {
+    return create_batch(n_systems, n_atoms, charge, radius, params, out);
+}
+
+gbi_status gbi_get_layout(gbi_ctx* ctx, int32_t* n_systems, int32_t* n_atoms_total) {
+    if (!ctx) return fail(GBI_ERR_INVALID_ARGUMENT, "null context");
+    if (n_systems) *n_systems = ctx->S;
+    if (n_atoms_total) *n_atoms_total = ctx->N;
     return GBI_OK;
 }
 
@@ -632,18 +885,16 @@ gbi_status gbi_update_parameters(gbi_ctx* ctx, int32_t n_atoms, const double* ch
     if (!ctx || !charge || !radius) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
     if (n_atoms != ctx->N)  // CharmmReferenceKernels.cpp:373-374
         return fail(GBI_ERR_PARTICLE_COUNT, "updateParametersInContext: The number of particles has changed");
-    for (int j = 0; j < n_atoms; ++j) {
-        if (!std::isfinite(charge[j]) || !std::isfinite(radius[j]))
-            return fail(GBI_ERR_NONFINITE, "non-finite charge or radius");
-        if (!(radius[j] > 0.0)) return fail(GBI_ERR_INVALID_ARGUMENT, "radii must be positive");
-    }
+    ST_TRY(check_atoms(n_atoms, charge, radius));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->charge.assign(charge, charge + n_atoms);
     ctx->radius.assign(radius, radius + n_atoms);
-    gbi_status st = upload_params(*ctx);
-    if (st != GBI_OK) return st;
-    // radii change the lookup reach: re-estimate the grid on the next call
-    if (ctx->have_positions) return size_from_device(*ctx);
+    ST_TRY(upload_atoms(*ctx));
+    // radii change the lookup reach: re-estimate the grid
+    if (ctx->have_positions) {
+        if (ctx->cell_total > 0) ST_TRY(layout(*ctx));  // new reach in the descriptors
+        return size_from_device(*ctx, 0);
+    }
     return GBI_OK;
 }
 
@@ -658,17 +909,15 @@ gbi_status gbi_set_positions(gbi_ctx* ctx, const double* pos, int32_t pos_on_dev
 
 gbi_status gbi_compute(gbi_ctx* ctx, uint32_t flags) {
     if (!ctx) return fail(GBI_ERR_INVALID_ARGUMENT, "null context");
-    (void)flags;
-    return run(*ctx);
+    return run(*ctx, flags);
 }
 
 gbi_status gbi_synchronize(gbi_ctx* ctx) {
     if (!ctx) return fail(GBI_ERR_INVALID_ARGUMENT, "null context");
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     bool retry = false;
-    gbi_status st = check_flags(*ctx, retry);
-    if (st != GBI_OK) return st;
-    if (retry) return run_checked(*ctx);
+    ST_TRY(check_flags(*ctx, retry));
+    if (retry) return run_checked(*ctx, ctx->computed_adjoint ? GBI_ADJOINT : 0u);
     return GBI_OK;
 }
 
@@ -678,22 +927,29 @@ gbi_status gbi_execute(gbi_ctx* ctx, const double* pos, int32_t include_forces, 
     if (include_forces && !forces) return fail(GBI_ERR_INVALID_ARGUMENT, "null force array");
     for (int a = 0; a < 3 * ctx->N; ++a)
         if (!std::isfinite(pos[a])) return fail(GBI_ERR_NONFINITE, "non-finite atom position");
-    gbi_status st = gbi_set_positions(ctx, pos, 0);
-    if (st != GBI_OK) return st;
-    st = run_checked(*ctx);
-    if (st != GBI_OK) return st;
-    if (include_energy && energy) {
-        st = gbi_get_energy(ctx, energy);
-        if (st != GBI_OK) return st;
-    }
+    ST_TRY(gbi_set_positions(ctx, pos, 0));
+    ST_TRY(run_checked(*ctx, GBI_FORCES | GBI_ENERGY));
+    if (include_energy && energy) ST_TRY(gbi_get_energy(ctx, energy));
     if (include_forces) return gbi_get_forces(ctx, forces, 1);
     return GBI_OK;
+}
+
+gbi_status gbi_execute_adjoint(gbi_ctx* ctx, const double* pos, double* energy, double* forces, double* dEdR,
+                               double* dEdq, double* dEdtheta) {
+    if (!ctx || !pos) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
+    for (int a = 0; a < 3 * ctx->N; ++a)
+        if (!std::isfinite(pos[a])) return fail(GBI_ERR_NONFINITE, "non-finite atom position");
+    ST_TRY(gbi_set_positions(ctx, pos, 0));
+    ST_TRY(run_checked(*ctx, GBI_FORCES | GBI_ENERGY | GBI_ADJOINT));
+    if (energy) ST_TRY(gbi_get_energy(ctx, energy));
+    if (forces) ST_TRY(gbi_get_forces(ctx, forces, 1));
+    return gbi_get_adjoint(ctx, dEdR, dEdq, dEdtheta);
 }
 
 gbi_status gbi_get_energy(gbi_ctx* ctx, double* energy) {
     if (!ctx || !energy) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
     if (!ctx->computed) return fail(GBI_ERR_INVALID_ARGUMENT, "nothing computed yet");
-    HIP_TRY(hipMemcpyAsync(energy, ctx->energy.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(energy, ctx->energy.p, sizeof(double) * ctx->S, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return GBI_OK;
 }
@@ -718,6 +974,21 @@ gbi_status gbi_get_forces(gbi_ctx* ctx, double* forces, int32_t add) {
     return GBI_OK;
 }
 
+gbi_status gbi_get_adjoint(gbi_ctx* ctx, double* dEdR, double* dEdq, double* dEdtheta) {
+    if (!ctx) return fail(GBI_ERR_INVALID_ARGUMENT, "null context");
+    if (!ctx->computed || !ctx->computed_adjoint)
+        return fail(GBI_ERR_INVALID_ARGUMENT, "the last evaluation did not compute the adjoint");
+    if (dEdR)
+        HIP_TRY(hipMemcpyAsync(dEdR, ctx->dedr.p, sizeof(double) * ctx->N, hipMemcpyDeviceToHost, ctx->stream));
+    if (dEdq)
+        HIP_TRY(hipMemcpyAsync(dEdq, ctx->dedq.p, sizeof(double) * ctx->N, hipMemcpyDeviceToHost, ctx->stream));
+    if (dEdtheta)
+        HIP_TRY(hipMemcpyAsync(dEdtheta, ctx->theta.p, sizeof(double) * ctx->S * gbi::TH_COUNT,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return GBI_OK;
+}
+
 gbi_status gbi_get_born_radii(gbi_ctx* ctx, double* born) {
     if (!ctx || !born) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
     HIP_TRY(hipMemcpyAsync(born, ctx->born.p, sizeof(double) * ctx->N, hipMemcpyDeviceToHost, ctx->stream));
@@ -737,7 +1008,7 @@ gbi_status gbi_get_stats(gbi_ctx* ctx, gbi_stats* stats) {
     if (!ctx || !stats) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
     if (!ctx->computed) return fail(GBI_ERR_INVALID_ARGUMENT, "nothing computed yet");
     HIP_TRY(hipMemsetAsync(ctx->stats.p, 0, sizeof(double) * 4, ctx->stream));
-    gbi::KParams k = make_kparams(*ctx);
+    gbi::KParams k = make_kparams(*ctx, false);
     HIP_TRY(gbi::launch_stats(k, ctx->stats.p, ctx->stream));
     double h[4];
     HIP_TRY(hipMemcpyAsync(h, ctx->stats.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
@@ -747,7 +1018,102 @@ gbi_status gbi_get_stats(gbi_ctx* ctx, gbi_stats* stats) {
     stats->n_cells = h[2];
     stats->n_entries = h[3];
     stats->n_atoms = ctx->N;
-    stats->n_quad = ctx->Q;
+    stats->n_quad = (double)ctx->NQ / ctx->N;
+    return GBI_OK;
+}
+
+gbi_status gbi_set_graph(gbi_ctx* ctx, int32_t enable) {
+    if (!ctx) return fail(GBI_ERR_INVALID_ARGUMENT, "null context");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->use_graph = enable != 0;
+    if (!ctx->use_graph) ctx->drop_graphs();
+    return GBI_OK;
+}
+
+gbi_status gbi_set_tuning(gbi_ctx* ctx, const gbi_tuning* t) {
+    if (!ctx || !t) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
+    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 256)
+        return fail(GBI_ERR_INVALID_ARGUMENT, "blocks_per_cu must be in 1..256");
+    if (t->pair_columns < 0) return fail(GBI_ERR_INVALID_ARGUMENT, "pair_columns must be >= 0");
+    if (t->tight_grid_length != 0.0 && !(t->tight_grid_length >= 0.05 && t->tight_grid_length <= 1.0))
+        return fail(GBI_ERR_INVALID_ARGUMENT, "tight_grid_length must be 0 or in [0.05, 1] nm");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const bool tiles = t->pair_columns != ctx->tuning.pair_columns;
+    ctx->drop_graphs();
+    ctx->tuning.blocks_per_cu = t->blocks_per_cu;
+    ctx->tuning.pair_columns = t->pair_columns;
+    ctx->tuning.tight_grid_length = t->tight_grid_length;
+    if (tiles) {
+        ST_TRY(build_tiles(*ctx));
+        if (ctx->cell_total > 0) ST_TRY(layout(*ctx));  // pair splits live in the descriptors
+    }
+    if (t->tight_grid_length > 0.0) ST_TRY(apply_tight_grid(*ctx, t->tight_grid_length));
+    return GBI_OK;
+}
+
+gbi_status gbi_get_tuning(gbi_ctx* ctx, gbi_tuning* out) {
+    if (!ctx || !out) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
+    *out = ctx->tuning;
+    return GBI_OK;
+}
+
+gbi_status gbi_autotune(gbi_ctx* ctx, int32_t evals, gbi_tuning* out) {
+    if (!ctx) return fail(GBI_ERR_INVALID_ARGUMENT, "null context");
+    if (!ctx->have_positions) return fail(GBI_ERR_INVALID_ARGUMENT, "positions were never set");
+    if (evals < 1) evals = 3;
+    bool tight = false;
+    for (const SysHost& s : ctx->sys) tight |= s.kind == gbi::GRID_TIGHT;
+    std::vector<gbi_tuning> cands;
+    const double grids_tight[] = {0.15, 0.12, 0.2};
+    for (double g : grids_tight) {
+        if (!tight && g != 0.15) continue;
+        for (int bpc : {8, 16, 32, 64})
+            for (int cols : {0, 1024, 16384}) {
+                gbi_tuning t{};
+                t.blocks_per_cu = bpc;
+                t.pair_columns = cols;
+                t.tight_grid_length = tight ? g : 0.0;
+                cands.push_back(t);
+            }
+    }
+    const bool prof = ctx->profiling;
+    ctx->profiling = false;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    double best = 1e300;
+    gbi_tuning best_t = ctx->tuning;
+    gbi_status st = GBI_OK;
+    for (const gbi_tuning& t : cands) {
+        st = gbi_set_tuning(ctx, &t);
+        if (st != GBI_OK) break;
+        st = run_checked(*ctx, GBI_FORCES | GBI_ENERGY);  // sizes the buffers for this shape
+        if (st != GBI_OK) break;
+        if (hipEventRecord(e0, ctx->stream) != hipSuccess) { st = fail(GBI_ERR_HIP, "hipEventRecord"); break; }
+        for (int r = 0; r < evals && st == GBI_OK; ++r) st = run(*ctx, GBI_FORCES | GBI_ENERGY);
+        if (st != GBI_OK) break;
+        if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            st = fail(GBI_ERR_HIP, "hipEventSynchronize");
+            break;
+        }
+        bool retry = false;
+        st = check_flags(*ctx, retry);
+        if (st != GBI_OK) break;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { st = fail(GBI_ERR_HIP, "hipEventElapsedTime"); break; }
+        if (!retry && ms / evals < best) {
+            best = ms / evals;
+            best_t = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->profiling = prof;
+    if (st != GBI_OK) return st;
+    ST_TRY(gbi_set_tuning(ctx, &best_t));
+    ST_TRY(run_checked(*ctx, GBI_FORCES | GBI_ENERGY));
+    ctx->tuning.ms_per_eval = best;
+    if (out) *out = ctx->tuning;
     return GBI_OK;
 }
 

@@ -1,5 +1,9 @@
 // Internal declarations shared by the gfx950 kernels (gbi_kernels.hip) and the host
 // orchestration (gbi_context.cpp).  Not part of the public ABI.
+//
+// A context evaluates a batch of S >= 1 independent systems with one launch per pass:
+// atoms, quadrature points, lookup cells and cell-list entries of all systems live in
+// shared arrays, each system owning a static range of atoms, points, cells and slots.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,116 +15,164 @@ namespace gbi {
 enum GridKind : int {
     GRID_FAITHFUL = 0,  // CustomGBIntegral lookup grid: corner-anchored membership, cap in index order
     GRID_TIGHT = 1,     // exact GBSW lists: atoms whose switching sphere reaches the cell box
-    GRID_BRUTE = 2      // one cell holding every atom
+    GRID_BRUTE = 2      // one cell holding every atom of the system
 };
 
 // Volume function of the integral.
-enum VolKind : int { VOL_GBSW = 0, VOL_GBMV1 = 1, VOL_GBMV2 = 2 };
+enum VolKind : int { VOL_GBSW = 0, VOL_GBMV1 = 1, VOL_GBMV2 = 2, VOL_KINDS = 3 };
 
-// Device-side flags (bit set in Geom::flags).
+// Device-side flags (SysGeom::flags of the system, OR-ed into *flags_all).
 enum : int {
-    FLAG_CELL_OVERFLOW = 1,   // more cells than allocated
-    FLAG_ENTRY_OVERFLOW = 2,  // more cell-list entries than allocated
-    FLAG_GRID_TOO_LARGE = 4,  // an axis longer than 1000 nm (CustomGBIntegral.cpp:182-183)
+    FLAG_CELL_OVERFLOW = 1,    // more cells than the system's range
+    FLAG_ENTRY_OVERFLOW = 2,   // more cell-list entries than allocated
+    FLAG_GRID_TOO_LARGE = 4,   // an axis longer than 1000 nm (CustomGBIntegral.cpp:182-183)
     FLAG_NONFINITE_POS = 8,
     FLAG_NONFINITE_RESULT = 16,
-    FLAG_POINT_OVERFLOW = 32  // more binned points than allocated (never expected)
+    FLAG_POINT_OVERFLOW = 32   // more points in a cell than its slots
+};
+constexpr int kFatalFlags =
+    FLAG_CELL_OVERFLOW | FLAG_ENTRY_OVERFLOW | FLAG_GRID_TOO_LARGE | FLAG_NONFINITE_POS | FLAG_POINT_OVERFLOW;
+
+constexpr int kMaxAtomChunk = 256;       // atoms staged in LDS per chunk (volume pass)
+constexpr int kQBits = 11;               // packed point = (atom << 11) | q, Q <= 2047
+constexpr int kMaxAtoms = 1 << (32 - kQBits);
+constexpr float kQuantum = 16777216.0f;  // 2^24: chain-rule partials are summed exactly on this grid
+
+// Derivatives of the energy with respect to the constants of a system (adjoint).
+enum Theta : int {
+    TH_PREFACTOR = 0,
+    TH_ALPHA0,
+    TH_ALPHA1,
+    TH_ETA,
+    TH_SLOPE,
+    TH_SHIFT,
+    TH_SW,       // GBSW switching half-width w
+    TH_RSCALE,   // GBSW R' = rscale (R + roffset)
+    TH_ROFFSET,
+    TH_GAMMA0,   // GBMV-I
+    TH_BETA,     // GBMV-I/II
+    TH_LAMBDA,   // GBMV-I/II
+    TH_P1,       // GBMV-II
+    TH_P2,
+    TH_S0,
+    TH_COUNT
+};
+// Force-pass contributions to dE/dtheta, accumulated per system in fixed point 2^32 as
+// sums over points p and list atoms j of s fac_p dV_p/d(.) (s = +1 GBSW, -1 GBMV):
+enum VolAcc : int {
+    VA_W = 0,   // dV/dw = V sum_j (dH_j/dw) / H_j                  (GBSW)
+    VA_A,       // dV/dP2 = sum_j dV/dF_j (dF_j/dC_j + dF_j/db_j)    (GBMV-II, b_j = C_j - 100 R_j^2)
+    VA_AR,      // dV/dP1 = same terms times 10 R_j                  (GBMV-II)
+    VA_GAMMA,   // sum_j dV/dgamma_j gamma_j                         (GBMV-I)
+    VA_COUNT
 };
 
-// Geometry of the lookup grid, written on the device every evaluation.
-struct Geom {
+// Static description of one system (host-built, uploaded once per layout).
+struct SysDesc {
+    int grid_kind, vol_kind;
+    int atom_begin, atom_end;  // global atom range
+    int Q;                     // quadrature points per atom
+    int quad_base;             // first row of this system's quadrature tables
+    int pt_base;               // pid = pt_base + (i - atom_begin) * Q + q
+    int cell_base, cell_cap;   // global cells [cell_base, cell_base + cell_cap)
+    int pt_cap;                // point slots per cell
+    long long prec_base;       // slot s of local cell c: prec_base + c * pt_cap + s
+    int cap;                   // atoms kept per cell
+    int pair_splits;           // column splits of the pair pass
+    int i4, i7;                // which integral is order 4 / 7
+    // grid constants (double, computed on the host with the oracle's expressions)
+    double g, inv_g, gpad, halfdiag, buffer;
+    // volume constants
+    float k1, k3;              // GBSW: H = 1/2 + x (k1 - k3 x^2), k1 = 3/(4w), k3 = 1/(4w^3)
+    float sw, inv_sw, rscale, roffset;
+    float gamma0, beta, lambda, S0, vguard, fclamp, ssat, P1;
+    float v_empty;             // V of a point with an empty list
+    float sign_chain;          // +1 for GBSW (I accumulates 1 - V), -1 for GBMV
+    // Born map / energy
+    double alpha0, alpha1, eta, slope, shift, prefactor;
+};
+
+// Grid geometry of one system, written on the device every evaluation.
+struct SysGeom {
     double origin[3];  // grid minimum corner (brute: molecule centre)
     double max[3];     // last grid point (faithful) / upper edge (tight)
     int n[3];
     int ncell;
     int flags;
-    int total_entries;
-    int total_points;
-    int pad_;
+    int pad_[3];
 };
 
-constexpr int kMaxAtomChunk = 256;   // atoms staged in LDS per chunk (volume pass)
-constexpr int kQBits = 11;           // packed point = (atom << 11) | q, Q <= 2047
-constexpr float kQuantum = 16777216.0f;  // 2^24: chain-rule partials are summed exactly on this grid
-
-// Per-atom float parameters of the volume function (host-precomputed).
-//   GBSW : x = R', y = (R' - w)^2, z = (R' + w)^2, w = 0
-//   GBMV1: x = gamma_j = gamma0 ln(lambda) / R^4
-//   GBMV2: x = C_j, y = C_j - 100 R^2, z = 400 C_j^2
 struct KParams {
-    // sizes
-    int N, Q, NQ;
-    int cell_cap, entry_cap;
-    int grid_kind, vol_kind;
-    int cap;               // atoms kept per cell
-    int n_lebedev;
-    // grid constants (double, computed on the host with the oracle's expressions)
-    double g;              // cell length
-    double inv_g;          // tight grid: 1/g
-    double gpad;           // faithful: 0.03 + sqrt(3)/2 g + buffer; tight: max reach
-    double halfdiag;       // sqrt(3.0)/2.0*g
-    double buffer;
-    double tight_margin;
-    double sw_d;           // GBSW switching half-width (double)
-    // volume constants (float)
-    float sw, k1, k3;      // H = 1/2 + x (k1 - k3 x^2), k1 = 3/(4w), k3 = 1/(4w^3)
-    float beta, lambda, S0, vguard, fclamp, ssat;
-    float v_empty;
-    float sign_chain;      // +1 for GBSW (I accumulates 1 - V), -1 for GBMV
-    // Born map / energy (double)
-    double alpha0, alpha1, eta, slope, shift, prefactor;
-    int orders[2];
-    int i4, i7;            // which integral is order 4 / 7
-    int pair_splits;       // column splits of the pair pass
+    int S, N, NQ;            // systems, atoms, quadrature points (all systems)
+    int cell_total;          // sum of the systems' cell ranges
+    int entry_cap;
+    int n_tiles;             // pair tiles
+    int max_splits;          // pair_part planes
+    int adjoint;             // also compute dE/dR, dE/dq, dE/dtheta
+    int perm_stride;         // atom visiting order of the binning pass (coprime to N)
+    int persist_blocks;      // workgroups of the per-cell passes
+    const SysDesc* sys;
+    SysGeom* geom;
+    int* flags_all;          // OR of every system's flags
+    const int* atom_sys;     // N
+    const int* cell_sys;     // cell_total
     // inputs
-    const double* pos;     // N x 3
-    const double* charge;  // N
-    const double* radius;  // N
-    const double* reach;   // N: tight grid reach R' + w + margin (double)
-    const float4* aparam;  // N
-    const double* quad_off;  // 3Q
-    const double* quad_pw;   // 2Q: w_q / r_q^(order_k - 2)
+    const double* pos;       // N x 3
+    const double* charge;    // N
+    const double* radius;    // N
+    const double* reach;     // N: tight grid reach R' + w + margin
+    const float4* aparam;    // N: GBSW (R', (R'-w)^2, (R'+w)^2, R); GBMV1 (gamma, 0, 0, R);
+                             //    GBMV2 (C, C - 100R^2, 400C^2, R)
+    const double* quad_off;  // 3 x rows (rows = sum of the systems' Q)
+    const double* quad_pw;   // 2 x rows: w_q / r_q^(order_k - 2)
+    const float* quad_r2;    // rows: |offset_q|^2
+    int quad_rows;
+    const int4* pair_tiles;  // (row begin, column begin, column end, split), global atom indices
     // grid
-    Geom* geom;
-    int* cell_count;       // cell_cap
-    int* cell_fill;        // cell_cap
-    int* cell_start;       // cell_cap + 1
-    int* cell_count_eff;   // cell_cap
-    int* cell_atoms_tmp;   // entry_cap
-    int* cell_atoms;       // entry_cap
-    float4* ent_pos;       // entry_cap: atom position relative to the cell corner, .w = atom index bits
-    float4* ent_par;       // entry_cap: per-atom volume parameters (aparam)
+    int* cell_count;         // cell_total
+    int* cell_fill;          // cell_total
+    int* cell_start;         // cell_total + 1
+    int* cell_count_eff;     // cell_total
+    int* cell_atoms_tmp;     // entry_cap
+    int* cell_atoms;         // entry_cap
+    float4* ent_pos;         // entry_cap: position relative to the cell corner, .w = atom index bits
+    float4* ent_par;         // entry_cap: aparam of the atom
     // points
-    int pt_cap;            // point slots per cell
-    int* cell_pt_count;    // cell_cap
-    int* cell_act_count;   // cell_cap: points with a nonzero chain-rule factor
-    int* frec_base;        // cell_cap + 1: first record of the cell's active points (scan of counts)
-    int* worklist;         // cell_cap: cells holding points
-    float4* prec;          // cell_cap x pt_cap: point relative to the cell corner, .w = packed index bits
-    float4* frec0;         // NQ (active points, grouped by cell): copy of prec
-    float4* frec1;         // NQ: GBSW (V, 0, 0, 0) / GBMV1 (dV/dS, V, 0, 0) / GBMV2 (c1, c2, c3, vx)
-    float4* frec2;         // NQ: GBMV2 (vy, vz, V, 0)
-    float4* vrec;          // NQ by (atom, q): (V, owner vector) -- GBSW dV/dp, GBMV owner force
-    const float* quad_r2;  // Q: |offset_q|^2
-    int perm_stride;       // atom visiting order of the binning pass
+    int* cell_pt_count;      // cell_total
+    int* cell_act_count;     // cell_total
+    int* frec_base;          // cell_total + 1: exclusive scan of the point counts
+    int* worklist;           // VOL_KINDS x cell_total
+    int* worklist_sys;       // VOL_KINDS x cell_total: system of each worklist cell
+    float4* prec;            // per system: cell_cap x pt_cap slots
+    float4* frec0;           // NQ: active points, grouped by cell (copy of prec)
+    float4* frec1;           // NQ: GBSW (V, 0, 0, 0) / GBMV1 (dV/dS, V, 0, 0) / GBMV2 (c1, c2, c3, vx)
+    float4* frec2;           // NQ: GBMV2 (vy, vz, V, 0)
+    float4* vrec;            // NQ by pid: (V, owner vector)
+    float* vsig;             // NQ (adjoint, GBMV) by pid: S of the sigmoid, NaN where V has no derivative
     // per atom
-    double* integrals;     // 2 x N
-    double* born;          // N
-    double* dbdi;          // 2 x N
-    double* dedi;          // 2 x N
-    double* pair_part;     // pair_splits x 5 x N
-    double* atom_energy;   // N
-    long long* force_fix;  // 3 x N, scale 2^32
-    double* energy;        // 1
-    int* work;             // work counters [4]
-    int* scan_tmp;         // scan block sums
+    double* integrals;       // 2 x N
+    double* born;            // N
+    double* dbdi;            // 2 x N
+    double* dedi;            // 2 x N
+    double* pair_part;       // max_splits x 6 x N (fx, fy, fz, dE/dB, E, dE/dq)
+    double* atom_energy;     // N
+    double* dedq;            // N (adjoint)
+    long long* dedr_fix;     // N (adjoint): sum_p s fac_p dV_p/dR_j (GBSW: dR'_j), 2^32
+    double* dedr;            // N (adjoint): dE/dR
+    long long* vacc_fix;     // S x VA_COUNT (adjoint), 2^32
+    double* theta_atom;      // TH_COUNT x N (adjoint): per-atom parts of dE/dtheta
+    double* theta;           // S x TH_COUNT (adjoint): dE/dtheta
+    long long* force_fix;    // 3 x N, scale 2^32
+    double* energy;          // S
+    int* work;               // counters [8]
+    int* scan_tmp;           // scan block sums
 };
 
 // Launchers (gbi_kernels.hip).  All enqueue on `stream`.
 // marks: nullptr or kPasses + 1 events recorded before each pass and after the last.
+// kinds: which volume kinds occur in the batch.
 constexpr int kPasses = 8;
-hipError_t launch_evaluate(const KParams& p, unsigned flags, hipStream_t stream, const hipEvent_t* marks);
-hipError_t launch_bbox_only(const KParams& p, hipStream_t stream);
+hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stream, const hipEvent_t* marks);
 hipError_t launch_stats(const KParams& p, double* out4, hipStream_t stream);
 
 }  // namespace gbi

@@ -1,11 +1,13 @@
 // gfx950 kernels of the integral Generalized-Born engine.
 //
-// One evaluation (launch_evaluate) runs, on one stream:
-//   clear -> bbox/grid geometry -> cell lists (count, scan, fill, sort+cap)
-//   -> point binning (count, scan, scatter) -> volume pass (per cell, lanes = points)
-//   -> integrals + Born map (per atom) -> STILL pair pass (row blocks x column splits)
-//   -> per-atom chain (dE/dB -> dE/dI, self energy) -> energy reduction
-//   -> chain-rule force pass (per cell, lanes = list atoms) -> owner-force reduction.
+// One evaluation (launch_evaluate) of a batch of systems runs, on one stream, one launch
+// per pass for all systems together:
+//   clear -> bounding boxes / grid geometry (workgroup per system) -> cell lists (count,
+//   scan, fill, sort+cap) -> point binning -> volume pass (per cell, lanes = points, one
+//   launch per volume kind present) -> integrals + Born map (per atom) -> STILL pair pass
+//   (tiles of 64 rows x a column range inside one system) -> per-atom chain (dE/dB ->
+//   dE/dI, self energy) -> energy per system -> chain-rule force pass (per cell, lanes =
+//   list atoms x points) -> owner-force reduction [-> adjoint reductions].
 //
 // Determinism: every sum is taken in a fixed order, except the per-(cell, atom) force
 // partials, which are quantised to 2^-24 and summed exactly in FP64 before one int64
@@ -16,7 +18,8 @@
 // computeIntegral.cu + compute{GBSW,GBMV1,GBMV2}Volume.cu (volume pass),
 // customGBValuePerParticle.cu (Born map), customGBEnergyN2.cu (pair pass),
 // customGBEnergyPerParticle.cu (dE/dB -> dE/dI), reduceIntegralForce.cu +
-// compute{GBSW,GBMV1,GBMV2}Force.cu (chain-rule pass).
+// compute{GBSW,GBMV1,GBMV2}Force.cu (chain-rule pass).  The adjoint passes have no
+// reference counterpart.
 #include "gbi_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -34,8 +37,6 @@ constexpr int kWave = 64;
 constexpr int kScanThreads = 1024;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kScanThreads * kScanItems;
-constexpr int kFatal = FLAG_CELL_OVERFLOW | FLAG_ENTRY_OVERFLOW | FLAG_GRID_TOO_LARGE |
-                       FLAG_NONFINITE_POS | FLAG_POINT_OVERFLOW;
 constexpr double kFix32 = 4294967296.0;  // 2^32
 
 __device__ inline double wave_sum(double v) {
@@ -58,29 +59,44 @@ __device__ inline int wave_scan_incl(int v, int lane) {
     return v;
 }
 
-__device__ inline bool fatal(const Geom* g) { return (g->flags & kFatal) != 0; }
+// Any system of the batch hit a fatal condition: the host grows its buffers and redoes
+// the whole evaluation, so every later pass stops early.
+__device__ inline bool batch_fatal(const KParams& p) { return (*p.flags_all & kFatalFlags) != 0; }
+
+__device__ inline void raise_flag(const KParams& p, int s, int flag) {
+    atomicOr(&p.geom[s].flags, flag);
+    atomicOr(p.flags_all, flag);
+}
 
 __device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// Exact sum of quantised partials: round x to the 2^-24 grid; the double holds an integer.
+__device__ inline double quant(float x) { return (double)rintf(x * kQuantum); }
+
+// 2^-24 units (exact integers in a double) -> int64 in 2^-32 units.
+__device__ inline void add_fix(long long* dst, double units24) {
+    atomicAdd((unsigned long long*)dst, (unsigned long long)(long long)(units24 * 256.0));
+}
 
 // ---------------------------------------------------------------------------
 // Grid geometry and membership (FP64, no contraction: bitwise the oracle's arithmetic)
 // ---------------------------------------------------------------------------
 
-// Cell of a point; -1 outside.  Faithful: CustomGBIntegral.cpp:310-324.
-__device__ inline int point_cell(const KParams& p, const Geom& G, double px, double py, double pz) {
+// Local cell of a point; -1 outside.  Faithful: CustomGBIntegral.cpp:310-324.
+__device__ inline int point_cell(const SysDesc& sd, const SysGeom& G, double px, double py, double pz) {
 #pragma clang fp contract(off)
-    if (p.grid_kind == GRID_BRUTE) return 0;
+    if (sd.grid_kind == GRID_BRUTE) return 0;
     const double pt[3] = {px, py, pz};
     int idx[3];
-    if (p.grid_kind == GRID_FAITHFUL) {
+    if (sd.grid_kind == GRID_FAITHFUL) {
         for (int d = 0; d < 3; ++d) {
             if (pt[d] < G.origin[d] || pt[d] > G.max[d]) return -1;
-            idx[d] = (int)floor((pt[d] - G.origin[d]) / p.g);
+            idx[d] = (int)floor((pt[d] - G.origin[d]) / sd.g);
         }
     } else {
         for (int d = 0; d < 3; ++d) {
-            double t = (pt[d] - G.origin[d]) * p.inv_g;
+            double t = (pt[d] - G.origin[d]) * sd.inv_g;
             if (!(t >= 0.0) || t >= (double)G.n[d]) return -1;
             idx[d] = (int)floor(t);
             if (idx[d] >= G.n[d]) return -1;
@@ -89,9 +105,9 @@ __device__ inline int point_cell(const KParams& p, const Geom& G, double px, dou
     return (idx[0] * G.n[1] + idx[1]) * G.n[2] + idx[2];
 }
 
-__device__ inline void cell_corner(const KParams& p, const Geom& G, int c, double* corner) {
+__device__ inline void cell_corner(const SysDesc& sd, const SysGeom& G, int c, double* corner) {
 #pragma clang fp contract(off)
-    if (p.grid_kind == GRID_BRUTE) {
+    if (sd.grid_kind == GRID_BRUTE) {
         corner[0] = G.origin[0];
         corner[1] = G.origin[1];
         corner[2] = G.origin[2];
@@ -102,35 +118,43 @@ __device__ inline void cell_corner(const KParams& p, const Geom& G, int c, doubl
     const int rem = c - ix * nyz;
     const int iy = rem / G.n[2];
     const int iz = rem - iy * G.n[2];
-    corner[0] = G.origin[0] + ix * p.g;
-    corner[1] = G.origin[1] + iy * p.g;
-    corner[2] = G.origin[2] + iz * p.g;
+    corner[0] = G.origin[0] + ix * sd.g;
+    corner[1] = G.origin[1] + iy * sd.g;
+    corner[2] = G.origin[2] + iz * sd.g;
 }
 
 // ---------------------------------------------------------------------------
-// Small kernels
+// Clear and geometry
 // ---------------------------------------------------------------------------
 
 __global__ void k_clear(KParams p) {
     const int stride = gridDim.x * blockDim.x;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < p.cell_cap; i += stride) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < p.cell_total; i += stride) {
         p.cell_count[i] = 0;
         p.cell_fill[i] = 0;
         p.cell_pt_count[i] = 0;
     }
-    if (blockIdx.x == 0 && threadIdx.x < 4) p.work[threadIdx.x] = 0;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < 8) p.work[threadIdx.x] = 0;
+        if (threadIdx.x == 0) *p.flags_all = 0;
+        if (p.adjoint)
+            for (int k = threadIdx.x; k < p.S * VA_COUNT; k += blockDim.x) p.vacc_fix[k] = 0;
+    }
 }
 
-__device__ void bbox_to_geom(const KParams& p, double (*smin)[16], double (*smax)[16], const int* sbad, int nw);
+__device__ void bbox_to_geom(const KParams& p, int s, double (*smin)[16], double (*smax)[16], const int* sbad,
+                             int nw);
 
-// Bounding box -> grid geometry (one workgroup).
+// Bounding box -> grid geometry (one workgroup per system).
 __global__ void __launch_bounds__(1024) k_bbox(KParams p) {
     __shared__ double smin[3][16], smax[3][16];
     __shared__ int sbad[16];
+    const int s = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const int ab = p.sys[s].atom_begin, ae = p.sys[s].atom_end;
     double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
     int bad = 0;
-    for (int a = tid; a < p.N; a += blockDim.x) {
+    for (int a = ab + tid; a < ae; a += blockDim.x) {
         for (int d = 0; d < 3; ++d) {
             const double v = p.pos[3 * a + d];
             if (!isfinite(v)) bad = 1;
@@ -152,12 +176,14 @@ __global__ void __launch_bounds__(1024) k_bbox(KParams p) {
     }
     __syncthreads();
     if (tid != 0) return;
-    bbox_to_geom(p, smin, smax, sbad, (blockDim.x + kWave - 1) / kWave);
+    bbox_to_geom(p, s, smin, smax, sbad, (blockDim.x + kWave - 1) / kWave);
 }
 
 // Grid geometry from the per-wave bounding boxes (thread 0 of k_bbox).
-__device__ void bbox_to_geom(const KParams& p, double (*smin)[16], double (*smax)[16], const int* sbad, int nw) {
+__device__ void bbox_to_geom(const KParams& p, int s, double (*smin)[16], double (*smax)[16], const int* sbad,
+                             int nw) {
 #pragma clang fp contract(off)
+    const SysDesc& sd = p.sys[s];
     double mn[3], mx[3];
     int isbad = 0;
     for (int d = 0; d < 3; ++d) {
@@ -169,11 +195,9 @@ __device__ void bbox_to_geom(const KParams& p, double (*smin)[16], double (*smax
         }
     }
     for (int w = 0; w < nw; ++w) isbad |= sbad[w];
-    Geom G;
+    SysGeom G;
     G.flags = 0;
-    G.total_entries = 0;
-    G.total_points = 0;
-    G.pad_ = 0;
+    G.pad_[0] = G.pad_[1] = G.pad_[2] = 0;
     long long ncell = 1;
     if (isbad) {
         G.flags |= FLAG_NONFINITE_POS;
@@ -183,46 +207,53 @@ __device__ void bbox_to_geom(const KParams& p, double (*smin)[16], double (*smax
             G.max[d] = 0.0;
             G.n[d] = 0;
         }
-    } else if (p.grid_kind == GRID_BRUTE) {
+    } else if (sd.grid_kind == GRID_BRUTE) {
         for (int d = 0; d < 3; ++d) {
             G.origin[d] = 0.5 * (mn[d] + mx[d]);
             G.max[d] = DBL_MAX;
             G.n[d] = 1;
         }
-    } else if (p.grid_kind == GRID_FAITHFUL) {
+    } else if (sd.grid_kind == GRID_FAITHFUL) {
         // CustomGBIntegral.cpp:173-187
         for (int d = 0; d < 3; ++d) {
-            const double lo = mn[d] - p.gpad;
-            const double hi = mx[d] + p.gpad;
+            const double lo = mn[d] - sd.gpad;
+            const double hi = mx[d] + sd.gpad;
             const double len = hi - lo;
             if (len > 1000.0) G.flags |= FLAG_GRID_TOO_LARGE;
-            const int n = len > 1000.0 ? 1 : (int)ceil(len / p.g) + 1;
+            const int n = len > 1000.0 ? 1 : (int)ceil(len / sd.g) + 1;
             G.n[d] = n;
             G.origin[d] = lo;
-            G.max[d] = lo + (n - 1) * p.g;
+            G.max[d] = lo + (n - 1) * sd.g;
             ncell *= n;
         }
     } else {
         for (int d = 0; d < 3; ++d) {
-            const double lo = mn[d] - p.gpad;
-            const double hi = mx[d] + p.gpad;
+            const double lo = mn[d] - sd.gpad;
+            const double hi = mx[d] + sd.gpad;
             const double len = hi - lo;
             if (len > 1000.0) G.flags |= FLAG_GRID_TOO_LARGE;
-            const int n = len > 1000.0 ? 1 : (int)floor(len * p.inv_g) + 1;
+            const int n = len > 1000.0 ? 1 : (int)floor(len * sd.inv_g) + 1;
             G.n[d] = n;
             G.origin[d] = lo;
-            G.max[d] = lo + n * p.g;
+            G.max[d] = lo + n * sd.g;
             ncell *= n;
         }
     }
-    if (ncell > p.cell_cap) {
+    if (ncell > sd.cell_cap) {
         G.flags |= FLAG_CELL_OVERFLOW;
         ncell = 0;
     }
-    if (G.flags & kFatal) ncell = 0;
+    if (G.flags & kFatalFlags) ncell = 0;
     G.ncell = (int)ncell;
-    *p.geom = G;
+    p.geom[s] = G;
+    if (G.flags) atomicOr(p.flags_all, G.flags);
+    // the brute cell holds every atom of the system (its count is known here)
+    if (sd.grid_kind == GRID_BRUTE && ncell == 1) p.cell_count[sd.cell_base] = sd.atom_end - sd.atom_begin;
 }
+
+// ---------------------------------------------------------------------------
+// Cell lists
+// ---------------------------------------------------------------------------
 
 // Candidate cells of atom j (before the membership test), clamped to the grid.
 struct CandBox {
@@ -230,16 +261,16 @@ struct CandBox {
     int count;
 };
 
-__device__ inline CandBox candidate_box(const KParams& p, const Geom& G, const double* x, double pad) {
+__device__ inline CandBox candidate_box(const SysDesc& sd, const SysGeom& G, const double* x, double pad) {
 #pragma clang fp contract(off)
     CandBox B;
     for (int d = 0; d < 3; ++d) {
-        if (p.grid_kind == GRID_FAITHFUL) {
-            B.b[d] = (int)floor((x[d] - pad - G.origin[d]) / p.g);
-            B.e[d] = (int)ceil((x[d] + pad - G.origin[d]) / p.g);
+        if (sd.grid_kind == GRID_FAITHFUL) {
+            B.b[d] = (int)floor((x[d] - pad - G.origin[d]) / sd.g);
+            B.e[d] = (int)ceil((x[d] + pad - G.origin[d]) / sd.g);
         } else {
-            B.b[d] = (int)floor((x[d] - pad - G.origin[d]) * p.inv_g);
-            B.e[d] = (int)floor((x[d] + pad - G.origin[d]) * p.inv_g);
+            B.b[d] = (int)floor((x[d] - pad - G.origin[d]) * sd.inv_g);
+            B.e[d] = (int)floor((x[d] + pad - G.origin[d]) * sd.inv_g);
         }
         if (B.b[d] < 0) B.b[d] = 0;
         if (B.e[d] > G.n[d] - 1) B.e[d] = G.n[d] - 1;
@@ -252,11 +283,11 @@ __device__ inline CandBox candidate_box(const KParams& p, const Geom& G, const d
 // Membership of atom j in cell (i, jj, k).
 // Faithful: corner within pad_j = R_j + sqrt3/2 g + buffer (CustomGBIntegral.cpp:210-223).
 // Tight: distance from x_j to the cell box below reach_j = R'_j + w + margin.
-__device__ inline bool cell_member(const KParams& p, const Geom& G, const double* x, double pad, int i, int jj,
+__device__ inline bool cell_member(const SysDesc& sd, const SysGeom& G, const double* x, double pad, int i, int jj,
                                    int k) {
 #pragma clang fp contract(off)
-    const double g = p.g;
-    if (p.grid_kind == GRID_FAITHFUL) {
+    const double g = sd.g;
+    if (sd.grid_kind == GRID_FAITHFUL) {
         const double dx = (G.origin[0] + i * g) - x[0];
         const double dy = (G.origin[1] + jj * g) - x[1];
         const double dz = (G.origin[2] + k * g) - x[2];
@@ -269,46 +300,57 @@ __device__ inline bool cell_member(const KParams& p, const Geom& G, const double
     return dx * dx + dy * dy + dz * dz < pad * pad;
 }
 
-__device__ inline double atom_pad(const KParams& p, int j) {
+__device__ inline double atom_pad(const KParams& p, const SysDesc& sd, int j) {
 #pragma clang fp contract(off)
-    return p.grid_kind == GRID_FAITHFUL ? (p.radius[j] + p.halfdiag) + p.buffer : p.reach[j];
+    return sd.grid_kind == GRID_FAITHFUL ? (p.radius[j] + sd.halfdiag) + sd.buffer : p.reach[j];
 }
 
 // One workgroup per atom, lanes over its candidate cells: count the members of each cell.
 __global__ void __launch_bounds__(256) k_grid_count(KParams p) {
-    const Geom G = *p.geom;
-    if (G.flags & kFatal) return;
+    if (batch_fatal(p)) return;
     const int j = blockIdx.x;
+    const int s = p.atom_sys[j];
+    const SysDesc& sd = p.sys[s];
+    if (sd.grid_kind == GRID_BRUTE) return;  // counted by k_bbox
+    const SysGeom G = p.geom[s];
     const double x[3] = {p.pos[3 * j], p.pos[3 * j + 1], p.pos[3 * j + 2]};
-    const double pad = atom_pad(p, j);
-    const CandBox B = candidate_box(p, G, x, pad);
+    const double pad = atom_pad(p, sd, j);
+    const CandBox B = candidate_box(sd, G, x, pad);
     const int ny = B.e[1] - B.b[1] + 1, nz = B.e[2] - B.b[2] + 1;
     for (int t = threadIdx.x; t < B.count; t += blockDim.x) {
         const int i = B.b[0] + t / (ny * nz);
         const int r = t - (i - B.b[0]) * (ny * nz);
         const int jj = B.b[1] + r / nz;
         const int k = B.b[2] + (r - (jj - B.b[1]) * nz);
-        if (cell_member(p, G, x, pad, i, jj, k)) atomicAdd(&p.cell_count[(i * G.n[1] + jj) * G.n[2] + k], 1);
+        if (cell_member(sd, G, x, pad, i, jj, k))
+            atomicAdd(&p.cell_count[sd.cell_base + (i * G.n[1] + jj) * G.n[2] + k], 1);
     }
 }
 
 __global__ void __launch_bounds__(256) k_grid_fill(KParams p) {
-    const Geom G = *p.geom;
-    if (G.flags & kFatal) return;
+    if (batch_fatal(p)) return;
     const int j = blockIdx.x;
+    const int s = p.atom_sys[j];
+    const SysDesc& sd = p.sys[s];
+    if (sd.grid_kind == GRID_BRUTE) {
+        // index order, no atomics
+        if (threadIdx.x == 0) p.cell_atoms_tmp[p.cell_start[sd.cell_base] + (j - sd.atom_begin)] = j;
+        return;
+    }
+    const SysGeom G = p.geom[s];
     const double x[3] = {p.pos[3 * j], p.pos[3 * j + 1], p.pos[3 * j + 2]};
-    const double pad = atom_pad(p, j);
-    const CandBox B = candidate_box(p, G, x, pad);
+    const double pad = atom_pad(p, sd, j);
+    const CandBox B = candidate_box(sd, G, x, pad);
     const int ny = B.e[1] - B.b[1] + 1, nz = B.e[2] - B.b[2] + 1;
     for (int t = threadIdx.x; t < B.count; t += blockDim.x) {
         const int i = B.b[0] + t / (ny * nz);
         const int r = t - (i - B.b[0]) * (ny * nz);
         const int jj = B.b[1] + r / nz;
         const int k = B.b[2] + (r - (jj - B.b[1]) * nz);
-        if (cell_member(p, G, x, pad, i, jj, k)) {
-            const int c = (i * G.n[1] + jj) * G.n[2] + k;
-            const int s = atomicAdd(&p.cell_fill[c], 1);
-            const int idx = p.cell_start[c] + s;
+        if (cell_member(sd, G, x, pad, i, jj, k)) {
+            const int c = sd.cell_base + (i * G.n[1] + jj) * G.n[2] + k;
+            const int slot = atomicAdd(&p.cell_fill[c], 1);
+            const int idx = p.cell_start[c] + slot;
             if (idx < p.entry_cap) p.cell_atoms_tmp[idx] = j;
         }
     }
@@ -323,22 +365,28 @@ __device__ inline void write_entry(const KParams& p, int e, int j, const double*
 // One wave per cell: order the cell's atoms by index (rank sort), keep the first `cap`
 // (the reference appends in ascending atom order and stops at the cap,
 // CustomGBIntegral.cpp:219-222) and write their corner-relative entries.
+constexpr int kSortLds = 256;  // cells with more raw members sort in global memory
+
 __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
-    __shared__ int s_in[1024], s_out[1024];
-    const Geom G = *p.geom;
-    if (G.flags & kFatal) return;
+    __shared__ int s_in[kSortLds], s_out[kSortLds];
+    if (batch_fatal(p)) return;
     const int lane = threadIdx.x;
-    for (int c = blockIdx.x; c < G.ncell; c += gridDim.x) {
+    for (int c = blockIdx.x; c < p.cell_total; c += gridDim.x) {
         const int n = p.cell_count[c];
         if (n == 0) {
             if (lane == 0) p.cell_count_eff[c] = 0;
             continue;
         }
+        const int s = p.cell_sys[c];
+        const SysDesc& sd = p.sys[s];
+        const SysGeom G = p.geom[s];
         const int base = p.cell_start[c];
-        const int eff = n < p.cap ? n : p.cap;
+        const int eff = n < sd.cap ? n : sd.cap;
         double corner[3];
-        cell_corner(p, G, c, corner);
-        if (n <= 1024) {
+        cell_corner(sd, G, c - sd.cell_base, corner);
+        if (sd.grid_kind == GRID_BRUTE) {
+            for (int k = lane; k < eff; k += kWave) write_entry(p, base + k, p.cell_atoms_tmp[base + k], corner);
+        } else if (n <= kSortLds) {
             for (int k = lane; k < n; k += kWave) s_in[k] = p.cell_atoms_tmp[base + k];
             __syncthreads();
             for (int k = lane; k < n; k += kWave) {
@@ -348,7 +396,7 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
                 s_out[r] = a;
             }
             __syncthreads();
-            for (int s = lane; s < eff; s += kWave) write_entry(p, base + s, s_out[s], corner);
+            for (int k = lane; k < eff; k += kWave) write_entry(p, base + k, s_out[k], corner);
         } else {
             for (int k = lane; k < n; k += kWave) {
                 const int a = p.cell_atoms_tmp[base + k];
@@ -358,24 +406,10 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
             }
             __threadfence_block();
             __syncthreads();
-            for (int s = lane; s < eff; s += kWave) write_entry(p, base + s, p.cell_atoms[base + s], corner);
+            for (int k = lane; k < eff; k += kWave) write_entry(p, base + k, p.cell_atoms[base + k], corner);
         }
         if (lane == 0) p.cell_count_eff[c] = eff;
         __syncthreads();
-    }
-}
-
-__global__ void k_grid_brute(KParams p) {
-    const Geom G = *p.geom;
-    if (G.flags & kFatal) return;
-    const int stride = gridDim.x * blockDim.x;
-    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < p.N; a += stride) write_entry(p, a, a, G.origin);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        p.cell_count[0] = p.N;
-        p.cell_count_eff[0] = p.N;
-        p.cell_start[0] = 0;
-        p.cell_start[1] = p.N;
-        p.geom->total_entries = p.N;
     }
 }
 
@@ -413,8 +447,8 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int* in, int*
     }
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_scan_top(int* block_sums, int nblocks, int* out, int n,
-                                                           int* total_dst, int limit, int* flags, int flag) {
+__global__ void __launch_bounds__(kScanThreads) k_scan_top(int* block_sums, int nblocks, int* out, int n, int limit,
+                                                           int* flags, int flag) {
     __shared__ int s_wave[kScanThreads / kWave];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     int v[kScanItems];
@@ -434,7 +468,6 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_top(int* block_sums, int 
         if (lane < nw) s_wave[lane] = wi - w;
         if (lane == nw - 1) {
             out[n] = wi;
-            if (total_dst) *total_dst = wi;
             if (flags && wi > limit) atomicOr(flags, flag);
         }
     }
@@ -456,13 +489,13 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_add(int* out, int n, cons
     }
 }
 
-hipError_t scan_exclusive(const int* in, int* out, int n, int* tmp, int* total_dst, int limit, int* flags,
-                          int flag, hipStream_t stream) {
+// out[0..n] = exclusive scan of in[0..n-1] (out[n] = total); flags |= flag when total > limit.
+hipError_t scan_exclusive(const int* in, int* out, int n, int* tmp, int limit, int* flags, int flag,
+                          hipStream_t stream) {
     const int nblocks = (n + kScanTile - 1) / kScanTile;
-    if (nblocks > kScanTile) return hipErrorInvalidValue;
+    if (nblocks > kScanTile || nblocks < 1) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_scan_tiles, dim3(nblocks), dim3(kScanThreads), 0, stream, in, out, n, tmp);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, stream, tmp, nblocks, out, n, total_dst, limit,
-                       flags, flag);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, stream, tmp, nblocks, out, n, limit, flags, flag);
     hipLaunchKernelGGL(k_scan_add, dim3(nblocks), dim3(kScanThreads), 0, stream, out, n, (const int*)tmp);
     return hipGetLastError();
 }
@@ -471,61 +504,89 @@ hipError_t scan_exclusive(const int* in, int* out, int n, int* tmp, int* total_d
 // Point binning
 // ---------------------------------------------------------------------------
 
-// One workgroup per atom (visited with a stride coprime to N so concurrent workgroups hit
-// distant cells): the cell of every quadrature point x_i + off_q in FP64 (the reference's
-// r_q = x_i + quad_q, GBSWIntegral.cpp:132-133), and its corner-relative FP32 record in the
-// cell's slots.  Points outside the grid, in an empty cell, or (GBSW) inside the owner's own
-// burial radius get their volume directly.
-__device__ inline void quad_point(const KParams& p, double xi, double yi, double zi, int q, double* pt) {
+// r_q = x_i + off_q in FP64 (the reference's point, GBSWIntegral.cpp:132-133).
+__device__ inline void quad_point(const KParams& p, double xi, double yi, double zi, int row, double* pt) {
 #pragma clang fp contract(off)
-    pt[0] = xi + p.quad_off[3 * q];
-    pt[1] = yi + p.quad_off[3 * q + 1];
-    pt[2] = zi + p.quad_off[3 * q + 2];
+    pt[0] = xi + p.quad_off[3 * row];
+    pt[1] = yi + p.quad_off[3 * row + 1];
+    pt[2] = zi + p.quad_off[3 * row + 2];
 }
 
+// One workgroup per atom (visited with a stride coprime to N so concurrent workgroups hit
+// distant cells): the cell of every quadrature point and its corner-relative FP32 record in
+// the cell's slots.  Points outside the grid, in an empty cell, or (GBSW) inside the
+// owner's own burial radius get their volume directly.
 __global__ void __launch_bounds__(256) k_point_bin(KParams p) {
-    const Geom G = *p.geom;
-    if (G.flags & kFatal) return;
+    if (batch_fatal(p)) return;
     const int i = (int)(((long long)blockIdx.x * p.perm_stride) % p.N);
+    const int s = p.atom_sys[i];
+    const SysDesc& sd = p.sys[s];
+    const SysGeom G = p.geom[s];
+    const int Q = sd.Q, qb = sd.quad_base, cb = sd.cell_base, pt_cap = sd.pt_cap;
+    const long long prec_base = sd.prec_base;
+    const int pbase = sd.pt_base + (i - sd.atom_begin) * Q;
+    // only lists that surely hold the owner (exact or brute) may skip it; the reference
+    // GBMV-form grid is under-padded and can miss it (GBSWIntegral.cpp:59-60)
+    const bool shortcut = sd.vol_kind == VOL_GBSW && sd.grid_kind != GRID_FAITHFUL;
+    const float v_empty = sd.v_empty;
+    const float sig_empty = sd.vol_kind == VOL_GBMV1 ? 0.0f : __int_as_float(0x7fc00000);
     const double xi = p.pos[3 * i], yi = p.pos[3 * i + 1], zi = p.pos[3 * i + 2];
     const float own_lo2 = p.aparam[i].y;
-    for (int q = threadIdx.x; q < p.Q; q += blockDim.x) {
-        const int pid = i * p.Q + q;
+    for (int q = threadIdx.x; q < Q; q += blockDim.x) {
+        const int row = qb + q;
         int c = -1;
         double pt[3] = {0.0, 0.0, 0.0};
-        // only lists that surely hold the owner (exact or brute) may skip it; the reference
-        // GBMV-form grid is under-padded and can miss it (GBSWIntegral.cpp:59-60)
-        const bool owner_buried =
-            p.vol_kind == VOL_GBSW && p.grid_kind != GRID_FAITHFUL && p.quad_r2[q] <= own_lo2;
+        const bool owner_buried = shortcut && p.quad_r2[row] <= own_lo2;
         if (!owner_buried) {
-            quad_point(p, xi, yi, zi, q, pt);
-            c = point_cell(p, G, pt[0], pt[1], pt[2]);
-            if (c >= 0 && p.cell_count_eff[c] == 0) c = -1;
+            quad_point(p, xi, yi, zi, row, pt);
+            c = point_cell(sd, G, pt[0], pt[1], pt[2]);
+            if (c >= 0 && p.cell_count_eff[cb + c] == 0) c = -1;
         }
         if (c >= 0) {
-            const int s = atomicAdd(&p.cell_pt_count[c], 1);
-            if (s < p.pt_cap) {
+            const int slot = atomicAdd(&p.cell_pt_count[cb + c], 1);
+            if (slot < pt_cap) {
                 double corner[3];
-                cell_corner(p, G, c, corner);
-                p.prec[(size_t)c * p.pt_cap + s] =
+                cell_corner(sd, G, c, corner);
+                p.prec[prec_base + (long long)c * pt_cap + slot] =
                     make_float4((float)(pt[0] - corner[0]), (float)(pt[1] - corner[1]), (float)(pt[2] - corner[2]),
                                 __uint_as_float(((uint32_t)i << kQBits) | (uint32_t)q));
             } else {
-                atomicOr(&p.geom->flags, FLAG_POINT_OVERFLOW);
+                raise_flag(p, s, FLAG_POINT_OVERFLOW);
             }
         } else {
-            p.vrec[pid] = make_float4(owner_buried ? 0.0f : p.v_empty, 0.f, 0.f, 0.f);
+            p.vrec[pbase + q] = make_float4(owner_buried ? 0.0f : v_empty, 0.f, 0.f, 0.f);
+            if (p.adjoint) p.vsig[pbase + q] = owner_buried ? __int_as_float(0x7fc00000) : sig_empty;
         }
     }
 }
 
-// Cells holding points (order irrelevant: every later sum is order independent).
+// Cells holding points, one list per volume kind (order irrelevant: every later sum is
+// order independent).  One atomic per wave and kind.
 __global__ void k_worklist(KParams p) {
-    const Geom G = *p.geom;
-    if (G.flags & kFatal) return;
+    if (batch_fatal(p)) return;
+    const int lane = threadIdx.x & (kWave - 1);
     const int stride = gridDim.x * blockDim.x;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < G.ncell; c += stride)
-        if (p.cell_pt_count[c] > 0) p.worklist[atomicAdd(&p.work[2], 1)] = c;
+    for (int base = blockIdx.x * blockDim.x; base < p.cell_total; base += stride) {
+        const int c = base + (int)threadIdx.x;
+        int vk = -1, s = 0;
+        if (c < p.cell_total && p.cell_pt_count[c] > 0) {
+            s = p.cell_sys[c];
+            vk = p.sys[s].vol_kind;
+        }
+        for (int k = 0; k < VOL_KINDS; ++k) {
+            const unsigned long long m = __ballot(vk == k);
+            if (m == 0ull) continue;
+            const int leader = __ffsll((long long)m) - 1;
+            int off = 0;
+            if (lane == leader) off = atomicAdd(&p.work[k], __popcll(m));
+            off = __shfl(off, leader, kWave);
+            if (vk == k) {
+                const int slot = k * p.cell_total + off + __popcll(m & ((1ull << lane) - 1ull));
+                p.worklist[slot] = c;
+                p.worklist_sys[slot] = s;
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -542,18 +603,24 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
     __shared__ float s_x[kMaxAtomChunk], s_y[kMaxAtomChunk], s_z[kMaxAtomChunk];
     __shared__ float s_a[kMaxAtomChunk], s_b[kMaxAtomChunk], s_c[kMaxAtomChunk];
     __shared__ int s_j[kMaxAtomChunk];
-    const Geom G = *p.geom;
-    if (G.flags & kFatal) return;
+    if (batch_fatal(p)) return;
     const int lane = threadIdx.x;
-    const int nwork = p.work[2];
+    const int nwork = p.work[VK];
+    const int* worklist = p.worklist + VK * p.cell_total;
+    const int* worklist_sys = p.worklist_sys + VK * p.cell_total;
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-        const int c = p.worklist[w];
-        const int np = min(p.cell_pt_count[c], p.pt_cap);
+        const int c = worklist[w];
+        const SysDesc& sd = p.sys[worklist_sys[w]];
+        const int pt_cap = sd.pt_cap;
+        const int np = min(p.cell_pt_count[c], pt_cap);
         const int na = p.cell_count_eff[c];
-        const size_t ps = (size_t)c * p.pt_cap;
+        const long long ps = sd.prec_base + (long long)(c - sd.cell_base) * pt_cap;
         const int as = p.cell_start[c];
         const bool single = na <= kMaxAtomChunk;
         const int fb = p.frec_base[c];  // exclusive scan of the point counts
+        const int pt_base = sd.pt_base, abeg = sd.atom_begin, Q = sd.Q;
+        const float k1 = sd.k1, k3 = sd.k3, fclamp = sd.fclamp, S0 = sd.S0, vguard = sd.vguard, ssat = sd.ssat;
+        const float beta = sd.beta, lambda = sd.lambda;
         auto stage = [&](int ab, int cnt) {
             for (int k = lane; k < cnt; k += kWave) {
                 const float4 e = p.ent_pos[as + ab + k];
@@ -601,14 +668,14 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                             } else {
                                 const float d = fast_sqrt(d2);
                                 const float x = d - s_a[m];
-                                const float h = fmaf(x, fmaf(-p.k3 * x, x, p.k1), 0.5f);
+                                const float h = fmaf(x, fmaf(-k3 * x, x, k1), 0.5f);
                                 if (h <= 0.f) {
                                     done = true;
                                     V = 0.f;
                                 } else {
                                     V *= h;
                                     if (s_j[m] != i) {
-                                        const float f = fmaf(-3.f * p.k3 * x, x, p.k1) * fast_rcp(h) * fast_rcp(d);
+                                        const float f = fmaf(-3.f * k3 * x, x, k1) * fast_rcp(h) * fast_rcp(d);
                                         lx = fmaf(f, dx, lx);
                                         ly = fmaf(f, dy, ly);
                                         lz = fmaf(f, dz, lz);
@@ -624,7 +691,7 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                             const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                             const float t = fmaf(100.f, d2, s_b[m]);
                             const float ct = s_a[m] * fast_rcp(t);
-                            const float F = fminf(ct * ct, p.fclamp);
+                            const float F = fminf(ct * ct, fclamp);
                             s1 += F;
                             s2 = fmaf(d2, F * F, s2);
                             vx = fmaf(F, dx, vx);
@@ -645,7 +712,7 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
             bool active = false;
             float4 r1 = make_float4(0.f, 0.f, 0.f, 0.f), r2 = make_float4(0.f, 0.f, 0.f, 0.f);
             if (valid) {
-                const int pid = i * p.Q + q;
+                const int pid = pt_base + (i - abeg) * Q + q;
                 if (VK == VOL_GBSW) {
                     active = V > 0.f && V < 1.f;
                     const float gs = active ? -V : 0.f;  // dV/dp = -V sum_{j != owner} (H'/H) (x_j - p)/d
@@ -653,28 +720,30 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                     r1.x = V;
                 } else if (VK == VOL_GBMV2) {
                     const float vsq = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
-                    const float S = p.S0 * s1 * s2 / vsq;
-                    const bool sat = !(vsq >= p.vguard) || !(S <= p.ssat);
+                    const float S = S0 * s1 * s2 / vsq;
+                    const bool sat = !(vsq >= vguard) || !(S <= ssat);
                     float Vv = 1.f;
                     if (!sat) {
-                        const float pre = __expf(p.beta * (S - p.lambda));
+                        const float pre = __expf(beta * (S - lambda));
                         const float opre = 1.f + pre;
                         Vv = 1.f / opre;
-                        const float kk = -p.beta * pre / (opre * opre) * p.S0;
+                        const float kk = -beta * pre / (opre * opre) * S0;
                         const float ivsq = 1.f / vsq;
                         r1 = make_float4(kk * s2 * ivsq, 2.f * kk * s1 * ivsq, -2.f * kk * s1 * s2 * ivsq * ivsq, vx);
                         r2 = make_float4(vy, vz, Vv, 0.f);
                         active = true;
                     }
                     p.vrec[pid] = make_float4(Vv, 0.f, 0.f, 0.f);  // the force pass fills the owner force
+                    if (p.adjoint) p.vsig[pid] = sat ? __int_as_float(0x7fc00000) : S;
                 } else {
-                    const float pre = __expf(p.beta * (s1 - p.lambda));
+                    const float pre = __expf(beta * (s1 - lambda));
                     const float opre = 1.f + pre;
                     const float Vv = 1.f / opre;
                     p.vrec[pid] = make_float4(Vv, 0.f, 0.f, 0.f);
-                    r1.x = -p.beta * pre / (opre * opre);
+                    r1.x = -beta * pre / (opre * opre);
                     r1.y = Vv;
                     active = r1.x != 0.f;
+                    if (p.adjoint) p.vsig[pid] = s1;
                 }
             }
             const unsigned long long mask = __ballot(active);
@@ -699,57 +768,64 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
 // ---------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(64) k_integrals(KParams p) {
-    const Geom G = *p.geom;
-    if (G.flags & kFatal) return;
+    if (batch_fatal(p)) return;
     const int i = blockIdx.x, lane = threadIdx.x;
+    const int s = p.atom_sys[i];
+    const SysDesc& sd = p.sys[s];
+    const int Q = sd.Q, qb = sd.quad_base;
+    const int pbase = sd.pt_base + (i - sd.atom_begin) * Q;
     double a0 = 0.0, a1 = 0.0;
-    const bool gbsw = p.vol_kind == VOL_GBSW;
-    for (int q = lane; q < p.Q; q += kWave) {
-        const double v = (double)p.vrec[i * p.Q + q].x;
+    const bool gbsw = sd.vol_kind == VOL_GBSW;
+    for (int q = lane; q < Q; q += kWave) {
+        const double v = (double)p.vrec[pbase + q].x;
         const double f = gbsw ? 1.0 - v : v;
-        a0 += p.quad_pw[q] * f;
-        a1 += p.quad_pw[p.Q + q] * f;
+        a0 += p.quad_pw[qb + q] * f;
+        a1 += p.quad_pw[p.quad_rows + qb + q] * f;
     }
     a0 = wave_sum(a0);
     a1 = wave_sum(a1);
     if (lane != 0) return;
     p.integrals[i] = a0;
     p.integrals[p.N + i] = a1;
-    const double I4 = p.i4 == 0 ? a0 : a1;
-    const double I7 = p.i7 == 0 ? a0 : a1;
-    const double eta4 = p.eta * p.eta * p.eta * p.eta;
+    const double I4 = sd.i4 == 0 ? a0 : a1;
+    const double I7 = sd.i7 == 0 ? a0 : a1;
+    const double eta = sd.eta;
+    const double eta4 = eta * eta * eta * eta;
     const double T = 1.0 / (4.0 * eta4) - I7;
-    const double D = p.alpha0 * (1.0 / p.eta - I4) + p.alpha1 * pow(T, 0.25);
-    const double B = p.slope / D + p.shift;
+    const double D = sd.alpha0 * (1.0 / eta - I4) + sd.alpha1 * pow(T, 0.25);
+    const double B = sd.slope / D + sd.shift;
     p.born[i] = B;
-    p.dbdi[p.i4 * p.N + i] = p.slope * p.alpha0 / (D * D);
-    p.dbdi[p.i7 * p.N + i] = p.slope * 0.25 * p.alpha1 * pow(T, -0.75) / (D * D);
-    if (!isfinite(B) || !(B > 0.0)) atomicOr(&p.geom->flags, FLAG_NONFINITE_RESULT);
+    p.dbdi[sd.i4 * p.N + i] = sd.slope * sd.alpha0 / (D * D);
+    p.dbdi[sd.i7 * p.N + i] = sd.slope * 0.25 * sd.alpha1 * pow(T, -0.75) / (D * D);
+    if (!isfinite(B) || !(B > 0.0)) raise_flag(p, s, FLAG_NONFINITE_RESULT);
 }
 
 // ---------------------------------------------------------------------------
-// STILL pair pass: workgroup = 64 atoms i x one column split; every ordered pair
-// (i, j != i) is evaluated from i's side, so no atomics and a fixed summation order.
-// E_ij = P q_i q_j / f_ij, f = sqrt(r^2 + B_i B_j exp(-r^2/(4 B_i B_j)))
-// (CharmmReferenceGBSW.cpp:248-282).
+// STILL pair pass: workgroup = one tile (64 atoms i x a column range of the same system);
+// every ordered pair (i, j != i) is evaluated from i's side, so no atomics and a fixed
+// summation order.  E_ij = P q_i q_j / f_ij, f = sqrt(r^2 + B_i B_j exp(-r^2/(4 B_i B_j)))
+// (CharmmReferenceGBSW.cpp:248-282).  ADJ: also sum_j P q_j / f_ij = dE/dq_i - P q_i/B_i.
 // ---------------------------------------------------------------------------
 
+template <bool ADJ>
 __global__ void __launch_bounds__(64) k_pairs(KParams p) {
     __shared__ double s_px[64], s_py[64], s_pz[64];
     __shared__ float s_q[64], s_B[64];
-    const Geom G = *p.geom;
-    if (G.flags & kFatal) return;
+    if (batch_fatal(p)) return;
     const int lane = threadIdx.x;
-    const int i = blockIdx.x * kWave + lane;
-    const int s = blockIdx.y, S = gridDim.y;
-    const int jlo = (int)((long long)s * p.N / S), jhi = (int)((long long)(s + 1) * p.N / S);
-    const bool iv = i < p.N;
+    const int4 tile = p.pair_tiles[blockIdx.x];
+    const int s = p.atom_sys[tile.x];
+    const int row_end = min(tile.x + kWave, p.sys[s].atom_end);
+    const int i = tile.x + lane;
+    const int jlo = tile.y, jhi = tile.z, split = tile.w;
+    const bool iv = i < row_end;
     const double xi = iv ? p.pos[3 * i] : 0.0, yi = iv ? p.pos[3 * i + 1] : 0.0, zi = iv ? p.pos[3 * i + 2] : 0.0;
+    const float P = (float)p.sys[s].prefactor;
     const float qi = iv ? (float)p.charge[i] : 0.f;
     const float Bi = iv ? (float)p.born[i] : 1.f;
-    const float Pq = (float)p.prefactor * qi;
+    const float Pq = P * qi;
     const float q4Bi = 0.25f / Bi;
-    double fx = 0.0, fy = 0.0, fz = 0.0, dedb = 0.0, e = 0.0;
+    double fx = 0.0, fy = 0.0, fz = 0.0, dedb = 0.0, e = 0.0, dq = 0.0;
     for (int jb = jlo; jb < jhi; jb += kWave) {
         const int jj = jb + lane;
         __syncthreads();
@@ -763,7 +839,7 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
         __syncthreads();
         const int cnt = min(kWave, jhi - jb);
         if (iv) {
-            float efs = 0.f;
+            float efs = 0.f, dqs = 0.f;
             float fxs = 0.f, fys = 0.f, fzs = 0.f, dbs = 0.f;
             for (int m = 0; m < cnt; ++m) {
                 if (jb + m == i) continue;
@@ -777,6 +853,7 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
                 const float rf3 = rf * rf * rf;
                 const float Pqq = Pq * s_q[m];
                 efs = fmaf(0.5f * Pqq, rf, efs);
+                if (ADJ) dqs = fmaf(P * s_q[m], rf, dqs);
                 const float dGdr = -0.25f * Pqq * (4.f - ex) * rf3;
                 fxs = fmaf(dx, dGdr, fxs);
                 fys = fmaf(dy, dGdr, fys);
@@ -789,62 +866,90 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
             fy += fys;
             fz += fzs;
             dedb += dbs;
+            if (ADJ) dq += dqs;
         }
     }
     if (iv) {
-        double* out = p.pair_part + (size_t)s * 5 * p.N;
+        double* out = p.pair_part + (size_t)split * 6 * p.N;
         out[i] = fx;
         out[p.N + i] = fy;
         out[2 * p.N + i] = fz;
         out[3 * p.N + i] = dedb;
         out[4 * p.N + i] = e;
+        if (ADJ) out[5 * p.N + i] = dq;
     }
 }
 
 // Per atom: pair partials in split order, self term, dE/dI = dE/dB dB/dI
 // (customGBEnergyPerParticle.cu; CharmmReferenceGBSW.cpp:257-273).
+// ADJ: dE/dq and the Born-map parts of dE/dtheta.
+template <bool ADJ>
 __global__ void k_atom_chain(KParams p) {
-    const Geom G = *p.geom;
-    if (G.flags & kFatal) return;
+    if (batch_fatal(p)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.N) return;
-    double fx = 0.0, fy = 0.0, fz = 0.0, dedb = 0.0, e = 0.0;
-    for (int s = 0; s < p.pair_splits; ++s) {
-        const double* in = p.pair_part + (size_t)s * 5 * p.N;
+    const SysDesc& sd = p.sys[p.atom_sys[i]];
+    double fx = 0.0, fy = 0.0, fz = 0.0, dedb = 0.0, e = 0.0, dq = 0.0;
+    const int splits = sd.pair_splits;
+    for (int k = 0; k < splits; ++k) {
+        const double* in = p.pair_part + (size_t)k * 6 * p.N;
         fx += in[i];
         fy += in[p.N + i];
         fz += in[2 * p.N + i];
         dedb += in[3 * p.N + i];
         e += in[4 * p.N + i];
+        if (ADJ) dq += in[5 * p.N + i];
     }
-    const double q = p.charge[i], B = p.born[i];
-    e += 0.5 * p.prefactor * q * q / B;
-    dedb += -0.5 * p.prefactor * q * q / (B * B);
+    const double q = p.charge[i], B = p.born[i], P = sd.prefactor;
+    e += 0.5 * P * q * q / B;
+    dedb += -0.5 * P * q * q / (B * B);
     p.atom_energy[i] = e;
     p.force_fix[i] = (long long)llrint(fx * kFix32);
     p.force_fix[p.N + i] = (long long)llrint(fy * kFix32);
     p.force_fix[2 * p.N + i] = (long long)llrint(fz * kFix32);
     p.dedi[i] = dedb * p.dbdi[i];
     p.dedi[p.N + i] = dedb * p.dbdi[p.N + i];
+    if (ADJ) {
+        p.dedq[i] = dq + P * q / B;
+        p.dedr_fix[i] = 0;
+        // B = slope / D + shift, D = a0 (1/eta - I4) + a1 T^(1/4), T = 1/(4 eta^4) - I7
+        const double I4 = p.integrals[sd.i4 * p.N + i], I7 = p.integrals[sd.i7 * p.N + i];
+        const double eta = sd.eta, eta4 = eta * eta * eta * eta;
+        const double T = 1.0 / (4.0 * eta4) - I7;
+        const double T4 = pow(T, 0.25);
+        const double D = sd.alpha0 * (1.0 / eta - I4) + sd.alpha1 * T4;
+        const double g = -sd.slope / (D * D) * dedb;  // dE/dD
+        const double dDdeta = -sd.alpha0 / (eta * eta) - sd.alpha1 * 0.25 * pow(T, -0.75) / (eta4 * eta);
+        double* th = p.theta_atom;
+        th[TH_PREFACTOR * p.N + i] = P != 0.0 ? e / P : 0.0;
+        th[TH_ALPHA0 * p.N + i] = g * (1.0 / eta - I4);
+        th[TH_ALPHA1 * p.N + i] = g * T4;
+        th[TH_ETA * p.N + i] = g * dDdeta;
+        th[TH_SLOPE * p.N + i] = dedb / D;
+        th[TH_SHIFT * p.N + i] = dedb;
+    }
 }
 
+// Energy of each system: one workgroup per system, fixed-order FP64 sum.
 __global__ void __launch_bounds__(1024) k_energy(KParams p) {
     __shared__ double s_w[16];
+    const int s = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
-    if (fatal(p.geom)) {
-        if (tid == 0) *p.energy = 0.0;
+    if (batch_fatal(p)) {
+        if (tid == 0) p.energy[s] = 0.0;
         return;
     }
+    const int ab = p.sys[s].atom_begin, ae = p.sys[s].atom_end;
     double acc = 0.0;
-    for (int i = tid; i < p.N; i += blockDim.x) acc += p.atom_energy[i];
+    for (int i = ab + tid; i < ae; i += blockDim.x) acc += p.atom_energy[i];
     acc = wave_sum(acc);
     if (lane == 0) s_w[wid] = acc;
     __syncthreads();
     if (tid == 0) {
         double t = 0.0;
         for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += s_w[w];
-        *p.energy = t;
-        if (!isfinite(t)) atomicOr(&p.geom->flags, FLAG_NONFINITE_RESULT);
+        p.energy[s] = t;
+        if (!isfinite(t)) raise_flag(p, s, FLAG_NONFINITE_RESULT);
     }
 }
 
@@ -858,6 +963,8 @@ __global__ void __launch_bounds__(1024) k_energy(KParams p) {
 // (GBSWIntegral.cpp:271-314, GBMVIntegralTypeI.cpp:195-224, GBMVIntegralTypeII.cpp:221-282;
 //  CharmmReferenceGBMV.cpp:376-383).  Atom partials are quantised to 2^-24 and summed
 // exactly in FP64, so the per-cell totals do not depend on the point order.
+// ADJ: also s fac dV_p/dR_j (GBSW: dR'_j) for every list atom including the owner, and
+// the per-system parameter sums (VolAcc).
 // ---------------------------------------------------------------------------
 
 // Sum over the 8 lanes 8g..8g+7 (DPP: quad xor 1, quad xor 2, half-row mirror).
@@ -868,33 +975,40 @@ __device__ inline float sum8(float v) {
     return v;
 }
 
-template <int VK>
-__global__ void __launch_bounds__(64) k_force(KParams p) {
+template <int VK, bool ADJ>
+__global__ void __launch_bounds__(64, ADJ ? 2 : 3) k_force(KParams p) {
     __shared__ float q_x[kWave], q_y[kWave], q_z[kWave];
     __shared__ float q_0[kWave], q_1[kWave], q_2[kWave], q_3[kWave], q_4[kWave], q_5[kWave];
     __shared__ int q_own[kWave], q_pid[kWave];
     __shared__ float q_V[kWave];
-    const Geom G = *p.geom;
-    if (G.flags & kFatal) return;
+    if (batch_fatal(p)) return;
     const int lane = threadIdx.x, ag = lane & 7, pg = lane >> 3;
-    const int nwork = p.work[2];
+    const int nwork = p.work[VK];
+    const int* worklist = p.worklist + VK * p.cell_total;
+    const int* worklist_sys = p.worklist_sys + VK * p.cell_total;
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-        const int c = p.worklist[w];
+        const int c = worklist[w];
+        const int s = worklist_sys[w];
         const int nact = p.cell_act_count[c];
         if (nact == 0) continue;
+        const SysDesc& sd = p.sys[s];
         const int na = p.cell_count_eff[c];
         const int ps = p.frec_base[c];
         const int as = p.cell_start[c];
+        const int pt_base = sd.pt_base, abeg = sd.atom_begin, Q = sd.Q, qb = sd.quad_base;
+        const float k1 = sd.k1, k3 = sd.k3, fclamp = sd.fclamp, sign_chain = sd.sign_chain;
+        const float inv_sw = sd.inv_sw, P1x10 = 10.f * sd.P1;
+        double pacc0 = 0.0, pacc1 = 0.0;  // ADJ: VolAcc partials of this lane
         for (int ab = 0; ab < na; ab += kWave) {
-            float ax[8], ay[8], az[8], a0[8], a1[8], a2[8];
+            float ax[8], ay[8], az[8], a0[8], a1[8], a2[8], aR[8];
             int aj[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                const int s = ab + ag + 8 * k;
+                const int sl = ab + ag + 8 * k;
                 float4 e = make_float4(0.f, 0.f, 0.f, __int_as_float(-1)), q = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (s < na) {
-                    e = p.ent_pos[as + s];
-                    q = p.ent_par[as + s];
+                if (sl < na) {
+                    e = p.ent_pos[as + sl];
+                    q = p.ent_par[as + sl];
                 }
                 ax[k] = e.x;
                 ay[k] = e.y;
@@ -903,10 +1017,15 @@ __global__ void __launch_bounds__(64) k_force(KParams p) {
                 a0[k] = q.x;
                 a1[k] = q.y;
                 a2[k] = q.z;
+                aR[k] = q.w;
             }
             double acc[8][3];
+            double accr[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
+            for (int k = 0; k < 8; ++k) {
+                acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
+                accr[k] = 0.0;
+            }
             for (int pb = 0; pb < nact; pb += kWave) {
                 const int cntp = min(kWave, nact - pb);
                 __syncthreads();
@@ -916,13 +1035,13 @@ __global__ void __launch_bounds__(64) k_force(KParams p) {
                     const float4 r1 = p.frec1[o];
                     const uint32_t pk = __float_as_uint(r0.w);
                     const int i = (int)(pk >> kQBits), q = (int)(pk & ((1u << kQBits) - 1));
-                    const double fac = p.dedi[i] * p.quad_pw[q] + p.dedi[p.N + i] * p.quad_pw[p.Q + q];
-                    const float facs = p.sign_chain * (float)fac;
+                    const double fac = p.dedi[i] * p.quad_pw[qb + q] + p.dedi[p.N + i] * p.quad_pw[p.quad_rows + qb + q];
+                    const float facs = sign_chain * (float)fac;
                     q_x[lane] = r0.x;
                     q_y[lane] = r0.y;
                     q_z[lane] = r0.z;
                     q_own[lane] = i;
-                    q_pid[lane] = i * p.Q + q;
+                    q_pid[lane] = pt_base + (i - abeg) * Q + q;
                     if (VK == VOL_GBSW) {
                         q_0[lane] = facs;
                         q_1[lane] = r1.x;
@@ -954,40 +1073,66 @@ __global__ void __launch_bounds__(64) k_force(KParams p) {
                         const float vz = VK == VOL_GBMV2 ? q_5[m] : 0.f;
 #pragma unroll
                         for (int k = 0; k < 8; ++k) {
-                            if (aj[k] < 0 || aj[k] == own) continue;
+                            if (aj[k] < 0) continue;
+                            const bool is_own = aj[k] == own;
+                            if (!ADJ && is_own) continue;
                             const float dx = ax[k] - px, dy = ay[k] - py, dz = az[k] - pz;
                             const float dd = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                            float s = 0.f, c3F = 0.f;
+                            float sv = 0.f, c3F = 0.f;
                             if (VK == VOL_GBSW) {
                                 // shell atoms only: (R'-w)^2 < d^2 < (R'+w)^2
                                 if (dd < a2[k] && dd > a1[k]) {
                                     const float d = fast_sqrt(dd);
                                     const float x = d - a0[k];
-                                    const float h = fmaf(x, fmaf(-p.k3 * x, x, p.k1), 0.5f);
+                                    const float h = fmaf(x, fmaf(-k3 * x, x, k1), 0.5f);
                                     if (h > 0.f) {
-                                        const float dh = fmaf(-3.f * p.k3 * x, x, p.k1);
-                                        s = d0 * (d1 * fast_rcp(h)) * dh * fast_rcp(d);
+                                        const float dh = fmaf(-3.f * k3 * x, x, k1);
+                                        const float gV = d0 * (d1 * fast_rcp(h));  // s fac V / H_j
+                                        sv = gV * dh * fast_rcp(d);
+                                        if (ADJ) {
+                                            // dV/dR'_j = -V H'_j/H_j; dV/dw = V (dH_j/dw)/H_j
+                                            accr[k] += quant(-gV * dh);
+                                            const float xw = x * inv_sw;
+                                            pacc0 += quant(gV * 0.75f * xw * inv_sw * fmaf(xw, xw, -1.f));
+                                        }
                                     }
                                 }
                             } else if (VK == VOL_GBMV2) {
                                 const float tt = fmaf(100.f, dd, a1[k]);
                                 const float rt = fast_rcp(tt);
                                 const float ct = a0[k] * rt;
-                                const float F = fminf(ct * ct, p.fclamp);
+                                const float F = fminf(ct * ct, fclamp);
                                 const float D = -a2[k] * rt * rt * rt;
                                 const float vdr = fmaf(vx, dx, fmaf(vy, dy, vz * dz));
-                                s = fmaf(d0, D, fmaf(d1 * F, fmaf(D, dd, F), d2c * vdr * D));
+                                sv = fmaf(d0, D, fmaf(d1 * F, fmaf(D, dd, F), d2c * vdr * D));
                                 c3F = d2c * F;
+                                if (ADJ) {
+                                    // s fac dV/dF_j; F = (C/t)^2, t = b + 100 d^2, b = C - 100 R^2
+                                    const float G = fmaf(d1 * dd, F, fmaf(d2c, vdr, d0));
+                                    const float dFdC = 2.f * ct * rt, dFdb = -2.f * ct * ct * rt;
+                                    const float gab = G * (dFdC + dFdb);
+                                    accr[k] += quant(fmaf(G * dFdb, -200.f * aR[k], gab * P1x10));
+                                    pacc0 += quant(gab);
+                                    pacc1 += quant(gab * 10.f * aR[k]);
+                                }
                             } else {
                                 const float gam = a0[k];
-                                s = d0 * gam * __expf(gam * (dd * dd)) * 4.f * dd;
+                                const float ex = __expf(gam * (dd * dd));
+                                sv = d0 * gam * ex * 4.f * dd;
+                                if (ADJ) {
+                                    // dS/dgamma_j = d^4 e_j; dgamma_j/dR_j = -4 gamma_j/R_j
+                                    const float tg = d0 * (dd * dd) * ex;
+                                    accr[k] += quant(tg * (-4.f * gam / aR[k]));
+                                    pacc0 += quant(tg * gam);
+                                }
                             }
-                            const float cx = fmaf(s, dx, c3F * vx);
-                            const float cy = fmaf(s, dy, c3F * vy);
-                            const float cz = fmaf(s, dz, c3F * vz);
-                            acc[k][0] += (double)rintf(cx * kQuantum);
-                            acc[k][1] += (double)rintf(cy * kQuantum);
-                            acc[k][2] += (double)rintf(cz * kQuantum);
+                            if (ADJ && is_own) continue;  // the owner moves with its points
+                            const float cx = fmaf(sv, dx, c3F * vx);
+                            const float cy = fmaf(sv, dy, c3F * vy);
+                            const float cz = fmaf(sv, dz, c3F * vz);
+                            acc[k][0] += quant(cx);
+                            acc[k][1] += quant(cy);
+                            acc[k][2] += quant(cz);
                             ox += cx;
                             oy += cy;
                             oz += cz;
@@ -1011,7 +1156,7 @@ __global__ void __launch_bounds__(64) k_force(KParams p) {
             }
             // exact sums over the point groups (lanes ag + 8 pg)
 #pragma unroll
-            for (int k = 0; k < 8; ++k)
+            for (int k = 0; k < 8; ++k) {
                 for (int d = 0; d < 3; ++d) {
                     double v = acc[k][d];
                     v += __shfl_xor(v, 8, kWave);
@@ -1019,15 +1164,32 @@ __global__ void __launch_bounds__(64) k_force(KParams p) {
                     v += __shfl_xor(v, 32, kWave);
                     acc[k][d] = v;
                 }
+                if (ADJ) {
+                    double v = accr[k];
+                    v += __shfl_xor(v, 8, kWave);
+                    v += __shfl_xor(v, 16, kWave);
+                    v += __shfl_xor(v, 32, kWave);
+                    accr[k] = v;
+                }
+            }
             if (pg == 0) {
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                     if (aj[k] < 0) continue;
                     for (int d = 0; d < 3; ++d)
-                        if (acc[k][d] != 0.0)  // exact integers in units of 2^-24; x 2^8 -> 2^-32 units
-                            atomicAdd((unsigned long long*)&p.force_fix[d * p.N + aj[k]],
-                                      (unsigned long long)(long long)(acc[k][d] * 256.0));
+                        if (acc[k][d] != 0.0) add_fix(&p.force_fix[d * p.N + aj[k]], acc[k][d]);
+                    if (ADJ && accr[k] != 0.0) add_fix(&p.dedr_fix[aj[k]], accr[k]);
                 }
+            }
+        }
+        if (ADJ) {
+            pacc0 = wave_sum(pacc0);
+            pacc1 = wave_sum(pacc1);
+            if (lane == 0) {
+                long long* va = p.vacc_fix + s * VA_COUNT;
+                const int i0 = VK == VOL_GBSW ? VA_W : VK == VOL_GBMV2 ? VA_A : VA_GAMMA;
+                if (pacc0 != 0.0) add_fix(&va[i0], pacc0);
+                if (VK == VOL_GBMV2 && pacc1 != 0.0) add_fix(&va[VA_AR], pacc1);
             }
         }
     }
@@ -1035,33 +1197,122 @@ __global__ void __launch_bounds__(64) k_force(KParams p) {
 
 // Owner forces: one wave per atom, fixed-order FP64 sum over its quadrature points.
 // GBSW: F_i += sum_q fac(i,q) dV/dp(i,q); GBMV: F_i += sum_q owner force(i,q).
+// ADJ (GBMV): the sigmoid parts of dE/dbeta, dE/dlambda, dE/dS0 with
+// V = 1/(1 + exp(beta (S - lambda))): dV/dbeta = -V(1-V)(S - lambda), dV/dlambda = V(1-V) beta,
+// dV/dS0 = -V(1-V) beta S/S0; dE/dtheta = -sum s fac dV/dtheta.
+template <bool ADJ>
 __global__ void __launch_bounds__(64) k_owner_reduce(KParams p) {
-    const Geom G = *p.geom;
-    if (G.flags & kFatal) return;
+    if (batch_fatal(p)) return;
     const int i = blockIdx.x, lane = threadIdx.x;
-    const bool gbsw = p.vol_kind == VOL_GBSW;
+    const int s = p.atom_sys[i];
+    const SysDesc& sd = p.sys[s];
+    const int Q = sd.Q, qb = sd.quad_base;
+    const int pbase = sd.pt_base + (i - sd.atom_begin) * Q;
+    const bool gbsw = sd.vol_kind == VOL_GBSW;
     const double e4 = p.dedi[i], e7 = p.dedi[p.N + i];
+    const double sc = sd.sign_chain;
+    const double beta = sd.beta, lambda = sd.lambda;
     double a[3] = {0.0, 0.0, 0.0};
-    for (int q = lane; q < p.Q; q += kWave) {
-        const int pid = i * p.Q + q;
-        const double wq = gbsw ? (double)p.sign_chain * (e4 * p.quad_pw[q] + e7 * p.quad_pw[p.Q + q]) : 1.0;
+    double tb = 0.0, tl = 0.0, ts = 0.0;
+    for (int q = lane; q < Q; q += kWave) {
+        const int pid = pbase + q;
+        const double facs = sc * (e4 * p.quad_pw[qb + q] + e7 * p.quad_pw[p.quad_rows + qb + q]);
+        const double wq = gbsw ? facs : 1.0;
         const float4 v = p.vrec[pid];
         a[0] += wq * (double)v.y;
         a[1] += wq * (double)v.z;
         a[2] += wq * (double)v.w;
+        if (ADJ && !gbsw) {
+            const float S = p.vsig[pid];
+            if (!isnan(S)) {
+                const double V = v.x;
+                const double u = V * (1.0 - V);
+                tb += facs * u * ((double)S - lambda);
+                tl -= facs * u * beta;
+                ts += facs * u * beta * (double)S;
+            }
+        }
     }
     for (int d = 0; d < 3; ++d) a[d] = wave_sum(a[d]);
+    if (ADJ) {
+        tb = wave_sum(tb);
+        tl = wave_sum(tl);
+        ts = wave_sum(ts);
+    }
     if (lane == 0) {
         for (int d = 0; d < 3; ++d) p.force_fix[d * p.N + i] += (long long)llrint(a[d] * kFix32);
+        if (ADJ) {
+            p.theta_atom[TH_BETA * p.N + i] = tb;
+            p.theta_atom[TH_LAMBDA * p.N + i] = tl;
+            p.theta_atom[TH_S0 * p.N + i] = sd.S0 != 0.f ? ts / (double)sd.S0 : 0.0;
+        }
+    }
+}
+
+// Adjoint, per atom: dE/dR from the fixed-point chain sums; GBSW also the per-atom parts of
+// dE/drscale and dE/droffset (R' = rscale (R + roffset)).
+__global__ void k_adjoint_atoms(KParams p) {
+    if (batch_fatal(p)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.N) return;
+    const SysDesc& sd = p.sys[p.atom_sys[i]];
+    const double g = -(double)p.dedr_fix[i] / kFix32;  // dE/dR'_j (GBSW) or dE/dR_j
+    double* th = p.theta_atom;
+    if (sd.vol_kind == VOL_GBSW) {
+        p.dedr[i] = g * (double)sd.rscale;
+        th[TH_RSCALE * p.N + i] = g * (p.radius[i] + (double)sd.roffset);
+        th[TH_ROFFSET * p.N + i] = g * (double)sd.rscale;
+    } else {
+        p.dedr[i] = g;
+        th[TH_RSCALE * p.N + i] = 0.0;
+        th[TH_ROFFSET * p.N + i] = 0.0;
+    }
+}
+
+// Adjoint, per system: fixed-order sums of the per-atom parts plus the force-pass sums.
+__global__ void __launch_bounds__(256) k_theta_reduce(KParams p) {
+    __shared__ double s_w[4];
+    const int s = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const SysDesc& sd = p.sys[s];
+    if (batch_fatal(p)) {
+        if (tid < TH_COUNT) p.theta[s * TH_COUNT + tid] = 0.0;
+        return;
+    }
+    const int atom_rows[] = {TH_PREFACTOR, TH_ALPHA0, TH_ALPHA1, TH_ETA, TH_SLOPE, TH_SHIFT,
+                             TH_RSCALE,    TH_ROFFSET, TH_BETA,  TH_LAMBDA, TH_S0};
+    for (int r = 0; r < (int)(sizeof(atom_rows) / sizeof(int)); ++r) {
+        const double* row = p.theta_atom + (size_t)atom_rows[r] * p.N;
+        double acc = 0.0;
+        for (int i = sd.atom_begin + tid; i < sd.atom_end; i += blockDim.x) acc += row[i];
+        acc = wave_sum(acc);
+        __syncthreads();
+        if (lane == 0) s_w[wid] = acc;
+        __syncthreads();
+        if (tid == 0) p.theta[s * TH_COUNT + atom_rows[r]] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    }
+    if (tid == 0) {
+        const long long* va = p.vacc_fix + s * VA_COUNT;
+        double* th = p.theta + s * TH_COUNT;
+        th[TH_SW] = th[TH_GAMMA0] = th[TH_P1] = th[TH_P2] = 0.0;
+        if (sd.vol_kind == VOL_GBSW) {
+            th[TH_SW] = -(double)va[VA_W] / kFix32;
+        } else if (sd.vol_kind == VOL_GBMV2) {
+            th[TH_P1] = -(double)va[VA_AR] / kFix32;
+            th[TH_P2] = -(double)va[VA_A] / kFix32;
+        } else {
+            const double gsum = -(double)va[VA_GAMMA] / kFix32;
+            th[TH_GAMMA0] = gsum / (double)sd.gamma0;
+            th[TH_LAMBDA] += gsum / ((double)sd.lambda * log((double)sd.lambda));
+        }
     }
 }
 
 __global__ void k_stats(KParams p, double* out) {
-    const Geom G = *p.geom;
-    if (G.flags & kFatal) return;
+    if (batch_fatal(p)) return;
     double np = 0.0, pairs = 0.0;
     const int stride = gridDim.x * blockDim.x;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < G.ncell; c += stride) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < p.cell_total; c += stride) {
         const double a = p.cell_pt_count[c];
         np += a;
         pairs += a * p.cell_count_eff[c];
@@ -1073,81 +1324,83 @@ __global__ void k_stats(KParams p, double* out) {
         atomicAdd(&out[1], pairs);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        out[2] = G.ncell;
-        out[3] = G.total_entries;
+        double nc = 0.0;
+        for (int s = 0; s < p.S; ++s) nc += p.geom[s].ncell;
+        out[2] = nc;
+        out[3] = p.cell_start[p.cell_total];
     }
 }
 
-int g_num_cus = 0;
-
-int persistent_blocks() {
-    if (g_num_cus == 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&g_num_cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            g_num_cus <= 0)
-            g_num_cus = 256;
+template <bool ADJ>
+void launch_force(const KParams& p, int vk, int blocks, hipStream_t stream) {
+    switch (vk) {
+    case VOL_GBSW: hipLaunchKernelGGL((k_force<VOL_GBSW, ADJ>), dim3(blocks), dim3(64), 0, stream, p); break;
+    case VOL_GBMV1: hipLaunchKernelGGL((k_force<VOL_GBMV1, ADJ>), dim3(blocks), dim3(64), 0, stream, p); break;
+    default: hipLaunchKernelGGL((k_force<VOL_GBMV2, ADJ>), dim3(blocks), dim3(64), 0, stream, p); break;
     }
-    return g_num_cus * 32;
 }
 
 }  // namespace
 
-hipError_t launch_bbox_only(const KParams& p, hipStream_t stream) {
-    hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, stream, p);
-    return hipGetLastError();
-}
-
-hipError_t launch_evaluate(const KParams& p, unsigned flags, hipStream_t stream, const hipEvent_t* marks) {
-    (void)flags;
+hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stream, const hipEvent_t* marks) {
     hipError_t err;
     auto mark = [&](int k) {
         if (marks) (void)hipEventRecord(marks[k], stream);
     };
+    const bool adj = p.adjoint != 0;
     mark(0);
-    const int clear_blocks = (p.cell_cap + 255) / 256;
-    hipLaunchKernelGGL(k_clear, dim3(clear_blocks > 0 ? clear_blocks : 1), dim3(256), 0, stream, p);
-    hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, stream, p);
-    const int atom_blocks = (p.N + 255) / 256;
-    if (p.grid_kind == GRID_BRUTE) {
-        hipLaunchKernelGGL(k_grid_brute, dim3(atom_blocks), dim3(256), 0, stream, p);
-    } else {
-        hipLaunchKernelGGL(k_grid_count, dim3(p.N), dim3(256), 0, stream, p);
-        err = scan_exclusive(p.cell_count, p.cell_start, p.cell_cap, p.scan_tmp, &p.geom->total_entries,
-                             p.entry_cap, &p.geom->flags, FLAG_ENTRY_OVERFLOW, stream);
-        if (err != hipSuccess) return err;
-        hipLaunchKernelGGL(k_grid_fill, dim3(p.N), dim3(256), 0, stream, p);
-        hipLaunchKernelGGL(k_grid_sort, dim3(std::min(p.cell_cap, 8192)), dim3(64), 0, stream, p);
-    }
+    const int clear_blocks = std::max(1, std::min((p.cell_total + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_clear, dim3(clear_blocks), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(k_bbox, dim3(p.S), dim3(1024), 0, stream, p);
+    hipLaunchKernelGGL(k_grid_count, dim3(p.N), dim3(256), 0, stream, p);
+    err = scan_exclusive(p.cell_count, p.cell_start, p.cell_total, p.scan_tmp, p.entry_cap, p.flags_all,
+                         FLAG_ENTRY_OVERFLOW, stream);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_grid_fill, dim3(p.N), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(k_grid_sort, dim3(std::min(p.cell_total, 8192)), dim3(64), 0, stream, p);
     mark(1);
     hipLaunchKernelGGL(k_point_bin, dim3(p.N), dim3(256), 0, stream, p);
     // records of the active points of cell c start at the exclusive prefix sum of the point counts
-    err = scan_exclusive(p.cell_pt_count, p.frec_base, p.cell_cap, p.scan_tmp, &p.geom->total_points, p.NQ,
-                         &p.geom->flags, FLAG_POINT_OVERFLOW, stream);
+    err = scan_exclusive(p.cell_pt_count, p.frec_base, p.cell_total, p.scan_tmp, p.NQ, p.flags_all,
+                         FLAG_POINT_OVERFLOW, stream);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k_worklist, dim3(std::min((p.cell_cap + 255) / 256, 4096)), dim3(256), 0, stream, p);
-    const int pb = std::min(p.cell_cap, persistent_blocks());
+    hipLaunchKernelGGL(k_worklist, dim3(std::max(1, std::min((p.cell_total + 255) / 256, 4096))), dim3(256), 0,
+                       stream, p);
+    const int pb = std::max(1, std::min(p.cell_total, p.persist_blocks));
     mark(2);
-    switch (p.vol_kind) {
-    case VOL_GBSW: hipLaunchKernelGGL(k_volume<VOL_GBSW>, dim3(pb), dim3(64), 0, stream, p); break;
-    case VOL_GBMV1: hipLaunchKernelGGL(k_volume<VOL_GBMV1>, dim3(pb), dim3(64), 0, stream, p); break;
-    default: hipLaunchKernelGGL(k_volume<VOL_GBMV2>, dim3(pb), dim3(64), 0, stream, p); break;
-    }
+    if (kinds[VOL_GBSW]) hipLaunchKernelGGL(k_volume<VOL_GBSW>, dim3(pb), dim3(64), 0, stream, p);
+    if (kinds[VOL_GBMV1]) hipLaunchKernelGGL(k_volume<VOL_GBMV1>, dim3(pb), dim3(64), 0, stream, p);
+    if (kinds[VOL_GBMV2]) hipLaunchKernelGGL(k_volume<VOL_GBMV2>, dim3(pb), dim3(64), 0, stream, p);
     mark(3);
     hipLaunchKernelGGL(k_integrals, dim3(p.N), dim3(64), 0, stream, p);
     mark(4);
-    hipLaunchKernelGGL(k_pairs, dim3((p.N + kWave - 1) / kWave, p.pair_splits), dim3(64), 0, stream, p);
+    if (adj)
+        hipLaunchKernelGGL(k_pairs<true>, dim3(p.n_tiles), dim3(64), 0, stream, p);
+    else
+        hipLaunchKernelGGL(k_pairs<false>, dim3(p.n_tiles), dim3(64), 0, stream, p);
     mark(5);
-    hipLaunchKernelGGL(k_atom_chain, dim3(atom_blocks), dim3(256), 0, stream, p);
-    hipLaunchKernelGGL(k_energy, dim3(1), dim3(1024), 0, stream, p);
+    const int atom_blocks = (p.N + 255) / 256;
+    if (adj)
+        hipLaunchKernelGGL(k_atom_chain<true>, dim3(atom_blocks), dim3(256), 0, stream, p);
+    else
+        hipLaunchKernelGGL(k_atom_chain<false>, dim3(atom_blocks), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(k_energy, dim3(p.S), dim3(1024), 0, stream, p);
     mark(6);
-    switch (p.vol_kind) {
-    case VOL_GBSW: hipLaunchKernelGGL(k_force<VOL_GBSW>, dim3(pb), dim3(64), 0, stream, p); break;
-    case VOL_GBMV1: hipLaunchKernelGGL(k_force<VOL_GBMV1>, dim3(pb), dim3(64), 0, stream, p); break;
-    default: hipLaunchKernelGGL(k_force<VOL_GBMV2>, dim3(pb), dim3(64), 0, stream, p); break;
+    for (int vk = 0; vk < VOL_KINDS; ++vk) {
+        if (!kinds[vk]) continue;
+        if (adj)
+            launch_force<true>(p, vk, pb, stream);
+        else
+            launch_force<false>(p, vk, pb, stream);
     }
     mark(7);
-    hipLaunchKernelGGL(k_owner_reduce, dim3(p.N), dim3(64), 0, stream, p);
+    if (adj) {
+        hipLaunchKernelGGL(k_owner_reduce<true>, dim3(p.N), dim3(64), 0, stream, p);
+        hipLaunchKernelGGL(k_adjoint_atoms, dim3(atom_blocks), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL(k_theta_reduce, dim3(p.S), dim3(256), 0, stream, p);
+    } else {
+        hipLaunchKernelGGL(k_owner_reduce<false>, dim3(p.N), dim3(64), 0, stream, p);
+    }
     mark(8);
     return hipGetLastError();
 }
