@@ -83,6 +83,18 @@ __device__ inline void add_fix(long long* dst, double units24) {
 // Grid geometry and membership (FP64, no contraction: bitwise the oracle's arithmetic)
 // ---------------------------------------------------------------------------
 
+// floor(fl(a / g)) for 0 <= a / g < 2^20 without a division where it cannot matter:
+// t = a * fl(1/g) differs from fl(a / g) by at most 3.3 ulp(a / g) < 1e-9, so floor(t) is
+// the same integer unless t lies within 1e-9 of an integer; there the division decides.
+__device__ inline int floor_div(double a, double g, double inv_g) {
+#pragma clang fp contract(off)
+    const double t = a * inv_g;
+    const double f = floor(t);
+    const double fr = t - f;
+    if (fr > 1e-9 && fr < 1.0 - 1e-9) return (int)f;
+    return (int)floor(a / g);
+}
+
 // Local cell of a point; -1 outside.  Faithful: CustomGBIntegral.cpp:310-324.
 __device__ inline int point_cell(const SysDesc& sd, const SysGeom& G, double px, double py, double pz) {
 #pragma clang fp contract(off)
@@ -92,7 +104,7 @@ __device__ inline int point_cell(const SysDesc& sd, const SysGeom& G, double px,
     if (sd.grid_kind == GRID_FAITHFUL) {
         for (int d = 0; d < 3; ++d) {
             if (pt[d] < G.origin[d] || pt[d] > G.max[d]) return -1;
-            idx[d] = (int)floor((pt[d] - G.origin[d]) / sd.g);
+            idx[d] = floor_div(pt[d] - G.origin[d], sd.g, sd.inv_g);
         }
     } else {
         for (int d = 0; d < 3; ++d) {
@@ -514,14 +526,21 @@ __device__ inline void quad_point(const KParams& p, double xi, double yi, double
 
 // One workgroup per atom (visited with a stride coprime to N so concurrent workgroups hit
 // distant cells): the cell of every quadrature point and its corner-relative FP32 record in
-// the cell's slots.  Points outside the grid, in an empty cell, or (GBSW) inside the
-// owner's own burial radius get their volume directly.
+// the cell's slots.  An atom's points fall into about half as many distinct cells, so the
+// workgroup first counts its points per cell in an LDS hash table (local ranks), then
+// reserves each cell's slots with one global atomic.  Points outside the grid, in an empty
+// cell, or (GBSW) inside the owner's own burial radius get their volume directly.
+constexpr int kBinHash = 2048;                      // > Q: the table never fills
+constexpr int kBinPer = kBinHash / 256;             // points per thread
+
 __global__ void __launch_bounds__(256) k_point_bin(KParams p) {
+    __shared__ int t_key[kBinHash], t_cnt[kBinHash], t_base[kBinHash];
     if (batch_fatal(p)) return;
+    const int tid = threadIdx.x;
     const int i = (int)(((long long)blockIdx.x * p.perm_stride) % p.N);
     const int s = p.atom_sys[i];
     const SysDesc& sd = p.sys[s];
-    const SysGeom G = p.geom[s];
+    const SysGeom& G = p.geom[s];
     const int Q = sd.Q, qb = sd.quad_base, cb = sd.cell_base, pt_cap = sd.pt_cap;
     const long long prec_base = sd.prec_base;
     const int pbase = sd.pt_base + (i - sd.atom_begin) * Q;
@@ -532,7 +551,20 @@ __global__ void __launch_bounds__(256) k_point_bin(KParams p) {
     const float sig_empty = sd.vol_kind == VOL_GBMV1 ? 0.0f : __int_as_float(0x7fc00000);
     const double xi = p.pos[3 * i], yi = p.pos[3 * i + 1], zi = p.pos[3 * i + 2];
     const float own_lo2 = p.aparam[i].y;
-    for (int q = threadIdx.x; q < Q; q += blockDim.x) {
+    for (int k = tid; k < kBinHash; k += 256) {
+        t_key[k] = -1;
+        t_cnt[k] = 0;
+    }
+    __syncthreads();
+    int h[kBinPer], rank[kBinPer];
+    float rx[kBinPer], ry[kBinPer], rz[kBinPer];
+#pragma unroll
+    for (int u = 0; u < kBinPer; ++u) {
+        h[u] = -1;
+        rank[u] = 0;
+        rx[u] = ry[u] = rz[u] = 0.f;
+        const int q = tid + 256 * u;
+        if (q >= Q) continue;
         const int row = qb + q;
         int c = -1;
         double pt[3] = {0.0, 0.0, 0.0};
@@ -542,20 +574,43 @@ __global__ void __launch_bounds__(256) k_point_bin(KParams p) {
             c = point_cell(sd, G, pt[0], pt[1], pt[2]);
             if (c >= 0 && p.cell_count_eff[cb + c] == 0) c = -1;
         }
-        if (c >= 0) {
-            const int slot = atomicAdd(&p.cell_pt_count[cb + c], 1);
-            if (slot < pt_cap) {
-                double corner[3];
-                cell_corner(sd, G, c, corner);
-                p.prec[prec_base + (long long)c * pt_cap + slot] =
-                    make_float4((float)(pt[0] - corner[0]), (float)(pt[1] - corner[1]), (float)(pt[2] - corner[2]),
-                                __uint_as_float(((uint32_t)i << kQBits) | (uint32_t)q));
-            } else {
-                raise_flag(p, s, FLAG_POINT_OVERFLOW);
-            }
-        } else {
+        if (c < 0) {
             p.vrec[pbase + q] = make_float4(owner_buried ? 0.0f : v_empty, 0.f, 0.f, 0.f);
             if (p.adjoint) p.vsig[pbase + q] = owner_buried ? __int_as_float(0x7fc00000) : sig_empty;
+            continue;
+        }
+        double corner[3];
+        cell_corner(sd, G, c, corner);
+        rx[u] = (float)(pt[0] - corner[0]);
+        ry[u] = (float)(pt[1] - corner[1]);
+        rz[u] = (float)(pt[2] - corner[2]);
+        // open addressing; at most Q < kBinHash distinct keys, so a free entry always exists
+        int e = (int)(((unsigned)c * 2654435761u) >> 21);
+        for (int probe = 0; probe < kBinHash; ++probe) {
+            const int old = atomicCAS(&t_key[e], -1, c);
+            if (old == -1 || old == c) break;
+            e = (e + 1) & (kBinHash - 1);
+        }
+        h[u] = e;
+        rank[u] = atomicAdd(&t_cnt[e], 1);
+    }
+    __syncthreads();
+    for (int k = tid; k < kBinHash; k += 256) {
+        const int n = t_cnt[k];
+        if (n > 0) t_base[k] = atomicAdd(&p.cell_pt_count[cb + t_key[k]], n);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kBinPer; ++u) {
+        if (h[u] < 0) continue;
+        const int q = tid + 256 * u;
+        const int slot = t_base[h[u]] + rank[u];
+        const int c = t_key[h[u]];
+        if (slot < pt_cap) {
+            p.prec[prec_base + (long long)c * pt_cap + slot] =
+                make_float4(rx[u], ry[u], rz[u], __uint_as_float(((uint32_t)i << kQBits) | (uint32_t)q));
+        } else {
+            raise_flag(p, s, FLAG_POINT_OVERFLOW);
         }
     }
 }
