@@ -40,6 +40,11 @@ WORKLOADS = {
                             "smoothing 0.03 nm, all pairs"),
     "gbsw-2500": dict(model="gbsw", integral=0, n_atoms=2500, n_lebedev=38, seed=0,
                       desc="CharmmGBSWForce, 2,500-atom globule, 24x38 quadrature, all pairs"),
+    "batch64": dict(model="batch", integral=-1, n_atoms=0, n_lebedev=0, seed=0,
+                    desc="64 systems in one batched context, N ~ U[300, 6000] (seeded), alternating "
+                         "CharmmGBSWForce 24x38 and CharmmGBMVForce GBMVIntegralTypeII 24x50, hipGraph replay; "
+                         "beside it the same 64 systems as 64 separate contexts, enqueued back to back on one "
+                         "stream (sequential) and on one stream each (concurrent)"),
 }
 
 
@@ -52,6 +57,8 @@ def parse():
     ap.add_argument("--cpu-baseline-evals", type=int, default=1,
                     help="evaluations of the single-threaded oracle timed for cpu_baseline (0: skip)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 accuracy check")
+    ap.add_argument("--no-baseline", action="store_true",
+                    help="batch64: skip the per-system contexts (for profiling the batched launch alone)")
     return ap.parse_args()
 
 
@@ -70,6 +77,8 @@ def main() -> int:
     from openmmcustomgbintegralforce_amd import synthetic as S
 
     w = WORKLOADS[args.workload]
+    if w["model"] == "batch":
+        return run_batch(args, w)
     kind = kind_name(w)
     sysm = S.globule(w["n_atoms"], w["seed"])
     if w["model"] == "gbsw":
@@ -178,6 +187,120 @@ def main() -> int:
                    "n_frc_pairs": n_frc, "n_cells": stats["n_cells"]},
     }
     eng.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def run_batch(args, w) -> int:
+    """One step = one evaluation of all 64 systems; value = system evaluations per second."""
+    from openmmcustomgbintegralforce_amd import _native as N
+    from openmmcustomgbintegralforce_amd import synthetic as S
+
+    sizes = S.batch_sizes(64, 300, 6000, seed=w["seed"])
+    systems = [S.globule(int(n), 1000 + k) for k, n in enumerate(sizes)]
+    params = []
+    for k in range(64):
+        if k % 2 == 0:
+            p = N.default_params(N.MODEL_GBSW)
+        else:
+            p = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV2)
+            p.n_lebedev = 50
+        params.append(p)
+    pos = np.concatenate([s.pos + 40.0 * k for k, s in enumerate(systems)])
+
+    def timed(step_fn, sync_fn):
+        for _ in range(args.warmup):
+            step_fn()
+        sync_fn()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step_fn()
+        sync_fn()
+        return time.perf_counter() - t0
+
+    # batched: one context, every pass one launch over all systems, replayed from a graph
+    eng = N.Engine.batch(params, [s.charge for s in systems], [s.radius for s in systems])
+    e_b, f_b = eng.execute(pos)
+    eng.set_graph(True)
+    eng.set_positions(pos)
+    eng.compute()
+    eng.synchronize()
+    t_batch = timed(eng.compute, eng.synchronize)
+    eng.close()
+
+    if args.no_baseline:
+        print(json.dumps({"metric": METRIC, "value": 64 * args.steps / t_batch, "unit": "evals/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t_batch / args.steps,
+                          "config": {"workload": args.workload, "systems": 64}}), flush=True)
+        return 0
+
+    # the same systems one context each, enqueued back to back, one synchronisation per round
+    engines = [N.Engine(p, s.charge, s.radius) for p, s in zip(params, systems)]
+    for k, (e1, s) in enumerate(zip(engines, systems)):
+        e1.execute(s.pos + 40.0 * k)
+        e1.set_positions(s.pos + 40.0 * k)
+
+    def seq_step():
+        for e1 in engines:
+            e1.compute()
+
+    def seq_sync():
+        for e1 in engines:
+            e1.synchronize()
+
+    # 64 streams (one per context): the systems overlap on the device
+    t_conc = timed(seq_step, seq_sync)
+    # one shared stream: 64 sequential per-system evaluations
+    shared = engines[0].device_view().stream
+    for e1 in engines[1:]:
+        e1.set_stream(shared)
+    t_seq = timed(seq_step, seq_sync)
+    for e1 in engines:
+        e1.set_stream(None)
+        e1.close()
+
+    acc = {}
+    if not args.no_accuracy:
+        import oracle
+
+        offs = np.concatenate([[0], np.cumsum(sizes)])
+        worst_f = worst_e = 0.0
+        checked = []
+        for k in (0, 1, 2, 3):
+            ref = oracle.compute(oracle.params_from_abi(params[k], nthreads=min(16, os.cpu_count() or 1)),
+                                 systems[k].pos, systems[k].charge, systems[k].radius)
+            f = f_b[offs[k]:offs[k + 1]]
+            worst_f = max(worst_f, float(np.linalg.norm(f - ref.forces) / np.linalg.norm(ref.forces)))
+            worst_e = max(worst_e, float(abs(e_b[k] - ref.energy) / abs(ref.energy)))
+            checked.append(int(sizes[k]))
+        acc = {"force_rel_rms": worst_f, "energy_rel_err": worst_e, "accuracy_systems": checked}
+
+    value = 64 * args.steps / t_batch
+    seq_value = 64 * args.steps / t_seq
+    conc_value = 64 * args.steps / t_conc
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_batch / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: 64 seeded globules at 100 atoms/nm^3, sizes synthetic.batch_sizes(64, 300, 6000, seed 0)",
+        "config": {"workload": args.workload, "description": w["desc"], "n_atoms": int(np.sum(sizes)),
+                   "systems": 64},
+        **acc,
+        "sequential_evals_per_s": seq_value,
+        "batch_speedup": value / seq_value,
+        "concurrent_evals_per_s": conc_value,
+        "batch_speedup_vs_concurrent": value / conc_value,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
     print(json.dumps(out), flush=True)
     return 0
 

@@ -94,7 +94,6 @@ struct gbi_ctx {
     hipStream_t stream = nullptr;
     int device = 0;
     int num_cus = 256;
-    int perm_stride = 1;
     bool have_positions = false;
     bool computed = false;
     bool computed_adjoint = false;
@@ -103,7 +102,7 @@ struct gbi_ctx {
     // device buffers
     DevBuf<gbi::SysDesc> d_sys;
     DevBuf<gbi::SysGeom> geom;
-    DevBuf<int> flags_all, atom_sys, cell_sys;
+    DevBuf<int> flags_all, atom_sys, cell_sys, bin_order;
     DevBuf<int4> pair_tiles;
     int n_tiles = 0, max_splits = 1;
     DevBuf<double> pos, d_charge, d_radius, d_reach, quad_off, quad_pw, integrals, born, dbdi, dedi, pair_part,
@@ -150,7 +149,7 @@ struct gbi_ctx {
         for (auto* b : d) b->release();
         aparam.release();
         pair_tiles.release();
-        DevBuf<int>* ib[] = {&flags_all,      &atom_sys,     &cell_sys,       &cell_count, &cell_fill,
+        DevBuf<int>* ib[] = {&flags_all,      &atom_sys,     &cell_sys,       &bin_order,       &cell_count, &cell_fill,
                              &cell_start,     &cell_count_eff, &cell_atoms_tmp, &cell_atoms, &cell_pt_count,
                              &cell_act_count, &frec_base,    &worklist,       &worklist_sys, &work, &scan_tmp};
         for (auto* b : ib) b->release();
@@ -453,7 +452,7 @@ gbi::KParams make_kparams(gbi_ctx& c, bool adjoint) {
     k.n_tiles = c.n_tiles;
     k.max_splits = c.max_splits;
     k.adjoint = adjoint ? 1 : 0;
-    k.perm_stride = c.perm_stride;
+    k.bin_order = c.bin_order.p;
     k.persist_blocks = c.num_cus * std::max(1, c.tuning.blocks_per_cu);
     k.sys = c.d_sys.p;
     k.geom = c.geom.p;
@@ -753,13 +752,20 @@ gbi_status create_batch(int S, const int32_t* n_atoms, const double* charge, con
     CTX_HIP(hipMemcpyAsync(c->quad_pw.p, pw.data(), sizeof(double) * pw.size(), hipMemcpyHostToDevice, c->stream));
     CTX_HIP(hipMemcpyAsync(c->quad_r2.p, r2.data(), sizeof(float) * r2.size(), hipMemcpyHostToDevice, c->stream));
     CTX_HIP(hipMemcpyAsync(c->atom_sys.p, as.data(), sizeof(int) * N, hipMemcpyHostToDevice, c->stream));
-    // binning visits atoms with a stride coprime to N so concurrent workgroups hit distant cells
-    c->perm_stride = 1;
-    for (int cand : {7919, 7927, 7933, 7937, 7949})
-        if (c->N % cand != 0) {
-            c->perm_stride = cand % c->N == 0 ? 1 : cand;
-            break;
-        }
+    // binning visits the systems in turn, each with a stride coprime to its size so that
+    // concurrent workgroups hit distant cells of the same system's counters
+    std::vector<int> order(N);
+    for (const SysHost& h : c->sys) {
+        long long stride = 1;
+        for (int cand : {7919, 7927, 7933, 7937, 7949})
+            if (h.N % cand != 0) {
+                stride = cand;
+                break;
+            }
+        for (int k = 0; k < h.N; ++k) order[h.atom_begin + k] = h.atom_begin + (int)((k * stride) % h.N);
+    }
+    CTX_HIP(c->bin_order.ensure(N));
+    CTX_HIP(hipMemcpyAsync(c->bin_order.p, order.data(), sizeof(int) * N, hipMemcpyHostToDevice, c->stream));
     CTX_HIP(hipStreamSynchronize(c->stream));
     CTX_TRY(upload_atoms(*c));
     CTX_TRY(build_tiles(*c));

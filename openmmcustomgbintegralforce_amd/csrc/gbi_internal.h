@@ -109,7 +109,8 @@ struct KParams {
     int n_tiles;             // pair tiles
     int max_splits;          // pair_part planes
     int adjoint;             // also compute dE/dR, dE/dq, dE/dtheta
-    int perm_stride;         // atom visiting order of the binning pass (coprime to N)
+    const int* bin_order;    // N: atom visiting order of the binning pass (systems in turn, each
+                             // with a stride coprime to its size so concurrent workgroups hit distant cells)
     int persist_blocks;      // workgroups of the per-cell passes
     const SysDesc* sys;
     SysGeom* geom;

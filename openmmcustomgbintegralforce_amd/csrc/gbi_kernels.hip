@@ -524,8 +524,8 @@ __device__ inline void quad_point(const KParams& p, double xi, double yi, double
     pt[2] = zi + p.quad_off[3 * row + 2];
 }
 
-// One workgroup per atom (visited with a stride coprime to N so concurrent workgroups hit
-// distant cells): the cell of every quadrature point and its corner-relative FP32 record in
+// One workgroup per atom (systems in turn, each visited with a stride coprime to its size so
+// concurrent workgroups hit distant cells of one system's counters): the cell of every quadrature point and its corner-relative FP32 record in
 // the cell's slots.  An atom's points fall into about half as many distinct cells, so the
 // workgroup first counts its points per cell in an LDS hash table (local ranks), then
 // reserves each cell's slots with one global atomic.  Points outside the grid, in an empty
@@ -537,7 +537,7 @@ __global__ void __launch_bounds__(256) k_point_bin(KParams p) {
     __shared__ int t_key[kBinHash], t_cnt[kBinHash], t_base[kBinHash];
     if (batch_fatal(p)) return;
     const int tid = threadIdx.x;
-    const int i = (int)(((long long)blockIdx.x * p.perm_stride) % p.N);
+    const int i = p.bin_order[blockIdx.x];
     const int s = p.atom_sys[i];
     const SysDesc& sd = p.sys[s];
     const SysGeom& G = p.geom[s];
