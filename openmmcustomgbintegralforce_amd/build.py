@@ -30,6 +30,45 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+PLUGIN = os.path.join(PKG, "openmm_plugin")
+PLUGIN_LIB = os.path.join(LIBDIR, "libgbi_openmm.so")
+DRIVER = os.path.join(LIBDIR, "gbi_openmm_driver")
+
+
+def build_openmm_plugin(verbose: bool = False) -> str:
+    """The OpenMM kernel classes (HipCalcCharmmGB{SW,MV}ForceKernel, factory,
+    registerKernelFactories) as lib/libgbi_openmm.so, plus the test driver.  Built against
+    OpenMM when OPENMM_DIR and GBI_CHARMM_API_DIR (the plugin's openmmapi/include) are set,
+    otherwise against the OpenMM-shaped stand-in headers."""
+    cxx = shutil.which("g++") or "g++"
+    omm, api = os.environ.get("OPENMM_DIR"), os.environ.get("GBI_CHARMM_API_DIR")
+    if omm and api:
+        incs = ["-I", os.path.join(omm, "include"), "-I", api]
+        libs = ["-L", os.path.join(omm, "lib"), "-lOpenMM"]
+        defs = []
+    else:
+        incs = ["-I", os.path.join(PLUGIN, "stand_in")]
+        libs = []
+        defs = ["-DGBI_OPENMM_STAND_IN"]
+    common = [cxx, "-std=c++17", "-O2", "-fPIC", "-Wall", *defs, "-I", os.path.join(ROOT, "include"),
+              "-I", os.path.join(PLUGIN, "src"), *incs]
+    srcs = [os.path.join(PLUGIN, "src", f) for f in ("HipCharmmKernels.cpp", "HipCharmmKernelFactory.cpp")]
+    tmp = PLUGIN_LIB + ".tmp"
+    cmd = common + ["-shared", "-o", tmp, *srcs, "-L", LIBDIR, "-lgbi", *libs, "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, PLUGIN_LIB)
+    if not (omm and api):
+        cmd = common + ["-o", DRIVER + ".tmp", os.path.join(PLUGIN, "driver", "gbi_openmm_driver.cpp"), "-L", LIBDIR,
+                        "-lgbi_openmm", "-lgbi", "-Wl,-rpath,$ORIGIN"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(DRIVER + ".tmp", DRIVER)
+    return PLUGIN_LIB
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     objs = []
@@ -57,6 +96,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_openmm_plugin(verbose)
     return LIB
 
 

@@ -1,0 +1,141 @@
+// Runs one CharmmGBSWForce or canonical CharmmGBMVForce through the HIP kernels the way
+// OpenMM does: the kernel is created by name from the "HIP" platform's registered factory,
+// then initialize / execute / copyParametersToContext.  Used by the tests.
+//
+// Input (binary, native endianness): int32 model (0 CharmmGBSWForce, 1 CharmmGBMVForce),
+// int32 integral type, int32 Lebedev rule (0 = the reference's), int32 N, then N x 3
+// positions, N charges, N radii (doubles).  Output (text): energy, forces, the energy after
+// copyParametersToContext with charges x 1.5, and the outcome of the error paths.
+#include <cstdio>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "HipCharmmKernels.h"
+#include "openmm/Platform.h"
+
+extern "C" void registerKernelFactories();
+
+using namespace OpenMM;
+
+namespace {
+
+template <class T>
+void rd(FILE* f, T* p, size_t n) {
+    if (fread(p, sizeof(T), n, f) != n) throw std::runtime_error("short input");
+}
+
+CharmmGBMVForce* canonicalGBMV(const std::vector<double>& q, const std::vector<double>& r, int type, int rule) {
+    auto* f = new CharmmGBMVForce();
+    f->addPerParticleParameter("q");
+    f->addPerParticleParameter("radius");
+    f->addGlobalParameter("alpha0", -0.180);
+    f->addGlobalParameter("alpha1", 1.817);
+    f->addGlobalParameter("eta", 0.05);
+    f->addGlobalParameter("slope", 1.0);
+    f->addGlobalParameter("shift", 0.0);
+    f->addGlobalParameter("P", -138.935456 * (1.0 - 1.0 / 80.0));
+    std::vector<int> i4 = {4}, i7 = {7};
+    if (rule > 0) {
+        i4.push_back(rule);
+        i7.push_back(rule);
+    }
+    f->addGBIntegral("I4", i4);
+    f->addGBIntegral("I7", i7);
+    f->setGBIntegralType((CharmmGBMVForce::GBIntegralType)type);
+    f->addComputedValue("B", gbi_openmm::kCanonicalBornValue, CharmmGBMVForce::SingleParticle);
+    f->addEnergyTerm(gbi_openmm::kCanonicalSelfEnergy, CharmmGBMVForce::SingleParticle);
+    f->addEnergyTerm(gbi_openmm::kCanonicalPairEnergy, CharmmGBMVForce::ParticlePairNoExclusions);
+    for (size_t i = 0; i < q.size(); ++i) f->addParticle({q[i], r[i]});
+    return f;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc != 3) {
+        fprintf(stderr, "usage: %s input.bin output.txt\n", argv[0]);
+        return 2;
+    }
+    try {
+        FILE* in = fopen(argv[1], "rb");
+        if (!in) throw std::runtime_error("cannot open input");
+        int32_t hdr[4];
+        rd(in, hdr, 4);
+        const int model = hdr[0], type = hdr[1], rule = hdr[2], n = hdr[3];
+        std::vector<double> x(3 * (size_t)n), q(n), r(n);
+        rd(in, x.data(), x.size());
+        rd(in, q.data(), q.size());
+        rd(in, r.data(), r.size());
+        fclose(in);
+
+        Platform::registerPlatform(new Platform("HIP"));
+        registerKernelFactories();
+        Platform& platform = Platform::getPlatformByName("HIP");
+        System system;
+        for (int i = 0; i < n; ++i) system.addParticle(12.0);
+        ContextImpl context(n);
+        std::vector<Vec3> pos(n);
+        for (int i = 0; i < n; ++i) pos[i] = Vec3(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+        context.setPositions(pos);
+
+        FILE* out = fopen(argv[2], "w");
+        if (!out) throw std::runtime_error("cannot open output");
+        double e = 0.0, e_scaled = 0.0, e_noforce = 0.0;
+        std::vector<double> qs(q);
+        for (double& v : qs) v *= 1.5;
+        if (model == 0) {
+            CharmmGBSWForce force;
+            for (int i = 0; i < n; ++i) force.addParticle(q[i], r[i], 1.0);
+            std::unique_ptr<KernelImpl> k(platform.createKernel(CalcCharmmGBSWForceKernel::Name(), context));
+            auto& kernel = dynamic_cast<CalcCharmmGBSWForceKernel&>(*k);
+            kernel.initialize(system, force);
+            e = kernel.execute(context, true, true);
+            std::vector<Vec3> f = context.getForces();
+            fprintf(out, "energy %.17g\n", e);
+            for (int i = 0; i < n; ++i) fprintf(out, "%.17g %.17g %.17g\n", f[i][0], f[i][1], f[i][2]);
+            e_noforce = kernel.execute(context, false, true);
+            for (int i = 0; i < n; ++i) force.setParticleParameters(i, qs[i], r[i], 1.0);
+            kernel.copyParametersToContext(context, force);
+            e_scaled = kernel.execute(context, false, true);
+            CharmmGBSWForce shorter;
+            for (int i = 0; i + 1 < n; ++i) shorter.addParticle(q[i], r[i], 1.0);
+            int count_throws = 0;
+            try {
+                kernel.copyParametersToContext(context, shorter);
+            } catch (const OpenMMException&) {
+                count_throws = 1;
+            }
+            fprintf(out, "count_change_throws %d\n", count_throws);
+        } else {
+            std::unique_ptr<CharmmGBMVForce> force(canonicalGBMV(q, r, type, rule));
+            std::unique_ptr<KernelImpl> k(platform.createKernel(CalcCharmmGBMVForceKernel::Name(), context));
+            auto& kernel = dynamic_cast<CalcCharmmGBMVForceKernel&>(*k);
+            kernel.initialize(system, *force);
+            e = kernel.execute(context, true, true);
+            std::vector<Vec3> f = context.getForces();
+            fprintf(out, "energy %.17g\n", e);
+            for (int i = 0; i < n; ++i) fprintf(out, "%.17g %.17g %.17g\n", f[i][0], f[i][1], f[i][2]);
+            e_noforce = kernel.execute(context, false, true);
+            std::unique_ptr<CharmmGBMVForce> scaled(canonicalGBMV(qs, r, type, rule));
+            kernel.copyParametersToContext(context, *scaled);
+            e_scaled = kernel.execute(context, false, true);
+            fprintf(out, "count_change_throws 1\n");
+        }
+        fprintf(out, "energy_no_forces %.17g\n", e_noforce);
+        fprintf(out, "energy_scaled %.17g\n", e_scaled);
+        int illegal = 0;
+        try {
+            platform.createKernel("CalcSomethingElse", context);
+        } catch (const OpenMMException&) {
+            illegal = 1;
+        }
+        fprintf(out, "illegal_name_throws %d\n", illegal);
+        fclose(out);
+    } catch (const std::exception& ex) {
+        fprintf(stderr, "gbi_openmm_driver: %s\n", ex.what());
+        return 1;
+    }
+    return 0;
+}

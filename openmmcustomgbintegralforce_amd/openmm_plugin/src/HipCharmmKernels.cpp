@@ -1,0 +1,201 @@
+// HIP kernels for CharmmGBSWForce / CharmmGBMVForce on the gbi C ABI.
+#include "HipCharmmKernels.h"
+
+#include <algorithm>
+#include <cctype>
+
+#include "openmm/OpenMMException.h"
+
+using namespace OpenMM;
+
+namespace gbi_openmm {
+
+const char* const kCanonicalBornValue = "slope*1/(alpha0*(1/eta-I4)+alpha1*(1/(4*eta^4)-I7)^(1/4))+shift";
+const char* const kCanonicalSelfEnergy = "0.5*P*q^2/B";
+const char* const kCanonicalPairEnergy = "P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))";
+
+namespace {
+
+std::string strip(const std::string& s) {
+    std::string out;
+    for (char c : s)
+        if (!std::isspace((unsigned char)c)) out.push_back(c);
+    return out;
+}
+
+int lebedevPoints(int rule) {
+    static const int pts[] = {0, 6, 14, 26, 38, 50};
+    if (rule < 1 || rule > 5) throw OpenMMException("CharmmGBMVForce: Lebedev rule must be 1..5 on the HIP platform");
+    return pts[rule];
+}
+
+}  // namespace
+
+HipGBKernelBase::~HipGBKernelBase() { gbi_destroy(ctx_); }
+
+void HipGBKernelBase::check(gbi_status status) {
+    if (status != GBI_OK) throw OpenMMException(gbi_last_error());
+}
+
+void HipGBKernelBase::create(int n, const std::vector<double>& charge, const std::vector<double>& radius,
+                             const gbi_params& params) {
+    gbi_destroy(ctx_);
+    ctx_ = nullptr;
+    n_ = n;
+    check(gbi_create(n, charge.data(), radius.data(), &params, &ctx_));
+}
+
+void HipGBKernelBase::update(const std::vector<double>& charge, const std::vector<double>& radius) {
+    // status GBI_ERR_PARTICLE_COUNT carries the reference's message (CharmmReferenceKernels.cpp:373-374)
+    check(gbi_update_parameters(ctx_, (int32_t)charge.size(), charge.data(), radius.data()));
+}
+
+double HipGBKernelBase::run(ContextImpl& context, bool includeForces, bool includeEnergy) {
+    std::vector<Vec3> pos;
+    context.getPositions(pos);
+    if ((int)pos.size() != n_) throw OpenMMException("HIP GB kernel: the Context's particle count changed");
+    std::vector<double> x(3 * (size_t)n_), f(3 * (size_t)n_, 0.0);
+    for (int a = 0; a < n_; ++a)
+        for (int d = 0; d < 3; ++d) x[3 * a + d] = pos[a][d];
+    double energy = 0.0;
+    check(gbi_execute(ctx_, x.data(), includeForces ? 1 : 0, includeEnergy ? 1 : 0, &energy,
+                      includeForces ? f.data() : nullptr));
+    if (includeForces) addForces(context, f);
+    return includeEnergy ? energy : 0.0;
+}
+
+void HipGBKernelBase::addForces(ContextImpl& context, const std::vector<double>& forces) {
+#ifdef GBI_OPENMM_STAND_IN
+    std::vector<Vec3>& acc = context.getForces();
+    for (int a = 0; a < n_; ++a) acc[a] += Vec3(forces[3 * a], forces[3 * a + 1], forces[3 * a + 2]);
+#else
+    (void)context;
+    (void)forces;
+    throw OpenMMException("HIP GB kernel: the platform must override addForces() to reach its force buffer");
+#endif
+}
+
+// ---------------------------------------------------------------------------- GBSW
+
+void HipCalcCharmmGBSWForceKernel::initialize(const System& system, const CharmmGBSWForce& force) {
+    const int n = force.getNumParticles();
+    if (system.getNumParticles() != n)  // CharmmGBSWForceImpl.cpp:47-48
+        throw OpenMMException("CharmmGBSWForce must have exactly as many particles as the System it belongs to.");
+    if (force.getNonbondedMethod() != CharmmGBSWForce::NoCutoff)
+        throw OpenMMException("CharmmGBSWForce: only NoCutoff is supported on the HIP platform");
+    std::vector<double> q(n), r(n);
+    for (int i = 0; i < n; ++i) {
+        double s;
+        force.getParticleParameters(i, q[i], r[i], s);
+    }
+    gbi_params p;
+    check(gbi_params_default(GBI_MODEL_GBSW, 0, &p));
+    if (honor_) p.prefactor = -138.935456 * (1.0 / force.getSoluteDielectric() - 1.0 / force.getSolventDielectric());
+    create(n, q, r, p);
+}
+
+double HipCalcCharmmGBSWForceKernel::execute(ContextImpl& context, bool includeForces, bool includeEnergy) {
+    return run(context, includeForces, includeEnergy);
+}
+
+void HipCalcCharmmGBSWForceKernel::copyParametersToContext(ContextImpl& context, const CharmmGBSWForce& force) {
+    const int n = force.getNumParticles();
+    std::vector<double> q(n), r(n);
+    for (int i = 0; i < n; ++i) {
+        double s;
+        force.getParticleParameters(i, q[i], r[i], s);
+    }
+    update(q, r);
+    context.systemChanged();
+}
+
+// ---------------------------------------------------------------------------- GBMV
+
+void gbmvParamsFromForce(const CharmmGBMVForce& force, gbi_params& p, std::vector<double>& charge,
+                         std::vector<double>& radius) {
+    const CharmmGBMVForce::GBIntegralType type = force.getGBIntegralType();
+    if (type == CharmmGBMVForce::CustomIntegral)  // CharmmReferenceKernels.cpp:129-133
+        throw OpenMMException("CharmmGBMVForce: CustomIntegral (type 3) is not implemented by the reference either");
+    if (force.getNonbondedMethod() != CharmmGBMVForce::NoCutoff)
+        throw OpenMMException("CharmmGBMVForce: only NoCutoff is supported on the HIP platform");
+    int iq = -1, ir = -1;
+    for (int k = 0; k < force.getNumPerParticleParameters(); ++k) {
+        if (force.getPerParticleParameterName(k) == "q") iq = k;
+        if (force.getPerParticleParameterName(k) == "radius") ir = k;
+    }
+    if (ir < 0)  // GBSWIntegral.cpp:79-80
+        throw OpenMMException("GBSWIntegral: the per perticle parameter 'radius' must be defined");
+    if (iq < 0) throw OpenMMException("CharmmGBMVForce: the canonical energy needs a per-particle parameter 'q'");
+    std::vector<std::vector<int>> ints(force.getNumGBIntegrals());
+    for (int k = 0; k < force.getNumGBIntegrals(); ++k) {
+        std::string name;
+        force.getGBIntegralParameters(k, name, ints[k]);
+        if (ints[k].empty()) throw OpenMMException("GBSWIntegral: the order of integral must be given");
+        if (ints[k][0] < 2) throw OpenMMException("GBSWIntegral: the order of integral must be greater or equal to 2");
+    }
+    if (ints.size() != 2 || !((ints[0][0] == 4 && ints[1][0] == 7) || (ints[0][0] == 7 && ints[1][0] == 4)))
+        throw OpenMMException("CharmmGBMVForce: the HIP platform implements the canonical Born map over integrals of order 4 and 7");
+    bool canonical = force.getNumComputedValues() == 1 && force.getNumEnergyTerms() == 2;
+    if (canonical) {
+        std::string name, expr;
+        CharmmGBMVForce::ComputationType t;
+        force.getComputedValueParameters(0, name, expr, t);
+        canonical = name == "B" && strip(expr) == kCanonicalBornValue && t == CharmmGBMVForce::SingleParticle;
+        force.getEnergyTermParameters(0, expr, t);
+        canonical = canonical && strip(expr) == kCanonicalSelfEnergy && t == CharmmGBMVForce::SingleParticle;
+        force.getEnergyTermParameters(1, expr, t);
+        canonical = canonical && strip(expr) == kCanonicalPairEnergy && t == CharmmGBMVForce::ParticlePairNoExclusions;
+    }
+    if (!canonical) throw OpenMMException("CharmmGBMVForce: only the canonical GB expressions run on the HIP platform");
+    if (gbi_params_default(GBI_MODEL_GBMV, (int32_t)type, &p) != GBI_OK) throw OpenMMException(gbi_last_error());
+    if (ints[0].size() > 1) p.n_lebedev = lebedevPoints(ints[0][1]);
+    if (ints[0].size() > 2) p.n_radial = ints[0][2];
+    p.orders[0] = ints[0][0];
+    p.orders[1] = ints[1][0];
+    bool seen[6] = {false, false, false, false, false, false};
+    for (int k = 0; k < force.getNumGlobalParameters(); ++k) {
+        const std::string& g = force.getGlobalParameterName(k);
+        const double v = force.getGlobalParameterDefaultValue(k);
+        if (g == "alpha0") p.alpha0 = v, seen[0] = true;
+        else if (g == "alpha1") p.alpha1 = v, seen[1] = true;
+        else if (g == "eta") p.eta = v, seen[2] = true;
+        else if (g == "slope") p.slope = v, seen[3] = true;
+        else if (g == "shift") p.shift = v, seen[4] = true;
+        else if (g == "P") p.prefactor = v, seen[5] = true;
+    }
+    if (!std::all_of(seen, seen + 6, [](bool b) { return b; }))
+        throw OpenMMException("CharmmGBMVForce: global parameters alpha0, alpha1, eta, slope, shift and P must be defined");
+    const int n = force.getNumParticles();
+    charge.resize(n);
+    radius.resize(n);
+    for (int i = 0; i < n; ++i) {
+        std::vector<double> v;
+        force.getParticleParameters(i, v);
+        if ((int)v.size() <= std::max(iq, ir)) throw OpenMMException("CharmmGBMVForce: a particle lacks parameters");
+        charge[i] = v[iq];
+        radius[i] = v[ir];
+    }
+}
+
+void HipCalcCharmmGBMVForceKernel::initialize(const System& system, const CharmmGBMVForce& force) {
+    if (system.getNumParticles() != force.getNumParticles())  // CharmmGBMVForceImpl.cpp:58-59
+        throw OpenMMException("CharmmGBMVForce must have exactly as many particles as the System it belongs to.");
+    gbi_params p;
+    std::vector<double> q, r;
+    gbmvParamsFromForce(force, p, q, r);
+    create(force.getNumParticles(), q, r, p);
+}
+
+double HipCalcCharmmGBMVForceKernel::execute(ContextImpl& context, bool includeForces, bool includeEnergy) {
+    return run(context, includeForces, includeEnergy);
+}
+
+void HipCalcCharmmGBMVForceKernel::copyParametersToContext(ContextImpl& context, const CharmmGBMVForce& force) {
+    gbi_params p;
+    std::vector<double> q, r;
+    gbmvParamsFromForce(force, p, q, r);
+    update(q, r);
+    context.systemChanged();
+}
+
+}  // namespace gbi_openmm

@@ -1,0 +1,75 @@
+// OpenMM kernel classes of the HIP platform for CharmmGBSWForce and CharmmGBMVForce,
+// implemented on the C ABI of include/gbi.h.  They replace the Reference platform's
+// ReferenceCalcCharmmGB{SW,MV}ForceKernel (CharmmReferenceKernels.h) and the CUDA
+// platform's CudaCalcCharmmGBMVForceKernel behind the interfaces of
+// openmmapi/include/openmm/charmmKernels.h:49-129.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "gbi.h"
+#include "openmm/CharmmGBMVForce.h"
+#include "openmm/CharmmGBSWForce.h"
+#include "openmm/charmmKernels.h"
+
+namespace gbi_openmm {
+
+// Base of both kernels: one gbi context, positions in, forces added out.
+class HipGBKernelBase {
+public:
+    virtual ~HipGBKernelBase();
+    gbi_ctx* gbiContext() const { return ctx_; }
+
+protected:
+    double run(OpenMM::ContextImpl& context, bool includeForces, bool includeEnergy);
+    // Adds N x 3 forces (kJ/mol/nm) to the platform's force accumulator.  The OpenMM-shaped
+    // stand-in ContextImpl keeps a host array; a HIP OpenMM platform overrides this and adds
+    // gbi_get_device_view()'s int64 fixed-point forces to its own buffer on the device.
+    virtual void addForces(OpenMM::ContextImpl& context, const std::vector<double>& forces);
+    static void check(gbi_status status);
+    void create(int n, const std::vector<double>& charge, const std::vector<double>& radius, const gbi_params& params);
+    void update(const std::vector<double>& charge, const std::vector<double>& radius);
+
+    gbi_ctx* ctx_ = nullptr;
+    int n_ = 0;
+};
+
+// CalcCharmmGBSWForceKernel on HIP (CharmmReferenceKernels.cpp:380-445 is the Reference one).
+class HipCalcCharmmGBSWForceKernel : public OpenMM::CalcCharmmGBSWForceKernel, public HipGBKernelBase {
+public:
+    // honorDielectrics: use (1/soluteDielectric - 1/solventDielectric); the reference
+    // hard-codes (1 - 1/80) (CharmmReferenceGBSW.cpp:177-178), which stays the default.
+    HipCalcCharmmGBSWForceKernel(std::string name, const OpenMM::Platform& platform, bool honorDielectrics = false)
+        : CalcCharmmGBSWForceKernel(name, platform), honor_(honorDielectrics) {}
+    void initialize(const OpenMM::System& system, const OpenMM::CharmmGBSWForce& force) override;
+    double execute(OpenMM::ContextImpl& context, bool includeForces, bool includeEnergy) override;
+    void copyParametersToContext(OpenMM::ContextImpl& context, const OpenMM::CharmmGBSWForce& force) override;
+
+private:
+    bool honor_;
+};
+
+// CalcCharmmGBMVForceKernel on HIP for the canonical GB energy (integrals of order 4 and 7,
+// B = slope/(alpha0 (1/eta - I4) + alpha1 (1/(4 eta^4) - I7)^(1/4)) + shift, STILL energy);
+// other expressions throw.
+class HipCalcCharmmGBMVForceKernel : public OpenMM::CalcCharmmGBMVForceKernel, public HipGBKernelBase {
+public:
+    HipCalcCharmmGBMVForceKernel(std::string name, const OpenMM::Platform& platform)
+        : CalcCharmmGBMVForceKernel(name, platform) {}
+    void initialize(const OpenMM::System& system, const OpenMM::CharmmGBMVForce& force) override;
+    double execute(OpenMM::ContextImpl& context, bool includeForces, bool includeEnergy) override;
+    void copyParametersToContext(OpenMM::ContextImpl& context, const OpenMM::CharmmGBMVForce& force) override;
+};
+
+// The canonical CharmmGBMVForce expressions recognised by HipCalcCharmmGBMVForceKernel.
+extern const char* const kCanonicalBornValue;  // computed value "B"
+extern const char* const kCanonicalSelfEnergy;
+extern const char* const kCanonicalPairEnergy;
+
+// gbi_params, charges and radii of a canonical CharmmGBMVForce; throws OpenMMException
+// otherwise.
+void gbmvParamsFromForce(const OpenMM::CharmmGBMVForce& force, gbi_params& params, std::vector<double>& charge,
+                         std::vector<double>& radius);
+
+}  // namespace gbi_openmm

@@ -1,0 +1,38 @@
+"""The C++ OpenMM kernel classes (lib/libgbi_openmm.so) on the CPU: plugin entry points,
+and the driver's error path without a device (no compute)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+PKG = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "openmmcustomgbintegralforce_amd")
+PLUGIN = os.path.join(PKG, "lib", "libgbi_openmm.so")
+DRIVER = os.path.join(PKG, "lib", "gbi_openmm_driver")
+
+
+def write_input(path, model, itype, rule, pos, q, R):
+    with open(path, "wb") as fh:
+        np.array([model, itype, rule, len(q)], dtype=np.int32).tofile(fh)
+        np.ascontiguousarray(pos, dtype=np.float64).tofile(fh)
+        np.ascontiguousarray(q, dtype=np.float64).tofile(fh)
+        np.ascontiguousarray(R, dtype=np.float64).tofile(fh)
+
+
+def test_plugin_exports_openmm_entry_points():
+    lib = ctypes.CDLL(PLUGIN)
+    for name in ("registerPlatforms", "registerKernelFactories", "registerCharmmHipKernelFactories"):
+        assert hasattr(lib, name), name
+
+
+def test_driver_reports_missing_device(tmp_path):
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a device is present: the GPU test covers the driver")
+    inp, out = tmp_path / "in.bin", tmp_path / "out.txt"
+    write_input(inp, 0, 0, 0, np.zeros((2, 3)) + [[0, 0, 0], [0.3, 0, 0]], [0.5, -0.5], [0.2, 0.2])
+    r = subprocess.run([DRIVER, str(inp), str(out)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "no HIP device" in r.stderr
