@@ -316,7 +316,7 @@ class Engine:
         _check(self._lib.gbi_autotune(self._h, int(evals), ctypes.byref(t)))
         return {f: getattr(t, f) for f, _ in Tuning._fields_}
 
-    def set_tuning(self, blocks_per_cu: int = 32, pair_columns: int = 0, tight_grid_length: float = 0.0) -> None:
+    def set_tuning(self, blocks_per_cu: int = 128, pair_columns: int = 0, tight_grid_length: float = 0.0) -> None:
         t = Tuning(blocks_per_cu, pair_columns, tight_grid_length, 0.0)
         _check(self._lib.gbi_set_tuning(self._h, ctypes.byref(t)))
 

@@ -46,7 +46,7 @@ gbi_status fail(gbi_status st, const std::string& msg) {
 constexpr double kOne4PiEps0 = 138.935456;  // OpenMM 7.x SimTKOpenMMRealType.h
 constexpr double kTightMargin = 1e-6;       // nm, guards the exact GBSW lists against rounding
 constexpr long long kMaxCells = 16ll * 1024 * 1024;  // limit of the three-pass scan
-constexpr int kDefaultBlocksPerCU = 32;
+constexpr int kDefaultBlocksPerCU = 128;  // about one workgroup per occupied cell: the dispatcher balances them
 
 template <class T>
 struct DevBuf {
@@ -1073,7 +1073,7 @@ gbi_status gbi_autotune(gbi_ctx* ctx, int32_t evals, gbi_tuning* out) {
     const double grids_tight[] = {0.15, 0.12, 0.2};
     for (double g : grids_tight) {
         if (!tight && g != 0.15) continue;
-        for (int bpc : {8, 16, 32, 64})
+        for (int bpc : {16, 32, 64, 128, 256})
             for (int cols : {0, 1024, 16384}) {
                 gbi_tuning t{};
                 t.blocks_per_cu = bpc;

@@ -712,6 +712,7 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                     __syncthreads();
                 }
                 if (VK == VOL_GBSW) {
+                    #pragma unroll 4
                     for (int m = 0; m < cnt; ++m) {
                         if (__all(done)) break;
                         const float dx = s_x[m] - prx, dy = s_y[m] - pry, dz = s_z[m] - prz;
@@ -741,6 +742,7 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                     }
                 } else if (VK == VOL_GBMV2) {
                     if (valid) {
+                        #pragma unroll 4
                         for (int m = 0; m < cnt; ++m) {
                             const float dx = s_x[m] - prx, dy = s_y[m] - pry, dz = s_z[m] - prz;
                             const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
@@ -756,6 +758,7 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                     }
                 } else {
                     if (valid) {
+                        #pragma unroll 4
                         for (int m = 0; m < cnt; ++m) {
                             const float dx = s_x[m] - prx, dy = s_y[m] - pry, dz = s_z[m] - prz;
                             const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
@@ -1412,7 +1415,7 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
                          FLAG_ENTRY_OVERFLOW, stream);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(k_grid_fill, dim3(p.N), dim3(256), 0, stream, p);
-    hipLaunchKernelGGL(k_grid_sort, dim3(std::min(p.cell_total, 8192)), dim3(64), 0, stream, p);
+    hipLaunchKernelGGL(k_grid_sort, dim3(std::max(1, std::min(p.cell_total, 65536))), dim3(64), 0, stream, p);
     mark(1);
     hipLaunchKernelGGL(k_point_bin, dim3(p.N), dim3(256), 0, stream, p);
     // records of the active points of cell c start at the exclusive prefix sum of the point counts
