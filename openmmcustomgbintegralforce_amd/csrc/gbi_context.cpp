@@ -111,7 +111,7 @@ struct gbi_ctx {
     DevBuf<int> cell_count, cell_fill, cell_start, cell_count_eff, cell_atoms_tmp, cell_atoms, cell_pt_count,
         cell_act_count, frec_base, worklist, worklist_sys, work, scan_tmp;
     DevBuf<float4> ent_pos, ent_par, prec, frec0, frec1, frec2, vrec;
-    DevBuf<float> quad_r2, vsig;
+    DevBuf<float> quad_r2, vsig, vval;
     DevBuf<long long> force_fix, dedr_fix, vacc_fix;
 
     // graph replay (index: adjoint evaluation)
@@ -157,6 +157,7 @@ struct gbi_ctx {
         for (auto* b : fb) b->release();
         quad_r2.release();
         vsig.release();
+        vval.release();
         force_fix.release();
         dedr_fix.release();
         vacc_fix.release();
@@ -487,6 +488,7 @@ gbi::KParams make_kparams(gbi_ctx& c, bool adjoint) {
     k.frec1 = c.frec1.p;
     k.frec2 = c.frec2.p;
     k.vrec = c.vrec.p;
+    k.vval = c.vval.p;
     k.vsig = c.vsig.p;
     k.integrals = c.integrals.p;
     k.born = c.born.p;
@@ -728,6 +730,7 @@ gbi_status create_batch(int S, const int32_t* n_atoms, const double* charge, con
     CTX_HIP(c->frec1.ensure(NQ));
     CTX_HIP(c->frec2.ensure(c->kinds[gbi::VOL_GBMV2] ? NQ : 1));
     CTX_HIP(c->vrec.ensure(NQ));
+    CTX_HIP(c->vval.ensure(NQ));
     CTX_HIP(c->force_fix.ensure(3 * N));
     // quadrature tables: offsets and w_q / r_q^(order-2) per integral, one block of rows per system
     std::vector<double> off(3 * (size_t)qrow), pw(2 * (size_t)qrow);

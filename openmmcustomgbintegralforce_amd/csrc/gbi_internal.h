@@ -148,7 +148,8 @@ struct KParams {
     float4* frec0;           // NQ: active points, grouped by cell (copy of prec)
     float4* frec1;           // NQ: GBSW (V, 0, 0, 0) / GBMV1 (dV/dS, V, 0, 0) / GBMV2 (c1, c2, c3, vx)
     float4* frec2;           // NQ: GBMV2 (vy, vz, V, 0)
-    float4* vrec;            // NQ by pid: (V, owner vector)
+    float* vval;             // NQ by pid: V, sign bit set on the points with a chain-rule factor
+    float4* vrec;            // NQ by pid: owner force of those points only (others stale), .w unused
     float* vsig;             // NQ (adjoint, GBMV) by pid: S of the sigmoid, NaN where V has no derivative
     // per atom
     double* integrals;       // 2 x N

@@ -525,11 +525,14 @@ __device__ inline void quad_point(const KParams& p, double xi, double yi, double
 }
 
 // One workgroup per atom (systems in turn, each visited with a stride coprime to its size so
-// concurrent workgroups hit distant cells of one system's counters): the cell of every quadrature point and its corner-relative FP32 record in
-// the cell's slots.  An atom's points fall into about half as many distinct cells, so the
-// workgroup first counts its points per cell in an LDS hash table (local ranks), then
-// reserves each cell's slots with one global atomic.  Points outside the grid, in an empty
-// cell, or (GBSW) inside the owner's own burial radius get their volume directly.
+// concurrent workgroups hit distant cells of one system's counters): the cell of every
+// quadrature point and its corner-relative FP32 record in the cell's slots.  An atom's points
+// fall into about half as many distinct cells, so the workgroup first counts its points per
+// cell in an LDS hash table (local ranks), then reserves each cell's slots with one global
+// atomic; the global atomics bound this pass (about 26 G/s on MI355X, whatever the scope or
+// the order of issue: issuing a thread's atomics together was measured slower).  Points
+// outside the grid, in an empty cell, or (GBSW) inside the owner's own burial radius get
+// their volume directly.
 constexpr int kBinHash = 2048;                      // > Q: the table never fills
 constexpr int kBinPer = kBinHash / 256;             // points per thread
 
@@ -575,7 +578,7 @@ __global__ void __launch_bounds__(256) k_point_bin(KParams p) {
             if (c >= 0 && p.cell_count_eff[cb + c] == 0) c = -1;
         }
         if (c < 0) {
-            p.vrec[pbase + q] = make_float4(owner_buried ? 0.0f : v_empty, 0.f, 0.f, 0.f);
+            p.vval[pbase + q] = owner_buried ? 0.0f : v_empty;
             if (p.adjoint) p.vsig[pbase + q] = owner_buried ? __int_as_float(0x7fc00000) : sig_empty;
             continue;
         }
@@ -646,18 +649,23 @@ __global__ void k_worklist(KParams p) {
 
 // ---------------------------------------------------------------------------
 // Volume pass: one wave per cell, lanes = quadrature points, the cell's atoms staged in
-// LDS (corner-relative FP32).  Writes V per point, the compacted records of the points
-// that carry a chain-rule factor, and (GBSW) the owner gradient dV/dp.
+// LDS (corner-relative FP32).  Writes V per point (vval, by point; stored negated, -0 for
+// V = 0, on the points that carry a chain-rule factor) and their compacted records.
 // GBSW: V = prod_j H_j, early exit on burial (GBSWIntegral.cpp:187-224);
 // GBMV-I: S = sum exp(gamma_j d^4) (GBMVIntegralTypeI.cpp:155-173);
 // GBMV-II: sum1, sum2, v, S = S0 sum1 sum2/|v|^2 (GBMVIntegralTypeII.cpp:163-191).
 // ---------------------------------------------------------------------------
 
+// V >= 0 with the sign bit as the "has a chain-rule factor" mark (also set for V = 0)
+__device__ inline float mark_active(float v, bool active) {
+    return __int_as_float(__float_as_int(v) | (active ? (int)0x80000000 : 0));
+}
+__device__ inline float volume_of(float vv) { return __int_as_float(__float_as_int(vv) & 0x7fffffff); }
+
 template <int VK>
 __global__ void __launch_bounds__(64) k_volume(KParams p) {
     __shared__ float s_x[kMaxAtomChunk], s_y[kMaxAtomChunk], s_z[kMaxAtomChunk];
     __shared__ float s_a[kMaxAtomChunk], s_b[kMaxAtomChunk], s_c[kMaxAtomChunk];
-    __shared__ int s_j[kMaxAtomChunk];
     if (batch_fatal(p)) return;
     const int lane = threadIdx.x;
     const int nwork = p.work[VK];
@@ -683,7 +691,6 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                 s_x[k] = e.x;
                 s_y[k] = e.y;
                 s_z[k] = e.z;
-                s_j[k] = __float_as_int(e.w);
                 s_a[k] = q.x;
                 s_b[k] = q.y;
                 s_c[k] = q.z;
@@ -703,7 +710,6 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
             float V = 1.f;
             bool done = !valid;
             float s1 = 0.f, s2 = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;
-            float lx = 0.f, ly = 0.f, lz = 0.f;
             for (int ab = 0; ab < na; ab += kMaxAtomChunk) {
                 const int cnt = min(kMaxAtomChunk, na - ab);
                 if (!single) {
@@ -730,12 +736,6 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                                     V = 0.f;
                                 } else {
                                     V *= h;
-                                    if (s_j[m] != i) {
-                                        const float f = fmaf(-3.f * k3 * x, x, k1) * fast_rcp(h) * fast_rcp(d);
-                                        lx = fmaf(f, dx, lx);
-                                        ly = fmaf(f, dy, ly);
-                                        lz = fmaf(f, dz, lz);
-                                    }
                                 }
                             }
                         }
@@ -773,8 +773,7 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                 const int pid = pt_base + (i - abeg) * Q + q;
                 if (VK == VOL_GBSW) {
                     active = V > 0.f && V < 1.f;
-                    const float gs = active ? -V : 0.f;  // dV/dp = -V sum_{j != owner} (H'/H) (x_j - p)/d
-                    p.vrec[pid] = make_float4(V, gs * lx, gs * ly, gs * lz);
+                    p.vval[pid] = mark_active(V, active);
                     r1.x = V;
                 } else if (VK == VOL_GBMV2) {
                     const float vsq = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
@@ -791,16 +790,16 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                         r2 = make_float4(vy, vz, Vv, 0.f);
                         active = true;
                     }
-                    p.vrec[pid] = make_float4(Vv, 0.f, 0.f, 0.f);  // the force pass fills the owner force
+                    p.vval[pid] = mark_active(Vv, active);
                     if (p.adjoint) p.vsig[pid] = sat ? __int_as_float(0x7fc00000) : S;
                 } else {
                     const float pre = __expf(beta * (s1 - lambda));
                     const float opre = 1.f + pre;
                     const float Vv = 1.f / opre;
-                    p.vrec[pid] = make_float4(Vv, 0.f, 0.f, 0.f);
                     r1.x = -beta * pre / (opre * opre);
                     r1.y = Vv;
                     active = r1.x != 0.f;
+                    p.vval[pid] = mark_active(Vv, active);
                     if (p.adjoint) p.vsig[pid] = s1;
                 }
             }
@@ -835,7 +834,7 @@ __global__ void __launch_bounds__(64) k_integrals(KParams p) {
     double a0 = 0.0, a1 = 0.0;
     const bool gbsw = sd.vol_kind == VOL_GBSW;
     for (int q = lane; q < Q; q += kWave) {
-        const double v = (double)p.vrec[pbase + q].x;
+        const double v = (double)volume_of(p.vval[pbase + q]);
         const double f = gbsw ? 1.0 - v : v;
         a0 += p.quad_pw[qb + q] * f;
         a1 += p.quad_pw[p.quad_rows + qb + q] * f;
@@ -1017,7 +1016,7 @@ __global__ void __launch_bounds__(1024) k_energy(KParams p) {
 // 8 points of each 64-point chunk (LDS).  For a point p of owner i with
 // fac = sum_k dE/dI_k(i) w_q / r_q^(n_k - 2) and s = +1 (GBSW) / -1 (GBMV):
 //   F_j     += s fac dV_p/dx_j                 (j != i)
-//   F_owner -= s fac sum_j dV_p/dx_j           (GBMV here; GBSW uses dV/dp from the volume pass)
+//   F_owner -= s fac sum_j dV_p/dx_j           (translation invariance; written per point to vrec)
 // (GBSWIntegral.cpp:271-314, GBMVIntegralTypeI.cpp:195-224, GBMVIntegralTypeII.cpp:221-282;
 //  CharmmReferenceGBMV.cpp:376-383).  Atom partials are quantised to 2^-24 and summed
 // exactly in FP64, so the per-cell totals do not depend on the point order.
@@ -1038,7 +1037,6 @@ __global__ void __launch_bounds__(64, ADJ ? 2 : 3) k_force(KParams p) {
     __shared__ float q_x[kWave], q_y[kWave], q_z[kWave];
     __shared__ float q_0[kWave], q_1[kWave], q_2[kWave], q_3[kWave], q_4[kWave], q_5[kWave];
     __shared__ int q_own[kWave], q_pid[kWave];
-    __shared__ float q_V[kWave];
     if (batch_fatal(p)) return;
     const int lane = threadIdx.x, ag = lane & 7, pg = lane >> 3;
     const int nwork = p.work[VK];
@@ -1111,10 +1109,8 @@ __global__ void __launch_bounds__(64, ADJ ? 2 : 3) k_force(KParams p) {
                         q_3[lane] = r1.w;
                         q_4[lane] = r2.x;
                         q_5[lane] = r2.y;
-                        q_V[lane] = r2.z;
                     } else {
                         q_0[lane] = facs * r1.x;
-                        q_V[lane] = r1.y;
                     }
                 }
                 __syncthreads();
@@ -1196,17 +1192,17 @@ __global__ void __launch_bounds__(64, ADJ ? 2 : 3) k_force(KParams p) {
                             oz += cz;
                         }
                     }
-                    if (VK != VOL_GBSW) {
+                    {
                         ox = sum8(ox);
                         oy = sum8(oy);
                         oz = sum8(oz);
                         if (ag == 0 && m < cntp) {
                             const int pid = q_pid[m];
                             if (ab == 0) {
-                                p.vrec[pid] = make_float4(q_V[m], -ox, -oy, -oz);
+                                p.vrec[pid] = make_float4(-ox, -oy, -oz, 0.f);
                             } else {
                                 float4 v = p.vrec[pid];
-                                p.vrec[pid] = make_float4(v.x, v.y - ox, v.z - oy, v.w - oz);
+                                p.vrec[pid] = make_float4(v.x - ox, v.y - oy, v.z - oz, 0.f);
                             }
                         }
                     }
@@ -1253,8 +1249,8 @@ __global__ void __launch_bounds__(64, ADJ ? 2 : 3) k_force(KParams p) {
     }
 }
 
-// Owner forces: one wave per atom, fixed-order FP64 sum over its quadrature points.
-// GBSW: F_i += sum_q fac(i,q) dV/dp(i,q); GBMV: F_i += sum_q owner force(i,q).
+// Owner forces: one wave per atom, fixed-order FP64 sum of its points' owner forces, read
+// only where the sign bit of V marks a point with a chain-rule factor (V >= 0 always).
 // ADJ (GBMV): the sigmoid parts of dE/dbeta, dE/dlambda, dE/dS0 with
 // V = 1/(1 + exp(beta (S - lambda))): dV/dbeta = -V(1-V)(S - lambda), dV/dlambda = V(1-V) beta,
 // dV/dS0 = -V(1-V) beta S/S0; dE/dtheta = -sum s fac dV/dtheta.
@@ -1275,15 +1271,17 @@ __global__ void __launch_bounds__(64) k_owner_reduce(KParams p) {
     for (int q = lane; q < Q; q += kWave) {
         const int pid = pbase + q;
         const double facs = sc * (e4 * p.quad_pw[qb + q] + e7 * p.quad_pw[p.quad_rows + qb + q]);
-        const double wq = gbsw ? facs : 1.0;
-        const float4 v = p.vrec[pid];
-        a[0] += wq * (double)v.y;
-        a[1] += wq * (double)v.z;
-        a[2] += wq * (double)v.w;
+        const float vv = p.vval[pid];
+        if (__float_as_int(vv) < 0) {
+            const float4 v = p.vrec[pid];
+            a[0] += (double)v.x;
+            a[1] += (double)v.y;
+            a[2] += (double)v.z;
+        }
         if (ADJ && !gbsw) {
             const float S = p.vsig[pid];
             if (!isnan(S)) {
-                const double V = v.x;
+                const double V = volume_of(vv);
                 const double u = V * (1.0 - V);
                 tb += facs * u * ((double)S - lambda);
                 tl -= facs * u * beta;
