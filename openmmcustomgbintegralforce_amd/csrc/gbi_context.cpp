@@ -77,7 +77,8 @@ struct SysHost {
     double g = 0.15, gpad = 0.0, halfdiag = 0.0, max_reach = 0.0;
     long long cell_cap = 0;
     int pt_cap = 1;
-    int pair_splits = 1;
+    int pair_splits = 1, pair_k = 1, nblk = 1;
+    long long colpart_base = 0;
 };
 
 }  // namespace
@@ -104,7 +105,10 @@ struct gbi_ctx {
     DevBuf<gbi::SysGeom> geom;
     DevBuf<int> flags_all, atom_sys, cell_sys, bin_order;
     DevBuf<int4> pair_tiles;
-    int n_tiles = 0, max_splits = 1;
+    DevBuf<int> colblk;
+    DevBuf<float> colpart;
+    int n_tiles = 0, max_splits = 1, n_colblocks = 0;
+    long long colpart_total = 1;
     DevBuf<double> pos, d_charge, d_radius, d_reach, quad_off, quad_pw, integrals, born, dbdi, dedi, pair_part,
         atom_energy, energy, stats, dedq, dedr, theta_atom, theta;
     DevBuf<float4> aparam;
@@ -149,6 +153,8 @@ struct gbi_ctx {
         for (auto* b : d) b->release();
         aparam.release();
         pair_tiles.release();
+        colblk.release();
+        colpart.release();
         DevBuf<int>* ib[] = {&flags_all,      &atom_sys,     &cell_sys,       &bin_order,       &cell_count, &cell_fill,
                              &cell_start,     &cell_count_eff, &cell_atoms_tmp, &cell_atoms, &cell_pt_count,
                              &cell_act_count, &frec_base,    &worklist,       &worklist_sys, &work, &scan_tmp};
@@ -267,6 +273,9 @@ gbi::SysDesc make_desc(const gbi_ctx& c, const SysHost& s, long long cell_base, 
     d.prec_base = prec_base;
     d.cap = (P.cell_cap > 0 && s.kind != gbi::GRID_BRUTE) ? P.cell_cap : INT_MAX;
     d.pair_splits = s.pair_splits;
+    d.pair_k = s.pair_k;
+    d.nblk = s.nblk;
+    d.colpart_base = s.colpart_base;
     d.i4 = P.orders[0] == 4 ? 0 : 1;
     d.i7 = 1 - d.i4;
     d.g = s.g;
@@ -344,33 +353,59 @@ gbi_status layout(gbi_ctx& c) {
     return GBI_OK;
 }
 
-// Pair tiles: 64 rows x a column range of the same system.  Enough tiles to fill the
-// device, and at most `pair_columns` columns per tile for load balance across systems.
+// Pair tiles: row block I x column blocks [J0, J1) of the same system, J >= I (each
+// unordered pair once).  K column blocks per tile: `pair_columns` / 64 when set, else the
+// smallest K that keeps about 8192 tiles over the batch (every system's share of the block
+// pairs).  Column partials: one entry per (row block, atom) of each system.
 gbi_status build_tiles(gbi_ctx& c) {
-    long long rows = 0;
-    for (const SysHost& s : c.sys) rows += (s.N + 63) / 64;
-    const int base = (int)std::max<long long>(1, (2048 + rows - 1) / rows);
-    const int cols = c.tuning.pair_columns > 0 ? c.tuning.pair_columns : 8192;
+    long long pairs_total = 0;
+    for (const SysHost& s : c.sys) {
+        const long long nb = (s.N + 63) / 64;
+        pairs_total += nb * (nb + 1) / 2;
+    }
+    const long long target = 8192;
     std::vector<int4> tiles;
+    std::vector<int> colblk;
     c.max_splits = 1;
+    long long cp = 0;
     for (SysHost& s : c.sys) {
-        int splits = std::max(base, (s.N + cols - 1) / cols);
-        splits = std::max(1, std::min(splits, std::min(64, s.N / 64)));
-        s.pair_splits = splits;
-        c.max_splits = std::max(c.max_splits, splits);
-        for (int r = 0; r < s.N; r += 64)
-            for (int k = 0; k < splits; ++k) {
-                const int lo = s.atom_begin + (int)((long long)k * s.N / splits);
-                const int hi = s.atom_begin + (int)((long long)(k + 1) * s.N / splits);
-                tiles.push_back(make_int4(s.atom_begin + r, lo, hi, k));
+        const int nb = (s.N + 63) / 64;
+        int K;
+        if (c.tuning.pair_columns > 0) {
+            K = std::max(1, c.tuning.pair_columns / 64);
+        } else {
+            const long long pairs = (long long)nb * (nb + 1) / 2;
+            const double share = (double)target * pairs / (double)pairs_total;  // tiles for this system
+            K = (int)std::ceil((double)pairs / std::max(1.0, share));
+        }
+        K = std::max(1, std::min(K, nb));
+        s.pair_k = K;
+        s.nblk = nb;
+        s.pair_splits = (nb + K - 1) / K;  // pieces of row block 0, the most of any block
+        s.colpart_base = cp;
+        cp += (long long)nb * s.N;
+        c.max_splits = std::max(c.max_splits, s.pair_splits);
+        for (int I = 0; I < nb; ++I) {
+            colblk.push_back(s.atom_begin + 64 * I);
+            for (int J0 = I, k = 0; J0 < nb; J0 += K, ++k) {
+                const int J1 = std::min(nb, J0 + K);
+                tiles.push_back(make_int4(s.atom_begin + 64 * I, s.atom_begin + 64 * J0,
+                                          s.atom_begin + std::min(s.N, 64 * J1), k));
             }
+        }
     }
     c.n_tiles = (int)tiles.size();
+    c.n_colblocks = (int)colblk.size();
+    c.colpart_total = std::max<long long>(1, cp);
     c.drop_graphs();
     HIP_TRY(hipStreamSynchronize(c.stream));
     HIP_TRY(c.pair_tiles.ensure(tiles.size()));
+    HIP_TRY(c.colblk.ensure(colblk.size()));
     HIP_TRY(c.pair_part.ensure((size_t)c.max_splits * 6 * c.N));
+    HIP_TRY(c.colpart.ensure((size_t)5 * c.colpart_total));
     HIP_TRY(hipMemcpyAsync(c.pair_tiles.p, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice,
+                           c.stream));
+    HIP_TRY(hipMemcpyAsync(c.colblk.p, colblk.data(), sizeof(int) * colblk.size(), hipMemcpyHostToDevice,
                            c.stream));
     HIP_TRY(hipStreamSynchronize(c.stream));
     return GBI_OK;
@@ -452,6 +487,8 @@ gbi::KParams make_kparams(gbi_ctx& c, bool adjoint) {
     k.entry_cap = c.entry_cap;
     k.n_tiles = c.n_tiles;
     k.max_splits = c.max_splits;
+    k.n_colblocks = c.n_colblocks;
+    k.colpart_total = c.colpart_total;
     k.adjoint = adjoint ? 1 : 0;
     k.bin_order = c.bin_order.p;
     k.persist_blocks = c.num_cus * std::max(1, c.tuning.blocks_per_cu);
@@ -470,6 +507,7 @@ gbi::KParams make_kparams(gbi_ctx& c, bool adjoint) {
     k.quad_r2 = c.quad_r2.p;
     k.quad_rows = c.quad_rows;
     k.pair_tiles = c.pair_tiles.p;
+    k.colblk = c.colblk.p;
     k.cell_count = c.cell_count.p;
     k.cell_fill = c.cell_fill.p;
     k.cell_start = c.cell_start.p;
@@ -495,6 +533,7 @@ gbi::KParams make_kparams(gbi_ctx& c, bool adjoint) {
     k.dbdi = c.dbdi.p;
     k.dedi = c.dedi.p;
     k.pair_part = c.pair_part.p;
+    k.colpart = c.colpart.p;
     k.atom_energy = c.atom_energy.p;
     k.dedq = c.dedq.p;
     k.dedr_fix = c.dedr_fix.p;

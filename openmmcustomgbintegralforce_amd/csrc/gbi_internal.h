@@ -78,7 +78,10 @@ struct SysDesc {
     int pt_cap;                // point slots per cell
     long long prec_base;       // slot s of local cell c: prec_base + c * pt_cap + s
     int cap;                   // atoms kept per cell
-    int pair_splits;           // column splits of the pair pass
+    int pair_splits;           // pair tiles of row block 0 (the most of any block)
+    int pair_k;                // column blocks per pair tile
+    int nblk;                  // 64-atom blocks
+    long long colpart_base;    // column partials of row block I, atom j: colpart_base + I * n + (j - atom_begin)
     int i4, i7;                // which integral is order 4 / 7
     // grid constants (double, computed on the host with the oracle's expressions)
     double g, inv_g, gpad, halfdiag, buffer;
@@ -107,7 +110,9 @@ struct KParams {
     int cell_total;          // sum of the systems' cell ranges
     int entry_cap;
     int n_tiles;             // pair tiles
-    int max_splits;          // pair_part planes
+    int max_splits;          // pair_part planes (row pieces of block 0, the most of any block)
+    int n_colblocks;         // 64-atom blocks of all systems (k_atom_chain workgroups)
+    long long colpart_total; // column partial entries per plane
     int adjoint;             // also compute dE/dR, dE/dq, dE/dtheta
     const int* bin_order;    // N: atom visiting order of the binning pass (systems in turn, each
                              // with a stride coprime to its size so concurrent workgroups hit distant cells)
@@ -128,7 +133,8 @@ struct KParams {
     const double* quad_pw;   // 2 x rows: w_q / r_q^(order_k - 2)
     const float* quad_r2;    // rows: |offset_q|^2
     int quad_rows;
-    const int4* pair_tiles;  // (row begin, column begin, column end, split), global atom indices
+    const int4* pair_tiles;  // (row begin, column begin, column end, piece), global atom indices
+    const int* colblk;       // n_colblocks: first atom of every 64-atom block
     // grid
     int* cell_count;         // cell_total
     int* cell_fill;          // cell_total
@@ -156,7 +162,8 @@ struct KParams {
     double* born;            // N
     double* dbdi;            // 2 x N
     double* dedi;            // 2 x N
-    double* pair_part;       // max_splits x 6 x N (fx, fy, fz, dE/dB, E, dE/dq)
+    double* pair_part;       // max_splits x 6 x N (fx, fy, fz, dE/dB, E, dE/dq): row sums per tile
+    float* colpart;          // 5 planes x colpart_total (fx, fy, fz, dE/dB, dE/dq) per (row block, atom)
     double* atom_energy;     // N
     double* dedq;            // N (adjoint)
     long long* dedr_fix;     // N (adjoint): sum_p s fac_p dV_p/dR_j (GBSW: dR'_j), 2^32

@@ -718,7 +718,6 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                     __syncthreads();
                 }
                 if (VK == VOL_GBSW) {
-                    #pragma unroll 4
                     for (int m = 0; m < cnt; ++m) {
                         if (__all(done)) break;
                         const float dx = s_x[m] - prx, dy = s_y[m] - pry, dz = s_z[m] - prz;
@@ -858,76 +857,136 @@ __global__ void __launch_bounds__(64) k_integrals(KParams p) {
 }
 
 // ---------------------------------------------------------------------------
-// STILL pair pass: workgroup = one tile (64 atoms i x a column range of the same system);
-// every ordered pair (i, j != i) is evaluated from i's side, so no atomics and a fixed
-// summation order.  E_ij = P q_i q_j / f_ij, f = sqrt(r^2 + B_i B_j exp(-r^2/(4 B_i B_j)))
-// (CharmmReferenceGBSW.cpp:248-282).  ADJ: also sum_j P q_j / f_ij = dE/dq_i - P q_i/B_i.
+// STILL pair pass: E_ij = P q_i q_j / f_ij, f = sqrt(r^2 + B_i B_j exp(-r^2/(4 B_i B_j)))
+// (CharmmReferenceGBSW.cpp:248-282).  A system's atoms form 64-atom blocks; a tile (one
+// wave) is row block I with column blocks J0..J1-1, J >= I.  The diagonal block is
+// evaluated from the row side (every ordered pair i != j); an off-diagonal block evaluates
+// each unordered pair once: lane l holds row atom i and at step m meets column (l + m) mod
+// 64, whose partial sums then move one lane down, so after 64 steps lane l holds column l's
+// sums (fixed order, no atomics).  Row sums go to pair_part (one plane per tile of the row),
+// column sums to colpart (per row block and atom); k_atom_chain adds them in a fixed order.
+// Each block's terms are summed in FP32 (64 terms) and folded into FP64.
+// ADJ: also sum_j P q_j / f_ij = dE/dq_i - P q_i/B_i, on both sides.
 // ---------------------------------------------------------------------------
+
+// One pair's terms (from i's side: d = x_j - x_i).
+struct PairTerms {
+    float dx, dy, dz, rf, ex, rf3, r2;
+};
+
+__device__ inline PairTerms pair_terms(double xj, double yj, double zj, double xi, double yi, double zi, float BB) {
+    PairTerms t;
+    t.dx = (float)(xj - xi);
+    t.dy = (float)(yj - yi);
+    t.dz = (float)(zj - zi);
+    t.r2 = fmaf(t.dx, t.dx, fmaf(t.dy, t.dy, t.dz * t.dz));
+    t.ex = __expf(-t.r2 * 0.25f * fast_rcp(BB));
+    const float f2 = fmaf(BB, t.ex, t.r2);
+    t.rf = __builtin_amdgcn_rsqf(f2);
+    t.rf3 = t.rf * t.rf * t.rf;
+    return t;
+}
+
+__device__ inline float rotate_down(float v, int lane) { return __shfl(v, (lane + 1) & (kWave - 1), kWave); }
 
 template <bool ADJ>
 __global__ void __launch_bounds__(64) k_pairs(KParams p) {
     __shared__ double s_px[64], s_py[64], s_pz[64];
-    __shared__ float s_q[64], s_B[64];
+    __shared__ float s_q[64], s_B[64], s_q4B[64];
     if (batch_fatal(p)) return;
     const int lane = threadIdx.x;
     const int4 tile = p.pair_tiles[blockIdx.x];
     const int s = p.atom_sys[tile.x];
-    const int row_end = min(tile.x + kWave, p.sys[s].atom_end);
+    const SysDesc& sd = p.sys[s];
+    const int ab = sd.atom_begin, ae = sd.atom_end, ns = ae - ab;
     const int i = tile.x + lane;
-    const int jlo = tile.y, jhi = tile.z, split = tile.w;
-    const bool iv = i < row_end;
+    const bool iv = i < ae;
     const double xi = iv ? p.pos[3 * i] : 0.0, yi = iv ? p.pos[3 * i + 1] : 0.0, zi = iv ? p.pos[3 * i + 2] : 0.0;
-    const float P = (float)p.sys[s].prefactor;
-    const float qi = iv ? (float)p.charge[i] : 0.f;
+    const float P = (float)sd.prefactor;
+    const float qi = iv ? (float)p.charge[i] : 0.f;  // zero charge: an absent row adds nothing
     const float Bi = iv ? (float)p.born[i] : 1.f;
     const float Pq = P * qi;
     const float q4Bi = 0.25f / Bi;
+    const long long cp_row = sd.colpart_base + (long long)((tile.x - ab) / kWave) * ns;
     double fx = 0.0, fy = 0.0, fz = 0.0, dedb = 0.0, e = 0.0, dq = 0.0;
-    for (int jb = jlo; jb < jhi; jb += kWave) {
+    for (int jb = tile.y; jb < tile.z; jb += kWave) {
         const int jj = jb + lane;
+        const bool jv = jj < ae;
         __syncthreads();
-        if (jj < jhi) {
-            s_px[lane] = p.pos[3 * jj];
-            s_py[lane] = p.pos[3 * jj + 1];
-            s_pz[lane] = p.pos[3 * jj + 2];
-            s_q[lane] = (float)p.charge[jj];
-            s_B[lane] = (float)p.born[jj];
-        }
+        s_px[lane] = jv ? p.pos[3 * jj] : 0.0;
+        s_py[lane] = jv ? p.pos[3 * jj + 1] : 0.0;
+        s_pz[lane] = jv ? p.pos[3 * jj + 2] : 0.0;
+        s_q[lane] = jv ? (float)p.charge[jj] : 0.f;
+        const float Bj_l = jv ? (float)p.born[jj] : 1.f;
+        s_B[lane] = Bj_l;
+        s_q4B[lane] = 0.25f / Bj_l;
         __syncthreads();
-        const int cnt = min(kWave, jhi - jb);
-        if (iv) {
-            float efs = 0.f, dqs = 0.f;
-            float fxs = 0.f, fys = 0.f, fzs = 0.f, dbs = 0.f;
+        float efs = 0.f, dqs = 0.f, fxs = 0.f, fys = 0.f, fzs = 0.f, dbs = 0.f;
+        if (jb == tile.x) {
+            // diagonal block: ordered pairs from the row side, half the pair energy each
+            const int cnt = min(kWave, ae - jb);
             for (int m = 0; m < cnt; ++m) {
                 if (jb + m == i) continue;
-                const float dx = (float)(s_px[m] - xi), dy = (float)(s_py[m] - yi), dz = (float)(s_pz[m] - zi);
-                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                 const float Bj = s_B[m];
-                const float BB = Bi * Bj;
-                const float ex = __expf(-r2 * 0.25f * fast_rcp(BB));
-                const float f2 = fmaf(BB, ex, r2);
-                const float rf = __builtin_amdgcn_rsqf(f2);
-                const float rf3 = rf * rf * rf;
+                const PairTerms t = pair_terms(s_px[m], s_py[m], s_pz[m], xi, yi, zi, Bi * Bj);
                 const float Pqq = Pq * s_q[m];
-                efs = fmaf(0.5f * Pqq, rf, efs);
-                if (ADJ) dqs = fmaf(P * s_q[m], rf, dqs);
-                const float dGdr = -0.25f * Pqq * (4.f - ex) * rf3;
-                fxs = fmaf(dx, dGdr, fxs);
-                fys = fmaf(dy, dGdr, fys);
-                fzs = fmaf(dz, dGdr, fzs);
-                dbs = fmaf(-0.5f * Pqq * ex * rf3, fmaf(r2, q4Bi, Bj), dbs);
+                efs = fmaf(0.5f * Pqq, t.rf, efs);
+                if (ADJ) dqs = fmaf(P * s_q[m], t.rf, dqs);
+                const float dGdr = -0.25f * Pqq * (4.f - t.ex) * t.rf3;
+                fxs = fmaf(t.dx, dGdr, fxs);
+                fys = fmaf(t.dy, dGdr, fys);
+                fzs = fmaf(t.dz, dGdr, fzs);
+                dbs = fmaf(-0.5f * Pqq * t.ex * t.rf3, fmaf(t.r2, q4Bi, Bj), dbs);
             }
-            // fold the tile's FP32 partials into FP64 accumulators (64 terms per tile)
-            e += efs;
-            fx += fxs;
-            fy += fys;
-            fz += fzs;
-            dedb += dbs;
-            if (ADJ) dq += dqs;
+        } else {
+            // off-diagonal block: each unordered pair once, column sums rotate through the lanes
+            float cfx = 0.f, cfy = 0.f, cfz = 0.f, cdb = 0.f, cdq = 0.f;
+            for (int m = 0; m < kWave; ++m) {
+                const int c = (lane + m) & (kWave - 1);
+                const float Bj = s_B[c];
+                const PairTerms t = pair_terms(s_px[c], s_py[c], s_pz[c], xi, yi, zi, Bi * Bj);
+                const float Pqq = Pq * s_q[c];
+                efs = fmaf(Pqq, t.rf, efs);
+                const float dGdr = -0.25f * Pqq * (4.f - t.ex) * t.rf3;
+                const float gx = t.dx * dGdr, gy = t.dy * dGdr, gz = t.dz * dGdr;
+                fxs += gx;
+                fys += gy;
+                fzs += gz;
+                const float hb = -0.5f * Pqq * t.ex * t.rf3;
+                dbs = fmaf(hb, fmaf(t.r2, q4Bi, Bj), dbs);
+                cfx -= gx;
+                cfy -= gy;
+                cfz -= gz;
+                cdb = fmaf(hb, fmaf(t.r2, s_q4B[c], Bi), cdb);
+                if (ADJ) {
+                    dqs = fmaf(P * s_q[c], t.rf, dqs);
+                    cdq = fmaf(Pq, t.rf, cdq);
+                }
+                cfx = rotate_down(cfx, lane);
+                cfy = rotate_down(cfy, lane);
+                cfz = rotate_down(cfz, lane);
+                cdb = rotate_down(cdb, lane);
+                if (ADJ) cdq = rotate_down(cdq, lane);
+            }
+            if (jv) {
+                const long long o = cp_row + (jj - ab);
+                const long long T = p.colpart_total;
+                p.colpart[o] = cfx;
+                p.colpart[T + o] = cfy;
+                p.colpart[2 * T + o] = cfz;
+                p.colpart[3 * T + o] = cdb;
+                if (ADJ) p.colpart[4 * T + o] = cdq;
+            }
         }
+        e += efs;
+        fx += fxs;
+        fy += fys;
+        fz += fzs;
+        dedb += dbs;
+        if (ADJ) dq += dqs;
     }
     if (iv) {
-        double* out = p.pair_part + (size_t)split * 6 * p.N;
+        double* out = p.pair_part + (size_t)tile.w * 6 * p.N;
         out[i] = fx;
         out[p.N + i] = fy;
         out[2 * p.N + i] = fz;
@@ -937,26 +996,57 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
     }
 }
 
-// Per atom: pair partials in split order, self term, dE/dI = dE/dB dB/dI
-// (customGBEnergyPerParticle.cu; CharmmReferenceGBSW.cpp:257-273).
-// ADJ: dE/dq and the Born-map parts of dE/dtheta.
+// Per atom, one workgroup per 64-atom block: the pair sums in a fixed order, then the self
+// term and dE/dI = dE/dB dB/dI (customGBEnergyPerParticle.cu; CharmmReferenceGBSW.cpp:257-273).
+// Thread (g, lane) sums, for atom lane of the block, the row pieces k = g, g + G, ... and the
+// column partials of row blocks I = g, g + G, ... < bj in FP64 (G = 16 wave groups); the
+// groups are added in order.  ADJ: dE/dq and the Born-map parts of dE/dtheta.
+constexpr int kChainGroups = 16;
+
 template <bool ADJ>
-__global__ void k_atom_chain(KParams p) {
+__global__ void __launch_bounds__(kChainGroups * kWave) k_atom_chain(KParams p) {
+    __shared__ double s_acc[kChainGroups][6][kWave];
     if (batch_fatal(p)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.N) return;
-    const SysDesc& sd = p.sys[p.atom_sys[i]];
-    double fx = 0.0, fy = 0.0, fz = 0.0, dedb = 0.0, e = 0.0, dq = 0.0;
-    const int splits = sd.pair_splits;
-    for (int k = 0; k < splits; ++k) {
-        const double* in = p.pair_part + (size_t)k * 6 * p.N;
-        fx += in[i];
-        fy += in[p.N + i];
-        fz += in[2 * p.N + i];
-        dedb += in[3 * p.N + i];
-        e += in[4 * p.N + i];
-        if (ADJ) dq += in[5 * p.N + i];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), grp = tid / kWave;
+    const int j0 = p.colblk[blockIdx.x];
+    const SysDesc& sd = p.sys[p.atom_sys[j0]];
+    const int ab = sd.atom_begin, ns = sd.atom_end - ab;
+    const int bj = (j0 - ab) / kWave;
+    const int i = j0 + lane;
+    const bool iv = i < sd.atom_end;
+    const int pieces = (sd.nblk - bj + sd.pair_k - 1) / sd.pair_k;
+    const long long T = p.colpart_total;
+    double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (iv) {
+        for (int k = grp; k < pieces; k += kChainGroups) {
+            const double* in = p.pair_part + (size_t)k * 6 * p.N;
+            a[0] += in[i];
+            a[1] += in[p.N + i];
+            a[2] += in[2 * p.N + i];
+            a[3] += in[3 * p.N + i];
+            a[4] += in[4 * p.N + i];
+            if (ADJ) a[5] += in[5 * p.N + i];
+        }
+        for (int I = grp; I < bj; I += kChainGroups) {
+            const long long o = sd.colpart_base + (long long)I * ns + (i - ab);
+            a[0] += p.colpart[o];
+            a[1] += p.colpart[T + o];
+            a[2] += p.colpart[2 * T + o];
+            a[3] += p.colpart[3 * T + o];
+            if (ADJ) a[5] += p.colpart[4 * T + o];
+        }
     }
+    for (int v = 0; v < 6; ++v) s_acc[grp][v][lane] = a[v];
+    __syncthreads();
+    if (grp != 0 || !iv) return;
+    double r[6];
+    for (int v = 0; v < 6; ++v) {
+        r[v] = s_acc[0][v][lane];
+        for (int g = 1; g < kChainGroups; ++g) r[v] += s_acc[g][v][lane];
+    }
+    const double fx = r[0], fy = r[1], fz = r[2];
+    double dedb = r[3], e = r[4];
+    const double dq = r[5];
     const double q = p.charge[i], B = p.born[i], P = sd.prefactor;
     e += 0.5 * P * q * q / B;
     dedb += -0.5 * P * q * q / (B * B);
@@ -1430,16 +1520,16 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
     mark(3);
     hipLaunchKernelGGL(k_integrals, dim3(p.N), dim3(64), 0, stream, p);
     mark(4);
-    if (adj)
+    if (adj) {
         hipLaunchKernelGGL(k_pairs<true>, dim3(p.n_tiles), dim3(64), 0, stream, p);
-    else
+    } else {
         hipLaunchKernelGGL(k_pairs<false>, dim3(p.n_tiles), dim3(64), 0, stream, p);
+    }
     mark(5);
-    const int atom_blocks = (p.N + 255) / 256;
     if (adj)
-        hipLaunchKernelGGL(k_atom_chain<true>, dim3(atom_blocks), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL(k_atom_chain<true>, dim3(p.n_colblocks), dim3(kChainGroups * kWave), 0, stream, p);
     else
-        hipLaunchKernelGGL(k_atom_chain<false>, dim3(atom_blocks), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL(k_atom_chain<false>, dim3(p.n_colblocks), dim3(kChainGroups * kWave), 0, stream, p);
     hipLaunchKernelGGL(k_energy, dim3(p.S), dim3(1024), 0, stream, p);
     mark(6);
     for (int vk = 0; vk < VOL_KINDS; ++vk) {
@@ -1452,7 +1542,7 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
     mark(7);
     if (adj) {
         hipLaunchKernelGGL(k_owner_reduce<true>, dim3(p.N), dim3(64), 0, stream, p);
-        hipLaunchKernelGGL(k_adjoint_atoms, dim3(atom_blocks), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL(k_adjoint_atoms, dim3((p.N + 255) / 256), dim3(256), 0, stream, p);
         hipLaunchKernelGGL(k_theta_reduce, dim3(p.S), dim3(256), 0, stream, p);
     } else {
         hipLaunchKernelGGL(k_owner_reduce<false>, dim3(p.N), dim3(64), 0, stream, p);
