@@ -869,30 +869,33 @@ __global__ void __launch_bounds__(64) k_integrals(KParams p) {
 // ADJ: also sum_j P q_j / f_ij = dE/dq_i - P q_i/B_i, on both sides.
 // ---------------------------------------------------------------------------
 
-// One pair's terms (from i's side: d = x_j - x_i).
+// One pair's terms (from i's side: d = x_j - x_i).  ex = exp(-r^2/(4 B_i B_j)) as
+// exp2(-(r^2 kexp_i) / B_j) with kexp_i = log2(e) / (4 B_i); rf = 1/f.
 struct PairTerms {
-    float dx, dy, dz, rf, ex, rf3, r2;
+    float dx, dy, dz, r2, ex, rf;
 };
 
-__device__ inline PairTerms pair_terms(double xj, double yj, double zj, double xi, double yi, double zi, float BB) {
+__device__ inline PairTerms pair_terms(double xj, double yj, double zj, double xi, double yi, double zi, float Bi,
+                                       float Bj, float kexp_i, float iBj) {
     PairTerms t;
     t.dx = (float)(xj - xi);
     t.dy = (float)(yj - yi);
     t.dz = (float)(zj - zi);
     t.r2 = fmaf(t.dx, t.dx, fmaf(t.dy, t.dy, t.dz * t.dz));
-    t.ex = __expf(-t.r2 * 0.25f * fast_rcp(BB));
-    const float f2 = fmaf(BB, t.ex, t.r2);
-    t.rf = __builtin_amdgcn_rsqf(f2);
-    t.rf3 = t.rf * t.rf * t.rf;
+    t.ex = __builtin_amdgcn_exp2f(-(t.r2 * kexp_i) * iBj);
+    t.rf = __builtin_amdgcn_rsqf(fmaf(Bi * Bj, t.ex, t.r2));
     return t;
 }
 
-__device__ inline float rotate_down(float v, int lane) { return __shfl(v, (lane + 1) & (kWave - 1), kWave); }
+// Lane l receives lane l + 1's value (mod 64): DPP wave_rol:1.
+__device__ inline float rotate_down(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
+}
 
 template <bool ADJ>
 __global__ void __launch_bounds__(64) k_pairs(KParams p) {
     __shared__ double s_px[64], s_py[64], s_pz[64];
-    __shared__ float s_q[64], s_B[64], s_q4B[64];
+    __shared__ float4 s_col[64];  // (q, B, 1/B, 1/(4B))
     if (batch_fatal(p)) return;
     const int lane = threadIdx.x;
     const int4 tile = p.pair_tiles[blockIdx.x];
@@ -907,6 +910,7 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
     const float Bi = iv ? (float)p.born[i] : 1.f;
     const float Pq = P * qi;
     const float q4Bi = 0.25f / Bi;
+    const float kexp = 1.4426950408889634f * q4Bi;
     const long long cp_row = sd.colpart_base + (long long)((tile.x - ab) / kWave) * ns;
     double fx = 0.0, fy = 0.0, fz = 0.0, dedb = 0.0, e = 0.0, dq = 0.0;
     for (int jb = tile.y; jb < tile.z; jb += kWave) {
@@ -916,10 +920,11 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
         s_px[lane] = jv ? p.pos[3 * jj] : 0.0;
         s_py[lane] = jv ? p.pos[3 * jj + 1] : 0.0;
         s_pz[lane] = jv ? p.pos[3 * jj + 2] : 0.0;
-        s_q[lane] = jv ? (float)p.charge[jj] : 0.f;
-        const float Bj_l = jv ? (float)p.born[jj] : 1.f;
-        s_B[lane] = Bj_l;
-        s_q4B[lane] = 0.25f / Bj_l;
+        {
+            const float Bj = jv ? (float)p.born[jj] : 1.f;
+            const float iB = 1.f / Bj;
+            s_col[lane] = make_float4(jv ? (float)p.charge[jj] : 0.f, Bj, iB, 0.25f * iB);
+        }
         __syncthreads();
         float efs = 0.f, dqs = 0.f, fxs = 0.f, fys = 0.f, fzs = 0.f, dbs = 0.f;
         if (jb == tile.x) {
@@ -927,46 +932,47 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
             const int cnt = min(kWave, ae - jb);
             for (int m = 0; m < cnt; ++m) {
                 if (jb + m == i) continue;
-                const float Bj = s_B[m];
-                const PairTerms t = pair_terms(s_px[m], s_py[m], s_pz[m], xi, yi, zi, Bi * Bj);
-                const float Pqq = Pq * s_q[m];
-                efs = fmaf(0.5f * Pqq, t.rf, efs);
-                if (ADJ) dqs = fmaf(P * s_q[m], t.rf, dqs);
-                const float dGdr = -0.25f * Pqq * (4.f - t.ex) * t.rf3;
+                const float4 cj = s_col[m];
+                const PairTerms t = pair_terms(s_px[m], s_py[m], s_pz[m], xi, yi, zi, Bi, cj.y, kexp, cj.z);
+                const float eij = Pq * cj.x * t.rf;
+                efs = fmaf(0.5f, eij, efs);
+                if (ADJ) dqs = fmaf(P * cj.x, t.rf, dqs);
+                const float t3 = eij * (t.rf * t.rf);
+                const float dGdr = t3 * fmaf(0.25f, t.ex, -1.f);  // -Pqq (4 - ex) / (4 f^3)
                 fxs = fmaf(t.dx, dGdr, fxs);
                 fys = fmaf(t.dy, dGdr, fys);
                 fzs = fmaf(t.dz, dGdr, fzs);
-                dbs = fmaf(-0.5f * Pqq * t.ex * t.rf3, fmaf(t.r2, q4Bi, Bj), dbs);
+                dbs = fmaf(-0.5f * t3 * t.ex, fmaf(t.r2, q4Bi, cj.y), dbs);
             }
         } else {
             // off-diagonal block: each unordered pair once, column sums rotate through the lanes
             float cfx = 0.f, cfy = 0.f, cfz = 0.f, cdb = 0.f, cdq = 0.f;
             for (int m = 0; m < kWave; ++m) {
                 const int c = (lane + m) & (kWave - 1);
-                const float Bj = s_B[c];
-                const PairTerms t = pair_terms(s_px[c], s_py[c], s_pz[c], xi, yi, zi, Bi * Bj);
-                const float Pqq = Pq * s_q[c];
-                efs = fmaf(Pqq, t.rf, efs);
-                const float dGdr = -0.25f * Pqq * (4.f - t.ex) * t.rf3;
-                const float gx = t.dx * dGdr, gy = t.dy * dGdr, gz = t.dz * dGdr;
-                fxs += gx;
-                fys += gy;
-                fzs += gz;
-                const float hb = -0.5f * Pqq * t.ex * t.rf3;
-                dbs = fmaf(hb, fmaf(t.r2, q4Bi, Bj), dbs);
-                cfx -= gx;
-                cfy -= gy;
-                cfz -= gz;
-                cdb = fmaf(hb, fmaf(t.r2, s_q4B[c], Bi), cdb);
+                const float4 cj = s_col[c];
+                const PairTerms t = pair_terms(s_px[c], s_py[c], s_pz[c], xi, yi, zi, Bi, cj.y, kexp, cj.z);
+                const float eij = Pq * cj.x * t.rf;  // P q_i q_j / f
+                efs += eij;
+                const float t3 = eij * (t.rf * t.rf);
+                const float dGdr = t3 * fmaf(0.25f, t.ex, -1.f);  // -Pqq (4 - ex) / (4 f^3)
+                const float hb = -0.5f * t3 * t.ex;               // dE/dB_i = hb (r^2/(4B_i) + B_j)
+                fxs = fmaf(t.dx, dGdr, fxs);
+                fys = fmaf(t.dy, dGdr, fys);
+                fzs = fmaf(t.dz, dGdr, fzs);
+                cfx = fmaf(-t.dx, dGdr, cfx);
+                cfy = fmaf(-t.dy, dGdr, cfy);
+                cfz = fmaf(-t.dz, dGdr, cfz);
+                dbs = fmaf(hb, fmaf(t.r2, q4Bi, cj.y), dbs);
+                cdb = fmaf(hb, fmaf(t.r2, cj.w, Bi), cdb);
                 if (ADJ) {
-                    dqs = fmaf(P * s_q[c], t.rf, dqs);
+                    dqs = fmaf(P * cj.x, t.rf, dqs);
                     cdq = fmaf(Pq, t.rf, cdq);
                 }
-                cfx = rotate_down(cfx, lane);
-                cfy = rotate_down(cfy, lane);
-                cfz = rotate_down(cfz, lane);
-                cdb = rotate_down(cdb, lane);
-                if (ADJ) cdq = rotate_down(cdq, lane);
+                cfx = rotate_down(cfx);
+                cfy = rotate_down(cfy);
+                cfz = rotate_down(cfz);
+                cdb = rotate_down(cdb);
+                if (ADJ) cdq = rotate_down(cdq);
             }
             if (jv) {
                 const long long o = cp_row + (jj - ab);
