@@ -376,7 +376,8 @@ __device__ inline void write_entry(const KParams& p, int e, int j, const double*
 
 // One wave per cell: order the cell's atoms by index (rank sort), keep the first `cap`
 // (the reference appends in ascending atom order and stops at the cap,
-// CustomGBIntegral.cpp:219-222) and write their corner-relative entries.
+// CustomGBIntegral.cpp:219-222) and write their corner-relative entries.  (Ordering GBSW
+// lists by distance to the cell, to reach V = 0 sooner, was measured slower.)
 constexpr int kSortLds = 256;  // cells with more raw members sort in global memory
 
 __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
@@ -695,6 +696,12 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                 s_b[k] = q.y;
                 s_c[k] = q.z;
             }
+            if (VK == VOL_GBSW && lane < ((cnt + 3) & ~3) - cnt) {
+                // pad to a multiple of 4 with atoms that neither bury nor touch any point
+                const int k = cnt + lane;
+                s_x[k] = s_y[k] = s_z[k] = s_a[k] = 0.f;
+                s_b[k] = s_c[k] = -1.f;
+            }
         };
         __syncthreads();
         if (single) stage(0, na);
@@ -718,25 +725,21 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                     __syncthreads();
                 }
                 if (VK == VOL_GBSW) {
-                    for (int m = 0; m < cnt; ++m) {
-                        if (__all(done)) break;
-                        const float dx = s_x[m] - prx, dy = s_y[m] - pry, dz = s_z[m] - prz;
-                        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                        if (!done && d2 < s_c[m]) {
-                            if (d2 <= s_b[m]) {
-                                done = true;
-                                V = 0.f;
-                            } else {
-                                const float d = fast_sqrt(d2);
-                                const float x = d - s_a[m];
-                                const float h = fmaf(x, fmaf(-k3 * x, x, k1), 0.5f);
-                                if (h <= 0.f) {
-                                    done = true;
-                                    V = 0.f;
-                                } else {
-                                    V *= h;
-                                }
-                            }
+                    // branch-free: H = 0 inside R' - w, the cubic in the shell, 1 beyond R' + w;
+                    // the wave leaves once every point is buried (V = 0), checked every 4 atoms
+                    if (done) V = 0.f;
+                    const int cnt4 = (cnt + 3) & ~3;
+                    for (int m = 0; m < cnt4; m += 4) {
+                        if (__all(V == 0.f)) break;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const float dx = s_x[m + u] - prx, dy = s_y[m + u] - pry, dz = s_z[m + u] - prz;
+                            const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                            const float x = fast_sqrt(d2) - s_a[m + u];
+                            float h = fmaxf(fmaf(x, fmaf(-k3 * x, x, k1), 0.5f), 0.f);
+                            h = d2 < s_c[m + u] ? h : 1.f;
+                            h = d2 <= s_b[m + u] ? 0.f : h;
+                            V *= h;
                         }
                     }
                 } else if (VK == VOL_GBMV2) {
