@@ -121,6 +121,23 @@ def test_regrow_after_expansion():
     eng.close()
 
 
+def test_regrow_dense_cell_lists():
+    """A globule compressed to ~10x the density has more raw members per cell than the
+    first sizing allows (and cells past the LDS sort size): the lists grow and the
+    evaluation is redone, matching the oracle."""
+    for kind, n, seed in (("gbmv2-50", 400, 21), ("gbsw", 400, 22)):
+        s = S.globule(n, seed)
+        c = s.pos.mean(axis=0)
+        dense = (s.pos - c) * 0.45 + c
+        p = _params(kind)
+        eng = N.Engine(p, s.charge, s.radius)
+        e, f = eng.execute(dense)
+        ref = O.compute(oracle_params(p), dense, s.charge, s.radius)
+        print(f"[dense] {kind}: energy {e:.6f} vs {ref.energy:.6f}", flush=True)
+        assert rel_err(e, ref.energy) < ENERGY_TOL and rel_rms(f, ref.forces) < FORCE_TOL, kind
+        eng.close()
+
+
 def test_autotune_keeps_parity():
     s = S.globule(900, 8)
     for p in (_params("gbsw"), _params("gbmv2-50")):
