@@ -77,8 +77,6 @@ struct SysHost {
     double g = 0.15, gpad = 0.0, halfdiag = 0.0, max_reach = 0.0;
     long long cell_cap = 0;
     int pt_cap = 1;
-    int raw_cap = 128;  // raw list slots per cell (grown on overflow; brute: N)
-    long long raw_base = 0;
     int pair_splits = 1, pair_k = 1, nblk = 1;
     long long colpart_base = 0;
 };
@@ -114,7 +112,7 @@ struct gbi_ctx {
     DevBuf<double> pos, d_charge, d_radius, d_reach, quad_off, quad_pw, integrals, born, dbdi, dedi, pair_part,
         atom_energy, energy, stats, dedq, dedr, theta_atom, theta;
     DevBuf<float4> aparam;
-    DevBuf<int> cell_count, cell_start, cell_count_eff, cell_atoms_tmp, cell_atoms, cell_pt_count,
+    DevBuf<int> cell_count, cell_fill, cell_start, cell_count_eff, cell_atoms_tmp, cell_atoms, cell_pt_count,
         cell_act_count, frec_base, worklist, worklist_sys, work, scan_tmp;
     DevBuf<float4> ent_pos, ent_par, prec, frec0, frec1, frec2, vrec;
     DevBuf<float> quad_r2, vsig, vval;
@@ -157,7 +155,7 @@ struct gbi_ctx {
         pair_tiles.release();
         colblk.release();
         colpart.release();
-        DevBuf<int>* ib[] = {&flags_all,      &atom_sys,     &cell_sys,       &bin_order,       &cell_count,
+        DevBuf<int>* ib[] = {&flags_all,      &atom_sys,     &cell_sys,       &bin_order,       &cell_count, &cell_fill,
                              &cell_start,     &cell_count_eff, &cell_atoms_tmp, &cell_atoms, &cell_pt_count,
                              &cell_act_count, &frec_base,    &worklist,       &worklist_sys, &work, &scan_tmp};
         for (auto* b : ib) b->release();
@@ -274,8 +272,6 @@ gbi::SysDesc make_desc(const gbi_ctx& c, const SysHost& s, long long cell_base, 
     d.pt_cap = s.pt_cap;
     d.prec_base = prec_base;
     d.cap = (P.cell_cap > 0 && s.kind != gbi::GRID_BRUTE) ? P.cell_cap : INT_MAX;
-    d.raw_cap = s.raw_cap;
-    d.raw_base = s.raw_base;
     d.pair_splits = s.pair_splits;
     d.pair_k = s.pair_k;
     d.nblk = s.nblk;
@@ -318,16 +314,11 @@ gbi::SysDesc make_desc(const gbi_ctx& c, const SysHost& s, long long cell_base, 
 // Cell, slot and worklist buffers for the systems' current capacities; uploads the
 // system descriptors and the cell -> system map.
 gbi_status layout(gbi_ctx& c) {
-    long long cells = 0, slots = 0, raw = 0;
-    for (SysHost& s : c.sys) {
+    long long cells = 0, slots = 0;
+    for (const SysHost& s : c.sys) {
         cells += s.cell_cap;
         slots += s.cell_cap * (long long)s.pt_cap;
-        s.raw_cap = s.kind == gbi::GRID_BRUTE ? s.N : std::max(1, std::min(s.raw_cap, s.N));
-        s.raw_base = raw;
-        raw += s.cell_cap * (long long)s.raw_cap;
     }
-    if (raw * (long long)sizeof(int) > (64ll << 30))
-        return fail(GBI_ERR_GRID_TOO_LARGE, "cell-list slots of the lookup grids exceed 64 GB");
     if (cells > kMaxCells) return fail(GBI_ERR_GRID_TOO_LARGE, "more than 16M lookup cells in the batch");
     if (slots * (long long)sizeof(float4) > (200ll << 30))
         return fail(GBI_ERR_GRID_TOO_LARGE, "point slots of the lookup grids exceed 200 GB");
@@ -336,6 +327,7 @@ gbi_status layout(gbi_ctx& c) {
     c.cell_total = cells;
     c.prec_total = slots;
     HIP_TRY(c.cell_count.ensure(cells));
+    HIP_TRY(c.cell_fill.ensure(cells));
     HIP_TRY(c.cell_start.ensure(cells + 1));
     HIP_TRY(c.cell_count_eff.ensure(cells));
     HIP_TRY(c.cell_pt_count.ensure(cells));
@@ -346,7 +338,6 @@ gbi_status layout(gbi_ctx& c) {
     HIP_TRY(c.cell_sys.ensure(cells));
     HIP_TRY(c.scan_tmp.ensure((cells + 4095) / 4096 + 16));
     HIP_TRY(c.prec.ensure(slots));
-    HIP_TRY(c.cell_atoms_tmp.ensure(raw));
     std::vector<int> cs(cells);
     std::vector<gbi::SysDesc> desc(c.S);
     long long cb = 0, pb = 0;
@@ -478,6 +469,7 @@ gbi_status size_from_device(gbi_ctx& c, long long min_entries) {
         c.drop_graphs();
         HIP_TRY(hipStreamSynchronize(c.stream));
         c.entry_cap = (int)entries;
+        HIP_TRY(c.cell_atoms_tmp.ensure(entries));
         HIP_TRY(c.cell_atoms.ensure(entries));
         HIP_TRY(c.ent_pos.ensure(entries));
         HIP_TRY(c.ent_par.ensure(entries));
@@ -517,6 +509,7 @@ gbi::KParams make_kparams(gbi_ctx& c, bool adjoint) {
     k.pair_tiles = c.pair_tiles.p;
     k.colblk = c.colblk.p;
     k.cell_count = c.cell_count.p;
+    k.cell_fill = c.cell_fill.p;
     k.cell_start = c.cell_start.p;
     k.cell_count_eff = c.cell_count_eff.p;
     k.cell_atoms_tmp = c.cell_atoms_tmp.p;
@@ -622,17 +615,12 @@ gbi_status check_flags(gbi_ctx& c, bool& retry) {
     if (all & gbi::FLAG_GRID_TOO_LARGE)
         return fail(GBI_ERR_GRID_TOO_LARGE,
                     "CharmmGBMVForce: CustomGBIntegral lookup table dimension is too large, check atom positions!");
-    if (all & (gbi::FLAG_CELL_OVERFLOW | gbi::FLAG_ENTRY_OVERFLOW | gbi::FLAG_POINT_OVERFLOW |
-               gbi::FLAG_LIST_OVERFLOW)) {
+    if (all & (gbi::FLAG_CELL_OVERFLOW | gbi::FLAG_ENTRY_OVERFLOW | gbi::FLAG_POINT_OVERFLOW)) {
         bool grew = false;
         for (int s = 0; s < c.S; ++s) {
             SysHost& h = c.sys[s];
             if (G[s].flags & gbi::FLAG_CELL_OVERFLOW) {
                 h.cell_cap *= 2;
-                grew = true;
-            }
-            if (G[s].flags & gbi::FLAG_LIST_OVERFLOW) {
-                h.raw_cap = std::min(2 * h.raw_cap, h.N);
                 grew = true;
             }
             if (G[s].flags & gbi::FLAG_POINT_OVERFLOW) {

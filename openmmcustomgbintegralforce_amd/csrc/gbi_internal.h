@@ -28,11 +28,10 @@ enum : int {
     FLAG_GRID_TOO_LARGE = 4,   // an axis longer than 1000 nm (CustomGBIntegral.cpp:182-183)
     FLAG_NONFINITE_POS = 8,
     FLAG_NONFINITE_RESULT = 16,
-    FLAG_POINT_OVERFLOW = 32,  // more points in a cell than its slots
-    FLAG_LIST_OVERFLOW = 64    // more raw members in a cell than its slots
+    FLAG_POINT_OVERFLOW = 32   // more points in a cell than its slots
 };
-constexpr int kFatalFlags = FLAG_CELL_OVERFLOW | FLAG_ENTRY_OVERFLOW | FLAG_GRID_TOO_LARGE | FLAG_NONFINITE_POS |
-                            FLAG_POINT_OVERFLOW | FLAG_LIST_OVERFLOW;
+constexpr int kFatalFlags =
+    FLAG_CELL_OVERFLOW | FLAG_ENTRY_OVERFLOW | FLAG_GRID_TOO_LARGE | FLAG_NONFINITE_POS | FLAG_POINT_OVERFLOW;
 
 constexpr int kMaxAtomChunk = 256;       // atoms staged in LDS per chunk (volume pass)
 constexpr int kQBits = 11;               // packed point = (atom << 11) | q, Q <= 2047
@@ -79,8 +78,6 @@ struct SysDesc {
     int pt_cap;                // point slots per cell
     long long prec_base;       // slot s of local cell c: prec_base + c * pt_cap + s
     int cap;                   // atoms kept per cell
-    int raw_cap;               // raw member slots per cell (brute: the atom count)
-    long long raw_base;        // raw slot s of local cell c: raw_base + c * raw_cap + s
     int pair_splits;           // pair tiles of row block 0 (the most of any block)
     int pair_k;                // column blocks per pair tile
     int nblk;                  // 64-atom blocks
@@ -140,9 +137,10 @@ struct KParams {
     const int* colblk;       // n_colblocks: first atom of every 64-atom block
     // grid
     int* cell_count;         // cell_total
+    int* cell_fill;          // cell_total
     int* cell_start;         // cell_total + 1
     int* cell_count_eff;     // cell_total
-    int* cell_atoms_tmp;     // raw members, raw_cap slots per cell (SysDesc::raw_base)
+    int* cell_atoms_tmp;     // entry_cap
     int* cell_atoms;         // entry_cap
     float4* ent_pos;         // entry_cap: position relative to the cell corner, .w = atom index bits
     float4* ent_par;         // entry_cap: aparam of the atom

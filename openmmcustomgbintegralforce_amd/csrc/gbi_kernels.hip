@@ -143,6 +143,7 @@ __global__ void k_clear(KParams p) {
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < p.cell_total; i += stride) {
         p.cell_count[i] = 0;
+        p.cell_fill[i] = 0;
         p.cell_pt_count[i] = 0;
     }
     if (blockIdx.x == 0) {
@@ -317,16 +318,36 @@ __device__ inline double atom_pad(const KParams& p, const SysDesc& sd, int j) {
     return sd.grid_kind == GRID_FAITHFUL ? (p.radius[j] + sd.halfdiag) + sd.buffer : p.reach[j];
 }
 
-// One workgroup per atom, lanes over its candidate cells: every member cell counts the atom
-// and stores it in its fixed-capacity raw slots (one atomic per membership; the members are
-// put in index order by k_grid_sort).  Brute systems: one cell in index order, no atomics.
+// One workgroup per atom, lanes over its candidate cells: count the members of each cell.
+__global__ void __launch_bounds__(256) k_grid_count(KParams p) {
+    if (batch_fatal(p)) return;
+    const int j = blockIdx.x;
+    const int s = p.atom_sys[j];
+    const SysDesc& sd = p.sys[s];
+    if (sd.grid_kind == GRID_BRUTE) return;  // counted by k_bbox
+    const SysGeom G = p.geom[s];
+    const double x[3] = {p.pos[3 * j], p.pos[3 * j + 1], p.pos[3 * j + 2]};
+    const double pad = atom_pad(p, sd, j);
+    const CandBox B = candidate_box(sd, G, x, pad);
+    const int ny = B.e[1] - B.b[1] + 1, nz = B.e[2] - B.b[2] + 1;
+    for (int t = threadIdx.x; t < B.count; t += blockDim.x) {
+        const int i = B.b[0] + t / (ny * nz);
+        const int r = t - (i - B.b[0]) * (ny * nz);
+        const int jj = B.b[1] + r / nz;
+        const int k = B.b[2] + (r - (jj - B.b[1]) * nz);
+        if (cell_member(sd, G, x, pad, i, jj, k))
+            atomicAdd(&p.cell_count[sd.cell_base + (i * G.n[1] + jj) * G.n[2] + k], 1);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_grid_fill(KParams p) {
     if (batch_fatal(p)) return;
     const int j = blockIdx.x;
     const int s = p.atom_sys[j];
     const SysDesc& sd = p.sys[s];
     if (sd.grid_kind == GRID_BRUTE) {
-        if (threadIdx.x == 0) p.cell_atoms_tmp[sd.raw_base + (j - sd.atom_begin)] = j;
+        // index order, no atomics
+        if (threadIdx.x == 0) p.cell_atoms_tmp[p.cell_start[sd.cell_base] + (j - sd.atom_begin)] = j;
         return;
     }
     const SysGeom G = p.geom[s];
@@ -340,12 +361,10 @@ __global__ void __launch_bounds__(256) k_grid_fill(KParams p) {
         const int jj = B.b[1] + r / nz;
         const int k = B.b[2] + (r - (jj - B.b[1]) * nz);
         if (cell_member(sd, G, x, pad, i, jj, k)) {
-            const int lc = (i * G.n[1] + jj) * G.n[2] + k;
-            const int slot = atomicAdd(&p.cell_count[sd.cell_base + lc], 1);
-            if (slot < sd.raw_cap)
-                p.cell_atoms_tmp[sd.raw_base + (long long)lc * sd.raw_cap + slot] = j;
-            else
-                raise_flag(p, s, FLAG_LIST_OVERFLOW);
+            const int c = sd.cell_base + (i * G.n[1] + jj) * G.n[2] + k;
+            const int slot = atomicAdd(&p.cell_fill[c], 1);
+            const int idx = p.cell_start[c] + slot;
+            if (idx < p.entry_cap) p.cell_atoms_tmp[idx] = j;
         }
     }
 }
@@ -367,24 +386,22 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
     if (batch_fatal(p)) return;
     const int lane = threadIdx.x;
     for (int c = blockIdx.x; c < p.cell_total; c += gridDim.x) {
-        const int n_raw = p.cell_count[c];
-        if (n_raw == 0) {
+        const int n = p.cell_count[c];
+        if (n == 0) {
             if (lane == 0) p.cell_count_eff[c] = 0;
             continue;
         }
         const int s = p.cell_sys[c];
         const SysDesc& sd = p.sys[s];
-        const int n = min(n_raw, sd.raw_cap);  // more only with a list overflow (fatal, redone)
         const SysGeom G = p.geom[s];
         const int base = p.cell_start[c];
         const int eff = n < sd.cap ? n : sd.cap;
-        const int* raw = p.cell_atoms_tmp + sd.raw_base + (long long)(c - sd.cell_base) * sd.raw_cap;
         double corner[3];
         cell_corner(sd, G, c - sd.cell_base, corner);
         if (sd.grid_kind == GRID_BRUTE) {
-            for (int k = lane; k < eff; k += kWave) write_entry(p, base + k, raw[k], corner);
+            for (int k = lane; k < eff; k += kWave) write_entry(p, base + k, p.cell_atoms_tmp[base + k], corner);
         } else if (n <= kSortLds) {
-            for (int k = lane; k < n; k += kWave) s_in[k] = raw[k];
+            for (int k = lane; k < n; k += kWave) s_in[k] = p.cell_atoms_tmp[base + k];
             __syncthreads();
             for (int k = lane; k < n; k += kWave) {
                 const int a = s_in[k];
@@ -396,9 +413,9 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
             for (int k = lane; k < eff; k += kWave) write_entry(p, base + k, s_out[k], corner);
         } else {
             for (int k = lane; k < n; k += kWave) {
-                const int a = raw[k];
+                const int a = p.cell_atoms_tmp[base + k];
                 int r = 0;
-                for (int m = 0; m < n; ++m) r += raw[m] < a;
+                for (int m = 0; m < n; ++m) r += p.cell_atoms_tmp[base + m] < a;
                 p.cell_atoms[base + r] = a;
             }
             __threadfence_block();
@@ -1357,8 +1374,6 @@ __global__ void __launch_bounds__(64) k_owner_reduce(KParams p) {
         const int pid = pbase + q;
         const double facs = sc * (e4 * p.quad_pw[qb + q] + e7 * p.quad_pw[p.quad_rows + qb + q]);
         const float vv = p.vval[pid];
-        // unmarked points read one shared entry and add nothing (no branch: the loads of
-        // several points stay in flight together)
         const bool marked = __float_as_int(vv) < 0;
         const float4 v = p.vrec[marked ? pid : pbase];
         a[0] += marked ? (double)v.x : 0.0;
@@ -1494,10 +1509,11 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
     const int clear_blocks = std::max(1, std::min((p.cell_total + 255) / 256, 4096));
     hipLaunchKernelGGL(k_clear, dim3(clear_blocks), dim3(256), 0, stream, p);
     hipLaunchKernelGGL(k_bbox, dim3(p.S), dim3(1024), 0, stream, p);
-    hipLaunchKernelGGL(k_grid_fill, dim3(p.N), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(k_grid_count, dim3(p.N), dim3(256), 0, stream, p);
     err = scan_exclusive(p.cell_count, p.cell_start, p.cell_total, p.scan_tmp, p.entry_cap, p.flags_all,
                          FLAG_ENTRY_OVERFLOW, stream);
     if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_grid_fill, dim3(p.N), dim3(256), 0, stream, p);
     hipLaunchKernelGGL(k_grid_sort, dim3(std::max(1, std::min(p.cell_total, 65536))), dim3(64), 0, stream, p);
     mark(1);
     hipLaunchKernelGGL(k_point_bin, dim3(p.N), dim3(256), 0, stream, p);

@@ -122,9 +122,9 @@ def test_regrow_after_expansion():
 
 
 def test_regrow_dense_cell_lists():
-    """A globule compressed to ~10x the density has more raw members per cell than the
-    first sizing allows (and cells past the LDS sort size): the lists grow and the
-    evaluation is redone, matching the oracle."""
+    """A globule compressed to ~10x the density: cells with up to 400 raw members (past
+    the LDS sort size, sorted in global memory) and more list entries than the first
+    sizing allows; the evaluation is redone with larger buffers and matches the oracle."""
     for kind, n, seed in (("gbmv2-50", 400, 21), ("gbsw", 400, 22)):
         s = S.globule(n, seed)
         c = s.pos.mean(axis=0)
