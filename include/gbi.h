@@ -100,7 +100,7 @@ typedef struct gbi_params {
     /* energy E = sum_i P q_i^2/(2 B_i) + sum_{i<j} P q_i q_j / f_ij */
     double prefactor;
     /* performance knobs (0 = automatic) */
-    double tight_grid_length; /* GBSW model: cell size of the exact neighbour grid */
+    double tight_grid_length; /* GBSW model: cell edge of the exact neighbour grid, nm (0: 0.20) */
     int32_t block_volume;     /* threads per workgroup of the volume pass */
     int32_t block_force;      /* threads per workgroup of the chain-rule pass */
 } gbi_params;

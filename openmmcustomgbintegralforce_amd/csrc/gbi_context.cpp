@@ -47,6 +47,9 @@ constexpr double kOne4PiEps0 = 138.935456;  // OpenMM 7.x SimTKOpenMMRealType.h
 constexpr double kTightMargin = 1e-6;       // nm, guards the exact GBSW lists against rounding
 constexpr long long kMaxCells = 16ll * 1024 * 1024;  // limit of the three-pass scan
 constexpr int kDefaultBlocksPerCU = 128;  // about one workgroup per occupied cell: the dispatcher balances them
+// Cell edge of the exact GBSW grid (lists are exact at any edge): 0.20 nm measured fastest at
+// 5k and 25k atoms (0.15 nm: 14 % slower, more distinct cells to bin; 0.30 nm: longer lists).
+constexpr double kDefaultTightGrid = 0.20;
 
 template <class T>
 struct DevBuf {
@@ -712,7 +715,7 @@ gbi_status create_batch(int S, const int32_t* n_atoms, const double* charge, con
         SysHost h;
         h.P = params[s];
         if (h.P.model == GBI_MODEL_GBSW) h.P.integral_type = GBI_INTEGRAL_GBSW;
-        if (h.P.tight_grid_length <= 0.0) h.P.tight_grid_length = 0.15;
+        if (h.P.tight_grid_length <= 0.0) h.P.tight_grid_length = kDefaultTightGrid;
         h.N = n_atoms[s];
         h.atom_begin = atom;
         h.Q = h.P.n_radial * h.P.n_lebedev;
@@ -850,7 +853,7 @@ gbi_status gbi_params_default(int32_t model, int32_t integral_type, gbi_params* 
     P.slope = 1.0;
     P.shift = 0.0;
     P.prefactor = -kOne4PiEps0 * (1.0 - 1.0 / 80.0);  // CharmmReferenceGBSW.cpp:177-178
-    P.tight_grid_length = 0.15;
+    P.tight_grid_length = kDefaultTightGrid;
     if (model == GBI_MODEL_GBSW) {
         // CharmmReferenceGBSW.cpp:20-52: rule 4 (38 points), buffer 0.20 nm, no cap
         P.n_lebedev = 38;
@@ -1112,9 +1115,9 @@ gbi_status gbi_autotune(gbi_ctx* ctx, int32_t evals, gbi_tuning* out) {
     bool tight = false;
     for (const SysHost& s : ctx->sys) tight |= s.kind == gbi::GRID_TIGHT;
     std::vector<gbi_tuning> cands;
-    const double grids_tight[] = {0.15, 0.12, 0.2};
+    const double grids_tight[] = {kDefaultTightGrid, 0.15, 0.25, 0.3};
     for (double g : grids_tight) {
-        if (!tight && g != 0.15) continue;
+        if (!tight && g != grids_tight[0]) continue;
         for (int bpc : {16, 32, 64, 128, 256})
             for (int cols : {0, 1024, 16384}) {
                 gbi_tuning t{};
