@@ -158,7 +158,7 @@ typedef enum gbi_theta {
 /* Launch-shape knobs (gbi_autotune / gbi_set_tuning). */
 typedef struct gbi_tuning {
     int32_t blocks_per_cu;    /* workgroups per compute unit of the per-cell passes */
-    int32_t pair_columns;     /* target columns per pair tile (0: automatic) */
+    int32_t pair_columns;     /* columns per pair tile, in 64-atom blocks of pair_columns/64 (0: automatic) */
     double tight_grid_length; /* nm, cell of the exact GBSW grid (GBSW model systems) */
     double ms_per_eval;       /* gbi_autotune: measured time of the chosen shape */
 } gbi_tuning;

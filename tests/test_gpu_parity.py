@@ -71,3 +71,29 @@ def test_deterministic_bitwise():
     eng.close()
     assert e1 == e2
     assert np.array_equal(f1, f2)
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 65, 129])
+@pytest.mark.parametrize("kind", ["gbsw", "gbmv2"])
+def test_tiny_and_block_edge_systems(kind, n):
+    """Systems of 1, 2 and 7 atoms and at the 64-atom block edges (pair tiles, per-block
+    reductions), with charges that do not cancel so every term is nonzero."""
+    if kind == "gbsw":
+        p = N.default_params(N.MODEL_GBSW)
+    else:
+        p = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV2)
+        p.n_lebedev = 50
+    s = S.globule(max(n, 2), 40 + n)
+    pos, radius = s.pos[:n], s.radius[:n]
+    charge = np.random.default_rng(n).uniform(-0.8, 0.8, n) + 0.3
+    print(f"[parity] tiny {kind} N={n}", flush=True)
+    eng = N.Engine(p, charge, radius)
+    e, f = eng.execute(pos)
+    born = eng.born_radii()
+    eng.close()
+    ref = O.compute(oracle_params(p), pos, charge, radius)
+    fe = float(np.linalg.norm(f - ref.forces) / max(np.linalg.norm(ref.forces), 1e-3))
+    print(f"[parity] tiny {kind} N={n}: energy rel {rel_err(e, ref.energy):.2e} force err {fe:.2e}", flush=True)
+    assert rel_err(e, ref.energy) < ENERGY_TOL
+    assert fe < FORCE_TOL
+    assert np.max(np.abs(born - ref.born) / ref.born) < 1e-5
