@@ -386,7 +386,7 @@ gbi_status build_tiles(gbi_ctx& c) {
         s.nblk = nb;
         s.pair_splits = (nb + K - 1) / K;  // pieces of row block 0, the most of any block
         s.colpart_base = cp;
-        cp += (long long)nb * s.N;
+        cp += 64ll * nb * (nb - 1) / 2;  // row blocks I < J for every column block J
         c.max_splits = std::max(c.max_splits, s.pair_splits);
         for (int I = 0; I < nb; ++I) {
             colblk.push_back(s.atom_begin + 64 * I);

@@ -81,7 +81,7 @@ struct SysDesc {
     int pair_splits;           // pair tiles of row block 0 (the most of any block)
     int pair_k;                // column blocks per pair tile
     int nblk;                  // 64-atom blocks
-    long long colpart_base;    // column partials of row block I, atom j: colpart_base + I * n + (j - atom_begin)
+    long long colpart_base;    // column partials (row block I < block J of atom j): colpart_base + 64 (J(J-1)/2 + I) + j % 64
     int i4, i7;                // which integral is order 4 / 7
     // grid constants (double, computed on the host with the oracle's expressions)
     double g, inv_g, gpad, halfdiag, buffer;

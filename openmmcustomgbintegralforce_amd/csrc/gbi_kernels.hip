@@ -892,6 +892,13 @@ __device__ inline PairTerms pair_terms(double xj, double yj, double zj, double x
     return t;
 }
 
+// Column partial of row block I for local atom j (block J = j / 64 > I): triangular, the
+// row blocks of one column block stored one after the other, 64 atoms each.
+__device__ inline long long colpart_index(const SysDesc& sd, int I, int j) {
+    const long long J = j / kWave;
+    return sd.colpart_base + (long long)kWave * (J * (J - 1) / 2 + I) + (j - kWave * J);
+}
+
 // Lane l receives lane l + 1's value (mod 64): DPP wave_rol:1.
 __device__ inline float rotate_down(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
@@ -906,7 +913,7 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
     const int4 tile = p.pair_tiles[blockIdx.x];
     const int s = p.atom_sys[tile.x];
     const SysDesc& sd = p.sys[s];
-    const int ab = sd.atom_begin, ae = sd.atom_end, ns = ae - ab;
+    const int ab = sd.atom_begin, ae = sd.atom_end;
     const int i = tile.x + lane;
     const bool iv = i < ae;
     const double xi = iv ? p.pos[3 * i] : 0.0, yi = iv ? p.pos[3 * i + 1] : 0.0, zi = iv ? p.pos[3 * i + 2] : 0.0;
@@ -916,7 +923,7 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
     const float Pq = P * qi;
     const float q4Bi = 0.25f / Bi;
     const float kexp = 1.4426950408889634f * q4Bi;
-    const long long cp_row = sd.colpart_base + (long long)((tile.x - ab) / kWave) * ns;
+    const int I_row = (tile.x - ab) / kWave;
     double fx = 0.0, fy = 0.0, fz = 0.0, dedb = 0.0, e = 0.0, dq = 0.0;
     for (int jb = tile.y; jb < tile.z; jb += kWave) {
         const int jj = jb + lane;
@@ -980,7 +987,7 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
                 if (ADJ) cdq = rotate_down(cdq);
             }
             if (jv) {
-                const long long o = cp_row + (jj - ab);
+                const long long o = colpart_index(sd, I_row, jj - ab);
                 const long long T = p.colpart_total;
                 p.colpart[o] = cfx;
                 p.colpart[T + o] = cfy;
@@ -1021,7 +1028,7 @@ __global__ void __launch_bounds__(kChainGroups * kWave) k_atom_chain(KParams p) 
     const int tid = threadIdx.x, lane = tid & (kWave - 1), grp = tid / kWave;
     const int j0 = p.colblk[blockIdx.x];
     const SysDesc& sd = p.sys[p.atom_sys[j0]];
-    const int ab = sd.atom_begin, ns = sd.atom_end - ab;
+    const int ab = sd.atom_begin;
     const int bj = (j0 - ab) / kWave;
     const int i = j0 + lane;
     const bool iv = i < sd.atom_end;
@@ -1039,7 +1046,7 @@ __global__ void __launch_bounds__(kChainGroups * kWave) k_atom_chain(KParams p) 
             if (ADJ) a[5] += in[5 * p.N + i];
         }
         for (int I = grp; I < bj; I += kChainGroups) {
-            const long long o = sd.colpart_base + (long long)I * ns + (i - ab);
+            const long long o = colpart_index(sd, I, i - ab);
             a[0] += p.colpart[o];
             a[1] += p.colpart[T + o];
             a[2] += p.colpart[2 * T + o];
