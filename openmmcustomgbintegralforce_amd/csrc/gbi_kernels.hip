@@ -1145,6 +1145,7 @@ __global__ void __launch_bounds__(64, ADJ ? 2 : 3) k_force(KParams p) {
     __shared__ float q_x[kWave], q_y[kWave], q_z[kWave];
     __shared__ float q_0[kWave], q_1[kWave], q_2[kWave], q_3[kWave], q_4[kWave], q_5[kWave];
     __shared__ int q_own[kWave], q_pid[kWave];
+    __shared__ double s_red[8][kWave];  // one axis of the per-lane atom partials at a time
     if (batch_fatal(p)) return;
     const int lane = threadIdx.x, ag = lane & 7, pg = lane >> 3;
     const int nwork = p.work[VK];
@@ -1222,7 +1223,8 @@ __global__ void __launch_bounds__(64, ADJ ? 2 : 3) k_force(KParams p) {
                     }
                 }
                 __syncthreads();
-                for (int t = 0; t < 8; ++t) {
+                const int tmax = (cntp + 7) >> 3;  // point slots pg + 8 t < cntp exist
+                for (int t = 0; t < tmax; ++t) {
                     const int m = pg + 8 * t;
                     float ox = 0.f, oy = 0.f, oz = 0.f;
                     if (m < cntp) {
@@ -1316,31 +1318,25 @@ __global__ void __launch_bounds__(64, ADJ ? 2 : 3) k_force(KParams p) {
                     }
                 }
             }
-            // exact sums over the point groups (lanes ag + 8 pg)
+            // exact sums over the point groups: list atom ab + L (atom group L & 7, slot L >> 3)
+            // is summed by lane L over the lanes (L & 7) + 8 pg, one axis at a time through LDS;
+            // then every lane adds its atom's totals (the partials are integers on the 2^-24
+            // grid, so any order gives the same sum)
+            const int my_sl = ab + lane;
+            const int my_atom = my_sl < na ? __float_as_int(p.ent_pos[as + my_sl].w) : -1;
+            for (int d = 0; d < (ADJ ? 4 : 3); ++d) {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                for (int d = 0; d < 3; ++d) {
-                    double v = acc[k][d];
-                    v += __shfl_xor(v, 8, kWave);
-                    v += __shfl_xor(v, 16, kWave);
-                    v += __shfl_xor(v, 32, kWave);
-                    acc[k][d] = v;
-                }
-                if (ADJ) {
-                    double v = accr[k];
-                    v += __shfl_xor(v, 8, kWave);
-                    v += __shfl_xor(v, 16, kWave);
-                    v += __shfl_xor(v, 32, kWave);
-                    accr[k] = v;
-                }
-            }
-            if (pg == 0) {
+                for (int k = 0; k < 8; ++k) s_red[k][lane] = d < 3 ? acc[k][d] : accr[k];
+                __syncthreads();
+                double tot = 0.0;
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    if (aj[k] < 0) continue;
-                    for (int d = 0; d < 3; ++d)
-                        if (acc[k][d] != 0.0) add_fix(&p.force_fix[d * p.N + aj[k]], acc[k][d]);
-                    if (ADJ && accr[k] != 0.0) add_fix(&p.dedr_fix[aj[k]], accr[k]);
+                for (int g = 0; g < 8; ++g) tot += s_red[lane >> 3][(lane & 7) + 8 * g];
+                __syncthreads();
+                if (my_atom >= 0 && tot != 0.0) {
+                    if (d < 3)
+                        add_fix(&p.force_fix[d * p.N + my_atom], tot);
+                    else
+                        add_fix(&p.dedr_fix[my_atom], tot);
                 }
             }
         }
