@@ -41,6 +41,11 @@ WORKLOADS = {
                             "0.15 nm lookup grid, which is complete for this force)"),
     "gbsw-2500": dict(model="gbsw", integral=0, n_atoms=2500, n_lebedev=38, seed=0,
                       desc="CharmmGBSWForce, 2,500-atom globule, 24x38 quadrature, all pairs"),
+    "adjoint-2000x8": dict(model="adjoint", integral=0, n_atoms=2000, n_lebedev=38, seed=26,
+                           desc="CharmmGBSWForce adjoint: 2,000-atom globule, 8 conformations (0.02 nm Gaussian "
+                                "perturbations of one base geometry) batched in one context, 24x38 quadrature; "
+                                "energy, forces, dE/dR, dE/dq and dE/d(GB constants) per conformation, beside "
+                                "the same batch without the adjoint"),
     "batch64": dict(model="batch", integral=-1, n_atoms=0, n_lebedev=0, seed=0,
                     desc="64 systems in one batched context, N ~ U[300, 6000] (seeded), alternating "
                          "CharmmGBSWForce 24x38 and CharmmGBMVForce GBMVIntegralTypeII 24x50, hipGraph replay; "
@@ -80,6 +85,8 @@ def main() -> int:
     w = WORKLOADS[args.workload]
     if w["model"] == "batch":
         return run_batch(args, w)
+    if w["model"] == "adjoint":
+        return run_adjoint(args, w)
     kind = kind_name(w)
     sysm = S.globule(w["n_atoms"], w["seed"])
     if w["model"] == "gbsw":
@@ -197,6 +204,102 @@ def main() -> int:
     }
     eng.close()
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def run_adjoint(args, w) -> int:
+    """One step = energy, forces and the adjoint of all 8 conformations (one batched context,
+    graph replay); value = conformation evaluations per second."""
+    from openmmcustomgbintegralforce_amd import _native as N
+    from openmmcustomgbintegralforce_amd import synthetic as S
+
+    base = S.globule(w["n_atoms"], w["seed"])
+    confs = S.conformations(base, 8, seed=w["seed"] + 1)
+    p = N.default_params(N.MODEL_GBSW)
+    p.n_lebedev = w["n_lebedev"]
+    n = base.n
+    pos = np.concatenate([c.pos + 40.0 * k for k, c in enumerate(confs)])
+    eng = N.Engine.batch([p] * 8, [c.charge for c in confs], [c.radius for c in confs])
+    out = eng.execute_adjoint(pos)
+    eng.set_graph(True)
+    eng.set_positions(pos)
+
+    def timed(flags):
+        for _ in range(args.warmup):
+            eng.compute(flags)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            eng.compute(flags)
+        eng.synchronize()
+        return time.perf_counter() - t0
+
+    t_adj = timed(N.FORCES | N.ENERGY | N.ADJOINT)
+    t_plain = timed(N.FORCES | N.ENERGY)
+    eng.close()
+
+    acc = {}
+    if not args.no_accuracy:
+        import oracle
+
+        # size-independent identity on every conformation: E is homogeneous of degree 2 in q
+        e = out["energy"]
+        q_res = max(abs(float(np.dot(confs[k].charge, out["dEdq"][k * n:(k + 1) * n])) - 2.0 * e[k]) / abs(e[k])
+                    for k in range(8))
+        op = oracle.params_from_abi(p, nthreads=min(16, os.cpu_count() or 1))
+        c0 = confs[0]
+        ref = oracle.compute(op, c0.pos, c0.charge, c0.radius)
+        # central differences of the FP64 oracle energy: two constants and four radii of conformation 0
+        fd_err = 0.0
+        for name, field in (("alpha0", "alpha0"), ("rscale", "rscale")):
+            v = getattr(op, field)
+            h = 1e-5 * abs(v)
+            setattr(op, field, v + h)
+            ep = oracle.energy(op, c0.pos, c0.charge, c0.radius)
+            setattr(op, field, v - h)
+            em = oracle.energy(op, c0.pos, c0.charge, c0.radius)
+            setattr(op, field, v)
+            fd = (ep - em) / (2 * h)
+            fd_err = max(fd_err, abs(out["dEdtheta"][name][0] - fd) / abs(fd))
+        atoms = np.random.default_rng(3).choice(n, 4, replace=False)
+        fd_r = []
+        for j in atoms:
+            R = c0.radius.copy()
+            R[j] += 1e-4
+            ep = oracle.energy(op, c0.pos, c0.charge, R)
+            R[j] -= 2e-4
+            em = oracle.energy(op, c0.pos, c0.charge, R)
+            fd_r.append((ep - em) / 2e-4)
+        fd_r = np.array(fd_r)
+        fd_err = max(fd_err, float(np.max(np.abs(out["dEdR"][atoms] - fd_r)) / np.max(np.abs(fd_r))))
+        acc = {"force_rel_rms": float(np.linalg.norm(out["forces"][:n] - ref.forces) / np.linalg.norm(ref.forces)),
+               "energy_rel_err": float(abs(e[0] - ref.energy) / abs(ref.energy)),
+               "adjoint_fd_max_rel_err": fd_err, "adjoint_fd_checked": "dE/dalpha0, dE/drscale, dE/dR of 4 atoms",
+               "charge_identity_max_rel_residual": q_res}
+    value = 8 * args.steps / t_adj
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_adj / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": f"synthetic: seeded {n}-atom globule (synthetic.globule seed {w['seed']}) and 8 conformations "
+                f"(synthetic.conformations seed {w['seed'] + 1})",
+        "config": {"workload": args.workload, "description": w["desc"], "n_atoms": n, "conformations": 8,
+                   "quadrature": "24x38"},
+        **acc,
+        "plain_evals_per_s": 8 * args.steps / t_plain,
+        "adjoint_overhead": t_adj / t_plain,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res), flush=True)
     return 0
 
 
