@@ -60,7 +60,13 @@ class Params(ctypes.Structure):
         ("slope", ctypes.c_double),
         ("shift", ctypes.c_double),
         ("prefactor", ctypes.c_double),
+        ("method", ctypes.c_int),
+        ("cutoff", ctypes.c_double),
+        ("box", ctypes.c_double * 3),
     ]
+
+
+NO_CUTOFF, CUTOFF_NON_PERIODIC, CUTOFF_PERIODIC = 0, 1, 2
 
 
 class Stats(ctypes.Structure):
@@ -146,9 +152,10 @@ def gbmv_params(integral_type: int, **kw) -> Params:
 def _apply(p: Params, kw) -> Params:
     for k, v in kw.items():
         key = "lambda_" if k == "lambda" else k
-        if key == "orders":
+        if key in ("orders", "box"):
+            arr = getattr(p, key)
             for i, o in enumerate(v):
-                p.orders[i] = o
+                arr[i] = o
         else:
             setattr(p, key, v)
     return p
@@ -159,9 +166,10 @@ _SHARED_FIELDS = ["n_radial", "n_lebedev", "r0", "r1", "grid_length", "buffer", 
                   "shift", "prefactor"]
 
 
-def params_from_abi(p, nthreads: int = 8, dense: int = 0) -> Params:
+def params_from_abi(p, nthreads: int = 8, dense: int = 0, box=None) -> Params:
     """Oracle parameters for the model described by a gbi_params structure `p`
-    (read by attribute name: model, integral_type, lookup_policy, cell_cap, orders, ...)."""
+    (read by attribute name: model, integral_type, lookup_policy, cell_cap, orders,
+    nonbonded_method, cutoff, ...); `box`: the three edges of a periodic box."""
     op = gbsw_standalone_params() if p.model == 0 else gbmv_params(p.integral_type)
     for f in _SHARED_FIELDS:
         setattr(op, f, getattr(p, f))
@@ -170,6 +178,11 @@ def params_from_abi(p, nthreads: int = 8, dense: int = 0) -> Params:
     op.policy = POLICY_BRUTE if p.lookup_policy == 1 else POLICY_GRID
     op.nthreads = nthreads
     op.dense = dense
+    op.method = p.nonbonded_method
+    op.cutoff = p.cutoff
+    if box is not None:
+        for d in range(3):
+            op.box[d] = float(box[d])
     return op
 
 

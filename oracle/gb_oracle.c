@@ -259,6 +259,37 @@ void orc_defaults_gbmv(orc_params* p, int integral_type) {
 /* Lookup grids                                                              */
 /* ------------------------------------------------------------------------- */
 
+/* d = a - b, reduced to the minimum image in a rectangular periodic box: OpenMM's
+ * ReferenceForce::getDeltaRPeriodic, d -= L floor(d/L + 1/2) per axis (called with
+ * (point, atom) at GBSWIntegral.cpp:168-169, CharmmReferenceGBSW.cpp:620-623 and with
+ * (atom i, atom j) at CharmmReferenceGBSW.cpp:242-245, CharmmReferenceGBMV.cpp:295-296). */
+static inline void delta(const orc_params* p, const double* a, const double* b, double* dx, double* dy,
+                         double* dz) {
+    double d[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+    if (p->method == ORC_CUTOFF_PERIODIC)
+        for (int k = 0; k < 3; ++k) d[k] -= p->box[k] * floor(d[k] / p->box[k] + 0.5);
+    *dx = d[0];
+    *dy = d[1];
+    *dz = d[2];
+}
+
+/* Squared distance of atoms i and j (x_j - x_i, minimum image), summed as OpenMM's DOT3. */
+static inline double pair_r2(const orc_params* p, const double* pos, int i, int j) {
+    double dx, dy, dz;
+    delta(p, pos + 3 * j, pos + 3 * i, &dx, &dy, &dz);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+/* Pair (i, j) inside the cutoff.  The standalone GBSW force skips r > c
+ * (CharmmReferenceGBSW.cpp:246-247, :290-291); the CharmmGBMVForce pair terms skip r >= c
+ * (CharmmReferenceGBMV.cpp:300-301).  Compared on r^2 against c^2: the same decision except
+ * where sqrt rounding ties a distance to the cutoff itself. */
+static inline int within_cutoff(const orc_params* p, double r2) {
+    if (p->method == ORC_NO_CUTOFF) return 1;
+    const double c2 = p->cutoff * p->cutoff;
+    return p->model == ORC_MODEL_GBSW_STANDALONE ? !(r2 > c2) : r2 < c2;
+}
+
 typedef struct {
     double min[3], max[3], g;
     int n[3];
@@ -267,6 +298,8 @@ typedef struct {
     int* atoms;  /* CSR, ascending atom index per cell */
     int brute;   /* every point sees every atom */
     int N;
+    int periodic;            /* cells tile the box; points are wrapped into it */
+    double L[3], step[3];    /* periodic: box edges and cell edges L / n */
 } grid_t;
 
 static void grid_free(grid_t* G) {
@@ -274,6 +307,96 @@ static void grid_free(grid_t* G) {
     free(G->atoms);
     G->start = NULL;
     G->atoms = NULL;
+}
+
+/* Periodic lookup grid for a rectangular box (CustomGBIntegral.cpp:45-160): origin at the
+ * centre of geometry minus half the box (:53-79), n = ceil(L/g) + 1 cells per axis of edge
+ * L/n (:87-107), pad R_j + sqrt3/2 max(L/n) + buffer (:114-119).  Membership follows the
+ * non-periodic corner rule, taken at the image of each cell nearest the atom: atom j,
+ * wrapped into the box, joins the cells of the unwrapped index range around it, each index
+ * reduced modulo n.  (The reference tests an atom-anchored sample lattice walked with
+ * floating accumulation, :126-157, whose offsets inside the cells are an accident of
+ * rounding.)  Atoms in ascending index, cap in index order.  The standalone GBSW model uses
+ * its one pad (CharmmReferenceGBSW.cpp:454-455) with the cell edge max(L/n); its own
+ * periodic lookup keeps the non-periodic grid and shifts only points outside it
+ * (:527-536), which misses the images of atoms across the box faces. */
+static int grid_build_periodic(grid_t* G, const orc_params* p, int N, const double* pos, const double* R) {
+    const int standalone = (p->model == ORC_MODEL_GBSW_STANDALONE);
+    const double g = p->grid_length;
+    double cog[3] = {0.0, 0.0, 0.0}, maxR = 0.0, maxstep = 0.0;
+    for (int a = 0; a < N; ++a) {
+        for (int d = 0; d < 3; ++d) cog[d] += pos[3 * a + d];
+        if (R[a] > maxR) maxR = R[a];
+    }
+    long long ncell = 1;
+    for (int d = 0; d < 3; ++d) {
+        cog[d] /= N;
+        const double L = p->box[d];
+        G->L[d] = L;
+        G->n[d] = (int)ceil(L / g) + 1;
+        G->step[d] = L / G->n[d];
+        G->min[d] = cog[d] - L / 2.0;
+        G->max[d] = G->min[d] + L;
+        if (G->step[d] > maxstep) maxstep = G->step[d];
+        ncell *= G->n[d];
+    }
+    G->periodic = 1;
+    G->g = maxstep;
+    G->ncell = ncell;
+    const double halfdiag = sqrt(3.0) / 2.0 * maxstep;
+    const double gpad = p->buffer + maxR + p->sw + halfdiag + 1e-6;
+    /* every cell's list must hold one image of an atom at most, and that image must be the
+     * nearest to every point of the cell */
+    const double padmax = standalone ? gpad : maxR + halfdiag + p->buffer;
+    for (int d = 0; d < 3; ++d)
+        if (!(G->L[d] > 2.0 * (padmax + sqrt(3.0) * maxstep))) return ORC_ERR_ARG;
+    const int cap = p->cap > 0 ? p->cap : INT32_MAX;
+    int* count = calloc((size_t)ncell, sizeof(int));
+    G->start = calloc((size_t)ncell + 1, sizeof(int));
+    if (!count || !G->start) { free(count); return ORC_ERR_MEM; }
+    const int nx = G->n[0], ny = G->n[1], nz = G->n[2];
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            long long s = 0;
+            for (long long c = 0; c < ncell; ++c) { G->start[c] = (int)s; s += count[c]; }
+            G->start[ncell] = (int)s;
+            G->atoms = malloc(sizeof(int) * (size_t)(s > 0 ? s : 1));
+            if (!G->atoms) { free(count); return ORC_ERR_MEM; }
+            memset(count, 0, sizeof(int) * (size_t)ncell);
+        }
+        for (int a = 0; a < N; ++a) {
+            double xw[3];
+            for (int d = 0; d < 3; ++d)
+                xw[d] = pos[3 * a + d] - G->L[d] * floor((pos[3 * a + d] - G->min[d]) / G->L[d]);
+            const double pad = standalone ? gpad : R[a] + halfdiag + p->buffer;
+            int b[3], e[3];
+            for (int d = 0; d < 3; ++d) {
+                b[d] = (int)floor((xw[d] - pad - G->min[d]) / G->step[d]);
+                e[d] = (int)ceil((xw[d] + pad - G->min[d]) / G->step[d]);
+            }
+            for (int i = b[0]; i <= e[0]; ++i)
+                for (int j = b[1]; j <= e[1]; ++j)
+                    for (int k = b[2]; k <= e[2]; ++k) {
+                        double dx = (G->min[0] + i * G->step[0]) - xw[0];
+                        double dy = (G->min[1] + j * G->step[1]) - xw[1];
+                        double dz = (G->min[2] + k * G->step[2]) - xw[2];
+                        int member;
+                        if (standalone)
+                            member = sqrt(dx * dx + dy * dy + dz * dz) < pad;
+                        else
+                            member = dx * dx + dy * dy + dz * dz < pad * pad;
+                        if (!member) continue;
+                        const int iw = ((i % nx) + nx) % nx, jw = ((j % ny) + ny) % ny, kw = ((k % nz) + nz) % nz;
+                        long long c = ((long long)iw * ny + jw) * nz + kw;
+                        if (count[c] < cap) {
+                            if (pass == 1) G->atoms[G->start[c] + count[c]] = a;
+                            count[c]++;
+                        }
+                    }
+        }
+    }
+    free(count);
+    return 0;
 }
 
 /* Atom j joins cell (i,j,k) when the cell's lower corner lies within pad of x_j;
@@ -289,6 +412,7 @@ static int grid_build(grid_t* G, const orc_params* p, int N, const double* pos, 
         G->brute = 1;
         return 0;
     }
+    if (p->method == ORC_CUTOFF_PERIODIC) return grid_build_periodic(G, p, N, pos, R);
     const int standalone = (p->model == ORC_MODEL_GBSW_STANDALONE);
     const double g = p->grid_length;
     double mn[3], mx[3];
@@ -371,6 +495,16 @@ static int grid_build(grid_t* G, const orc_params* p, int N, const double* pos, 
  * (CustomGBIntegral.cpp:310-324, CharmmReferenceGBSW.cpp:538-549). */
 static long long grid_cell(const grid_t* G, const double* pt) {
     int idx[3];
+    if (G->periodic) {
+        /* the point wrapped into the box (CustomGBIntegral.cpp:294-301) */
+        for (int d = 0; d < 3; ++d) {
+            const double pw = pt[d] - floor((pt[d] - G->min[d]) / G->L[d]) * G->L[d];
+            idx[d] = (int)floor((pw - G->min[d]) / G->step[d]);
+            if (idx[d] < 0) idx[d] = 0;
+            if (idx[d] > G->n[d] - 1) idx[d] = G->n[d] - 1;
+        }
+        return ((long long)idx[0] * G->n[1] + idx[1]) * G->n[2] + idx[2];
+    }
     for (int d = 0; d < 3; ++d) {
         if (pt[d] < G->min[d] || pt[d] > G->max[d]) return -1;
         idx[d] = (int)floor((pt[d] - G->min[d]) / G->g);
@@ -409,7 +543,8 @@ static double gbsw_volume(const orc_params* p, const double* pos, const double* 
     double V = 1.0;
     for (int k = 0; k < n; ++k) {
         int j = list[k];
-        double dx = pos[3 * j] - pt[0], dy = pos[3 * j + 1] - pt[1], dz = pos[3 * j + 2] - pt[2];
+        double dx, dy, dz;
+        delta(p, pos + 3 * j, pt, &dx, &dy, &dz);
         double r = sqrt(dx * dx + dy * dy + dz * dz);
         double Rj = (R[j] + p->roffset) * p->rscale;
         if (r <= Rj - p->sw) return 0.0;
@@ -423,15 +558,19 @@ static double gbsw_volume(const orc_params* p, const double* pos, const double* 
 
 /* dI/dx for I += pref (1 - V): g_j = -pref V H'_j/H_j (x_j - p)/d for shell atoms j != owner,
  * g_owner = -sum_j g_j (GBSWIntegral.cpp:271-314; CharmmReferenceGBSW.cpp:641-674).
- * Adds scale * g to out (N x 3). */
+ * Adds scale * g to out (N x 3).  trunc: the standalone force with a cutoff applies
+ * dB_i/dx_j only to atoms j within the cutoff of the owner i, while the owner keeps the
+ * whole sum (CharmmReferenceGBSW.cpp:285-296 skips r_ij > c; dB_i/dx_i, r = 0, collects
+ * every j, :670-671). */
 static int gbsw_gradient(const orc_params* p, const double* pos, const double* R, const double* pt,
-                         int owner, const int* list, int n, double V, double pref, double* out) {
+                         int owner, const int* list, int n, double V, double pref, double* out, int trunc) {
     if (V == 0.0) return 0;
     int m = 0;
     for (int k = 0; k < n; ++k) {
         int j = list[k];
         if (j == owner) continue;
-        double dx = pos[3 * j] - pt[0], dy = pos[3 * j + 1] - pt[1], dz = pos[3 * j + 2] - pt[2];
+        double dx, dy, dz;
+        delta(p, pos + 3 * j, pt, &dx, &dy, &dz);
         double r = sqrt(dx * dx + dy * dy + dz * dz);
         double Rj = (R[j] + p->roffset) * p->rscale;
         if (!(r > Rj - p->sw && r < Rj + p->sw)) continue;
@@ -441,10 +580,11 @@ static int gbsw_gradient(const orc_params* p, const double* pos, const double* R
         out[3 * owner] += dx * f;
         out[3 * owner + 1] += dy * f;
         out[3 * owner + 2] += dz * f;
+        ++m;
+        if (trunc && !within_cutoff(p, pair_r2(p, pos, owner, j))) continue;
         out[3 * j] -= dx * f;
         out[3 * j + 1] -= dy * f;
         out[3 * j + 2] -= dz * f;
-        ++m;
     }
     return m;
 }
@@ -456,7 +596,8 @@ static double gbmv1_pre(const orc_params* p, const double* pos, const double* R,
     double S = 0.0;
     for (int k = 0; k < n; ++k) {
         int j = list[k];
-        double dx = pos[3 * j] - pt[0], dy = pos[3 * j + 1] - pt[1], dz = pos[3 * j + 2] - pt[2];
+        double dx, dy, dz;
+        delta(p, pos + 3 * j, pt, &dx, &dy, &dz);
         double r = sqrt(dx * dx + dy * dy + dz * dz);
         double r4 = r * r * r * r;
         double R4 = R[j] * R[j] * R[j] * R[j];
@@ -473,7 +614,8 @@ static int gbmv1_gradient(const orc_params* p, const double* pos, const double* 
     for (int k = 0; k < n; ++k) {
         int j = list[k];
         if (j == owner) continue; /* its two terms cancel in the reference */
-        double dx = pos[3 * j] - pt[0], dy = pos[3 * j + 1] - pt[1], dz = pos[3 * j + 2] - pt[2];
+        double dx, dy, dz;
+        delta(p, pos + 3 * j, pt, &dx, &dy, &dz);
         double r = sqrt(dx * dx + dy * dy + dz * dz);
         double r3 = r * r * r, r4 = r3 * r;
         double R4 = R[j] * R[j] * R[j] * R[j];
@@ -503,7 +645,8 @@ static void gbmv2_point(const orc_params* p, const double* pos, const double* R,
     double sum1 = 0.0, sum2 = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0;
     for (int k = 0; k < n; ++k) {
         int j = list[k];
-        double dx = pos[3 * j] - pt[0], dy = pos[3 * j + 1] - pt[1], dz = pos[3 * j + 2] - pt[2];
+        double dx, dy, dz;
+        delta(p, pos + 3 * j, pt, &dx, &dy, &dz);
         double d2 = dx * dx + dy * dy + dz * dz;
         double C = p->P1 * R[j] * 10.0 + p->P2;
         double F = C / (C + d2 * 100.0 - R[j] * R[j] * 100.0);
@@ -540,7 +683,8 @@ static int gbmv2_gradient(const orc_params* p, const double* pos, const double* 
     for (int k = 0; k < n; ++k) {
         int j = list[k];
         if (j == owner) continue; /* its terms cancel in the reference */
-        double dx = pos[3 * j] - pt[0], dy = pos[3 * j + 1] - pt[1], dz = pos[3 * j + 2] - pt[2];
+        double dx, dy, dz;
+        delta(p, pos + 3 * j, pt, &dx, &dy, &dz);
         double d2 = dx * dx + dy * dy + dz * dz;
         double C = p->P1 * R[j] * 10.0 + p->P2;
         double t = C + d2 * 100.0 - R[j] * R[j] * 100.0;
@@ -583,18 +727,21 @@ static void born_map(const orc_params* p, double I4, double I7, double* B, doubl
 /* STILL pair + self energy (CharmmReferenceGBSW.cpp:237-283):
  * E = sum_i P q_i^2/(2 B_i) + sum_{i<j} P q_i q_j / f_ij,
  * f_ij = sqrt(r^2 + B_i B_j exp(-r^2/(4 B_i B_j))).  Adds pair forces to F,
- * overwrites dEdB. */
-static double still_energy(int N, const double* pos, const double* q, const double* B, double P,
-                           double* F, double* dEdB) {
+ * overwrites dEdB.  With a cutoff only pairs inside it count (minimum image when
+ * periodic); the self term always does (CharmmReferenceGBSW.cpp:188-199: every atom
+ * pairs with itself; CharmmReferenceGBMV.cpp: it is a SingleParticle term). */
+static double still_energy(const orc_params* p, int N, const double* pos, const double* q, const double* B,
+                           double P, double* F, double* dEdB) {
     double E = 0.0;
     for (int i = 0; i < N; ++i) dEdB[i] = 0.0;
     for (int i = 0; i < N; ++i) {
         E += 0.5 * P * q[i] * q[i] / B[i];
         dEdB[i] += -0.5 * P * q[i] * q[i] / (B[i] * B[i]);
         for (int j = i + 1; j < N; ++j) {
-            double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
-                   dz = pos[3 * j + 2] - pos[3 * i + 2];
+            double dx, dy, dz;
+            delta(p, pos + 3 * j, pos + 3 * i, &dx, &dy, &dz);
             double r2 = dx * dx + dy * dy + dz * dz;
+            if (!within_cutoff(p, r2)) continue;
             double BB = B[i] * B[j];
             double D = r2 / (4.0 * BB);
             double ex = exp(-D);
@@ -630,6 +777,7 @@ typedef struct {
     double* pw;          /* nint x Q : w_q / r_q^(order-2) */
     grid_t G;
     int* brute_list;
+    int trunc;  /* standalone GBSW with a cutoff: truncated chain rule */
 } ctx_t;
 
 /* V of one point and its list; kind-specific sums in *s2 / *pre. */
@@ -659,7 +807,7 @@ static int point_gradient(const ctx_t* c, const double* pt, int owner, const int
     const orc_params* p = c->p;
     switch (p->integral_type) {
     case ORC_INT_GBSW:
-        return n == 0 ? 0 : gbsw_gradient(p, c->pos, c->R, pt, owner, list, n, V, pref, out);
+        return n == 0 ? 0 : gbsw_gradient(p, c->pos, c->R, pt, owner, list, n, V, pref, out, c->trunc);
     case ORC_INT_GBMV1:
         return gbmv1_gradient(p, c->pos, c->R, pt, owner, list, n, pre, pref, out);
     default:
@@ -671,7 +819,8 @@ static int gbsw_exact_count(const ctx_t* c, const double* pt, const int* list, i
     int m = 0;
     for (int k = 0; k < n; ++k) {
         int j = list[k];
-        double dx = c->pos[3 * j] - pt[0], dy = c->pos[3 * j + 1] - pt[1], dz = c->pos[3 * j + 2] - pt[2];
+        double dx, dy, dz;
+        delta(c->p, c->pos + 3 * j, pt, &dx, &dy, &dz);
         double r = sqrt(dx * dx + dy * dy + dz * dz);
         double Rj = (c->R[j] + c->p->roffset) * c->p->rscale;
         if (r < Rj + c->p->sw) ++m;
@@ -696,8 +845,15 @@ int orc_compute(const orc_params* p, int N, const double* pos, const double* cha
     }
     if (i4 < 0 || i7 < 0) return ORC_ERR_ARG; /* canonical Born map needs orders 4 and 7 */
 
+    if (pp.method < ORC_NO_CUTOFF || pp.method > ORC_CUTOFF_PERIODIC) return ORC_ERR_ARG;
+    if (pp.method != ORC_NO_CUTOFF && !(pp.cutoff > 0.0)) return ORC_ERR_ARG;
+    if (pp.method == ORC_CUTOFF_PERIODIC)
+        for (int d = 0; d < 3; ++d) /* CharmmGBMVForceImpl.cpp:99-104, CharmmGBSWForceImpl.cpp:48-54 */
+            if (!(pp.box[d] >= 2.0 * pp.cutoff)) return ORC_ERR_ARG;
+
     ctx_t c;
     memset(&c, 0, sizeof(c));
+    c.trunc = standalone && pp.method != ORC_NO_CUTOFF;
     c.p = &pp;
     c.N = N;
     c.pos = pos;
@@ -792,22 +948,25 @@ int orc_compute(const orc_params* p, int N, const double* pos, const double* cha
                 double V = gbsw_volume(&pp, pos, radius, pt, list, n);
                 double pref = dB[i4 * N + i] * c.pw[i4 * Q + q] + dB[i7 * N + i] * c.pw[i7 * Q + q];
                 /* gbsw_gradient returns dI/dx for I += pref(1-V); dB/dx has the same form */
-                gbsw_gradient(&pp, pos, radius, pt, i, list, n, V, pref, dense + (size_t)i * N * 3);
+                gbsw_gradient(&pp, pos, radius, pt, i, list, n, V, pref, dense + (size_t)i * N * 3, 0);
             }
         }
     }
 
     /* ---- pair + self energy ---- */
-    double E = still_energy(N, pos, charge, B, pp.prefactor, forces, dEdB);
+    double E = still_energy(&pp, N, pos, charge, B, pp.prefactor, forces, dEdB);
     for (int k = 0; k < 2; ++k)
         for (int i = 0; i < N; ++i) dEdI[k * N + i] = dEdB[i] * dB[k * N + i];
 
     /* ---- chain rule ---- */
     if (pp.dense) {
         if (standalone) {
+            /* CharmmReferenceGBSW.cpp:285-296: with a cutoff, j beyond it of i is skipped */
             for (int i = 0; i < N; ++i)
-                for (int j = 0; j < N; ++j)
+                for (int j = 0; j < N; ++j) {
+                    if (c.trunc && j != i && !within_cutoff(&pp, pair_r2(&pp, pos, i, j))) continue;
                     for (int d = 0; d < 3; ++d) forces[3 * j + d] -= dEdB[i] * dense[((size_t)i * N + j) * 3 + d];
+                }
         } else {
             for (int i = 0; i < N; ++i)
                 for (int j = 0; j < N; ++j)

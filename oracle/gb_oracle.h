@@ -49,7 +49,13 @@ typedef struct {
     double alpha0, alpha1, eta, slope, shift;
     /* energy prefactor P: E = sum_i P q_i^2 / (2 B_i) + sum_{i<j} P q_i q_j / f_ij */
     double prefactor;
+    /* nonbonded method (CharmmGBMVForce.h / CharmmGBSWForce.h NonbondedMethod) */
+    int method;         /* ORC_NO_CUTOFF, ORC_CUTOFF_NON_PERIODIC, ORC_CUTOFF_PERIODIC */
+    double cutoff;      /* nm */
+    double box[3];      /* rectangular periodic box edges (ORC_CUTOFF_PERIODIC) */
 } orc_params;
+
+enum { ORC_NO_CUTOFF = 0, ORC_CUTOFF_NON_PERIODIC = 1, ORC_CUTOFF_PERIODIC = 2 };
 
 typedef struct {
     double energy;

@@ -6,9 +6,10 @@ import numpy as np
 import oracle as O
 
 
-def oracle_params(p, nthreads: int = 8, dense: int = 0) -> O.Params:
-    """Oracle parameters describing the same model as the ABI parameters `p`."""
-    return O.params_from_abi(p, nthreads=nthreads, dense=dense)
+def oracle_params(p, nthreads: int = 8, dense: int = 0, box=None) -> O.Params:
+    """Oracle parameters describing the same model as the ABI parameters `p` (`box`: the
+    edges of the periodic box of a CutoffPeriodic system)."""
+    return O.params_from_abi(p, nthreads=nthreads, dense=dense, box=box)
 
 
 def rel_rms(f, ref) -> float:
