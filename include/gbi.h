@@ -29,12 +29,12 @@
 extern "C" {
 #endif
 
-#define GBI_ABI_VERSION 2
+#define GBI_ABI_VERSION 3
 
 typedef enum gbi_status {
     GBI_OK = 0,
     GBI_ERR_INVALID_ARGUMENT = 1,
-    GBI_ERR_UNSUPPORTED = 2,       /* e.g. CustomIntegral (type 3) or a cutoff method */
+    GBI_ERR_UNSUPPORTED = 2,       /* e.g. CustomIntegral (type 3), a triclinic box */
     GBI_ERR_PARTICLE_COUNT = 3,    /* updateParametersInContext with another N */
     GBI_ERR_HIP = 4,               /* a HIP runtime call failed */
     GBI_ERR_OUT_OF_MEMORY = 5,
@@ -64,11 +64,20 @@ typedef enum gbi_lookup_policy {
     GBI_LOOKUP_BRUTE = 1      /* every atom (the reference's computeVolume paths); small N only */
 } gbi_lookup_policy;
 
-/* CalcCharmmGBMVForceKernel::NonbondedMethod (charmmKernels.h:52-56). */
+/*
+ * CalcCharmmGBMVForceKernel::NonbondedMethod (charmmKernels.h:52-56).  With a cutoff, the
+ * STILL pair terms count only pairs inside it: r <= c for GBI_MODEL_GBSW
+ * (CharmmReferenceGBSW.cpp:246-247), r < c for GBI_MODEL_GBMV (CharmmReferenceGBMV.cpp:300-301);
+ * the self terms always count.  GBI_MODEL_GBSW also truncates its chain rule: dB_i/dx_j
+ * reaches only atoms j within the cutoff of i (CharmmReferenceGBSW.cpp:285-296), so its
+ * forces are not the exact gradient; GBI_MODEL_GBMV's chain rule is complete.  Periodic
+ * systems take minimum-image distances everywhere and tile their rectangular box with the
+ * lookup grid (gbi_set_periodic_box).  The adjoint (GBI_ADJOINT) is NoCutoff only.
+ */
 typedef enum gbi_nonbonded_method {
     GBI_NO_CUTOFF = 0,
-    GBI_CUTOFF_NON_PERIODIC = 1, /* not yet supported on this platform */
-    GBI_CUTOFF_PERIODIC = 2      /* not yet supported on this platform */
+    GBI_CUTOFF_NON_PERIODIC = 1,
+    GBI_CUTOFF_PERIODIC = 2
 } gbi_nonbonded_method;
 
 /*
@@ -198,6 +207,16 @@ gbi_status gbi_execute(gbi_ctx* ctx, const double* pos, int32_t include_forces, 
  * same N; GBI_ERR_PARTICLE_COUNT when n_atoms differs.
  */
 gbi_status gbi_update_parameters(gbi_ctx* ctx, int32_t n_atoms, const double* charge, const double* radius);
+
+/*
+ * The periodic box of a GBI_CUTOFF_PERIODIC system: box_vectors = 3 x 3 doubles (a, b, c
+ * row-major), read by the reference kernels from the Context every evaluation
+ * (GBSWIntegral.cpp:108-110, CharmmReferenceKernels.cpp:351, :430).  Rectangular only
+ * (GBI_ERR_UNSUPPORTED otherwise, as CustomGBIntegral.cpp:47-50); every edge at least twice
+ * the cutoff (CharmmGBMVForceImpl.cpp:99-104) and above twice the lookup grid's reach.
+ * Must be called before the first evaluation of such a system and whenever the box changes.
+ */
+gbi_status gbi_set_periodic_box(gbi_ctx* ctx, int32_t system, const double* box_vectors);
 
 /* Asynchronous pieces of gbi_execute for device-resident callers. */
 gbi_status gbi_set_positions(gbi_ctx* ctx, const double* pos, int32_t pos_on_device);

@@ -21,6 +21,7 @@ STATUS_NAMES = {
 MODEL_GBSW, MODEL_GBMV = 0, 1
 INTEGRAL_GBSW, INTEGRAL_GBMV1, INTEGRAL_GBMV2, INTEGRAL_CUSTOM = 0, 1, 2, 3
 LOOKUP_REFERENCE, LOOKUP_BRUTE = 0, 1
+NO_CUTOFF, CUTOFF_NON_PERIODIC, CUTOFF_PERIODIC = 0, 1, 2
 FORCES, ENERGY, ADJOINT = 1, 2, 4
 
 # dE/dtheta order of gbi_get_adjoint (gbi_theta in gbi.h)
@@ -34,7 +35,7 @@ EXPORTS = [
     "gbi_get_forces", "gbi_get_born_radii", "gbi_get_integrals", "gbi_get_stats", "gbi_get_device_view",
     "gbi_set_stream", "gbi_get_adjoint", "gbi_execute_adjoint", "gbi_set_graph", "gbi_autotune",
     "gbi_set_tuning", "gbi_get_tuning", "gbi_destroy", "gbi_last_error", "gbi_abi_version", "gbi_quadrature",
-    "gbi_profile_enable", "gbi_profile_read", "gbi_profile_pass_name",
+    "gbi_profile_enable", "gbi_profile_read", "gbi_profile_pass_name", "gbi_set_periodic_box",
 ]
 PROFILE_PASSES = ["grid", "bin", "volume", "integrals", "pairs", "chain", "force", "owner"]
 
@@ -134,6 +135,7 @@ def load() -> ctypes.CDLL:
     L.gbi_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     L.gbi_execute.argtypes = [vp, dp, ctypes.c_int32, ctypes.c_int32, dp, dp]
     L.gbi_update_parameters.argtypes = [vp, ctypes.c_int32, dp, dp]
+    L.gbi_set_periodic_box.argtypes = [vp, ctypes.c_int32, dp]
     L.gbi_set_positions.argtypes = [vp, vp, ctypes.c_int32]
     L.gbi_compute.argtypes = [vp, ctypes.c_uint32]
     L.gbi_synchronize.argtypes = [vp]
@@ -264,6 +266,12 @@ class Engine:
         radius = np.ascontiguousarray(radius, dtype=np.float64)
         _check(self._lib.gbi_update_parameters(self._h, int(charge.shape[0]), _dp(charge), _dp(radius)))
         self.charge, self.radius = charge, radius
+
+    def set_periodic_box(self, box, system: int = 0) -> None:
+        """Box of a CutoffPeriodic system: three edges (rectangular) or 3 x 3 box vectors."""
+        b = np.asarray(box, dtype=np.float64)
+        v = np.diag(b) if b.shape == (3,) else np.ascontiguousarray(b.reshape(3, 3))
+        _check(self._lib.gbi_set_periodic_box(self._h, int(system), _dp(np.ascontiguousarray(v))))
 
     def set_positions(self, pos, on_device: bool = False) -> None:
         if on_device:

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cfloat>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -80,8 +81,12 @@ struct SysHost {
     double g = 0.15, gpad = 0.0, halfdiag = 0.0, max_reach = 0.0;
     long long cell_cap = 0;
     int pt_cap = 1;
-    int pair_splits = 1, pair_k = 1, nblk = 1;
+    int pair_splits = 1, pair_k = 1, nblk = 1, blk_base = 0;
     long long colpart_base = 0;
+    double box[3] = {0.0, 0.0, 0.0};  // periodic box edges (gbi_set_periodic_box)
+    bool box_set = false;
+    bool periodic() const { return P.nonbonded_method == GBI_CUTOFF_PERIODIC; }
+    bool cutoff() const { return P.nonbonded_method != GBI_NO_CUTOFF; }
 };
 
 }  // namespace
@@ -109,6 +114,9 @@ struct gbi_ctx {
     DevBuf<int> flags_all, atom_sys, cell_sys, bin_order;
     DevBuf<int4> pair_tiles;
     DevBuf<int> colblk;
+    DevBuf<double> d_box;
+    DevBuf<float4> blk_lo, blk_hi;
+    bool any_cut = false, any_trunc = false;
     DevBuf<float> colpart;
     int n_tiles = 0, max_splits = 1, n_colblocks = 0;
     long long colpart_total = 1;
@@ -157,6 +165,9 @@ struct gbi_ctx {
         aparam.release();
         pair_tiles.release();
         colblk.release();
+        d_box.release();
+        blk_lo.release();
+        blk_hi.release();
         colpart.release();
         DevBuf<int>* ib[] = {&flags_all,      &atom_sys,     &cell_sys,       &bin_order,       &cell_count, &cell_fill,
                              &cell_start,     &cell_count_eff, &cell_atoms_tmp, &cell_atoms, &cell_pt_count,
@@ -184,8 +195,12 @@ gbi_status validate_params(const gbi_params& P, int N) {
         return fail(GBI_ERR_UNSUPPORTED, "CustomIntegral (type 3) is not implemented by the reference either");
     if (P.model == GBI_MODEL_GBMV && (P.integral_type < 0 || P.integral_type > 2))
         return fail(GBI_ERR_INVALID_ARGUMENT, "unknown GB integral type");
-    if (P.nonbonded_method != GBI_NO_CUTOFF)
-        return fail(GBI_ERR_UNSUPPORTED, "only NoCutoff is supported on the HIP platform");
+    if (P.nonbonded_method < GBI_NO_CUTOFF || P.nonbonded_method > GBI_CUTOFF_PERIODIC)
+        return fail(GBI_ERR_INVALID_ARGUMENT, "unknown nonbonded method");
+    if (P.nonbonded_method != GBI_NO_CUTOFF && !(P.cutoff > 0.0))
+        return fail(GBI_ERR_INVALID_ARGUMENT, "the cutoff distance must be positive");
+    if (P.nonbonded_method == GBI_CUTOFF_PERIODIC && P.lookup_policy == GBI_LOOKUP_BRUTE)
+        return fail(GBI_ERR_UNSUPPORTED, "the brute-force lookup has no periodic form");
     if (P.n_radial <= 0 || P.n_radial * P.n_lebedev > (1 << gbi::kQBits) - 1)
         return fail(GBI_ERR_INVALID_ARGUMENT, "quadrature must have 1..2047 points per atom");
     if (!(P.n_lebedev == 6 || P.n_lebedev == 14 || P.n_lebedev == 26 || P.n_lebedev == 38 || P.n_lebedev == 50))
@@ -281,6 +296,13 @@ gbi::SysDesc make_desc(const gbi_ctx& c, const SysHost& s, long long cell_base, 
     d.colpart_base = s.colpart_base;
     d.i4 = P.orders[0] == 4 ? 0 : 1;
     d.i7 = 1 - d.i4;
+    d.periodic = s.periodic() ? 1 : 0;
+    d.cut_mode = !s.cutoff() ? 0 : (P.model == GBI_MODEL_GBSW ? 1 : 2);
+    d.chain_trunc = (s.cutoff() && P.model == GBI_MODEL_GBSW) ? 1 : 0;
+    d.blk_base = s.blk_base;
+    d.cut2 = s.cutoff() ? P.cutoff * P.cutoff : 0.0;
+    d.cut2_lo = s.cutoff() ? (float)(d.cut2 * (1.0 - 1e-4)) : FLT_MAX;
+    d.cut2_hi = s.cutoff() ? (float)(d.cut2 * (1.0 + 1e-4)) : FLT_MAX;
     d.g = s.g;
     d.inv_g = 1.0 / s.g;
     d.gpad = s.kind == gbi::GRID_FAITHFUL ? s.gpad : s.max_reach;
@@ -386,6 +408,7 @@ gbi_status build_tiles(gbi_ctx& c) {
         s.nblk = nb;
         s.pair_splits = (nb + K - 1) / K;  // pieces of row block 0, the most of any block
         s.colpart_base = cp;
+        s.blk_base = (int)colblk.size();
         cp += 64ll * nb * (nb - 1) / 2;  // row blocks I < J for every column block J
         c.max_splits = std::max(c.max_splits, s.pair_splits);
         for (int I = 0; I < nb; ++I) {
@@ -404,6 +427,8 @@ gbi_status build_tiles(gbi_ctx& c) {
     HIP_TRY(hipStreamSynchronize(c.stream));
     HIP_TRY(c.pair_tiles.ensure(tiles.size()));
     HIP_TRY(c.colblk.ensure(colblk.size()));
+    HIP_TRY(c.blk_lo.ensure(colblk.size()));
+    HIP_TRY(c.blk_hi.ensure(colblk.size()));
     HIP_TRY(c.pair_part.ensure((size_t)c.max_splits * 6 * c.N));
     HIP_TRY(c.colpart.ensure((size_t)5 * c.colpart_total));
     HIP_TRY(hipMemcpyAsync(c.pair_tiles.p, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice,
@@ -433,7 +458,8 @@ void estimate_caps(const gbi_ctx& c, const SysHost& s, const double* pos, long l
     const double gridpad = s.kind == gbi::GRID_FAITHFUL ? s.gpad : s.max_reach;
     cells = 1;
     for (int d = 0; d < 3; ++d) {
-        const double len = (mx[d] - mn[d]) * 1.1 + 2.0 * gridpad + 1.0;
+        // periodic: the box's cells (ceil(L/g) + 1 per axis), with room for a 5 % larger box
+        const double len = s.periodic() ? s.box[d] * 1.05 : (mx[d] - mn[d]) * 1.1 + 2.0 * gridpad + 1.0;
         cells *= (long long)std::ceil(len / g) + 2;
     }
     // entries: box of candidate cells per atom (an upper bound of the members)
@@ -497,6 +523,11 @@ gbi::KParams make_kparams(gbi_ctx& c, bool adjoint) {
     k.persist_blocks = c.num_cus * std::max(1, c.tuning.blocks_per_cu);
     k.sys = c.d_sys.p;
     k.geom = c.geom.p;
+    k.box = c.d_box.p;
+    k.any_cut = c.any_cut ? 1 : 0;
+    k.any_trunc = c.any_trunc ? 1 : 0;
+    k.blk_lo = c.blk_lo.p;
+    k.blk_hi = c.blk_hi.p;
     k.flags_all = c.flags_all.p;
     k.atom_sys = c.atom_sys.p;
     k.cell_sys = c.cell_sys.p;
@@ -565,9 +596,35 @@ gbi_status ensure_adjoint_buffers(gbi_ctx& c) {
     return GBI_OK;
 }
 
+// Box edges a periodic system accepts: at least twice the cutoff (CharmmGBMVForceImpl.cpp:99-104,
+// CharmmGBSWForceImpl.cpp:48-54), and more than twice a cell list's reach plus a cell diagonal,
+// so a list holds one image of an atom at most and that image is the nearest to every point of
+// its cell.
+gbi_status check_box(const gbi_ctx& c, const SysHost& s, const double* L) {
+    double maxR = 0.0;
+    for (int j = s.atom_begin; j < s.atom_begin + s.N; ++j) maxR = std::max(maxR, c.radius[j]);
+    const double reach = s.kind == gbi::GRID_FAITHFUL ? maxR + s.halfdiag + s.P.buffer : s.max_reach;
+    for (int d = 0; d < 3; ++d) {
+        if (!std::isfinite(L[d]) || !(L[d] > 0.0)) return fail(GBI_ERR_INVALID_ARGUMENT, "box edges must be positive");
+        if (s.P.cutoff > 0.5 * L[d])
+            return fail(GBI_ERR_INVALID_ARGUMENT,
+                        "The cutoff distance cannot be greater than half the periodic box size.");
+        if (!(L[d] > 2.0 * (reach + std::sqrt(3.0) * s.g)))
+            return fail(GBI_ERR_INVALID_ARGUMENT, "periodic box smaller than twice the lookup-grid reach");
+    }
+    return GBI_OK;
+}
+
 gbi_status run(gbi_ctx& c, uint32_t flags) {
     if (!c.have_positions) return fail(GBI_ERR_INVALID_ARGUMENT, "positions were never set");
     const bool adjoint = (flags & GBI_ADJOINT) != 0;
+    for (const SysHost& s : c.sys)
+        if (s.periodic()) {
+            if (!s.box_set)
+                return fail(GBI_ERR_INVALID_ARGUMENT, "CutoffPeriodic system without a periodic box (gbi_set_periodic_box)");
+            ST_TRY(check_box(c, s, s.box));
+        }
+    if (adjoint && c.any_cut) return fail(GBI_ERR_UNSUPPORTED, "the adjoint is implemented for NoCutoff systems only");
     if (c.cell_total == 0) ST_TRY(size_from_device(c, 0));
     if (adjoint) ST_TRY(ensure_adjoint_buffers(c));
     gbi::KParams k = make_kparams(c, adjoint);
@@ -735,6 +792,8 @@ gbi_status create_batch(int S, const int32_t* n_atoms, const double* charge, con
         h.pt_cap = h.kind == gbi::GRID_BRUTE ? h.N * h.Q : (int)std::min<double>(est, (double)h.N * h.Q);
         h.pt_cap = std::max(h.pt_cap, 1);
         c->kinds[h.vol] = true;
+        c->any_cut = c->any_cut || h.cutoff();
+        c->any_trunc = c->any_trunc || (h.cutoff() && h.P.model == GBI_MODEL_GBSW);
         atom += h.N;
         qrow += h.Q;
         pt += h.N * h.Q;
@@ -750,6 +809,8 @@ gbi_status create_batch(int S, const int32_t* n_atoms, const double* charge, con
     const size_t N = c->N, NQ = c->NQ;
     CTX_HIP(c->d_sys.ensure(S));
     CTX_HIP(c->geom.ensure(S));
+    CTX_HIP(c->d_box.ensure(3 * (size_t)S));
+    CTX_HIP(hipMemsetAsync(c->d_box.p, 0, sizeof(double) * 3 * S, c->stream));
     CTX_HIP(c->flags_all.ensure(1));
     CTX_HIP(c->atom_sys.ensure(N));
     CTX_HIP(c->pos.ensure(3 * N));
@@ -946,6 +1007,23 @@ gbi_status gbi_update_parameters(gbi_ctx* ctx, int32_t n_atoms, const double* ch
         if (ctx->cell_total > 0) ST_TRY(layout(*ctx));  // new reach in the descriptors
         return size_from_device(*ctx, 0);
     }
+    return GBI_OK;
+}
+
+gbi_status gbi_set_periodic_box(gbi_ctx* ctx, int32_t system, const double* v) {
+    if (!ctx || !v) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
+    if (system < 0 || system >= ctx->S) return fail(GBI_ERR_INVALID_ARGUMENT, "no such system");
+    SysHost& s = ctx->sys[system];
+    if (!s.periodic()) return fail(GBI_ERR_INVALID_ARGUMENT, "the system is not CutoffPeriodic");
+    if (v[1] != 0.0 || v[2] != 0.0 || v[3] != 0.0 || v[5] != 0.0 || v[6] != 0.0 || v[7] != 0.0)
+        return fail(GBI_ERR_UNSUPPORTED, "CharmmGBMVForce: Only Rectangular Periodic Box is Supported!");
+    const double L[3] = {v[0], v[4], v[8]};
+    ST_TRY(check_box(*ctx, s, L));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int d = 0; d < 3; ++d) s.box[d] = L[d];
+    s.box_set = true;
+    HIP_TRY(hipMemcpyAsync(ctx->d_box.p + 3 * system, s.box, sizeof(double) * 3, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return GBI_OK;
 }
 

@@ -83,6 +83,13 @@ struct SysDesc {
     int nblk;                  // 64-atom blocks
     long long colpart_base;    // column partials (row block I < block J of atom j): colpart_base + 64 (J(J-1)/2 + I) + j % 64
     int i4, i7;                // which integral is order 4 / 7
+    // nonbonded method
+    int periodic;              // rectangular periodic box (KParams::box), lookup grid tiling it
+    int cut_mode;              // 0: no cutoff; 1: pairs with r <= c (standalone GBSW); 2: r < c (CharmmGBMVForce)
+    int chain_trunc;           // standalone GBSW with a cutoff: dB_i/dx_j reaches only atoms j within c of i
+    int blk_base;              // index of the system's first 64-atom block in KParams::colblk
+    double cut2;               // c^2
+    float cut2_lo, cut2_hi;    // FP32 band around c^2 inside which the FP64 distance decides
     // grid constants (double, computed on the host with the oracle's expressions)
     double g, inv_g, gpad, halfdiag, buffer;
     // volume constants
@@ -99,6 +106,9 @@ struct SysDesc {
 struct SysGeom {
     double origin[3];  // grid minimum corner (brute: molecule centre)
     double max[3];     // last grid point (faithful) / upper edge (tight)
+    double step[3];    // periodic: cell edges L / n
+    double L[3];       // periodic: box edges
+    double halfdiag;   // periodic faithful grid: sqrt3/2 max(step)
     int n[3];
     int ncell;
     int flags;
@@ -119,6 +129,11 @@ struct KParams {
     int persist_blocks;      // workgroups of the per-cell passes
     const SysDesc* sys;
     SysGeom* geom;
+    const double* box;       // S x 3: edges of each periodic system's box
+    int any_cut;             // some system has a cutoff: pair tiles test distances
+    int any_trunc;           // some system truncates the chain rule at the cutoff
+    float4* blk_lo;          // n_colblocks: bounding box of each 64-atom block (cutoff systems)
+    float4* blk_hi;
     int* flags_all;          // OR of every system's flags
     const int* atom_sys;     // N
     const int* cell_sys;     // cell_total

@@ -95,6 +95,27 @@ __device__ inline int floor_div(double a, double g, double inv_g) {
     return (int)floor(a / g);
 }
 
+// Periodic grid: the point wrapped into the box (written back) and its cell, as
+// CustomGBIntegral.cpp:294-301: x0 = x - floor((x - min)/L) L, index floor((x0 - min)/step).
+__device__ inline int point_cell_periodic(const SysGeom& G, double* pt) {
+#pragma clang fp contract(off)
+    int idx[3];
+    for (int d = 0; d < 3; ++d) {
+        const double pw = pt[d] - floor((pt[d] - G.origin[d]) / G.L[d]) * G.L[d];
+        int k = (int)floor((pw - G.origin[d]) / G.step[d]);
+        k = k < 0 ? 0 : (k > G.n[d] - 1 ? G.n[d] - 1 : k);
+        idx[d] = k;
+        pt[d] = pw;
+    }
+    return (idx[0] * G.n[1] + idx[1]) * G.n[2] + idx[2];
+}
+
+// x wrapped into the periodic box [origin, origin + L) (the cell-list build and its entries).
+__device__ inline double wrap_coord(const SysGeom& G, int d, double x) {
+#pragma clang fp contract(off)
+    return x - G.L[d] * floor((x - G.origin[d]) / G.L[d]);
+}
+
 // Local cell of a point; -1 outside.  Faithful: CustomGBIntegral.cpp:310-324.
 __device__ inline int point_cell(const SysDesc& sd, const SysGeom& G, double px, double py, double pz) {
 #pragma clang fp contract(off)
@@ -130,6 +151,12 @@ __device__ inline void cell_corner(const SysDesc& sd, const SysGeom& G, int c, d
     const int rem = c - ix * nyz;
     const int iy = rem / G.n[2];
     const int iz = rem - iy * G.n[2];
+    if (sd.periodic) {
+        corner[0] = G.origin[0] + ix * G.step[0];
+        corner[1] = G.origin[1] + iy * G.step[1];
+        corner[2] = G.origin[2] + iz * G.step[2];
+        return;
+    }
     corner[0] = G.origin[0] + ix * sd.g;
     corner[1] = G.origin[1] + iy * sd.g;
     corner[2] = G.origin[2] + iz * sd.g;
@@ -154,17 +181,17 @@ __global__ void k_clear(KParams p) {
     }
 }
 
-__device__ void bbox_to_geom(const KParams& p, int s, double (*smin)[16], double (*smax)[16], const int* sbad,
-                             int nw);
+__device__ void bbox_to_geom(const KParams& p, int s, double (*smin)[16], double (*smax)[16], double (*ssum)[16],
+                             const int* sbad, int nw);
 
-// Bounding box -> grid geometry (one workgroup per system).
+// Bounding box and centre of geometry -> grid geometry (one workgroup per system).
 __global__ void __launch_bounds__(1024) k_bbox(KParams p) {
-    __shared__ double smin[3][16], smax[3][16];
+    __shared__ double smin[3][16], smax[3][16], ssum[3][16];
     __shared__ int sbad[16];
     const int s = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int ab = p.sys[s].atom_begin, ae = p.sys[s].atom_end;
-    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX}, sm[3] = {0.0, 0.0, 0.0};
     int bad = 0;
 #pragma unroll 4
     for (int a = ab + tid; a < ae; a += blockDim.x) {
@@ -173,44 +200,55 @@ __global__ void __launch_bounds__(1024) k_bbox(KParams p) {
             if (!isfinite(v)) bad = 1;
             mn[d] = fmin(mn[d], v);
             mx[d] = fmax(mx[d], v);
+            sm[d] += v;
         }
     }
     for (int d = 0; d < 3; ++d) {
         mn[d] = wave_min(mn[d]);
         mx[d] = wave_max(mx[d]);
+        sm[d] = wave_sum(sm[d]);
     }
     const int anybad = __any(bad);
     if (lane == 0) {
         for (int d = 0; d < 3; ++d) {
             smin[d][wid] = mn[d];
             smax[d][wid] = mx[d];
+            ssum[d][wid] = sm[d];
         }
         sbad[wid] = anybad;
     }
     __syncthreads();
     if (tid != 0) return;
-    bbox_to_geom(p, s, smin, smax, sbad, (blockDim.x + kWave - 1) / kWave);
+    bbox_to_geom(p, s, smin, smax, ssum, sbad, (blockDim.x + kWave - 1) / kWave);
 }
 
 // Grid geometry from the per-wave bounding boxes (thread 0 of k_bbox).
-__device__ void bbox_to_geom(const KParams& p, int s, double (*smin)[16], double (*smax)[16], const int* sbad,
-                             int nw) {
+__device__ void bbox_to_geom(const KParams& p, int s, double (*smin)[16], double (*smax)[16], double (*ssum)[16],
+                             const int* sbad, int nw) {
 #pragma clang fp contract(off)
     const SysDesc& sd = p.sys[s];
-    double mn[3], mx[3];
+    double mn[3], mx[3], cog[3];
     int isbad = 0;
     for (int d = 0; d < 3; ++d) {
         mn[d] = smin[d][0];
         mx[d] = smax[d][0];
+        cog[d] = ssum[d][0];
         for (int w = 1; w < nw; ++w) {
             mn[d] = fmin(mn[d], smin[d][w]);
             mx[d] = fmax(mx[d], smax[d][w]);
+            cog[d] += ssum[d][w];
         }
+        cog[d] /= (double)(sd.atom_end - sd.atom_begin);
     }
     for (int w = 0; w < nw; ++w) isbad |= sbad[w];
     SysGeom G;
     G.flags = 0;
     G.pad_[0] = G.pad_[1] = G.pad_[2] = 0;
+    G.halfdiag = sd.halfdiag;
+    for (int d = 0; d < 3; ++d) {
+        G.step[d] = sd.g;
+        G.L[d] = 0.0;
+    }
     long long ncell = 1;
     if (isbad) {
         G.flags |= FLAG_NONFINITE_POS;
@@ -220,6 +258,24 @@ __device__ void bbox_to_geom(const KParams& p, int s, double (*smin)[16], double
             G.max[d] = 0.0;
             G.n[d] = 0;
         }
+    } else if (sd.periodic) {
+        // CustomGBIntegral.cpp:53-107: origin at the centre of geometry minus half the box,
+        // ceil(L/g) + 1 cells of edge L/n per axis
+        double maxstep = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const double L = p.box[3 * s + d];
+            const double cells = ceil(L / sd.g) + 1.0;
+            if (!(cells <= 1.0e6)) G.flags |= FLAG_GRID_TOO_LARGE;
+            const int n = cells <= 1.0e6 ? (int)cells : 1;
+            G.n[d] = n;
+            G.L[d] = L;
+            G.step[d] = L / n;
+            G.origin[d] = cog[d] - L / 2.0;
+            G.max[d] = G.origin[d] + L;
+            maxstep = fmax(maxstep, G.step[d]);
+            ncell *= n;
+        }
+        G.halfdiag = sqrt(3.0) / 2.0 * maxstep;
     } else if (sd.grid_kind == GRID_BRUTE) {
         for (int d = 0; d < 3; ++d) {
             G.origin[d] = 0.5 * (mn[d] + mx[d]);
@@ -277,6 +333,17 @@ struct CandBox {
 __device__ inline CandBox candidate_box(const SysDesc& sd, const SysGeom& G, const double* x, double pad) {
 #pragma clang fp contract(off)
     CandBox B;
+    if (sd.periodic) {
+        // unwrapped cell indices around the wrapped atom; each is reduced modulo n later
+        for (int d = 0; d < 3; ++d) {
+            B.b[d] = (int)floor((x[d] - pad - G.origin[d]) / G.step[d]);
+            B.e[d] = sd.grid_kind == GRID_FAITHFUL ? (int)ceil((x[d] + pad - G.origin[d]) / G.step[d])
+                                                   : (int)floor((x[d] + pad - G.origin[d]) / G.step[d]);
+        }
+        const int nx = B.e[0] - B.b[0] + 1, ny = B.e[1] - B.b[1] + 1, nz = B.e[2] - B.b[2] + 1;
+        B.count = (nx > 0 && ny > 0 && nz > 0) ? nx * ny * nz : 0;
+        return B;
+    }
     for (int d = 0; d < 3; ++d) {
         if (sd.grid_kind == GRID_FAITHFUL) {
             B.b[d] = (int)floor((x[d] - pad - G.origin[d]) / sd.g);
@@ -299,6 +366,19 @@ __device__ inline CandBox candidate_box(const SysDesc& sd, const SysGeom& G, con
 __device__ inline bool cell_member(const SysDesc& sd, const SysGeom& G, const double* x, double pad, int i, int jj,
                                    int k) {
 #pragma clang fp contract(off)
+    if (sd.periodic) {
+        // the cell image (i, jj, k) of the periodic grid; x wrapped into the box
+        const double lx = G.origin[0] + i * G.step[0], ly = G.origin[1] + jj * G.step[1],
+                     lz = G.origin[2] + k * G.step[2];
+        if (sd.grid_kind == GRID_FAITHFUL) {
+            const double dx = lx - x[0], dy = ly - x[1], dz = lz - x[2];
+            return dx * dx + dy * dy + dz * dz < pad * pad;
+        }
+        const double dx = fmax(fmax(lx - x[0], x[0] - (lx + G.step[0])), 0.0);
+        const double dy = fmax(fmax(ly - x[1], x[1] - (ly + G.step[1])), 0.0);
+        const double dz = fmax(fmax(lz - x[2], x[2] - (lz + G.step[2])), 0.0);
+        return dx * dx + dy * dy + dz * dz < pad * pad;
+    }
     const double g = sd.g;
     if (sd.grid_kind == GRID_FAITHFUL) {
         const double dx = (G.origin[0] + i * g) - x[0];
@@ -313,9 +393,48 @@ __device__ inline bool cell_member(const SysDesc& sd, const SysGeom& G, const do
     return dx * dx + dy * dy + dz * dz < pad * pad;
 }
 
-__device__ inline double atom_pad(const KParams& p, const SysDesc& sd, int j) {
+__device__ inline double atom_pad(const KParams& p, const SysDesc& sd, const SysGeom& G, int j) {
 #pragma clang fp contract(off)
-    return sd.grid_kind == GRID_FAITHFUL ? (p.radius[j] + sd.halfdiag) + sd.buffer : p.reach[j];
+    if (sd.grid_kind != GRID_FAITHFUL) return p.reach[j];
+    return (p.radius[j] + (sd.periodic ? G.halfdiag : sd.halfdiag)) + sd.buffer;
+}
+
+// Cell-list key: atom index << 6 | image code; the code holds, per axis in two bits, the
+// box vector (-1, 0, +1) between the wrapped atom and the cell image it joined (0 when
+// not periodic).  Keys sort by atom index first.
+constexpr int kImageBits = 6;
+
+// i = img n + iw with 0 <= iw < n (img is -1, 0 or +1 for boxes the host accepts; the
+// floor division keeps every index inside the grid whatever i is).
+__device__ inline int wrap_index(int i, int n, int& img) {
+    img = i >= 0 ? i / n : -((n - 1 - i) / n);
+    return i - img * n;
+}
+
+// The atom and cell of candidate t of box B (unwrapped indices reduced for periodic grids).
+__device__ inline bool candidate_cell(const SysDesc& sd, const SysGeom& G, const CandBox& B, int t, const double* x,
+                                      double pad, int& cell, int& code) {
+    const int ny = B.e[1] - B.b[1] + 1, nz = B.e[2] - B.b[2] + 1;
+    const int i = B.b[0] + t / (ny * nz);
+    const int r = t - (i - B.b[0]) * (ny * nz);
+    const int jj = B.b[1] + r / nz;
+    const int k = B.b[2] + (r - (jj - B.b[1]) * nz);
+    if (!cell_member(sd, G, x, pad, i, jj, k)) return false;
+    if (sd.periodic) {
+        int ox, oy, oz;
+        const int iw = wrap_index(i, G.n[0], ox), jw = wrap_index(jj, G.n[1], oy), kw = wrap_index(k, G.n[2], oz);
+        cell = (iw * G.n[1] + jw) * G.n[2] + kw;
+        code = (ox & 3) | ((oy & 3) << 2) | ((oz & 3) << 4);
+    } else {
+        cell = (i * G.n[1] + jj) * G.n[2] + k;
+        code = 0;
+    }
+    return true;
+}
+
+// The atom position the lists use: wrapped into the box for periodic systems.
+__device__ inline void list_position(const KParams& p, const SysDesc& sd, const SysGeom& G, int j, double* x) {
+    for (int d = 0; d < 3; ++d) x[d] = sd.periodic ? wrap_coord(G, d, p.pos[3 * j + d]) : p.pos[3 * j + d];
 }
 
 // One workgroup per atom, lanes over its candidate cells: count the members of each cell.
@@ -326,17 +445,13 @@ __global__ void __launch_bounds__(256) k_grid_count(KParams p) {
     const SysDesc& sd = p.sys[s];
     if (sd.grid_kind == GRID_BRUTE) return;  // counted by k_bbox
     const SysGeom G = p.geom[s];
-    const double x[3] = {p.pos[3 * j], p.pos[3 * j + 1], p.pos[3 * j + 2]};
-    const double pad = atom_pad(p, sd, j);
+    double x[3];
+    list_position(p, sd, G, j, x);
+    const double pad = atom_pad(p, sd, G, j);
     const CandBox B = candidate_box(sd, G, x, pad);
-    const int ny = B.e[1] - B.b[1] + 1, nz = B.e[2] - B.b[2] + 1;
     for (int t = threadIdx.x; t < B.count; t += blockDim.x) {
-        const int i = B.b[0] + t / (ny * nz);
-        const int r = t - (i - B.b[0]) * (ny * nz);
-        const int jj = B.b[1] + r / nz;
-        const int k = B.b[2] + (r - (jj - B.b[1]) * nz);
-        if (cell_member(sd, G, x, pad, i, jj, k))
-            atomicAdd(&p.cell_count[sd.cell_base + (i * G.n[1] + jj) * G.n[2] + k], 1);
+        int cell, code;
+        if (candidate_cell(sd, G, B, t, x, pad, cell, code)) atomicAdd(&p.cell_count[sd.cell_base + cell], 1);
     }
 }
 
@@ -347,31 +462,42 @@ __global__ void __launch_bounds__(256) k_grid_fill(KParams p) {
     const SysDesc& sd = p.sys[s];
     if (sd.grid_kind == GRID_BRUTE) {
         // index order, no atomics
-        if (threadIdx.x == 0) p.cell_atoms_tmp[p.cell_start[sd.cell_base] + (j - sd.atom_begin)] = j;
+        if (threadIdx.x == 0) p.cell_atoms_tmp[p.cell_start[sd.cell_base] + (j - sd.atom_begin)] = j << kImageBits;
         return;
     }
     const SysGeom G = p.geom[s];
-    const double x[3] = {p.pos[3 * j], p.pos[3 * j + 1], p.pos[3 * j + 2]};
-    const double pad = atom_pad(p, sd, j);
+    double x[3];
+    list_position(p, sd, G, j, x);
+    const double pad = atom_pad(p, sd, G, j);
     const CandBox B = candidate_box(sd, G, x, pad);
-    const int ny = B.e[1] - B.b[1] + 1, nz = B.e[2] - B.b[2] + 1;
     for (int t = threadIdx.x; t < B.count; t += blockDim.x) {
-        const int i = B.b[0] + t / (ny * nz);
-        const int r = t - (i - B.b[0]) * (ny * nz);
-        const int jj = B.b[1] + r / nz;
-        const int k = B.b[2] + (r - (jj - B.b[1]) * nz);
-        if (cell_member(sd, G, x, pad, i, jj, k)) {
-            const int c = sd.cell_base + (i * G.n[1] + jj) * G.n[2] + k;
+        int cell, code;
+        if (candidate_cell(sd, G, B, t, x, pad, cell, code)) {
+            const int c = sd.cell_base + cell;
             const int slot = atomicAdd(&p.cell_fill[c], 1);
             const int idx = p.cell_start[c] + slot;
-            if (idx < p.entry_cap) p.cell_atoms_tmp[idx] = j;
+            if (idx < p.entry_cap) p.cell_atoms_tmp[idx] = (j << kImageBits) | code;
         }
     }
 }
 
-__device__ inline void write_entry(const KParams& p, int e, int j, const double* corner) {
-    p.ent_pos[e] = make_float4((float)(p.pos[3 * j] - corner[0]), (float)(p.pos[3 * j + 1] - corner[1]),
-                               (float)(p.pos[3 * j + 2] - corner[2]), __int_as_float(j));
+// Entry of key (atom, image code) relative to the cell corner: for periodic systems the
+// wrapped atom moved by the code's box vectors, i.e. its image nearest the cell.
+__device__ inline void write_entry(const KParams& p, const SysDesc& sd, const SysGeom& G, int e, int key,
+                                   const double* corner) {
+#pragma clang fp contract(off)
+    const int j = key >> kImageBits;
+    double x[3];
+    for (int d = 0; d < 3; ++d) {
+        x[d] = p.pos[3 * j + d];
+        if (sd.periodic) {
+            const int bits = (key >> (2 * d)) & 3;
+            const int img = bits == 3 ? -1 : bits;
+            x[d] = wrap_coord(G, d, x[d]) - img * G.L[d];  // the cell image sits img boxes away
+        }
+    }
+    p.ent_pos[e] = make_float4((float)(x[0] - corner[0]), (float)(x[1] - corner[1]), (float)(x[2] - corner[2]),
+                               __int_as_float(j));
     p.ent_par[e] = p.aparam[j];
 }
 
@@ -399,7 +525,7 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
         double corner[3];
         cell_corner(sd, G, c - sd.cell_base, corner);
         if (sd.grid_kind == GRID_BRUTE) {
-            for (int k = lane; k < eff; k += kWave) write_entry(p, base + k, p.cell_atoms_tmp[base + k], corner);
+            for (int k = lane; k < eff; k += kWave) write_entry(p, sd, G, base + k, p.cell_atoms_tmp[base + k], corner);
         } else if (n <= kSortLds) {
             for (int k = lane; k < n; k += kWave) s_in[k] = p.cell_atoms_tmp[base + k];
             __syncthreads();
@@ -410,7 +536,7 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
                 s_out[r] = a;
             }
             __syncthreads();
-            for (int k = lane; k < eff; k += kWave) write_entry(p, base + k, s_out[k], corner);
+            for (int k = lane; k < eff; k += kWave) write_entry(p, sd, G, base + k, s_out[k], corner);
         } else {
             for (int k = lane; k < n; k += kWave) {
                 const int a = p.cell_atoms_tmp[base + k];
@@ -420,7 +546,7 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
             }
             __threadfence_block();
             __syncthreads();
-            for (int k = lane; k < eff; k += kWave) write_entry(p, base + k, p.cell_atoms[base + k], corner);
+            for (int k = lane; k < eff; k += kWave) write_entry(p, sd, G, base + k, p.cell_atoms[base + k], corner);
         }
         if (lane == 0) p.cell_count_eff[c] = eff;
         __syncthreads();
@@ -576,7 +702,7 @@ __global__ void __launch_bounds__(256) k_point_bin(KParams p) {
         const bool owner_buried = shortcut && p.quad_r2[row] <= own_lo2;
         if (!owner_buried) {
             quad_point(p, xi, yi, zi, row, pt);
-            c = point_cell(sd, G, pt[0], pt[1], pt[2]);
+            c = sd.periodic ? point_cell_periodic(G, pt) : point_cell(sd, G, pt[0], pt[1], pt[2]);
             if (c >= 0 && p.cell_count_eff[cb + c] == 0) c = -1;
         }
         if (c < 0) {
@@ -892,6 +1018,107 @@ __device__ inline PairTerms pair_terms(double xj, double yj, double zj, double x
     return t;
 }
 
+// Cutoff systems.  FP64 squared distance of atoms i and j as the oracle takes it:
+// d = x_j - x_i, reduced to the minimum image by d -= L floor(d/L + 1/2) (OpenMM's
+// ReferenceForce::getDeltaRPeriodic), r^2 = (dx dx + dy dy) + dz dz.
+__device__ inline double exact_r2(const KParams& p, int s, const SysDesc& sd, int i, int j) {
+#pragma clang fp contract(off)
+    double d[3];
+    for (int k = 0; k < 3; ++k) {
+        d[k] = p.pos[3 * j + k] - p.pos[3 * i + k];
+        if (sd.periodic) d[k] -= p.box[3 * s + k] * floor(d[k] / p.box[3 * s + k] + 0.5);
+    }
+    return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+}
+
+// Pair inside the cutoff: r <= c for the standalone GBSW force (CharmmReferenceGBSW.cpp:246-247,
+// :290-291), r < c for CharmmGBMVForce (CharmmReferenceGBMV.cpp:300-301).  The FP32 distance
+// decides outside a band of +-1e-4 c^2 around the cutoff, the FP64 one inside it, so the
+// decision is the oracle's.
+__device__ inline bool within_cutoff(const KParams& p, int s, const SysDesc& sd, float r2, int i, int j) {
+    if (r2 < sd.cut2_lo) return true;
+    if (r2 > sd.cut2_hi) return false;
+    const double e = exact_r2(p, s, sd, i, j);
+    return sd.cut_mode == 1 ? !(e > sd.cut2) : e < sd.cut2;
+}
+
+// Box vectors of a periodic system in FP32, for minimum images of short differences.
+struct BoxF {
+    float L[3], inv[3];
+};
+
+__device__ inline BoxF box_f(const KParams& p, int s, const SysDesc& sd) {
+    BoxF b;
+    for (int d = 0; d < 3; ++d) {
+        b.L[d] = sd.periodic ? (float)p.box[3 * s + d] : 0.f;
+        b.inv[d] = sd.periodic ? (float)(1.0 / p.box[3 * s + d]) : 0.f;
+    }
+    return b;
+}
+
+// d reduced to the minimum image (|d| below 1.5 box edges; L = 0: not periodic).
+__device__ inline float min_image(float d, float L, float inv) { return fmaf(-L, rintf(d * inv), d); }
+
+// A pair of a cutoff system: d at its minimum image (positions wrapped into the box; L = 0
+// when not periodic leaves d as it is).
+__device__ inline PairTerms pair_terms_cut(double xj, double yj, double zj, double xi, double yi, double zi,
+                                           const BoxF& bx, float Bi, float Bj, float kexp_i, float iBj) {
+    PairTerms t;
+    t.dx = min_image((float)(xj - xi), bx.L[0], bx.inv[0]);
+    t.dy = min_image((float)(yj - yi), bx.L[1], bx.inv[1]);
+    t.dz = min_image((float)(zj - zi), bx.L[2], bx.inv[2]);
+    t.r2 = fmaf(t.dx, t.dx, fmaf(t.dy, t.dy, t.dz * t.dz));
+    t.ex = __builtin_amdgcn_exp2f(-(t.r2 * kexp_i) * iBj);
+    t.rf = __builtin_amdgcn_rsqf(fmaf(Bi * Bj, t.ex, t.r2));
+    return t;
+}
+
+__device__ inline double wrap_box(double x, double L) {
+#pragma clang fp contract(off)
+    return L > 0.0 ? x - L * floor(x / L) : x;
+}
+
+// Bounding box of every 64-atom block (positions wrapped into the box for periodic systems),
+// for skipping pair blocks beyond the cutoff.
+__global__ void __launch_bounds__(64) k_block_box(KParams p) {
+    if (batch_fatal(p)) return;
+    const int b = blockIdx.x, lane = threadIdx.x;
+    const int j0 = p.colblk[b];
+    const int s = p.atom_sys[j0];
+    const SysDesc& sd = p.sys[s];
+    const int j = min(j0 + lane, sd.atom_end - 1);
+    float lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        const float v = (float)wrap_box(p.pos[3 * j + d], sd.periodic ? p.box[3 * s + d] : 0.0);
+        lo[d] = (float)wave_min(v);
+        hi[d] = (float)wave_max(v);
+    }
+    if (lane == 0) {
+        p.blk_lo[b] = make_float4(lo[0], lo[1], lo[2], 0.f);
+        p.blk_hi[b] = make_float4(hi[0], hi[1], hi[2], 0.f);
+    }
+}
+
+// Blocks a and b surely farther apart than the cutoff (conservative: 0.1 % on c^2 and
+// no test along an axis where the two boxes span half the periodic box).
+__device__ inline bool blocks_apart(const KParams& p, const SysDesc& sd, const BoxF& bx, int a, int b) {
+    const float4 la = p.blk_lo[a], ha = p.blk_hi[a], lb = p.blk_lo[b], hb = p.blk_hi[b];
+    const float lo_a[3] = {la.x, la.y, la.z}, hi_a[3] = {ha.x, ha.y, ha.z};
+    const float lo_b[3] = {lb.x, lb.y, lb.z}, hi_b[3] = {hb.x, hb.y, hb.z};
+    float gap2 = 0.f;
+    for (int d = 0; d < 3; ++d) {
+        const float h = 0.5f * ((hi_a[d] - lo_a[d]) + (hi_b[d] - lo_b[d]));
+        float dc = 0.5f * ((lo_b[d] + hi_b[d]) - (lo_a[d] + hi_a[d]));
+        if (sd.periodic) {
+            if (h >= 0.5f * bx.L[d]) continue;
+            dc = min_image(dc, bx.L[d], bx.inv[d]);
+        }
+        const float g = fabsf(dc) - h;
+        if (g > 0.f) gap2 = fmaf(g, g, gap2);
+    }
+    return gap2 > sd.cut2_hi * 1.001f;
+}
+
 // Column partial of row block I for local atom j (block J = j / 64 > I): triangular, the
 // row blocks of one column block stored one after the other, 64 atoms each.
 __device__ inline long long colpart_index(const SysDesc& sd, int I, int j) {
@@ -904,7 +1131,11 @@ __device__ inline float rotate_down(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
 }
 
-template <bool ADJ>
+// CUT: systems with a cutoff drop the pairs outside it (rf = 0 zeroes every term), reduce
+// periodic differences to the minimum image (positions wrapped into the box on load, so one
+// box vector at most) and skip column blocks whose bounding boxes are beyond the cutoff
+// (their column partials are written as zeros).
+template <bool ADJ, bool CUT>
 __global__ void __launch_bounds__(64) k_pairs(KParams p) {
     __shared__ double s_px[64], s_py[64], s_pz[64];
     __shared__ float4 s_col[64];  // (q, B, 1/B, 1/(4B))
@@ -916,7 +1147,24 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
     const int ab = sd.atom_begin, ae = sd.atom_end;
     const int i = tile.x + lane;
     const bool iv = i < ae;
-    const double xi = iv ? p.pos[3 * i] : 0.0, yi = iv ? p.pos[3 * i + 1] : 0.0, zi = iv ? p.pos[3 * i + 2] : 0.0;
+    const bool cut = CUT && sd.cut_mode != 0;
+    const bool per = CUT && sd.periodic;
+    BoxF bx;
+    double Lx = 0.0, Ly = 0.0, Lz = 0.0;
+    if (CUT) {
+        bx = box_f(p, s, sd);
+        if (per) {
+            Lx = p.box[3 * s];
+            Ly = p.box[3 * s + 1];
+            Lz = p.box[3 * s + 2];
+        }
+    }
+    double xi = iv ? p.pos[3 * i] : 0.0, yi = iv ? p.pos[3 * i + 1] : 0.0, zi = iv ? p.pos[3 * i + 2] : 0.0;
+    if (per) {
+        xi = wrap_box(xi, Lx);
+        yi = wrap_box(yi, Ly);
+        zi = wrap_box(zi, Lz);
+    }
     const float P = (float)sd.prefactor;
     const float qi = iv ? (float)p.charge[i] : 0.f;  // zero charge: an absent row adds nothing
     const float Bi = iv ? (float)p.born[i] : 1.f;
@@ -928,10 +1176,30 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
     for (int jb = tile.y; jb < tile.z; jb += kWave) {
         const int jj = jb + lane;
         const bool jv = jj < ae;
+        if (cut && jb != tile.x && blocks_apart(p, sd, bx, sd.blk_base + I_row, sd.blk_base + (jb - ab) / kWave)) {
+            if (jv) {
+                const long long o = colpart_index(sd, I_row, jj - ab);
+                const long long T = p.colpart_total;
+                p.colpart[o] = 0.f;
+                p.colpart[T + o] = 0.f;
+                p.colpart[2 * T + o] = 0.f;
+                p.colpart[3 * T + o] = 0.f;
+                if (ADJ) p.colpart[4 * T + o] = 0.f;
+            }
+            continue;
+        }
         __syncthreads();
-        s_px[lane] = jv ? p.pos[3 * jj] : 0.0;
-        s_py[lane] = jv ? p.pos[3 * jj + 1] : 0.0;
-        s_pz[lane] = jv ? p.pos[3 * jj + 2] : 0.0;
+        {
+            double xj = jv ? p.pos[3 * jj] : 0.0, yj = jv ? p.pos[3 * jj + 1] : 0.0, zj = jv ? p.pos[3 * jj + 2] : 0.0;
+            if (per) {
+                xj = wrap_box(xj, Lx);
+                yj = wrap_box(yj, Ly);
+                zj = wrap_box(zj, Lz);
+            }
+            s_px[lane] = xj;
+            s_py[lane] = yj;
+            s_pz[lane] = zj;
+        }
         {
             const float Bj = jv ? (float)p.born[jj] : 1.f;
             const float iB = 1.f / Bj;
@@ -945,7 +1213,13 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
             for (int m = 0; m < cnt; ++m) {
                 if (jb + m == i) continue;
                 const float4 cj = s_col[m];
-                const PairTerms t = pair_terms(s_px[m], s_py[m], s_pz[m], xi, yi, zi, Bi, cj.y, kexp, cj.z);
+                PairTerms t;
+                if (cut) {
+                    t = pair_terms_cut(s_px[m], s_py[m], s_pz[m], xi, yi, zi, bx, Bi, cj.y, kexp, cj.z);
+                    if (!iv || !within_cutoff(p, s, sd, t.r2, i, jb + m)) t.rf = 0.f;
+                } else {
+                    t = pair_terms(s_px[m], s_py[m], s_pz[m], xi, yi, zi, Bi, cj.y, kexp, cj.z);
+                }
                 const float eij = Pq * cj.x * t.rf;
                 efs = fmaf(0.5f, eij, efs);
                 if (ADJ) dqs = fmaf(P * cj.x, t.rf, dqs);
@@ -962,7 +1236,13 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
             for (int m = 0; m < kWave; ++m) {
                 const int c = (lane + m) & (kWave - 1);
                 const float4 cj = s_col[c];
-                const PairTerms t = pair_terms(s_px[c], s_py[c], s_pz[c], xi, yi, zi, Bi, cj.y, kexp, cj.z);
+                PairTerms t;
+                if (cut) {
+                    t = pair_terms_cut(s_px[c], s_py[c], s_pz[c], xi, yi, zi, bx, Bi, cj.y, kexp, cj.z);
+                    if (!iv || jb + c >= ae || !within_cutoff(p, s, sd, t.r2, i, jb + c)) t.rf = 0.f;
+                } else {
+                    t = pair_terms(s_px[c], s_py[c], s_pz[c], xi, yi, zi, Bi, cj.y, kexp, cj.z);
+                }
                 const float eij = Pq * cj.x * t.rf;  // P q_i q_j / f
                 efs += eij;
                 const float t3 = eij * (t.rf * t.rf);
@@ -1140,9 +1420,13 @@ __device__ inline float sum8(float v) {
     return v;
 }
 
-template <int VK, bool ADJ>
-__global__ void __launch_bounds__(64, ADJ ? 2 : 3) k_force(KParams p) {
+// TRUNC (GBSW): the standalone force with a cutoff applies dB_i/dx_j only to list atoms j
+// within the cutoff of the owner i (minimum image); the owner still takes the whole sum
+// (CharmmReferenceGBSW.cpp:285-296).
+template <int VK, bool ADJ, bool TRUNC>
+__global__ void __launch_bounds__(64, (ADJ || TRUNC) ? 2 : 3) k_force(KParams p) {
     __shared__ float q_x[kWave], q_y[kWave], q_z[kWave];
+    __shared__ float q_w[TRUNC ? 3 : 1][kWave];  // TRUNC: owner position relative to the cell corner
     __shared__ float q_0[kWave], q_1[kWave], q_2[kWave], q_3[kWave], q_4[kWave], q_5[kWave];
     __shared__ int q_own[kWave], q_pid[kWave];
     __shared__ double s_red[8][kWave];  // one axis of the per-lane atom partials at a time
@@ -1163,6 +1447,9 @@ __global__ void __launch_bounds__(64, ADJ ? 2 : 3) k_force(KParams p) {
         const int pt_base = sd.pt_base, abeg = sd.atom_begin, Q = sd.Q, qb = sd.quad_base;
         const float k1 = sd.k1, k3 = sd.k3, fclamp = sd.fclamp, sign_chain = sd.sign_chain;
         const float inv_sw = sd.inv_sw, P1x10 = 10.f * sd.P1;
+        const bool trunc = TRUNC && sd.chain_trunc;
+        BoxF bx;
+        if (TRUNC) bx = box_f(p, s, sd);
         double pacc0 = 0.0, pacc1 = 0.0;  // ADJ: VolAcc partials of this lane
         for (int ab = 0; ab < na; ab += kWave) {
             float ax[8], ay[8], az[8], a0[8], a1[8], a2[8], aR[8];
@@ -1205,6 +1492,12 @@ __global__ void __launch_bounds__(64, ADJ ? 2 : 3) k_force(KParams p) {
                     q_x[lane] = r0.x;
                     q_y[lane] = r0.y;
                     q_z[lane] = r0.z;
+                    if (TRUNC) {
+                        const int row = qb + q;
+                        q_w[0][lane] = r0.x - (float)p.quad_off[3 * row];
+                        q_w[1][lane] = r0.y - (float)p.quad_off[3 * row + 1];
+                        q_w[2][lane] = r0.z - (float)p.quad_off[3 * row + 2];
+                    }
                     q_own[lane] = i;
                     q_pid[lane] = pt_base + (i - abeg) * Q + q;
                     if (VK == VOL_GBSW) {
@@ -1235,6 +1528,8 @@ __global__ void __launch_bounds__(64, ADJ ? 2 : 3) k_force(KParams p) {
                         const float vx = VK == VOL_GBMV2 ? q_3[m] : 0.f;
                         const float vy = VK == VOL_GBMV2 ? q_4[m] : 0.f;
                         const float vz = VK == VOL_GBMV2 ? q_5[m] : 0.f;
+                        const float wx = TRUNC ? q_w[0][m] : 0.f, wy = TRUNC ? q_w[1][m] : 0.f,
+                                    wz = TRUNC ? q_w[2][m] : 0.f;
 #pragma unroll
                         for (int k = 0; k < 8; ++k) {
                             if (aj[k] < 0) continue;
@@ -1294,9 +1589,18 @@ __global__ void __launch_bounds__(64, ADJ ? 2 : 3) k_force(KParams p) {
                             const float cx = fmaf(sv, dx, c3F * vx);
                             const float cy = fmaf(sv, dy, c3F * vy);
                             const float cz = fmaf(sv, dz, c3F * vz);
-                            acc[k][0] += quant(cx);
-                            acc[k][1] += quant(cy);
-                            acc[k][2] += quant(cz);
+                            bool keep = true;
+                            if (TRUNC && trunc && sv != 0.f) {
+                                const float ex = min_image(ax[k] - wx, bx.L[0], bx.inv[0]);
+                                const float ey = min_image(ay[k] - wy, bx.L[1], bx.inv[1]);
+                                const float ez = min_image(az[k] - wz, bx.L[2], bx.inv[2]);
+                                keep = within_cutoff(p, s, sd, fmaf(ex, ex, fmaf(ey, ey, ez * ez)), own, aj[k]);
+                            }
+                            if (keep) {
+                                acc[k][0] += quant(cx);
+                                acc[k][1] += quant(cy);
+                                acc[k][2] += quant(cz);
+                            }
                             ox += cx;
                             oy += cy;
                             oz += cz;
@@ -1494,9 +1798,14 @@ __global__ void k_stats(KParams p, double* out) {
 template <bool ADJ>
 void launch_force(const KParams& p, int vk, int blocks, hipStream_t stream) {
     switch (vk) {
-    case VOL_GBSW: hipLaunchKernelGGL((k_force<VOL_GBSW, ADJ>), dim3(blocks), dim3(64), 0, stream, p); break;
-    case VOL_GBMV1: hipLaunchKernelGGL((k_force<VOL_GBMV1, ADJ>), dim3(blocks), dim3(64), 0, stream, p); break;
-    default: hipLaunchKernelGGL((k_force<VOL_GBMV2, ADJ>), dim3(blocks), dim3(64), 0, stream, p); break;
+    case VOL_GBSW:
+        if (!ADJ && p.any_trunc)
+            hipLaunchKernelGGL((k_force<VOL_GBSW, false, true>), dim3(blocks), dim3(64), 0, stream, p);
+        else
+            hipLaunchKernelGGL((k_force<VOL_GBSW, ADJ, false>), dim3(blocks), dim3(64), 0, stream, p);
+        break;
+    case VOL_GBMV1: hipLaunchKernelGGL((k_force<VOL_GBMV1, ADJ, false>), dim3(blocks), dim3(64), 0, stream, p); break;
+    default: hipLaunchKernelGGL((k_force<VOL_GBMV2, ADJ, false>), dim3(blocks), dim3(64), 0, stream, p); break;
     }
 }
 
@@ -1535,9 +1844,13 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
     hipLaunchKernelGGL(k_integrals, dim3(p.N), dim3(64), 0, stream, p);
     mark(4);
     if (adj) {
-        hipLaunchKernelGGL(k_pairs<true>, dim3(p.n_tiles), dim3(64), 0, stream, p);
+        // the host refuses the adjoint of cutoff systems
+        hipLaunchKernelGGL((k_pairs<true, false>), dim3(p.n_tiles), dim3(64), 0, stream, p);
+    } else if (p.any_cut) {
+        hipLaunchKernelGGL(k_block_box, dim3(p.n_colblocks), dim3(64), 0, stream, p);
+        hipLaunchKernelGGL((k_pairs<false, true>), dim3(p.n_tiles), dim3(64), 0, stream, p);
     } else {
-        hipLaunchKernelGGL(k_pairs<false>, dim3(p.n_tiles), dim3(64), 0, stream, p);
+        hipLaunchKernelGGL((k_pairs<false, false>), dim3(p.n_tiles), dim3(64), 0, stream, p);
     }
     mark(5);
     if (adj)

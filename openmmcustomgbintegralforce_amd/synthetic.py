@@ -98,6 +98,37 @@ def globule(n_atoms: int, seed: int, dmin: float = 0.20, iters: int = 200,
     return System(np.ascontiguousarray(pos), charge, radius.astype(np.float64), meta)
 
 
+def liquid(box, seed: int, density: float = None, dmin: float = 0.20, iters: int = 200) -> System:
+    """Atoms filling a rectangular periodic box (edges ``box``, nm) at ``density`` atoms/nm^3
+    (default: the globule density), pushed apart to ``dmin`` under minimum-image distances
+    and wrapped into [0, box); serpentine order, charges and radii drawn as for ``globule``."""
+    from scipy.spatial import cKDTree
+
+    L = np.asarray(box, dtype=np.float64)
+    rng = np.random.default_rng(seed)
+    n = int(round((density or DENSITY) * float(np.prod(L))))
+    pos = rng.uniform(0.0, 1.0, (n, 3)) * L
+    for _ in range(iters):
+        pairs = cKDTree(pos, boxsize=L).query_pairs(dmin, output_type="ndarray")
+        if len(pairs) == 0:
+            break
+        d = pos[pairs[:, 1]] - pos[pairs[:, 0]]
+        d -= L * np.round(d / L)
+        r = np.maximum(np.linalg.norm(d, axis=1), 1e-6)
+        push = (0.55 * (dmin - r) / r)[:, None] * d
+        disp = np.zeros_like(pos)
+        np.add.at(disp, pairs[:, 0], -push)
+        np.add.at(disp, pairs[:, 1], push)
+        pos = np.mod(pos + disp, L)
+    pos = pos[_serpentine_order(pos)]
+    charge = rng.uniform(-0.8, 0.8, n)
+    charge -= charge.mean()
+    radius = RADII[rng.integers(0, len(RADII), n)]
+    meta = {"n_atoms": n, "seed": seed, "box_nm": L.tolist(), "density_per_nm3": density or DENSITY,
+            "dmin_target_nm": dmin}
+    return System(np.ascontiguousarray(pos), charge, radius.astype(np.float64), meta)
+
+
 def conformations(base: System, count: int, seed: int, sigma: float = 0.02) -> list:
     """``count`` perturbed copies of ``base``: x + N(0, sigma^2) per coordinate."""
     rng = np.random.default_rng(seed)

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     for s in syms:
         assert hasattr(lib, s), s
     assert sorted(N.EXPORTS) == syms
-    assert lib.gbi_abi_version() == 2
+    assert lib.gbi_abi_version() == 3
 
 
 def test_defaults_are_reference_constants():
@@ -69,7 +69,15 @@ def test_invalid_arguments_are_rejected_before_touching_a_device():
     bad.n_lebedev = 7
     assert _create(bad) == 1
     bad = N.Params.from_buffer_copy(p)
-    bad.nonbonded_method = 1
+    bad.nonbonded_method = N.CUTOFF_NON_PERIODIC
+    bad.cutoff = 0.0
+    assert _create(bad) == 1
+    bad = N.Params.from_buffer_copy(p)
+    bad.nonbonded_method = 3
+    assert _create(bad) == 1
+    bad = N.Params.from_buffer_copy(p)
+    bad.nonbonded_method = N.CUTOFF_PERIODIC
+    bad.lookup_policy = N.LOOKUP_BRUTE
     assert _create(bad) == 2
     bad = N.Params.from_buffer_copy(p)
     bad.orders[1] = 5
