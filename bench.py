@@ -46,6 +46,15 @@ WORKLOADS = {
                                 "perturbations of one base geometry) batched in one context, 24x38 quadrature; "
                                 "energy, forces, dE/dR, dE/dq and dE/d(GB constants) per conformation, beside "
                                 "the same batch without the adjoint"),
+    "gbsw-25000-cutoff": dict(model="gbsw", integral=0, n_atoms=25000, n_lebedev=38, seed=0, method=1, cutoff=1.6,
+                              desc="CharmmGBSWForce CutoffNonPeriodic 1.6 nm, 25,000-atom globule, 24x38 "
+                                   "quadrature (pairs beyond the cutoff and their 64-atom blocks skipped; the "
+                                   "chain rule truncated at the cutoff as the reference does)"),
+    "gbmv2-liquid-periodic": dict(model="gbmv", integral=2, n_atoms=0, n_lebedev=50, seed=1, method=2, cutoff=1.6,
+                                  box=(4.0, 4.0, 4.0),
+                                  desc="CharmmGBMVForce GBMVIntegralTypeII CutoffPeriodic 1.6 nm: atoms filling a "
+                                       "4 nm box at 100 atoms/nm^3 (6,400 atoms), 24x50 quadrature, periodic "
+                                       "lookup grid 0.15 nm"),
     "batch64": dict(model="batch", integral=-1, n_atoms=0, n_lebedev=0, seed=0,
                     desc="64 systems in one batched context, N ~ U[300, 6000] (seeded), alternating "
                          "CharmmGBSWForce 24x38 and CharmmGBMVForce GBMVIntegralTypeII 24x50, hipGraph replay; "
@@ -88,13 +97,19 @@ def main() -> int:
     if w["model"] == "adjoint":
         return run_adjoint(args, w)
     kind = kind_name(w)
-    sysm = S.globule(w["n_atoms"], w["seed"])
+    box = w.get("box")
+    sysm = S.liquid(box, w["seed"]) if box else S.globule(w["n_atoms"], w["seed"])
+    n_atoms = sysm.n
     if w["model"] == "gbsw":
         p = N.default_params(N.MODEL_GBSW)
     else:
         p = N.default_params(N.MODEL_GBMV, w["integral"])
     p.n_lebedev = w["n_lebedev"]
+    p.nonbonded_method = w.get("method", N.NO_CUTOFF)
+    p.cutoff = w.get("cutoff", 1.0)
     eng = N.Engine(p, sysm.charge, sysm.radius)
+    if box:
+        eng.set_periodic_box(box)
 
     # first evaluation sizes the device buffers; result kept for the accuracy check
     e_gpu, f_gpu = eng.execute(sysm.pos)
@@ -127,7 +142,7 @@ def main() -> int:
     if not args.no_accuracy:
         import oracle
 
-        op = oracle.params_from_abi(p, nthreads=min(16, os.cpu_count() or 1))
+        op = oracle.params_from_abi(p, nthreads=min(16, os.cpu_count() or 1), box=box)
         ref = oracle.compute(op, sysm.pos, sysm.charge, sysm.radius, want_stats=(kind == "gbsw"))
         acc = {"force_rel_rms": float(np.linalg.norm(f_gpu - ref.forces) / np.linalg.norm(ref.forces)),
                "energy_rel_err": float(abs(e_gpu - ref.energy) / abs(ref.energy)),
@@ -141,7 +156,8 @@ def main() -> int:
     else:
         n_vol_alg = n_vol_exact if n_frc is not None else None
 
-    kernel_flops = {"pairs": FL.pair_flops(w["n_atoms"])}
+    # with a cutoff the pair pass skips blocks: its work is not the all-pairs count
+    kernel_flops = {} if p.nonbonded_method != N.NO_CUTOFF else {"pairs": FL.pair_flops(n_atoms)}
     if n_vol_alg is not None:
         kernel_flops["volume"] = FL.volume_flops(kind, n_vol_alg, stats["n_points"])
     if n_frc is not None:
@@ -167,13 +183,13 @@ def main() -> int:
     if args.cpu_baseline_evals > 0:
         import oracle
 
-        op = oracle.params_from_abi(p, nthreads=1, dense=1)
+        op = oracle.params_from_abi(p, nthreads=1, dense=1, box=box)
         tc = time.perf_counter()
         for _ in range(args.cpu_baseline_evals):
             oracle.compute(op, sysm.pos, sysm.charge, sysm.radius)
         dt = time.perf_counter() - tc
         cpu = {"value": args.cpu_baseline_evals / dt, "unit": "evals/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_baseline_evals} full evaluation(s) of the same {w['n_atoms']}-atom system: "
+               "sample": f"{args.cpu_baseline_evals} full evaluation(s) of the same {n_atoms}-atom system: "
                          "FP64 oracle in the Reference platform's loop order (q-outer integral loop, dense "
                          "dI/dx arrays, N^2 chain rule), one thread",
                "seconds": dt}
@@ -190,8 +206,10 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": f"synthetic: seeded {w['n_atoms']}-atom globule at 100 atoms/nm^3 (synthetic.globule seed {w['seed']})",
-        "config": {"workload": args.workload, "description": w["desc"], "n_atoms": w["n_atoms"],
+        "data": (f"synthetic: seeded {n_atoms}-atom liquid filling a {box} nm box (synthetic.liquid seed {w['seed']})"
+                 if box else
+                 f"synthetic: seeded {n_atoms}-atom globule at 100 atoms/nm^3 (synthetic.globule seed {w['seed']})"),
+        "config": {"workload": args.workload, "description": w["desc"], "n_atoms": n_atoms,
                    "quadrature": f"24x{w['n_lebedev']}", "precision": "FP32 pair/quadrature math, FP64 "
                    "integral and energy sums, int64 fixed-point forces"},
         **acc,

@@ -303,6 +303,10 @@ gbi::SysDesc make_desc(const gbi_ctx& c, const SysHost& s, long long cell_base, 
     d.cut2 = s.cutoff() ? P.cutoff * P.cutoff : 0.0;
     d.cut2_lo = s.cutoff() ? (float)(d.cut2 * (1.0 - 1e-4)) : FLT_MAX;
     d.cut2_hi = s.cutoff() ? (float)(d.cut2 * (1.0 + 1e-4)) : FLT_MAX;
+    // GBSW shell atoms lie within max(R' + w) of a point (max_reach); 1e-4 nm absorbs FP32 rounding
+    d.trunc_in = d.chain_trunc ? (float)(std::sqrt(d.cut2 * (1.0 - 1e-4)) - s.max_reach - 1e-4) : FLT_MAX;
+    d.trunc_out = (d.chain_trunc && !s.periodic()) ? (float)(std::sqrt(d.cut2 * (1.0 + 1e-4)) + s.max_reach + 1e-4)
+                                                   : FLT_MAX;
     d.g = s.g;
     d.inv_g = 1.0 / s.g;
     d.gpad = s.kind == gbi::GRID_FAITHFUL ? s.gpad : s.max_reach;

@@ -90,6 +90,8 @@ struct SysDesc {
     int blk_base;              // index of the system's first 64-atom block in KParams::colblk
     double cut2;               // c^2
     float cut2_lo, cut2_hi;    // FP32 band around c^2 inside which the FP64 distance decides
+    float trunc_in, trunc_out; // chain_trunc: points nearer their owner than trunc_in keep every shell
+                               // atom, points farther than trunc_out none (FLT_MAX when periodic)
     // grid constants (double, computed on the host with the oracle's expressions)
     double g, inv_g, gpad, halfdiag, buffer;
     // volume constants

@@ -1427,6 +1427,7 @@ template <int VK, bool ADJ, bool TRUNC>
 __global__ void __launch_bounds__(64, (ADJ || TRUNC) ? 2 : 3) k_force(KParams p) {
     __shared__ float q_x[kWave], q_y[kWave], q_z[kWave];
     __shared__ float q_w[TRUNC ? 3 : 1][kWave];  // TRUNC: owner position relative to the cell corner
+    __shared__ int q_tf[TRUNC ? kWave : 1];      // TRUNC: 0 every shell atom within c of the owner, 1 test, 2 none
     __shared__ float q_0[kWave], q_1[kWave], q_2[kWave], q_3[kWave], q_4[kWave], q_5[kWave];
     __shared__ int q_own[kWave], q_pid[kWave];
     __shared__ double s_red[8][kWave];  // one axis of the per-lane atom partials at a time
@@ -1493,10 +1494,15 @@ __global__ void __launch_bounds__(64, (ADJ || TRUNC) ? 2 : 3) k_force(KParams p)
                     q_y[lane] = r0.y;
                     q_z[lane] = r0.z;
                     if (TRUNC) {
+                        // shell atoms lie within max(R' + w) of the point, the point r_q from its
+                        // owner: r_q + reach below c keeps them all, r_q - reach above c none
+                        // (not periodic: another image may be nearer)
                         const int row = qb + q;
                         q_w[0][lane] = r0.x - (float)p.quad_off[3 * row];
                         q_w[1][lane] = r0.y - (float)p.quad_off[3 * row + 1];
                         q_w[2][lane] = r0.z - (float)p.quad_off[3 * row + 2];
+                        const float rq = fast_sqrt(p.quad_r2[row]);
+                        q_tf[lane] = !trunc || rq < sd.trunc_in ? 0 : (rq > sd.trunc_out ? 2 : 1);
                     }
                     q_own[lane] = i;
                     q_pid[lane] = pt_base + (i - abeg) * Q + q;
@@ -1530,6 +1536,7 @@ __global__ void __launch_bounds__(64, (ADJ || TRUNC) ? 2 : 3) k_force(KParams p)
                         const float vz = VK == VOL_GBMV2 ? q_5[m] : 0.f;
                         const float wx = TRUNC ? q_w[0][m] : 0.f, wy = TRUNC ? q_w[1][m] : 0.f,
                                     wz = TRUNC ? q_w[2][m] : 0.f;
+                        const int tf = TRUNC ? q_tf[m] : 0;
 #pragma unroll
                         for (int k = 0; k < 8; ++k) {
                             if (aj[k] < 0) continue;
@@ -1589,11 +1596,14 @@ __global__ void __launch_bounds__(64, (ADJ || TRUNC) ? 2 : 3) k_force(KParams p)
                             const float cx = fmaf(sv, dx, c3F * vx);
                             const float cy = fmaf(sv, dy, c3F * vy);
                             const float cz = fmaf(sv, dz, c3F * vz);
-                            bool keep = true;
-                            if (TRUNC && trunc && sv != 0.f) {
-                                const float ex = min_image(ax[k] - wx, bx.L[0], bx.inv[0]);
-                                const float ey = min_image(ay[k] - wy, bx.L[1], bx.inv[1]);
-                                const float ez = min_image(az[k] - wz, bx.L[2], bx.inv[2]);
+                            bool keep = !TRUNC || tf != 2;
+                            if (TRUNC && tf == 1 && sv != 0.f) {
+                                float ex = ax[k] - wx, ey = ay[k] - wy, ez = az[k] - wz;
+                                if (sd.periodic) {
+                                    ex = min_image(ex, bx.L[0], bx.inv[0]);
+                                    ey = min_image(ey, bx.L[1], bx.inv[1]);
+                                    ez = min_image(ez, bx.L[2], bx.inv[2]);
+                                }
                                 keep = within_cutoff(p, s, sd, fmaf(ex, ex, fmaf(ey, ey, ez * ez)), own, aj[k]);
                             }
                             if (keep) {

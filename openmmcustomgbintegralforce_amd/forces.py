@@ -255,8 +255,6 @@ def gbmv_params_from_force(force: CharmmGBMVForce) -> tuple:
     t = force.getGBIntegralType()
     if t == CharmmGBMVForce.CustomIntegral:
         raise N.GbiError(2, "CustomIntegral (type 3) is not implemented by the reference either")
-    if force.getNonbondedMethod() != CharmmGBMVForce.NoCutoff:
-        raise N.GbiError(2, "only NoCutoff is supported on the HIP platform")
     names = [force.getPerParticleParameterName(i) for i in range(force.getNumPerParticleParameters())]
     if "radius" not in names:  # GBSWIntegral.cpp:79-80
         raise OpenMMException("GBSWIntegral: the per perticle parameter 'radius' must be defined")
@@ -288,6 +286,8 @@ def gbmv_params_from_force(force: CharmmGBMVForce) -> tuple:
     for k in ("alpha0", "alpha1", "eta", "slope", "shift"):
         setattr(p, k, g[k])
     p.prefactor = g["P"]
+    p.nonbonded_method = force.getNonbondedMethod()
+    p.cutoff = force.getCutoffDistance()
     iq, ir = names.index("q"), names.index("radius")
     parts = np.array([force.getParticleParameters(i) for i in range(force.getNumParticles())], dtype=np.float64)
     return p, parts[:, iq].copy(), parts[:, ir].copy()
@@ -305,11 +305,24 @@ class System:
 
 
 class ContextImpl:
-    """Holds positions (N x 3 nm) and the force accumulator, as ReferencePlatform::PlatformData."""
+    """Holds positions (N x 3 nm), the periodic box vectors and the force accumulator, as
+    ReferencePlatform::PlatformData."""
 
-    def __init__(self, positions):
+    def __init__(self, positions, box_vectors=None):
         self.positions = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
         self.forces = np.zeros_like(self.positions)
+        self.box_vectors = None if box_vectors is None else np.asarray(box_vectors, dtype=np.float64).reshape(3, 3)
+
+    def getPeriodicBoxVectors(self):
+        return self.box_vectors
+
+
+def _set_box(engine, method: int, context: ContextImpl) -> None:
+    """The box is read from the context at every evaluation (CharmmReferenceKernels.cpp:351, :430)."""
+    if method == N.CUTOFF_PERIODIC:
+        if context.box_vectors is None:
+            raise OpenMMException("CutoffPeriodic needs the context's periodic box vectors")
+        engine.set_periodic_box(context.box_vectors)
 
 
 # --------------------------------------------------------------------------------------
@@ -330,10 +343,11 @@ class HipCalcCharmmGBSWForceKernel:
     def initialize(self, system: System, force: CharmmGBSWForce) -> None:
         if system.getNumParticles() != force.getNumParticles():  # CharmmGBSWForceImpl.cpp:47-48
             raise OpenMMException("CharmmGBSWForce must have exactly as many particles as the System it belongs to.")
-        if force.getNonbondedMethod() != CharmmGBSWForce.NoCutoff:
-            raise N.GbiError(2, "only NoCutoff is supported on the HIP platform")
         q, r, _ = force.arrays()
         p = N.default_params(N.MODEL_GBSW)
+        p.nonbonded_method = force.getNonbondedMethod()
+        p.cutoff = force.getCutoffDistance()
+        self._method = p.nonbonded_method
         if self._honor:
             p.prefactor = -ONE_4PI_EPS0 * (1.0 / force.getSoluteDielectric() - 1.0 / force.getSolventDielectric())
         for k, v in self._overrides.items():
@@ -341,6 +355,7 @@ class HipCalcCharmmGBSWForceKernel:
         self._engine = N.Engine(p, q, r)
 
     def execute(self, context: ContextImpl, includeForces: bool = True, includeEnergy: bool = True) -> float:
+        _set_box(self._engine, self._method, context)
         e, _ = self._engine.execute(context.positions, includeForces, includeEnergy, forces=context.forces)
         return e if includeEnergy else 0.0
 
@@ -370,9 +385,11 @@ class HipCalcCharmmGBMVForceKernel:
         p, q, r = gbmv_params_from_force(force)
         for k, v in self._overrides.items():
             setattr(p, "lambda_" if k == "lambda" else k, v)
+        self._method = p.nonbonded_method
         self._engine = N.Engine(p, q, r)
 
     def execute(self, context: ContextImpl, includeForces: bool = True, includeEnergy: bool = True) -> float:
+        _set_box(self._engine, self._method, context)
         e, _ = self._engine.execute(context.positions, includeForces, includeEnergy, forces=context.forces)
         return e if includeEnergy else 0.0
 

@@ -4,8 +4,10 @@
 //
 // Input (binary, native endianness): int32 model (0 CharmmGBSWForce, 1 CharmmGBMVForce),
 // int32 integral type, int32 Lebedev rule (0 = the reference's), int32 N, then N x 3
-// positions, N charges, N radii (doubles).  Output (text): energy, forces, the energy after
-// copyParametersToContext with charges x 1.5, and the outcome of the error paths.
+// positions, N charges, N radii (doubles), optionally followed by int32 nonbonded method,
+// double cutoff and 3 doubles of a rectangular periodic box.  Output (text): energy,
+// forces, the energy after copyParametersToContext with charges x 1.5, and the outcome of
+// the error paths.
 #include <cstdio>
 #include <cstdint>
 #include <memory>
@@ -68,6 +70,12 @@ int main(int argc, char** argv) {
         rd(in, x.data(), x.size());
         rd(in, q.data(), q.size());
         rd(in, r.data(), r.size());
+        int32_t method = 0;
+        double cutoff = 1.0, box[3] = {2.0, 2.0, 2.0};
+        if (fread(&method, sizeof(method), 1, in) == 1) {
+            rd(in, &cutoff, 1);
+            rd(in, box, 3);
+        }
         fclose(in);
 
         Platform::registerPlatform(new Platform("HIP"));
@@ -79,6 +87,7 @@ int main(int argc, char** argv) {
         std::vector<Vec3> pos(n);
         for (int i = 0; i < n; ++i) pos[i] = Vec3(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
         context.setPositions(pos);
+        context.setPeriodicBoxVectors(Vec3(box[0], 0.0, 0.0), Vec3(0.0, box[1], 0.0), Vec3(0.0, 0.0, box[2]));
 
         FILE* out = fopen(argv[2], "w");
         if (!out) throw std::runtime_error("cannot open output");
@@ -87,6 +96,8 @@ int main(int argc, char** argv) {
         for (double& v : qs) v *= 1.5;
         if (model == 0) {
             CharmmGBSWForce force;
+            force.setNonbondedMethod((CharmmGBSWForce::NonbondedMethod)method);
+            force.setCutoffDistance(cutoff);
             for (int i = 0; i < n; ++i) force.addParticle(q[i], r[i], 1.0);
             std::unique_ptr<KernelImpl> k(platform.createKernel(CalcCharmmGBSWForceKernel::Name(), context));
             auto& kernel = dynamic_cast<CalcCharmmGBSWForceKernel&>(*k);
@@ -110,6 +121,8 @@ int main(int argc, char** argv) {
             fprintf(out, "count_change_throws %d\n", count_throws);
         } else {
             std::unique_ptr<CharmmGBMVForce> force(canonicalGBMV(q, r, type, rule));
+            force->setNonbondedMethod((CharmmGBMVForce::NonbondedMethod)method);
+            force->setCutoffDistance(cutoff);
             std::unique_ptr<KernelImpl> k(platform.createKernel(CalcCharmmGBMVForceKernel::Name(), context));
             auto& kernel = dynamic_cast<CalcCharmmGBMVForceKernel&>(*k);
             kernel.initialize(system, *force);
@@ -119,6 +132,8 @@ int main(int argc, char** argv) {
             for (int i = 0; i < n; ++i) fprintf(out, "%.17g %.17g %.17g\n", f[i][0], f[i][1], f[i][2]);
             e_noforce = kernel.execute(context, false, true);
             std::unique_ptr<CharmmGBMVForce> scaled(canonicalGBMV(qs, r, type, rule));
+            scaled->setNonbondedMethod((CharmmGBMVForce::NonbondedMethod)method);
+            scaled->setCutoffDistance(cutoff);
             kernel.copyParametersToContext(context, *scaled);
             e_scaled = kernel.execute(context, false, true);
             fprintf(out, "count_change_throws 1\n");

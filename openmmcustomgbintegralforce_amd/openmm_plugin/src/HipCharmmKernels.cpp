@@ -42,6 +42,7 @@ void HipGBKernelBase::create(int n, const std::vector<double>& charge, const std
     gbi_destroy(ctx_);
     ctx_ = nullptr;
     n_ = n;
+    periodic_ = params.nonbonded_method == GBI_CUTOFF_PERIODIC;
     check(gbi_create(n, charge.data(), radius.data(), &params, &ctx_));
 }
 
@@ -54,6 +55,12 @@ double HipGBKernelBase::run(ContextImpl& context, bool includeForces, bool inclu
     std::vector<Vec3> pos;
     context.getPositions(pos);
     if ((int)pos.size() != n_) throw OpenMMException("HIP GB kernel: the Context's particle count changed");
+    if (periodic_) {  // the box is read every evaluation (CharmmReferenceKernels.cpp:351, :430)
+        Vec3 a, b, c;
+        context.getPeriodicBoxVectors(a, b, c);
+        const double v[9] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2]};
+        check(gbi_set_periodic_box(ctx_, 0, v));
+    }
     std::vector<double> x(3 * (size_t)n_), f(3 * (size_t)n_, 0.0);
     for (int a = 0; a < n_; ++a)
         for (int d = 0; d < 3; ++d) x[3 * a + d] = pos[a][d];
@@ -81,8 +88,6 @@ void HipCalcCharmmGBSWForceKernel::initialize(const System& system, const Charmm
     const int n = force.getNumParticles();
     if (system.getNumParticles() != n)  // CharmmGBSWForceImpl.cpp:47-48
         throw OpenMMException("CharmmGBSWForce must have exactly as many particles as the System it belongs to.");
-    if (force.getNonbondedMethod() != CharmmGBSWForce::NoCutoff)
-        throw OpenMMException("CharmmGBSWForce: only NoCutoff is supported on the HIP platform");
     std::vector<double> q(n), r(n);
     for (int i = 0; i < n; ++i) {
         double s;
@@ -91,6 +96,8 @@ void HipCalcCharmmGBSWForceKernel::initialize(const System& system, const Charmm
     gbi_params p;
     check(gbi_params_default(GBI_MODEL_GBSW, 0, &p));
     if (honor_) p.prefactor = -138.935456 * (1.0 / force.getSoluteDielectric() - 1.0 / force.getSolventDielectric());
+    p.nonbonded_method = (int32_t)force.getNonbondedMethod();
+    p.cutoff = force.getCutoffDistance();
     create(n, q, r, p);
 }
 
@@ -116,8 +123,6 @@ void gbmvParamsFromForce(const CharmmGBMVForce& force, gbi_params& p, std::vecto
     const CharmmGBMVForce::GBIntegralType type = force.getGBIntegralType();
     if (type == CharmmGBMVForce::CustomIntegral)  // CharmmReferenceKernels.cpp:129-133
         throw OpenMMException("CharmmGBMVForce: CustomIntegral (type 3) is not implemented by the reference either");
-    if (force.getNonbondedMethod() != CharmmGBMVForce::NoCutoff)
-        throw OpenMMException("CharmmGBMVForce: only NoCutoff is supported on the HIP platform");
     int iq = -1, ir = -1;
     for (int k = 0; k < force.getNumPerParticleParameters(); ++k) {
         if (force.getPerParticleParameterName(k) == "q") iq = k;
@@ -152,6 +157,8 @@ void gbmvParamsFromForce(const CharmmGBMVForce& force, gbi_params& p, std::vecto
     if (ints[0].size() > 2) p.n_radial = ints[0][2];
     p.orders[0] = ints[0][0];
     p.orders[1] = ints[1][0];
+    p.nonbonded_method = (int32_t)force.getNonbondedMethod();
+    p.cutoff = force.getCutoffDistance();
     bool seen[6] = {false, false, false, false, false, false};
     for (int k = 0; k < force.getNumGlobalParameters(); ++k) {
         const std::string& g = force.getGlobalParameterName(k);

@@ -33,6 +33,7 @@ protected:
 
     gbi_ctx* ctx_ = nullptr;
     int n_ = 0;
+    bool periodic_ = false;  // CutoffPeriodic: the Context's box goes to the engine every evaluation
 };
 
 // CalcCharmmGBSWForceKernel on HIP (CharmmReferenceKernels.cpp:380-445 is the Reference one).

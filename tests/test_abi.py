@@ -111,6 +111,11 @@ def test_force_api_round_trip():
     assert np.array_equal(qq, q) and np.array_equal(rr, r)
     assert p.prefactor == pytest.approx(-138.935456 * (1 - 1 / 80))
     assert F.CustomGBIntegralForce is F.CharmmGBMVForce
+    assert p.nonbonded_method == N.NO_CUTOFF
+    f.setNonbondedMethod(F.CharmmGBMVForce.CutoffPeriodic)
+    f.setCutoffDistance(1.2)
+    p, _, _ = F.gbmv_params_from_force(f)
+    assert (p.nonbonded_method, p.cutoff) == (N.CUTOFF_PERIODIC, 1.2)
 
 
 def test_force_api_rejects_what_the_hip_platform_does_not_implement():

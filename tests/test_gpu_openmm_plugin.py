@@ -17,9 +17,9 @@ from test_openmm_plugin import DRIVER, write_input
 pytestmark = pytest.mark.gpu
 
 
-def _run(tmp_path, model, itype, rule, s):
+def _run(tmp_path, model, itype, rule, s, **method):
     inp, out = tmp_path / "in.bin", tmp_path / "out.txt"
-    write_input(inp, model, itype, rule, s.pos, s.charge, s.radius)
+    write_input(inp, model, itype, rule, s.pos, s.charge, s.radius, **method)
     r = subprocess.run([DRIVER, str(inp), str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     lines = out.read_text().splitlines()
@@ -49,3 +49,24 @@ def test_cpp_kernel_classes_match_oracle(tmp_path, case):
     assert abs(float(rest["energy_scaled"]) - 2.25 * e) < 1e-6 * abs(e)
     assert rest["illegal_name_throws"] == "1"
     assert rest["count_change_throws"] == "1"
+
+
+@pytest.mark.parametrize("case", ["gbsw", "gbmv2"])
+def test_cpp_kernel_classes_cutoff_periodic(tmp_path, case):
+    """CutoffPeriodic through the C++ kernel classes: the box comes from the Context at
+    every execute (CharmmReferenceKernels.cpp:351, :430)."""
+    L = [2.7, 2.6, 2.8]
+    s = S.liquid(L, 42)
+    if case == "gbsw":
+        p = N.default_params(N.MODEL_GBSW)
+        model, itype, rule = 0, 0, 0
+    else:
+        p = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV2)
+        model, itype, rule = 1, 2, 0
+    p.nonbonded_method, p.cutoff = N.CUTOFF_PERIODIC, 1.2
+    e, f, rest = _run(tmp_path, model, itype, rule, s, method=N.CUTOFF_PERIODIC, cutoff=1.2, box=L)
+    print(f"[openmm-plugin] {case} periodic N={s.n} energy {e:.6f}", flush=True)
+    ref = O.compute(oracle_params(p, box=L), s.pos, s.charge, s.radius)
+    assert rel_err(e, ref.energy) < 1e-5
+    assert rel_rms(f, ref.forces) < 1e-5
+    assert float(rest["energy_no_forces"]) == e

@@ -12,12 +12,15 @@ PLUGIN = os.path.join(PKG, "lib", "libgbi_openmm.so")
 DRIVER = os.path.join(PKG, "lib", "gbi_openmm_driver")
 
 
-def write_input(path, model, itype, rule, pos, q, R):
+def write_input(path, model, itype, rule, pos, q, R, method=None, cutoff=1.0, box=(2.0, 2.0, 2.0)):
     with open(path, "wb") as fh:
         np.array([model, itype, rule, len(q)], dtype=np.int32).tofile(fh)
         np.ascontiguousarray(pos, dtype=np.float64).tofile(fh)
         np.ascontiguousarray(q, dtype=np.float64).tofile(fh)
         np.ascontiguousarray(R, dtype=np.float64).tofile(fh)
+        if method is not None:
+            np.array([method], dtype=np.int32).tofile(fh)
+            np.array([cutoff, *box], dtype=np.float64).tofile(fh)
 
 
 def test_plugin_exports_openmm_entry_points():
