@@ -51,6 +51,7 @@ constexpr int kDefaultBlocksPerCU = 128;  // about one workgroup per occupied ce
 // Cell edge of the exact GBSW grid (lists are exact at any edge): 0.20 nm measured fastest at
 // 5k and 25k atoms (0.15 nm: 14 % slower, more distinct cells to bin; 0.30 nm: longer lists).
 constexpr double kDefaultTightGrid = 0.20;
+constexpr int kCutoffTileBlocks = 3;  // column blocks per pair tile of a cutoff system
 
 template <class T>
 struct DevBuf {
@@ -116,6 +117,7 @@ struct gbi_ctx {
     DevBuf<int> colblk;
     DevBuf<double> d_box;
     DevBuf<float4> blk_lo, blk_hi;
+    DevBuf<int> pair_atom;
     bool any_cut = false, any_trunc = false;
     DevBuf<float> colpart;
     int n_tiles = 0, max_splits = 1, n_colblocks = 0;
@@ -168,6 +170,7 @@ struct gbi_ctx {
         d_box.release();
         blk_lo.release();
         blk_hi.release();
+        pair_atom.release();
         colpart.release();
         DevBuf<int>* ib[] = {&flags_all,      &atom_sys,     &cell_sys,       &bin_order,       &cell_count, &cell_fill,
                              &cell_start,     &cell_count_eff, &cell_atoms_tmp, &cell_atoms, &cell_pt_count,
@@ -407,6 +410,9 @@ gbi_status build_tiles(gbi_ctx& c) {
             const double share = (double)target * pairs / (double)pairs_total;  // tiles for this system
             K = (int)std::ceil((double)pairs / std::max(1.0, share));
         }
+        // cutoff systems: most block pairs are skipped and the work gathers near the diagonal,
+        // so narrow tiles spread it (3 blocks measured best at 25k atoms, cutoff 0.5-1.6 nm)
+        if (c.tuning.pair_columns <= 0 && s.cutoff()) K = std::min(K, kCutoffTileBlocks);
         K = std::max(1, std::min(K, nb));
         s.pair_k = K;
         s.nblk = nb;
@@ -478,6 +484,46 @@ void estimate_caps(const gbi_ctx& c, const SysHost& s, const double* pos, long l
     entries = (long long)(side * side * side * s.N * 1.05) + 1024;
 }
 
+// Spreads the low 10 bits of v to every third bit.
+uint32_t spread3(uint32_t v) {
+    v &= 1023u;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// Pair slots of a cutoff system in Morton order of 0.25 nm cells (positions wrapped into a
+// periodic box), so that 64 consecutive slots are a compact cluster and whole block pairs
+// beyond the cutoff are skipped; other systems keep the identity.  Any order gives the same
+// pairs: the order only decides which blocks can be skipped.
+void pair_order(const gbi_ctx& c, const SysHost& s, const double* pos, int* out) {
+    for (int k = 0; k < s.N; ++k) out[k] = s.atom_begin + k;
+    if (!s.cutoff() || s.N <= 64) return;
+    double mn[3] = {pos[0], pos[1], pos[2]};
+    std::vector<double> x(3 * (size_t)s.N);
+    for (int k = 0; k < s.N; ++k)
+        for (int d = 0; d < 3; ++d) {
+            double v = pos[3 * k + d];
+            if (s.periodic() && s.box[d] > 0.0) v -= s.box[d] * std::floor(v / s.box[d]);
+            x[3 * k + d] = v;
+            mn[d] = k == 0 ? v : std::min(mn[d], v);
+        }
+    std::vector<std::pair<uint32_t, int>> key(s.N);
+    for (int k = 0; k < s.N; ++k) {
+        uint32_t code = 0;
+        for (int d = 0; d < 3; ++d) {
+            const double t = std::floor((x[3 * k + d] - mn[d]) / 0.25);
+            code |= spread3((uint32_t)std::min(1023.0, std::max(0.0, t))) << d;
+        }
+        key[k] = {code, s.atom_begin + k};
+    }
+    std::sort(key.begin(), key.end());
+    for (int k = 0; k < s.N; ++k) out[k] = key[k].second;
+    (void)c;
+}
+
 // Grow the systems' capacities for the device positions (first call, new radii, overflow).
 gbi_status size_from_device(gbi_ctx& c, long long min_entries) {
     std::vector<double> host(3 * (size_t)c.N);
@@ -487,6 +533,13 @@ gbi_status size_from_device(gbi_ctx& c, long long min_entries) {
         if (!std::isfinite(v)) return fail(GBI_ERR_NONFINITE, "non-finite atom position");
     bool changed = c.cell_total == 0;
     long long entries = 0;
+    if (c.any_cut) {
+        std::vector<int> order(c.N);
+        for (const SysHost& s : c.sys)
+            pair_order(c, s, host.data() + 3 * (size_t)s.atom_begin, order.data() + s.atom_begin);
+        HIP_TRY(hipMemcpyAsync(c.pair_atom.p, order.data(), sizeof(int) * c.N, hipMemcpyHostToDevice, c.stream));
+        HIP_TRY(hipStreamSynchronize(c.stream));
+    }
     for (SysHost& s : c.sys) {
         long long cells, ent;
         estimate_caps(c, s, host.data() + 3 * (size_t)s.atom_begin, cells, ent);
@@ -532,6 +585,7 @@ gbi::KParams make_kparams(gbi_ctx& c, bool adjoint) {
     k.any_trunc = c.any_trunc ? 1 : 0;
     k.blk_lo = c.blk_lo.p;
     k.blk_hi = c.blk_hi.p;
+    k.pair_atom = c.pair_atom.p;
     k.flags_all = c.flags_all.p;
     k.atom_sys = c.atom_sys.p;
     k.cell_sys = c.cell_sys.p;
@@ -875,6 +929,13 @@ gbi_status create_batch(int S, const int32_t* n_atoms, const double* charge, con
         for (int k = 0; k < h.N; ++k) order[h.atom_begin + k] = h.atom_begin + (int)((k * stride) % h.N);
     }
     CTX_HIP(c->bin_order.ensure(N));
+    CTX_HIP(c->pair_atom.ensure(N));
+    {
+        std::vector<int> ident(N);
+        for (size_t k = 0; k < N; ++k) ident[k] = (int)k;
+        CTX_HIP(hipMemcpyAsync(c->pair_atom.p, ident.data(), sizeof(int) * N, hipMemcpyHostToDevice, c->stream));
+        CTX_HIP(hipStreamSynchronize(c->stream));
+    }
     CTX_HIP(hipMemcpyAsync(c->bin_order.p, order.data(), sizeof(int) * N, hipMemcpyHostToDevice, c->stream));
     CTX_HIP(hipStreamSynchronize(c->stream));
     CTX_TRY(upload_atoms(*c));

@@ -150,7 +150,9 @@ struct KParams {
     const double* quad_pw;   // 2 x rows: w_q / r_q^(order_k - 2)
     const float* quad_r2;    // rows: |offset_q|^2
     int quad_rows;
-    const int4* pair_tiles;  // (row begin, column begin, column end, piece), global atom indices
+    const int4* pair_tiles;  // (row begin, column begin, column end, piece), global pair slots
+    const int* pair_atom;    // N: the atom of each pair slot, a permutation inside each system (identity,
+                             // or for cutoff systems a Morton order so 64-slot blocks are compact)
     const int* colblk;       // n_colblocks: first atom of every 64-atom block
     // grid
     int* cell_count;         // cell_total

@@ -1059,14 +1059,19 @@ __device__ inline BoxF box_f(const KParams& p, int s, const SysDesc& sd) {
 // d reduced to the minimum image (|d| below 1.5 box edges; L = 0: not periodic).
 __device__ inline float min_image(float d, float L, float inv) { return fmaf(-L, rintf(d * inv), d); }
 
-// A pair of a cutoff system: d at its minimum image (positions wrapped into the box; L = 0
-// when not periodic leaves d as it is).
+// A pair of a cutoff system: d at its minimum image when periodic (positions wrapped into the
+// box, so one box vector at most).
 __device__ inline PairTerms pair_terms_cut(double xj, double yj, double zj, double xi, double yi, double zi,
-                                           const BoxF& bx, float Bi, float Bj, float kexp_i, float iBj) {
+                                           bool per, const BoxF& bx, float Bi, float Bj, float kexp_i, float iBj) {
     PairTerms t;
-    t.dx = min_image((float)(xj - xi), bx.L[0], bx.inv[0]);
-    t.dy = min_image((float)(yj - yi), bx.L[1], bx.inv[1]);
-    t.dz = min_image((float)(zj - zi), bx.L[2], bx.inv[2]);
+    t.dx = (float)(xj - xi);
+    t.dy = (float)(yj - yi);
+    t.dz = (float)(zj - zi);
+    if (per) {
+        t.dx = min_image(t.dx, bx.L[0], bx.inv[0]);
+        t.dy = min_image(t.dy, bx.L[1], bx.inv[1]);
+        t.dz = min_image(t.dz, bx.L[2], bx.inv[2]);
+    }
     t.r2 = fmaf(t.dx, t.dx, fmaf(t.dy, t.dy, t.dz * t.dz));
     t.ex = __builtin_amdgcn_exp2f(-(t.r2 * kexp_i) * iBj);
     t.rf = __builtin_amdgcn_rsqf(fmaf(Bi * Bj, t.ex, t.r2));
@@ -1086,7 +1091,7 @@ __global__ void __launch_bounds__(64) k_block_box(KParams p) {
     const int j0 = p.colblk[b];
     const int s = p.atom_sys[j0];
     const SysDesc& sd = p.sys[s];
-    const int j = min(j0 + lane, sd.atom_end - 1);
+    const int j = p.pair_atom[min(j0 + lane, sd.atom_end - 1)];
     float lo[3], hi[3];
     for (int d = 0; d < 3; ++d) {
         const float v = (float)wrap_box(p.pos[3 * j + d], sd.periodic ? p.box[3 * s + d] : 0.0);
@@ -1139,14 +1144,16 @@ template <bool ADJ, bool CUT>
 __global__ void __launch_bounds__(64) k_pairs(KParams p) {
     __shared__ double s_px[64], s_py[64], s_pz[64];
     __shared__ float4 s_col[64];  // (q, B, 1/B, 1/(4B))
+    __shared__ int s_atom[64];
     if (batch_fatal(p)) return;
     const int lane = threadIdx.x;
     const int4 tile = p.pair_tiles[blockIdx.x];
     const int s = p.atom_sys[tile.x];
     const SysDesc& sd = p.sys[s];
     const int ab = sd.atom_begin, ae = sd.atom_end;
-    const int i = tile.x + lane;
+    const int i = tile.x + lane;  // pair slot; its atom is pair_atom[i]
     const bool iv = i < ae;
+    const int ai = iv ? p.pair_atom[i] : 0;
     const bool cut = CUT && sd.cut_mode != 0;
     const bool per = CUT && sd.periodic;
     BoxF bx;
@@ -1159,15 +1166,15 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
             Lz = p.box[3 * s + 2];
         }
     }
-    double xi = iv ? p.pos[3 * i] : 0.0, yi = iv ? p.pos[3 * i + 1] : 0.0, zi = iv ? p.pos[3 * i + 2] : 0.0;
+    double xi = iv ? p.pos[3 * ai] : 0.0, yi = iv ? p.pos[3 * ai + 1] : 0.0, zi = iv ? p.pos[3 * ai + 2] : 0.0;
     if (per) {
         xi = wrap_box(xi, Lx);
         yi = wrap_box(yi, Ly);
         zi = wrap_box(zi, Lz);
     }
     const float P = (float)sd.prefactor;
-    const float qi = iv ? (float)p.charge[i] : 0.f;  // zero charge: an absent row adds nothing
-    const float Bi = iv ? (float)p.born[i] : 1.f;
+    const float qi = iv ? (float)p.charge[ai] : 0.f;  // zero charge: an absent row adds nothing
+    const float Bi = iv ? (float)p.born[ai] : 1.f;
     const float Pq = P * qi;
     const float q4Bi = 0.25f / Bi;
     const float kexp = 1.4426950408889634f * q4Bi;
@@ -1189,8 +1196,10 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
             continue;
         }
         __syncthreads();
+        const int aj = jv ? p.pair_atom[jj] : 0;
+        s_atom[lane] = aj;
         {
-            double xj = jv ? p.pos[3 * jj] : 0.0, yj = jv ? p.pos[3 * jj + 1] : 0.0, zj = jv ? p.pos[3 * jj + 2] : 0.0;
+            double xj = jv ? p.pos[3 * aj] : 0.0, yj = jv ? p.pos[3 * aj + 1] : 0.0, zj = jv ? p.pos[3 * aj + 2] : 0.0;
             if (per) {
                 xj = wrap_box(xj, Lx);
                 yj = wrap_box(yj, Ly);
@@ -1201,9 +1210,9 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
             s_pz[lane] = zj;
         }
         {
-            const float Bj = jv ? (float)p.born[jj] : 1.f;
+            const float Bj = jv ? (float)p.born[aj] : 1.f;
             const float iB = 1.f / Bj;
-            s_col[lane] = make_float4(jv ? (float)p.charge[jj] : 0.f, Bj, iB, 0.25f * iB);
+            s_col[lane] = make_float4(jv ? (float)p.charge[aj] : 0.f, Bj, iB, 0.25f * iB);
         }
         __syncthreads();
         float efs = 0.f, dqs = 0.f, fxs = 0.f, fys = 0.f, fzs = 0.f, dbs = 0.f;
@@ -1215,8 +1224,8 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
                 const float4 cj = s_col[m];
                 PairTerms t;
                 if (cut) {
-                    t = pair_terms_cut(s_px[m], s_py[m], s_pz[m], xi, yi, zi, bx, Bi, cj.y, kexp, cj.z);
-                    if (!iv || !within_cutoff(p, s, sd, t.r2, i, jb + m)) t.rf = 0.f;
+                    t = pair_terms_cut(s_px[m], s_py[m], s_pz[m], xi, yi, zi, per, bx, Bi, cj.y, kexp, cj.z);
+                    if (!iv || !within_cutoff(p, s, sd, t.r2, ai, s_atom[m])) t.rf = 0.f;
                 } else {
                     t = pair_terms(s_px[m], s_py[m], s_pz[m], xi, yi, zi, Bi, cj.y, kexp, cj.z);
                 }
@@ -1238,8 +1247,8 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
                 const float4 cj = s_col[c];
                 PairTerms t;
                 if (cut) {
-                    t = pair_terms_cut(s_px[c], s_py[c], s_pz[c], xi, yi, zi, bx, Bi, cj.y, kexp, cj.z);
-                    if (!iv || jb + c >= ae || !within_cutoff(p, s, sd, t.r2, i, jb + c)) t.rf = 0.f;
+                    t = pair_terms_cut(s_px[c], s_py[c], s_pz[c], xi, yi, zi, per, bx, Bi, cj.y, kexp, cj.z);
+                    if (!iv || jb + c >= ae || !within_cutoff(p, s, sd, t.r2, ai, s_atom[c])) t.rf = 0.f;
                 } else {
                     t = pair_terms(s_px[c], s_py[c], s_pz[c], xi, yi, zi, Bi, cj.y, kexp, cj.z);
                 }
@@ -1310,7 +1319,7 @@ __global__ void __launch_bounds__(kChainGroups * kWave) k_atom_chain(KParams p) 
     const SysDesc& sd = p.sys[p.atom_sys[j0]];
     const int ab = sd.atom_begin;
     const int bj = (j0 - ab) / kWave;
-    const int i = j0 + lane;
+    const int i = j0 + lane;  // pair slot
     const bool iv = i < sd.atom_end;
     const int pieces = (sd.nblk - bj + sd.pair_k - 1) / sd.pair_k;
     const long long T = p.colpart_total;
@@ -1337,6 +1346,7 @@ __global__ void __launch_bounds__(kChainGroups * kWave) k_atom_chain(KParams p) 
     for (int v = 0; v < 6; ++v) s_acc[grp][v][lane] = a[v];
     __syncthreads();
     if (grp != 0 || !iv) return;
+    const int ia = p.pair_atom[i];  // the atom of pair slot i
     double r[6];
     for (int v = 0; v < 6; ++v) {
         r[v] = s_acc[0][v][lane];
@@ -1345,20 +1355,20 @@ __global__ void __launch_bounds__(kChainGroups * kWave) k_atom_chain(KParams p) 
     const double fx = r[0], fy = r[1], fz = r[2];
     double dedb = r[3], e = r[4];
     const double dq = r[5];
-    const double q = p.charge[i], B = p.born[i], P = sd.prefactor;
+    const double q = p.charge[ia], B = p.born[ia], P = sd.prefactor;
     e += 0.5 * P * q * q / B;
     dedb += -0.5 * P * q * q / (B * B);
-    p.atom_energy[i] = e;
-    p.force_fix[i] = (long long)llrint(fx * kFix32);
-    p.force_fix[p.N + i] = (long long)llrint(fy * kFix32);
-    p.force_fix[2 * p.N + i] = (long long)llrint(fz * kFix32);
-    p.dedi[i] = dedb * p.dbdi[i];
-    p.dedi[p.N + i] = dedb * p.dbdi[p.N + i];
+    p.atom_energy[ia] = e;
+    p.force_fix[ia] = (long long)llrint(fx * kFix32);
+    p.force_fix[p.N + ia] = (long long)llrint(fy * kFix32);
+    p.force_fix[2 * p.N + ia] = (long long)llrint(fz * kFix32);
+    p.dedi[ia] = dedb * p.dbdi[ia];
+    p.dedi[p.N + ia] = dedb * p.dbdi[p.N + ia];
     if (ADJ) {
-        p.dedq[i] = dq + P * q / B;
-        p.dedr_fix[i] = 0;
+        p.dedq[ia] = dq + P * q / B;
+        p.dedr_fix[ia] = 0;
         // B = slope / D + shift, D = a0 (1/eta - I4) + a1 T^(1/4), T = 1/(4 eta^4) - I7
-        const double I4 = p.integrals[sd.i4 * p.N + i], I7 = p.integrals[sd.i7 * p.N + i];
+        const double I4 = p.integrals[sd.i4 * p.N + ia], I7 = p.integrals[sd.i7 * p.N + ia];
         const double eta = sd.eta, eta4 = eta * eta * eta * eta;
         const double T = 1.0 / (4.0 * eta4) - I7;
         const double T4 = pow(T, 0.25);
@@ -1366,12 +1376,12 @@ __global__ void __launch_bounds__(kChainGroups * kWave) k_atom_chain(KParams p) 
         const double g = -sd.slope / (D * D) * dedb;  // dE/dD
         const double dDdeta = -sd.alpha0 / (eta * eta) - sd.alpha1 * 0.25 * pow(T, -0.75) / (eta4 * eta);
         double* th = p.theta_atom;
-        th[TH_PREFACTOR * p.N + i] = P != 0.0 ? e / P : 0.0;
-        th[TH_ALPHA0 * p.N + i] = g * (1.0 / eta - I4);
-        th[TH_ALPHA1 * p.N + i] = g * T4;
-        th[TH_ETA * p.N + i] = g * dDdeta;
-        th[TH_SLOPE * p.N + i] = dedb / D;
-        th[TH_SHIFT * p.N + i] = dedb;
+        th[TH_PREFACTOR * p.N + ia] = P != 0.0 ? e / P : 0.0;
+        th[TH_ALPHA0 * p.N + ia] = g * (1.0 / eta - I4);
+        th[TH_ALPHA1 * p.N + ia] = g * T4;
+        th[TH_ETA * p.N + ia] = g * dDdeta;
+        th[TH_SLOPE * p.N + ia] = dedb / D;
+        th[TH_SHIFT * p.N + ia] = dedb;
     }
 }
 
