@@ -152,7 +152,7 @@ struct KParams {
     int quad_rows;
     const int4* pair_tiles;  // (row begin, column begin, column end, piece), global pair slots
     const int* pair_atom;    // N: the atom of each pair slot, a permutation inside each system (identity,
-                             // or for cutoff systems a Morton order so 64-slot blocks are compact)
+                             // or for cutoff systems a serpentine order so 64-slot blocks are compact)
     const int* colblk;       // n_colblocks: first atom of every 64-atom block
     // grid
     int* cell_count;         // cell_total
