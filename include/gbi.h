@@ -218,6 +218,15 @@ gbi_status gbi_update_parameters(gbi_ctx* ctx, int32_t n_atoms, const double* ch
  */
 gbi_status gbi_set_periodic_box(gbi_ctx* ctx, int32_t system, const double* box_vectors);
 
+/*
+ * Excluded pairs of a GBI_MODEL_GBMV system whose pair energy term is ParticlePair
+ * (CharmmGBMVForce::addExclusion; CharmmReferenceGBMV.cpp:272-284 skips them; a
+ * ParticlePairNoExclusions term takes none): n_pairs pairs of local atom indices (2 x int32
+ * each), replacing the system's previous list (n_pairs = 0 clears it).  The self terms and
+ * the Born radii are unaffected.  GBI_MODEL_GBSW systems have no exclusions.
+ */
+gbi_status gbi_set_exclusions(gbi_ctx* ctx, int32_t system, int32_t n_pairs, const int32_t* pairs);
+
 /* Asynchronous pieces of gbi_execute for device-resident callers. */
 gbi_status gbi_set_positions(gbi_ctx* ctx, const double* pos, int32_t pos_on_device);
 gbi_status gbi_compute(gbi_ctx* ctx, uint32_t flags);   /* enqueue one evaluation */

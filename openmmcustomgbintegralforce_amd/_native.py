@@ -36,6 +36,7 @@ EXPORTS = [
     "gbi_set_stream", "gbi_get_adjoint", "gbi_execute_adjoint", "gbi_set_graph", "gbi_autotune",
     "gbi_set_tuning", "gbi_get_tuning", "gbi_destroy", "gbi_last_error", "gbi_abi_version", "gbi_quadrature",
     "gbi_profile_enable", "gbi_profile_read", "gbi_profile_pass_name", "gbi_set_periodic_box",
+    "gbi_set_exclusions",
 ]
 PROFILE_PASSES = ["grid", "bin", "volume", "integrals", "pairs", "chain", "force", "owner"]
 
@@ -136,6 +137,7 @@ def load() -> ctypes.CDLL:
     L.gbi_execute.argtypes = [vp, dp, ctypes.c_int32, ctypes.c_int32, dp, dp]
     L.gbi_update_parameters.argtypes = [vp, ctypes.c_int32, dp, dp]
     L.gbi_set_periodic_box.argtypes = [vp, ctypes.c_int32, dp]
+    L.gbi_set_exclusions.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
     L.gbi_set_positions.argtypes = [vp, vp, ctypes.c_int32]
     L.gbi_compute.argtypes = [vp, ctypes.c_uint32]
     L.gbi_synchronize.argtypes = [vp]
@@ -272,6 +274,12 @@ class Engine:
         b = np.asarray(box, dtype=np.float64)
         v = np.diag(b) if b.shape == (3,) else np.ascontiguousarray(b.reshape(3, 3))
         _check(self._lib.gbi_set_periodic_box(self._h, int(system), _dp(np.ascontiguousarray(v))))
+
+    def set_exclusions(self, pairs, system: int = 0) -> None:
+        """Excluded pairs (k x 2 local atom indices) of a ParticlePair pair term."""
+        a = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+        ptr = a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if a.shape[0] else None
+        _check(self._lib.gbi_set_exclusions(self._h, int(system), int(a.shape[0]), ptr))
 
     def set_positions(self, pos, on_device: bool = False) -> None:
         if on_device:

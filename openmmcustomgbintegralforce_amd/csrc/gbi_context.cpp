@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -86,6 +87,7 @@ struct SysHost {
     long long colpart_base = 0;
     double box[3] = {0.0, 0.0, 0.0};  // periodic box edges (gbi_set_periodic_box)
     bool box_set = false;
+    std::vector<int32_t> excl;         // excluded pairs, local atom indices (gbi_set_exclusions)
     bool periodic() const { return P.nonbonded_method == GBI_CUTOFF_PERIODIC; }
     bool cutoff() const { return P.nonbonded_method != GBI_NO_CUTOFF; }
 };
@@ -118,6 +120,10 @@ struct gbi_ctx {
     DevBuf<double> d_box;
     DevBuf<float4> blk_lo, blk_hi;
     DevBuf<int> pair_atom;
+    std::vector<int> pair_order_host;  // host copy of pair_atom
+    DevBuf<int> excl_start, excl_col;
+    DevBuf<unsigned long long> excl_mask;
+    bool any_excl = false;
     bool any_cut = false, any_trunc = false;
     DevBuf<float> colpart;
     int n_tiles = 0, max_splits = 1, n_colblocks = 0;
@@ -171,6 +177,9 @@ struct gbi_ctx {
         blk_lo.release();
         blk_hi.release();
         pair_atom.release();
+        excl_start.release();
+        excl_col.release();
+        excl_mask.release();
         colpart.release();
         DevBuf<int>* ib[] = {&flags_all,      &atom_sys,     &cell_sys,       &bin_order,       &cell_count, &cell_fill,
                              &cell_start,     &cell_count_eff, &cell_atoms_tmp, &cell_atoms, &cell_pt_count,
@@ -524,6 +533,53 @@ void pair_order(const SysHost& s, const double* pos, int* out) {
     for (int k = 0; k < s.N; ++k) out[k] = key[k].atom;
 }
 
+// Exclusion masks in pair-slot space: one entry per (row block, column block >= row block)
+// holding excluded pairs, 64 row masks each (diagonal blocks hold both orders of a pair).
+gbi_status build_exclusions(gbi_ctx& c) {
+    std::vector<int> slot(c.N);
+    for (int k = 0; k < c.N; ++k) slot[c.pair_order_host[k]] = k;
+    std::map<std::pair<int, int>, int> entry;
+    std::vector<unsigned long long> masks;
+    for (const SysHost& s : c.sys)
+        for (size_t k = 0; k + 1 < s.excl.size(); k += 2) {
+            int sa = slot[s.atom_begin + s.excl[k]] - s.atom_begin, sb = slot[s.atom_begin + s.excl[k + 1]] - s.atom_begin;
+            if (sa / 64 > sb / 64) std::swap(sa, sb);
+            const std::pair<int, int> key(s.blk_base + sa / 64, s.blk_base + sb / 64);
+            auto it = entry.find(key);
+            if (it == entry.end()) {
+                it = entry.emplace(key, (int)(masks.size() / 64)).first;
+                masks.resize(masks.size() + 64, 0ull);
+            }
+            unsigned long long* m = masks.data() + 64 * (size_t)it->second;
+            m[sa % 64] |= 1ull << (sb % 64);
+            if (key.first == key.second) m[sb % 64] |= 1ull << (sa % 64);
+        }
+    std::vector<int> start(c.n_colblocks + 1, 0), col;
+    std::vector<unsigned long long> sorted;
+    col.reserve(entry.size());
+    sorted.reserve(masks.size());
+    for (const auto& kv : entry) {  // std::map: ordered by (row block, column block)
+        start[kv.first.first + 1]++;
+        col.push_back(kv.first.second);
+        sorted.insert(sorted.end(), masks.begin() + 64 * (size_t)kv.second, masks.begin() + 64 * (size_t)kv.second + 64);
+    }
+    for (int b = 0; b < c.n_colblocks; ++b) start[b + 1] += start[b];
+    c.drop_graphs();
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    HIP_TRY(c.excl_start.ensure(start.size()));
+    HIP_TRY(c.excl_col.ensure(std::max<size_t>(1, col.size())));
+    HIP_TRY(c.excl_mask.ensure(std::max<size_t>(1, sorted.size())));
+    HIP_TRY(hipMemcpyAsync(c.excl_start.p, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice, c.stream));
+    if (!col.empty()) {
+        HIP_TRY(hipMemcpyAsync(c.excl_col.p, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice, c.stream));
+        HIP_TRY(hipMemcpyAsync(c.excl_mask.p, sorted.data(), sizeof(unsigned long long) * sorted.size(),
+                               hipMemcpyHostToDevice, c.stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    c.any_excl = !col.empty();
+    return GBI_OK;
+}
+
 // Grow the systems' capacities for the device positions (first call, new radii, overflow).
 gbi_status size_from_device(gbi_ctx& c, long long min_entries) {
     std::vector<double> host(3 * (size_t)c.N);
@@ -539,6 +595,8 @@ gbi_status size_from_device(gbi_ctx& c, long long min_entries) {
             pair_order(s, host.data() + 3 * (size_t)s.atom_begin, order.data() + s.atom_begin);
         HIP_TRY(hipMemcpyAsync(c.pair_atom.p, order.data(), sizeof(int) * c.N, hipMemcpyHostToDevice, c.stream));
         HIP_TRY(hipStreamSynchronize(c.stream));
+        c.pair_order_host = order;
+        if (c.any_excl) ST_TRY(build_exclusions(c));
     }
     for (SysHost& s : c.sys) {
         long long cells, ent;
@@ -586,6 +644,10 @@ gbi::KParams make_kparams(gbi_ctx& c, bool adjoint) {
     k.blk_lo = c.blk_lo.p;
     k.blk_hi = c.blk_hi.p;
     k.pair_atom = c.pair_atom.p;
+    k.any_excl = c.any_excl ? 1 : 0;
+    k.excl_start = c.excl_start.p;
+    k.excl_col = c.excl_col.p;
+    k.excl_mask = c.excl_mask.p;
     k.flags_all = c.flags_all.p;
     k.atom_sys = c.atom_sys.p;
     k.cell_sys = c.cell_sys.p;
@@ -935,6 +997,7 @@ gbi_status create_batch(int S, const int32_t* n_atoms, const double* charge, con
         for (size_t k = 0; k < N; ++k) ident[k] = (int)k;
         CTX_HIP(hipMemcpyAsync(c->pair_atom.p, ident.data(), sizeof(int) * N, hipMemcpyHostToDevice, c->stream));
         CTX_HIP(hipStreamSynchronize(c->stream));
+        c->pair_order_host = ident;
     }
     CTX_HIP(hipMemcpyAsync(c->bin_order.p, order.data(), sizeof(int) * N, hipMemcpyHostToDevice, c->stream));
     CTX_HIP(hipStreamSynchronize(c->stream));
@@ -1090,6 +1153,20 @@ gbi_status gbi_set_periodic_box(gbi_ctx* ctx, int32_t system, const double* v) {
     HIP_TRY(hipMemcpyAsync(ctx->d_box.p + 3 * system, s.box, sizeof(double) * 3, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return GBI_OK;
+}
+
+gbi_status gbi_set_exclusions(gbi_ctx* ctx, int32_t system, int32_t n_pairs, const int32_t* pairs) {
+    if (!ctx || (n_pairs > 0 && !pairs)) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
+    if (system < 0 || system >= ctx->S) return fail(GBI_ERR_INVALID_ARGUMENT, "no such system");
+    if (n_pairs < 0) return fail(GBI_ERR_INVALID_ARGUMENT, "negative pair count");
+    SysHost& s = ctx->sys[system];
+    if (s.P.model != GBI_MODEL_GBMV && n_pairs > 0)
+        return fail(GBI_ERR_INVALID_ARGUMENT, "CharmmGBSWForce has no exclusions");
+    for (int32_t k = 0; k < 2 * n_pairs; k += 2)
+        if (pairs[k] < 0 || pairs[k + 1] < 0 || pairs[k] >= s.N || pairs[k + 1] >= s.N || pairs[k] == pairs[k + 1])
+            return fail(GBI_ERR_INVALID_ARGUMENT, "excluded pair out of range or of an atom with itself");
+    s.excl.assign(pairs, pairs + 2 * (size_t)n_pairs);
+    return build_exclusions(*ctx);
 }
 
 gbi_status gbi_set_positions(gbi_ctx* ctx, const double* pos, int32_t pos_on_device) {

@@ -136,6 +136,10 @@ struct KParams {
     int any_trunc;           // some system truncates the chain rule at the cutoff
     float4* blk_lo;          // n_colblocks: bounding box of each 64-atom block (cutoff systems)
     float4* blk_hi;
+    int any_excl;            // some system has excluded pairs
+    const int* excl_start;   // n_colblocks + 1: entries of row block b are [excl_start[b], excl_start[b + 1])
+    const int* excl_col;     // per entry: its column block (>= the row block)
+    const unsigned long long* excl_mask;  // per entry, 64 rows: bit c = pair (row, column c) excluded
     int* flags_all;          // OR of every system's flags
     const int* atom_sys;     // N
     const int* cell_sys;     // cell_total

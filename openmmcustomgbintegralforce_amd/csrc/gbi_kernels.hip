@@ -1139,7 +1139,8 @@ __device__ inline float rotate_down(float v) {
 // CUT: systems with a cutoff drop the pairs outside it (rf = 0 zeroes every term), reduce
 // periodic differences to the minimum image (positions wrapped into the box on load, so one
 // box vector at most) and skip column blocks whose bounding boxes are beyond the cutoff
-// (their column partials are written as zeros).
+// (their column partials are written as zeros); excluded pairs (a ParticlePair term,
+// CharmmReferenceGBMV.cpp:272-284) are dropped through the block's 64 x 64 bit mask.
 template <bool ADJ, bool CUT>
 __global__ void __launch_bounds__(64) k_pairs(KParams p) {
     __shared__ double s_px[64], s_py[64], s_pz[64];
@@ -1195,6 +1196,15 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
             }
             continue;
         }
+        unsigned long long emask = 0ull;  // bit c: pair (this row, column c of the block) excluded
+        if (CUT && p.any_excl) {
+            const int rb = sd.blk_base + I_row, cb = sd.blk_base + (jb - ab) / kWave;
+            for (int e = p.excl_start[rb]; e < p.excl_start[rb + 1]; ++e)
+                if (p.excl_col[e] == cb) {
+                    emask = p.excl_mask[(size_t)e * kWave + lane];
+                    break;
+                }
+        }
         __syncthreads();
         const int aj = jv ? p.pair_atom[jj] : 0;
         s_atom[lane] = aj;
@@ -1229,6 +1239,7 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
                 } else {
                     t = pair_terms(s_px[m], s_py[m], s_pz[m], xi, yi, zi, Bi, cj.y, kexp, cj.z);
                 }
+                if (CUT && ((emask >> m) & 1ull)) t.rf = 0.f;
                 const float eij = Pq * cj.x * t.rf;
                 efs = fmaf(0.5f, eij, efs);
                 if (ADJ) dqs = fmaf(P * cj.x, t.rf, dqs);
@@ -1252,6 +1263,7 @@ __global__ void __launch_bounds__(64) k_pairs(KParams p) {
                 } else {
                     t = pair_terms(s_px[c], s_py[c], s_pz[c], xi, yi, zi, Bi, cj.y, kexp, cj.z);
                 }
+                if (CUT && ((emask >> c) & 1ull)) t.rf = 0.f;
                 const float eij = Pq * cj.x * t.rf;  // P q_i q_j / f
                 efs += eij;
                 const float t3 = eij * (t.rf * t.rf);
@@ -1864,9 +1876,14 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
     hipLaunchKernelGGL(k_integrals, dim3(p.N), dim3(64), 0, stream, p);
     mark(4);
     if (adj) {
-        // the host refuses the adjoint of cutoff systems
-        hipLaunchKernelGGL((k_pairs<true, false>), dim3(p.n_tiles), dim3(64), 0, stream, p);
-    } else if (p.any_cut) {
+        // the host refuses the adjoint of cutoff systems; exclusions take the general path
+        if (p.any_excl) {
+            hipLaunchKernelGGL(k_block_box, dim3(p.n_colblocks), dim3(64), 0, stream, p);
+            hipLaunchKernelGGL((k_pairs<true, true>), dim3(p.n_tiles), dim3(64), 0, stream, p);
+        } else {
+            hipLaunchKernelGGL((k_pairs<true, false>), dim3(p.n_tiles), dim3(64), 0, stream, p);
+        }
+    } else if (p.any_cut || p.any_excl) {
         hipLaunchKernelGGL(k_block_box, dim3(p.n_colblocks), dim3(64), 0, stream, p);
         hipLaunchKernelGGL((k_pairs<false, true>), dim3(p.n_tiles), dim3(64), 0, stream, p);
     } else {

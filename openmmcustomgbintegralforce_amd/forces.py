@@ -217,12 +217,14 @@ CANONICAL_PAIR = "P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))"
 def canonical_gbmv_force(charges, radii, integral_type: int, *, prefactor: float | None = None,
                          lebedev_rule: int | None = None, n_radial: int = 24,
                          alpha0=-0.180, alpha1=1.817, eta=0.05, slope=1.0, shift=0.0,
-                         solute_dielectric=1.0, solvent_dielectric=80.0) -> CharmmGBMVForce:
+                         solute_dielectric=1.0, solvent_dielectric=80.0,
+                         pair_type: int = 2) -> CharmmGBMVForce:
     """A CharmmGBMVForce holding the canonical GB energy the HIP platform implements:
 
     integrals I4 (order 4) and I7 (order 7); computed value
     B = slope/(alpha0 (1/eta - I4) + alpha1 (1/(4 eta^4) - I7)^(1/4)) + shift;
-    energy 0.5 P q^2/B (single particle) + P q1 q2/f (pairs, no exclusions),
+    energy 0.5 P q^2/B (single particle) + P q1 q2/f (pairs: ``pair_type``
+    ParticlePairNoExclusions by default, or ParticlePair, which honours ``addExclusion``),
     P = -ONE_4PI_EPS0 (1/solute - 1/solvent).
     ``lebedev_rule`` (parametersInt[1]) and ``n_radial`` (parametersInt[2]) extend the
     reference, which fixes rule 3 and 24 radial points (GBSWIntegral.cpp:37-40).
@@ -241,7 +243,7 @@ def canonical_gbmv_force(charges, radii, integral_type: int, *, prefactor: float
     f.setGBIntegralType(integral_type)
     f.addComputedValue("B", "slope*" + CANONICAL_BORN[0] + "+shift", CharmmGBMVForce.SingleParticle)
     f.addEnergyTerm(CANONICAL_SELF, CharmmGBMVForce.SingleParticle)
-    f.addEnergyTerm(CANONICAL_PAIR, CharmmGBMVForce.ParticlePairNoExclusions)
+    f.addEnergyTerm(CANONICAL_PAIR, pair_type)
     for q, r in zip(charges, radii):
         f.addParticle([q, r])
     return f
@@ -270,9 +272,9 @@ def gbmv_params_from_force(force: CharmmGBMVForce) -> tuple:
     values = [(n, _norm(e), ty) for n, e, ty in force._values]
     energies = [(_norm(e), ty) for e, ty in force._energy]
     canon_values = [("B", _norm("slope*" + CANONICAL_BORN[0] + "+shift"), CharmmGBMVForce.SingleParticle)]
-    canon_energy = [(_norm(CANONICAL_SELF), CharmmGBMVForce.SingleParticle),
-                    (_norm(CANONICAL_PAIR), CharmmGBMVForce.ParticlePairNoExclusions)]
-    if values != canon_values or energies != canon_energy:
+    canon_energy = [[(_norm(CANONICAL_SELF), CharmmGBMVForce.SingleParticle), (_norm(CANONICAL_PAIR), t)]
+                    for t in (CharmmGBMVForce.ParticlePairNoExclusions, CharmmGBMVForce.ParticlePair)]
+    if values != canon_values or energies not in canon_energy:
         raise N.GbiError(2, "only the canonical GB expressions run on the HIP platform")
     g = {force.getGlobalParameterName(i): force.getGlobalParameterDefaultValue(i)
          for i in range(force.getNumGlobalParameters())}
@@ -368,6 +370,15 @@ class HipCalcCharmmGBSWForceKernel:
         return self._engine
 
 
+def gbmv_exclusions(force: CharmmGBMVForce) -> np.ndarray:
+    """Excluded pairs (k x 2) the canonical pair term honours: those of a ParticlePair term,
+    none for ParticlePairNoExclusions (CharmmReferenceGBMV.cpp:196-199, :272-284)."""
+    pair_type = force._energy[1][1] if len(force._energy) > 1 else CharmmGBMVForce.ParticlePairNoExclusions
+    if pair_type != CharmmGBMVForce.ParticlePair or not force._exclusions:
+        return np.zeros((0, 2), dtype=np.int32)
+    return np.asarray(force._exclusions, dtype=np.int32).reshape(-1, 2)
+
+
 class HipCalcCharmmGBMVForceKernel:
     """CalcCharmmGBMVForceKernel on HIP for the canonical GB energy."""
 
@@ -387,6 +398,9 @@ class HipCalcCharmmGBMVForceKernel:
             setattr(p, "lambda_" if k == "lambda" else k, v)
         self._method = p.nonbonded_method
         self._engine = N.Engine(p, q, r)
+        ex = gbmv_exclusions(force)
+        if len(ex):
+            self._engine.set_exclusions(ex)
 
     def execute(self, context: ContextImpl, includeForces: bool = True, includeEnergy: bool = True) -> float:
         _set_box(self._engine, self._method, context)

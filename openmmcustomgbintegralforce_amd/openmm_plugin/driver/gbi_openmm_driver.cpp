@@ -5,7 +5,8 @@
 // Input (binary, native endianness): int32 model (0 CharmmGBSWForce, 1 CharmmGBMVForce),
 // int32 integral type, int32 Lebedev rule (0 = the reference's), int32 N, then N x 3
 // positions, N charges, N radii (doubles), optionally followed by int32 nonbonded method,
-// double cutoff and 3 doubles of a rectangular periodic box.  Output (text): energy,
+// double cutoff and 3 doubles of a rectangular periodic box, then optionally int32 n and n
+// excluded pairs (2 x int32) that make the GBMV pair term ParticlePair.  Output (text): energy,
 // forces, the energy after copyParametersToContext with charges x 1.5, and the outcome of
 // the error paths.
 #include <cstdio>
@@ -28,7 +29,8 @@ void rd(FILE* f, T* p, size_t n) {
     if (fread(p, sizeof(T), n, f) != n) throw std::runtime_error("short input");
 }
 
-CharmmGBMVForce* canonicalGBMV(const std::vector<double>& q, const std::vector<double>& r, int type, int rule) {
+CharmmGBMVForce* canonicalGBMV(const std::vector<double>& q, const std::vector<double>& r, int type, int rule,
+                               const std::vector<int32_t>& excl) {
     auto* f = new CharmmGBMVForce();
     f->addPerParticleParameter("q");
     f->addPerParticleParameter("radius");
@@ -48,8 +50,10 @@ CharmmGBMVForce* canonicalGBMV(const std::vector<double>& q, const std::vector<d
     f->setGBIntegralType((CharmmGBMVForce::GBIntegralType)type);
     f->addComputedValue("B", gbi_openmm::kCanonicalBornValue, CharmmGBMVForce::SingleParticle);
     f->addEnergyTerm(gbi_openmm::kCanonicalSelfEnergy, CharmmGBMVForce::SingleParticle);
-    f->addEnergyTerm(gbi_openmm::kCanonicalPairEnergy, CharmmGBMVForce::ParticlePairNoExclusions);
+    f->addEnergyTerm(gbi_openmm::kCanonicalPairEnergy,
+                     excl.empty() ? CharmmGBMVForce::ParticlePairNoExclusions : CharmmGBMVForce::ParticlePair);
     for (size_t i = 0; i < q.size(); ++i) f->addParticle({q[i], r[i]});
+    for (size_t k = 0; k + 1 < excl.size(); k += 2) f->addExclusion(excl[k], excl[k + 1]);
     return f;
 }
 
@@ -72,9 +76,15 @@ int main(int argc, char** argv) {
         rd(in, r.data(), r.size());
         int32_t method = 0;
         double cutoff = 1.0, box[3] = {2.0, 2.0, 2.0};
+        std::vector<int32_t> excl;
         if (fread(&method, sizeof(method), 1, in) == 1) {
             rd(in, &cutoff, 1);
             rd(in, box, 3);
+            int32_t nex = 0;
+            if (fread(&nex, sizeof(nex), 1, in) == 1 && nex > 0) {
+                excl.resize(2 * (size_t)nex);
+                rd(in, excl.data(), excl.size());
+            }
         }
         fclose(in);
 
@@ -120,7 +130,7 @@ int main(int argc, char** argv) {
             }
             fprintf(out, "count_change_throws %d\n", count_throws);
         } else {
-            std::unique_ptr<CharmmGBMVForce> force(canonicalGBMV(q, r, type, rule));
+            std::unique_ptr<CharmmGBMVForce> force(canonicalGBMV(q, r, type, rule, excl));
             force->setNonbondedMethod((CharmmGBMVForce::NonbondedMethod)method);
             force->setCutoffDistance(cutoff);
             std::unique_ptr<KernelImpl> k(platform.createKernel(CalcCharmmGBMVForceKernel::Name(), context));
@@ -131,7 +141,7 @@ int main(int argc, char** argv) {
             fprintf(out, "energy %.17g\n", e);
             for (int i = 0; i < n; ++i) fprintf(out, "%.17g %.17g %.17g\n", f[i][0], f[i][1], f[i][2]);
             e_noforce = kernel.execute(context, false, true);
-            std::unique_ptr<CharmmGBMVForce> scaled(canonicalGBMV(qs, r, type, rule));
+            std::unique_ptr<CharmmGBMVForce> scaled(canonicalGBMV(qs, r, type, rule, excl));
             scaled->setNonbondedMethod((CharmmGBMVForce::NonbondedMethod)method);
             scaled->setCutoffDistance(cutoff);
             kernel.copyParametersToContext(context, *scaled);

@@ -149,7 +149,8 @@ void gbmvParamsFromForce(const CharmmGBMVForce& force, gbi_params& p, std::vecto
         force.getEnergyTermParameters(0, expr, t);
         canonical = canonical && strip(expr) == kCanonicalSelfEnergy && t == CharmmGBMVForce::SingleParticle;
         force.getEnergyTermParameters(1, expr, t);
-        canonical = canonical && strip(expr) == kCanonicalPairEnergy && t == CharmmGBMVForce::ParticlePairNoExclusions;
+        canonical = canonical && strip(expr) == kCanonicalPairEnergy &&
+                    (t == CharmmGBMVForce::ParticlePairNoExclusions || t == CharmmGBMVForce::ParticlePair);
     }
     if (!canonical) throw OpenMMException("CharmmGBMVForce: only the canonical GB expressions run on the HIP platform");
     if (gbi_params_default(GBI_MODEL_GBMV, (int32_t)type, &p) != GBI_OK) throw OpenMMException(gbi_last_error());
@@ -191,6 +192,20 @@ void HipCalcCharmmGBMVForceKernel::initialize(const System& system, const Charmm
     std::vector<double> q, r;
     gbmvParamsFromForce(force, p, q, r);
     create(force.getNumParticles(), q, r, p);
+    // a ParticlePair term skips the force's exclusions (CharmmReferenceGBMV.cpp:272-284)
+    std::string expr;
+    CharmmGBMVForce::ComputationType type;
+    force.getEnergyTermParameters(1, expr, type);
+    if (type == CharmmGBMVForce::ParticlePair && force.getNumExclusions() > 0) {
+        std::vector<int32_t> pairs(2 * (size_t)force.getNumExclusions());
+        for (int k = 0; k < force.getNumExclusions(); ++k) {
+            int a, b;
+            force.getExclusionParticles(k, a, b);
+            pairs[2 * k] = a;
+            pairs[2 * k + 1] = b;
+        }
+        check(gbi_set_exclusions(ctx_, 0, force.getNumExclusions(), pairs.data()));
+    }
 }
 
 double HipCalcCharmmGBMVForceKernel::execute(ContextImpl& context, bool includeForces, bool includeEnergy) {

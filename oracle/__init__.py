@@ -63,6 +63,8 @@ class Params(ctypes.Structure):
         ("method", ctypes.c_int),
         ("cutoff", ctypes.c_double),
         ("box", ctypes.c_double * 3),
+        ("n_excl", ctypes.c_int),
+        ("excl", ctypes.POINTER(ctypes.c_int)),
     ]
 
 
@@ -194,6 +196,16 @@ class Result:
     integrals: np.ndarray
     dEdI: np.ndarray
     stats: dict
+
+
+def set_exclusions(params: Params, pairs) -> Params:
+    """Excluded pairs (k x 2 atom indices) of a ParticlePair term; the array is kept on the
+    Params object so it outlives the call."""
+    arr = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+    params._excl_keep = arr
+    params.n_excl = arr.shape[0]
+    params.excl = arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)) if arr.shape[0] else None
+    return params
 
 
 def compute(params: Params, pos, charge, radius, want_stats: bool = False) -> Result:

@@ -730,9 +730,27 @@ static void born_map(const orc_params* p, double I4, double I7, double* B, doubl
  * overwrites dEdB.  With a cutoff only pairs inside it count (minimum image when
  * periodic); the self term always does (CharmmReferenceGBSW.cpp:188-199: every atom
  * pairs with itself; CharmmReferenceGBMV.cpp: it is a SingleParticle term). */
+static int cmp_ll(const void* a, const void* b) {
+    const long long x = *(const long long*)a, y = *(const long long*)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
 static double still_energy(const orc_params* p, int N, const double* pos, const double* q, const double* B,
                            double P, double* F, double* dEdB) {
     double E = 0.0;
+    /* excluded pairs as sorted keys i N + j (i < j); CharmmReferenceGBMV.cpp:272-284 skips
+     * them in a ParticlePair term */
+    long long* ex = NULL;
+    int nex = 0;
+    if (p->model == ORC_MODEL_GBMV_FORM && p->n_excl > 0 && p->excl) {
+        ex = malloc(sizeof(long long) * (size_t)p->n_excl);
+        for (int k = 0; k < p->n_excl && ex; ++k) {
+            int a = p->excl[2 * k], b = p->excl[2 * k + 1];
+            if (a > b) { int t = a; a = b; b = t; }
+            ex[nex++] = (long long)a * N + b;
+        }
+        if (ex) qsort(ex, (size_t)nex, sizeof(long long), cmp_ll);
+    }
     for (int i = 0; i < N; ++i) dEdB[i] = 0.0;
     for (int i = 0; i < N; ++i) {
         E += 0.5 * P * q[i] * q[i] / B[i];
@@ -742,6 +760,10 @@ static double still_energy(const orc_params* p, int N, const double* pos, const 
             delta(p, pos + 3 * j, pos + 3 * i, &dx, &dy, &dz);
             double r2 = dx * dx + dy * dy + dz * dz;
             if (!within_cutoff(p, r2)) continue;
+            if (ex) {
+                const long long key = (long long)i * N + j;
+                if (bsearch(&key, ex, (size_t)nex, sizeof(long long), cmp_ll)) continue;
+            }
             double BB = B[i] * B[j];
             double D = r2 / (4.0 * BB);
             double ex = exp(-D);
@@ -761,6 +783,7 @@ static double still_energy(const orc_params* p, int N, const double* pos, const 
             dEdB[j] += common * (B[i] + r2 / (4.0 * B[j]));
         }
     }
+    free(ex);
     return E;
 }
 
@@ -846,6 +869,11 @@ int orc_compute(const orc_params* p, int N, const double* pos, const double* cha
     if (i4 < 0 || i7 < 0) return ORC_ERR_ARG; /* canonical Born map needs orders 4 and 7 */
 
     if (pp.method < ORC_NO_CUTOFF || pp.method > ORC_CUTOFF_PERIODIC) return ORC_ERR_ARG;
+    if (pp.n_excl < 0 || (pp.n_excl > 0 && !pp.excl)) return ORC_ERR_ARG;
+    for (int k = 0; k < pp.n_excl; ++k) {
+        const int a = pp.excl[2 * k], b = pp.excl[2 * k + 1];
+        if (a < 0 || b < 0 || a >= N || b >= N || a == b) return ORC_ERR_ARG;
+    }
     if (pp.method != ORC_NO_CUTOFF && !(pp.cutoff > 0.0)) return ORC_ERR_ARG;
     if (pp.method == ORC_CUTOFF_PERIODIC)
         for (int d = 0; d < 3; ++d) /* CharmmGBMVForceImpl.cpp:99-104, CharmmGBSWForceImpl.cpp:48-54 */

@@ -53,6 +53,10 @@ typedef struct {
     int method;         /* ORC_NO_CUTOFF, ORC_CUTOFF_NON_PERIODIC, ORC_CUTOFF_PERIODIC */
     double cutoff;      /* nm */
     double box[3];      /* rectangular periodic box edges (ORC_CUTOFF_PERIODIC) */
+    /* excluded pairs of a ParticlePair energy term (CharmmGBMVForce::addExclusion):
+     * n_excl pairs of atom indices (2 ints each); GBMV form only */
+    int n_excl;
+    const int* excl;
 } orc_params;
 
 enum { ORC_NO_CUTOFF = 0, ORC_CUTOFF_NON_PERIODIC = 1, ORC_CUTOFF_PERIODIC = 2 };

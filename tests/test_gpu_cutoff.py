@@ -1,11 +1,12 @@
-"""Cutoff and periodic systems on the HIP path vs the FP64 oracle (needs an MI355X).
+"""Cutoff, periodic and exclusion paths on the HIP path vs the FP64 oracle (needs an MI355X).
 
 Tolerances as in test_gpu_parity: force relative RMS <= 1e-5, energy relative error <= 1e-5.
 Covered: CutoffNonPeriodic for both forces (the standalone GBSW chain rule truncated at
 the cutoff, CharmmReferenceGBSW.cpp:285-296), cutoffs short enough that whole 64-atom
 blocks are skipped, CutoffPeriodic for a globule interacting with its images and for a
 liquid filling the box, moves by box vectors, batches mixing methods, graph replay across a
-box change (the periodic grid regrows), and the rejected cases.
+box change (the periodic grid regrows), excluded pairs of a ParticlePair term
+(CharmmReferenceGBMV.cpp:272-284) with and without a cutoff, and the rejected cases.
 """
 import numpy as np
 import pytest
@@ -167,3 +168,62 @@ def test_rejected_cases():
     with pytest.raises(N.GbiError):
         flat.set_periodic_box([3.0, 3.0, 3.0])
     flat.close()
+
+
+def _bonded_like_exclusions(pos, k=3):
+    """Each atom with its k nearest neighbours (the shape of 1-2/1-3 exclusions)."""
+    from scipy.spatial import cKDTree
+
+    _, nb = cKDTree(pos).query(pos, k=k + 1)
+    pairs = {(min(i, int(j)), max(i, int(j))) for i in range(len(pos)) for j in nb[i, 1:]}
+    return np.array(sorted(pairs), dtype=np.int32)
+
+
+@pytest.mark.parametrize("method", [N.NO_CUTOFF, N.CUTOFF_NON_PERIODIC, N.CUTOFF_PERIODIC])
+def test_exclusions(method):
+    s = S.globule(600, 18)
+    L = np.array([4.4, 4.2, 4.6])
+    pos = s.pos - s.pos.mean(axis=0) + L / 2
+    ex = _bonded_like_exclusions(pos)
+    p = _params("gbmv2-50", method, 1.3)
+    eng = N.Engine(p, s.charge, s.radius)
+    box = L if method == N.CUTOFF_PERIODIC else None
+    if box is not None:
+        eng.set_periodic_box(L)
+    eng.set_exclusions(ex)
+    e, f = eng.execute(pos)
+    op = O.set_exclusions(oracle_params(p, box=box), ex)
+    ref = O.compute(op, pos, s.charge, s.radius)
+    plain = O.compute(oracle_params(p, box=box), pos, s.charge, s.radius)
+    ee, fe = rel_err(e, ref.energy), rel_rms(f, ref.forces)
+    print(f"[cutoff] exclusions method {method}: {len(ex)} pairs, energy {ee:.2e} force {fe:.2e} "
+          f"(the exclusions move E by {abs(ref.energy - plain.energy):.1f} kJ/mol)", flush=True)
+    assert ee < ENERGY_TOL and fe < FORCE_TOL
+    assert abs(ref.energy - plain.energy) > 1e3 * ENERGY_TOL * abs(ref.energy)
+    # cleared again: the plain energy
+    eng.set_exclusions(np.zeros((0, 2), dtype=np.int32))
+    e2, _ = eng.execute(pos)
+    assert rel_err(e2, plain.energy) < ENERGY_TOL
+    eng.close()
+
+
+def test_exclusions_adjoint_and_errors():
+    s = S.globule(300, 19)
+    ex = _bonded_like_exclusions(s.pos)
+    p = _params("gbmv1", N.NO_CUTOFF, 1.0)
+    eng = N.Engine(p, s.charge, s.radius)
+    eng.set_exclusions(ex)
+    e, f = eng.execute(s.pos)
+    out = eng.execute_adjoint(s.pos)
+    print(f"[cutoff] exclusions adjoint: E {e:.6f} vs {out['energy']:.6f}", flush=True)
+    assert abs(out["energy"] - e) <= 1e-12 * abs(e)
+    dq = out["dEdq"]
+    # E is quadratic in the charges: sum_i q_i dE/dq_i = 2E
+    assert abs(float(np.dot(s.charge, dq)) - 2.0 * e) < 1e-5 * abs(e)
+    with pytest.raises(N.GbiError):
+        eng.set_exclusions([[0, s.n]])
+    eng.close()
+    g = N.Engine(_params("gbsw", N.NO_CUTOFF, 1.0), s.charge, s.radius)
+    with pytest.raises(N.GbiError):
+        g.set_exclusions(ex)
+    g.close()

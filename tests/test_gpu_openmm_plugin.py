@@ -70,3 +70,18 @@ def test_cpp_kernel_classes_cutoff_periodic(tmp_path, case):
     assert rel_err(e, ref.energy) < 1e-5
     assert rel_rms(f, ref.forces) < 1e-5
     assert float(rest["energy_no_forces"]) == e
+
+
+def test_cpp_kernel_classes_particle_pair_exclusions(tmp_path):
+    """A canonical CharmmGBMVForce whose pair term is ParticlePair: the kernel class passes
+    the force's exclusions (CharmmReferenceGBMV.cpp:272-284)."""
+    from test_gpu_cutoff import _bonded_like_exclusions
+
+    s = S.globule(400, 43)
+    ex = _bonded_like_exclusions(s.pos)
+    p = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV2)
+    e, f, rest = _run(tmp_path, 1, 2, 0, s, method=N.NO_CUTOFF, cutoff=1.0, exclusions=ex)
+    print(f"[openmm-plugin] gbmv2 ParticlePair with {len(ex)} exclusions: energy {e:.6f}", flush=True)
+    ref = O.compute(O.set_exclusions(oracle_params(p), ex), s.pos, s.charge, s.radius)
+    assert rel_err(e, ref.energy) < 1e-5
+    assert rel_rms(f, ref.forces) < 1e-5

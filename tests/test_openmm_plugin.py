@@ -12,7 +12,7 @@ PLUGIN = os.path.join(PKG, "lib", "libgbi_openmm.so")
 DRIVER = os.path.join(PKG, "lib", "gbi_openmm_driver")
 
 
-def write_input(path, model, itype, rule, pos, q, R, method=None, cutoff=1.0, box=(2.0, 2.0, 2.0)):
+def write_input(path, model, itype, rule, pos, q, R, method=None, cutoff=1.0, box=(2.0, 2.0, 2.0), exclusions=None):
     with open(path, "wb") as fh:
         np.array([model, itype, rule, len(q)], dtype=np.int32).tofile(fh)
         np.ascontiguousarray(pos, dtype=np.float64).tofile(fh)
@@ -21,6 +21,10 @@ def write_input(path, model, itype, rule, pos, q, R, method=None, cutoff=1.0, bo
         if method is not None:
             np.array([method], dtype=np.int32).tofile(fh)
             np.array([cutoff, *box], dtype=np.float64).tofile(fh)
+            if exclusions is not None:
+                ex = np.asarray(exclusions, dtype=np.int32).reshape(-1, 2)
+                np.array([ex.shape[0]], dtype=np.int32).tofile(fh)
+                ex.tofile(fh)
 
 
 def test_plugin_exports_openmm_entry_points():

@@ -158,3 +158,39 @@ def test_periodic_box_checks():
         O.compute(O.gbmv_params(2, method=O.CUTOFF_PERIODIC, cutoff=0.5, box=[1.2, 4.0, 4.0]), s.pos, s.charge, s.radius)
     with pytest.raises(RuntimeError):
         O.compute(O.gbmv_params(2, method=O.CUTOFF_NON_PERIODIC, cutoff=0.0), s.pos, s.charge, s.radius)
+
+
+def test_exclusions_of_a_particle_pair_term():
+    """Excluded pairs leave a ParticlePair term (CharmmReferenceGBMV.cpp:272-284): for two
+    atoms the excluded pair is the cutoff that drops it; for many, the energy is additive
+    over disjoint exclusion sets (radii do not depend on exclusions) and forces stay -grad E."""
+    pos = np.array([[0.0, 0.0, 0.0], [0.6, 0.0, 0.0]])
+    q, R = [0.5, -0.4], [0.15, 0.2]
+    ex = O.compute(O.set_exclusions(O.gbmv_params(2), [[1, 0]]), pos, q, R)
+    cut = O.compute(O.gbmv_params(2, method=O.CUTOFF_NON_PERIODIC, cutoff=0.5), pos, q, R)
+    assert ex.energy == cut.energy and np.array_equal(ex.forces, cut.forces)
+
+    s = S.globule(120, 6)
+    rng = np.random.default_rng(2)
+    pairs = []
+    while len(pairs) < 40:
+        a, b = rng.integers(0, s.n, 2)
+        if a != b and (a, b) not in pairs and (b, a) not in pairs:
+            pairs.append((int(a), int(b)))
+    A, B = pairs[:20], pairs[20:]
+    for method in (O.NO_CUTOFF, O.CUTOFF_NON_PERIODIC):
+        def energy(sel):
+            p = O.set_exclusions(O.gbmv_params(2, method=method, cutoff=1.0), sel)
+            return O.compute(p, s.pos, s.charge, s.radius)
+        e0, ea, eb, eab = energy([]), energy(A), energy(B), energy(A + B)
+        assert abs((eab.energy - ea.energy) - (eb.energy - e0.energy)) < 1e-10 * abs(e0.energy)
+        assert eab.energy != e0.energy
+        assert np.array_equal(eab.born, e0.born)
+        _fd_check(O.set_exclusions(O.gbmv_params(2, method=method, cutoff=1.0), A + B), s.pos, s.charge,
+                  s.radius, eab)
+    # the standalone GBSW force has no exclusions: the list is ignored
+    g = O.compute(O.gbsw_standalone_params(), s.pos, s.charge, s.radius)
+    gx = O.compute(O.set_exclusions(O.gbsw_standalone_params(), A), s.pos, s.charge, s.radius)
+    assert g.energy == gx.energy
+    with pytest.raises(RuntimeError):
+        O.compute(O.set_exclusions(O.gbmv_params(2), [[3, 3]]), s.pos, s.charge, s.radius)
