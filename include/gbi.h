@@ -227,6 +227,48 @@ gbi_status gbi_set_periodic_box(gbi_ctx* ctx, int32_t system, const double* box_
  */
 gbi_status gbi_set_exclusions(gbi_ctx* ctx, int32_t system, int32_t n_pairs, const int32_t* pairs);
 
+/*
+ * General CharmmGBMVForce expressions (CharmmGBMVForce.h: addPerParticleParameter,
+ * addGlobalParameter, addGBIntegral, addComputedValue, addEnergyTerm), Lepton syntax:
+ * + - * / ^, unary minus, parentheses, sqrt exp log sin cos tan asin acos atan sinh cosh tanh
+ * erf erfc abs step delta square cube recip min max select.  Computed values are
+ * SingleParticle (as the reference evaluates them, CharmmReferenceGBMV.cpp:173-176) and may
+ * use parameters, globals, integrals and earlier values; SingleParticle energy terms see the
+ * same; pair terms (ParticlePair, ParticlePairNoExclusions) see r, globals and parameters and
+ * values with suffixes 1 and 2 (atom1 < atom2).  Integrals: one or two, of orders
+ * params->orders[0..n_integrals-1] (>= 2, GBSWIntegral.cpp:92-95), of params->integral_type.
+ */
+typedef struct gbi_custom_spec {
+    int32_t n_params;
+    const char* const* param_names;
+    const double* param_values;      /* n_atoms x n_params, row-major */
+    int32_t radius_param;            /* the parameter that is the atomic radius of the integrals */
+    int32_t n_globals;
+    const char* const* global_names;
+    const double* global_values;
+    int32_t n_integrals;             /* 1 or 2 */
+    const char* const* integral_names;
+    int32_t n_values;                /* 0..4 */
+    const char* const* value_names;
+    const char* const* value_exprs;
+    int32_t n_terms;
+    const char* const* term_exprs;
+    const int32_t* term_types;       /* 0 SingleParticle, 1 ParticlePair, 2 ParticlePairNoExclusions */
+} gbi_custom_spec;
+
+/*
+ * A one-system context for general expressions: replaces CalcCharmmGBMVForceKernel::initialize
+ * (charmmKernels.h:71) for a force whose expressions are not the canonical GB forms.  The
+ * expressions are compiled for the device here (parse errors: GBI_ERR_INVALID_ARGUMENT with
+ * the message).  Evaluations, cutoffs, periodic boxes and exclusions work as for other
+ * contexts; the adjoint and gbi_update_parameters are canonical-only.
+ */
+gbi_status gbi_create_custom(int32_t n_atoms, const gbi_params* params, const gbi_custom_spec* spec, gbi_ctx** out);
+/* New values of the global parameters (Context::setParameter), n_globals doubles. */
+gbi_status gbi_set_custom_globals(gbi_ctx* ctx, int32_t n_globals, const double* values);
+/* The computed values of the last evaluation: n_values x N doubles (value-major). */
+gbi_status gbi_get_custom_values(gbi_ctx* ctx, double* values);
+
 /* Asynchronous pieces of gbi_execute for device-resident callers. */
 gbi_status gbi_set_positions(gbi_ctx* ctx, const double* pos, int32_t pos_on_device);
 gbi_status gbi_compute(gbi_ctx* ctx, uint32_t flags);   /* enqueue one evaluation */

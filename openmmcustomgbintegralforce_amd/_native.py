@@ -36,7 +36,7 @@ EXPORTS = [
     "gbi_set_stream", "gbi_get_adjoint", "gbi_execute_adjoint", "gbi_set_graph", "gbi_autotune",
     "gbi_set_tuning", "gbi_get_tuning", "gbi_destroy", "gbi_last_error", "gbi_abi_version", "gbi_quadrature",
     "gbi_profile_enable", "gbi_profile_read", "gbi_profile_pass_name", "gbi_set_periodic_box",
-    "gbi_set_exclusions",
+    "gbi_set_exclusions", "gbi_create_custom", "gbi_set_custom_globals", "gbi_get_custom_values",
 ]
 PROFILE_PASSES = ["grid", "bin", "volume", "integrals", "pairs", "chain", "force", "owner"]
 
@@ -109,6 +109,35 @@ class DeviceView(ctypes.Structure):
     ]
 
 
+class CustomSpec(ctypes.Structure):
+    """gbi_custom_spec (include/gbi.h)."""
+    _fields_ = [
+        ("n_params", ctypes.c_int32),
+        ("param_names", ctypes.POINTER(ctypes.c_char_p)),
+        ("param_values", ctypes.POINTER(ctypes.c_double)),
+        ("radius_param", ctypes.c_int32),
+        ("n_globals", ctypes.c_int32),
+        ("global_names", ctypes.POINTER(ctypes.c_char_p)),
+        ("global_values", ctypes.POINTER(ctypes.c_double)),
+        ("n_integrals", ctypes.c_int32),
+        ("integral_names", ctypes.POINTER(ctypes.c_char_p)),
+        ("n_values", ctypes.c_int32),
+        ("value_names", ctypes.POINTER(ctypes.c_char_p)),
+        ("value_exprs", ctypes.POINTER(ctypes.c_char_p)),
+        ("n_terms", ctypes.c_int32),
+        ("term_exprs", ctypes.POINTER(ctypes.c_char_p)),
+        ("term_types", ctypes.POINTER(ctypes.c_int32)),
+    ]
+
+
+TERM_SINGLE, TERM_PAIR, TERM_PAIR_NO_EXCLUSIONS = 0, 1, 2
+
+
+def _strings(items):
+    arr = (ctypes.c_char_p * max(1, len(items)))(*[x.encode() for x in items])
+    return arr
+
+
 _lib = None
 
 
@@ -138,6 +167,10 @@ def load() -> ctypes.CDLL:
     L.gbi_update_parameters.argtypes = [vp, ctypes.c_int32, dp, dp]
     L.gbi_set_periodic_box.argtypes = [vp, ctypes.c_int32, dp]
     L.gbi_set_exclusions.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+    L.gbi_create_custom.argtypes = [ctypes.c_int32, ctypes.POINTER(Params), ctypes.POINTER(CustomSpec),
+                                    ctypes.POINTER(vp)]
+    L.gbi_set_custom_globals.argtypes = [vp, ctypes.c_int32, dp]
+    L.gbi_get_custom_values.argtypes = [vp, dp]
     L.gbi_set_positions.argtypes = [vp, vp, ctypes.c_int32]
     L.gbi_compute.argtypes = [vp, ctypes.c_uint32]
     L.gbi_synchronize.argtypes = [vp]
@@ -214,6 +247,49 @@ class Engine:
         self.n_systems = len(self.params)
         self.offsets = np.concatenate([[0], np.cumsum(self.n_atoms)]).astype(np.int64)
         self._h = handle
+
+    @classmethod
+    def custom(cls, params, param_names, param_values, radius_param, integrals, values, terms, globals_=None):
+        """A context for general CharmmGBMVForce expressions (gbi_create_custom).
+
+        param_values: N x len(param_names); integrals: names (1 or 2; orders from
+        params.orders); values: [(name, expression)]; terms: [(expression, TERM_*)];
+        globals_: {name: value}."""
+        eng = cls.__new__(cls)
+        eng._lib = load()
+        pv = np.ascontiguousarray(np.asarray(param_values, dtype=np.float64).reshape(-1, len(param_names)))
+        n = pv.shape[0]
+        globals_ = dict(globals_ or {})
+        gnames = list(globals_)
+        gvals = np.ascontiguousarray([float(globals_[k]) for k in gnames], dtype=np.float64)
+        ttypes = np.ascontiguousarray([int(t) for _, t in terms], dtype=np.int32)
+        keep = [_strings(list(param_names)), _strings(gnames), _strings(list(integrals)),
+                _strings([v for v, _ in values]), _strings([e for _, e in values]), _strings([e for e, _ in terms])]
+        spec = CustomSpec(len(param_names), keep[0], _dp(pv), int(radius_param), len(gnames), keep[1],
+                          _dp(gvals) if len(gnames) else None, len(integrals), keep[2], len(values), keep[3], keep[4],
+                          len(terms), keep[5], ttypes.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        handle = ctypes.c_void_p()
+        _check(eng._lib.gbi_create_custom(n, ctypes.byref(params), ctypes.byref(spec), ctypes.byref(handle)))
+        eng._h = handle
+        eng.n = n
+        eng.charge = np.zeros(n)
+        eng.radius = pv[:, radius_param].copy()
+        eng.params = [params]
+        eng.n_atoms = np.array([n], dtype=np.int32)
+        eng.n_systems = 1
+        eng.offsets = np.array([0, n], dtype=np.int64)
+        eng.n_values = len(values)
+        return eng
+
+    def custom_values(self) -> np.ndarray:
+        """Computed values of the last evaluation (n_values x N)."""
+        out = np.zeros((self.n_values, self.n))
+        _check(self._lib.gbi_get_custom_values(self._h, _dp(out)))
+        return out
+
+    def set_custom_globals(self, values) -> None:
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        _check(self._lib.gbi_set_custom_globals(self._h, int(v.shape[0]), _dp(v)))
 
     @classmethod
     def batch(cls, params_list, charges, radii):

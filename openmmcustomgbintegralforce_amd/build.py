@@ -17,7 +17,7 @@ LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libgbi.so")
 ARCH = os.environ.get("GBI_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["gbi_kernels.hip", "gbi_context.cpp", "quadrature.cpp"]
+SOURCES = ["gbi_kernels.hip", "gbi_context.cpp", "gbi_custom.cpp", "quadrature.cpp"]
 
 # Packed-FP32 instructions stay disabled: the compiler must not form v_pk_*_f32.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -89,7 +89,9 @@ def build(verbose: bool = False) -> str:
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    # hiprtc compiles the kernels of general expressions when a context is created
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + [
+        "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -16,9 +16,11 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
+#include "gbi_custom.h"
 #include "gbi_internal.h"
 #include "quadrature.h"
 
@@ -88,6 +90,7 @@ struct SysHost {
     double box[3] = {0.0, 0.0, 0.0};  // periodic box edges (gbi_set_periodic_box)
     bool box_set = false;
     std::vector<int32_t> excl;         // excluded pairs, local atom indices (gbi_set_exclusions)
+    bool custom = false;               // general expressions (gbi_create_custom)
     bool periodic() const { return P.nonbonded_method == GBI_CUTOFF_PERIODIC; }
     bool cutoff() const { return P.nonbonded_method != GBI_NO_CUTOFF; }
 };
@@ -124,6 +127,12 @@ struct gbi_ctx {
     DevBuf<int> excl_start, excl_col;
     DevBuf<unsigned long long> excl_mask;
     bool any_excl = false;
+
+    // general expressions (one system)
+    std::unique_ptr<gbi::CustomProgram> cprog;
+    gbi::CustomLaunch claunch;
+    DevBuf<double> c_prm, c_glob, c_vals, c_dvdi, c_esp, c_despdi, c_part;
+    DevBuf<int> c_ex_ptr, c_ex_idx;
     bool any_cut = false, any_trunc = false;
     DevBuf<float> colpart;
     int n_tiles = 0, max_splits = 1, n_colblocks = 0;
@@ -180,6 +189,11 @@ struct gbi_ctx {
         excl_start.release();
         excl_col.release();
         excl_mask.release();
+        DevBuf<double>* cb[] = {&c_prm, &c_glob, &c_vals, &c_dvdi, &c_esp, &c_despdi, &c_part};
+        for (auto* b : cb) b->release();
+        c_ex_ptr.release();
+        c_ex_idx.release();
+        cprog.reset();
         colpart.release();
         DevBuf<int>* ib[] = {&flags_all,      &atom_sys,     &cell_sys,       &bin_order,       &cell_count, &cell_fill,
                              &cell_start,     &cell_count_eff, &cell_atoms_tmp, &cell_atoms, &cell_pt_count,
@@ -199,7 +213,7 @@ struct gbi_ctx {
 
 namespace {
 
-gbi_status validate_params(const gbi_params& P, int N) {
+gbi_status validate_params(const gbi_params& P, int N, bool custom = false) {
     if (N <= 0) return fail(GBI_ERR_INVALID_ARGUMENT, "n_atoms must be positive");
     if (P.model != GBI_MODEL_GBSW && P.model != GBI_MODEL_GBMV)
         return fail(GBI_ERR_INVALID_ARGUMENT, "unknown model");
@@ -226,7 +240,7 @@ gbi_status validate_params(const gbi_params& P, int N) {
         if (P.orders[k] < 2)  // GBSWIntegral.cpp:92-95
             return fail(GBI_ERR_INVALID_ARGUMENT, "the order of an integral must be >= 2");
     const bool o47 = (P.orders[0] == 4 && P.orders[1] == 7) || (P.orders[0] == 7 && P.orders[1] == 4);
-    if (!o47) return fail(GBI_ERR_UNSUPPORTED, "the canonical Born map needs integrals of order 4 and 7");
+    if (!o47 && !custom) return fail(GBI_ERR_UNSUPPORTED, "the canonical Born map needs integrals of order 4 and 7");
     return GBI_OK;
 }
 
@@ -312,6 +326,7 @@ gbi::SysDesc make_desc(const gbi_ctx& c, const SysHost& s, long long cell_base, 
     d.cut_mode = !s.cutoff() ? 0 : (P.model == GBI_MODEL_GBSW ? 1 : 2);
     d.chain_trunc = (s.cutoff() && P.model == GBI_MODEL_GBSW) ? 1 : 0;
     d.blk_base = s.blk_base;
+    d.custom = s.custom ? 1 : 0;
     d.cut2 = s.cutoff() ? P.cutoff * P.cutoff : 0.0;
     d.cut2_lo = s.cutoff() ? (float)(d.cut2 * (1.0 - 1e-4)) : FLT_MAX;
     d.cut2_hi = s.cutoff() ? (float)(d.cut2 * (1.0 + 1e-4)) : FLT_MAX;
@@ -580,6 +595,34 @@ gbi_status build_exclusions(gbi_ctx& c) {
     return GBI_OK;
 }
 
+// Excluded pairs of a general-expression system as sorted partner lists (CSR, both directions).
+gbi_status build_custom_exclusions(gbi_ctx& c) {
+    const SysHost& s = c.sys[0];
+    std::vector<std::vector<int>> partners(s.N);
+    for (size_t k = 0; k + 1 < s.excl.size(); k += 2) {
+        partners[s.excl[k]].push_back(s.excl[k + 1]);
+        partners[s.excl[k + 1]].push_back(s.excl[k]);
+    }
+    std::vector<int> ptr(s.N + 1, 0), idx;
+    for (int a = 0; a < s.N; ++a) {
+        std::sort(partners[a].begin(), partners[a].end());
+        partners[a].erase(std::unique(partners[a].begin(), partners[a].end()), partners[a].end());
+        ptr[a + 1] = ptr[a] + (int)partners[a].size();
+        idx.insert(idx.end(), partners[a].begin(), partners[a].end());
+    }
+    c.drop_graphs();
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    HIP_TRY(c.c_ex_ptr.ensure(ptr.size()));
+    HIP_TRY(c.c_ex_idx.ensure(std::max<size_t>(1, idx.size())));
+    HIP_TRY(hipMemcpyAsync(c.c_ex_ptr.p, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice, c.stream));
+    if (!idx.empty())
+        HIP_TRY(hipMemcpyAsync(c.c_ex_idx.p, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    c.claunch.ex_ptr = c.c_ex_ptr.p;
+    c.claunch.ex_idx = c.c_ex_idx.p;
+    return GBI_OK;
+}
+
 // Grow the systems' capacities for the device positions (first call, new radii, overflow).
 gbi_status size_from_device(gbi_ctx& c, long long min_entries) {
     std::vector<double> host(3 * (size_t)c.N);
@@ -644,6 +687,7 @@ gbi::KParams make_kparams(gbi_ctx& c, bool adjoint) {
     k.blk_lo = c.blk_lo.p;
     k.blk_hi = c.blk_hi.p;
     k.pair_atom = c.pair_atom.p;
+    k.custom = c.cprog ? &c.claunch : nullptr;
     k.any_excl = c.any_excl ? 1 : 0;
     k.excl_start = c.excl_start.p;
     k.excl_col = c.excl_col.p;
@@ -745,6 +789,7 @@ gbi_status run(gbi_ctx& c, uint32_t flags) {
             ST_TRY(check_box(c, s, s.box));
         }
     if (adjoint && c.any_cut) return fail(GBI_ERR_UNSUPPORTED, "the adjoint is implemented for NoCutoff systems only");
+    if (adjoint && c.cprog) return fail(GBI_ERR_UNSUPPORTED, "the adjoint covers the canonical GB forms only");
     if (c.cell_total == 0) ST_TRY(size_from_device(c, 0));
     if (adjoint) ST_TRY(ensure_adjoint_buffers(c));
     gbi::KParams k = make_kparams(c, adjoint);
@@ -847,13 +892,13 @@ gbi_status apply_tight_grid(gbi_ctx& c, double g) {
 }
 
 gbi_status create_batch(int S, const int32_t* n_atoms, const double* charge, const double* radius,
-                        const gbi_params* params, gbi_ctx** out) {
+                        const gbi_params* params, gbi_ctx** out, bool custom = false) {
     if (!out || !n_atoms || !charge || !radius || !params) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     if (S <= 0) return fail(GBI_ERR_INVALID_ARGUMENT, "n_systems must be positive");
     long long Ntot = 0, NQtot = 0;
     for (int s = 0; s < S; ++s) {
-        ST_TRY(validate_params(params[s], n_atoms[s]));
+        ST_TRY(validate_params(params[s], n_atoms[s], custom));
         Ntot += n_atoms[s];
         NQtot += (long long)n_atoms[s] * params[s].n_radial * params[s].n_lebedev;
     }
@@ -1125,6 +1170,8 @@ gbi_status gbi_update_parameters(gbi_ctx* ctx, int32_t n_atoms, const double* ch
     if (!ctx || !charge || !radius) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
     if (n_atoms != ctx->N)  // CharmmReferenceKernels.cpp:373-374
         return fail(GBI_ERR_PARTICLE_COUNT, "updateParametersInContext: The number of particles has changed");
+    if (ctx->cprog)
+        return fail(GBI_ERR_UNSUPPORTED, "a general-expression context takes its parameters at gbi_create_custom");
     ST_TRY(check_atoms(n_atoms, charge, radius));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->charge.assign(charge, charge + n_atoms);
@@ -1166,7 +1213,119 @@ gbi_status gbi_set_exclusions(gbi_ctx* ctx, int32_t system, int32_t n_pairs, con
         if (pairs[k] < 0 || pairs[k + 1] < 0 || pairs[k] >= s.N || pairs[k + 1] >= s.N || pairs[k] == pairs[k + 1])
             return fail(GBI_ERR_INVALID_ARGUMENT, "excluded pair out of range or of an atom with itself");
     s.excl.assign(pairs, pairs + 2 * (size_t)n_pairs);
+    if (ctx->cprog) return build_custom_exclusions(*ctx);
     return build_exclusions(*ctx);
+}
+
+gbi_status gbi_create_custom(int32_t n_atoms, const gbi_params* params, const gbi_custom_spec* spec, gbi_ctx** out) {
+    if (!params || !spec || !out) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (params->model != GBI_MODEL_GBMV) return fail(GBI_ERR_INVALID_ARGUMENT, "general expressions need GBI_MODEL_GBMV");
+    if (n_atoms <= 0) return fail(GBI_ERR_INVALID_ARGUMENT, "n_atoms must be positive");
+    if (spec->n_params < 1 || !spec->param_names || !spec->param_values || spec->radius_param < 0 ||
+        spec->radius_param >= spec->n_params)
+        return fail(GBI_ERR_INVALID_ARGUMENT, "GBSWIntegral: the per perticle parameter 'radius' must be defined");
+    if (spec->n_integrals < 1 || spec->n_integrals > gbi::kMaxCustomIntegrals || !spec->integral_names)
+        return fail(GBI_ERR_UNSUPPORTED, "general expressions take one or two integrals");
+    if (spec->n_values < 0 || spec->n_values > gbi::kMaxCustomValues || spec->n_terms < 1 ||
+        (spec->n_values > 0 && (!spec->value_names || !spec->value_exprs)) || !spec->term_exprs || !spec->term_types ||
+        (spec->n_globals > 0 && (!spec->global_names || !spec->global_values)) || spec->n_globals < 0)
+        return fail(GBI_ERR_INVALID_ARGUMENT, "incomplete expression specification");
+    gbi::CustomSpec cs;
+    for (int k = 0; k < spec->n_params; ++k) cs.params.push_back(spec->param_names[k] ? spec->param_names[k] : "");
+    for (int k = 0; k < spec->n_globals; ++k) cs.globals.push_back(spec->global_names[k] ? spec->global_names[k] : "");
+    for (int k = 0; k < spec->n_integrals; ++k)
+        cs.integrals.push_back(spec->integral_names[k] ? spec->integral_names[k] : "");
+    for (int k = 0; k < spec->n_values; ++k) {
+        cs.value_names.push_back(spec->value_names[k] ? spec->value_names[k] : "");
+        cs.value_exprs.push_back(spec->value_exprs[k] ? spec->value_exprs[k] : "");
+    }
+    for (int k = 0; k < spec->n_terms; ++k) {
+        cs.term_exprs.push_back(spec->term_exprs[k] ? spec->term_exprs[k] : "");
+        cs.term_types.push_back(spec->term_types[k]);
+    }
+    // the expressions are checked before any device work
+    auto prog = std::make_unique<gbi::CustomProgram>();
+    try {
+        gbi::generate_custom_source(cs, *prog);
+    } catch (const std::exception& e) {
+        return fail(GBI_ERR_INVALID_ARGUMENT, e.what());
+    }
+    gbi_params P = *params;
+    if (spec->n_integrals == 1) P.orders[1] = P.orders[0];
+    std::vector<double> radius(n_atoms), charge(n_atoms, 0.0);
+    for (int a = 0; a < n_atoms; ++a) radius[a] = spec->param_values[(size_t)a * spec->n_params + spec->radius_param];
+    for (size_t k = 0; k < (size_t)n_atoms * spec->n_params; ++k)
+        if (!std::isfinite(spec->param_values[k])) return fail(GBI_ERR_NONFINITE, "non-finite per-particle parameter");
+    gbi_ctx* c = nullptr;
+    const int32_t n = n_atoms;
+    ST_TRY(create_batch(1, &n, charge.data(), radius.data(), &P, &c, true));
+    std::unique_ptr<gbi_ctx> guard(c);
+    c->sys[0].custom = true;
+    try {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) return fail(GBI_ERR_HIP, "hipGetDeviceProperties");
+        gbi::compile_custom_program(prop.gcnArchName, *prog);
+    } catch (const std::exception& e) {
+        return fail(GBI_ERR_INVALID_ARGUMENT, e.what());
+    }
+    const size_t N = n_atoms;
+    const int NV = prog->n_values, NPL = prog->n_planes;
+    HIP_TRY(c->c_prm.ensure(N * spec->n_params));
+    HIP_TRY(c->c_glob.ensure(std::max(1, spec->n_globals)));
+    HIP_TRY(c->c_vals.ensure(std::max<size_t>(1, NV * N)));
+    HIP_TRY(c->c_dvdi.ensure(std::max<size_t>(1, 2 * NV * N)));
+    HIP_TRY(c->c_esp.ensure(N));
+    HIP_TRY(c->c_despdi.ensure(2 * N));
+    HIP_TRY(c->c_part.ensure((size_t)gbi::kCustomChunks * NPL * N));
+    HIP_TRY(hipMemcpyAsync(c->c_prm.p, spec->param_values, sizeof(double) * N * spec->n_params, hipMemcpyHostToDevice,
+                           c->stream));
+    if (spec->n_globals > 0)
+        HIP_TRY(hipMemcpyAsync(c->c_glob.p, spec->global_values, sizeof(double) * spec->n_globals,
+                               hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    gbi::CustomLaunch& L = c->claunch;
+    L.values = prog->values;
+    L.pairs = prog->pairs;
+    L.chain = prog->chain;
+    L.N = n_atoms;
+    L.chunks = gbi::kCustomChunks;
+    L.chunk = (n_atoms + gbi::kCustomChunks - 1) / gbi::kCustomChunks;
+    L.method = P.nonbonded_method;
+    L.cut2 = P.nonbonded_method != GBI_NO_CUTOFF ? P.cutoff * P.cutoff : 0.0;
+    L.box = c->d_box.p;
+    L.prm = c->c_prm.p;
+    L.glob = c->c_glob.p;
+    L.vals = c->c_vals.p;
+    L.dvdi = c->c_dvdi.p;
+    L.esp = c->c_esp.p;
+    L.despdi = c->c_despdi.p;
+    L.part = c->c_part.p;
+    c->cprog = std::move(prog);
+    ST_TRY(build_custom_exclusions(*c));
+    *out = guard.release();
+    return GBI_OK;
+}
+
+gbi_status gbi_set_custom_globals(gbi_ctx* ctx, int32_t n_globals, const double* values) {
+    if (!ctx || (n_globals > 0 && !values)) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->cprog) return fail(GBI_ERR_INVALID_ARGUMENT, "not a general-expression context");
+    if (n_globals != ctx->cprog->n_globals) return fail(GBI_ERR_INVALID_ARGUMENT, "wrong number of global parameters");
+    if (n_globals == 0) return GBI_OK;
+    HIP_TRY(hipMemcpyAsync(ctx->c_glob.p, values, sizeof(double) * n_globals, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return GBI_OK;
+}
+
+gbi_status gbi_get_custom_values(gbi_ctx* ctx, double* values) {
+    if (!ctx || !values) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->cprog) return fail(GBI_ERR_INVALID_ARGUMENT, "not a general-expression context");
+    if (!ctx->computed) return fail(GBI_ERR_INVALID_ARGUMENT, "no evaluation yet");
+    const size_t n = (size_t)ctx->cprog->n_values * ctx->N;
+    if (n == 0) return GBI_OK;
+    HIP_TRY(hipMemcpyAsync(values, ctx->c_vals.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return GBI_OK;
 }
 
 gbi_status gbi_set_positions(gbi_ctx* ctx, const double* pos, int32_t pos_on_device) {

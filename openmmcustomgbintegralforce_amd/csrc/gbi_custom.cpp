@@ -1,0 +1,461 @@
+// General CharmmGBMVForce expressions on the device: expression parser, HIP code generation
+// over forward-mode dual numbers, and hiprtc compilation.
+//
+// Reference counterparts: the Reference platform evaluates the expressions with Lepton and
+// symbolic derivatives (CharmmReferenceGBMV.cpp:150-204 calculateIxn, :206-227 values,
+// :229-266 single-particle terms, :268-348 pair terms, :350-446 chain rule); the CUDA
+// platform turns them into generated CUDA source (CharmmCudaKernels.cpp:228-827).  Here one
+// evaluation of an expression over dual numbers gives its value and every derivative the
+// chain rule needs, so no symbolic derivative is formed.
+#include "gbi_custom.h"
+
+#include <hip/hiprtc.h>
+
+#include <cctype>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+
+namespace gbi {
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Expression parser: + - * / ^ (right associative, above unary minus), unary minus,
+// parentheses, numbers, names and the functions below.  Emits HIP source over the dual type D.
+// ---------------------------------------------------------------------------
+
+struct Token {
+    enum Kind { NUM, NAME, OP, END } kind;
+    std::string text;
+};
+
+std::vector<Token> tokenize(const std::string& s) {
+    std::vector<Token> out;
+    size_t i = 0;
+    while (i < s.size()) {
+        const char c = s[i];
+        if (std::isspace((unsigned char)c)) {
+            ++i;
+        } else if (std::isdigit((unsigned char)c) || (c == '.' && i + 1 < s.size() && std::isdigit((unsigned char)s[i + 1]))) {
+            size_t j = i;
+            while (j < s.size() && (std::isdigit((unsigned char)s[j]) || s[j] == '.')) ++j;
+            if (j < s.size() && (s[j] == 'e' || s[j] == 'E')) {
+                size_t k = j + 1;
+                if (k < s.size() && (s[k] == '+' || s[k] == '-')) ++k;
+                if (k < s.size() && std::isdigit((unsigned char)s[k])) {
+                    j = k;
+                    while (j < s.size() && std::isdigit((unsigned char)s[j])) ++j;
+                }
+            }
+            out.push_back({Token::NUM, s.substr(i, j - i)});
+            i = j;
+        } else if (std::isalpha((unsigned char)c) || c == '_') {
+            size_t j = i;
+            while (j < s.size() && (std::isalnum((unsigned char)s[j]) || s[j] == '_')) ++j;
+            out.push_back({Token::NAME, s.substr(i, j - i)});
+            i = j;
+        } else if (std::string("+-*/^(),").find(c) != std::string::npos) {
+            out.push_back({Token::OP, std::string(1, c)});
+            ++i;
+        } else {
+            throw std::runtime_error(std::string("unexpected character '") + c + "' in expression \"" + s + "\"");
+        }
+    }
+    out.push_back({Token::END, ""});
+    return out;
+}
+
+// Functions: name -> (arity, device function over D)
+const std::map<std::string, std::pair<int, const char*>>& functions() {
+    static const std::map<std::string, std::pair<int, const char*>> f = {
+        {"sqrt", {1, "d_sqrt"}},   {"exp", {1, "d_exp"}},     {"log", {1, "d_log"}},   {"sin", {1, "d_sin"}},
+        {"cos", {1, "d_cos"}},     {"tan", {1, "d_tan"}},     {"asin", {1, "d_asin"}}, {"acos", {1, "d_acos"}},
+        {"atan", {1, "d_atan"}},   {"sinh", {1, "d_sinh"}},   {"cosh", {1, "d_cosh"}}, {"tanh", {1, "d_tanh"}},
+        {"erf", {1, "d_erf"}},     {"erfc", {1, "d_erfc"}},   {"abs", {1, "d_abs"}},   {"step", {1, "d_step"}},
+        {"delta", {1, "d_delta"}}, {"square", {1, "d_square"}}, {"cube", {1, "d_cube"}}, {"recip", {1, "d_recip"}},
+        {"min", {2, "d_min"}},     {"max", {2, "d_max"}},     {"select", {3, "d_select"}}};
+    return f;
+}
+
+class Parser {
+public:
+    Parser(const std::string& expr, const std::map<std::string, std::string>& vars)
+        : src_(expr), toks_(tokenize(expr)), vars_(vars) {}
+
+    std::string parse() {
+        std::string e = sum();
+        if (toks_[pos_].kind != Token::END) fail("unexpected '" + toks_[pos_].text + "'");
+        return e;
+    }
+
+private:
+    [[noreturn]] void fail(const std::string& what) const {
+        throw std::runtime_error(what + " in expression \"" + src_ + "\"");
+    }
+    bool is_op(const char* op) const { return toks_[pos_].kind == Token::OP && toks_[pos_].text == op; }
+    void expect(const char* op) {
+        if (!is_op(op)) fail(std::string("expected '") + op + "'");
+        ++pos_;
+    }
+    std::string sum() {
+        std::string a = product();
+        while (is_op("+") || is_op("-")) {
+            const std::string op = toks_[pos_++].text;
+            a = "(" + a + " " + op + " " + product() + ")";
+        }
+        return a;
+    }
+    std::string product() {
+        std::string a = unary();
+        while (is_op("*") || is_op("/")) {
+            const std::string op = toks_[pos_++].text;
+            a = "(" + a + " " + op + " " + unary() + ")";
+        }
+        return a;
+    }
+    std::string unary() {
+        if (is_op("-")) {
+            ++pos_;
+            return "(-" + unary() + ")";
+        }
+        if (is_op("+")) {
+            ++pos_;
+            return unary();
+        }
+        return power();
+    }
+    std::string power() {
+        std::string a = primary();
+        if (is_op("^")) {
+            ++pos_;
+            // a numeric exponent (optionally negative) differentiates without log(a)
+            if (toks_[pos_].kind == Token::NUM ||
+                (is_op("-") && toks_[pos_ + 1].kind == Token::NUM &&
+                 !(toks_[pos_ + 2].kind == Token::OP && toks_[pos_ + 2].text == "^"))) {
+                std::string num;
+                if (is_op("-")) {
+                    num = "-";
+                    ++pos_;
+                }
+                num += toks_[pos_++].text;
+                if (is_op("^")) {  // a^2^3 = a^(2^3): fall back to the general form
+                    ++pos_;
+                    const std::string rest = unary();
+                    return "d_pow(" + a + ", d_pow(D::c(" + num + "), " + rest + "))";
+                }
+                return "d_powc(" + a + ", " + num + ")";
+            }
+            return "d_pow(" + a + ", " + unary() + ")";
+        }
+        return a;
+    }
+    std::string primary() {
+        const Token t = toks_[pos_];
+        if (t.kind == Token::NUM) {
+            ++pos_;
+            return "D::c(" + t.text + ")";
+        }
+        if (t.kind == Token::NAME) {
+            ++pos_;
+            if (is_op("(")) {
+                auto it = functions().find(t.text);
+                if (it == functions().end()) fail("unknown function '" + t.text + "'");
+                ++pos_;
+                std::vector<std::string> args;
+                if (!is_op(")")) {
+                    args.push_back(sum());
+                    while (is_op(",")) {
+                        ++pos_;
+                        args.push_back(sum());
+                    }
+                }
+                expect(")");
+                if ((int)args.size() != it->second.first)
+                    fail("function '" + t.text + "' takes " + std::to_string(it->second.first) + " argument(s)");
+                std::string call = std::string(it->second.second) + "(";
+                for (size_t k = 0; k < args.size(); ++k) call += (k ? ", " : "") + args[k];
+                return call + ")";
+            }
+            auto v = vars_.find(t.text);
+            if (v == vars_.end()) fail("unknown variable '" + t.text + "'");
+            return v->second;
+        }
+        if (is_op("(")) {
+            ++pos_;
+            std::string e = sum();
+            expect(")");
+            return e;
+        }
+        fail(t.kind == Token::END ? std::string("unexpected end") : "unexpected '" + t.text + "'");
+    }
+
+    std::string src_;
+    std::vector<Token> toks_;
+    std::map<std::string, std::string> vars_;
+    size_t pos_ = 0;
+};
+
+bool valid_name(const std::string& n) {
+    if (n.empty() || !(std::isalpha((unsigned char)n[0]) || n[0] == '_')) return false;
+    for (char c : n)
+        if (!(std::isalnum((unsigned char)c) || c == '_')) return false;
+    return functions().find(n) == functions().end();
+}
+
+// The dual-number type and its functions (device source, prepended to the kernels).
+const char* kDualSource = R"(
+#pragma clang fp contract(off)
+template <int M> struct Du {
+    double v;
+    double d[M];
+    static __device__ inline Du c(double x) { Du r; r.v = x; for (int k = 0; k < M; ++k) r.d[k] = 0.0; return r; }
+    static __device__ inline Du var(double x, int slot) { Du r = c(x); r.d[slot] = 1.0; return r; }
+};
+template <int M> __device__ inline Du<M> d_chain(const Du<M>& a, double f, double df) {
+    Du<M> r; r.v = f; for (int k = 0; k < M; ++k) r.d[k] = df * a.d[k]; return r;
+}
+template <int M> __device__ inline Du<M> operator+(const Du<M>& a, const Du<M>& b) {
+    Du<M> r; r.v = a.v + b.v; for (int k = 0; k < M; ++k) r.d[k] = a.d[k] + b.d[k]; return r;
+}
+template <int M> __device__ inline Du<M> operator-(const Du<M>& a, const Du<M>& b) {
+    Du<M> r; r.v = a.v - b.v; for (int k = 0; k < M; ++k) r.d[k] = a.d[k] - b.d[k]; return r;
+}
+template <int M> __device__ inline Du<M> operator-(const Du<M>& a) {
+    Du<M> r; r.v = -a.v; for (int k = 0; k < M; ++k) r.d[k] = -a.d[k]; return r;
+}
+template <int M> __device__ inline Du<M> operator*(const Du<M>& a, const Du<M>& b) {
+    Du<M> r; r.v = a.v * b.v; for (int k = 0; k < M; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k]; return r;
+}
+template <int M> __device__ inline Du<M> operator/(const Du<M>& a, const Du<M>& b) {
+    Du<M> r; r.v = a.v / b.v; const double ib = 1.0 / b.v;
+    for (int k = 0; k < M; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib; return r;
+}
+template <int M> __device__ inline Du<M> d_sqrt(const Du<M>& a) { const double f = sqrt(a.v); return d_chain(a, f, 0.5 / f); }
+template <int M> __device__ inline Du<M> d_exp(const Du<M>& a) { const double f = exp(a.v); return d_chain(a, f, f); }
+template <int M> __device__ inline Du<M> d_log(const Du<M>& a) { return d_chain(a, log(a.v), 1.0 / a.v); }
+template <int M> __device__ inline Du<M> d_sin(const Du<M>& a) { return d_chain(a, sin(a.v), cos(a.v)); }
+template <int M> __device__ inline Du<M> d_cos(const Du<M>& a) { return d_chain(a, cos(a.v), -sin(a.v)); }
+template <int M> __device__ inline Du<M> d_tan(const Du<M>& a) { const double f = tan(a.v); return d_chain(a, f, 1.0 + f * f); }
+template <int M> __device__ inline Du<M> d_asin(const Du<M>& a) { return d_chain(a, asin(a.v), 1.0 / sqrt(1.0 - a.v * a.v)); }
+template <int M> __device__ inline Du<M> d_acos(const Du<M>& a) { return d_chain(a, acos(a.v), -1.0 / sqrt(1.0 - a.v * a.v)); }
+template <int M> __device__ inline Du<M> d_atan(const Du<M>& a) { return d_chain(a, atan(a.v), 1.0 / (1.0 + a.v * a.v)); }
+template <int M> __device__ inline Du<M> d_sinh(const Du<M>& a) { return d_chain(a, sinh(a.v), cosh(a.v)); }
+template <int M> __device__ inline Du<M> d_cosh(const Du<M>& a) { return d_chain(a, cosh(a.v), sinh(a.v)); }
+template <int M> __device__ inline Du<M> d_tanh(const Du<M>& a) { const double f = tanh(a.v); return d_chain(a, f, 1.0 - f * f); }
+template <int M> __device__ inline Du<M> d_erf(const Du<M>& a) {
+    return d_chain(a, erf(a.v), 1.1283791670955126 * exp(-a.v * a.v));
+}
+template <int M> __device__ inline Du<M> d_erfc(const Du<M>& a) {
+    return d_chain(a, erfc(a.v), -1.1283791670955126 * exp(-a.v * a.v));
+}
+template <int M> __device__ inline Du<M> d_abs(const Du<M>& a) { return d_chain(a, fabs(a.v), a.v < 0.0 ? -1.0 : 1.0); }
+template <int M> __device__ inline Du<M> d_step(const Du<M>& a) { return Du<M>::c(a.v >= 0.0 ? 1.0 : 0.0); }
+template <int M> __device__ inline Du<M> d_delta(const Du<M>& a) { return Du<M>::c(a.v == 0.0 ? 1.0 : 0.0); }
+template <int M> __device__ inline Du<M> d_square(const Du<M>& a) { return d_chain(a, a.v * a.v, 2.0 * a.v); }
+template <int M> __device__ inline Du<M> d_cube(const Du<M>& a) { return d_chain(a, a.v * a.v * a.v, 3.0 * a.v * a.v); }
+template <int M> __device__ inline Du<M> d_recip(const Du<M>& a) { return d_chain(a, 1.0 / a.v, -1.0 / (a.v * a.v)); }
+template <int M> __device__ inline Du<M> d_min(const Du<M>& a, const Du<M>& b) { return b.v < a.v ? b : a; }
+template <int M> __device__ inline Du<M> d_max(const Du<M>& a, const Du<M>& b) { return b.v > a.v ? b : a; }
+template <int M> __device__ inline Du<M> d_select(const Du<M>& x, const Du<M>& a, const Du<M>& b) { return x.v != 0.0 ? a : b; }
+template <int M> __device__ inline Du<M> d_powc(const Du<M>& a, double e) {
+    return d_chain(a, pow(a.v, e), e == 0.0 ? 0.0 : e * pow(a.v, e - 1.0));
+}
+template <int M> __device__ inline Du<M> d_pow(const Du<M>& a, const Du<M>& b) {
+    const double f = pow(a.v, b.v);
+    const double da = b.v == 0.0 ? 0.0 : b.v * pow(a.v, b.v - 1.0);
+    Du<M> r; r.v = f;
+    for (int k = 0; k < M; ++k) r.d[k] = da * a.d[k] + (b.d[k] != 0.0 ? f * log(a.v) * b.d[k] : 0.0);
+    return r;
+}
+)";
+
+void check_hiprtc(hiprtcResult r, const char* what) {
+    if (r != HIPRTC_SUCCESS) throw std::runtime_error(std::string(what) + ": " + hiprtcGetErrorString(r));
+}
+
+}  // namespace
+
+CustomProgram::~CustomProgram() {
+    if (module) (void)hipModuleUnload(module);
+}
+
+std::string translate_expression(const std::string& expr, const CustomSpec& spec, int context, int index) {
+    std::map<std::string, std::string> vars;
+    for (size_t k = 0; k < spec.globals.size(); ++k) vars[spec.globals[k]] = "G" + std::to_string(k);
+    if (context == 2) {
+        vars["r"] = "r_";
+        for (size_t k = 0; k < spec.params.size(); ++k) {
+            vars[spec.params[k] + "1"] = "P1_" + std::to_string(k);
+            vars[spec.params[k] + "2"] = "P2_" + std::to_string(k);
+        }
+        for (size_t k = 0; k < spec.value_names.size(); ++k) {
+            vars[spec.value_names[k] + "1"] = "V1_" + std::to_string(k);
+            vars[spec.value_names[k] + "2"] = "V2_" + std::to_string(k);
+        }
+    } else {
+        for (size_t k = 0; k < spec.params.size(); ++k) vars[spec.params[k]] = "P_" + std::to_string(k);
+        for (size_t k = 0; k < spec.integrals.size(); ++k) vars[spec.integrals[k]] = "I_" + std::to_string(k);
+        const int nv = context == 0 ? index : (int)spec.value_names.size();
+        for (int k = 0; k < nv; ++k) vars[spec.value_names[k]] = "V_" + std::to_string(k);
+    }
+    return Parser(expr, vars).parse();
+}
+
+void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
+    const int NP = (int)spec.params.size(), NG = (int)spec.globals.size(), NI = (int)spec.integrals.size();
+    const int NV = (int)spec.value_names.size();
+    if (NI < 1 || NI > kMaxCustomIntegrals) throw std::runtime_error("one or two integrals are supported");
+    if (NV > kMaxCustomValues) throw std::runtime_error("at most 4 computed values are supported");
+    if (spec.value_exprs.size() != spec.value_names.size() || spec.term_exprs.size() != spec.term_types.size())
+        throw std::runtime_error("inconsistent expression lists");
+    std::map<std::string, int> seen;
+    auto name_ok = [&](const std::string& n, const char* what) {
+        if (!valid_name(n)) throw std::runtime_error(std::string("invalid ") + what + " name '" + n + "'");
+        if (n == "r") throw std::runtime_error("'r' is reserved for the pair distance");
+        if (seen.count(n)) throw std::runtime_error("name '" + n + "' defined twice");
+        seen[n] = 1;
+    };
+    for (const auto& n : spec.params) name_ok(n, "parameter");
+    for (const auto& n : spec.globals) name_ok(n, "global parameter");
+    for (const auto& n : spec.integrals) name_ok(n, "integral");
+    for (const auto& n : spec.value_names) name_ok(n, "computed value");
+    for (int t : spec.term_types)
+        if (t < TERM_SINGLE || t > TERM_PAIR_NO_EXCL) throw std::runtime_error("unknown energy term type");
+
+    std::ostringstream s;
+    s << kDualSource;
+    s << "#define NP " << NP << "\n#define NG " << (NG > 0 ? NG : 1) << "\n#define NV " << NV << "\n";
+    const int NPL = 4 + NV;
+    s << "#define NPL " << NPL << "\n";
+
+    // ---- per atom: computed values and single-particle terms, duals over the integrals ----
+    s << "extern \"C\" __global__ void cg_values(int N, const double* __restrict__ prm, const double* __restrict__ "
+         "integ, const double* __restrict__ glob, double* __restrict__ vals, double* __restrict__ dvdi, double* "
+         "__restrict__ esp, double* __restrict__ despdi) {\n"
+         "    const int i = blockIdx.x * blockDim.x + threadIdx.x;\n    if (i >= N) return;\n"
+         "    typedef Du<2> D;\n";
+    for (int k = 0; k < NG; ++k) s << "    const D G" << k << " = D::c(glob[" << k << "]);\n";
+    for (int k = 0; k < NP; ++k) s << "    const D P_" << k << " = D::c(prm[(size_t)i * NP + " << k << "]);\n";
+    for (int k = 0; k < NI; ++k) s << "    const D I_" << k << " = D::var(integ[" << k << " * N + i], " << k << ");\n";
+    for (int k = 0; k < NV; ++k) {
+        s << "    const D V_" << k << " = " << translate_expression(spec.value_exprs[k], spec, 0, k) << ";\n";
+        s << "    vals[" << k << " * N + i] = V_" << k << ".v;\n";
+        s << "    dvdi[(" << k << " * 2 + 0) * N + i] = V_" << k << ".d[0];\n";
+        s << "    dvdi[(" << k << " * 2 + 1) * N + i] = V_" << k << ".d[1];\n";
+    }
+    s << "    D E_ = D::c(0.0);\n";
+    for (size_t t = 0; t < spec.term_exprs.size(); ++t)
+        if (spec.term_types[t] == TERM_SINGLE)
+            s << "    E_ = E_ + (" << translate_expression(spec.term_exprs[t], spec, 1, 0) << ");\n";
+    s << "    esp[i] = E_.v;\n    despdi[i] = E_.d[0];\n    despdi[N + i] = E_.d[1];\n}\n";
+
+    // ---- pairs: each thread takes its atom's side of every pair of its column chunk ----
+    // (atom1 < atom2 as the reference orders a pair, CharmmReferenceGBMV.cpp:279-286;
+    // the energy is counted by atom1's thread)
+    std::string pair_excl, pair_noexcl;
+    for (size_t t = 0; t < spec.term_exprs.size(); ++t) {
+        if (spec.term_types[t] == TERM_SINGLE) continue;
+        const std::string e = "    E_ = E_ + (" + translate_expression(spec.term_exprs[t], spec, 2, 0) + ");\n";
+        (spec.term_types[t] == TERM_PAIR ? pair_excl : pair_noexcl) += e;
+        out.any_pair = true;
+    }
+    s << "extern \"C\" __global__ void cg_pairs(int N, int chunk, const double* __restrict__ pos, const double* "
+         "__restrict__ prm, const double* __restrict__ vals, const double* __restrict__ glob, int method, double "
+         "cut2, const double* __restrict__ box, const int* __restrict__ ex_ptr, const int* __restrict__ ex_idx, "
+         "double* __restrict__ part) {\n"
+         "    const int i = blockIdx.x * blockDim.x + threadIdx.x;\n    if (i >= N) return;\n"
+         "    const int j0 = blockIdx.y * chunk, j1 = min(N, j0 + chunk);\n"
+         "    typedef Du<1 + 2 * NV> D;\n";
+    for (int k = 0; k < NG; ++k) s << "    const D G" << k << " = D::c(glob[" << k << "]);\n";
+    s << "    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];\n"
+         "    double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;\n"
+         "    double dv[NV > 0 ? NV : 1];\n    for (int k = 0; k < NV; ++k) dv[k] = 0.0;\n"
+         "    int ep = ex_ptr[i];\n    const int ee = ex_ptr[i + 1];\n"
+         "    for (int j = j0; j < j1; ++j) {\n"
+         "        while (ep < ee && ex_idx[ep] < j) ++ep;\n"
+         "        const bool excluded = ep < ee && ex_idx[ep] == j;\n"
+         "        if (j == i) continue;\n"
+         "        double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;\n"
+         "        if (method == 2) {\n"
+         "            dx -= box[0] * floor(dx / box[0] + 0.5);\n"
+         "            dy -= box[1] * floor(dy / box[1] + 0.5);\n"
+         "            dz -= box[2] * floor(dz / box[2] + 0.5);\n"
+         "        }\n"
+         "        const double r2 = dx * dx + dy * dy + dz * dz;\n"
+         "        if (method != 0 && !(r2 < cut2)) continue;\n"
+         "        const int a1 = i < j ? i : j, a2 = i < j ? j : i;\n"
+         "        const double rr = sqrt(r2);\n"
+         "        const D r_ = D::var(rr, 0);\n";
+    for (int k = 0; k < NP; ++k) {
+        s << "        const D P1_" << k << " = D::c(prm[(size_t)a1 * NP + " << k << "]);\n";
+        s << "        const D P2_" << k << " = D::c(prm[(size_t)a2 * NP + " << k << "]);\n";
+    }
+    for (int k = 0; k < NV; ++k) {
+        s << "        const D V1_" << k << " = D::var(vals[" << k << " * N + a1], " << 1 + k << ");\n";
+        s << "        const D V2_" << k << " = D::var(vals[" << k << " * N + a2], " << 1 + NV + k << ");\n";
+    }
+    s << "        D E_ = D::c(0.0);\n" << pair_noexcl << "        if (!excluded) {\n" << pair_excl << "        }\n"
+      << "        if (i < j) e += E_.v;\n"
+         "        const double g = E_.d[0] / rr;\n"
+         "        fx += g * dx;\n        fy += g * dy;\n        fz += g * dz;\n"
+         "        for (int k = 0; k < NV; ++k) dv[k] += i < j ? E_.d[1 + k] : E_.d[1 + NV + k];\n"
+         "    }\n"
+         "    double* o = part + (size_t)blockIdx.y * NPL * N;\n"
+         "    o[i] = fx;\n    o[N + i] = fy;\n    o[2 * N + i] = fz;\n    o[3 * N + i] = e;\n"
+         "    for (int k = 0; k < NV; ++k) o[(4 + k) * N + i] = dv[k];\n}\n";
+
+    // ---- per atom: pair sums in chunk order, energy, pair forces, dE/dI ----
+    s << "extern \"C\" __global__ void cg_chain(int N, int chunks, const double* __restrict__ part, const double* "
+         "__restrict__ dvdi, const double* __restrict__ esp, const double* __restrict__ despdi, double* __restrict__ "
+         "atom_energy, long long* __restrict__ force_fix, double* __restrict__ dedi) {\n"
+         "    const int i = blockIdx.x * blockDim.x + threadIdx.x;\n    if (i >= N) return;\n"
+         "    double a[NPL];\n    for (int p = 0; p < NPL; ++p) a[p] = 0.0;\n"
+         "    for (int c = 0; c < chunks; ++c)\n"
+         "        for (int p = 0; p < NPL; ++p) a[p] += part[((size_t)c * NPL + p) * N + i];\n"
+         "    atom_energy[i] = esp[i] + a[3];\n"
+         "    force_fix[i] = (long long)llrint(a[0] * 4294967296.0);\n"
+         "    force_fix[N + i] = (long long)llrint(a[1] * 4294967296.0);\n"
+         "    force_fix[2 * N + i] = (long long)llrint(a[2] * 4294967296.0);\n"
+         "    for (int m = 0; m < 2; ++m) {\n"
+         "        double g = despdi[m * N + i];\n"
+         "        for (int k = 0; k < NV; ++k) g += a[4 + k] * dvdi[(k * 2 + m) * N + i];\n"
+         "        dedi[m * N + i] = g;\n    }\n}\n";
+    out.source = s.str();
+    out.n_params = NP;
+    out.n_globals = NG;
+    out.n_values = NV;
+    out.n_planes = NPL;
+}
+
+void compile_custom_program(const std::string& arch, CustomProgram& out) {
+    hiprtcProgram prog;
+    check_hiprtc(hiprtcCreateProgram(&prog, out.source.c_str(), "gbi_custom.hip", 0, nullptr, nullptr),
+                 "hiprtcCreateProgram");
+    const std::string arch_opt = "--offload-arch=" + arch;
+    // packed FP32 stays disabled, as for the library's own kernels
+    const char* opts[] = {arch_opt.c_str(), "-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    const hiprtcResult cr = hiprtcCompileProgram(prog, 6, opts);
+    if (cr != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(prog, &n);
+        std::string log(n, '\0');
+        if (n) hiprtcGetProgramLog(prog, &log[0]);
+        hiprtcDestroyProgram(&prog);
+        throw std::runtime_error("generated kernels did not compile: " + log);
+    }
+    size_t size = 0;
+    check_hiprtc(hiprtcGetCodeSize(prog, &size), "hiprtcGetCodeSize");
+    std::vector<char> code(size);
+    check_hiprtc(hiprtcGetCode(prog, code.data()), "hiprtcGetCode");
+    hiprtcDestroyProgram(&prog);
+    if (hipModuleLoadData(&out.module, code.data()) != hipSuccess) throw std::runtime_error("hipModuleLoadData failed");
+    if (hipModuleGetFunction(&out.values, out.module, "cg_values") != hipSuccess ||
+        hipModuleGetFunction(&out.pairs, out.module, "cg_pairs") != hipSuccess ||
+        hipModuleGetFunction(&out.chain, out.module, "cg_chain") != hipSuccess)
+        throw std::runtime_error("generated kernels not found in the module");
+}
+
+}  // namespace gbi

@@ -1,0 +1,63 @@
+// General CharmmGBMVForce expressions: computed values, single-particle and pair energy
+// terms given as expression strings (the Lepton syntax of CharmmGBMVForce::addComputedValue /
+// addEnergyTerm).  The expressions are translated to HIP source in which every quantity is a
+// forward-mode dual number, so one evaluation yields the value and the derivatives the chain
+// rule needs; the source is compiled with hiprtc for the device when the context is created.
+// Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+namespace gbi {
+
+constexpr int kMaxCustomValues = 4;
+constexpr int kMaxCustomIntegrals = 2;
+constexpr int kCustomChunks = 32;  // column chunks of the pair kernel (partial sums added in order)
+
+enum CustomTermType : int { TERM_SINGLE = 0, TERM_PAIR = 1, TERM_PAIR_NO_EXCL = 2 };
+
+struct CustomSpec {
+    std::vector<std::string> params;     // per-particle parameter names
+    std::vector<std::string> globals;    // global parameter names
+    std::vector<std::string> integrals;  // integral names (1 or 2)
+    std::vector<std::string> value_names, value_exprs;
+    std::vector<std::string> term_exprs;
+    std::vector<int> term_types;
+};
+
+// Compiled kernels of one spec.  Kernel signatures (all arrays device, double unless noted):
+//   cg_values(int N, prm[N x NP], integ[2 x N], glob[NG], vals[NV x N], dvdi[NV x 2 x N],
+//             esp[N], despdi[2 x N])
+//   cg_pairs(int N, int chunk, pos[N x 3], prm, vals, glob, int method, double cut2,
+//            const double* box3, const int* ex_ptr, const int* ex_idx, part[chunks x NPL x N])
+//   cg_chain(int N, int chunks, part, dvdi, esp, despdi, atom_energy[N], long long* force_fix[3 x N],
+//            dedi[2 x N])
+struct CustomProgram {
+    hipModule_t module = nullptr;
+    hipFunction_t values = nullptr, pairs = nullptr, chain = nullptr;
+    int n_params = 0, n_globals = 0, n_values = 0, n_planes = 0;
+    bool any_pair = false;
+    std::string source;
+    CustomProgram() = default;
+    CustomProgram(const CustomProgram&) = delete;
+    CustomProgram& operator=(const CustomProgram&) = delete;
+    ~CustomProgram();
+};
+
+// Translates one expression to HIP source over the dual type D.  context: 0 computed value
+// number `index` (sees parameters, globals, integrals and earlier values), 1 single-particle
+// energy term, 2 pair energy term (sees r, parameters and values with suffixes 1 and 2,
+// globals).  Throws std::runtime_error naming the problem.
+std::string translate_expression(const std::string& expr, const CustomSpec& spec, int context, int index);
+
+// Validates the spec and generates the kernels' source (host only, no device); throws
+// std::runtime_error with the parser's message.
+void generate_custom_source(const CustomSpec& spec, CustomProgram& out);
+// Compiles out.source for `arch` (e.g. "gfx950") and loads the kernels; throws
+// std::runtime_error with the compiler's message.
+void compile_custom_program(const std::string& arch, CustomProgram& out);
+
+}  // namespace gbi

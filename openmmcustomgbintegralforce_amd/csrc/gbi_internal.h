@@ -88,6 +88,7 @@ struct SysDesc {
     int cut_mode;              // 0: no cutoff; 1: pairs with r <= c (standalone GBSW); 2: r < c (CharmmGBMVForce)
     int chain_trunc;           // standalone GBSW with a cutoff: dB_i/dx_j reaches only atoms j within c of i
     int blk_base;              // index of the system's first 64-atom block in KParams::colblk
+    int custom;                // general expressions: no Born map, generated pair and chain kernels
     double cut2;               // c^2
     float cut2_lo, cut2_hi;    // FP32 band around c^2 inside which the FP64 distance decides
     float trunc_in, trunc_out; // chain_trunc: points nearer their owner than trunc_in keep every shell
@@ -102,6 +103,24 @@ struct SysDesc {
     float sign_chain;          // +1 for GBSW (I accumulates 1 - V), -1 for GBMV
     // Born map / energy
     double alpha0, alpha1, eta, slope, shift, prefactor;
+};
+
+// Launch arguments of the generated kernels of a system with general expressions
+// (gbi_custom.h); host-side, read by launch_evaluate.
+struct CustomLaunch {
+    hipFunction_t values = nullptr, pairs = nullptr, chain = nullptr;
+    int N = 0, chunk = 0, chunks = 0, method = 0;
+    double cut2 = 0.0;
+    const double* box = nullptr;  // device: the system's 3 box edges
+    const double* prm = nullptr;
+    const double* glob = nullptr;
+    double* vals = nullptr;
+    double* dvdi = nullptr;
+    double* esp = nullptr;
+    double* despdi = nullptr;
+    double* part = nullptr;
+    const int* ex_ptr = nullptr;
+    const int* ex_idx = nullptr;
 };
 
 // Grid geometry of one system, written on the device every evaluation.
@@ -136,6 +155,7 @@ struct KParams {
     int any_trunc;           // some system truncates the chain rule at the cutoff
     float4* blk_lo;          // n_colblocks: bounding box of each 64-atom block (cutoff systems)
     float4* blk_hi;
+    const CustomLaunch* custom;  // host pointer: the generated kernels replace the pair and chain passes
     int any_excl;            // some system has excluded pairs
     const int* excl_start;   // n_colblocks + 1: entries of row block b are [excl_start[b], excl_start[b + 1])
     const int* excl_col;     // per entry: its column block (>= the row block)

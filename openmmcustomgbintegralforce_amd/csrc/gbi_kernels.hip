@@ -974,6 +974,7 @@ __global__ void __launch_bounds__(64) k_integrals(KParams p) {
     if (lane != 0) return;
     p.integrals[i] = a0;
     p.integrals[p.N + i] = a1;
+    if (sd.custom) return;  // the generated kernels map the integrals
     const double I4 = sd.i4 == 0 ? a0 : a1;
     const double I7 = sd.i7 == 0 ? a0 : a1;
     const double eta = sd.eta;
@@ -1875,7 +1876,30 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
     mark(3);
     hipLaunchKernelGGL(k_integrals, dim3(p.N), dim3(64), 0, stream, p);
     mark(4);
-    if (adj) {
+    if (p.custom) {
+        // general expressions (one system): values, pairs (column chunks), chain
+        const CustomLaunch& c = *p.custom;
+        int N = c.N, chunk = c.chunk, chunks = c.chunks, method = c.method;
+        double cut2 = c.cut2;
+        const double* pos = p.pos;
+        const double* integ = p.integrals;
+        const double *prm = c.prm, *glob = c.glob, *box = c.box;
+        double *vals = c.vals, *dvdi = c.dvdi, *esp = c.esp, *despdi = c.despdi, *part = c.part;
+        const int *ex_ptr = c.ex_ptr, *ex_idx = c.ex_idx;
+        double* atom_energy = p.atom_energy;
+        long long* force_fix = p.force_fix;
+        double* dedi = p.dedi;
+        void* va[] = {&N, &prm, &integ, &glob, &vals, &dvdi, &esp, &despdi};
+        err = hipModuleLaunchKernel(c.values, (N + 255) / 256, 1, 1, 256, 1, 1, 0, stream, va, nullptr);
+        if (err != hipSuccess) return err;
+        void* pa[] = {&N, &chunk, &pos, &prm, &vals, &glob, &method, &cut2, &box, &ex_ptr, &ex_idx, &part};
+        err = hipModuleLaunchKernel(c.pairs, (N + 63) / 64, chunks, 1, 64, 1, 1, 0, stream, pa, nullptr);
+        if (err != hipSuccess) return err;
+        mark(5);
+        void* ca[] = {&N, &chunks, &part, &dvdi, &esp, &despdi, &atom_energy, &force_fix, &dedi};
+        err = hipModuleLaunchKernel(c.chain, (N + 255) / 256, 1, 1, 256, 1, 1, 0, stream, ca, nullptr);
+        if (err != hipSuccess) return err;
+    } else if (adj) {
         // the host refuses the adjoint of cutoff systems; exclusions take the general path
         if (p.any_excl) {
             hipLaunchKernelGGL(k_block_box, dim3(p.n_colblocks), dim3(64), 0, stream, p);
@@ -1889,11 +1913,13 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
     } else {
         hipLaunchKernelGGL((k_pairs<false, false>), dim3(p.n_tiles), dim3(64), 0, stream, p);
     }
-    mark(5);
-    if (adj)
-        hipLaunchKernelGGL(k_atom_chain<true>, dim3(p.n_colblocks), dim3(kChainGroups * kWave), 0, stream, p);
-    else
-        hipLaunchKernelGGL(k_atom_chain<false>, dim3(p.n_colblocks), dim3(kChainGroups * kWave), 0, stream, p);
+    if (!p.custom) {
+        mark(5);
+        if (adj)
+            hipLaunchKernelGGL(k_atom_chain<true>, dim3(p.n_colblocks), dim3(kChainGroups * kWave), 0, stream, p);
+        else
+            hipLaunchKernelGGL(k_atom_chain<false>, dim3(p.n_colblocks), dim3(kChainGroups * kWave), 0, stream, p);
+    }
     hipLaunchKernelGGL(k_energy, dim3(p.S), dim3(1024), 0, stream, p);
     mark(6);
     for (int vk = 0; vk < VOL_KINDS; ++vk) {

@@ -1,0 +1,168 @@
+"""FP64 restatement of CharmmGBMVForce with general expressions (test infrastructure).
+
+TEST INFRASTRUCTURE ONLY: the checker of the HIP library's general-expression path.  It
+follows the Reference platform's CharmmReferenceGBMV::calculateIxn
+(CharmmReferenceGBMV.cpp:150-204): integrals and their gradients (the C oracle's
+orc_integral_chain, CustomGBIntegral / GBSWIntegral / GBMVIntegralType{I,II}), computed
+values as SingleParticle (:173-176, :206-227), single-particle energy terms (:229-266), pair
+terms over i < j with the cutoff rule r >= c skipped and exclusions of ParticlePair terms
+(:268-348), and the chain rule dE/dV -> dE/dI -> forces (:350-446).  The expressions are
+parsed by sympy and differentiated symbolically, independently of the library's dual-number
+code generation.
+"""
+from __future__ import annotations
+
+import ctypes
+import re
+
+import numpy as np
+import sympy as sp
+
+from . import CUTOFF_PERIODIC, NO_CUTOFF, Params, _dp, lib
+
+SINGLE, PAIR, PAIR_NO_EXCLUSIONS = 0, 1, 2
+
+
+def _functions():
+    x, y, z = sp.symbols("x y z")
+    return {
+        "sqrt": sp.sqrt, "exp": sp.exp, "log": sp.log, "sin": sp.sin, "cos": sp.cos, "tan": sp.tan,
+        "asin": sp.asin, "acos": sp.acos, "atan": sp.atan, "sinh": sp.sinh, "cosh": sp.cosh, "tanh": sp.tanh,
+        "erf": sp.erf, "erfc": sp.erfc, "abs": sp.Abs,
+        "step": lambda a: sp.Piecewise((1, a >= 0), (0, True)),
+        "delta": lambda a: sp.Piecewise((1, sp.Eq(a, 0)), (0, True)),
+        "square": lambda a: a ** 2, "cube": lambda a: a ** 3, "recip": lambda a: 1 / a,
+        "min": sp.Min, "max": sp.Max,
+        "select": lambda a, b, c: sp.Piecewise((c, sp.Eq(a, 0)), (b, True)),
+    }
+
+
+def parse(expr: str, names) -> sp.Expr:
+    """Lepton syntax -> sympy ('^' is the power operator)."""
+    local = dict(_functions())
+    for n in names:
+        local[n] = sp.Symbol(n)
+    text = re.sub(r"\^", "**", expr)
+    return sp.sympify(text, locals=local, rational=False)
+
+
+def _lam(args, expr):
+    return sp.lambdify(args, expr, modules=["numpy", {"Heaviside": lambda a, h=0.5: np.heaviside(a, h)}])
+
+
+def integral_chain(params: Params, pos, radius, dEdI=None):
+    """(integrals [2 x N], forces [N x 3] or None) from the C oracle."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    radius = np.ascontiguousarray(radius, dtype=np.float64)
+    n = pos.shape[0]
+    integ = np.zeros((2, n))
+    forces = np.zeros((n, 3)) if dEdI is not None else None
+    d = None if dEdI is None else np.ascontiguousarray(dEdI, dtype=np.float64)
+    L = lib()
+    L.orc_integral_chain.argtypes = [ctypes.POINTER(Params), ctypes.c_int] + [ctypes.POINTER(ctypes.c_double)] * 5
+    rc = L.orc_integral_chain(ctypes.byref(params), n, _dp(pos), _dp(radius), _dp(d) if d is not None else None,
+                              _dp(integ), _dp(forces) if forces is not None else None)
+    if rc != 0:
+        raise RuntimeError(f"oracle orc_integral_chain failed with {rc}")
+    return integ, forces
+
+
+def compute(params: Params, pos, param_names, param_values, radius_param, integrals, values, terms, globals_=None,
+            exclusions=None):
+    """Energy, forces and computed values of a general CharmmGBMVForce.
+
+    params: oracle Params (integral type, quadrature, grid, orders, method, cutoff, box);
+    integrals: names (1 or 2); values: [(name, expr)]; terms: [(expr, type)]."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    n = pos.shape[0]
+    P = np.asarray(param_values, dtype=np.float64).reshape(n, len(param_names))
+    G = dict(globals_ or {})
+    radius = P[:, radius_param].copy()
+    if len(integrals) == 1:
+        params.orders[1] = params.orders[0]
+    I, _ = integral_chain(params, pos, radius)
+    nI, nV = len(integrals), len(values)
+    gsub = {sp.Symbol(k): v for k, v in G.items()}
+    psym = [sp.Symbol(k) for k in param_names]
+    isym = [sp.Symbol(k) for k in integrals]
+    vsym = [sp.Symbol(k) for k, _ in values]
+
+    # computed values, in order, and their total derivatives dV_k/dI_m
+    V = np.zeros((nV, n))
+    dVdI = np.zeros((nV, 2, n))
+    args = psym + isym + vsym
+    cols = [P[:, k] for k in range(len(psym))] + [I[m] for m in range(nI)]
+    for k, (name, expr) in enumerate(values):
+        e = parse(expr, list(param_names) + list(G) + list(integrals) + [v for v, _ in values[:k]]).subs(gsub)
+        a = args[: len(psym) + nI + k]
+        vals_in = cols + [V[j] for j in range(k)]
+        V[k] = np.broadcast_to(_lam(a, e)(*vals_in), (n,))
+        for m in range(nI):
+            tot = np.broadcast_to(_lam(a, sp.diff(e, isym[m]))(*vals_in), (n,)).astype(float)
+            for j in range(k):
+                tot = tot + np.broadcast_to(_lam(a, sp.diff(e, vsym[j]))(*vals_in), (n,)) * dVdI[j, m]
+            dVdI[k, m] = tot
+
+    # single-particle terms: energy and total dE/dI
+    E = 0.0
+    dEdI = np.zeros((2, n))
+    sp_args = psym + isym + vsym
+    sp_in = cols + [V[j] for j in range(nV)]
+    for expr, ttype in terms:
+        if ttype != SINGLE:
+            continue
+        e = parse(expr, list(param_names) + list(G) + list(integrals) + [v for v, _ in values]).subs(gsub)
+        E += float(np.sum(np.broadcast_to(_lam(sp_args, e)(*sp_in), (n,))))
+        for m in range(nI):
+            tot = np.broadcast_to(_lam(sp_args, sp.diff(e, isym[m]))(*sp_in), (n,)).astype(float)
+            for j in range(nV):
+                tot = tot + np.broadcast_to(_lam(sp_args, sp.diff(e, vsym[j]))(*sp_in), (n,)) * dVdI[j, m]
+            dEdI[m] += tot
+
+    # pair terms over i < j (atom1 = i)
+    forces = np.zeros((n, 3))
+    dEdV = np.zeros((nV, n))
+    iu, ju = np.triu_indices(n, 1)
+    d = pos[ju] - pos[iu]  # x_2 - x_1
+    if params.method == CUTOFF_PERIODIC:
+        box = np.array([params.box[0], params.box[1], params.box[2]])
+        d = d - box * np.floor(d / box + 0.5)
+    r2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+    keep = np.ones(len(iu), dtype=bool) if params.method == NO_CUTOFF else r2 < params.cutoff ** 2
+    excluded = np.zeros(len(iu), dtype=bool)
+    if exclusions is not None and len(exclusions):
+        ex = np.sort(np.asarray(exclusions, dtype=np.int64).reshape(-1, 2), axis=1)
+        key = set((ex[:, 0] * n + ex[:, 1]).tolist())
+        excluded = np.fromiter(((a * n + b) in key for a, b in zip(iu, ju)), dtype=bool, count=len(iu))
+    r = np.sqrt(r2)
+    rs = sp.Symbol("r")
+    p1 = [sp.Symbol(k + "1") for k in param_names]
+    p2 = [sp.Symbol(k + "2") for k in param_names]
+    v1 = [sp.Symbol(k + "1") for k, _ in values]
+    v2 = [sp.Symbol(k + "2") for k, _ in values]
+    pair_args = [rs] + p1 + p2 + v1 + v2
+    pair_names = ["r"] + [k + s for s in "12" for k in param_names] + [k + s for s in "12" for k, _ in values] + list(G)
+    for expr, ttype in terms:
+        if ttype == SINGLE:
+            continue
+        e = parse(expr, pair_names).subs(gsub)
+        m = keep & (~excluded if ttype == PAIR else True)
+        a, b = iu[m], ju[m]
+        ins = [r[m]] + [P[a, k] for k in range(len(p1))] + [P[b, k] for k in range(len(p2))] + \
+              [V[k, a] for k in range(nV)] + [V[k, b] for k in range(nV)]
+        cnt = len(a)
+        E += float(np.sum(np.broadcast_to(_lam(pair_args, e)(*ins), (cnt,))))
+        dEdr = np.broadcast_to(_lam(pair_args, sp.diff(e, rs))(*ins), (cnt,))
+        g = (dEdr / r[m])[:, None] * d[m]  # dE/dr (x2 - x1)/r: force on atom1, minus on atom2
+        np.add.at(forces, a, g)
+        np.add.at(forces, b, -g)
+        for k in range(nV):
+            np.add.at(dEdV[k], a, np.broadcast_to(_lam(pair_args, sp.diff(e, v1[k]))(*ins), (cnt,)))
+            np.add.at(dEdV[k], b, np.broadcast_to(_lam(pair_args, sp.diff(e, v2[k]))(*ins), (cnt,)))
+
+    for m in range(nI):
+        for k in range(nV):
+            dEdI[m] += dEdV[k] * dVdI[k, m]
+    _, fchain = integral_chain(params, pos, radius, dEdI)
+    forces += fchain
+    return float(E), forces, V, I

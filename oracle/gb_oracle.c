@@ -1122,3 +1122,103 @@ long long orc_grid_query(const double* geom, const double* point) {
     G.g = geom[6];
     return grid_cell(&G, point);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Integrals of any orders and their chain rule (general CharmmGBMVForce      */
+/* expressions: CharmmReferenceGBMV.cpp:157-160 evaluates the integrals and   */
+/* their gradients, :363-371 applies dE/dI_k(i) dI_k(i)/dx_j)                */
+/* ------------------------------------------------------------------------- */
+
+int orc_integral_chain(const orc_params* p, int N, const double* pos, const double* radius, const double* dEdI,
+                       double* integrals_out, double* forces_out) {
+    if (!p || N <= 0 || !pos || !radius || !integrals_out) return ORC_ERR_ARG;
+    if (p->model != ORC_MODEL_GBMV_FORM || p->n_int != 2) return ORC_ERR_ARG;
+    if (p->orders[0] < 2 || p->orders[1] < 2) return ORC_ERR_ARG; /* GBSWIntegral.cpp:92-95 */
+    if (p->method < ORC_NO_CUTOFF || p->method > ORC_CUTOFF_PERIODIC) return ORC_ERR_ARG;
+    if (dEdI && !forces_out) return ORC_ERR_ARG;
+    orc_params pp = *p;
+    const int type = pp.integral_type;
+    ctx_t c;
+    memset(&c, 0, sizeof(c));
+    c.p = &pp;
+    c.N = N;
+    c.pos = pos;
+    c.R = radius;
+    c.nint = 2;
+    c.Q = pp.n_radial * pp.n_lebedev;
+    const int Q = c.Q;
+    double* quad = malloc(sizeof(double) * 5 * Q);
+    c.pw = malloc(sizeof(double) * 2 * Q);
+    c.brute_list = malloc(sizeof(int) * N);
+    int rc = 0;
+    if (!quad || !c.pw || !c.brute_list) { rc = ORC_ERR_MEM; goto out; }
+    for (int a = 0; a < N; ++a) c.brute_list[a] = a;
+    if ((rc = orc_quad_table(pp.n_radial, pp.r0, pp.r1, pp.n_lebedev, quad)) != 0) goto out;
+    c.quad = quad;
+    for (int k = 0; k < 2; ++k)
+        for (int q = 0; q < Q; ++q) c.pw[k * Q + q] = quad[5 * q + 4] / pow(quad[5 * q + 3], pp.orders[k] - 2);
+    if ((rc = grid_build(&c.G, &pp, N, pos, radius)) != 0) goto out;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4) num_threads(pp.nthreads > 1 ? pp.nthreads : 1)
+#endif
+    for (int i = 0; i < N; ++i) {
+        double acc[2] = {0, 0};
+        for (int q = 0; q < Q; ++q) {
+            double pt[3] = {pos[3 * i] + quad[5 * q], pos[3 * i + 1] + quad[5 * q + 1], pos[3 * i + 2] + quad[5 * q + 2]};
+            const int* list;
+            int n = point_list(&c.G, pt, &list, c.brute_list);
+            if (type == ORC_INT_GBSW && n == 0) continue;
+            gbmv2_sums s2;
+            double pre = 0;
+            double V = point_volume(&c, pt, list, n, &s2, &pre);
+            for (int k = 0; k < 2; ++k) acc[k] += c.pw[k * Q + q] * point_integrand(type, V);
+        }
+        integrals_out[i] = acc[0];
+        integrals_out[N + i] = acc[1];
+    }
+    if (dEdI) {
+        memset(forces_out, 0, sizeof(double) * 3 * N);
+        int nt = 1;
+#ifdef _OPENMP
+        nt = pp.nthreads > 1 ? pp.nthreads : 1;
+#endif
+        double* part = calloc((size_t)nt * N * 3, sizeof(double));
+        if (!part) { rc = ORC_ERR_MEM; goto out; }
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+#endif
+        {
+            int tid = 0;
+#ifdef _OPENMP
+            tid = omp_get_thread_num();
+#endif
+            double* g = part + (size_t)tid * N * 3;
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+            for (int i = 0; i < N; ++i) {
+                for (int q = 0; q < Q; ++q) {
+                    double pt[3] = {pos[3 * i] + quad[5 * q], pos[3 * i + 1] + quad[5 * q + 1],
+                                    pos[3 * i + 2] + quad[5 * q + 2]};
+                    const int* list;
+                    int n = point_list(&c.G, pt, &list, c.brute_list);
+                    if (type == ORC_INT_GBSW && n == 0) continue;
+                    gbmv2_sums s2;
+                    double pre = 0;
+                    double V = point_volume(&c, pt, list, n, &s2, &pre);
+                    double pref = dEdI[i] * c.pw[q] + dEdI[N + i] * c.pw[Q + q];
+                    point_gradient(&c, pt, i, list, n, V, &s2, pre, pref, g);
+                }
+            }
+        }
+        for (int t = 0; t < nt; ++t)
+            for (int a = 0; a < 3 * N; ++a) forces_out[a] -= part[(size_t)t * N * 3 + a];
+        free(part);
+    }
+out:
+    grid_free(&c.G);
+    free(quad);
+    free(c.pw);
+    free(c.brute_list);
+    return rc;
+}

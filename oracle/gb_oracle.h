@@ -93,6 +93,14 @@ int orc_compute(const orc_params* p, int N, const double* pos, const double* cha
                 double* integrals, double* dEdI, orc_stats* stats);
 
 /*
+ * General CharmmGBMVForce expressions: the two integrals of orders p->orders[0..1] (any
+ * order >= 2; integral-major N x 2) and, when dEdI (2 x N) is given, the chain-rule forces
+ * F_j = -sum_i sum_k dEdI_k(i) dI_k(i)/dx_j (N x 3, overwritten).  GBMV form only.
+ */
+int orc_integral_chain(const orc_params* p, int N, const double* pos, const double* radius, const double* dEdI,
+                       double* integrals, double* forces);
+
+/*
  * Lookup-grid inspection (tests): builds the grid of the model and returns its
  * geometry; cells/entries filled when the buffers are large enough.
  * geom: min[3], max[3], g, n[3] (as doubles).  counts: ncell ints, lists: sum(counts).

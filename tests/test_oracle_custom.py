@@ -1,0 +1,107 @@
+"""The general-expression oracle (oracle/custom.py) pinned on the CPU.
+
+* the canonical GB forms written as expressions (Born value, 0.5 P q^2/B, the STILL pair
+  term) reproduce the C oracle's canonical energy, forces and Born radii to 1e-12, for the
+  three integral types, with a cutoff, in a periodic box and with exclusions of a
+  ParticlePair term;
+* forces = -grad E by central differences for non-canonical expressions (two computed
+  values, a value depending on an earlier one, single-particle terms using integrals and
+  values, a pair term using parameters and values of both atoms, globals, one integral);
+* the library rejects malformed expressions before any device work (status 1).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import custom as C
+from openmmcustomgbintegralforce_amd import _native as N
+from openmmcustomgbintegralforce_amd import synthetic as S
+
+G = dict(alpha0=-0.180, alpha1=1.817, eta=0.05, slope=1.0, shift=0.0, P=-138.935456 * (1 - 1 / 80))
+BORN = ("B", "slope*1/(alpha0*(1/eta-I4)+alpha1*(1/(4*eta^4)-I7)^(1/4))+shift")
+SELF = ("0.5*P*q^2/B", C.SINGLE)
+STILL = "P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))"
+
+# a non-canonical form: a second value built on B, terms mixing integrals, values and globals
+VALUES2 = [BORN, ("C", "B*(1+0.05*tanh(q))+0.001*I7^0.25")]
+TERMS2 = [("0.5*P*q^2/C+k*radius^2*I4", C.SINGLE),
+          ("P*q1*q2/sqrt(r^2+C1*C2*exp(-r^2/(2*C1*C2)))*(1+0.1*step(r-0.5)*square(radius1-radius2))", C.PAIR)]
+
+
+def _canonical(s, itype, pair_type=C.PAIR_NO_EXCLUSIONS, **kw):
+    return C.compute(O.gbmv_params(itype, **kw), s.pos, ["q", "radius"], np.stack([s.charge, s.radius], 1), 1,
+                     ["I4", "I7"], [BORN], [SELF, (STILL, pair_type)], G)
+
+
+@pytest.mark.parametrize("itype", [0, 1, 2])
+def test_canonical_forms_as_expressions(itype):
+    s = S.globule(150, 3)
+    ref = O.compute(O.gbmv_params(itype), s.pos, s.charge, s.radius)
+    E, F, V, _ = _canonical(s, itype)
+    assert abs(E - ref.energy) < 1e-12 * abs(ref.energy)
+    assert np.abs(F - ref.forces).max() < 1e-11 * np.abs(ref.forces).max()
+    assert np.abs(V[0] - ref.born).max() < 1e-12
+
+
+def test_canonical_forms_with_cutoff_box_and_exclusions():
+    s = S.globule(160, 4)
+    L = [3.6, 3.8, 3.7]
+    pos = s.pos - s.pos.mean(axis=0) + np.array(L) / 2
+    ex = [(0, 5), (3, 9), (10, 11), (40, 41), (7, 100)]
+    kw = dict(method=O.CUTOFF_PERIODIC, cutoff=1.2, box=L)
+    ref = O.compute(O.set_exclusions(O.gbmv_params(2, **kw), ex), pos, s.charge, s.radius)
+    E, F, _, _ = C.compute(O.gbmv_params(2, **kw), pos, ["q", "radius"], np.stack([s.charge, s.radius], 1), 1,
+                           ["I4", "I7"], [BORN], [SELF, (STILL, C.PAIR)], G, exclusions=ex)
+    assert abs(E - ref.energy) < 1e-12 * abs(ref.energy)
+    assert np.abs(F - ref.forces).max() < 1e-11 * np.abs(ref.forces).max()
+
+
+def _general(s, pos, one_integral=False, **kw):
+    names = ["q", "radius"]
+    pv = np.stack([s.charge, s.radius], 1)
+    if one_integral:
+        values = [("B", "1/(1/radius-0.5*I4*radius)")]
+        terms = [("0.5*P*q^2/B", C.SINGLE), (STILL, C.PAIR_NO_EXCLUSIONS)]
+        return C.compute(O.gbmv_params(2, **kw), pos, names, pv, 1, ["I4"], values, terms, dict(G))
+    return C.compute(O.gbmv_params(1, **kw), pos, names, pv, 1, ["I4", "I7"], VALUES2, TERMS2, dict(G, k=0.3))
+
+
+@pytest.mark.parametrize("one_integral", [False, True])
+def test_general_expressions_forces_are_the_gradient(one_integral):
+    s = S.globule(90, 8)
+    E, F, _, _ = _general(s, s.pos, one_integral)
+    rng = np.random.default_rng(4)
+    h = 1e-6
+    for _ in range(5):
+        i, d = rng.integers(s.n), rng.integers(3)
+        pp, pm = s.pos.copy(), s.pos.copy()
+        pp[i, d] += h
+        pm[i, d] -= h
+        fd = -(_general(s, pp, one_integral)[0] - _general(s, pm, one_integral)[0]) / (2 * h)
+        assert abs(fd - F[i, d]) <= 2e-5 * max(1.0, abs(F[i, d])), (i, d, fd, F[i, d])
+
+
+def _create(spec_kw, params=None):
+    p = params or N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV2)
+    pv = np.ascontiguousarray([[0.1, 0.2], [-0.1, 0.15]], dtype=np.float64)
+    base = dict(param_names=["q", "radius"], param_values=pv, radius_param=1, integrals=["I4", "I7"],
+                values=[BORN], terms=[SELF, (STILL, C.PAIR_NO_EXCLUSIONS)], globals_=G)
+    base.update(spec_kw)
+    try:
+        N.Engine.custom(p, **base).close()
+        return 0
+    except N.GbiError as e:
+        return e.status
+
+
+def test_library_rejects_malformed_expressions():
+    assert _create(dict(values=[("B", "1/(alpha0*(1/eta-I4)")])) == 1      # unbalanced
+    assert _create(dict(values=[("B", "foo(I4)")])) == 1                   # unknown function
+    assert _create(dict(terms=[("0.5*P*q^2/Bx", C.SINGLE)])) == 1          # unknown variable
+    assert _create(dict(terms=[("P*q1*q2/r*I41", C.PAIR)])) == 1           # integrals are not pair variables
+    assert _create(dict(values=[("B", "max(I4)")])) == 1                   # arity
+    assert _create(dict(values=[("r", "I4")])) == 1                        # reserved name
+    # well formed: only the device is missing here (or the context is created on a GPU)
+    assert _create({}) in (0, 8)
