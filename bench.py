@@ -55,6 +55,10 @@ WORKLOADS = {
                                   desc="CharmmGBMVForce GBMVIntegralTypeII CutoffPeriodic 1.6 nm: atoms filling a "
                                        "4 nm box at 100 atoms/nm^3 (6,400 atoms), 24x50 quadrature, periodic "
                                        "lookup grid 0.15 nm"),
+    "gbmv2-5000-expressions": dict(model="gbmv", integral=2, n_atoms=5000, n_lebedev=50, seed=0, expressions=True,
+                                   desc="CharmmGBMVForce GBMVIntegralTypeII, 5,000-atom globule, 24x50 quadrature, "
+                                        "the canonical GB forms given as general expressions: value, single-particle "
+                                        "and pair kernels generated over FP64 dual numbers and compiled with hiprtc"),
     "batch64": dict(model="batch", integral=-1, n_atoms=0, n_lebedev=0, seed=0,
                     desc="64 systems in one batched context, N ~ U[300, 6000] (seeded), alternating "
                          "CharmmGBSWForce 24x38 and CharmmGBMVForce GBMVIntegralTypeII 24x50, hipGraph replay; "
@@ -107,7 +111,14 @@ def main() -> int:
     p.n_lebedev = w["n_lebedev"]
     p.nonbonded_method = w.get("method", N.NO_CUTOFF)
     p.cutoff = w.get("cutoff", 1.0)
-    eng = N.Engine(p, sysm.charge, sysm.radius)
+    if w.get("expressions"):
+        G = dict(alpha0=p.alpha0, alpha1=p.alpha1, eta=p.eta, slope=p.slope, shift=p.shift, P=p.prefactor)
+        eng = N.Engine.custom(p, ["q", "radius"], np.stack([sysm.charge, sysm.radius], 1), 1, ["I4", "I7"],
+                              [("B", "slope*1/(alpha0*(1/eta-I4)+alpha1*(1/(4*eta^4)-I7)^(1/4))+shift")],
+                              [("0.5*P*q^2/B", N.TERM_SINGLE),
+                               ("P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))", N.TERM_PAIR_NO_EXCLUSIONS)], G)
+    else:
+        eng = N.Engine(p, sysm.charge, sysm.radius)
     if box:
         eng.set_periodic_box(box)
 
@@ -157,7 +168,9 @@ def main() -> int:
         n_vol_alg = n_vol_exact if n_frc is not None else None
 
     # with a cutoff the pair pass skips blocks: its work is not the all-pairs count
-    kernel_flops = {} if p.nonbonded_method != N.NO_CUTOFF else {"pairs": FL.pair_flops(n_atoms)}
+    # (and generated kernels evaluate every pair from both sides in FP64 duals)
+    kernel_flops = {} if p.nonbonded_method != N.NO_CUTOFF or w.get("expressions") else {
+        "pairs": FL.pair_flops(n_atoms)}
     if n_vol_alg is not None:
         kernel_flops["volume"] = FL.volume_flops(kind, n_vol_alg, stats["n_points"])
     if n_frc is not None:

@@ -5,9 +5,10 @@ Classes and method names follow ``openmmapi/include/openmm/CharmmGBSWForce.h``,
 through the C ABI (``include/gbi.h``).  ``CustomGBIntegralForce`` is an alias of
 ``CharmmGBMVForce`` (the "custom GB integral force").
 
-Only the canonical GB forms run on the HIP platform: a ``CharmmGBMVForce`` must define
-integrals of order 4 and 7, the Born radius from them, and the STILL self + pair energy
-(see ``canonical_gbmv_force``).  Other expressions raise ``GbiError(UNSUPPORTED)``.
+The canonical GB forms (integrals of order 4 and 7, the Born radius from them, the STILL
+self + pair energy; see ``canonical_gbmv_force``) run on the hand-written kernels; any other
+``CharmmGBMVForce`` expressions run on kernels generated from them (``Engine.custom``,
+``gbmv_custom_from_force``).
 """
 from __future__ import annotations
 
@@ -370,6 +371,44 @@ class HipCalcCharmmGBSWForceKernel:
         return self._engine
 
 
+def gbmv_custom_from_force(force: CharmmGBMVForce) -> dict:
+    """Arguments of ``Engine.custom`` for a CharmmGBMVForce with general expressions: its
+    per-particle and global parameters, integrals (orders, Lebedev rule and radial count from
+    parametersInt, GBSWIntegral.cpp:37-40), computed values (evaluated as SingleParticle, as
+    the reference does, CharmmReferenceGBMV.cpp:173-176), energy terms, cutoff and method."""
+    t = force.getGBIntegralType()
+    if t == CharmmGBMVForce.CustomIntegral:
+        raise N.GbiError(2, "CustomIntegral (type 3) is not implemented by the reference either")
+    names = [force.getPerParticleParameterName(i) for i in range(force.getNumPerParticleParameters())]
+    if "radius" not in names:  # GBSWIntegral.cpp:79-80
+        raise OpenMMException("GBSWIntegral: the per perticle parameter 'radius' must be defined")
+    integ = [tuple(force.getGBIntegralParameters(i)[:2]) for i in range(force.getNumGBIntegrals())]
+    for _, pi in integ:
+        if len(pi) == 0:
+            raise OpenMMException("GBSWIntegral: the order of integral must be given")
+        if pi[0] < 2:
+            raise OpenMMException("GBSWIntegral: the order of integral must be greater or equal to 2")
+    if not 1 <= len(integ) <= 2:
+        raise N.GbiError(2, "general expressions take one or two integrals on the HIP platform")
+    p = N.default_params(N.MODEL_GBMV, t)
+    first = integ[0][1]
+    if len(first) > 1:
+        p.n_lebedev = _LEBEDEV_RULE_NPTS[first[1]]
+    if len(first) > 2:
+        p.n_radial = first[2]
+    for k, (_, pi) in enumerate(integ):
+        p.orders[k] = pi[0]
+    p.nonbonded_method = force.getNonbondedMethod()
+    p.cutoff = force.getCutoffDistance()
+    values = [(n, e) for n, e, _ in force._values]
+    terms = [(e, ty) for e, ty in force._energy]
+    g = {force.getGlobalParameterName(i): force.getGlobalParameterDefaultValue(i)
+         for i in range(force.getNumGlobalParameters())}
+    pv = np.array([force.getParticleParameters(i) for i in range(force.getNumParticles())], dtype=np.float64)
+    return dict(params=p, param_names=names, param_values=pv.reshape(-1, len(names)), radius_param=names.index("radius"),
+                integrals=[n for n, _ in integ], values=values, terms=terms, globals_=g)
+
+
 def gbmv_exclusions(force: CharmmGBMVForce) -> np.ndarray:
     """Excluded pairs (k x 2) the canonical pair term honours: those of a ParticlePair term,
     none for ParticlePairNoExclusions (CharmmReferenceGBMV.cpp:196-199, :272-284)."""
@@ -393,12 +432,27 @@ class HipCalcCharmmGBMVForceKernel:
     def initialize(self, system: System, force: CharmmGBMVForce) -> None:
         if system.getNumParticles() != force.getNumParticles():  # CharmmGBMVForceImpl.cpp:58-59
             raise OpenMMException("CharmmGBMVForce must have exactly as many particles as the System it belongs to.")
-        p, q, r = gbmv_params_from_force(force)
-        for k, v in self._overrides.items():
-            setattr(p, "lambda_" if k == "lambda" else k, v)
-        self._method = p.nonbonded_method
-        self._engine = N.Engine(p, q, r)
-        ex = gbmv_exclusions(force)
+        try:
+            p, q, r = gbmv_params_from_force(force)
+            self._custom = False
+        except N.GbiError as err:
+            if err.status != 2 or force.getGBIntegralType() == CharmmGBMVForce.CustomIntegral:
+                raise
+            self._custom = True  # general expressions: kernels generated from the force's expressions
+        if self._custom:
+            spec = gbmv_custom_from_force(force)
+            p = spec.pop("params")
+            for k, v in self._overrides.items():
+                setattr(p, "lambda_" if k == "lambda" else k, v)
+            self._method = p.nonbonded_method
+            self._engine = N.Engine.custom(p, **spec)
+            ex = np.asarray(force._exclusions, dtype=np.int32).reshape(-1, 2)
+        else:
+            for k, v in self._overrides.items():
+                setattr(p, "lambda_" if k == "lambda" else k, v)
+            self._method = p.nonbonded_method
+            self._engine = N.Engine(p, q, r)
+            ex = gbmv_exclusions(force)
         if len(ex):
             self._engine.set_exclusions(ex)
 
@@ -408,6 +462,12 @@ class HipCalcCharmmGBMVForceKernel:
         return e if includeEnergy else 0.0
 
     def copyParametersToContext(self, context: ContextImpl, force: CharmmGBMVForce) -> None:
+        if self._custom:
+            if force.getNumParticles() != self._engine.n:  # CharmmReferenceKernels.cpp:373-374
+                raise OpenMMException("updateParametersInContext: The number of particles has changed")
+            self._engine.close()
+            self.initialize(System(force.getNumParticles()), force)
+            return
         _, q, r = gbmv_params_from_force(force)
         self._engine.update_parameters(q, r)
 

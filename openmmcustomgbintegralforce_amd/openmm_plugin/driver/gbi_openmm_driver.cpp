@@ -2,7 +2,8 @@
 // OpenMM does: the kernel is created by name from the "HIP" platform's registered factory,
 // then initialize / execute / copyParametersToContext.  Used by the tests.
 //
-// Input (binary, native endianness): int32 model (0 CharmmGBSWForce, 1 CharmmGBMVForce),
+// Input (binary, native endianness): int32 model (0 CharmmGBSWForce, 1 CharmmGBMVForce with
+// the canonical expressions, 2 CharmmGBMVForce with the general ones of generalGBMV),
 // int32 integral type, int32 Lebedev rule (0 = the reference's), int32 N, then N x 3
 // positions, N charges, N radii (doubles), optionally followed by int32 nonbonded method,
 // double cutoff and 3 doubles of a rectangular periodic box, then optionally int32 n and n
@@ -54,6 +55,22 @@ CharmmGBMVForce* canonicalGBMV(const std::vector<double>& q, const std::vector<d
                      excl.empty() ? CharmmGBMVForce::ParticlePairNoExclusions : CharmmGBMVForce::ParticlePair);
     for (size_t i = 0; i < q.size(); ++i) f->addParticle({q[i], r[i]});
     for (size_t k = 0; k + 1 < excl.size(); k += 2) f->addExclusion(excl[k], excl[k + 1]);
+    return f;
+}
+
+// A non-canonical CharmmGBMVForce: one integral, a Born value of another form, the STILL pair
+// term (the same expressions as tests/test_gpu_openmm_plugin.py's oracle call).
+CharmmGBMVForce* generalGBMV(const std::vector<double>& q, const std::vector<double>& r, int type) {
+    auto* f = new CharmmGBMVForce();
+    f->addPerParticleParameter("q");
+    f->addPerParticleParameter("radius");
+    f->addGlobalParameter("P", -138.935456 * (1.0 - 1.0 / 80.0));
+    f->addGBIntegral("I4", std::vector<int>{4});
+    f->setGBIntegralType((CharmmGBMVForce::GBIntegralType)type);
+    f->addComputedValue("B", "1/(1/radius-0.5*I4*radius)", CharmmGBMVForce::SingleParticle);
+    f->addEnergyTerm("0.5*P*q^2/B", CharmmGBMVForce::SingleParticle);
+    f->addEnergyTerm(gbi_openmm::kCanonicalPairEnergy, CharmmGBMVForce::ParticlePairNoExclusions);
+    for (size_t i = 0; i < q.size(); ++i) f->addParticle({q[i], r[i]});
     return f;
 }
 
@@ -130,7 +147,8 @@ int main(int argc, char** argv) {
             }
             fprintf(out, "count_change_throws %d\n", count_throws);
         } else {
-            std::unique_ptr<CharmmGBMVForce> force(canonicalGBMV(q, r, type, rule, excl));
+            std::unique_ptr<CharmmGBMVForce> force(model == 2 ? generalGBMV(q, r, type)
+                                                              : canonicalGBMV(q, r, type, rule, excl));
             force->setNonbondedMethod((CharmmGBMVForce::NonbondedMethod)method);
             force->setCutoffDistance(cutoff);
             std::unique_ptr<KernelImpl> k(platform.createKernel(CalcCharmmGBMVForceKernel::Name(), context));
@@ -141,7 +159,8 @@ int main(int argc, char** argv) {
             fprintf(out, "energy %.17g\n", e);
             for (int i = 0; i < n; ++i) fprintf(out, "%.17g %.17g %.17g\n", f[i][0], f[i][1], f[i][2]);
             e_noforce = kernel.execute(context, false, true);
-            std::unique_ptr<CharmmGBMVForce> scaled(canonicalGBMV(qs, r, type, rule, excl));
+            std::unique_ptr<CharmmGBMVForce> scaled(model == 2 ? generalGBMV(qs, r, type)
+                                                               : canonicalGBMV(qs, r, type, rule, excl));
             scaled->setNonbondedMethod((CharmmGBMVForce::NonbondedMethod)method);
             scaled->setCutoffDistance(cutoff);
             kernel.copyParametersToContext(context, *scaled);

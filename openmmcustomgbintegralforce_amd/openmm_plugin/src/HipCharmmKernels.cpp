@@ -185,9 +185,142 @@ void gbmvParamsFromForce(const CharmmGBMVForce& force, gbi_params& p, std::vecto
     }
 }
 
+bool isCanonicalGBMV(const CharmmGBMVForce& force) {
+    bool q = false;
+    for (int k = 0; k < force.getNumPerParticleParameters(); ++k) q = q || force.getPerParticleParameterName(k) == "q";
+    if (!q || force.getNumGBIntegrals() != 2 || force.getNumComputedValues() != 1 || force.getNumEnergyTerms() != 2)
+        return false;
+    std::vector<int> o[2];
+    for (int k = 0; k < 2; ++k) {
+        std::string name;
+        force.getGBIntegralParameters(k, name, o[k]);
+        if (o[k].empty()) return false;
+    }
+    if (!((o[0][0] == 4 && o[1][0] == 7) || (o[0][0] == 7 && o[1][0] == 4))) return false;
+    std::string name, expr;
+    CharmmGBMVForce::ComputationType t;
+    force.getComputedValueParameters(0, name, expr, t);
+    bool ok = name == "B" && strip(expr) == kCanonicalBornValue;
+    force.getEnergyTermParameters(0, expr, t);
+    ok = ok && strip(expr) == kCanonicalSelfEnergy && t == CharmmGBMVForce::SingleParticle;
+    force.getEnergyTermParameters(1, expr, t);
+    ok = ok && strip(expr) == kCanonicalPairEnergy &&
+         (t == CharmmGBMVForce::ParticlePairNoExclusions || t == CharmmGBMVForce::ParticlePair);
+    int seen = 0;
+    for (int k = 0; k < force.getNumGlobalParameters(); ++k) {
+        const std::string& g = force.getGlobalParameterName(k);
+        seen += g == "alpha0" || g == "alpha1" || g == "eta" || g == "slope" || g == "shift" || g == "P";
+    }
+    return ok && seen == 6;
+}
+
+void HipCalcCharmmGBMVForceKernel::createCustom(const CharmmGBMVForce& force) {
+    const CharmmGBMVForce::GBIntegralType type = force.getGBIntegralType();
+    if (type == CharmmGBMVForce::CustomIntegral)  // CharmmReferenceKernels.cpp:129-133
+        throw OpenMMException("CharmmGBMVForce: CustomIntegral (type 3) is not implemented by the reference either");
+    std::vector<std::string> pnames, gnames, inames, vnames, vexprs, texprs;
+    std::vector<double> gvals, pvals;
+    std::vector<int32_t> ttypes;
+    int radius = -1;
+    for (int k = 0; k < force.getNumPerParticleParameters(); ++k) {
+        pnames.push_back(force.getPerParticleParameterName(k));
+        if (pnames.back() == "radius") radius = k;
+    }
+    if (radius < 0)  // GBSWIntegral.cpp:79-80
+        throw OpenMMException("GBSWIntegral: the per perticle parameter 'radius' must be defined");
+    gbi_params p;
+    check(gbi_params_default(GBI_MODEL_GBMV, (int32_t)type, &p));
+    for (int k = 0; k < force.getNumGBIntegrals(); ++k) {
+        std::string name;
+        std::vector<int> ints;
+        force.getGBIntegralParameters(k, name, ints);
+        if (ints.empty()) throw OpenMMException("GBSWIntegral: the order of integral must be given");
+        if (ints[0] < 2) throw OpenMMException("GBSWIntegral: the order of integral must be greater or equal to 2");
+        if (k == 0 && ints.size() > 1) p.n_lebedev = lebedevPoints(ints[1]);
+        if (k == 0 && ints.size() > 2) p.n_radial = ints[2];
+        if (k < 2) p.orders[k] = ints[0];
+        inames.push_back(name);
+    }
+    for (int k = 0; k < force.getNumGlobalParameters(); ++k) {
+        gnames.push_back(force.getGlobalParameterName(k));
+        gvals.push_back(force.getGlobalParameterDefaultValue(k));
+    }
+    for (int k = 0; k < force.getNumComputedValues(); ++k) {
+        std::string name, expr;
+        CharmmGBMVForce::ComputationType t;
+        force.getComputedValueParameters(k, name, expr, t);  // evaluated as SingleParticle (CharmmReferenceGBMV.cpp:173-176)
+        vnames.push_back(name);
+        vexprs.push_back(expr);
+    }
+    bool excl = false;
+    for (int k = 0; k < force.getNumEnergyTerms(); ++k) {
+        std::string expr;
+        CharmmGBMVForce::ComputationType t;
+        force.getEnergyTermParameters(k, expr, t);
+        texprs.push_back(expr);
+        ttypes.push_back((int32_t)t);
+        excl = excl || t == CharmmGBMVForce::ParticlePair;
+    }
+    const int n = force.getNumParticles();
+    const int np = (int)pnames.size();
+    pvals.resize((size_t)n * np);
+    for (int i = 0; i < n; ++i) {
+        std::vector<double> v;
+        force.getParticleParameters(i, v);
+        if ((int)v.size() < np) throw OpenMMException("CharmmGBMVForce: a particle lacks parameters");
+        for (int k = 0; k < np; ++k) pvals[(size_t)i * np + k] = v[k];
+    }
+    p.nonbonded_method = (int32_t)force.getNonbondedMethod();
+    p.cutoff = force.getCutoffDistance();
+    auto cstr = [](const std::vector<std::string>& v) {
+        std::vector<const char*> out;
+        for (const auto& x : v) out.push_back(x.c_str());
+        return out;
+    };
+    const auto cp = cstr(pnames), cg = cstr(gnames), ci = cstr(inames), cvn = cstr(vnames), cve = cstr(vexprs),
+               ct = cstr(texprs);
+    gbi_custom_spec spec{};
+    spec.n_params = np;
+    spec.param_names = cp.data();
+    spec.param_values = pvals.data();
+    spec.radius_param = radius;
+    spec.n_globals = (int32_t)gnames.size();
+    spec.global_names = cg.data();
+    spec.global_values = gvals.data();
+    spec.n_integrals = (int32_t)inames.size();
+    spec.integral_names = ci.data();
+    spec.n_values = (int32_t)vnames.size();
+    spec.value_names = cvn.data();
+    spec.value_exprs = cve.data();
+    spec.n_terms = (int32_t)texprs.size();
+    spec.term_exprs = ct.data();
+    spec.term_types = ttypes.data();
+    gbi_destroy(ctx_);
+    ctx_ = nullptr;
+    n_ = n;
+    periodic_ = p.nonbonded_method == GBI_CUTOFF_PERIODIC;
+    check(gbi_create_custom(n, &p, &spec, &ctx_));
+    if (excl && force.getNumExclusions() > 0) {
+        std::vector<int32_t> pairs(2 * (size_t)force.getNumExclusions());
+        for (int k = 0; k < force.getNumExclusions(); ++k) {
+            int a, b;
+            force.getExclusionParticles(k, a, b);
+            pairs[2 * k] = a;
+            pairs[2 * k + 1] = b;
+        }
+        check(gbi_set_exclusions(ctx_, 0, force.getNumExclusions(), pairs.data()));
+    }
+    custom_ = true;
+}
+
 void HipCalcCharmmGBMVForceKernel::initialize(const System& system, const CharmmGBMVForce& force) {
     if (system.getNumParticles() != force.getNumParticles())  // CharmmGBMVForceImpl.cpp:58-59
         throw OpenMMException("CharmmGBMVForce must have exactly as many particles as the System it belongs to.");
+    if (force.getGBIntegralType() != CharmmGBMVForce::CustomIntegral && !isCanonicalGBMV(force)) {
+        createCustom(force);  // general expressions: kernels generated from the force's expressions
+        return;
+    }
+    custom_ = false;
     gbi_params p;
     std::vector<double> q, r;
     gbmvParamsFromForce(force, p, q, r);
@@ -213,6 +346,13 @@ double HipCalcCharmmGBMVForceKernel::execute(ContextImpl& context, bool includeF
 }
 
 void HipCalcCharmmGBMVForceKernel::copyParametersToContext(ContextImpl& context, const CharmmGBMVForce& force) {
+    if (custom_) {
+        if (force.getNumParticles() != n_)  // CharmmReferenceKernels.cpp:373-374
+            throw OpenMMException("updateParametersInContext: The number of particles has changed");
+        createCustom(force);
+        context.systemChanged();
+        return;
+    }
     gbi_params p;
     std::vector<double> q, r;
     gbmvParamsFromForce(force, p, q, r);

@@ -51,9 +51,10 @@ private:
     bool honor_;
 };
 
-// CalcCharmmGBMVForceKernel on HIP for the canonical GB energy (integrals of order 4 and 7,
-// B = slope/(alpha0 (1/eta - I4) + alpha1 (1/(4 eta^4) - I7)^(1/4)) + shift, STILL energy);
-// other expressions throw.
+// CalcCharmmGBMVForceKernel on HIP: the canonical GB energy (integrals of order 4 and 7,
+// B = slope/(alpha0 (1/eta - I4) + alpha1 (1/(4 eta^4) - I7)^(1/4)) + shift, STILL energy)
+// runs on the hand-written kernels; other expressions on kernels generated from them
+// (gbi_create_custom).
 class HipCalcCharmmGBMVForceKernel : public OpenMM::CalcCharmmGBMVForceKernel, public HipGBKernelBase {
 public:
     HipCalcCharmmGBMVForceKernel(std::string name, const OpenMM::Platform& platform)
@@ -61,7 +62,14 @@ public:
     void initialize(const OpenMM::System& system, const OpenMM::CharmmGBMVForce& force) override;
     double execute(OpenMM::ContextImpl& context, bool includeForces, bool includeEnergy) override;
     void copyParametersToContext(OpenMM::ContextImpl& context, const OpenMM::CharmmGBMVForce& force) override;
+
+private:
+    void createCustom(const OpenMM::CharmmGBMVForce& force);
+    bool custom_ = false;
 };
+
+// Whether a CharmmGBMVForce holds exactly the canonical expressions.
+bool isCanonicalGBMV(const OpenMM::CharmmGBMVForce& force);
 
 // The canonical CharmmGBMVForce expressions recognised by HipCalcCharmmGBMVForceKernel.
 extern const char* const kCanonicalBornValue;  // computed value "B"

@@ -85,3 +85,60 @@ def test_cpp_kernel_classes_particle_pair_exclusions(tmp_path):
     ref = O.compute(O.set_exclusions(oracle_params(p), ex), s.pos, s.charge, s.radius)
     assert rel_err(e, ref.energy) < 1e-5
     assert rel_rms(f, ref.forces) < 1e-5
+
+
+def test_cpp_kernel_classes_general_expressions(tmp_path):
+    """A CharmmGBMVForce with non-canonical expressions through the C++ kernel class: kernels
+    generated from its expressions (driver model 2: one integral, B = 1/(1/radius -
+    0.5 I4 radius), 0.5 P q^2/B and the STILL pair term)."""
+    from oracle import custom as C
+
+    s = S.globule(350, 44)
+    e, f, rest = _run(tmp_path, 2, 2, 0, s)
+    p = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV2)
+    p.orders[0] = 4
+    P = -138.935456 * (1.0 - 1.0 / 80.0)
+    E, F, _, _ = C.compute(oracle_params(p), s.pos, ["q", "radius"], np.stack([s.charge, s.radius], 1), 1, ["I4"],
+                           [("B", "1/(1/radius-0.5*I4*radius)")],
+                           [("0.5*P*q^2/B", C.SINGLE), ("P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))",
+                                                        C.PAIR_NO_EXCLUSIONS)], {"P": P})
+    print(f"[openmm-plugin] general expressions N={s.n}: energy {e:.6f} vs {E:.6f}", flush=True)
+    assert rel_err(e, E) < 1e-5
+    assert rel_rms(f, F) < 1e-5
+    assert abs(float(rest["energy_scaled"]) - 2.25 * e) < 1e-6 * abs(e)
+
+
+def test_python_kernel_mirror_general_expressions():
+    """forces.HipCalcCharmmGBMVForceKernel runs a non-canonical CharmmGBMVForce."""
+    from oracle import custom as C
+    from openmmcustomgbintegralforce_amd import forces as F
+
+    s = S.globule(300, 45)
+    f = F.CharmmGBMVForce()
+    f.addPerParticleParameter("q")
+    f.addPerParticleParameter("radius")
+    f.addGlobalParameter("P", -138.935456 * (1.0 - 1.0 / 80.0))
+    f.addGlobalParameter("k", 0.2)
+    f.addGBIntegral("I4", [4])
+    f.addGBIntegral("I7", [7])
+    f.setGBIntegralType(F.CharmmGBMVForce.GBMVIntegralTypeI)
+    born = "1/(-0.180*(1/0.05-I4)+1.817*(1/(4*0.05^4)-I7)^(1/4))"
+    f.addComputedValue("B", born, F.CharmmGBMVForce.SingleParticle)
+    f.addEnergyTerm("0.5*P*q^2/B+k*I4*radius", F.CharmmGBMVForce.SingleParticle)
+    f.addEnergyTerm("P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))", F.CharmmGBMVForce.ParticlePair)
+    for q, r in zip(s.charge, s.radius):
+        f.addParticle([q, r])
+    for i in range(0, s.n - 1, 4):
+        f.addExclusion(i, i + 1)
+    kern = F.HipCalcCharmmGBMVForceKernel()
+    kern.initialize(F.System(s.n), f)
+    ctx = F.ContextImpl(s.pos)
+    e = kern.execute(ctx)
+    p = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV1)
+    E, Fo, _, _ = C.compute(oracle_params(p), s.pos, ["q", "radius"], np.stack([s.charge, s.radius], 1), 1,
+                            ["I4", "I7"], [("B", born)],
+                            [("0.5*P*q^2/B+k*I4*radius", C.SINGLE),
+                             ("P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))", C.PAIR)],
+                            {"P": -138.935456 * (1.0 - 1.0 / 80.0), "k": 0.2}, exclusions=f._exclusions)
+    print(f"[openmm-plugin] python mirror general expressions: energy {e:.6f} vs {E:.6f}", flush=True)
+    assert rel_err(e, E) < 1e-5 and rel_rms(ctx.forces, Fo) < 1e-5
