@@ -1277,7 +1277,8 @@ gbi_status gbi_create_custom(int32_t n_atoms, const gbi_params* params, const gb
     HIP_TRY(c->c_dvdi.ensure(std::max<size_t>(1, 2 * NV * N)));
     HIP_TRY(c->c_esp.ensure(N));
     HIP_TRY(c->c_despdi.ensure(2 * N));
-    HIP_TRY(c->c_part.ensure((size_t)gbi::kCustomChunks * NPL * N));
+    const size_t nb = (N + 63) / 64;
+    HIP_TRY(c->c_part.ensure(nb * (nb + 1) / 2 * NPL * 64));
     HIP_TRY(hipMemcpyAsync(c->c_prm.p, spec->param_values, sizeof(double) * N * spec->n_params, hipMemcpyHostToDevice,
                            c->stream));
     if (spec->n_globals > 0)
@@ -1289,8 +1290,7 @@ gbi_status gbi_create_custom(int32_t n_atoms, const gbi_params* params, const gb
     L.pairs = prog->pairs;
     L.chain = prog->chain;
     L.N = n_atoms;
-    L.chunks = gbi::kCustomChunks;
-    L.chunk = (n_atoms + gbi::kCustomChunks - 1) / gbi::kCustomChunks;
+    L.nb = (int)nb;
     L.method = P.nonbonded_method;
     L.cut2 = P.nonbonded_method != GBI_NO_CUTOFF ? P.cutoff * P.cutoff : 0.0;
     L.box = c->d_box.p;

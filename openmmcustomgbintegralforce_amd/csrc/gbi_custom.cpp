@@ -328,8 +328,7 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
     std::ostringstream s;
     s << kDualSource;
     s << "#define NP " << NP << "\n#define NG " << (NG > 0 ? NG : 1) << "\n#define NV " << NV << "\n";
-    const int NPL = 4 + NV;
-    s << "#define NPL " << NPL << "\n";
+    const int NPL = (4 + NV) + (3 + NV);  // row and column planes of a pair tile
 
     // ---- per atom: computed values and single-particle terms, duals over the integrals ----
     s << "extern \"C\" __global__ void cg_values(int N, const double* __restrict__ prm, const double* __restrict__ "
@@ -352,77 +351,121 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
             s << "    E_ = E_ + (" << translate_expression(spec.term_exprs[t], spec, 1, 0) << ");\n";
     s << "    esp[i] = E_.v;\n    despdi[i] = E_.d[0];\n    despdi[N + i] = E_.d[1];\n}\n";
 
-    // ---- pairs: each thread takes its atom's side of every pair of its column chunk ----
-    // (atom1 < atom2 as the reference orders a pair, CharmmReferenceGBMV.cpp:279-286;
-    // the energy is counted by atom1's thread)
+    // ---- pairs: 64 x 64 tiles (row block I, column block J >= I), each pair once with atom1
+    // the row atom (atom1 < atom2 as the reference orders a pair, CharmmReferenceGBMV.cpp:
+    // 279-286); column sums rotate through the lanes (DPP), row and column partials per tile
+    // are added per atom in a fixed order by cg_chain ----
     std::string pair_excl, pair_noexcl;
     for (size_t t = 0; t < spec.term_exprs.size(); ++t) {
         if (spec.term_types[t] == TERM_SINGLE) continue;
-        const std::string e = "    E_ = E_ + (" + translate_expression(spec.term_exprs[t], spec, 2, 0) + ");\n";
+        const std::string e = "            E_ = E_ + (" + translate_expression(spec.term_exprs[t], spec, 2, 0) + ");\n";
         (spec.term_types[t] == TERM_PAIR ? pair_excl : pair_noexcl) += e;
         out.any_pair = true;
     }
-    s << "extern \"C\" __global__ void cg_pairs(int N, int chunk, const double* __restrict__ pos, const double* "
-         "__restrict__ prm, const double* __restrict__ vals, const double* __restrict__ glob, int method, double "
-         "cut2, const double* __restrict__ box, const int* __restrict__ ex_ptr, const int* __restrict__ ex_idx, "
+    s << "#define NPR (4 + NV)\n#define NPC (3 + NV)\n"
+         "__device__ inline double rot_down(double v) {\n"
+         "    const long long b = __builtin_bit_cast(long long, v);\n"
+         "    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x134, 0xF, 0xF, false);\n"
+         "    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x134, 0xF, 0xF, false);\n"
+         "    return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);\n}\n"
+         "__device__ inline long long tile_index(int I, int J, int nb) {\n"
+         "    return (long long)I * nb - (long long)I * (I - 1) / 2 + (J - I);\n}\n";
+    s << "extern \"C\" __global__ void __launch_bounds__(64) cg_pairs(int N, int nb, const double* __restrict__ pos, "
+         "const double* __restrict__ prm, const double* __restrict__ vals, const double* __restrict__ glob, int method, "
+         "double cut2, const double* __restrict__ box, const int* __restrict__ ex_ptr, const int* __restrict__ ex_idx, "
          "double* __restrict__ part) {\n"
-         "    const int i = blockIdx.x * blockDim.x + threadIdx.x;\n    if (i >= N) return;\n"
-         "    const int j0 = blockIdx.y * chunk, j1 = min(N, j0 + chunk);\n"
+         "    const int J = blockIdx.x, I = blockIdx.y;\n    if (J < I) return;\n"
+         "    const int lane = threadIdx.x;\n"
+         "    __shared__ double s_x[64], s_y[64], s_z[64];\n"
+         "    __shared__ double s_p[NP][64];\n"
+         "    __shared__ double s_v[NV > 0 ? NV : 1][64];\n"
+         "    const int i = 64 * I + lane, jl = 64 * J + lane;\n"
+         "    const bool iv = i < N, jv = jl < N;\n"
+         "    s_x[lane] = jv ? pos[3 * jl] : 0.0;\n    s_y[lane] = jv ? pos[3 * jl + 1] : 0.0;\n"
+         "    s_z[lane] = jv ? pos[3 * jl + 2] : 0.0;\n"
+         "    for (int k = 0; k < NP; ++k) s_p[k][lane] = jv ? prm[(size_t)jl * NP + k] : 0.0;\n"
+         "    for (int k = 0; k < NV; ++k) s_v[k][lane] = jv ? vals[k * N + jl] : 0.0;\n"
+         "    __syncthreads();\n"
          "    typedef Du<1 + 2 * NV> D;\n";
     for (int k = 0; k < NG; ++k) s << "    const D G" << k << " = D::c(glob[" << k << "]);\n";
-    s << "    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];\n"
-         "    double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;\n"
-         "    double dv[NV > 0 ? NV : 1];\n    for (int k = 0; k < NV; ++k) dv[k] = 0.0;\n"
-         "    int ep = ex_ptr[i];\n    const int ee = ex_ptr[i + 1];\n"
-         "    for (int j = j0; j < j1; ++j) {\n"
-         "        while (ep < ee && ex_idx[ep] < j) ++ep;\n"
-         "        const bool excluded = ep < ee && ex_idx[ep] == j;\n"
-         "        if (j == i) continue;\n"
-         "        double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;\n"
-         "        if (method == 2) {\n"
-         "            dx -= box[0] * floor(dx / box[0] + 0.5);\n"
-         "            dy -= box[1] * floor(dy / box[1] + 0.5);\n"
-         "            dz -= box[2] * floor(dz / box[2] + 0.5);\n"
+    s << "    const double xi = iv ? pos[3 * i] : 0.0, yi = iv ? pos[3 * i + 1] : 0.0, zi = iv ? pos[3 * i + 2] : 0.0;\n";
+    for (int k = 0; k < NP; ++k) s << "    const D P1_" << k << " = D::c(iv ? prm[(size_t)i * NP + " << k << "] : 0.0);\n";
+    for (int k = 0; k < NV; ++k) s << "    const D V1_" << k << " = D::var(iv ? vals[" << k << " * N + i] : 0.0, " << 1 + k << ");\n";
+    s << "    double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;\n"
+         "    double dv1[NV > 0 ? NV : 1], cv[NV > 0 ? NV : 1];\n"
+         "    for (int k = 0; k < NV; ++k) dv1[k] = cv[k] = 0.0;\n"
+         "    const bool has_ex = ex_ptr[N] > 0;\n"
+         "    for (int m = 0; m < 64; ++m) {\n"
+         "        const int c = (lane + m) & 63;\n"
+         "        const int j = 64 * J + c;\n"
+         "        bool ok = iv && j < N && j > i;\n"
+         "        double dx = 0.0, dy = 0.0, dz = 0.0, r2 = 1.0;\n"
+         "        if (ok) {\n"
+         "            dx = s_x[c] - xi;\n            dy = s_y[c] - yi;\n            dz = s_z[c] - zi;\n"
+         "            if (method == 2) {\n"
+         "                dx -= box[0] * floor(dx / box[0] + 0.5);\n"
+         "                dy -= box[1] * floor(dy / box[1] + 0.5);\n"
+         "                dz -= box[2] * floor(dz / box[2] + 0.5);\n"
+         "            }\n"
+         "            r2 = dx * dx + dy * dy + dz * dz;\n"
+         "            if (method != 0 && !(r2 < cut2)) ok = false;\n"
          "        }\n"
-         "        const double r2 = dx * dx + dy * dy + dz * dz;\n"
-         "        if (method != 0 && !(r2 < cut2)) continue;\n"
-         "        const int a1 = i < j ? i : j, a2 = i < j ? j : i;\n"
-         "        const double rr = sqrt(r2);\n"
-         "        const D r_ = D::var(rr, 0);\n";
-    for (int k = 0; k < NP; ++k) {
-        s << "        const D P1_" << k << " = D::c(prm[(size_t)a1 * NP + " << k << "]);\n";
-        s << "        const D P2_" << k << " = D::c(prm[(size_t)a2 * NP + " << k << "]);\n";
-    }
-    for (int k = 0; k < NV; ++k) {
-        s << "        const D V1_" << k << " = D::var(vals[" << k << " * N + a1], " << 1 + k << ");\n";
-        s << "        const D V2_" << k << " = D::var(vals[" << k << " * N + a2], " << 1 + NV + k << ");\n";
-    }
-    s << "        D E_ = D::c(0.0);\n" << pair_noexcl << "        if (!excluded) {\n" << pair_excl << "        }\n"
-      << "        if (i < j) e += E_.v;\n"
-         "        const double g = E_.d[0] / rr;\n"
-         "        fx += g * dx;\n        fy += g * dy;\n        fz += g * dz;\n"
-         "        for (int k = 0; k < NV; ++k) dv[k] += i < j ? E_.d[1 + k] : E_.d[1 + NV + k];\n"
+         "        if (ok) {\n"
+         "            bool excluded = false;\n"
+         "            if (has_ex) {\n"
+         "                int lo = ex_ptr[i], hi = ex_ptr[i + 1];\n"
+         "                const int end = hi;\n"
+         "                while (lo < hi) {\n"
+         "                    const int mid = (lo + hi) >> 1;\n"
+         "                    if (ex_idx[mid] < j) lo = mid + 1; else hi = mid;\n"
+         "                }\n"
+         "                excluded = lo < end && ex_idx[lo] == j;\n"
+         "            }\n"
+         "            const double rr = sqrt(r2);\n"
+         "            const D r_ = D::var(rr, 0);\n";
+    for (int k = 0; k < NP; ++k) s << "            const D P2_" << k << " = D::c(s_p[" << k << "][c]);\n";
+    for (int k = 0; k < NV; ++k) s << "            const D V2_" << k << " = D::var(s_v[" << k << "][c], " << 1 + NV + k << ");\n";
+    s << "            D E_ = D::c(0.0);\n" << pair_noexcl << "            if (!excluded) {\n" << pair_excl
+      << "            }\n"
+         "            e += E_.v;\n"
+         "            const double g = E_.d[0] / rr;\n"
+         "            fx += g * dx;\n            fy += g * dy;\n            fz += g * dz;\n"
+         "            cx -= g * dx;\n            cy -= g * dy;\n            cz -= g * dz;\n"
+         "            for (int k = 0; k < NV; ++k) {\n"
+         "                dv1[k] += E_.d[1 + k];\n                cv[k] += E_.d[1 + NV + k];\n            }\n"
+         "        }\n"
+         "        cx = rot_down(cx);\n        cy = rot_down(cy);\n        cz = rot_down(cz);\n"
+         "        for (int k = 0; k < NV; ++k) cv[k] = rot_down(cv[k]);\n"
          "    }\n"
-         "    double* o = part + (size_t)blockIdx.y * NPL * N;\n"
-         "    o[i] = fx;\n    o[N + i] = fy;\n    o[2 * N + i] = fz;\n    o[3 * N + i] = e;\n"
-         "    for (int k = 0; k < NV; ++k) o[(4 + k) * N + i] = dv[k];\n}\n";
+         "    double* o = part + tile_index(I, J, nb) * (NPR + NPC) * 64;\n"
+         "    o[lane] = fx;\n    o[64 + lane] = fy;\n    o[128 + lane] = fz;\n    o[192 + lane] = e;\n"
+         "    for (int k = 0; k < NV; ++k) o[(4 + k) * 64 + lane] = dv1[k];\n"
+         "    o[NPR * 64 + lane] = cx;\n    o[(NPR + 1) * 64 + lane] = cy;\n    o[(NPR + 2) * 64 + lane] = cz;\n"
+         "    for (int k = 0; k < NV; ++k) o[(NPR + 3 + k) * 64 + lane] = cv[k];\n}\n";
 
-    // ---- per atom: pair sums in chunk order, energy, pair forces, dE/dI ----
-    s << "extern \"C\" __global__ void cg_chain(int N, int chunks, const double* __restrict__ part, const double* "
+    // ---- per atom: the tiles' row and column partials in a fixed order, energy, pair
+    // forces, dE/dI ----
+    s << "extern \"C\" __global__ void cg_chain(int N, int nb, const double* __restrict__ part, const double* "
          "__restrict__ dvdi, const double* __restrict__ esp, const double* __restrict__ despdi, double* __restrict__ "
          "atom_energy, long long* __restrict__ force_fix, double* __restrict__ dedi) {\n"
-         "    const int i = blockIdx.x * blockDim.x + threadIdx.x;\n    if (i >= N) return;\n"
-         "    double a[NPL];\n    for (int p = 0; p < NPL; ++p) a[p] = 0.0;\n"
-         "    for (int c = 0; c < chunks; ++c)\n"
-         "        for (int p = 0; p < NPL; ++p) a[p] += part[((size_t)c * NPL + p) * N + i];\n"
-         "    atom_energy[i] = esp[i] + a[3];\n"
-         "    force_fix[i] = (long long)llrint(a[0] * 4294967296.0);\n"
-         "    force_fix[N + i] = (long long)llrint(a[1] * 4294967296.0);\n"
-         "    force_fix[2 * N + i] = (long long)llrint(a[2] * 4294967296.0);\n"
+         "    const int a = blockIdx.x * blockDim.x + threadIdx.x;\n    if (a >= N) return;\n"
+         "    const int I = a / 64, l = a - 64 * (a / 64);\n"
+         "    double r[NPR], c[NPC];\n"
+         "    for (int p = 0; p < NPR; ++p) r[p] = 0.0;\n    for (int p = 0; p < NPC; ++p) c[p] = 0.0;\n"
+         "    for (int J = I; J < nb; ++J) {\n"
+         "        const double* o = part + tile_index(I, J, nb) * (NPR + NPC) * 64;\n"
+         "        for (int p = 0; p < NPR; ++p) r[p] += o[p * 64 + l];\n    }\n"
+         "    for (int K = 0; K <= I; ++K) {\n"
+         "        const double* o = part + tile_index(K, I, nb) * (NPR + NPC) * 64;\n"
+         "        for (int p = 0; p < NPC; ++p) c[p] += o[(NPR + p) * 64 + l];\n    }\n"
+         "    atom_energy[a] = esp[a] + r[3];\n"
+         "    force_fix[a] = (long long)llrint((r[0] + c[0]) * 4294967296.0);\n"
+         "    force_fix[N + a] = (long long)llrint((r[1] + c[1]) * 4294967296.0);\n"
+         "    force_fix[2 * N + a] = (long long)llrint((r[2] + c[2]) * 4294967296.0);\n"
          "    for (int m = 0; m < 2; ++m) {\n"
-         "        double g = despdi[m * N + i];\n"
-         "        for (int k = 0; k < NV; ++k) g += a[4 + k] * dvdi[(k * 2 + m) * N + i];\n"
-         "        dedi[m * N + i] = g;\n    }\n}\n";
+         "        double g = despdi[m * N + a];\n"
+         "        for (int k = 0; k < NV; ++k) g += (r[4 + k] + c[3 + k]) * dvdi[(k * 2 + m) * N + a];\n"
+         "        dedi[m * N + a] = g;\n    }\n}\n";
     out.source = s.str();
     out.n_params = NP;
     out.n_globals = NG;

@@ -15,7 +15,6 @@ namespace gbi {
 
 constexpr int kMaxCustomValues = 4;
 constexpr int kMaxCustomIntegrals = 2;
-constexpr int kCustomChunks = 32;  // column chunks of the pair kernel (partial sums added in order)
 
 enum CustomTermType : int { TERM_SINGLE = 0, TERM_PAIR = 1, TERM_PAIR_NO_EXCL = 2 };
 
@@ -30,11 +29,13 @@ struct CustomSpec {
 
 // Compiled kernels of one spec.  Kernel signatures (all arrays device, double unless noted):
 //   cg_values(int N, prm[N x NP], integ[2 x N], glob[NG], vals[NV x N], dvdi[NV x 2 x N],
-//             esp[N], despdi[2 x N])
-//   cg_pairs(int N, int chunk, pos[N x 3], prm, vals, glob, int method, double cut2,
-//            const double* box3, const int* ex_ptr, const int* ex_idx, part[chunks x NPL x N])
-//   cg_chain(int N, int chunks, part, dvdi, esp, despdi, atom_energy[N], long long* force_fix[3 x N],
-//            dedi[2 x N])
+//             esp[N], despdi[2 x N])                                  grid N/256, block 256
+//   cg_pairs(int N, int nb, pos[N x 3], prm, vals, glob, int method, double cut2,
+//            const double* box3, const int* ex_ptr, const int* ex_idx,
+//            part[nb (nb + 1) / 2 tiles x NPL x 64])                grid (nb, nb), block 64
+//   cg_chain(int N, int nb, part, dvdi, esp, despdi, atom_energy[N], long long* force_fix[3 x N],
+//            dedi[2 x N])                                            grid N/256, block 256
+// (nb = ceil(N / 64) atom blocks; NPL = n_planes planes per tile: 4 + NV row, 3 + NV column)
 struct CustomProgram {
     hipModule_t module = nullptr;
     hipFunction_t values = nullptr, pairs = nullptr, chain = nullptr;

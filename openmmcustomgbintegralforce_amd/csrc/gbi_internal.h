@@ -109,7 +109,7 @@ struct SysDesc {
 // (gbi_custom.h); host-side, read by launch_evaluate.
 struct CustomLaunch {
     hipFunction_t values = nullptr, pairs = nullptr, chain = nullptr;
-    int N = 0, chunk = 0, chunks = 0, method = 0;
+    int N = 0, nb = 0, method = 0;  // nb: 64-atom blocks
     double cut2 = 0.0;
     const double* box = nullptr;  // device: the system's 3 box edges
     const double* prm = nullptr;

@@ -1877,9 +1877,9 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
     hipLaunchKernelGGL(k_integrals, dim3(p.N), dim3(64), 0, stream, p);
     mark(4);
     if (p.custom) {
-        // general expressions (one system): values, pairs (column chunks), chain
+        // general expressions (one system): values, pair tiles, chain
         const CustomLaunch& c = *p.custom;
-        int N = c.N, chunk = c.chunk, chunks = c.chunks, method = c.method;
+        int N = c.N, nb = c.nb, method = c.method;
         double cut2 = c.cut2;
         const double* pos = p.pos;
         const double* integ = p.integrals;
@@ -1892,11 +1892,11 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
         void* va[] = {&N, &prm, &integ, &glob, &vals, &dvdi, &esp, &despdi};
         err = hipModuleLaunchKernel(c.values, (N + 255) / 256, 1, 1, 256, 1, 1, 0, stream, va, nullptr);
         if (err != hipSuccess) return err;
-        void* pa[] = {&N, &chunk, &pos, &prm, &vals, &glob, &method, &cut2, &box, &ex_ptr, &ex_idx, &part};
-        err = hipModuleLaunchKernel(c.pairs, (N + 63) / 64, chunks, 1, 64, 1, 1, 0, stream, pa, nullptr);
+        void* pa[] = {&N, &nb, &pos, &prm, &vals, &glob, &method, &cut2, &box, &ex_ptr, &ex_idx, &part};
+        err = hipModuleLaunchKernel(c.pairs, nb, nb, 1, 64, 1, 1, 0, stream, pa, nullptr);
         if (err != hipSuccess) return err;
         mark(5);
-        void* ca[] = {&N, &chunks, &part, &dvdi, &esp, &despdi, &atom_energy, &force_fix, &dedi};
+        void* ca[] = {&N, &nb, &part, &dvdi, &esp, &despdi, &atom_energy, &force_fix, &dedi};
         err = hipModuleLaunchKernel(c.chain, (N + 255) / 256, 1, 1, 256, 1, 1, 0, stream, ca, nullptr);
         if (err != hipSuccess) return err;
     } else if (adj) {
