@@ -9,9 +9,12 @@ HBM before the timed region starts.  The default workload is the 5,000-atom GBMV
 
 Prints one JSON line (rank 0).  Beside the throughput it reports:
 * force relative RMS and energy relative error against the FP64 oracle (same input);
-* ``roofline`` of the dominant kernel: algorithmic FLOPs per launch (interaction counts
-  x fixed weights, openmmcustomgbintegralforce_amd/flops.py) / its average duration from
-  HIP events on the library's stream, against the 157.3 TFLOP/s FP32 vector peak;
+* ``roofline`` of the dominant (longest) pass and ``rooflines`` of every modelled pass:
+  volume, chain-rule and pair passes as algorithmic FLOPs per launch (interaction counts x
+  fixed weights, openmmcustomgbintegralforce_amd/flops.py) against the 157.3 TFLOP/s FP32
+  vector peak, the binning pass as algorithmic bytes against the 8 TB/s HBM peak; durations
+  from HIP events on the library's stream (the binning pass includes its scan and worklist
+  launches); ``traffic`` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summaries;
 * ``cpu_baseline``: the oracle in the reference's loop structure (dense, FP64, 1 thread)
   timed on this host over a bounded sample.
 """
@@ -30,6 +33,7 @@ import numpy as np  # noqa: E402
 
 METRIC = "GB energy+force evals/s at 2.5k–25k atoms (1 GPU); force RMS err vs FP64"
 PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
 
 WORKLOADS = {
     "gbmv2-5000": dict(model="gbmv", integral=2, n_atoms=5000, n_lebedev=50, seed=0,
@@ -168,28 +172,44 @@ def main() -> int:
         n_vol_alg = n_vol_exact if n_frc is not None else None
 
     # with a cutoff the pair pass skips blocks: its work is not the all-pairs count
-    # (and generated kernels evaluate every pair from both sides in FP64 duals)
+    # (and generated kernels evaluate the pairs in FP64 dual numbers)
     kernel_flops = {} if p.nonbonded_method != N.NO_CUTOFF or w.get("expressions") else {
         "pairs": FL.pair_flops(n_atoms)}
     if n_vol_alg is not None:
         kernel_flops["volume"] = FL.volume_flops(kind, n_vol_alg, stats["n_points"])
     if n_frc is not None:
         kernel_flops["force"] = FL.force_flops(kind, n_frc)
-    dominant = max((k for k in ("volume", "force", "pairs")), key=lambda k: passes.get(k, 0.0))
-    roof = None
-    if dominant in kernel_flops and passes[dominant] > 0:
-        ach = kernel_flops[dominant] / (passes[dominant] * 1e-3) / 1e12
+    # every modelled pass against its roof; "roofline" is the one that takes the longest
+    rooflines = []
+    for k in ("volume", "force", "pairs"):
+        if k not in kernel_flops or passes.get(k, 0.0) <= 0:
+            continue
+        ach = kernel_flops[k] / (passes[k] * 1e-3) / 1e12
         traffic = None
-        tpath = os.path.join(ROOT, "profiles", f"traffic_{dominant}_{args.workload}.json")
+        tpath = os.path.join(ROOT, "profiles", f"traffic_{k}_{args.workload}.json")
         if os.path.exists(tpath):
             with open(tpath) as fh:
                 traffic = json.load(fh).get("hbm_bytes_per_launch")
         # compute-bound against the published FP32 peak; the per-pair math has no matrix shape,
         # so it runs on the FP32 vector ALU (recorded in "engine")
-        roof = {"bound": "mfma", "engine": "fp32 vector ALU", "kernel": dominant, "achieved": ach,
-                "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": ach / PEAK_FP32_TFLOPS, "traffic": traffic,
-                "alg_flops_per_launch": kernel_flops[dominant], "avg_ms": passes[dominant]}
+        rooflines.append({"bound": "mfma", "engine": "fp32 vector ALU", "kernel": k, "achieved": ach,
+                          "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP32_TFLOPS,
+                          "traffic": traffic, "alg_flops_per_launch": kernel_flops[k], "avg_ms": passes[k]})
+    if passes.get("bin", 0.0) > 0:
+        # binning moves bytes: per quadrature point a 16 B record (binned) or a 4 B volume
+        # (decided directly), plus the atom positions; it is bound by its atomics' latency
+        nq = stats["n_quad"] * n_atoms
+        bbytes = FL.bin_bytes(stats["n_points"], nq, n_atoms)
+        ach = bbytes / (passes["bin"] * 1e-3) / 1e9
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", f"traffic_bin_{args.workload}.json")
+        if os.path.exists(tpath):
+            with open(tpath) as fh:
+                traffic = json.load(fh).get("hbm_bytes_per_launch")
+        rooflines.append({"bound": "hbm", "kernel": "bin", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                          "frac": ach / PEAK_HBM_GBS, "traffic": traffic, "alg_bytes_per_launch": bbytes,
+                          "avg_ms": passes["bin"]})
+    roof = max(rooflines, key=lambda r: r["avg_ms"]) if rooflines else None
 
     # ---- CPU baseline: the oracle in the reference's loop structure, 1 thread ----
     cpu = None
@@ -227,6 +247,7 @@ def main() -> int:
                    "integral and energy sums, int64 fixed-point forces"},
         **acc,
         "roofline": roof,
+        "rooflines": rooflines,
         "cpu_baseline": cpu,
         "passes_ms": passes,
         "profiled_launches": n_prof,

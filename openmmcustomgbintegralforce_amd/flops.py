@@ -21,3 +21,10 @@ def force_flops(kind: str, n_frc_pairs: float) -> float:
 
 def pair_flops(n_atoms: int) -> float:
     return C_PAIR * n_atoms * (n_atoms - 1) / 2.0
+
+
+def bin_bytes(n_binned: float, n_quad_points: float, n_atoms: int) -> float:
+    """Algorithmic bytes of the binning pass: a 16 B record per binned point, a 4 B volume per
+    point decided directly (outside the grid, empty cell, buried in its owner), 24 B of
+    position per atom."""
+    return 16.0 * n_binned + 4.0 * (n_quad_points - n_binned) + 24.0 * n_atoms
