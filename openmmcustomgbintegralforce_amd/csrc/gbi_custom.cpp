@@ -208,66 +208,71 @@ bool valid_name(const std::string& n) {
 // The dual-number type and its functions (device source, prepended to the kernels).
 const char* kDualSource = R"(
 #pragma clang fp contract(off)
-template <int M> struct Du {
-    double v;
-    double d[M];
-    static __device__ inline Du c(double x) { Du r; r.v = x; for (int k = 0; k < M; ++k) r.d[k] = 0.0; return r; }
-    static __device__ inline Du var(double x, int slot) { Du r = c(x); r.d[slot] = 1.0; return r; }
+#define GBI_M1(fn) \
+    __device__ inline float m_##fn(float x) { return fn##f(x); } \
+    __device__ inline double m_##fn(double x) { return fn(x); }
+GBI_M1(sqrt) GBI_M1(exp) GBI_M1(log) GBI_M1(sin) GBI_M1(cos) GBI_M1(tan) GBI_M1(asin) GBI_M1(acos)
+GBI_M1(atan) GBI_M1(sinh) GBI_M1(cosh) GBI_M1(tanh) GBI_M1(erf) GBI_M1(erfc) GBI_M1(fabs)
+__device__ inline float m_pow(float a, float b) { return powf(a, b); }
+__device__ inline double m_pow(double a, double b) { return pow(a, b); }
+template <typename T, int M> struct Du {
+    T v;
+    T d[M];
+    static __device__ inline Du c(T x) { Du r; r.v = x; for (int k = 0; k < M; ++k) r.d[k] = T(0); return r; }
+    static __device__ inline Du var(T x, int slot) { Du r = c(x); r.d[slot] = T(1); return r; }
 };
-template <int M> __device__ inline Du<M> d_chain(const Du<M>& a, double f, double df) {
-    Du<M> r; r.v = f; for (int k = 0; k < M; ++k) r.d[k] = df * a.d[k]; return r;
+#define DU template <typename T, int M> __device__ inline Du<T, M>
+DU d_chain(const Du<T, M>& a, T f, T df) {
+    Du<T, M> r; r.v = f; for (int k = 0; k < M; ++k) r.d[k] = df * a.d[k]; return r;
 }
-template <int M> __device__ inline Du<M> operator+(const Du<M>& a, const Du<M>& b) {
-    Du<M> r; r.v = a.v + b.v; for (int k = 0; k < M; ++k) r.d[k] = a.d[k] + b.d[k]; return r;
+DU operator+(const Du<T, M>& a, const Du<T, M>& b) {
+    Du<T, M> r; r.v = a.v + b.v; for (int k = 0; k < M; ++k) r.d[k] = a.d[k] + b.d[k]; return r;
 }
-template <int M> __device__ inline Du<M> operator-(const Du<M>& a, const Du<M>& b) {
-    Du<M> r; r.v = a.v - b.v; for (int k = 0; k < M; ++k) r.d[k] = a.d[k] - b.d[k]; return r;
+DU operator-(const Du<T, M>& a, const Du<T, M>& b) {
+    Du<T, M> r; r.v = a.v - b.v; for (int k = 0; k < M; ++k) r.d[k] = a.d[k] - b.d[k]; return r;
 }
-template <int M> __device__ inline Du<M> operator-(const Du<M>& a) {
-    Du<M> r; r.v = -a.v; for (int k = 0; k < M; ++k) r.d[k] = -a.d[k]; return r;
+DU operator-(const Du<T, M>& a) {
+    Du<T, M> r; r.v = -a.v; for (int k = 0; k < M; ++k) r.d[k] = -a.d[k]; return r;
 }
-template <int M> __device__ inline Du<M> operator*(const Du<M>& a, const Du<M>& b) {
-    Du<M> r; r.v = a.v * b.v; for (int k = 0; k < M; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k]; return r;
+DU operator*(const Du<T, M>& a, const Du<T, M>& b) {
+    Du<T, M> r; r.v = a.v * b.v; for (int k = 0; k < M; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k]; return r;
 }
-template <int M> __device__ inline Du<M> operator/(const Du<M>& a, const Du<M>& b) {
-    Du<M> r; r.v = a.v / b.v; const double ib = 1.0 / b.v;
+DU operator/(const Du<T, M>& a, const Du<T, M>& b) {
+    Du<T, M> r; r.v = a.v / b.v; const T ib = T(1) / b.v;
     for (int k = 0; k < M; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib; return r;
 }
-template <int M> __device__ inline Du<M> d_sqrt(const Du<M>& a) { const double f = sqrt(a.v); return d_chain(a, f, 0.5 / f); }
-template <int M> __device__ inline Du<M> d_exp(const Du<M>& a) { const double f = exp(a.v); return d_chain(a, f, f); }
-template <int M> __device__ inline Du<M> d_log(const Du<M>& a) { return d_chain(a, log(a.v), 1.0 / a.v); }
-template <int M> __device__ inline Du<M> d_sin(const Du<M>& a) { return d_chain(a, sin(a.v), cos(a.v)); }
-template <int M> __device__ inline Du<M> d_cos(const Du<M>& a) { return d_chain(a, cos(a.v), -sin(a.v)); }
-template <int M> __device__ inline Du<M> d_tan(const Du<M>& a) { const double f = tan(a.v); return d_chain(a, f, 1.0 + f * f); }
-template <int M> __device__ inline Du<M> d_asin(const Du<M>& a) { return d_chain(a, asin(a.v), 1.0 / sqrt(1.0 - a.v * a.v)); }
-template <int M> __device__ inline Du<M> d_acos(const Du<M>& a) { return d_chain(a, acos(a.v), -1.0 / sqrt(1.0 - a.v * a.v)); }
-template <int M> __device__ inline Du<M> d_atan(const Du<M>& a) { return d_chain(a, atan(a.v), 1.0 / (1.0 + a.v * a.v)); }
-template <int M> __device__ inline Du<M> d_sinh(const Du<M>& a) { return d_chain(a, sinh(a.v), cosh(a.v)); }
-template <int M> __device__ inline Du<M> d_cosh(const Du<M>& a) { return d_chain(a, cosh(a.v), sinh(a.v)); }
-template <int M> __device__ inline Du<M> d_tanh(const Du<M>& a) { const double f = tanh(a.v); return d_chain(a, f, 1.0 - f * f); }
-template <int M> __device__ inline Du<M> d_erf(const Du<M>& a) {
-    return d_chain(a, erf(a.v), 1.1283791670955126 * exp(-a.v * a.v));
+DU d_sqrt(const Du<T, M>& a) { const T f = m_sqrt(a.v); return d_chain(a, f, T(0.5) / f); }
+DU d_exp(const Du<T, M>& a) { const T f = m_exp(a.v); return d_chain(a, f, f); }
+DU d_log(const Du<T, M>& a) { return d_chain(a, m_log(a.v), T(1) / a.v); }
+DU d_sin(const Du<T, M>& a) { return d_chain(a, m_sin(a.v), m_cos(a.v)); }
+DU d_cos(const Du<T, M>& a) { return d_chain(a, m_cos(a.v), -m_sin(a.v)); }
+DU d_tan(const Du<T, M>& a) { const T f = m_tan(a.v); return d_chain(a, f, T(1) + f * f); }
+DU d_asin(const Du<T, M>& a) { return d_chain(a, m_asin(a.v), T(1) / m_sqrt(T(1) - a.v * a.v)); }
+DU d_acos(const Du<T, M>& a) { return d_chain(a, m_acos(a.v), T(-1) / m_sqrt(T(1) - a.v * a.v)); }
+DU d_atan(const Du<T, M>& a) { return d_chain(a, m_atan(a.v), T(1) / (T(1) + a.v * a.v)); }
+DU d_sinh(const Du<T, M>& a) { return d_chain(a, m_sinh(a.v), m_cosh(a.v)); }
+DU d_cosh(const Du<T, M>& a) { return d_chain(a, m_cosh(a.v), m_sinh(a.v)); }
+DU d_tanh(const Du<T, M>& a) { const T f = m_tanh(a.v); return d_chain(a, f, T(1) - f * f); }
+DU d_erf(const Du<T, M>& a) { return d_chain(a, m_erf(a.v), T(1.1283791670955126) * m_exp(-a.v * a.v)); }
+DU d_erfc(const Du<T, M>& a) { return d_chain(a, m_erfc(a.v), T(-1.1283791670955126) * m_exp(-a.v * a.v)); }
+DU d_abs(const Du<T, M>& a) { return d_chain(a, m_fabs(a.v), a.v < T(0) ? T(-1) : T(1)); }
+DU d_step(const Du<T, M>& a) { return Du<T, M>::c(a.v >= T(0) ? T(1) : T(0)); }
+DU d_delta(const Du<T, M>& a) { return Du<T, M>::c(a.v == T(0) ? T(1) : T(0)); }
+DU d_square(const Du<T, M>& a) { return d_chain(a, a.v * a.v, T(2) * a.v); }
+DU d_cube(const Du<T, M>& a) { return d_chain(a, a.v * a.v * a.v, T(3) * a.v * a.v); }
+DU d_recip(const Du<T, M>& a) { return d_chain(a, T(1) / a.v, T(-1) / (a.v * a.v)); }
+DU d_min(const Du<T, M>& a, const Du<T, M>& b) { return b.v < a.v ? b : a; }
+DU d_max(const Du<T, M>& a, const Du<T, M>& b) { return b.v > a.v ? b : a; }
+DU d_select(const Du<T, M>& x, const Du<T, M>& a, const Du<T, M>& b) { return x.v != T(0) ? a : b; }
+DU d_powc(const Du<T, M>& a, double e) {
+    const T te = T(e);
+    return d_chain(a, m_pow(a.v, te), te == T(0) ? T(0) : te * m_pow(a.v, te - T(1)));
 }
-template <int M> __device__ inline Du<M> d_erfc(const Du<M>& a) {
-    return d_chain(a, erfc(a.v), -1.1283791670955126 * exp(-a.v * a.v));
-}
-template <int M> __device__ inline Du<M> d_abs(const Du<M>& a) { return d_chain(a, fabs(a.v), a.v < 0.0 ? -1.0 : 1.0); }
-template <int M> __device__ inline Du<M> d_step(const Du<M>& a) { return Du<M>::c(a.v >= 0.0 ? 1.0 : 0.0); }
-template <int M> __device__ inline Du<M> d_delta(const Du<M>& a) { return Du<M>::c(a.v == 0.0 ? 1.0 : 0.0); }
-template <int M> __device__ inline Du<M> d_square(const Du<M>& a) { return d_chain(a, a.v * a.v, 2.0 * a.v); }
-template <int M> __device__ inline Du<M> d_cube(const Du<M>& a) { return d_chain(a, a.v * a.v * a.v, 3.0 * a.v * a.v); }
-template <int M> __device__ inline Du<M> d_recip(const Du<M>& a) { return d_chain(a, 1.0 / a.v, -1.0 / (a.v * a.v)); }
-template <int M> __device__ inline Du<M> d_min(const Du<M>& a, const Du<M>& b) { return b.v < a.v ? b : a; }
-template <int M> __device__ inline Du<M> d_max(const Du<M>& a, const Du<M>& b) { return b.v > a.v ? b : a; }
-template <int M> __device__ inline Du<M> d_select(const Du<M>& x, const Du<M>& a, const Du<M>& b) { return x.v != 0.0 ? a : b; }
-template <int M> __device__ inline Du<M> d_powc(const Du<M>& a, double e) {
-    return d_chain(a, pow(a.v, e), e == 0.0 ? 0.0 : e * pow(a.v, e - 1.0));
-}
-template <int M> __device__ inline Du<M> d_pow(const Du<M>& a, const Du<M>& b) {
-    const double f = pow(a.v, b.v);
-    const double da = b.v == 0.0 ? 0.0 : b.v * pow(a.v, b.v - 1.0);
-    Du<M> r; r.v = f;
-    for (int k = 0; k < M; ++k) r.d[k] = da * a.d[k] + (b.d[k] != 0.0 ? f * log(a.v) * b.d[k] : 0.0);
+DU d_pow(const Du<T, M>& a, const Du<T, M>& b) {
+    const T f = m_pow(a.v, b.v);
+    const T da = b.v == T(0) ? T(0) : b.v * m_pow(a.v, b.v - T(1));
+    Du<T, M> r; r.v = f;
+    for (int k = 0; k < M; ++k) r.d[k] = da * a.d[k] + (b.d[k] != T(0) ? f * m_log(a.v) * b.d[k] : T(0));
     return r;
 }
 )";
@@ -335,7 +340,7 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
          "integ, const double* __restrict__ glob, double* __restrict__ vals, double* __restrict__ dvdi, double* "
          "__restrict__ esp, double* __restrict__ despdi) {\n"
          "    const int i = blockIdx.x * blockDim.x + threadIdx.x;\n    if (i >= N) return;\n"
-         "    typedef Du<2> D;\n";
+         "    typedef Du<double, 2> D;\n";
     for (int k = 0; k < NG; ++k) s << "    const D G" << k << " = D::c(glob[" << k << "]);\n";
     for (int k = 0; k < NP; ++k) s << "    const D P_" << k << " = D::c(prm[(size_t)i * NP + " << k << "]);\n";
     for (int k = 0; k < NI; ++k) s << "    const D I_" << k << " = D::var(integ[" << k << " * N + i], " << k << ");\n";
@@ -353,8 +358,12 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
 
     // ---- pairs: 64 x 64 tiles (row block I, column block J >= I), each pair once with atom1
     // the row atom (atom1 < atom2 as the reference orders a pair, CharmmReferenceGBMV.cpp:
-    // 279-286); column sums rotate through the lanes (DPP), row and column partials per tile
-    // are added per atom in a fixed order by cg_chain ----
+    // 279-286).  The pair difference, r^2, the minimum image and the cutoff test are FP64;
+    // r and the expressions (FP32 duals) are FP32, as are a wave's partial sums over its
+    // CG_STEPS column steps (the tile's 64 steps are split over CG_WAVES waves; within a wave
+    // the column sums rotate through the lanes by DPP).  The waves' row and column partials
+    // are added in FP64 in LDS in wave order, and cg_chain adds the tiles' partials per atom
+    // in FP64 in a fixed order ----
     std::string pair_excl, pair_noexcl;
     for (size_t t = 0; t < spec.term_exprs.size(); ++t) {
         if (spec.term_types[t] == TERM_SINGLE) continue;
@@ -362,40 +371,41 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
         (spec.term_types[t] == TERM_PAIR ? pair_excl : pair_noexcl) += e;
         out.any_pair = true;
     }
-    s << "#define NPR (4 + NV)\n#define NPC (3 + NV)\n"
-         "__device__ inline double rot_down(double v) {\n"
-         "    const long long b = __builtin_bit_cast(long long, v);\n"
-         "    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x134, 0xF, 0xF, false);\n"
-         "    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x134, 0xF, 0xF, false);\n"
-         "    return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);\n}\n"
+    s << "#define NPR (4 + NV)\n#define NPC (3 + NV)\n#define CG_WAVES " << kCustomPairWaves << "\n"
+         "#define CG_STEPS (64 / CG_WAVES)\n"
+         "__device__ inline float rot_down(float v) {\n"
+         "    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));\n}\n"
          "__device__ inline long long tile_index(int I, int J, int nb) {\n"
          "    return (long long)I * nb - (long long)I * (I - 1) / 2 + (J - I);\n}\n";
-    s << "extern \"C\" __global__ void __launch_bounds__(64) cg_pairs(int N, int nb, const double* __restrict__ pos, "
+    s << "extern \"C\" __global__ void __launch_bounds__(64 * CG_WAVES) cg_pairs(int N, int nb, const double* __restrict__ pos, "
          "const double* __restrict__ prm, const double* __restrict__ vals, const double* __restrict__ glob, int method, "
          "double cut2, const double* __restrict__ box, const int* __restrict__ ex_ptr, const int* __restrict__ ex_idx, "
          "double* __restrict__ part) {\n"
          "    const int J = blockIdx.x, I = blockIdx.y;\n    if (J < I) return;\n"
-         "    const int lane = threadIdx.x;\n"
+         "    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;\n"
          "    __shared__ double s_x[64], s_y[64], s_z[64];\n"
-         "    __shared__ double s_p[NP][64];\n"
-         "    __shared__ double s_v[NV > 0 ? NV : 1][64];\n"
+         "    __shared__ double s_red[CG_WAVES][NPR + NPC][64];\n"
+         "    __shared__ float s_p[NP][64];\n"
+         "    __shared__ float s_v[NV > 0 ? NV : 1][64];\n"
          "    const int i = 64 * I + lane, jl = 64 * J + lane;\n"
          "    const bool iv = i < N, jv = jl < N;\n"
-         "    s_x[lane] = jv ? pos[3 * jl] : 0.0;\n    s_y[lane] = jv ? pos[3 * jl + 1] : 0.0;\n"
-         "    s_z[lane] = jv ? pos[3 * jl + 2] : 0.0;\n"
-         "    for (int k = 0; k < NP; ++k) s_p[k][lane] = jv ? prm[(size_t)jl * NP + k] : 0.0;\n"
-         "    for (int k = 0; k < NV; ++k) s_v[k][lane] = jv ? vals[k * N + jl] : 0.0;\n"
+         "    if (w == 0) {\n"
+         "        s_x[lane] = jv ? pos[3 * jl] : 0.0;\n        s_y[lane] = jv ? pos[3 * jl + 1] : 0.0;\n"
+         "        s_z[lane] = jv ? pos[3 * jl + 2] : 0.0;\n"
+         "        for (int k = 0; k < NP; ++k) s_p[k][lane] = jv ? (float)prm[(size_t)jl * NP + k] : 0.0f;\n"
+         "        for (int k = 0; k < NV; ++k) s_v[k][lane] = jv ? (float)vals[k * N + jl] : 0.0f;\n"
+         "    }\n"
          "    __syncthreads();\n"
-         "    typedef Du<1 + 2 * NV> D;\n";
+         "    typedef Du<float, 1 + 2 * NV> D;\n";
     for (int k = 0; k < NG; ++k) s << "    const D G" << k << " = D::c(glob[" << k << "]);\n";
     s << "    const double xi = iv ? pos[3 * i] : 0.0, yi = iv ? pos[3 * i + 1] : 0.0, zi = iv ? pos[3 * i + 2] : 0.0;\n";
     for (int k = 0; k < NP; ++k) s << "    const D P1_" << k << " = D::c(iv ? prm[(size_t)i * NP + " << k << "] : 0.0);\n";
     for (int k = 0; k < NV; ++k) s << "    const D V1_" << k << " = D::var(iv ? vals[" << k << " * N + i] : 0.0, " << 1 + k << ");\n";
-    s << "    double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;\n"
-         "    double dv1[NV > 0 ? NV : 1], cv[NV > 0 ? NV : 1];\n"
-         "    for (int k = 0; k < NV; ++k) dv1[k] = cv[k] = 0.0;\n"
+    s << "    float fx = 0.f, fy = 0.f, fz = 0.f, e = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;\n"
+         "    float dv1[NV > 0 ? NV : 1], cv[NV > 0 ? NV : 1];\n"
+         "    for (int k = 0; k < NV; ++k) dv1[k] = cv[k] = 0.f;\n"
          "    const bool has_ex = ex_ptr[N] > 0;\n"
-         "    for (int m = 0; m < 64; ++m) {\n"
+         "    for (int m = CG_STEPS * w; m < CG_STEPS * (w + 1); ++m) {\n"
          "        const int c = (lane + m) & 63;\n"
          "        const int j = 64 * J + c;\n"
          "        bool ok = iv && j < N && j > i;\n"
@@ -421,27 +431,35 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
          "                }\n"
          "                excluded = lo < end && ex_idx[lo] == j;\n"
          "            }\n"
-         "            const double rr = sqrt(r2);\n"
+         "            const float rr = sqrtf((float)r2);\n"
          "            const D r_ = D::var(rr, 0);\n";
     for (int k = 0; k < NP; ++k) s << "            const D P2_" << k << " = D::c(s_p[" << k << "][c]);\n";
     for (int k = 0; k < NV; ++k) s << "            const D V2_" << k << " = D::var(s_v[" << k << "][c], " << 1 + NV + k << ");\n";
     s << "            D E_ = D::c(0.0);\n" << pair_noexcl << "            if (!excluded) {\n" << pair_excl
       << "            }\n"
          "            e += E_.v;\n"
-         "            const double g = E_.d[0] / rr;\n"
-         "            fx += g * dx;\n            fy += g * dy;\n            fz += g * dz;\n"
-         "            cx -= g * dx;\n            cy -= g * dy;\n            cz -= g * dz;\n"
+         "            const float g = E_.d[0] / rr, gx = g * (float)dx, gy = g * (float)dy, gz = g * (float)dz;\n"
+         "            fx += gx;\n            fy += gy;\n            fz += gz;\n"
+         "            cx -= gx;\n            cy -= gy;\n            cz -= gz;\n"
          "            for (int k = 0; k < NV; ++k) {\n"
          "                dv1[k] += E_.d[1 + k];\n                cv[k] += E_.d[1 + NV + k];\n            }\n"
          "        }\n"
          "        cx = rot_down(cx);\n        cy = rot_down(cy);\n        cz = rot_down(cz);\n"
          "        for (int k = 0; k < NV; ++k) cv[k] = rot_down(cv[k]);\n"
          "    }\n"
+         "    // after its steps, lane l holds the sums of column (l + CG_STEPS (w + 1)) mod 64\n"
+         "    const int cl = (lane + CG_STEPS * (w + 1)) & 63;\n"
+         "    s_red[w][0][lane] = fx;\n    s_red[w][1][lane] = fy;\n    s_red[w][2][lane] = fz;\n"
+         "    s_red[w][3][lane] = e;\n"
+         "    for (int k = 0; k < NV; ++k) s_red[w][4 + k][lane] = dv1[k];\n"
+         "    s_red[w][NPR][cl] = cx;\n    s_red[w][NPR + 1][cl] = cy;\n    s_red[w][NPR + 2][cl] = cz;\n"
+         "    for (int k = 0; k < NV; ++k) s_red[w][NPR + 3 + k][cl] = cv[k];\n"
+         "    __syncthreads();\n"
          "    double* o = part + tile_index(I, J, nb) * (NPR + NPC) * 64;\n"
-         "    o[lane] = fx;\n    o[64 + lane] = fy;\n    o[128 + lane] = fz;\n    o[192 + lane] = e;\n"
-         "    for (int k = 0; k < NV; ++k) o[(4 + k) * 64 + lane] = dv1[k];\n"
-         "    o[NPR * 64 + lane] = cx;\n    o[(NPR + 1) * 64 + lane] = cy;\n    o[(NPR + 2) * 64 + lane] = cz;\n"
-         "    for (int k = 0; k < NV; ++k) o[(NPR + 3 + k) * 64 + lane] = cv[k];\n}\n";
+         "    for (int t = threadIdx.x; t < (NPR + NPC) * 64; t += 64 * CG_WAVES) {\n"
+         "        double acc = s_red[0][t >> 6][t & 63];\n"
+         "        for (int v = 1; v < CG_WAVES; ++v) acc += s_red[v][t >> 6][t & 63];\n"
+         "        o[t] = acc;\n    }\n}\n";
 
     // ---- per atom: the tiles' row and column partials in a fixed order, energy, pair
     // forces, dE/dI ----

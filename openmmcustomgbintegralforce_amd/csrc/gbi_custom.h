@@ -15,6 +15,7 @@ namespace gbi {
 
 constexpr int kMaxCustomValues = 4;
 constexpr int kMaxCustomIntegrals = 2;
+constexpr int kCustomPairWaves = 4;  // waves of a cg_pairs workgroup (one 64 x 64 tile)
 
 enum CustomTermType : int { TERM_SINGLE = 0, TERM_PAIR = 1, TERM_PAIR_NO_EXCL = 2 };
 
@@ -32,7 +33,7 @@ struct CustomSpec {
 //             esp[N], despdi[2 x N])                                  grid N/256, block 256
 //   cg_pairs(int N, int nb, pos[N x 3], prm, vals, glob, int method, double cut2,
 //            const double* box3, const int* ex_ptr, const int* ex_idx,
-//            part[nb (nb + 1) / 2 tiles x NPL x 64])                grid (nb, nb), block 64
+//            part[nb (nb + 1) / 2 tiles x NPL x 64])     grid (nb, nb), block 64 kCustomPairWaves
 //   cg_chain(int N, int nb, part, dvdi, esp, despdi, atom_energy[N], long long* force_fix[3 x N],
 //            dedi[2 x N])                                            grid N/256, block 256
 // (nb = ceil(N / 64) atom blocks; NPL = n_planes planes per tile: 4 + NV row, 3 + NV column)

@@ -1893,7 +1893,7 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
         err = hipModuleLaunchKernel(c.values, (N + 255) / 256, 1, 1, 256, 1, 1, 0, stream, va, nullptr);
         if (err != hipSuccess) return err;
         void* pa[] = {&N, &nb, &pos, &prm, &vals, &glob, &method, &cut2, &box, &ex_ptr, &ex_idx, &part};
-        err = hipModuleLaunchKernel(c.pairs, nb, nb, 1, 64, 1, 1, 0, stream, pa, nullptr);
+        err = hipModuleLaunchKernel(c.pairs, nb, nb, 1, c.pair_threads, 1, 1, 0, stream, pa, nullptr);
         if (err != hipSuccess) return err;
         mark(5);
         void* ca[] = {&N, &nb, &part, &dvdi, &esp, &despdi, &atom_energy, &force_fix, &dedi};
