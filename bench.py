@@ -61,8 +61,9 @@ WORKLOADS = {
                                        "lookup grid 0.15 nm"),
     "gbmv2-5000-expressions": dict(model="gbmv", integral=2, n_atoms=5000, n_lebedev=50, seed=0, expressions=True,
                                    desc="CharmmGBMVForce GBMVIntegralTypeII, 5,000-atom globule, 24x50 quadrature, "
-                                        "the canonical GB forms given as general expressions: value, single-particle "
-                                        "and pair kernels generated over FP64 dual numbers and compiled with hiprtc"),
+                                        "the canonical GB forms given as general expressions: value and single-particle "
+                                        "kernel over FP64 dual numbers, pair kernel over FP32 dual numbers, generated "
+                                        "and compiled with hiprtc"),
     "batch64": dict(model="batch", integral=-1, n_atoms=0, n_lebedev=0, seed=0,
                     desc="64 systems in one batched context, N ~ U[300, 6000] (seeded), alternating "
                          "CharmmGBSWForce 24x38 and CharmmGBMVForce GBMVIntegralTypeII 24x50, hipGraph replay; "
@@ -172,7 +173,7 @@ def main() -> int:
         n_vol_alg = n_vol_exact if n_frc is not None else None
 
     # with a cutoff the pair pass skips blocks: its work is not the all-pairs count
-    # (and generated kernels evaluate the pairs in FP64 dual numbers)
+    # (and the generated pair kernel's dual-number work is not the hand-written pass's count)
     kernel_flops = {} if p.nonbonded_method != N.NO_CUTOFF or w.get("expressions") else {
         "pairs": FL.pair_flops(n_atoms)}
     if n_vol_alg is not None:
