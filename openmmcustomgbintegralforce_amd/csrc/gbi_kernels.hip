@@ -554,7 +554,7 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
 }
 
 // ---------------------------------------------------------------------------
-// Exclusive scan of int arrays (three passes, deterministic)
+// Exclusive scan of int arrays (two passes, deterministic)
 // ---------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int* in, int* out, int n, int* block_sums) {
@@ -587,43 +587,27 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int* in, int*
     }
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_scan_top(int* block_sums, int nblocks, int* out, int n, int limit,
+// Adds to tile t the sum of the tile sums before it (each workgroup sums them itself: at
+// most kScanTile ints, exact in any order); the last tile also writes the total.
+__global__ void __launch_bounds__(kScanThreads) k_scan_add(int* out, int n, const int* block_sums, int limit,
                                                            int* flags, int flag) {
     __shared__ int s_wave[kScanThreads / kWave];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
-    int v[kScanItems];
-    int tsum = 0;
-    for (int k = 0; k < kScanItems; ++k) {
-        const int idx = tid * kScanItems + k;
-        v[k] = idx < nblocks ? block_sums[idx] : 0;
-        tsum += v[k];
-    }
-    const int incl = wave_scan_incl(tsum, lane);
-    if (lane == kWave - 1) s_wave[wid] = incl;
+    const int t = blockIdx.x;
+    int part = 0;
+    for (int k = tid; k < t; k += kScanThreads) part += block_sums[k];
+    const int wsum = wave_scan_incl(part, lane);
+    if (lane == kWave - 1) s_wave[wid] = wsum;
     __syncthreads();
-    if (wid == 0) {
-        const int nw = kScanThreads / kWave;
-        int w = lane < nw ? s_wave[lane] : 0;
-        const int wi = wave_scan_incl(w, lane);
-        if (lane < nw) s_wave[lane] = wi - w;
-        if (lane == nw - 1) {
-            out[n] = wi;
-            if (flags && wi > limit) atomicOr(flags, flag);
-        }
+    int add = 0;
+    for (int w = 0; w < kScanThreads / kWave; ++w) add += s_wave[w];
+    if (t == (int)gridDim.x - 1 && tid == 0) {
+        const int total = add + block_sums[t];
+        out[n] = total;
+        if (flags && total > limit) atomicOr(flags, flag);
     }
-    __syncthreads();
-    int run = s_wave[wid] + incl - tsum;
-    for (int k = 0; k < kScanItems; ++k) {
-        const int idx = tid * kScanItems + k;
-        if (idx < nblocks) block_sums[idx] = run;
-        run += v[k];
-    }
-}
-
-__global__ void __launch_bounds__(kScanThreads) k_scan_add(int* out, int n, const int* block_sums) {
-    const long long base = (long long)blockIdx.x * kScanTile;
-    const int add = block_sums[blockIdx.x];
-    for (int k = threadIdx.x; k < kScanTile; k += kScanThreads) {
+    const long long base = (long long)t * kScanTile;
+    for (int k = tid; k < kScanTile; k += kScanThreads) {
         const long long idx = base + k;
         if (idx < n) out[idx] += add;
     }
@@ -635,8 +619,8 @@ hipError_t scan_exclusive(const int* in, int* out, int n, int* tmp, int limit, i
     const int nblocks = (n + kScanTile - 1) / kScanTile;
     if (nblocks > kScanTile || nblocks < 1) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_scan_tiles, dim3(nblocks), dim3(kScanThreads), 0, stream, in, out, n, tmp);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, stream, tmp, nblocks, out, n, limit, flags, flag);
-    hipLaunchKernelGGL(k_scan_add, dim3(nblocks), dim3(kScanThreads), 0, stream, out, n, (const int*)tmp);
+    hipLaunchKernelGGL(k_scan_add, dim3(nblocks), dim3(kScanThreads), 0, stream, out, n, (const int*)tmp, limit, flags,
+                       flag);
     return hipGetLastError();
 }
 
