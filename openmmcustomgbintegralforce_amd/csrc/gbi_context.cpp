@@ -85,6 +85,7 @@ struct SysHost {
     double g = 0.15, gpad = 0.0, halfdiag = 0.0, max_reach = 0.0;
     long long cell_cap = 0;
     int pt_cap = 1;
+    int tmp_cap = 0;  // raw member slots per cell: estimated from the positions, grown from the device counts
     int pair_splits = 1, pair_k = 1, nblk = 1, blk_base = 0;
     long long colpart_base = 0;
     double box[3] = {0.0, 0.0, 0.0};  // periodic box edges (gbi_set_periodic_box)
@@ -140,7 +141,7 @@ struct gbi_ctx {
     DevBuf<double> pos, d_charge, d_radius, d_reach, quad_off, quad_pw, integrals, born, dbdi, dedi, pair_part,
         atom_energy, energy, stats, dedq, dedr, theta_atom, theta;
     DevBuf<float4> aparam;
-    DevBuf<int> cell_count, cell_fill, cell_start, cell_count_eff, cell_atoms_tmp, cell_atoms, cell_pt_count,
+    DevBuf<int> cell_fill, cell_start, cell_count_eff, cell_atoms_tmp, cell_atoms, cell_pt_count,
         cell_act_count, frec_base, worklist, worklist_sys, work, scan_tmp;
     DevBuf<float4> ent_pos, ent_par, prec, frec0, frec1, frec2, vrec;
     DevBuf<float> quad_r2, vsig, vval;
@@ -195,7 +196,7 @@ struct gbi_ctx {
         c_ex_idx.release();
         cprog.reset();
         colpart.release();
-        DevBuf<int>* ib[] = {&flags_all,      &atom_sys,     &cell_sys,       &bin_order,       &cell_count, &cell_fill,
+        DevBuf<int>* ib[] = {&flags_all,      &atom_sys,     &cell_sys,       &bin_order,       &cell_fill,
                              &cell_start,     &cell_count_eff, &cell_atoms_tmp, &cell_atoms, &cell_pt_count,
                              &cell_act_count, &frec_base,    &worklist,       &worklist_sys, &work, &scan_tmp};
         for (auto* b : ib) b->release();
@@ -301,7 +302,8 @@ gbi_status upload_atoms(gbi_ctx& c) {
 }
 
 // System constants in the device layout.
-gbi::SysDesc make_desc(const gbi_ctx& c, const SysHost& s, long long cell_base, long long prec_base) {
+gbi::SysDesc make_desc(const gbi_ctx& c, const SysHost& s, long long cell_base, long long prec_base,
+                       long long tmp_base) {
     gbi::SysDesc d{};
     const gbi_params& P = s.P;
     d.grid_kind = s.kind;
@@ -314,6 +316,8 @@ gbi::SysDesc make_desc(const gbi_ctx& c, const SysHost& s, long long cell_base, 
     d.cell_base = (int)cell_base;
     d.cell_cap = (int)s.cell_cap;
     d.pt_cap = s.pt_cap;
+    d.tmp_base = tmp_base;
+    d.tmp_cap = s.tmp_cap;
     d.prec_base = prec_base;
     d.cap = (P.cell_cap > 0 && s.kind != gbi::GRID_BRUTE) ? P.cell_cap : INT_MAX;
     d.pair_splits = s.pair_splits;
@@ -370,19 +374,21 @@ gbi::SysDesc make_desc(const gbi_ctx& c, const SysHost& s, long long cell_base, 
 // Cell, slot and worklist buffers for the systems' current capacities; uploads the
 // system descriptors and the cell -> system map.
 gbi_status layout(gbi_ctx& c) {
-    long long cells = 0, slots = 0;
+    long long cells = 0, slots = 0, raw = 0;
     for (const SysHost& s : c.sys) {
         cells += s.cell_cap;
         slots += s.cell_cap * (long long)s.pt_cap;
+        raw += s.cell_cap * (long long)s.tmp_cap;
     }
     if (cells > kMaxCells) return fail(GBI_ERR_GRID_TOO_LARGE, "more than 16M lookup cells in the batch");
     if (slots * (long long)sizeof(float4) > (200ll << 30))
         return fail(GBI_ERR_GRID_TOO_LARGE, "point slots of the lookup grids exceed 200 GB");
+    if (raw * (long long)sizeof(int) > (64ll << 30))
+        return fail(GBI_ERR_GRID_TOO_LARGE, "raw cell lists of the lookup grids exceed 64 GB");
     c.drop_graphs();
     HIP_TRY(hipStreamSynchronize(c.stream));
     c.cell_total = cells;
     c.prec_total = slots;
-    HIP_TRY(c.cell_count.ensure(cells));
     HIP_TRY(c.cell_fill.ensure(cells));
     HIP_TRY(c.cell_start.ensure(cells + 1));
     HIP_TRY(c.cell_count_eff.ensure(cells));
@@ -394,14 +400,16 @@ gbi_status layout(gbi_ctx& c) {
     HIP_TRY(c.cell_sys.ensure(cells));
     HIP_TRY(c.scan_tmp.ensure((cells + 4095) / 4096 + 16));
     HIP_TRY(c.prec.ensure(slots));
+    HIP_TRY(c.cell_atoms_tmp.ensure(std::max<long long>(1, raw)));
     std::vector<int> cs(cells);
     std::vector<gbi::SysDesc> desc(c.S);
-    long long cb = 0, pb = 0;
+    long long cb = 0, pb = 0, tb = 0;
     for (int s = 0; s < c.S; ++s) {
-        desc[s] = make_desc(c, c.sys[s], cb, pb);
+        desc[s] = make_desc(c, c.sys[s], cb, pb, tb);
         std::fill(cs.begin() + cb, cs.begin() + cb + c.sys[s].cell_cap, s);
         cb += c.sys[s].cell_cap;
         pb += c.sys[s].cell_cap * (long long)c.sys[s].pt_cap;
+        tb += c.sys[s].cell_cap * (long long)c.sys[s].tmp_cap;
     }
     HIP_TRY(hipMemcpyAsync(c.cell_sys.p, cs.data(), sizeof(int) * cells, hipMemcpyHostToDevice, c.stream));
     HIP_TRY(hipMemcpyAsync(c.d_sys.p, desc.data(), sizeof(gbi::SysDesc) * c.S, hipMemcpyHostToDevice, c.stream));
@@ -474,10 +482,12 @@ gbi_status build_tiles(gbi_ctx& c) {
 }
 
 // Cell count and list entries of one system for the given positions, with headroom.
-void estimate_caps(const gbi_ctx& c, const SysHost& s, const double* pos, long long& cells, long long& entries) {
+void estimate_caps(const gbi_ctx& c, const SysHost& s, const double* pos, long long& cells, long long& entries,
+                   int& raw) {
     if (s.kind == gbi::GRID_BRUTE) {
         cells = 1;
         entries = s.N;
+        raw = s.N;
         return;
     }
     double mn[3], mx[3];
@@ -506,6 +516,13 @@ void estimate_caps(const gbi_ctx& c, const SysHost& s, const double* pos, long l
         side = std::floor(2.0 * s.max_reach / g) + 3.0;
     }
     entries = (long long)(side * side * side * s.N * 1.05) + 1024;
+    // raw members of a cell: twice the mean density over the bounding box (or the periodic
+    // box) times the candidate box of a cell; an overflow is sized from the device counts
+    double vol = 1.0;
+    for (int d = 0; d < 3; ++d) vol *= s.periodic() ? s.box[d] : std::max(mx[d] - mn[d], g);
+    const double cand = side * g;
+    const double est = 2.0 * s.N / vol * cand * cand * cand + 32.0;
+    raw = (int)std::min<double>(est, (double)s.N * (s.periodic() ? 8.0 : 1.0) + 32.0);
 }
 
 // Pair slots of a cutoff system along a serpentine path: columns of 0.45 nm x 0.45 nm in x
@@ -643,8 +660,13 @@ gbi_status size_from_device(gbi_ctx& c, long long min_entries) {
     }
     for (SysHost& s : c.sys) {
         long long cells, ent;
-        estimate_caps(c, s, host.data() + 3 * (size_t)s.atom_begin, cells, ent);
+        int raw;
+        estimate_caps(c, s, host.data() + 3 * (size_t)s.atom_begin, cells, ent, raw);
         entries += ent;
+        if (raw > s.tmp_cap) {
+            s.tmp_cap = raw;
+            changed = true;
+        }
         if (cells > s.cell_cap) {
             s.cell_cap = cells;
             changed = true;
@@ -656,7 +678,6 @@ gbi_status size_from_device(gbi_ctx& c, long long min_entries) {
         c.drop_graphs();
         HIP_TRY(hipStreamSynchronize(c.stream));
         c.entry_cap = (int)entries;
-        HIP_TRY(c.cell_atoms_tmp.ensure(entries));
         HIP_TRY(c.cell_atoms.ensure(entries));
         HIP_TRY(c.ent_pos.ensure(entries));
         HIP_TRY(c.ent_par.ensure(entries));
@@ -706,7 +727,6 @@ gbi::KParams make_kparams(gbi_ctx& c, bool adjoint) {
     k.quad_rows = c.quad_rows;
     k.pair_tiles = c.pair_tiles.p;
     k.colblk = c.colblk.p;
-    k.cell_count = c.cell_count.p;
     k.cell_fill = c.cell_fill.p;
     k.cell_start = c.cell_start.p;
     k.cell_count_eff = c.cell_count_eff.p;
@@ -840,8 +860,26 @@ gbi_status check_flags(gbi_ctx& c, bool& retry) {
     if (all & gbi::FLAG_GRID_TOO_LARGE)
         return fail(GBI_ERR_GRID_TOO_LARGE,
                     "CharmmGBMVForce: CustomGBIntegral lookup table dimension is too large, check atom positions!");
-    if (all & (gbi::FLAG_CELL_OVERFLOW | gbi::FLAG_ENTRY_OVERFLOW | gbi::FLAG_POINT_OVERFLOW)) {
+    if (all & (gbi::FLAG_CELL_OVERFLOW | gbi::FLAG_ENTRY_OVERFLOW | gbi::FLAG_POINT_OVERFLOW | gbi::FLAG_LIST_OVERFLOW)) {
         bool grew = false;
+        if (all & gbi::FLAG_LIST_OVERFLOW) {
+            // the fill's counts are exact past the capacity: room for the largest plus 25 %
+            std::vector<int> fill(c.cell_total);
+            HIP_TRY(hipMemcpyAsync(fill.data(), c.cell_fill.p, sizeof(int) * c.cell_total, hipMemcpyDeviceToHost,
+                                   c.stream));
+            HIP_TRY(hipStreamSynchronize(c.stream));
+            long long cb = 0;
+            for (int s = 0; s < c.S; ++s) {
+                SysHost& h = c.sys[s];
+                if (G[s].flags & gbi::FLAG_LIST_OVERFLOW) {
+                    int mx = 0;
+                    for (long long k = cb; k < cb + h.cell_cap && k < c.cell_total; ++k) mx = std::max(mx, fill[k]);
+                    h.tmp_cap = std::max(h.tmp_cap + 1, mx + mx / 4 + 16);
+                    grew = true;
+                }
+                cb += h.cell_cap;
+            }
+        }
         for (int s = 0; s < c.S; ++s) {
             SysHost& h = c.sys[s];
             if (G[s].flags & gbi::FLAG_CELL_OVERFLOW) {
@@ -881,6 +919,7 @@ gbi_status apply_tight_grid(gbi_ctx& c, double g) {
         if (s.kind == gbi::GRID_TIGHT && s.g != g) {
             s.g = g;
             s.cell_cap = 0;
+            s.tmp_cap = 0;
             const double est = 1.6 * 100.0 * s.Q * g * g * g + 64.0;
             s.pt_cap = std::max(1, (int)std::min<double>(est, (double)s.N * s.Q));
             changed = true;

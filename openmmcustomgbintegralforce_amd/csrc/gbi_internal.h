@@ -28,10 +28,12 @@ enum : int {
     FLAG_GRID_TOO_LARGE = 4,   // an axis longer than 1000 nm (CustomGBIntegral.cpp:182-183)
     FLAG_NONFINITE_POS = 8,
     FLAG_NONFINITE_RESULT = 16,
-    FLAG_POINT_OVERFLOW = 32   // more points in a cell than its slots
+    FLAG_POINT_OVERFLOW = 32,  // more points in a cell than its slots
+    FLAG_LIST_OVERFLOW = 64    // more raw members in a cell than its list slots
 };
 constexpr int kFatalFlags =
-    FLAG_CELL_OVERFLOW | FLAG_ENTRY_OVERFLOW | FLAG_GRID_TOO_LARGE | FLAG_NONFINITE_POS | FLAG_POINT_OVERFLOW;
+    FLAG_CELL_OVERFLOW | FLAG_ENTRY_OVERFLOW | FLAG_GRID_TOO_LARGE | FLAG_NONFINITE_POS | FLAG_POINT_OVERFLOW |
+    FLAG_LIST_OVERFLOW;
 
 constexpr int kMaxAtomChunk = 256;       // atoms staged in LDS per chunk (volume pass)
 constexpr int kQBits = 11;               // packed point = (atom << 11) | q, Q <= 2047
@@ -77,6 +79,8 @@ struct SysDesc {
     int cell_base, cell_cap;   // global cells [cell_base, cell_base + cell_cap)
     int pt_cap;                // point slots per cell
     long long prec_base;       // slot s of local cell c: prec_base + c * pt_cap + s
+    long long tmp_base;        // raw member k of local cell c: cell_atoms_tmp[tmp_base + c * tmp_cap + k]
+    int tmp_cap;               // raw member slots per cell (the system's size for the brute cell)
     int cap;                   // atoms kept per cell
     int pair_splits;           // pair tiles of row block 0 (the most of any block)
     int pair_k;                // column blocks per pair tile
@@ -180,7 +184,6 @@ struct KParams {
                              // or for cutoff systems a serpentine order so 64-slot blocks are compact)
     const int* colblk;       // n_colblocks: first atom of every 64-atom block
     // grid
-    int* cell_count;         // cell_total
     int* cell_fill;          // cell_total
     int* cell_start;         // cell_total + 1
     int* cell_count_eff;     // cell_total

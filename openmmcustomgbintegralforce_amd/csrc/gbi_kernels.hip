@@ -169,7 +169,6 @@ __device__ inline void cell_corner(const SysDesc& sd, const SysGeom& G, int c, d
 __global__ void k_clear(KParams p) {
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < p.cell_total; i += stride) {
-        p.cell_count[i] = 0;
         p.cell_fill[i] = 0;
         p.cell_pt_count[i] = 0;
     }
@@ -317,7 +316,7 @@ __device__ void bbox_to_geom(const KParams& p, int s, double (*smin)[16], double
     p.geom[s] = G;
     if (G.flags) atomicOr(p.flags_all, G.flags);
     // the brute cell holds every atom of the system (its count is known here)
-    if (sd.grid_kind == GRID_BRUTE && ncell == 1) p.cell_count[sd.cell_base] = sd.atom_end - sd.atom_begin;
+    if (sd.grid_kind == GRID_BRUTE && ncell == 1) p.cell_fill[sd.cell_base] = sd.atom_end - sd.atom_begin;
 }
 
 // ---------------------------------------------------------------------------
@@ -437,32 +436,22 @@ __device__ inline void list_position(const KParams& p, const SysDesc& sd, const 
     for (int d = 0; d < 3; ++d) x[d] = sd.periodic ? wrap_coord(G, d, p.pos[3 * j + d]) : p.pos[3 * j + d];
 }
 
-// One workgroup per atom, lanes over its candidate cells: count the members of each cell.
-__global__ void __launch_bounds__(256) k_grid_count(KParams p) {
-    if (batch_fatal(p)) return;
-    const int j = blockIdx.x;
-    const int s = p.atom_sys[j];
-    const SysDesc& sd = p.sys[s];
-    if (sd.grid_kind == GRID_BRUTE) return;  // counted by k_bbox
-    const SysGeom G = p.geom[s];
-    double x[3];
-    list_position(p, sd, G, j, x);
-    const double pad = atom_pad(p, sd, G, j);
-    const CandBox B = candidate_box(sd, G, x, pad);
-    for (int t = threadIdx.x; t < B.count; t += blockDim.x) {
-        int cell, code;
-        if (candidate_cell(sd, G, B, t, x, pad, cell, code)) atomicAdd(&p.cell_count[sd.cell_base + cell], 1);
-    }
-}
-
+// One workgroup per atom, lanes over its candidate cells: each member key goes to its cell's
+// raw list (fixed capacity per cell; the atomic's count stays exact on overflow, so the host
+// can size the lists from it).  The CSR layout of the sorted lists is the scan of the counts.
 __global__ void __launch_bounds__(256) k_grid_fill(KParams p) {
     if (batch_fatal(p)) return;
     const int j = blockIdx.x;
     const int s = p.atom_sys[j];
     const SysDesc& sd = p.sys[s];
     if (sd.grid_kind == GRID_BRUTE) {
-        // index order, no atomics
-        if (threadIdx.x == 0) p.cell_atoms_tmp[p.cell_start[sd.cell_base] + (j - sd.atom_begin)] = j << kImageBits;
+        // index order, no atomics (k_bbox set the count)
+        if (threadIdx.x == 0) {
+            if (j - sd.atom_begin < sd.tmp_cap)
+                p.cell_atoms_tmp[sd.tmp_base + (j - sd.atom_begin)] = j << kImageBits;
+            else
+                raise_flag(p, s, FLAG_LIST_OVERFLOW);
+        }
         return;
     }
     const SysGeom G = p.geom[s];
@@ -473,10 +462,11 @@ __global__ void __launch_bounds__(256) k_grid_fill(KParams p) {
     for (int t = threadIdx.x; t < B.count; t += blockDim.x) {
         int cell, code;
         if (candidate_cell(sd, G, B, t, x, pad, cell, code)) {
-            const int c = sd.cell_base + cell;
-            const int slot = atomicAdd(&p.cell_fill[c], 1);
-            const int idx = p.cell_start[c] + slot;
-            if (idx < p.entry_cap) p.cell_atoms_tmp[idx] = (j << kImageBits) | code;
+            const int slot = atomicAdd(&p.cell_fill[sd.cell_base + cell], 1);
+            if (slot < sd.tmp_cap)
+                p.cell_atoms_tmp[sd.tmp_base + (long long)cell * sd.tmp_cap + slot] = (j << kImageBits) | code;
+            else
+                raise_flag(p, s, FLAG_LIST_OVERFLOW);
         }
     }
 }
@@ -512,7 +502,7 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
     if (batch_fatal(p)) return;
     const int lane = threadIdx.x;
     for (int c = blockIdx.x; c < p.cell_total; c += gridDim.x) {
-        const int n = p.cell_count[c];
+        const int n = min(p.cell_fill[c], p.sys[p.cell_sys[c]].tmp_cap);  // (more is a fatal flag)
         if (n == 0) {
             if (lane == 0) p.cell_count_eff[c] = 0;
             continue;
@@ -522,12 +512,13 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
         const SysGeom G = p.geom[s];
         const int base = p.cell_start[c];
         const int eff = n < sd.cap ? n : sd.cap;
+        const int* raw = p.cell_atoms_tmp + sd.tmp_base + (long long)(c - sd.cell_base) * sd.tmp_cap;
         double corner[3];
         cell_corner(sd, G, c - sd.cell_base, corner);
         if (sd.grid_kind == GRID_BRUTE) {
-            for (int k = lane; k < eff; k += kWave) write_entry(p, sd, G, base + k, p.cell_atoms_tmp[base + k], corner);
+            for (int k = lane; k < eff; k += kWave) write_entry(p, sd, G, base + k, raw[k], corner);
         } else if (n <= kSortLds) {
-            for (int k = lane; k < n; k += kWave) s_in[k] = p.cell_atoms_tmp[base + k];
+            for (int k = lane; k < n; k += kWave) s_in[k] = raw[k];
             __syncthreads();
             for (int k = lane; k < n; k += kWave) {
                 const int a = s_in[k];
@@ -539,9 +530,9 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
             for (int k = lane; k < eff; k += kWave) write_entry(p, sd, G, base + k, s_out[k], corner);
         } else {
             for (int k = lane; k < n; k += kWave) {
-                const int a = p.cell_atoms_tmp[base + k];
+                const int a = raw[k];
                 int r = 0;
-                for (int m = 0; m < n; ++m) r += p.cell_atoms_tmp[base + m] < a;
+                for (int m = 0; m < n; ++m) r += raw[m] < a;
                 p.cell_atoms[base + r] = a;
             }
             __threadfence_block();
@@ -1838,11 +1829,10 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
     const int clear_blocks = std::max(1, std::min((p.cell_total + 255) / 256, 4096));
     hipLaunchKernelGGL(k_clear, dim3(clear_blocks), dim3(256), 0, stream, p);
     hipLaunchKernelGGL(k_bbox, dim3(p.S), dim3(1024), 0, stream, p);
-    hipLaunchKernelGGL(k_grid_count, dim3(p.N), dim3(256), 0, stream, p);
-    err = scan_exclusive(p.cell_count, p.cell_start, p.cell_total, p.scan_tmp, p.entry_cap, p.flags_all,
+    hipLaunchKernelGGL(k_grid_fill, dim3(p.N), dim3(256), 0, stream, p);
+    err = scan_exclusive(p.cell_fill, p.cell_start, p.cell_total, p.scan_tmp, p.entry_cap, p.flags_all,
                          FLAG_ENTRY_OVERFLOW, stream);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k_grid_fill, dim3(p.N), dim3(256), 0, stream, p);
     hipLaunchKernelGGL(k_grid_sort, dim3(std::max(1, std::min(p.cell_total, 65536))), dim3(64), 0, stream, p);
     mark(1);
     hipLaunchKernelGGL(k_point_bin, dim3(p.N), dim3(256), 0, stream, p);

@@ -138,6 +138,38 @@ def test_regrow_dense_cell_lists():
         eng.close()
 
 
+def test_regrow_raw_cell_lists():
+    """The per-cell raw member lists are sized from the positions at the first evaluation:
+    a globule compressed after that (and, under graph replay, replayed), and two dense blobs
+    far apart (a bounding box far emptier than the blobs) overflow them; the evaluation is
+    redone with lists sized from the device's counts and matches the oracle."""
+    for kind, n, seed, graph in (("gbmv2-50", 400, 23, False), ("gbsw", 400, 24, True)):
+        s = S.globule(n, seed)
+        c = s.pos.mean(axis=0)
+        p = _params(kind)
+        eng = N.Engine(p, s.charge, s.radius)
+        eng.set_graph(graph)
+        eng.execute(s.pos)
+        dense = (s.pos - c) * 0.45 + c
+        for rep in range(2):
+            e, f = eng.execute(dense)
+            ref = O.compute(oracle_params(p), dense, s.charge, s.radius)
+            print(f"[raw lists] {kind} compressed, graph {graph}, rep {rep}: energy {e:.6f} vs {ref.energy:.6f}",
+                  flush=True)
+            assert rel_err(e, ref.energy) < ENERGY_TOL and rel_rms(f, ref.forces) < FORCE_TOL, (kind, rep)
+        eng.close()
+    a, b = S.globule(300, 25), S.globule(300, 26)
+    pos = np.concatenate([(a.pos - a.pos.mean(0)) * 0.5, (b.pos - b.pos.mean(0)) * 0.5 + 8.0])
+    charge, radius = np.concatenate([a.charge, b.charge]), np.concatenate([a.radius, b.radius])
+    p = _params("gbmv2-50")
+    eng = N.Engine(p, charge, radius)
+    e, f = eng.execute(pos)
+    eng.close()
+    ref = O.compute(oracle_params(p), pos, charge, radius)
+    print(f"[raw lists] two blobs: energy {e:.6f} vs {ref.energy:.6f}", flush=True)
+    assert rel_err(e, ref.energy) < ENERGY_TOL and rel_rms(f, ref.forces) < FORCE_TOL
+
+
 def test_autotune_keeps_parity():
     s = S.globule(900, 8)
     for p in (_params("gbsw"), _params("gbmv2-50")):
