@@ -55,18 +55,21 @@ constexpr int kDefaultBlocksPerCU = 128;  // about one workgroup per occupied ce
 // 5k and 25k atoms (0.15 nm: 14 % slower, more distinct cells to bin; 0.30 nm: longer lists).
 constexpr double kDefaultTightGrid = 0.20;
 constexpr int kCutoffTileBlocks = 3;  // column blocks per pair tile of a cutoff system
+constexpr size_t kOwnAllocation = 2u << 20;  // bytes: a buffer not sub-allocated beside others
 
 template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
-    hipError_t ensure(size_t count) {
+    // min_bytes: a floor on the allocation's size (2 MB gives the buffer allocations of its own)
+    hipError_t ensure(size_t count, size_t min_bytes = 0) {
         if (count <= n && p) return hipSuccess;
         if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
-        hipError_t e = hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
-        if (e == hipSuccess) n = std::max<size_t>(count, 1);
+        const size_t cnt = std::max<size_t>({count, 1, (min_bytes + sizeof(T) - 1) / sizeof(T)});
+        hipError_t e = hipMalloc((void**)&p, cnt * sizeof(T));
+        if (e == hipSuccess) n = cnt;
         return e;
     }
     void release() {
@@ -392,7 +395,10 @@ gbi_status layout(gbi_ctx& c) {
     HIP_TRY(c.cell_fill.ensure(cells));
     HIP_TRY(c.cell_start.ensure(cells + 1));
     HIP_TRY(c.cell_count_eff.ensure(cells));
-    HIP_TRY(c.cell_pt_count.ensure(cells));
+    // the binning pass's atomic counters get an allocation of their own: sub-allocated beside
+    // other small buffers, that pass was measured 15-20 % slower in some processes (0.26 vs
+    // 0.215 ms at 5k atoms, 0.49 vs 0.445 ms at 25k), never with its own allocation
+    HIP_TRY(c.cell_pt_count.ensure(cells, kOwnAllocation));
     HIP_TRY(c.cell_act_count.ensure(cells));
     HIP_TRY(c.frec_base.ensure(cells + 1));
     HIP_TRY(c.worklist.ensure(gbi::VOL_KINDS * cells));
