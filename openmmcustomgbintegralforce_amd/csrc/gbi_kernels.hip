@@ -502,13 +502,14 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
     if (batch_fatal(p)) return;
     const int lane = threadIdx.x;
     for (int c = blockIdx.x; c < p.cell_total; c += gridDim.x) {
-        const int n = min(p.cell_fill[c], p.sys[p.cell_sys[c]].tmp_cap);  // (more is a fatal flag)
-        if (n == 0) {
+        const int n_raw = p.cell_fill[c];
+        if (n_raw == 0) {
             if (lane == 0) p.cell_count_eff[c] = 0;
             continue;
         }
         const int s = p.cell_sys[c];
         const SysDesc& sd = p.sys[s];
+        const int n = min(n_raw, sd.tmp_cap);  // (more is a fatal flag: this evaluation is redone)
         const SysGeom G = p.geom[s];
         const int base = p.cell_start[c];
         const int eff = n < sd.cap ? n : sd.cap;
