@@ -498,7 +498,8 @@ __device__ inline void write_entry(const KParams& p, const SysDesc& sd, const Sy
 constexpr int kSortLds = 256;  // cells with more raw members sort in global memory
 
 __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
-    __shared__ int s_in[kSortLds], s_out[kSortLds];
+    __shared__ __align__(16) int s_in[kSortLds];
+    __shared__ int s_out[kSortLds];
     if (batch_fatal(p)) return;
     const int lane = threadIdx.x;
     for (int c = blockIdx.x; c < p.cell_total; c += gridDim.x) {
@@ -519,12 +520,18 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
         if (sd.grid_kind == GRID_BRUTE) {
             for (int k = lane; k < eff; k += kWave) write_entry(p, sd, G, base + k, raw[k], corner);
         } else if (n <= kSortLds) {
-            for (int k = lane; k < n; k += kWave) s_in[k] = raw[k];
+            // keys are distinct; padded to a multiple of 4 with keys above them all, so each
+            // rank reads the keys 4 at a time
+            const int n4 = (n + 3) & ~3;
+            for (int k = lane; k < n4; k += kWave) s_in[k] = k < n ? raw[k] : INT_MAX;
             __syncthreads();
             for (int k = lane; k < n; k += kWave) {
                 const int a = s_in[k];
                 int r = 0;
-                for (int m = 0; m < n; ++m) r += s_in[m] < a;
+                for (int m = 0; m < n4; m += 4) {
+                    const int4 q = *reinterpret_cast<const int4*>(&s_in[m]);
+                    r += (q.x < a) + (q.y < a) + (q.z < a) + (q.w < a);
+                }
                 s_out[r] = a;
             }
             __syncthreads();
