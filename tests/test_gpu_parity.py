@@ -97,3 +97,53 @@ def test_tiny_and_block_edge_systems(kind, n):
     assert rel_err(e, ref.energy) < ENERGY_TOL
     assert fe < FORCE_TOL
     assert np.max(np.abs(born - ref.born) / ref.born) < 1e-5
+
+
+def _net_force_ok(f, tag):
+    """A translation-invariant energy has zero net force; FP32 terms leave a residue growing
+    like sqrt(N) rms|F| (a dropped owner or pair term would leave one of order N rms|F|)."""
+    net = np.abs(f.sum(axis=0)).max()
+    rms = float(np.sqrt(np.mean(np.sum(f * f, axis=1))))
+    bound = 1e-4 * np.sqrt(f.shape[0]) * rms  # observed ~1e-3 of this at 5k and 25k atoms
+    print(f"[invariants] {tag}: net force {net:.3e} (bound {bound:.3e}, rms |F| {rms:.3e})", flush=True)
+    assert net < bound, tag
+
+
+def test_full_size_invariants():
+    """Size-independent properties at the benchmark sizes, where the oracle is too slow to
+    run in a test: the GBSW all-pairs 25,000-atom globule and the GBMV-II 5,000-atom globule.
+    Charges enter the energy quadratically (q -> s q scales E by s^2, the Born radii do not
+    depend on q); the exact GBSW lists make the energy independent of where the grid's cells
+    fall, so a translation changes it only by rounding; and the net force vanishes."""
+    s = S.globule(25000, 0)
+    p = N.default_params(N.MODEL_GBSW)
+    eng = N.Engine(p, s.charge, s.radius)
+    e, f = eng.execute(s.pos)
+    eng.close()
+    _net_force_ok(f, "gbsw 25000")
+    t = np.array([0.37, -1.21, 2.05])
+    eng = N.Engine(p, s.charge, s.radius)
+    e_t, f_t = eng.execute(s.pos + t)
+    eng.close()
+    print(f"[invariants] gbsw 25000 translated: energy rel {rel_err(e_t, e):.2e} force rel-RMS {rel_rms(f_t, f):.2e}",
+          flush=True)
+    assert rel_err(e_t, e) < 1e-6 and rel_rms(f_t, f) < FORCE_TOL
+    eng = N.Engine(p, -1.5 * s.charge, s.radius)
+    e_q, f_q = eng.execute(s.pos)
+    eng.close()
+    print(f"[invariants] gbsw 25000 q -> -1.5 q: energy rel {rel_err(e_q, 2.25 * e):.2e} "
+          f"force rel-RMS {rel_rms(f_q, 2.25 * f):.2e}", flush=True)
+    assert rel_err(e_q, 2.25 * e) < 1e-6 and rel_rms(f_q, 2.25 * f) < FORCE_TOL
+
+    g = S.globule(5000, 0)
+    p = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV2)
+    p.n_lebedev = 50
+    eng = N.Engine(p, g.charge, g.radius)
+    e, f = eng.execute(g.pos)
+    eng.close()
+    _net_force_ok(f, "gbmv2 5000")
+    eng = N.Engine(p, -1.5 * g.charge, g.radius)
+    e_q, f_q = eng.execute(g.pos)
+    eng.close()
+    print(f"[invariants] gbmv2 5000 q -> -1.5 q: energy rel {rel_err(e_q, 2.25 * e):.2e}", flush=True)
+    assert rel_err(e_q, 2.25 * e) < 1e-6 and rel_rms(f_q, 2.25 * f) < FORCE_TOL
