@@ -1334,6 +1334,7 @@ gbi_status gbi_create_custom(int32_t n_atoms, const gbi_params* params, const gb
     L.values = prog->values;
     L.pairs = prog->pairs;
     L.pair_threads = 64 * gbi::kCustomPairWaves;
+    L.chain_threads = 64 * gbi::kCustomChainWaves;
     L.chain = prog->chain;
     L.N = n_atoms;
     L.nb = (int)nb;

@@ -461,21 +461,31 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
          "        for (int v = 1; v < CG_WAVES; ++v) acc += s_red[v][t >> 6][t & 63];\n"
          "        o[t] = acc;\n    }\n}\n";
 
-    // ---- per atom: the tiles' row and column partials in a fixed order, energy, pair
-    // forces, dE/dI ----
-    s << "extern \"C\" __global__ void cg_chain(int N, int nb, const double* __restrict__ part, const double* "
-         "__restrict__ dvdi, const double* __restrict__ esp, const double* __restrict__ despdi, double* __restrict__ "
-         "atom_energy, long long* __restrict__ force_fix, double* __restrict__ dedi) {\n"
-         "    const int a = blockIdx.x * blockDim.x + threadIdx.x;\n    if (a >= N) return;\n"
-         "    const int I = a / 64, l = a - 64 * (a / 64);\n"
+    // ---- per 64-atom block I (one workgroup of CG_CHAIN_WAVES waves): the tiles' row and
+    // column partials, wave w taking tiles w, w + CG_CHAIN_WAVES, ... and the waves' sums
+    // added in wave order (a fixed order), then energy, pair forces, dE/dI per atom ----
+    s << "#define CG_CHAIN_WAVES " << kCustomChainWaves << "\n";
+    s << "extern \"C\" __global__ void __launch_bounds__(64 * CG_CHAIN_WAVES) cg_chain(int N, int nb, const double* "
+         "__restrict__ part, const double* __restrict__ dvdi, const double* __restrict__ esp, const double* __restrict__ "
+         "despdi, double* __restrict__ atom_energy, long long* __restrict__ force_fix, double* __restrict__ dedi) {\n"
+         "    const int I = blockIdx.x, l = threadIdx.x & 63, w = threadIdx.x >> 6;\n"
+         "    const int a = 64 * I + l;\n"
+         "    __shared__ double s_sum[CG_CHAIN_WAVES][NPR + NPC][64];\n"
          "    double r[NPR], c[NPC];\n"
          "    for (int p = 0; p < NPR; ++p) r[p] = 0.0;\n    for (int p = 0; p < NPC; ++p) c[p] = 0.0;\n"
-         "    for (int J = I; J < nb; ++J) {\n"
+         "    for (int J = I + w; J < nb; J += CG_CHAIN_WAVES) {\n"
          "        const double* o = part + tile_index(I, J, nb) * (NPR + NPC) * 64;\n"
          "        for (int p = 0; p < NPR; ++p) r[p] += o[p * 64 + l];\n    }\n"
-         "    for (int K = 0; K <= I; ++K) {\n"
+         "    for (int K = w; K <= I; K += CG_CHAIN_WAVES) {\n"
          "        const double* o = part + tile_index(K, I, nb) * (NPR + NPC) * 64;\n"
          "        for (int p = 0; p < NPC; ++p) c[p] += o[(NPR + p) * 64 + l];\n    }\n"
+         "    for (int p = 0; p < NPR; ++p) s_sum[w][p][l] = r[p];\n"
+         "    for (int p = 0; p < NPC; ++p) s_sum[w][NPR + p][l] = c[p];\n"
+         "    __syncthreads();\n"
+         "    if (w != 0 || a >= N) return;\n"
+         "    for (int v = 1; v < CG_CHAIN_WAVES; ++v) {\n"
+         "        for (int p = 0; p < NPR; ++p) r[p] += s_sum[v][p][l];\n"
+         "        for (int p = 0; p < NPC; ++p) c[p] += s_sum[v][NPR + p][l];\n    }\n"
          "    atom_energy[a] = esp[a] + r[3];\n"
          "    force_fix[a] = (long long)llrint((r[0] + c[0]) * 4294967296.0);\n"
          "    force_fix[N + a] = (long long)llrint((r[1] + c[1]) * 4294967296.0);\n"

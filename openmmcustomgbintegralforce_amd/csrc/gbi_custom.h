@@ -16,6 +16,7 @@ namespace gbi {
 constexpr int kMaxCustomValues = 4;
 constexpr int kMaxCustomIntegrals = 2;
 constexpr int kCustomPairWaves = 4;  // waves of a cg_pairs workgroup (one 64 x 64 tile)
+constexpr int kCustomChainWaves = 4; // waves of a cg_chain workgroup (one 64-atom block)
 
 enum CustomTermType : int { TERM_SINGLE = 0, TERM_PAIR = 1, TERM_PAIR_NO_EXCL = 2 };
 
@@ -35,7 +36,7 @@ struct CustomSpec {
 //            const double* box3, const int* ex_ptr, const int* ex_idx,
 //            part[nb (nb + 1) / 2 tiles x NPL x 64])     grid (nb, nb), block 64 kCustomPairWaves
 //   cg_chain(int N, int nb, part, dvdi, esp, despdi, atom_energy[N], long long* force_fix[3 x N],
-//            dedi[2 x N])                                            grid N/256, block 256
+//            dedi[2 x N])                                  grid nb, block 64 kCustomChainWaves
 // (nb = ceil(N / 64) atom blocks; NPL = n_planes planes per tile: 4 + NV row, 3 + NV column)
 struct CustomProgram {
     hipModule_t module = nullptr;

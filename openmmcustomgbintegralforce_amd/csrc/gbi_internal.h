@@ -115,6 +115,7 @@ struct CustomLaunch {
     hipFunction_t values = nullptr, pairs = nullptr, chain = nullptr;
     int N = 0, nb = 0, method = 0;  // nb: 64-atom blocks
     int pair_threads = 64;          // block size of the pair kernel
+    int chain_threads = 64;         // block size of the chain kernel (one per 64-atom block)
     double cut2 = 0.0;
     const double* box = nullptr;  // device: the system's 3 box edges
     const double* prm = nullptr;

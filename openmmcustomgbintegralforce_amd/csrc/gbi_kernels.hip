@@ -1879,7 +1879,7 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
         if (err != hipSuccess) return err;
         mark(5);
         void* ca[] = {&N, &nb, &part, &dvdi, &esp, &despdi, &atom_energy, &force_fix, &dedi};
-        err = hipModuleLaunchKernel(c.chain, (N + 255) / 256, 1, 1, 256, 1, 1, 0, stream, ca, nullptr);
+        err = hipModuleLaunchKernel(c.chain, nb, 1, 1, c.chain_threads, 1, 1, 0, stream, ca, nullptr);
         if (err != hipSuccess) return err;
     } else if (adj) {
         // the host refuses the adjoint of cutoff systems; exclusions take the general path
