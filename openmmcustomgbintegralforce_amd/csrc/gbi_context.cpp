@@ -239,6 +239,10 @@ gbi_status validate_params(const gbi_params& P, int N, bool custom = false) {
     if (!(P.grid_length > 0.0)) return fail(GBI_ERR_INVALID_ARGUMENT, "grid_length must be positive");
     if (P.model == GBI_MODEL_GBSW || P.integral_type == GBI_INTEGRAL_GBSW) {
         if (!(P.sw > 0.0)) return fail(GBI_ERR_INVALID_ARGUMENT, "the switching half-width must be positive");
+    } else {
+        // V = 1/(1 + exp(beta (S - lambda))): beta = 0 makes V independent of the geometry
+        if (!std::isfinite(P.beta) || P.beta == 0.0 || !std::isfinite(P.lambda))
+            return fail(GBI_ERR_INVALID_ARGUMENT, "GBMV beta must be finite and nonzero, lambda finite");
     }
     for (int k = 0; k < 2; ++k)
         if (P.orders[k] < 2)  // GBSWIntegral.cpp:92-95
@@ -358,7 +362,6 @@ gbi::SysDesc make_desc(const gbi_ctx& c, const SysHost& s, long long cell_base, 
     d.S0 = (float)P.S0;
     d.vguard = (float)P.vguard;
     d.fclamp = 1.0e6f;
-    d.ssat = 6.0f;
     d.P1 = (float)P.P1;
     d.v_empty = s.vol == gbi::VOL_GBSW    ? 1.0f
                 : s.vol == gbi::VOL_GBMV2 ? 0.0f

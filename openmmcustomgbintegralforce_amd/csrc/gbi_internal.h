@@ -102,7 +102,7 @@ struct SysDesc {
     // volume constants
     float k1, k3;              // GBSW: H = 1/2 + x (k1 - k3 x^2), k1 = 3/(4w), k3 = 1/(4w^3)
     float sw, inv_sw, rscale, roffset;
-    float gamma0, beta, lambda, S0, vguard, fclamp, ssat, P1;
+    float gamma0, beta, lambda, S0, vguard, fclamp, P1;
     float v_empty;             // V of a point with an empty list
     float sign_chain;          // +1 for GBSW (I accumulates 1 - V), -1 for GBMV
     // Born map / energy

@@ -767,6 +767,12 @@ __global__ void k_worklist(KParams p) {
 // GBMV-II: sum1, sum2, v, S = S0 sum1 sum2/|v|^2 (GBMVIntegralTypeII.cpp:163-191).
 // ---------------------------------------------------------------------------
 
+// The sigmoid V = 1/(1 + exp(t)), t = beta (S - lambda), of GBMV-I/II is taken as its limit
+// (V = 1 for t < 0, V = 0 for t > 0, no chain-rule term) where |t| > kSigmoidSat: exp(t) then
+// underflows or overflows FP32, |1 - V| or V is below 2e-38 and dV/dt = -V (1 - V) below
+// 2e-38 as well, so the FP64 sigmoid agrees to that level whatever beta and lambda are.
+constexpr float kSigmoidSat = 87.0f;
+
 // V >= 0 with the sign bit as the "has a chain-rule factor" mark (also set for V = 0)
 __device__ inline float mark_active(float v, bool active) {
     return __int_as_float(__float_as_int(v) | (active ? (int)0x80000000 : 0));
@@ -793,7 +799,7 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
         const bool single = na <= kMaxAtomChunk;
         const int fb = p.frec_base[c];  // exclusive scan of the point counts
         const int pt_base = sd.pt_base, abeg = sd.atom_begin, Q = sd.Q;
-        const float k1 = sd.k1, k3 = sd.k3, fclamp = sd.fclamp, S0 = sd.S0, vguard = sd.vguard, ssat = sd.ssat;
+        const float k1 = sd.k1, k3 = sd.k3, fclamp = sd.fclamp, S0 = sd.S0, vguard = sd.vguard;
         const float beta = sd.beta, lambda = sd.lambda;
         auto stage = [&](int ab, int cnt) {
             for (int k = lane; k < cnt; k += kWave) {
@@ -889,11 +895,15 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                     r1.x = V;
                 } else if (VK == VOL_GBMV2) {
                     const float vsq = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
-                    const float S = S0 * s1 * s2 / vsq;
-                    const bool sat = !(vsq >= vguard) || !(S <= ssat);
-                    float Vv = 1.f;
+                    // |v|^2 below the guard: the S -> +inf limit; otherwise the sigmoid's
+                    // exponent t decides (its limit beyond +-kSigmoidSat)
+                    const bool guard = !(vsq >= vguard);
+                    const float S = guard ? 0.f : S0 * s1 * s2 / vsq;
+                    const float t = beta * (S - lambda);
+                    const bool sat = guard || !(fabsf(t) <= kSigmoidSat);
+                    float Vv = (guard ? beta < 0.f : t < 0.f) ? 1.f : 0.f;
                     if (!sat) {
-                        const float pre = __expf(beta * (S - lambda));
+                        const float pre = __expf(t);
                         const float opre = 1.f + pre;
                         Vv = 1.f / opre;
                         const float kk = -beta * pre / (opre * opre) * S0;
@@ -905,10 +915,14 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                     p.vval[pid] = mark_active(Vv, active);
                     if (p.adjoint) p.vsig[pid] = sat ? __int_as_float(0x7fc00000) : S;
                 } else {
-                    const float pre = __expf(beta * (s1 - lambda));
-                    const float opre = 1.f + pre;
-                    const float Vv = 1.f / opre;
-                    r1.x = -beta * pre / (opre * opre);
+                    const float t = beta * (s1 - lambda);
+                    float Vv = t < 0.f ? 1.f : 0.f;
+                    if (fabsf(t) <= kSigmoidSat) {
+                        const float pre = __expf(t);
+                        const float opre = 1.f + pre;
+                        Vv = 1.f / opre;
+                        r1.x = -beta * pre / (opre * opre);
+                    }
                     r1.y = Vv;
                     active = r1.x != 0.f;
                     p.vval[pid] = mark_active(Vv, active);

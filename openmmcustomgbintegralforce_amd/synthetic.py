@@ -143,3 +143,23 @@ def conformations(base: System, count: int, seed: int, sigma: float = 0.02) -> l
 def batch_sizes(count: int = 64, lo: int = 300, hi: int = 6000, seed: int = 0) -> np.ndarray:
     """Atom counts of the batched workload, U[lo, hi] (seeded)."""
     return np.random.default_rng(seed).integers(lo, hi + 1, count)
+
+
+def globules(specs, workers: int = 0) -> list:
+    """``globule(n, seed)`` for every (n, seed) of ``specs``, in order; with ``workers`` > 1 on
+    that many threads (the k-d tree and array work release the GIL).  Results are identical
+    either way."""
+    specs = [(int(n), int(s)) for n, s in specs]
+    if workers <= 1 or len(specs) < 2:
+        return [globule(n, s) for n, s in specs]
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(min(workers, len(specs))) as ex:
+        return list(ex.map(lambda a: globule(*a), specs))
+
+
+def batch64(seed: int = 0, workers: int = 0):
+    """The batched workload: 64 globules with N ~ U[300, 6000] (``batch_sizes(64, 300, 6000,
+    seed)``), system k seeded 1000 + k.  Returns (sizes, systems)."""
+    sizes = batch_sizes(64, 300, 6000, seed=seed)
+    return sizes, globules([(n, 1000 + k) for k, n in enumerate(sizes)], workers)

@@ -24,7 +24,8 @@
  *     (CharmmReferenceGBSW.cpp:235,562 never clear them);
  *   - GBMV-II brute force uses the lookup-path gradient (GBMVIntegralTypeII.cpp:243),
  *     not the one missing the x100 unit factor (GBMVIntegralTypeII.cpp:304);
- *   - GBMV-II with |v|^2 < vguard takes the S -> +inf limit (V = 1, no gradient);
+ *   - GBMV-II with |v|^2 < vguard takes the S -> +inf limit (V = 1 for beta < 0, V = 0 for
+ *     beta > 0; no gradient);
  *   - GBSW smoothing H <= 0 inside the switching shell is treated as burial (V = 0).
  */
 #include "gb_oracle.h"
@@ -817,7 +818,9 @@ static double point_volume(const ctx_t* c, const double* pt, const int* list, in
     default: {
         if (n == 0) { s2->guarded = 1; s2->pre = 0.0; return 0.0; }
         gbmv2_point(p, c->pos, c->R, pt, list, n, s2);
-        return s2->guarded ? 1.0 : 1.0 / (1.0 + s2->pre);
+        /* guarded: the S -> +inf limit of 1/(1 + exp(beta (S - lambda))) */
+        if (s2->guarded) return p->beta < 0.0 ? 1.0 : 0.0;
+        return 1.0 / (1.0 + s2->pre);
     }
     }
 }

@@ -85,6 +85,14 @@ def test_invalid_arguments_are_rejected_before_touching_a_device():
     bad = N.Params.from_buffer_copy(p)
     bad.orders[0] = 1
     assert _create(bad) == 1
+    # the GBMV sigmoid needs a finite, nonzero beta (beta = 0 makes V independent of the geometry)
+    for v in (0.0, np.inf):
+        bad = N.Params.from_buffer_copy(p)
+        bad.beta = v
+        assert _create(bad) == 1
+    bad = N.Params.from_buffer_copy(p)
+    bad.lambda_ = np.nan
+    assert _create(bad) == 1
     assert _create(p, r=(0.15, -0.2)) == 1
     assert _create(p, q=(np.nan, 0.0)) == 7
     assert N.load().gbi_params_default(N.MODEL_GBMV, N.INTEGRAL_CUSTOM, ctypes.byref(N.Params())) == 2

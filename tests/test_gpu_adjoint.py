@@ -4,8 +4,10 @@ differences of the FP64 oracle energy (needs an MI355X).
 The adjoint has no reference counterpart; the oracle energy is the function being
 differentiated.  Steps: h = 1e-4 nm for radii, 1e-3 e for charges (E is quadratic in q,
 so the central difference is exact up to rounding), 1e-5 relative for constants.
-Tolerance: |GPU - FD| <= 2e-4 * max|FD| over each tested set (the GPU sums FP32 terms in
-FP64; the FD truncation error at these steps is far below that).  Lookup grids whose
+Tolerances, per tested set: |GPU - FD| <= 2e-4 * max|FD| over the set (the GPU sums FP32
+terms in FP64; the FD truncation error at these steps is far below that), and element by
+element |GPU_i - FD_i| <= 1e-3 * max(|FD_i|, 1e-2 max|FD|), so a derivative much smaller than
+the largest of its set is still checked to 0.1 % of itself.  Lookup grids whose
 membership depends on R (the GBMV-form reference grid) are differentiated with the brute
 policy; the exact GBSW grid is smooth in R.
 
@@ -14,6 +16,8 @@ difference of two numbers near 4e4 for buried atoms, so the FP32 volume error in
 amplified by the conditioning of T (the closed form itself is pinned in FP64 by
 test_oracle_adjoint.py).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -79,11 +83,18 @@ def _fd_theta(op, s, name):
     return (ep - em) / (2 * h)
 
 
-def _check(tag, gpu, fd, tol=TOL):
+TOL_ELEM = 1e-3
+
+
+def _check(tag, gpu, fd, tol=TOL, tol_elem=TOL_ELEM):
     gpu, fd = np.atleast_1d(gpu), np.atleast_1d(fd)
-    err = np.max(np.abs(gpu - fd)) / max(np.max(np.abs(fd)), 1e-30)
-    print(f"[adjoint] {tag}: max err / max|FD| = {err:.2e} (max|FD| {np.max(np.abs(fd)):.3e})", flush=True)
+    scale = max(np.max(np.abs(fd)), 1e-30)
+    err = np.max(np.abs(gpu - fd)) / scale
+    elem = np.max(np.abs(gpu - fd) / np.maximum(np.abs(fd), 1e-2 * scale))
+    print(f"[adjoint] {tag}: max err / max|FD| = {err:.2e}, max element rel err {elem:.2e} "
+          f"(max|FD| {scale:.3e})", flush=True)
     assert err < tol, tag
+    assert elem < max(tol, tol_elem), tag
 
 
 def _run(p, s, model, r_atoms, q_atoms, theta_names, tag):
@@ -147,10 +158,10 @@ def test_adjoint_gbmv2_reference_grid_constants():
 
 
 def test_adjoint_conformations_2000():
-    """GBSW 24x38, 2,000 atoms, conformations of one base geometry (sigma 0.02 nm): all
-    constants, a seeded subset of radii and charges, and the q-homogeneity identity over
-    every atom, per conformation (two of the eight here; the rest are cheap on the GPU but
-    the FD reference is not)."""
+    """GBSW 24x38, 2,000 atoms, the 8 conformations of one base geometry (sigma 0.02 nm):
+    the q-homogeneity identity over every atom of each, and central FD of the oracle energy
+    on each: 12 radii, 6 charges and 5 constants on the first two, 4 radii, 2 charges and
+    2 constants (alpha0, rscale) on the other six."""
     base = S.globule(2000, 26)
     confs = S.conformations(base, 8, seed=27)
     p = N.default_params(N.MODEL_GBSW)
@@ -159,13 +170,12 @@ def test_adjoint_conformations_2000():
     for k, c in enumerate(confs):
         out = eng.execute_adjoint(c.pos)
         assert abs(np.dot(c.charge, out["dEdq"]) - 2 * out["energy"]) < 1e-5 * abs(out["energy"]), k
-        if k >= 2:
-            continue
-        op = oracle_params(p)
-        atoms = np.sort(rng.choice(2000, 12, replace=False))
+        op = oracle_params(p, nthreads=min(16, os.cpu_count() or 1))
+        full = k < 2
+        atoms = np.sort(rng.choice(2000, 12 if full else 4, replace=False))
         _check(f"conf {k} dE/dR", out["dEdR"][atoms], _fd_radius(op, c, atoms))
-        _check(f"conf {k} dE/dq", out["dEdq"][atoms[:6]], _fd_charge(op, c, atoms[:6]))
-        for name in ("alpha0", "eta", "sw", "rscale", "roffset"):
+        _check(f"conf {k} dE/dq", out["dEdq"][atoms[:6 if full else 2]], _fd_charge(op, c, atoms[:6 if full else 2]))
+        for name in (("alpha0", "eta", "sw", "rscale", "roffset") if full else ("alpha0", "rscale")):
             _check(f"conf {k} dE/d{name}", out["dEdtheta"][name], _fd_theta(op, c, name),
                    TOL_ETA if name == "eta" else TOL)
     eng.close()

@@ -6,6 +6,8 @@ Tolerances as in test_gpu_parity: force relative RMS <= 1e-5, energy relative er
 direct launches, a system in a batch vs alone) the results are compared bitwise or to
 FP64 summation-order level.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -200,3 +202,36 @@ def test_batch_of_64_mixed_systems_sampled():
         ref = O.compute(oracle_params(p), s.pos, s.charge, s.radius)
         assert rel_err(e[k], ref.energy) < ENERGY_TOL and rel_rms(f[a:b], ref.forces) < FORCE_TOL, k
     eng.close()
+
+
+def test_batch64_full_sizes_matches_oracle():
+    """The batched workload exactly as the benchmark builds it: 64 systems with
+    N ~ U[300, 6000] (seeded; 205,186 atoms), alternating CharmmGBSWForce 24 x 38 and
+    CharmmGBMVForce GBMVIntegralTypeII 24 x 50, one context.  Every eighth system and the
+    largest are checked against the oracle; the graph replay the benchmark times gives the
+    direct launches' results bitwise."""
+    sizes, systems = S.batch64(0, workers=min(16, os.cpu_count() or 1))
+    params = [_params("gbsw") if k % 2 == 0 else _params("gbmv2-50") for k in range(64)]
+    pos = np.concatenate([s.pos + 40.0 * k for k, s in enumerate(systems)])
+    print(f"[batch64] {len(systems)} systems, {int(np.sum(sizes))} atoms, largest {int(np.max(sizes))}",
+          flush=True)
+    eng = N.Engine.batch(params, [s.charge for s in systems], [s.radius for s in systems])
+    e, f = eng.execute(pos)
+    assert np.all(np.isfinite(f)) and np.all(np.isfinite(e))
+    eng.set_graph(True)
+    eng.set_positions(pos)
+    for _ in range(2):
+        eng.compute()
+        eng.synchronize()
+        assert np.array_equal(eng.energy(), e) and np.array_equal(eng.forces(), f)
+    eng.close()
+    check = sorted(set(range(0, 64, 8)) | {int(np.argmax(sizes))})
+    nthreads = min(16, os.cpu_count() or 1)
+    for k in check:
+        s, p = systems[k], params[k]
+        a, b = int(np.sum(sizes[:k])), int(np.sum(sizes[:k + 1]))
+        ref = O.compute(oracle_params(p, nthreads=nthreads), s.pos, s.charge, s.radius)
+        ee, fe = rel_err(e[k], ref.energy), rel_rms(f[a:b], ref.forces)
+        print(f"[batch64] system {k} N={s.n} {'gbsw' if k % 2 == 0 else 'gbmv2'}: energy rel {ee:.2e} "
+              f"force rel-RMS {fe:.2e}", flush=True)
+        assert ee < ENERGY_TOL and fe < FORCE_TOL, k

@@ -3,6 +3,8 @@
 Tolerances (floating point, from the workload specification): force relative RMS error
 <= 1e-5 and energy relative error <= 1e-5 against the FP64 Reference restatement.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -23,7 +25,7 @@ def _compare(p, sys_, tag, born_tol=1e-5):
     eng = N.Engine(p, sys_.charge, sys_.radius)
     e, f = eng.execute(sys_.pos)
     born = eng.born_radii()
-    ref = O.compute(oracle_params(p), sys_.pos, sys_.charge, sys_.radius)
+    ref = O.compute(oracle_params(p, nthreads=min(16, os.cpu_count() or 1)), sys_.pos, sys_.charge, sys_.radius)
     eng.close()
     assert np.all(np.isfinite(f))
     db = float(np.max(np.abs(born - ref.born) / ref.born))
@@ -40,6 +42,12 @@ def _compare(p, sys_, tag, born_tol=1e-5):
 def test_gbsw_standalone_500(seed):
     """CharmmGBSWForce, 500-atom globule, 24 x 38 quadrature: the 10 seeds of the CPU case."""
     _compare(N.default_params(N.MODEL_GBSW), S.globule(500, seed), f"gbsw500 seed {seed}")
+
+
+def test_gbsw_25000():
+    """CharmmGBSWForce all pairs, 25,000-atom globule, 24 x 38 quadrature (the 25k
+    benchmark system, seed 0) against the oracle."""
+    _compare(N.default_params(N.MODEL_GBSW), S.globule(25000, 0), "gbsw 25000")
 
 
 def test_gbmv2_5000():

@@ -63,7 +63,7 @@ def test_sizes(kind, n):
 @pytest.mark.parametrize("consts", [dict(gamma0=0.3, beta=-60.0, lambda_=0.2),
                                     dict(gamma0=0.5, beta=-150.0, lambda_=0.05),
                                     dict(beta=-40.0, lambda_=0.3, grid_length=0.12, buffer=0.25),
-                                    dict(cell_cap=16)])
+                                    dict(cell_cap=16), dict(beta=-5.0, lambda_=0.5)])
 def test_gbmv1_constants(consts):
     """GBMV-I (GBMVIntegralTypeI.cpp:18-57, 155-224) away from its defaults: the sigmoid
     steepness and threshold, gamma0, the lookup grid and a cell cap that truncates lists."""
@@ -74,8 +74,13 @@ def test_gbmv1_constants(consts):
 
 
 @pytest.mark.parametrize("consts", [dict(P1=0.4, P2=1.0, S0=0.6), dict(beta=-12.0, lambda_=0.7),
-                                    dict(orders=(7, 4))])
+                                    dict(orders=(7, 4)), dict(beta=-2.0), dict(beta=-5.0, lambda_=1.0),
+                                    dict(lambda_=6.0)])
 def test_gbmv2_constants(consts):
+    """GBMV-II (GBMVIntegralTypeII.cpp:19-62, 163-282) away from its defaults.  The soft
+    sigmoids (beta = -2, -5) and the high threshold (lambda = 6) put points at S well above 6
+    where V is neither 0 nor 1: the sigmoid is evaluated wherever exp(beta (S - lambda)) is
+    representable, so V and its chain-rule term match the FP64 oracle there too."""
     p = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV2)
     for k, v in consts.items():
         if k == "orders":

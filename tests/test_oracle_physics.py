@@ -54,6 +54,18 @@ def test_isolated_atom_gbmv2(R):
     assert np.allclose(res.integrals[:, 0], [(w * V / x ** 2).sum(), (w * V / x ** 5).sum()], rtol=1e-12)
 
 
+@pytest.mark.parametrize("beta", [-20.0, -2.0, 3.0])
+def test_gbmv2_guard_takes_the_infinite_s_limit(beta):
+    """A guard above every |v|^2 makes each point take the S -> +inf limit of
+    1/(1 + exp(beta (S - lambda))): V = 1 for beta < 0, V = 0 for beta > 0 (so I_n is the
+    plain quadrature sum of r^-(n-2), or zero).  A single atom has |v|^2 = F^2 d^2 > 0."""
+    x, w = O.gauss_legendre(0.05, 2.0, 24)
+    p = O.gbmv_params(O.INT_GBMV2, policy=O.POLICY_BRUTE, n_lebedev=50, beta=beta, vguard=1e30)
+    res = O.compute(p, np.zeros((1, 3)), [0.2], [0.17])
+    full = np.array([(w / x ** 2).sum(), (w / x ** 5).sum()])
+    assert np.allclose(res.integrals[:, 0], full if beta < 0 else 0.0, rtol=1e-13, atol=0.0)
+
+
 def test_isolated_atom_gbmv1():
     R = 0.17
     x, w = O.gauss_legendre(0.05, 2.0, 24)
