@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define GBI_ABI_VERSION 3
+#define GBI_ABI_VERSION 4
 
 typedef enum gbi_status {
     GBI_OK = 0,
@@ -103,7 +103,7 @@ typedef struct gbi_params {
     double sw, rscale, roffset;
     /* GBMV-I: gamma0, beta, lambda; GBMV-II: beta, lambda, P1, P2, S0 */
     double gamma0, beta, lambda, P1, P2, S0;
-    double vguard;          /* GBMV-II: |v|^2 below this means S = +inf (V = 1) */
+    double vguard;          /* GBMV-II: |v|^2 below this means S = +inf (V = 1 for beta < 0, 0 for beta > 0) */
     /* Born map B = slope / (alpha0 (1/eta - I4) + alpha1 (1/(4 eta^4) - I7)^(1/4)) + shift */
     double alpha0, alpha1, eta, slope, shift;
     /* energy E = sum_i P q_i^2/(2 B_i) + sum_{i<j} P q_i q_j / f_ij */
@@ -121,6 +121,8 @@ typedef struct gbi_stats {
     double n_entries;     /* cell list entries */
     double n_atoms;
     double n_quad;        /* quadrature points per atom */
+    double n_active;      /* points with a chain-rule term (0 < V < 1 for GBSW) */
+    double n_frc_pairs;   /* point/atom pairs the chain-rule pass visits (active points x their lists) */
 } gbi_stats;
 
 /* Device view of a context's buffers (for callers that keep data on the GPU). */
@@ -209,6 +211,15 @@ gbi_status gbi_execute(gbi_ctx* ctx, const double* pos, int32_t include_forces, 
 gbi_status gbi_update_parameters(gbi_ctx* ctx, int32_t n_atoms, const double* charge, const double* radius);
 
 /*
+ * New constants of system `system`: the global parameters a Context holds for the canonical
+ * CharmmGBMVForce (alpha0, alpha1, eta, slope, shift, prefactor P, read by the reference from
+ * the Context at every execute, CharmmReferenceKernels.cpp:355-357) and the volume constants
+ * (sw, rscale, roffset, gamma0, beta, lambda, P1, P2, S0, vguard) a fitting loop moves.  Every
+ * other field of `params` must equal the system's (GBI_ERR_INVALID_ARGUMENT otherwise).
+ */
+gbi_status gbi_update_constants(gbi_ctx* ctx, int32_t system, const gbi_params* params);
+
+/*
  * The periodic box of a GBI_CUTOFF_PERIODIC system: box_vectors = 3 x 3 doubles (a, b, c
  * row-major), read by the reference kernels from the Context every evaluation
  * (GBSWIntegral.cpp:108-110, CharmmReferenceKernels.cpp:351, :430).  Rectangular only
@@ -254,6 +265,10 @@ typedef struct gbi_custom_spec {
     int32_t n_terms;
     const char* const* term_exprs;
     const int32_t* term_types;       /* 0 SingleParticle, 1 ParticlePair, 2 ParticlePairNoExclusions */
+    /* CharmmGBMVForce::addEnergyParameterDerivative (CharmmGBMVForce.h:258): up to 4 global
+     * parameters whose energy derivative each evaluation also returns (gbi_get_custom_param_derivs) */
+    int32_t n_param_derivs;
+    const char* const* param_deriv_names;
 } gbi_custom_spec;
 
 /*
@@ -266,6 +281,19 @@ typedef struct gbi_custom_spec {
 gbi_status gbi_create_custom(int32_t n_atoms, const gbi_params* params, const gbi_custom_spec* spec, gbi_ctx** out);
 /* New values of the global parameters (Context::setParameter), n_globals doubles. */
 gbi_status gbi_set_custom_globals(gbi_ctx* ctx, int32_t n_globals, const double* values);
+/*
+ * dE/d(global parameter) of the last evaluation, one double per spec->param_deriv_names entry:
+ * the values the reference adds into the Context's energy-parameter derivatives after each
+ * execute (CharmmReferenceKernels.cpp:360-368, CharmmReferenceGBMV.cpp:222-228, 263-264,
+ * 345-346, 440-443).  The total derivative, through every computed value.
+ */
+gbi_status gbi_get_custom_param_derivs(gbi_ctx* ctx, int32_t n_derivs, double* out);
+/*
+ * New per-particle parameters of a general-expression context (n_atoms x n_params, row-major;
+ * the same N): replaces CalcCharmmGBMVForceKernel::copyParametersToContext
+ * (CharmmReferenceKernels.cpp:372-386) without recompiling the kernels.
+ */
+gbi_status gbi_update_custom_parameters(gbi_ctx* ctx, int32_t n_atoms, const double* param_values);
 /* The computed values of the last evaluation: n_values x N doubles (value-major). */
 gbi_status gbi_get_custom_values(gbi_ctx* ctx, double* values);
 

@@ -37,6 +37,7 @@ EXPORTS = [
     "gbi_set_tuning", "gbi_get_tuning", "gbi_destroy", "gbi_last_error", "gbi_abi_version", "gbi_quadrature",
     "gbi_profile_enable", "gbi_profile_read", "gbi_profile_pass_name", "gbi_set_periodic_box",
     "gbi_set_exclusions", "gbi_create_custom", "gbi_set_custom_globals", "gbi_get_custom_values",
+    "gbi_get_custom_param_derivs", "gbi_update_custom_parameters", "gbi_update_constants",
 ]
 PROFILE_PASSES = ["grid", "bin", "volume", "integrals", "pairs", "chain", "force", "owner"]
 
@@ -86,7 +87,7 @@ class Params(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
-                ("n_points", "n_vol_pairs", "n_cells", "n_entries", "n_atoms", "n_quad")]
+                ("n_points", "n_vol_pairs", "n_cells", "n_entries", "n_atoms", "n_quad", "n_active", "n_frc_pairs")]
 
 
 class Tuning(ctypes.Structure):
@@ -127,6 +128,8 @@ class CustomSpec(ctypes.Structure):
         ("n_terms", ctypes.c_int32),
         ("term_exprs", ctypes.POINTER(ctypes.c_char_p)),
         ("term_types", ctypes.POINTER(ctypes.c_int32)),
+        ("n_param_derivs", ctypes.c_int32),
+        ("param_deriv_names", ctypes.POINTER(ctypes.c_char_p)),
     ]
 
 
@@ -171,6 +174,9 @@ def load() -> ctypes.CDLL:
                                     ctypes.POINTER(vp)]
     L.gbi_set_custom_globals.argtypes = [vp, ctypes.c_int32, dp]
     L.gbi_get_custom_values.argtypes = [vp, dp]
+    L.gbi_get_custom_param_derivs.argtypes = [vp, ctypes.c_int32, dp]
+    L.gbi_update_custom_parameters.argtypes = [vp, ctypes.c_int32, dp]
+    L.gbi_update_constants.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(Params)]
     L.gbi_set_positions.argtypes = [vp, vp, ctypes.c_int32]
     L.gbi_compute.argtypes = [vp, ctypes.c_uint32]
     L.gbi_synchronize.argtypes = [vp]
@@ -249,12 +255,14 @@ class Engine:
         self._h = handle
 
     @classmethod
-    def custom(cls, params, param_names, param_values, radius_param, integrals, values, terms, globals_=None):
+    def custom(cls, params, param_names, param_values, radius_param, integrals, values, terms, globals_=None,
+               param_derivs=()):
         """A context for general CharmmGBMVForce expressions (gbi_create_custom).
 
         param_values: N x len(param_names); integrals: names (1 or 2; orders from
         params.orders); values: [(name, expression)]; terms: [(expression, TERM_*)];
-        globals_: {name: value}."""
+        globals_: {name: value}; param_derivs: names of globals whose energy derivative each
+        evaluation returns (custom_param_derivs)."""
         eng = cls.__new__(cls)
         eng._lib = load()
         pv = np.ascontiguousarray(np.asarray(param_values, dtype=np.float64).reshape(-1, len(param_names)))
@@ -264,10 +272,12 @@ class Engine:
         gvals = np.ascontiguousarray([float(globals_[k]) for k in gnames], dtype=np.float64)
         ttypes = np.ascontiguousarray([int(t) for _, t in terms], dtype=np.int32)
         keep = [_strings(list(param_names)), _strings(gnames), _strings(list(integrals)),
-                _strings([v for v, _ in values]), _strings([e for _, e in values]), _strings([e for e, _ in terms])]
+                _strings([v for v, _ in values]), _strings([e for _, e in values]), _strings([e for e, _ in terms]),
+                _strings(list(param_derivs))]
         spec = CustomSpec(len(param_names), keep[0], _dp(pv), int(radius_param), len(gnames), keep[1],
                           _dp(gvals) if len(gnames) else None, len(integrals), keep[2], len(values), keep[3], keep[4],
-                          len(terms), keep[5], ttypes.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+                          len(terms), keep[5], ttypes.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                          len(param_derivs), keep[6])
         handle = ctypes.c_void_p()
         _check(eng._lib.gbi_create_custom(n, ctypes.byref(params), ctypes.byref(spec), ctypes.byref(handle)))
         eng._h = handle
@@ -279,7 +289,25 @@ class Engine:
         eng.n_systems = 1
         eng.offsets = np.array([0, n], dtype=np.int64)
         eng.n_values = len(values)
+        eng.n_params = len(param_names)
+        eng.global_names = gnames
+        eng.param_derivs = list(param_derivs)
         return eng
+
+    def custom_param_derivs(self) -> dict:
+        """dE/d(global) of the last evaluation for each name given as ``param_derivs``."""
+        out = np.zeros(max(1, len(self.param_derivs)))
+        _check(self._lib.gbi_get_custom_param_derivs(self._h, len(self.param_derivs), _dp(out)))
+        return {k: float(out[m]) for m, k in enumerate(self.param_derivs)}
+
+    def update_custom_parameters(self, param_values) -> None:
+        """New per-particle parameters (N x n_params) of a general-expression context."""
+        pv = np.ascontiguousarray(np.asarray(param_values, dtype=np.float64).reshape(self.n, self.n_params))
+        _check(self._lib.gbi_update_custom_parameters(self._h, self.n, _dp(pv)))
+
+    def update_constants(self, params, system: int = 0) -> None:
+        """New constants (Born map, prefactor, volume constants) of one system."""
+        _check(self._lib.gbi_update_constants(self._h, int(system), ctypes.byref(params)))
 
     def custom_values(self) -> np.ndarray:
         """Computed values of the last evaluation (n_values x N)."""

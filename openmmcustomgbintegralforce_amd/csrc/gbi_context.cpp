@@ -55,6 +55,7 @@ constexpr int kDefaultBlocksPerCU = 128;  // about one workgroup per occupied ce
 // 5k and 25k atoms (0.15 nm: 14 % slower, more distinct cells to bin; 0.30 nm: longer lists).
 constexpr double kDefaultTightGrid = 0.20;
 constexpr int kCutoffTileBlocks = 3;  // column blocks per pair tile of a cutoff system
+constexpr double kSigmoidFp64Above = 16.0;  // |beta lambda| above which GBMV sums S in FP64
 constexpr size_t kOwnAllocation = 2u << 20;  // bytes: a buffer not sub-allocated beside others
 
 template <class T>
@@ -134,8 +135,9 @@ struct gbi_ctx {
 
     // general expressions (one system)
     std::unique_ptr<gbi::CustomProgram> cprog;
+    int custom_nparams = 0, custom_radius = 0;
     gbi::CustomLaunch claunch;
-    DevBuf<double> c_prm, c_glob, c_vals, c_dvdi, c_esp, c_despdi, c_part;
+    DevBuf<double> c_prm, c_glob, c_vals, c_dvdi, c_esp, c_despdi, c_part, c_dvdth, c_despdth, c_pdth;
     DevBuf<int> c_ex_ptr, c_ex_idx;
     bool any_cut = false, any_trunc = false;
     DevBuf<float> colpart;
@@ -193,7 +195,7 @@ struct gbi_ctx {
         excl_start.release();
         excl_col.release();
         excl_mask.release();
-        DevBuf<double>* cb[] = {&c_prm, &c_glob, &c_vals, &c_dvdi, &c_esp, &c_despdi, &c_part};
+        DevBuf<double>* cb[] = {&c_prm, &c_glob, &c_vals, &c_dvdi, &c_esp, &c_despdi, &c_part, &c_dvdth, &c_despdth, &c_pdth};
         for (auto* b : cb) b->release();
         c_ex_ptr.release();
         c_ex_idx.release();
@@ -367,6 +369,14 @@ gbi::SysDesc make_desc(const gbi_ctx& c, const SysHost& s, long long cell_base, 
                 : s.vol == gbi::VOL_GBMV2 ? 0.0f
                                           : (float)(1.0 / (1.0 + std::exp(P.beta * (0.0 - P.lambda))));
     d.sign_chain = s.vol == gbi::VOL_GBSW ? 1.0f : -1.0f;
+    // V = 1/(1 + exp(beta (S - lambda))) moves by |beta| S eps_S for a relative error eps_S of
+    // S, and its derivative by about as much relative to itself: the FP32 sums' eps_S ~ 3e-7
+    // stays well inside the force tolerance up to |beta lambda| ~ 16 (the reference constants
+    // have 10); steeper or later sigmoids sum S in FP64
+    d.acc64 = (s.vol != gbi::VOL_GBSW && std::fabs(P.beta * P.lambda) > kSigmoidFp64Above) ? 1 : 0;
+    d.beta64 = P.beta;
+    d.lambda64 = P.lambda;
+    d.S064 = P.S0;
     d.alpha0 = P.alpha0;
     d.alpha1 = P.alpha1;
     d.eta = P.eta;
@@ -908,7 +918,7 @@ gbi_status check_flags(gbi_ctx& c, bool& retry) {
         retry = true;
         return GBI_OK;
     }
-    if (all & gbi::FLAG_NONFINITE_RESULT) return fail(GBI_ERR_NONFINITE, "non-finite Born radius or energy");
+    if (all & gbi::FLAG_NONFINITE_RESULT) return fail(GBI_ERR_NONFINITE, "non-finite Born radius, energy or force");
     return GBI_OK;
 }
 
@@ -1040,7 +1050,7 @@ gbi_status create_batch(int S, const int32_t* n_atoms, const double* charge, con
     CTX_HIP(c->dedi.ensure(2 * N));
     CTX_HIP(c->atom_energy.ensure(N));
     CTX_HIP(c->energy.ensure(S));
-    CTX_HIP(c->stats.ensure(4));
+    CTX_HIP(c->stats.ensure(6));
     CTX_HIP(c->work.ensure(8));
     CTX_HIP(c->frec0.ensure(NQ));
     CTX_HIP(c->frec1.ensure(NQ));
@@ -1292,6 +1302,19 @@ gbi_status gbi_create_custom(int32_t n_atoms, const gbi_params* params, const gb
         cs.term_exprs.push_back(spec->term_exprs[k] ? spec->term_exprs[k] : "");
         cs.term_types.push_back(spec->term_types[k]);
     }
+    if (spec->n_param_derivs < 0 || spec->n_param_derivs > gbi::kMaxCustomDerivs ||
+        (spec->n_param_derivs > 0 && !spec->param_deriv_names))
+        return fail(GBI_ERR_INVALID_ARGUMENT, "at most 4 energy parameter derivatives, by name");
+    for (int m = 0; m < spec->n_param_derivs; ++m) {
+        const std::string name = spec->param_deriv_names[m] ? spec->param_deriv_names[m] : "";
+        int g = -1;
+        for (int k = 0; k < (int)cs.globals.size(); ++k)
+            if (cs.globals[k] == name) g = k;
+        if (g < 0)
+            return fail(GBI_ERR_INVALID_ARGUMENT,
+                        "energy parameter derivative of '" + name + "', which is not a global parameter");
+        cs.deriv_globals.push_back(g);
+    }
     // the expressions are checked before any device work
     auto prog = std::make_unique<gbi::CustomProgram>();
     try {
@@ -1319,12 +1342,16 @@ gbi_status gbi_create_custom(int32_t n_atoms, const gbi_params* params, const gb
     }
     const size_t N = n_atoms;
     const int NV = prog->n_values, NPL = prog->n_planes;
+    const size_t NDD = std::max(1, prog->n_derivs);
     HIP_TRY(c->c_prm.ensure(N * spec->n_params));
     HIP_TRY(c->c_glob.ensure(std::max(1, spec->n_globals)));
     HIP_TRY(c->c_vals.ensure(std::max<size_t>(1, NV * N)));
     HIP_TRY(c->c_dvdi.ensure(std::max<size_t>(1, 2 * NV * N)));
     HIP_TRY(c->c_esp.ensure(N));
     HIP_TRY(c->c_despdi.ensure(2 * N));
+    HIP_TRY(c->c_dvdth.ensure(std::max<size_t>(1, NV) * NDD * N));
+    HIP_TRY(c->c_despdth.ensure(NDD * N));
+    HIP_TRY(c->c_pdth.ensure(NDD * N));
     const size_t nb = (N + 63) / 64;
     HIP_TRY(c->c_part.ensure(nb * (nb + 1) / 2 * NPL * 64));
     HIP_TRY(hipMemcpyAsync(c->c_prm.p, spec->param_values, sizeof(double) * N * spec->n_params, hipMemcpyHostToDevice,
@@ -1351,6 +1378,11 @@ gbi_status gbi_create_custom(int32_t n_atoms, const gbi_params* params, const gb
     L.esp = c->c_esp.p;
     L.despdi = c->c_despdi.p;
     L.part = c->c_part.p;
+    L.dvdth = c->c_dvdth.p;
+    L.despdth = c->c_despdth.p;
+    L.pdth = c->c_pdth.p;
+    c->custom_nparams = spec->n_params;
+    c->custom_radius = spec->radius_param;
     c->cprog = std::move(prog);
     ST_TRY(build_custom_exclusions(*c));
     *out = guard.release();
@@ -1364,6 +1396,76 @@ gbi_status gbi_set_custom_globals(gbi_ctx* ctx, int32_t n_globals, const double*
     if (n_globals == 0) return GBI_OK;
     HIP_TRY(hipMemcpyAsync(ctx->c_glob.p, values, sizeof(double) * n_globals, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return GBI_OK;
+}
+
+gbi_status gbi_get_custom_param_derivs(gbi_ctx* ctx, int32_t n_derivs, double* out) {
+    if (!ctx || (n_derivs > 0 && !out)) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->cprog) return fail(GBI_ERR_INVALID_ARGUMENT, "not a general-expression context");
+    if (n_derivs != ctx->cprog->n_derivs) return fail(GBI_ERR_INVALID_ARGUMENT, "wrong number of parameter derivatives");
+    if (!ctx->computed) return fail(GBI_ERR_INVALID_ARGUMENT, "no evaluation yet");
+    if (n_derivs == 0) return GBI_OK;
+    std::vector<double> per_atom((size_t)n_derivs * ctx->N);
+    HIP_TRY(hipMemcpyAsync(per_atom.data(), ctx->c_pdth.p, sizeof(double) * per_atom.size(), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int m = 0; m < n_derivs; ++m) {  // fixed order: atom by atom
+        double acc = 0.0;
+        for (int a = 0; a < ctx->N; ++a) acc += per_atom[(size_t)m * ctx->N + a];
+        out[m] = acc;
+    }
+    return GBI_OK;
+}
+
+gbi_status gbi_update_custom_parameters(gbi_ctx* ctx, int32_t n_atoms, const double* param_values) {
+    if (!ctx || !param_values) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ctx->cprog) return fail(GBI_ERR_INVALID_ARGUMENT, "not a general-expression context");
+    if (n_atoms != ctx->N)  // CharmmReferenceKernels.cpp:373-374
+        return fail(GBI_ERR_PARTICLE_COUNT, "updateParametersInContext: The number of particles has changed");
+    const size_t np = ctx->custom_nparams;
+    for (size_t k = 0; k < (size_t)n_atoms * np; ++k)
+        if (!std::isfinite(param_values[k])) return fail(GBI_ERR_NONFINITE, "non-finite per-particle parameter");
+    std::vector<double> radius(n_atoms);
+    for (int a = 0; a < n_atoms; ++a) radius[a] = param_values[(size_t)a * np + ctx->custom_radius];
+    ST_TRY(check_atoms(n_atoms, ctx->charge.data(), radius.data()));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->c_prm.p, param_values, sizeof(double) * n_atoms * np, hipMemcpyHostToDevice,
+                           ctx->stream));
+    ctx->radius = radius;
+    ST_TRY(upload_atoms(*ctx));
+    if (ctx->have_positions) {
+        if (ctx->cell_total > 0) ST_TRY(layout(*ctx));
+        return size_from_device(*ctx, 0);
+    }
+    return GBI_OK;
+}
+
+gbi_status gbi_update_constants(gbi_ctx* ctx, int32_t system, const gbi_params* params) {
+    if (!ctx || !params) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
+    if (system < 0 || system >= ctx->S) return fail(GBI_ERR_INVALID_ARGUMENT, "no such system");
+    SysHost& s = ctx->sys[system];
+    gbi_params P = *params;
+    if (P.model == GBI_MODEL_GBSW) P.integral_type = GBI_INTEGRAL_GBSW;
+    if (P.tight_grid_length <= 0.0) P.tight_grid_length = s.P.tight_grid_length;
+    if (s.custom) {  // a general-expression system keeps the integral orders of its spec
+        P.orders[0] = s.P.orders[0];
+        P.orders[1] = s.P.orders[1];
+    }
+    // everything but the constants must stay as it is
+    const gbi_params& o = s.P;
+    const bool same = P.model == o.model && P.integral_type == o.integral_type && P.lookup_policy == o.lookup_policy &&
+                      P.nonbonded_method == o.nonbonded_method && P.cutoff == o.cutoff && P.n_radial == o.n_radial &&
+                      P.n_lebedev == o.n_lebedev && P.r0 == o.r0 && P.r1 == o.r1 && P.grid_length == o.grid_length &&
+                      P.buffer == o.buffer && P.cell_cap == o.cell_cap && P.orders[0] == o.orders[0] &&
+                      P.orders[1] == o.orders[1] && P.tight_grid_length == o.tight_grid_length &&
+                      P.block_volume == o.block_volume && P.block_force == o.block_force;
+    if (!same) return fail(GBI_ERR_INVALID_ARGUMENT, "gbi_update_constants changes only the model's constants");
+    ST_TRY(validate_params(P, s.N, s.custom));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    s.P = P;
+    ST_TRY(upload_atoms(*ctx));  // the volume constants enter each atom's parameters and reach
+    if (ctx->cell_total > 0) ST_TRY(layout(*ctx));  // the descriptors; drops captured graphs
+    if (ctx->have_positions && ctx->cell_total > 0) return size_from_device(*ctx, 0);
     return GBI_OK;
 }
 
@@ -1487,10 +1589,10 @@ gbi_status gbi_get_integrals(gbi_ctx* ctx, double* integrals) {
 gbi_status gbi_get_stats(gbi_ctx* ctx, gbi_stats* stats) {
     if (!ctx || !stats) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
     if (!ctx->computed) return fail(GBI_ERR_INVALID_ARGUMENT, "nothing computed yet");
-    HIP_TRY(hipMemsetAsync(ctx->stats.p, 0, sizeof(double) * 4, ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->stats.p, 0, sizeof(double) * 6, ctx->stream));
     gbi::KParams k = make_kparams(*ctx, false);
     HIP_TRY(gbi::launch_stats(k, ctx->stats.p, ctx->stream));
-    double h[4];
+    double h[6];
     HIP_TRY(hipMemcpyAsync(h, ctx->stats.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     stats->n_points = h[0];
@@ -1499,6 +1601,8 @@ gbi_status gbi_get_stats(gbi_ctx* ctx, gbi_stats* stats) {
     stats->n_entries = h[3];
     stats->n_atoms = ctx->N;
     stats->n_quad = (double)ctx->NQ / ctx->N;
+    stats->n_active = h[4];
+    stats->n_frc_pairs = h[5];
     return GBI_OK;
 }
 

@@ -329,19 +329,35 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
     for (const auto& n : spec.value_names) name_ok(n, "computed value");
     for (int t : spec.term_types)
         if (t < TERM_SINGLE || t > TERM_PAIR_NO_EXCL) throw std::runtime_error("unknown energy term type");
+    const int ND = (int)spec.deriv_globals.size();
+    if (ND > kMaxCustomDerivs) throw std::runtime_error("at most 4 energy parameter derivatives are supported");
+    std::vector<int> deriv_slot(NG, -1);  // global k -> its derivative slot
+    for (int m = 0; m < ND; ++m) {
+        const int g = spec.deriv_globals[m];
+        if (g < 0 || g >= NG || deriv_slot[g] >= 0)
+            throw std::runtime_error("an energy parameter derivative must name a global parameter, once");
+        deriv_slot[g] = m;
+    }
 
     std::ostringstream s;
     s << kDualSource;
-    s << "#define NP " << NP << "\n#define NG " << (NG > 0 ? NG : 1) << "\n#define NV " << NV << "\n";
-    const int NPL = (4 + NV) + (3 + NV);  // row and column planes of a pair tile
+    s << "#define NP " << NP << "\n#define NG " << (NG > 0 ? NG : 1) << "\n#define NV " << NV << "\n#define ND "
+      << ND << "\n#define NDD " << (ND > 0 ? ND : 1) << "\n";
+    const int NPL = (4 + NV + ND) + (3 + NV);  // row and column planes of a pair tile
 
-    // ---- per atom: computed values and single-particle terms, duals over the integrals ----
+    // ---- per atom: computed values and single-particle terms, duals over the integrals and
+    // the globals whose energy derivative is asked for ----
     s << "extern \"C\" __global__ void cg_values(int N, const double* __restrict__ prm, const double* __restrict__ "
          "integ, const double* __restrict__ glob, double* __restrict__ vals, double* __restrict__ dvdi, double* "
-         "__restrict__ esp, double* __restrict__ despdi) {\n"
+         "__restrict__ esp, double* __restrict__ despdi, double* __restrict__ dvdth, double* __restrict__ despdth) {\n"
          "    const int i = blockIdx.x * blockDim.x + threadIdx.x;\n    if (i >= N) return;\n"
-         "    typedef Du<double, 2> D;\n";
-    for (int k = 0; k < NG; ++k) s << "    const D G" << k << " = D::c(glob[" << k << "]);\n";
+         "    typedef Du<double, 2 + ND> D;\n";
+    for (int k = 0; k < NG; ++k) {
+        if (deriv_slot[k] >= 0)
+            s << "    const D G" << k << " = D::var(glob[" << k << "], " << 2 + deriv_slot[k] << ");\n";
+        else
+            s << "    const D G" << k << " = D::c(glob[" << k << "]);\n";
+    }
     for (int k = 0; k < NP; ++k) s << "    const D P_" << k << " = D::c(prm[(size_t)i * NP + " << k << "]);\n";
     for (int k = 0; k < NI; ++k) s << "    const D I_" << k << " = D::var(integ[" << k << " * N + i], " << k << ");\n";
     for (int k = 0; k < NV; ++k) {
@@ -349,12 +365,14 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
         s << "    vals[" << k << " * N + i] = V_" << k << ".v;\n";
         s << "    dvdi[(" << k << " * 2 + 0) * N + i] = V_" << k << ".d[0];\n";
         s << "    dvdi[(" << k << " * 2 + 1) * N + i] = V_" << k << ".d[1];\n";
+        s << "    for (int m = 0; m < ND; ++m) dvdth[(" << k << " * NDD + m) * N + i] = V_" << k << ".d[2 + m];\n";
     }
     s << "    D E_ = D::c(0.0);\n";
     for (size_t t = 0; t < spec.term_exprs.size(); ++t)
         if (spec.term_types[t] == TERM_SINGLE)
             s << "    E_ = E_ + (" << translate_expression(spec.term_exprs[t], spec, 1, 0) << ");\n";
-    s << "    esp[i] = E_.v;\n    despdi[i] = E_.d[0];\n    despdi[N + i] = E_.d[1];\n}\n";
+    s << "    esp[i] = E_.v;\n    despdi[i] = E_.d[0];\n    despdi[N + i] = E_.d[1];\n"
+         "    for (int m = 0; m < ND; ++m) despdth[m * N + i] = E_.d[2 + m];\n}\n";
 
     // ---- pairs: 64 x 64 tiles (row block I, column block J >= I), each pair once with atom1
     // the row atom (atom1 < atom2 as the reference orders a pair, CharmmReferenceGBMV.cpp:
@@ -371,7 +389,7 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
         (spec.term_types[t] == TERM_PAIR ? pair_excl : pair_noexcl) += e;
         out.any_pair = true;
     }
-    s << "#define NPR (4 + NV)\n#define NPC (3 + NV)\n#define CG_WAVES " << kCustomPairWaves << "\n"
+    s << "#define NPR (4 + NV + ND)\n#define NPC (3 + NV)\n#define CG_WAVES " << kCustomPairWaves << "\n"
          "#define CG_STEPS (64 / CG_WAVES)\n"
          "__device__ inline float rot_down(float v) {\n"
          "    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));\n}\n"
@@ -396,14 +414,20 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
          "        for (int k = 0; k < NV; ++k) s_v[k][lane] = jv ? (float)vals[k * N + jl] : 0.0f;\n"
          "    }\n"
          "    __syncthreads();\n"
-         "    typedef Du<float, 1 + 2 * NV> D;\n";
-    for (int k = 0; k < NG; ++k) s << "    const D G" << k << " = D::c(glob[" << k << "]);\n";
+         "    typedef Du<float, 1 + 2 * NV + ND> D;\n";
+    for (int k = 0; k < NG; ++k) {
+        if (deriv_slot[k] >= 0)
+            s << "    const D G" << k << " = D::var((float)glob[" << k << "], 1 + 2 * NV + " << deriv_slot[k] << ");\n";
+        else
+            s << "    const D G" << k << " = D::c(glob[" << k << "]);\n";
+    }
     s << "    const double xi = iv ? pos[3 * i] : 0.0, yi = iv ? pos[3 * i + 1] : 0.0, zi = iv ? pos[3 * i + 2] : 0.0;\n";
     for (int k = 0; k < NP; ++k) s << "    const D P1_" << k << " = D::c(iv ? prm[(size_t)i * NP + " << k << "] : 0.0);\n";
     for (int k = 0; k < NV; ++k) s << "    const D V1_" << k << " = D::var(iv ? vals[" << k << " * N + i] : 0.0, " << 1 + k << ");\n";
     s << "    float fx = 0.f, fy = 0.f, fz = 0.f, e = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;\n"
-         "    float dv1[NV > 0 ? NV : 1], cv[NV > 0 ? NV : 1];\n"
+         "    float dv1[NV > 0 ? NV : 1], cv[NV > 0 ? NV : 1], dth[NDD];\n"
          "    for (int k = 0; k < NV; ++k) dv1[k] = cv[k] = 0.f;\n"
+         "    for (int m = 0; m < ND; ++m) dth[m] = 0.f;\n"
          "    const bool has_ex = ex_ptr[N] > 0;\n"
          "    for (int m = CG_STEPS * w; m < CG_STEPS * (w + 1); ++m) {\n"
          "        const int c = (lane + m) & 63;\n"
@@ -443,6 +467,7 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
          "            cx -= gx;\n            cy -= gy;\n            cz -= gz;\n"
          "            for (int k = 0; k < NV; ++k) {\n"
          "                dv1[k] += E_.d[1 + k];\n                cv[k] += E_.d[1 + NV + k];\n            }\n"
+         "            for (int m = 0; m < ND; ++m) dth[m] += E_.d[1 + 2 * NV + m];\n"
          "        }\n"
          "        cx = rot_down(cx);\n        cy = rot_down(cy);\n        cz = rot_down(cz);\n"
          "        for (int k = 0; k < NV; ++k) cv[k] = rot_down(cv[k]);\n"
@@ -452,6 +477,7 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
          "    s_red[w][0][lane] = fx;\n    s_red[w][1][lane] = fy;\n    s_red[w][2][lane] = fz;\n"
          "    s_red[w][3][lane] = e;\n"
          "    for (int k = 0; k < NV; ++k) s_red[w][4 + k][lane] = dv1[k];\n"
+         "    for (int m = 0; m < ND; ++m) s_red[w][4 + NV + m][lane] = dth[m];\n"
          "    s_red[w][NPR][cl] = cx;\n    s_red[w][NPR + 1][cl] = cy;\n    s_red[w][NPR + 2][cl] = cz;\n"
          "    for (int k = 0; k < NV; ++k) s_red[w][NPR + 3 + k][cl] = cv[k];\n"
          "    __syncthreads();\n"
@@ -467,7 +493,8 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
     s << "#define CG_CHAIN_WAVES " << kCustomChainWaves << "\n";
     s << "extern \"C\" __global__ void __launch_bounds__(64 * CG_CHAIN_WAVES) cg_chain(int N, int nb, const double* "
          "__restrict__ part, const double* __restrict__ dvdi, const double* __restrict__ esp, const double* __restrict__ "
-         "despdi, double* __restrict__ atom_energy, long long* __restrict__ force_fix, double* __restrict__ dedi) {\n"
+         "despdi, double* __restrict__ atom_energy, long long* __restrict__ force_fix, double* __restrict__ dedi, "
+         "const double* __restrict__ dvdth, const double* __restrict__ despdth, double* __restrict__ pdth) {\n"
          "    const int I = blockIdx.x, l = threadIdx.x & 63, w = threadIdx.x >> 6;\n"
          "    const int a = 64 * I + l;\n"
          "    __shared__ double s_sum[CG_CHAIN_WAVES][NPR + NPC][64];\n"
@@ -493,12 +520,19 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
          "    for (int m = 0; m < 2; ++m) {\n"
          "        double g = despdi[m * N + a];\n"
          "        for (int k = 0; k < NV; ++k) g += (r[4 + k] + c[3 + k]) * dvdi[(k * 2 + m) * N + a];\n"
-         "        dedi[m * N + a] = g;\n    }\n}\n";
+         "        dedi[m * N + a] = g;\n    }\n"
+         "    // dE/dtheta of this atom: its single-particle terms (through the values too), its pair\n"
+         "    // terms' explicit derivative (row side: each pair once), and dE/dV dV/dtheta\n"
+         "    for (int m = 0; m < ND; ++m) {\n"
+         "        double g = despdth[m * N + a] + r[4 + NV + m];\n"
+         "        for (int k = 0; k < NV; ++k) g += (r[4 + k] + c[3 + k]) * dvdth[(k * NDD + m) * N + a];\n"
+         "        pdth[m * N + a] = g;\n    }\n}\n";
     out.source = s.str();
     out.n_params = NP;
     out.n_globals = NG;
     out.n_values = NV;
     out.n_planes = NPL;
+    out.n_derivs = ND;
 }
 
 void compile_custom_program(const std::string& arch, CustomProgram& out) {

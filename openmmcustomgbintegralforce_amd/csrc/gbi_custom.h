@@ -14,6 +14,7 @@
 namespace gbi {
 
 constexpr int kMaxCustomValues = 4;
+constexpr int kMaxCustomDerivs = 4;  // energy derivatives with respect to global parameters
 constexpr int kMaxCustomIntegrals = 2;
 constexpr int kCustomPairWaves = 4;  // waves of a cg_pairs workgroup (one 64 x 64 tile)
 constexpr int kCustomChainWaves = 4; // waves of a cg_chain workgroup (one 64-atom block)
@@ -27,21 +28,25 @@ struct CustomSpec {
     std::vector<std::string> value_names, value_exprs;
     std::vector<std::string> term_exprs;
     std::vector<int> term_types;
+    std::vector<int> deriv_globals;      // globals whose energy derivative is computed (indices)
 };
 
-// Compiled kernels of one spec.  Kernel signatures (all arrays device, double unless noted):
+// Compiled kernels of one spec.  Kernel signatures (all arrays device, double unless noted;
+// ND = the number of parameter derivatives, arrays sized with max(ND, 1)):
 //   cg_values(int N, prm[N x NP], integ[2 x N], glob[NG], vals[NV x N], dvdi[NV x 2 x N],
-//             esp[N], despdi[2 x N])                                  grid N/256, block 256
+//             esp[N], despdi[2 x N], dvdth[NV x ND x N], despdth[ND x N])
+//                                                                     grid N/256, block 256
 //   cg_pairs(int N, int nb, pos[N x 3], prm, vals, glob, int method, double cut2,
 //            const double* box3, const int* ex_ptr, const int* ex_idx,
 //            part[nb (nb + 1) / 2 tiles x NPL x 64])     grid (nb, nb), block 64 kCustomPairWaves
 //   cg_chain(int N, int nb, part, dvdi, esp, despdi, atom_energy[N], long long* force_fix[3 x N],
-//            dedi[2 x N])                                  grid nb, block 64 kCustomChainWaves
-// (nb = ceil(N / 64) atom blocks; NPL = n_planes planes per tile: 4 + NV row, 3 + NV column)
+//            dedi[2 x N], dvdth, despdth, pdth[ND x N])    grid nb, block 64 kCustomChainWaves
+// (nb = ceil(N / 64) atom blocks; NPL = n_planes planes per tile: 4 + NV + ND row, 3 + NV
+// column; pdth holds each atom's share of dE/dtheta, summed over atoms on the host)
 struct CustomProgram {
     hipModule_t module = nullptr;
     hipFunction_t values = nullptr, pairs = nullptr, chain = nullptr;
-    int n_params = 0, n_globals = 0, n_values = 0, n_planes = 0;
+    int n_params = 0, n_globals = 0, n_values = 0, n_planes = 0, n_derivs = 0;
     bool any_pair = false;
     std::string source;
     CustomProgram() = default;

@@ -104,6 +104,8 @@ struct SysDesc {
     float sw, inv_sw, rscale, roffset;
     float gamma0, beta, lambda, S0, vguard, fclamp, P1;
     float v_empty;             // V of a point with an empty list
+    int acc64;                 // GBMV: the sigmoid's S summed in FP64 (steep sigmoids at large S, |beta lambda| > 16)
+    double beta64, lambda64, S064;  // the constants of that FP64 evaluation
     float sign_chain;          // +1 for GBSW (I accumulates 1 - V), -1 for GBMV
     // Born map / energy
     double alpha0, alpha1, eta, slope, shift, prefactor;
@@ -125,6 +127,9 @@ struct CustomLaunch {
     double* esp = nullptr;
     double* despdi = nullptr;
     double* part = nullptr;
+    double* dvdth = nullptr;   // dV/dtheta per value, derivative and atom
+    double* despdth = nullptr; // single-particle dE/dtheta per atom
+    double* pdth = nullptr;    // each atom's share of dE/dtheta
     const int* ex_ptr = nullptr;
     const int* ex_idx = nullptr;
 };
@@ -230,6 +235,6 @@ struct KParams {
 // kinds: which volume kinds occur in the batch.
 constexpr int kPasses = 8;
 hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stream, const hipEvent_t* marks);
-hipError_t launch_stats(const KParams& p, double* out4, hipStream_t stream);
+hipError_t launch_stats(const KParams& p, double* out6, hipStream_t stream);
 
 }  // namespace gbi

@@ -74,6 +74,11 @@ __device__ inline float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 // Exact sum of quantised partials: round x to the 2^-24 grid; the double holds an integer.
 __device__ inline double quant(float x) { return (double)rintf(x * kQuantum); }
 
+// Largest |value| the int64 2^-32 fixed point holds, in 2^-24 units and as a plain number
+// (2^62 / 2^8 and 2^62 / 2^32: room for the sums of a few such values).
+constexpr double kFixLimit24 = 1.8e16;
+constexpr double kFixLimit = 1.0e9;
+
 // 2^-24 units (exact integers in a double) -> int64 in 2^-32 units.
 __device__ inline void add_fix(long long* dst, double units24) {
     atomicAdd((unsigned long long*)dst, (unsigned long long)(long long)(units24 * 256.0));
@@ -773,6 +778,16 @@ __global__ void k_worklist(KParams p) {
 // 2e-38 as well, so the FP64 sigmoid agrees to that level whatever beta and lambda are.
 constexpr float kSigmoidSat = 87.0f;
 
+// V = 1/(1 + exp(t)) and V (1 - V) = exp(t)/(1 + exp(t))^2, from e = exp(-|t|) <= 1 so that
+// nothing overflows for |t| up to kSigmoidSat (exp(t) (1 + exp(t))^-2 written directly
+// overflows FP32 from t = 44 on).
+__device__ inline void sigmoid(float t, float& V, float& vv1) {
+    const float e = __expf(-fabsf(t));
+    const float ie = 1.f / (1.f + e);
+    V = t <= 0.f ? ie : e * ie;
+    vv1 = e * ie * ie;
+}
+
 // V >= 0 with the sign bit as the "has a chain-rule factor" mark (also set for V = 0)
 __device__ inline float mark_active(float v, bool active) {
     return __int_as_float(__float_as_int(v) | (active ? (int)0x80000000 : 0));
@@ -801,6 +816,7 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
         const int pt_base = sd.pt_base, abeg = sd.atom_begin, Q = sd.Q;
         const float k1 = sd.k1, k3 = sd.k3, fclamp = sd.fclamp, S0 = sd.S0, vguard = sd.vguard;
         const float beta = sd.beta, lambda = sd.lambda;
+        const bool acc64 = VK != VOL_GBSW && sd.acc64;
         auto stage = [&](int ab, int cnt) {
             for (int k = lane; k < cnt; k += kWave) {
                 const float4 e = p.ent_pos[as + ab + k];
@@ -833,6 +849,7 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
             float V = 1.f;
             bool done = !valid;
             float s1 = 0.f, s2 = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;
+            double S1 = 0.0, S2 = 0.0, VX = 0.0, VY = 0.0, VZ = 0.0;  // acc64 systems
             for (int ab = 0; ab < na; ab += kMaxAtomChunk) {
                 const int cnt = min(kMaxAtomChunk, na - ab);
                 if (!single) {
@@ -859,7 +876,7 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                         }
                     }
                 } else if (VK == VOL_GBMV2) {
-                    if (valid) {
+                    if (valid && !acc64) {
                         #pragma unroll 4
                         for (int m = 0; m < cnt; ++m) {
                             const float dx = s_x[m] - prx, dy = s_y[m] - pry, dz = s_z[m] - prz;
@@ -873,14 +890,36 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                             vy = fmaf(F, dy, vy);
                             vz = fmaf(F, dz, vz);
                         }
+                    } else if (valid) {
+                        // the same terms, summed in FP64
+                        #pragma unroll 2
+                        for (int m = 0; m < cnt; ++m) {
+                            const float dx = s_x[m] - prx, dy = s_y[m] - pry, dz = s_z[m] - prz;
+                            const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                            const float t = fmaf(100.f, d2, s_b[m]);
+                            const float ct = s_a[m] * fast_rcp(t);
+                            const double F = (double)fminf(ct * ct, fclamp);
+                            S1 += F;
+                            S2 = fma((double)d2, F * F, S2);
+                            VX = fma(F, (double)dx, VX);
+                            VY = fma(F, (double)dy, VY);
+                            VZ = fma(F, (double)dz, VZ);
+                        }
                     }
                 } else {
-                    if (valid) {
+                    if (valid && !acc64) {
                         #pragma unroll 4
                         for (int m = 0; m < cnt; ++m) {
                             const float dx = s_x[m] - prx, dy = s_y[m] - pry, dz = s_z[m] - prz;
                             const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                             s1 += __expf(s_a[m] * (d2 * d2));
+                        }
+                    } else if (valid) {
+                        #pragma unroll 2
+                        for (int m = 0; m < cnt; ++m) {
+                            const float dx = s_x[m] - prx, dy = s_y[m] - pry, dz = s_z[m] - prz;
+                            const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                            S1 += (double)__expf(s_a[m] * (d2 * d2));
                         }
                     }
                 }
@@ -894,19 +933,35 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                     p.vval[pid] = mark_active(V, active);
                     r1.x = V;
                 } else if (VK == VOL_GBMV2) {
-                    const float vsq = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
-                    // |v|^2 below the guard: the S -> +inf limit; otherwise the sigmoid's
-                    // exponent t decides (its limit beyond +-kSigmoidSat)
-                    const bool guard = !(vsq >= vguard);
-                    const float S = guard ? 0.f : S0 * s1 * s2 / vsq;
-                    const float t = beta * (S - lambda);
+                    float vsq, S, t;
+                    bool guard;
+                    if (acc64) {
+                        // S and t = beta (S - lambda) in FP64, then the FP32 record
+                        const double vsq64 = fma(VX, VX, fma(VY, VY, VZ * VZ));
+                        guard = !(vsq64 >= (double)vguard);
+                        const double S64 = guard ? 0.0 : sd.S064 * S1 * S2 / vsq64;
+                        S = (float)S64;
+                        t = (float)(sd.beta64 * (S64 - sd.lambda64));
+                        s1 = (float)S1;
+                        s2 = (float)S2;
+                        vx = (float)VX;
+                        vy = (float)VY;
+                        vz = (float)VZ;
+                        vsq = (float)vsq64;
+                    } else {
+                        vsq = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
+                        // |v|^2 below the guard: the S -> +inf limit; otherwise the sigmoid's
+                        // exponent t decides (its limit beyond +-kSigmoidSat)
+                        guard = !(vsq >= vguard);
+                        S = guard ? 0.f : S0 * s1 * s2 / vsq;
+                        t = beta * (S - lambda);
+                    }
                     const bool sat = guard || !(fabsf(t) <= kSigmoidSat);
                     float Vv = (guard ? beta < 0.f : t < 0.f) ? 1.f : 0.f;
                     if (!sat) {
-                        const float pre = __expf(t);
-                        const float opre = 1.f + pre;
-                        Vv = 1.f / opre;
-                        const float kk = -beta * pre / (opre * opre) * S0;
+                        float vv1;
+                        sigmoid(t, Vv, vv1);
+                        const float kk = -beta * vv1 * S0;
                         const float ivsq = 1.f / vsq;
                         r1 = make_float4(kk * s2 * ivsq, 2.f * kk * s1 * ivsq, -2.f * kk * s1 * s2 * ivsq * ivsq, vx);
                         r2 = make_float4(vy, vz, Vv, 0.f);
@@ -915,13 +970,13 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
                     p.vval[pid] = mark_active(Vv, active);
                     if (p.adjoint) p.vsig[pid] = sat ? __int_as_float(0x7fc00000) : S;
                 } else {
-                    const float t = beta * (s1 - lambda);
+                    if (acc64) s1 = (float)S1;
+                    const float t = acc64 ? (float)(sd.beta64 * (S1 - sd.lambda64)) : beta * (s1 - lambda);
                     float Vv = t < 0.f ? 1.f : 0.f;
                     if (fabsf(t) <= kSigmoidSat) {
-                        const float pre = __expf(t);
-                        const float opre = 1.f + pre;
-                        Vv = 1.f / opre;
-                        r1.x = -beta * pre / (opre * opre);
+                        float vv1;
+                        sigmoid(t, Vv, vv1);
+                        r1.x = -beta * vv1;
                     }
                     r1.y = Vv;
                     active = r1.x != 0.f;
@@ -1369,6 +1424,8 @@ __global__ void __launch_bounds__(kChainGroups * kWave) k_atom_chain(KParams p) 
     e += 0.5 * P * q * q / B;
     dedb += -0.5 * P * q * q / (B * B);
     p.atom_energy[ia] = e;
+    if (!(fabs(fx) < kFixLimit && fabs(fy) < kFixLimit && fabs(fz) < kFixLimit))
+        raise_flag(p, p.atom_sys[ia], FLAG_NONFINITE_RESULT);
     p.force_fix[ia] = (long long)llrint(fx * kFix32);
     p.force_fix[p.N + ia] = (long long)llrint(fy * kFix32);
     p.force_fix[2 * p.N + ia] = (long long)llrint(fz * kFix32);
@@ -1667,7 +1724,9 @@ __global__ void __launch_bounds__(64, (ADJ || TRUNC) ? 2 : 3) k_force(KParams p)
                 for (int g = 0; g < 8; ++g) tot += s_red[lane >> 3][(lane & 7) + 8 * g];
                 __syncthreads();
                 if (my_atom >= 0 && tot != 0.0) {
-                    if (d < 3)
+                    if (!(fabs(tot) < kFixLimit24))  // NaN, inf, or beyond the int64 range
+                        raise_flag(p, s, FLAG_NONFINITE_RESULT);
+                    else if (d < 3)
                         add_fix(&p.force_fix[d * p.N + my_atom], tot);
                     else
                         add_fix(&p.dedr_fix[my_atom], tot);
@@ -1734,6 +1793,8 @@ __global__ void __launch_bounds__(64) k_owner_reduce(KParams p) {
         ts = wave_sum(ts);
     }
     if (lane == 0) {
+        if (!(fabs(a[0]) < kFixLimit && fabs(a[1]) < kFixLimit && fabs(a[2]) < kFixLimit))
+            raise_flag(p, s, FLAG_NONFINITE_RESULT);
         for (int d = 0; d < 3; ++d) p.force_fix[d * p.N + i] += (long long)llrint(a[d] * kFix32);
         if (ADJ) {
             p.theta_atom[TH_BETA * p.N + i] = tb;
@@ -1802,20 +1863,29 @@ __global__ void __launch_bounds__(256) k_theta_reduce(KParams p) {
     }
 }
 
+// Points, point/atom pairs of the volume pass, and the active points and point/atom pairs
+// of the chain-rule pass (counts of the last evaluation; diagnostics only).
 __global__ void k_stats(KParams p, double* out) {
     if (batch_fatal(p)) return;
-    double np = 0.0, pairs = 0.0;
+    double np = 0.0, pairs = 0.0, nact = 0.0, fpairs = 0.0;
     const int stride = gridDim.x * blockDim.x;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < p.cell_total; c += stride) {
         const double a = p.cell_pt_count[c];
+        const double act = a > 0.0 ? p.cell_act_count[c] : 0.0;  // written only for cells with points
         np += a;
         pairs += a * p.cell_count_eff[c];
+        nact += act;
+        fpairs += act * p.cell_count_eff[c];
     }
     np = wave_sum(np);
     pairs = wave_sum(pairs);
+    nact = wave_sum(nact);
+    fpairs = wave_sum(fpairs);
     if ((threadIdx.x & (kWave - 1)) == 0) {
         atomicAdd(&out[0], np);
         atomicAdd(&out[1], pairs);
+        atomicAdd(&out[4], nact);
+        atomicAdd(&out[5], fpairs);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         double nc = 0.0;
@@ -1881,18 +1951,19 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
         const double* integ = p.integrals;
         const double *prm = c.prm, *glob = c.glob, *box = c.box;
         double *vals = c.vals, *dvdi = c.dvdi, *esp = c.esp, *despdi = c.despdi, *part = c.part;
+        double *dvdth = c.dvdth, *despdth = c.despdth, *pdth = c.pdth;
         const int *ex_ptr = c.ex_ptr, *ex_idx = c.ex_idx;
         double* atom_energy = p.atom_energy;
         long long* force_fix = p.force_fix;
         double* dedi = p.dedi;
-        void* va[] = {&N, &prm, &integ, &glob, &vals, &dvdi, &esp, &despdi};
+        void* va[] = {&N, &prm, &integ, &glob, &vals, &dvdi, &esp, &despdi, &dvdth, &despdth};
         err = hipModuleLaunchKernel(c.values, (N + 255) / 256, 1, 1, 256, 1, 1, 0, stream, va, nullptr);
         if (err != hipSuccess) return err;
         void* pa[] = {&N, &nb, &pos, &prm, &vals, &glob, &method, &cut2, &box, &ex_ptr, &ex_idx, &part};
         err = hipModuleLaunchKernel(c.pairs, nb, nb, 1, c.pair_threads, 1, 1, 0, stream, pa, nullptr);
         if (err != hipSuccess) return err;
         mark(5);
-        void* ca[] = {&N, &nb, &part, &dvdi, &esp, &despdi, &atom_energy, &force_fix, &dedi};
+        void* ca[] = {&N, &nb, &part, &dvdi, &esp, &despdi, &atom_energy, &force_fix, &dedi, &dvdth, &despdth, &pdth};
         err = hipModuleLaunchKernel(c.chain, nb, 1, 1, c.chain_threads, 1, 1, 0, stream, ca, nullptr);
         if (err != hipSuccess) return err;
     } else if (adj) {
@@ -1937,8 +2008,8 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
     return hipGetLastError();
 }
 
-hipError_t launch_stats(const KParams& p, double* out4, hipStream_t stream) {
-    hipLaunchKernelGGL(k_stats, dim3(256), dim3(256), 0, stream, p, out4);
+hipError_t launch_stats(const KParams& p, double* out6, hipStream_t stream) {
+    hipLaunchKernelGGL(k_stats, dim3(256), dim3(256), 0, stream, p, out6);
     return hipGetLastError();
 }
 

@@ -160,6 +160,12 @@ class CharmmGBMVForce:
     def addEnergyParameterDerivative(self, name: str) -> None:
         self._param_derivs.append(name)
 
+    def getNumEnergyParameterDerivatives(self) -> int:
+        return len(self._param_derivs)
+
+    def getEnergyParameterDerivativeName(self, i) -> str:
+        return self._param_derivs[i]
+
     def addParticle(self, parameters=()) -> int:
         self._particles.append([float(v) for v in parameters])
         return len(self._particles) - 1
@@ -183,6 +189,9 @@ class CharmmGBMVForce:
 
     def getGBIntegralParameters(self, i):
         return self._integrals[i]
+
+    def setGBIntegralParameters(self, i, name: str, parametersInt=(), parametersReal=()) -> None:
+        self._integrals[i] = (name, [int(v) for v in parametersInt], [float(v) for v in parametersReal])
 
     def setGBIntegralType(self, t: int) -> None:
         self._integral_type = int(t)
@@ -268,8 +277,12 @@ def gbmv_params_from_force(force: CharmmGBMVForce) -> tuple:
         if pi[0] < 2:
             raise OpenMMException("GBSWIntegral: the order of integral must be greater or equal to 2")
     orders = [pi[0] for pi in ints]
-    if sorted(orders) != [4, 7]:
-        raise N.GbiError(2, "the HIP platform implements the canonical Born map over integrals of order 4 and 7")
+    # the canonical Born expression names its integrals: I4 must be of order 4, I7 of order 7
+    named = {force.getGBIntegralParameters(i)[0]: o for i, o in enumerate(orders)}
+    if len(orders) != 2 or named != {"I4": 4, "I7": 7}:
+        raise N.GbiError(2, "the canonical Born map takes an integral I4 of order 4 and an integral I7 of order 7")
+    if force.getNumEnergyParameterDerivatives() > 0:
+        raise N.GbiError(2, "energy parameter derivatives run on the generated kernels")
     values = [(n, _norm(e), ty) for n, e, ty in force._values]
     energies = [(_norm(e), ty) for e, ty in force._energy]
     canon_values = [("B", _norm("slope*" + CANONICAL_BORN[0] + "+shift"), CharmmGBMVForce.SingleParticle)]
@@ -309,15 +322,30 @@ class System:
 
 class ContextImpl:
     """Holds positions (N x 3 nm), the periodic box vectors and the force accumulator, as
-    ReferencePlatform::PlatformData."""
+    ReferencePlatform::PlatformData; the global parameters of ``forces`` starting at their
+    defaults (OpenMM's ContextImpl constructor); and the energy-parameter derivatives the
+    kernels add into (``energy_param_derivs``)."""
 
-    def __init__(self, positions, box_vectors=None):
+    def __init__(self, positions, box_vectors=None, forces=()):
         self.positions = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
         self.forces = np.zeros_like(self.positions)
         self.box_vectors = None if box_vectors is None else np.asarray(box_vectors, dtype=np.float64).reshape(3, 3)
+        self.parameters = {}
+        for f in forces:
+            for i in range(f.getNumGlobalParameters() if hasattr(f, "getNumGlobalParameters") else 0):
+                self.parameters[f.getGlobalParameterName(i)] = f.getGlobalParameterDefaultValue(i)
+        self.energy_param_derivs = {}
 
     def getPeriodicBoxVectors(self):
         return self.box_vectors
+
+    def getParameter(self, name: str) -> float:
+        if name not in self.parameters:
+            raise OpenMMException("Called getParameter() with invalid parameter name")
+        return self.parameters[name]
+
+    def setParameter(self, name: str, value: float) -> None:
+        self.parameters[name] = float(value)
 
 
 def _set_box(engine, method: int, context: ContextImpl) -> None:
@@ -406,7 +434,8 @@ def gbmv_custom_from_force(force: CharmmGBMVForce) -> dict:
          for i in range(force.getNumGlobalParameters())}
     pv = np.array([force.getParticleParameters(i) for i in range(force.getNumParticles())], dtype=np.float64)
     return dict(params=p, param_names=names, param_values=pv.reshape(-1, len(names)), radius_param=names.index("radius"),
-                integrals=[n for n, _ in integ], values=values, terms=terms, globals_=g)
+                integrals=[n for n, _ in integ], values=values, terms=terms, globals_=g,
+                param_derivs=list(force._param_derivs))
 
 
 def gbmv_exclusions(force: CharmmGBMVForce) -> np.ndarray:
@@ -432,6 +461,9 @@ class HipCalcCharmmGBMVForceKernel:
     def initialize(self, system: System, force: CharmmGBMVForce) -> None:
         if system.getNumParticles() != force.getNumParticles():  # CharmmGBMVForceImpl.cpp:58-59
             raise OpenMMException("CharmmGBMVForce must have exactly as many particles as the System it belongs to.")
+        self._globals = {force.getGlobalParameterName(i): force.getGlobalParameterDefaultValue(i)
+                         for i in range(force.getNumGlobalParameters())}
+        self._derivs = list(force._param_derivs)
         try:
             p, q, r = gbmv_params_from_force(force)
             self._custom = False
@@ -451,22 +483,43 @@ class HipCalcCharmmGBMVForceKernel:
             for k, v in self._overrides.items():
                 setattr(p, "lambda_" if k == "lambda" else k, v)
             self._method = p.nonbonded_method
+            self._params = p
             self._engine = N.Engine(p, q, r)
             ex = gbmv_exclusions(force)
         if len(ex):
             self._engine.set_exclusions(ex)
 
+    def _pull_globals(self, context: ContextImpl) -> None:
+        """The global parameters are read from the Context at every execute
+        (CharmmReferenceKernels.cpp:355-357); the engine gets those that changed."""
+        now = {k: context.getParameter(k) for k in self._globals}
+        if now == self._globals:
+            return
+        self._globals = now
+        if self._custom:
+            self._engine.set_custom_globals([now[k] for k in self._engine.global_names])
+            return
+        for k, field in (("alpha0", "alpha0"), ("alpha1", "alpha1"), ("eta", "eta"), ("slope", "slope"),
+                         ("shift", "shift"), ("P", "prefactor")):
+            setattr(self._params, field, now[k])
+        self._engine.update_constants(self._params)
+
     def execute(self, context: ContextImpl, includeForces: bool = True, includeEnergy: bool = True) -> float:
+        self._pull_globals(context)
         _set_box(self._engine, self._method, context)
         e, _ = self._engine.execute(context.positions, includeForces, includeEnergy, forces=context.forces)
+        if self._custom and self._derivs:  # CharmmReferenceKernels.cpp:360-368
+            for k, v in self._engine.custom_param_derivs().items():
+                context.energy_param_derivs[k] = context.energy_param_derivs.get(k, 0.0) + v
         return e if includeEnergy else 0.0
 
     def copyParametersToContext(self, context: ContextImpl, force: CharmmGBMVForce) -> None:
         if self._custom:
             if force.getNumParticles() != self._engine.n:  # CharmmReferenceKernels.cpp:373-374
                 raise OpenMMException("updateParametersInContext: The number of particles has changed")
-            self._engine.close()
-            self.initialize(System(force.getNumParticles()), force)
+            # per-particle parameters only (CharmmReferenceKernels.cpp:376-385); the kernels stay
+            self._engine.update_custom_parameters(
+                [force.getParticleParameters(i) for i in range(force.getNumParticles())])
             return
         _, q, r = gbmv_params_from_force(force)
         self._engine.update_parameters(q, r)

@@ -3,13 +3,19 @@
 // then initialize / execute / copyParametersToContext.  Used by the tests.
 //
 // Input (binary, native endianness): int32 model (0 CharmmGBSWForce, 1 CharmmGBMVForce with
-// the canonical expressions, 2 CharmmGBMVForce with the general ones of generalGBMV),
+// the canonical expressions, 2 CharmmGBMVForce with the general ones of generalGBMV, 3 the
+// general ones of derivGBMV, which asks for energy parameter derivatives),
 // int32 integral type, int32 Lebedev rule (0 = the reference's), int32 N, then N x 3
 // positions, N charges, N radii (doubles), optionally followed by int32 nonbonded method,
 // double cutoff and 3 doubles of a rectangular periodic box, then optionally int32 n and n
 // excluded pairs (2 x int32) that make the GBMV pair term ParticlePair.  Output (text): energy,
 // forces, the energy after copyParametersToContext with charges x 1.5, and the outcome of
-// the error paths.
+// the error paths; model 1 adds the energy after Context::setParameter("alpha0", -0.2),
+// model 3 the energy parameter derivatives, and the energy and dE/dk after
+// Context::setParameter("k", 0.45).
+//
+// `gbi_openmm_driver --canonical-check` (no device needed) prints which CharmmGBMVForce
+// variants the kernel class takes as the canonical forms.
 #include <cstdio>
 #include <cstdint>
 #include <memory>
@@ -74,9 +80,56 @@ CharmmGBMVForce* generalGBMV(const std::vector<double>& q, const std::vector<dou
     return f;
 }
 
+// generalGBMV with a global k inside the Born value and energy derivatives with respect to k
+// and P (CharmmGBMVForce::addEnergyParameterDerivative).
+CharmmGBMVForce* derivGBMV(const std::vector<double>& q, const std::vector<double>& r, int type) {
+    auto* f = new CharmmGBMVForce();
+    f->addPerParticleParameter("q");
+    f->addPerParticleParameter("radius");
+    f->addGlobalParameter("P", -138.935456 * (1.0 - 1.0 / 80.0));
+    f->addGlobalParameter("k", 0.5);
+    f->addGBIntegral("I4", std::vector<int>{4});
+    f->setGBIntegralType((CharmmGBMVForce::GBIntegralType)type);
+    f->addComputedValue("B", "1/(1/radius-k*I4*radius)", CharmmGBMVForce::SingleParticle);
+    f->addEnergyTerm("0.5*P*q^2/B", CharmmGBMVForce::SingleParticle);
+    f->addEnergyTerm(gbi_openmm::kCanonicalPairEnergy, CharmmGBMVForce::ParticlePairNoExclusions);
+    f->addEnergyParameterDerivative("k");
+    f->addEnergyParameterDerivative("P");
+    for (size_t i = 0; i < q.size(); ++i) f->addParticle({q[i], r[i]});
+    return f;
+}
+
+// The Context holds every global parameter of its forces, starting at their defaults.
+void setDefaults(ContextImpl& context, const CharmmGBMVForce& force) {
+    for (int k = 0; k < force.getNumGlobalParameters(); ++k)
+        context.setParameter(force.getGlobalParameterName(k), force.getGlobalParameterDefaultValue(k));
+}
+
+// Which forces the kernel class takes as canonical (host only).
+int canonicalCheck() {
+    const std::vector<double> q = {0.5, -0.5}, r = {0.2, 0.17};
+    std::unique_ptr<CharmmGBMVForce> canon(canonicalGBMV(q, r, 2, 0, {}));
+    // the integral named I4 of order 7 and the one named I7 of order 4: the Born expression
+    // names them, so this force is not the canonical one
+    std::unique_ptr<CharmmGBMVForce> swapped(canonicalGBMV(q, r, 2, 0, {}));
+    swapped->setGBIntegralParameters(0, "I4", std::vector<int>{7}, std::vector<double>());
+    swapped->setGBIntegralParameters(1, "I7", std::vector<int>{4}, std::vector<double>());
+    // the same integrals declared in the other order are still canonical
+    std::unique_ptr<CharmmGBMVForce> reordered(canonicalGBMV(q, r, 2, 0, {}));
+    reordered->setGBIntegralParameters(0, "I7", std::vector<int>{7}, std::vector<double>());
+    reordered->setGBIntegralParameters(1, "I4", std::vector<int>{4}, std::vector<double>());
+    std::unique_ptr<CharmmGBMVForce> deriv(canonicalGBMV(q, r, 2, 0, {}));
+    deriv->addEnergyParameterDerivative("alpha0");
+    printf("canonical %d\nswapped_names %d\nreordered %d\nwith_param_derivative %d\n",
+           (int)gbi_openmm::isCanonicalGBMV(*canon), (int)gbi_openmm::isCanonicalGBMV(*swapped),
+           (int)gbi_openmm::isCanonicalGBMV(*reordered), (int)gbi_openmm::isCanonicalGBMV(*deriv));
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
+    if (argc == 2 && std::string(argv[1]) == "--canonical-check") return canonicalCheck();
     if (argc != 3) {
         fprintf(stderr, "usage: %s input.bin output.txt\n", argv[0]);
         return 2;
@@ -147,10 +200,14 @@ int main(int argc, char** argv) {
             }
             fprintf(out, "count_change_throws %d\n", count_throws);
         } else {
-            std::unique_ptr<CharmmGBMVForce> force(model == 2 ? generalGBMV(q, r, type)
-                                                              : canonicalGBMV(q, r, type, rule, excl));
+            auto make = [&](const std::vector<double>& charges) {
+                return model == 3 ? derivGBMV(charges, r, type)
+                                  : (model == 2 ? generalGBMV(charges, r, type) : canonicalGBMV(charges, r, type, rule, excl));
+            };
+            std::unique_ptr<CharmmGBMVForce> force(make(q));
             force->setNonbondedMethod((CharmmGBMVForce::NonbondedMethod)method);
             force->setCutoffDistance(cutoff);
+            setDefaults(context, *force);
             std::unique_ptr<KernelImpl> k(platform.createKernel(CalcCharmmGBMVForceKernel::Name(), context));
             auto& kernel = dynamic_cast<CalcCharmmGBMVForceKernel&>(*k);
             kernel.initialize(system, *force);
@@ -158,9 +215,24 @@ int main(int argc, char** argv) {
             std::vector<Vec3> f = context.getForces();
             fprintf(out, "energy %.17g\n", e);
             for (int i = 0; i < n; ++i) fprintf(out, "%.17g %.17g %.17g\n", f[i][0], f[i][1], f[i][2]);
+            if (model == 3) {
+                // one execute adds dE/dk and dE/dP once; the second (with k = 0.45) adds again
+                auto& d = context.energyParameterDerivatives();
+                fprintf(out, "dEdk %.17g\ndEdP %.17g\n", d["k"], d["P"]);
+                context.setParameter("k", 0.45);
+                d.clear();
+                const double e2 = kernel.execute(context, false, true);
+                fprintf(out, "energy_k045 %.17g\ndEdk_k045 %.17g\n", e2, d["k"]);
+                context.setParameter("k", 0.5);
+            }
+            if (model == 1) {
+                // a global parameter changed in the Context reaches the next execute
+                context.setParameter("alpha0", -0.2);
+                fprintf(out, "energy_alpha0 %.17g\n", kernel.execute(context, false, true));
+                context.setParameter("alpha0", -0.180);
+            }
             e_noforce = kernel.execute(context, false, true);
-            std::unique_ptr<CharmmGBMVForce> scaled(model == 2 ? generalGBMV(qs, r, type)
-                                                               : canonicalGBMV(qs, r, type, rule, excl));
+            std::unique_ptr<CharmmGBMVForce> scaled(make(qs));
             scaled->setNonbondedMethod((CharmmGBMVForce::NonbondedMethod)method);
             scaled->setCutoffDistance(cutoff);
             kernel.copyParametersToContext(context, *scaled);

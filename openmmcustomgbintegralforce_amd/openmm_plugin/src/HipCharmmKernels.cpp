@@ -23,6 +23,19 @@ std::string strip(const std::string& s) {
     return out;
 }
 
+// The force's per-particle parameters, N x n_params row-major.
+std::vector<double> particleParameterArray(const CharmmGBMVForce& force) {
+    const int n = force.getNumParticles(), np = force.getNumPerParticleParameters();
+    std::vector<double> out((size_t)n * np);
+    for (int i = 0; i < n; ++i) {
+        std::vector<double> v;
+        force.getParticleParameters(i, v);
+        if ((int)v.size() < np) throw OpenMMException("CharmmGBMVForce: a particle lacks parameters");
+        for (int k = 0; k < np; ++k) out[(size_t)i * np + k] = v[k];
+    }
+    return out;
+}
+
 int lebedevPoints(int rule) {
     static const int pts[] = {0, 6, 14, 26, 38, 50};
     if (rule < 1 || rule > 5) throw OpenMMException("CharmmGBMVForce: Lebedev rule must be 1..5 on the HIP platform");
@@ -69,6 +82,19 @@ double HipGBKernelBase::run(ContextImpl& context, bool includeForces, bool inclu
                       includeForces ? f.data() : nullptr));
     if (includeForces) addForces(context, f);
     return includeEnergy ? energy : 0.0;
+}
+
+void HipGBKernelBase::addEnergyParameterDerivatives(ContextImpl& context, const std::vector<std::string>& names,
+                                                    const std::vector<double>& values) {
+#ifdef GBI_OPENMM_STAND_IN
+    auto& acc = context.energyParameterDerivatives();
+    for (size_t k = 0; k < names.size(); ++k) acc[names[k]] += values[k];
+#else
+    (void)context;
+    (void)names;
+    (void)values;
+    throw OpenMMException("HIP GB kernel: the platform must override addEnergyParameterDerivatives()");
+#endif
 }
 
 void HipGBKernelBase::addForces(ContextImpl& context, const std::vector<double>& forces) {
@@ -138,21 +164,8 @@ void gbmvParamsFromForce(const CharmmGBMVForce& force, gbi_params& p, std::vecto
         if (ints[k].empty()) throw OpenMMException("GBSWIntegral: the order of integral must be given");
         if (ints[k][0] < 2) throw OpenMMException("GBSWIntegral: the order of integral must be greater or equal to 2");
     }
-    if (ints.size() != 2 || !((ints[0][0] == 4 && ints[1][0] == 7) || (ints[0][0] == 7 && ints[1][0] == 4)))
-        throw OpenMMException("CharmmGBMVForce: the HIP platform implements the canonical Born map over integrals of order 4 and 7");
-    bool canonical = force.getNumComputedValues() == 1 && force.getNumEnergyTerms() == 2;
-    if (canonical) {
-        std::string name, expr;
-        CharmmGBMVForce::ComputationType t;
-        force.getComputedValueParameters(0, name, expr, t);
-        canonical = name == "B" && strip(expr) == kCanonicalBornValue && t == CharmmGBMVForce::SingleParticle;
-        force.getEnergyTermParameters(0, expr, t);
-        canonical = canonical && strip(expr) == kCanonicalSelfEnergy && t == CharmmGBMVForce::SingleParticle;
-        force.getEnergyTermParameters(1, expr, t);
-        canonical = canonical && strip(expr) == kCanonicalPairEnergy &&
-                    (t == CharmmGBMVForce::ParticlePairNoExclusions || t == CharmmGBMVForce::ParticlePair);
-    }
-    if (!canonical) throw OpenMMException("CharmmGBMVForce: only the canonical GB expressions run on the HIP platform");
+    if (!isCanonicalGBMV(force))
+        throw OpenMMException("CharmmGBMVForce: only the canonical GB expressions run on the hand-written kernels");
     if (gbi_params_default(GBI_MODEL_GBMV, (int32_t)type, &p) != GBI_OK) throw OpenMMException(gbi_last_error());
     if (ints[0].size() > 1) p.n_lebedev = lebedevPoints(ints[0][1]);
     if (ints[0].size() > 2) p.n_radial = ints[0][2];
@@ -190,13 +203,14 @@ bool isCanonicalGBMV(const CharmmGBMVForce& force) {
     for (int k = 0; k < force.getNumPerParticleParameters(); ++k) q = q || force.getPerParticleParameterName(k) == "q";
     if (!q || force.getNumGBIntegrals() != 2 || force.getNumComputedValues() != 1 || force.getNumEnergyTerms() != 2)
         return false;
-    std::vector<int> o[2];
+    if (force.getNumEnergyParameterDerivatives() > 0) return false;  // generated kernels compute them
+    // the Born expression refers to the integrals by name
     for (int k = 0; k < 2; ++k) {
         std::string name;
-        force.getGBIntegralParameters(k, name, o[k]);
-        if (o[k].empty()) return false;
+        std::vector<int> o;
+        force.getGBIntegralParameters(k, name, o);
+        if (o.empty() || !((name == "I4" && o[0] == 4) || (name == "I7" && o[0] == 7))) return false;
     }
-    if (!((o[0][0] == 4 && o[1][0] == 7) || (o[0][0] == 7 && o[1][0] == 4))) return false;
     std::string name, expr;
     CharmmGBMVForce::ComputationType t;
     force.getComputedValueParameters(0, name, expr, t);
@@ -263,13 +277,7 @@ void HipCalcCharmmGBMVForceKernel::createCustom(const CharmmGBMVForce& force) {
     }
     const int n = force.getNumParticles();
     const int np = (int)pnames.size();
-    pvals.resize((size_t)n * np);
-    for (int i = 0; i < n; ++i) {
-        std::vector<double> v;
-        force.getParticleParameters(i, v);
-        if ((int)v.size() < np) throw OpenMMException("CharmmGBMVForce: a particle lacks parameters");
-        for (int k = 0; k < np; ++k) pvals[(size_t)i * np + k] = v[k];
-    }
+    pvals = particleParameterArray(force);
     p.nonbonded_method = (int32_t)force.getNonbondedMethod();
     p.cutoff = force.getCutoffDistance();
     auto cstr = [](const std::vector<std::string>& v) {
@@ -278,7 +286,7 @@ void HipCalcCharmmGBMVForceKernel::createCustom(const CharmmGBMVForce& force) {
         return out;
     };
     const auto cp = cstr(pnames), cg = cstr(gnames), ci = cstr(inames), cvn = cstr(vnames), cve = cstr(vexprs),
-               ct = cstr(texprs);
+               ct = cstr(texprs), cd = cstr(derivs_);
     gbi_custom_spec spec{};
     spec.n_params = np;
     spec.param_names = cp.data();
@@ -295,6 +303,8 @@ void HipCalcCharmmGBMVForceKernel::createCustom(const CharmmGBMVForce& force) {
     spec.n_terms = (int32_t)texprs.size();
     spec.term_exprs = ct.data();
     spec.term_types = ttypes.data();
+    spec.n_param_derivs = (int32_t)derivs_.size();
+    spec.param_deriv_names = cd.data();
     gbi_destroy(ctx_);
     ctx_ = nullptr;
     n_ = n;
@@ -316,6 +326,15 @@ void HipCalcCharmmGBMVForceKernel::createCustom(const CharmmGBMVForce& force) {
 void HipCalcCharmmGBMVForceKernel::initialize(const System& system, const CharmmGBMVForce& force) {
     if (system.getNumParticles() != force.getNumParticles())  // CharmmGBMVForceImpl.cpp:58-59
         throw OpenMMException("CharmmGBMVForce must have exactly as many particles as the System it belongs to.");
+    globals_.clear();
+    global_values_.clear();
+    for (int k = 0; k < force.getNumGlobalParameters(); ++k) {
+        globals_.push_back(force.getGlobalParameterName(k));
+        global_values_.push_back(force.getGlobalParameterDefaultValue(k));
+    }
+    derivs_.clear();
+    for (int k = 0; k < force.getNumEnergyParameterDerivatives(); ++k)
+        derivs_.push_back(force.getEnergyParameterDerivativeName(k));
     if (force.getGBIntegralType() != CharmmGBMVForce::CustomIntegral && !isCanonicalGBMV(force)) {
         createCustom(force);  // general expressions: kernels generated from the force's expressions
         return;
@@ -324,6 +343,7 @@ void HipCalcCharmmGBMVForceKernel::initialize(const System& system, const Charmm
     gbi_params p;
     std::vector<double> q, r;
     gbmvParamsFromForce(force, p, q, r);
+    params_ = p;
     create(force.getNumParticles(), q, r, p);
     // a ParticlePair term skips the force's exclusions (CharmmReferenceGBMV.cpp:272-284)
     std::string expr;
@@ -341,15 +361,54 @@ void HipCalcCharmmGBMVForceKernel::initialize(const System& system, const Charmm
     }
 }
 
+// The reference reads every global parameter from the Context at each execute
+// (CharmmReferenceKernels.cpp:355-357); the engine gets the values that changed.
+void HipCalcCharmmGBMVForceKernel::pullGlobals(ContextImpl& context) {
+    bool changed = false;
+    for (size_t k = 0; k < globals_.size(); ++k) {
+        const double v = context.getParameter(globals_[k]);
+        if (v != global_values_[k]) {
+            global_values_[k] = v;
+            changed = true;
+        }
+    }
+    if (!changed) return;
+    if (custom_) {
+        check(gbi_set_custom_globals(ctx_, (int32_t)globals_.size(), global_values_.data()));
+        return;
+    }
+    for (size_t k = 0; k < globals_.size(); ++k) {
+        const std::string& g = globals_[k];
+        const double v = global_values_[k];
+        if (g == "alpha0") params_.alpha0 = v;
+        else if (g == "alpha1") params_.alpha1 = v;
+        else if (g == "eta") params_.eta = v;
+        else if (g == "slope") params_.slope = v;
+        else if (g == "shift") params_.shift = v;
+        else if (g == "P") params_.prefactor = v;
+    }
+    check(gbi_update_constants(ctx_, 0, &params_));
+}
+
 double HipCalcCharmmGBMVForceKernel::execute(ContextImpl& context, bool includeForces, bool includeEnergy) {
-    return run(context, includeForces, includeEnergy);
+    pullGlobals(context);
+    const double e = run(context, includeForces, includeEnergy);
+    if (custom_ && !derivs_.empty()) {  // CharmmReferenceKernels.cpp:360-368
+        std::vector<double> d(derivs_.size());
+        check(gbi_get_custom_param_derivs(ctx_, (int32_t)d.size(), d.data()));
+        addEnergyParameterDerivatives(context, derivs_, d);
+    }
+    return e;
 }
 
 void HipCalcCharmmGBMVForceKernel::copyParametersToContext(ContextImpl& context, const CharmmGBMVForce& force) {
     if (custom_) {
         if (force.getNumParticles() != n_)  // CharmmReferenceKernels.cpp:373-374
             throw OpenMMException("updateParametersInContext: The number of particles has changed");
-        createCustom(force);
+        // new per-particle parameters only (CharmmReferenceKernels.cpp:376-385): the generated
+        // kernels stay
+        const std::vector<double> pvals = particleParameterArray(force);
+        check(gbi_update_custom_parameters(ctx_, n_, pvals.data()));
         context.systemChanged();
         return;
     }

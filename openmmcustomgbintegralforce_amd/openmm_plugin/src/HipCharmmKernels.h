@@ -27,6 +27,11 @@ protected:
     // stand-in ContextImpl keeps a host array; a HIP OpenMM platform overrides this and adds
     // gbi_get_device_view()'s int64 fixed-point forces to its own buffer on the device.
     virtual void addForces(OpenMM::ContextImpl& context, const std::vector<double>& forces);
+    // Adds dE/d(global parameter) of the last evaluation to the Context's energy-parameter
+    // derivatives (CharmmReferenceKernels.cpp:366-368).  The stand-in ContextImpl keeps a host
+    // map; a HIP OpenMM platform overrides this with its own accumulator.
+    virtual void addEnergyParameterDerivatives(OpenMM::ContextImpl& context, const std::vector<std::string>& names,
+                                               const std::vector<double>& values);
     static void check(gbi_status status);
     void create(int n, const std::vector<double>& charge, const std::vector<double>& radius, const gbi_params& params);
     void update(const std::vector<double>& charge, const std::vector<double>& radius);
@@ -65,10 +70,18 @@ public:
 
 private:
     void createCustom(const OpenMM::CharmmGBMVForce& force);
+    // the Context's global parameters, pushed to the engine when they change
+    void pullGlobals(OpenMM::ContextImpl& context);
     bool custom_ = false;
+    gbi_params params_{};                   // canonical: the engine's constants
+    std::vector<std::string> globals_;      // global parameter names of the force
+    std::vector<double> global_values_;     // their values in the engine
+    std::vector<std::string> derivs_;       // energy parameter derivatives asked for
 };
 
-// Whether a CharmmGBMVForce holds exactly the canonical expressions.
+// Whether a CharmmGBMVForce holds exactly the canonical expressions (by expression text and
+// integral names: the Born expression names I4 and I7, so the integral named I4 must be of
+// order 4 and the one named I7 of order 7) and asks for no energy parameter derivatives.
 bool isCanonicalGBMV(const OpenMM::CharmmGBMVForce& force);
 
 // The canonical CharmmGBMVForce expressions recognised by HipCalcCharmmGBMVForceKernel.

@@ -1,10 +1,14 @@
 // OpenMM-shaped stand-in (used only when OpenMM is not installed): the positions of a
-// Context and the host force accumulator a kernel adds into (the role of
-// ReferencePlatform::PlatformData's force array).
+// Context, its global parameters, the host force accumulator a kernel adds into (the role of
+// ReferencePlatform::PlatformData's force array) and the energy-parameter derivatives a kernel
+// adds into (the map the Reference platform reaches through extractEnergyParameterDerivatives).
 #pragma once
 
+#include <map>
+#include <string>
 #include <vector>
 
+#include "openmm/OpenMMException.h"
 #include "openmm/Vec3.h"
 
 namespace OpenMM {
@@ -24,11 +28,21 @@ public:
         box_[1] = b;
         box_[2] = c;
     }
+    // Global parameters: a Context holds one value per global parameter of its forces,
+    // starting at the forces' defaults (ContextImpl's constructor in OpenMM).
+    double getParameter(const std::string& name) const {
+        auto it = parameters_.find(name);
+        if (it == parameters_.end()) throw OpenMMException("Called getParameter() with invalid parameter name");
+        return it->second;
+    }
+    void setParameter(const std::string& name, double value) { parameters_[name] = value; }
+    std::map<std::string, double>& energyParameterDerivatives() { return param_derivs_; }
     std::vector<Vec3>& getForces() { return forces_; }
     void systemChanged() {}
 
 private:
     std::vector<Vec3> positions_, forces_;
+    std::map<std::string, double> parameters_, param_derivs_;
     Vec3 box_[3] = {Vec3(2.0, 0.0, 0.0), Vec3(0.0, 2.0, 0.0), Vec3(0.0, 0.0, 2.0)};
 };
 

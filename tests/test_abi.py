@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     for s in syms:
         assert hasattr(lib, s), s
     assert sorted(N.EXPORTS) == syms
-    assert lib.gbi_abi_version() == 3
+    assert lib.gbi_abi_version() == 4
 
 
 def test_defaults_are_reference_constants():
@@ -144,6 +144,28 @@ def test_force_api_rejects_what_the_hip_platform_does_not_implement():
     g.addGBIntegral("I4", [1])
     with pytest.raises(F.OpenMMException):
         F.gbmv_params_from_force(g)
+
+
+def test_canonical_force_needs_integrals_named_by_their_order():
+    """The canonical Born expression names its integrals: swapping the names of the order-4
+    and order-7 integrals, or asking for energy parameter derivatives, sends the force to the
+    generated kernels (GbiError status 2 from the canonical mapping)."""
+    f = F.canonical_gbmv_force([0.1, -0.1], [0.2, 0.17], F.CharmmGBMVForce.GBMVIntegralTypeII)
+    F.gbmv_params_from_force(f)
+    f.setGBIntegralParameters(0, "I4", [7])
+    f.setGBIntegralParameters(1, "I7", [4])
+    with pytest.raises(N.GbiError) as err:
+        F.gbmv_params_from_force(f)
+    assert err.value.status == 2
+    g = F.canonical_gbmv_force([0.1, -0.1], [0.2, 0.17], F.CharmmGBMVForce.GBMVIntegralTypeII)
+    g.setGBIntegralParameters(0, "I7", [7])
+    g.setGBIntegralParameters(1, "I4", [4])
+    p, _, _ = F.gbmv_params_from_force(g)
+    assert (p.orders[0], p.orders[1]) == (7, 4)
+    g.addEnergyParameterDerivative("alpha0")
+    with pytest.raises(N.GbiError) as err:
+        F.gbmv_params_from_force(g)
+    assert err.value.status == 2
 
 
 def test_gbsw_force_api():

@@ -49,6 +49,13 @@ def test_cpp_kernel_classes_match_oracle(tmp_path, case):
     assert abs(float(rest["energy_scaled"]) - 2.25 * e) < 1e-6 * abs(e)
     assert rest["illegal_name_throws"] == "1"
     assert rest["count_change_throws"] == "1"
+    if case == "gbmv2":
+        # Context::setParameter("alpha0", -0.2) reaches the next execute (CharmmReferenceKernels.cpp:355-357)
+        pa = N.Params.from_buffer_copy(p)
+        pa.alpha0 = -0.2
+        ref_a = O.compute(oracle_params(pa), s.pos, s.charge, s.radius)
+        print(f"[openmm-plugin] alpha0 -0.2: energy {float(rest['energy_alpha0']):.6f} vs {ref_a.energy:.6f}", flush=True)
+        assert rel_err(float(rest["energy_alpha0"]), ref_a.energy) < 1e-5
 
 
 @pytest.mark.parametrize("case", ["gbsw", "gbmv2"])
@@ -132,7 +139,7 @@ def test_python_kernel_mirror_general_expressions():
         f.addExclusion(i, i + 1)
     kern = F.HipCalcCharmmGBMVForceKernel()
     kern.initialize(F.System(s.n), f)
-    ctx = F.ContextImpl(s.pos)
+    ctx = F.ContextImpl(s.pos, forces=[f])
     e = kern.execute(ctx)
     p = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV1)
     E, Fo, _, _ = C.compute(oracle_params(p), s.pos, ["q", "radius"], np.stack([s.charge, s.radius], 1), 1,
@@ -142,3 +149,100 @@ def test_python_kernel_mirror_general_expressions():
                             {"P": -138.935456 * (1.0 - 1.0 / 80.0), "k": 0.2}, exclusions=f._exclusions)
     print(f"[openmm-plugin] python mirror general expressions: energy {e:.6f} vs {E:.6f}", flush=True)
     assert rel_err(e, E) < 1e-5 and rel_rms(ctx.forces, Fo) < 1e-5
+
+
+DERIV_VALUES = [("B", "1/(1/radius-k*I4*radius)")]
+DERIV_TERMS = [("0.5*P*q^2/B", 0), ("P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))", 2)]
+P_DEFAULT = -138.935456 * (1.0 - 1.0 / 80.0)
+
+
+def _deriv_oracle(s, k, P):
+    from oracle import custom as C
+
+    p = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV2)
+    p.orders[0] = 4
+    E, _, _, _ = C.compute(oracle_params(p), s.pos, ["q", "radius"], np.stack([s.charge, s.radius], 1), 1, ["I4"],
+                           DERIV_VALUES, DERIV_TERMS, {"P": P, "k": k})
+    return E
+
+
+def _fd(s, k, P, which, h):
+    if which == "k":
+        return (_deriv_oracle(s, k + h, P) - _deriv_oracle(s, k - h, P)) / (2 * h)
+    return (_deriv_oracle(s, k, P + h) - _deriv_oracle(s, k, P - h)) / (2 * h)
+
+
+def test_cpp_kernel_classes_energy_parameter_derivatives(tmp_path):
+    """CharmmGBMVForce::addEnergyParameterDerivative through the C++ kernel class (driver
+    model 3: B = 1/(1/radius - k I4 radius), STILL energy, derivatives with respect to k and
+    P): each execute adds dE/dk and dE/dP to the Context (CharmmReferenceKernels.cpp:360-368),
+    checked against central differences of the oracle energy; Context::setParameter("k")
+    reaches the next execute."""
+    s = S.globule(300, 46)
+    e, f, rest = _run(tmp_path, 3, 2, 0, s)
+    E = _deriv_oracle(s, 0.5, P_DEFAULT)
+    fd_k, fd_P = _fd(s, 0.5, P_DEFAULT, "k", 1e-4), _fd(s, 0.5, P_DEFAULT, "P", 1e-3)
+    dk, dP = float(rest["dEdk"]), float(rest["dEdP"])
+    print(f"[openmm-plugin] dE/dk {dk:.6f} (FD {fd_k:.6f}), dE/dP {dP:.6f} (FD {fd_P:.6f})", flush=True)
+    assert rel_err(e, E) < 1e-5
+    assert rel_err(dk, fd_k) < 1e-4 and rel_err(dP, fd_P) < 1e-5
+    # E is linear in P: dE/dP = E/P exactly
+    assert rel_err(dP, e / P_DEFAULT) < 1e-6
+    e2, dk2 = float(rest["energy_k045"]), float(rest["dEdk_k045"])
+    assert rel_err(e2, _deriv_oracle(s, 0.45, P_DEFAULT)) < 1e-5
+    assert rel_err(dk2, _fd(s, 0.45, P_DEFAULT, "k", 1e-4)) < 1e-4
+
+
+def test_python_kernel_mirror_context_parameters_and_derivatives():
+    """forces.HipCalcCharmmGBMVForceKernel reads the global parameters from the Context at
+    every execute (canonical force: Born-map constants through gbi_update_constants; general
+    expressions: gbi_set_custom_globals) and adds energy parameter derivatives to it;
+    copyParametersToContext of a general force keeps the compiled kernels."""
+    from openmmcustomgbintegralforce_amd import forces as F
+
+    s = S.globule(300, 47)
+    # canonical: alpha0 and P changed in the Context
+    f = F.canonical_gbmv_force(s.charge, s.radius, F.CharmmGBMVForce.GBMVIntegralTypeII, lebedev_rule=5)
+    kern = F.HipCalcCharmmGBMVForceKernel()
+    kern.initialize(F.System(s.n), f)
+    assert not kern._custom
+    ctx = F.ContextImpl(s.pos, forces=[f])
+    ctx.setParameter("alpha0", -0.19)
+    ctx.setParameter("P", 0.9 * P_DEFAULT)
+    e = kern.execute(ctx)
+    p = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV2)
+    p.n_lebedev = 50
+    p.alpha0, p.prefactor = -0.19, 0.9 * P_DEFAULT
+    ref = O.compute(oracle_params(p), s.pos, s.charge, s.radius)
+    print(f"[openmm-plugin] python canonical with Context alpha0/P: {e:.6f} vs {ref.energy:.6f}", flush=True)
+    assert rel_err(e, ref.energy) < 1e-5 and rel_rms(ctx.forces, ref.forces) < 1e-5
+    # general expressions with derivatives and a Context parameter
+    g = F.CharmmGBMVForce()
+    g.addPerParticleParameter("q")
+    g.addPerParticleParameter("radius")
+    g.addGlobalParameter("P", P_DEFAULT)
+    g.addGlobalParameter("k", 0.5)
+    g.addGBIntegral("I4", [4])
+    g.setGBIntegralType(F.CharmmGBMVForce.GBMVIntegralTypeII)
+    g.addComputedValue("B", DERIV_VALUES[0][1], F.CharmmGBMVForce.SingleParticle)
+    for expr, ty in DERIV_TERMS:
+        g.addEnergyTerm(expr, ty)
+    g.addEnergyParameterDerivative("k")
+    for q, r in zip(s.charge, s.radius):
+        g.addParticle([q, r])
+    kg = F.HipCalcCharmmGBMVForceKernel()
+    kg.initialize(F.System(s.n), g)
+    assert kg._custom
+    cg = F.ContextImpl(s.pos, forces=[g])
+    cg.setParameter("k", 0.45)
+    e = kg.execute(cg)
+    fd = _fd(s, 0.45, P_DEFAULT, "k", 1e-4)
+    print(f"[openmm-plugin] python general k=0.45: dE/dk {cg.energy_param_derivs['k']:.6f} (FD {fd:.6f})", flush=True)
+    assert rel_err(e, _deriv_oracle(s, 0.45, P_DEFAULT)) < 1e-5
+    assert rel_err(cg.energy_param_derivs["k"], fd) < 1e-4
+    # new charges through copyParametersToContext: same kernels, E scales by 1.5^2
+    for i in range(s.n):
+        g.setParticleParameters(i, [1.5 * s.charge[i], s.radius[i]])
+    kg.copyParametersToContext(cg, g)
+    e15 = kg.execute(cg, False, True)
+    assert abs(e15 - 2.25 * e) < 1e-6 * abs(e)

@@ -43,3 +43,14 @@ def test_driver_reports_missing_device(tmp_path):
     r = subprocess.run([DRIVER, str(inp), str(out)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert "no HIP device" in r.stderr
+
+
+def test_canonical_detection_by_integral_names():
+    """The canonical Born expression names its integrals I4 and I7: a force whose integral
+    named I4 has order 7 (and I7 order 4) is not the canonical one and goes to the generated
+    kernels, as does one asking for energy parameter derivatives; declaring I7 before I4 is
+    still canonical."""
+    r = subprocess.run([DRIVER, "--canonical-check"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    got = dict(ln.split() for ln in r.stdout.splitlines())
+    assert got == {"canonical": "1", "swapped_names": "0", "reordered": "1", "with_param_derivative": "0"}
