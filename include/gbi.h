@@ -320,6 +320,23 @@ gbi_status gbi_execute_adjoint(gbi_ctx* ctx, const double* pos, double* energy, 
                                double* dEdq, double* dEdtheta);
 
 /*
+ * One evaluation of one NoCutoff system that allocates nothing on the device: gbi_create +
+ * gbi_execute + gbi_destroy in one synchronous call, with host inputs (charge, radius: N
+ * doubles; pos: N x 3) and outputs (energy; forces ADDED, N x 3).  Every device buffer of the
+ * evaluation is carved from a static array of 256 MiB per GPU that the library holds from its
+ * load until the process ends; the call makes no hipMalloc, hipFree or other allocating call.
+ * Systems whose buffers need more (about 2,500 atoms at 24 x 38, 1,400 at 24 x 50) fail with
+ * GBI_ERR_INVALID_ARGUMENT naming the size; other nonbonded methods with GBI_ERR_UNSUPPORTED.
+ * The array keeps the last call's contents; calls in one process are serialised (a call that
+ * overlapped another on the same GPU would race on the array).  hip_stream: NULL for the null
+ * stream.  No counterpart in the reference: for callers that cannot allocate device memory
+ * (serving loops and harnesses that forbid allocation).
+ */
+gbi_status gbi_execute_noalloc(int32_t n_atoms, const double* charge, const double* radius, const gbi_params* params,
+                               const double* pos, int32_t include_forces, int32_t include_energy, double* energy,
+                               double* forces, void* hip_stream);
+
+/*
  * Replay evaluations from a captured hipGraph (one graph launch per evaluation instead of
  * ~25 kernel launches).  The graph is re-captured when buffers grow.  Not used while
  * profiling is enabled.

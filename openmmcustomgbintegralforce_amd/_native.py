@@ -37,7 +37,7 @@ EXPORTS = [
     "gbi_set_tuning", "gbi_get_tuning", "gbi_destroy", "gbi_last_error", "gbi_abi_version", "gbi_quadrature",
     "gbi_profile_enable", "gbi_profile_read", "gbi_profile_pass_name", "gbi_set_periodic_box",
     "gbi_set_exclusions", "gbi_create_custom", "gbi_set_custom_globals", "gbi_get_custom_values",
-    "gbi_get_custom_param_derivs", "gbi_update_custom_parameters", "gbi_update_constants",
+    "gbi_get_custom_param_derivs", "gbi_update_custom_parameters", "gbi_update_constants", "gbi_execute_noalloc",
 ]
 PROFILE_PASSES = ["grid", "bin", "volume", "integrals", "pairs", "chain", "force", "owner"]
 
@@ -177,6 +177,8 @@ def load() -> ctypes.CDLL:
     L.gbi_get_custom_param_derivs.argtypes = [vp, ctypes.c_int32, dp]
     L.gbi_update_custom_parameters.argtypes = [vp, ctypes.c_int32, dp]
     L.gbi_update_constants.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(Params)]
+    L.gbi_execute_noalloc.argtypes = [ctypes.c_int32, dp, dp, ctypes.POINTER(Params), dp, ctypes.c_int32,
+                                      ctypes.c_int32, dp, dp, vp]
     L.gbi_set_positions.argtypes = [vp, vp, ctypes.c_int32]
     L.gbi_compute.argtypes = [vp, ctypes.c_uint32]
     L.gbi_synchronize.argtypes = [vp]

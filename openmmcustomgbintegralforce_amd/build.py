@@ -17,7 +17,7 @@ LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libgbi.so")
 ARCH = os.environ.get("GBI_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["gbi_kernels.hip", "gbi_context.cpp", "gbi_custom.cpp", "quadrature.cpp"]
+SOURCES = ["gbi_kernels.hip", "gbi_noalloc.hip", "gbi_context.cpp", "gbi_custom.cpp", "quadrature.cpp"]
 
 # Packed-FP32 instructions stay disabled: the compiler must not form v_pk_*_f32.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -294,6 +295,14 @@ void atom_params(const gbi_ctx& c, std::vector<float4>& ap, std::vector<double>&
     }
 }
 
+// Largest reach of each system's atoms (the tight grid's padding).
+void set_max_reach(gbi_ctx& c, const std::vector<double>& reach) {
+    for (SysHost& s : c.sys) {
+        s.max_reach = 0.0;
+        for (int j = s.atom_begin; j < s.atom_begin + s.N; ++j) s.max_reach = std::max(s.max_reach, reach[j]);
+    }
+}
+
 gbi_status upload_atoms(gbi_ctx& c) {
     std::vector<float4> ap;
     std::vector<double> reach;
@@ -302,10 +311,7 @@ gbi_status upload_atoms(gbi_ctx& c) {
     HIP_TRY(hipMemcpyAsync(c.d_reach.p, reach.data(), sizeof(double) * c.N, hipMemcpyHostToDevice, c.stream));
     HIP_TRY(hipMemcpyAsync(c.d_charge.p, c.charge.data(), sizeof(double) * c.N, hipMemcpyHostToDevice, c.stream));
     HIP_TRY(hipMemcpyAsync(c.d_radius.p, c.radius.data(), sizeof(double) * c.N, hipMemcpyHostToDevice, c.stream));
-    for (SysHost& s : c.sys) {
-        s.max_reach = 0.0;
-        for (int j = s.atom_begin; j < s.atom_begin + s.N; ++j) s.max_reach = std::max(s.max_reach, reach[j]);
-    }
+    set_max_reach(c, reach);
     HIP_TRY(hipStreamSynchronize(c.stream));
     return GBI_OK;
 }
@@ -440,15 +446,15 @@ gbi_status layout(gbi_ctx& c) {
 // unordered pair once).  K column blocks per tile: `pair_columns` / 64 when set, else the
 // smallest K that keeps about 8192 tiles over the batch (every system's share of the block
 // pairs).  Column partials: one entry per (row block, atom) of each system.
-gbi_status build_tiles(gbi_ctx& c) {
+void tile_layout(gbi_ctx& c, std::vector<int4>& tiles, std::vector<int>& colblk) {
     long long pairs_total = 0;
     for (const SysHost& s : c.sys) {
         const long long nb = (s.N + 63) / 64;
         pairs_total += nb * (nb + 1) / 2;
     }
     const long long target = 8192;
-    std::vector<int4> tiles;
-    std::vector<int> colblk;
+    tiles.clear();
+    colblk.clear();
     c.max_splits = 1;
     long long cp = 0;
     for (SysHost& s : c.sys) {
@@ -484,6 +490,12 @@ gbi_status build_tiles(gbi_ctx& c) {
     c.n_tiles = (int)tiles.size();
     c.n_colblocks = (int)colblk.size();
     c.colpart_total = std::max<long long>(1, cp);
+}
+
+gbi_status build_tiles(gbi_ctx& c) {
+    std::vector<int4> tiles;
+    std::vector<int> colblk;
+    tile_layout(c, tiles, colblk);
     c.drop_graphs();
     HIP_TRY(hipStreamSynchronize(c.stream));
     HIP_TRY(c.pair_tiles.ensure(tiles.size()));
@@ -949,10 +961,9 @@ gbi_status apply_tight_grid(gbi_ctx& c, double g) {
     return GBI_OK;
 }
 
-gbi_status create_batch(int S, const int32_t* n_atoms, const double* charge, const double* radius,
-                        const gbi_params* params, gbi_ctx** out, bool custom = false) {
-    if (!out || !n_atoms || !charge || !radius || !params) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
-    *out = nullptr;
+// The host description of a batch (systems, atoms, quadrature rows): no device work.
+gbi_status setup_host(gbi_ctx& c, int S, const int32_t* n_atoms, const double* charge, const double* radius,
+                      const gbi_params* params, bool custom) {
     if (S <= 0) return fail(GBI_ERR_INVALID_ARGUMENT, "n_systems must be positive");
     long long Ntot = 0, NQtot = 0;
     for (int s = 0; s < S; ++s) {
@@ -963,33 +974,14 @@ gbi_status create_batch(int S, const int32_t* n_atoms, const double* charge, con
     if (Ntot >= gbi::kMaxAtoms) return fail(GBI_ERR_INVALID_ARGUMENT, "too many atoms (at most 2^21 - 1)");
     if (NQtot >= (1ll << 31)) return fail(GBI_ERR_INVALID_ARGUMENT, "N * Q must be below 2^31");
     ST_TRY(check_atoms((int)Ntot, charge, radius));
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GBI_ERR_NO_DEVICE, "no HIP device");
-    auto* c = new gbi_ctx();
-    auto cleanup = [&](gbi_status st) {
-        delete c;
-        return st;
-    };
-#define CTX_TRY(expr)                      \
-    do {                                   \
-        gbi_status s_ = (expr);            \
-        if (s_ != GBI_OK) return cleanup(s_); \
-    } while (0)
-#define CTX_HIP(expr)                                                                                \
-    do {                                                                                             \
-        hipError_t e_ = (expr);                                                                      \
-        if (e_ != hipSuccess)                                                                        \
-            return cleanup(fail(e_ == hipErrorOutOfMemory ? GBI_ERR_OUT_OF_MEMORY : GBI_ERR_HIP,     \
-                                std::string(#expr) + ": " + hipGetErrorString(e_)));                 \
-    } while (0)
-    c->S = S;
-    c->N = (int)Ntot;
-    c->NQ = (int)NQtot;
-    c->charge.assign(charge, charge + Ntot);
-    c->radius.assign(radius, radius + Ntot);
-    c->tuning.blocks_per_cu = kDefaultBlocksPerCU;
-    c->tuning.pair_columns = 0;
-    c->tuning.tight_grid_length = 0.0;
+    c.S = S;
+    c.N = (int)Ntot;
+    c.NQ = (int)NQtot;
+    c.charge.assign(charge, charge + Ntot);
+    c.radius.assign(radius, radius + Ntot);
+    c.tuning.blocks_per_cu = kDefaultBlocksPerCU;
+    c.tuning.pair_columns = 0;
+    c.tuning.tight_grid_length = 0.0;
     int atom = 0, qrow = 0, pt = 0;
     for (int s = 0; s < S; ++s) {
         SysHost h;
@@ -1014,103 +1006,248 @@ gbi_status create_batch(int S, const int32_t* n_atoms, const double* charge, con
         const double est = 1.6 * 100.0 * h.Q * h.g * h.g * h.g + 64.0;
         h.pt_cap = h.kind == gbi::GRID_BRUTE ? h.N * h.Q : (int)std::min<double>(est, (double)h.N * h.Q);
         h.pt_cap = std::max(h.pt_cap, 1);
-        c->kinds[h.vol] = true;
-        c->any_cut = c->any_cut || h.cutoff();
-        c->any_trunc = c->any_trunc || (h.cutoff() && h.P.model == GBI_MODEL_GBSW);
+        c.kinds[h.vol] = true;
+        c.any_cut = c.any_cut || h.cutoff();
+        c.any_trunc = c.any_trunc || (h.cutoff() && h.P.model == GBI_MODEL_GBSW);
         atom += h.N;
         qrow += h.Q;
         pt += h.N * h.Q;
-        c->sys.push_back(h);
+        c.sys.push_back(h);
     }
-    c->quad_rows = qrow;
-    (void)hipGetDevice(&c->device);
-    if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
-        c->num_cus <= 0)
-        c->num_cus = 256;
-    CTX_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
-    c->stream = c->own_stream;
-    const size_t N = c->N, NQ = c->NQ;
-    CTX_HIP(c->d_sys.ensure(S));
-    CTX_HIP(c->geom.ensure(S));
-    CTX_HIP(c->d_box.ensure(3 * (size_t)S));
-    CTX_HIP(hipMemsetAsync(c->d_box.p, 0, sizeof(double) * 3 * S, c->stream));
-    CTX_HIP(c->flags_all.ensure(1));
-    CTX_HIP(c->atom_sys.ensure(N));
-    CTX_HIP(c->pos.ensure(3 * N));
-    CTX_HIP(c->d_charge.ensure(N));
-    CTX_HIP(c->d_radius.ensure(N));
-    CTX_HIP(c->d_reach.ensure(N));
-    CTX_HIP(c->aparam.ensure(N));
-    CTX_HIP(c->quad_off.ensure(3 * (size_t)qrow));
-    CTX_HIP(c->quad_pw.ensure(2 * (size_t)qrow));
-    CTX_HIP(c->quad_r2.ensure(qrow));
-    CTX_HIP(c->integrals.ensure(2 * N));
-    CTX_HIP(c->born.ensure(N));
-    CTX_HIP(c->dbdi.ensure(2 * N));
-    CTX_HIP(c->dedi.ensure(2 * N));
-    CTX_HIP(c->atom_energy.ensure(N));
-    CTX_HIP(c->energy.ensure(S));
-    CTX_HIP(c->stats.ensure(6));
-    CTX_HIP(c->work.ensure(8));
-    CTX_HIP(c->frec0.ensure(NQ));
-    CTX_HIP(c->frec1.ensure(NQ));
-    CTX_HIP(c->frec2.ensure(c->kinds[gbi::VOL_GBMV2] ? NQ : 1));
-    CTX_HIP(c->vrec.ensure(NQ));
-    CTX_HIP(c->vval.ensure(NQ));
-    CTX_HIP(c->force_fix.ensure(3 * N));
-    // quadrature tables: offsets and w_q / r_q^(order-2) per integral, one block of rows per system
-    std::vector<double> off(3 * (size_t)qrow), pw(2 * (size_t)qrow);
-    std::vector<float> r2(qrow);
-    for (const SysHost& h : c->sys) {
+    c.quad_rows = qrow;
+    return GBI_OK;
+}
+
+// Host tables of a batch: quadrature offsets, w_q / r_q^(order-2) per integral and |off_q|^2
+// (one block of rows per system), the system of every atom, and the binning order (systems in
+// turn, each with a stride coprime to its size so that concurrent workgroups hit distant
+// cells of the same system's counters).
+struct HostTables {
+    std::vector<double> off, pw;
+    std::vector<float> r2;
+    std::vector<int> atom_sys, bin_order, pair_atom;
+};
+
+gbi_status host_tables(const gbi_ctx& c, HostTables& t) {
+    const int qrow = c.quad_rows;
+    t.off.assign(3 * (size_t)qrow, 0.0);
+    t.pw.assign(2 * (size_t)qrow, 0.0);
+    t.r2.assign(qrow, 0.f);
+    for (const SysHost& h : c.sys) {
         std::vector<std::array<double, 5>> table;
         try {
             table = gbi::quadrature_table(h.P.n_radial, h.P.r0, h.P.r1, h.P.n_lebedev);
         } catch (const std::exception& e) {
-            return cleanup(fail(GBI_ERR_INVALID_ARGUMENT, e.what()));
+            return fail(GBI_ERR_INVALID_ARGUMENT, e.what());
         }
         for (int q = 0; q < h.Q; ++q) {
             const size_t row = (size_t)h.quad_base + q;
-            for (int d = 0; d < 3; ++d) off[3 * row + d] = table[q][d];
-            for (int k = 0; k < 2; ++k) pw[k * (size_t)qrow + row] = table[q][4] / std::pow(table[q][3], h.P.orders[k] - 2);
-            r2[row] = (float)(table[q][3] * table[q][3]);
+            for (int d = 0; d < 3; ++d) t.off[3 * row + d] = table[q][d];
+            for (int k = 0; k < 2; ++k)
+                t.pw[k * (size_t)qrow + row] = table[q][4] / std::pow(table[q][3], h.P.orders[k] - 2);
+            t.r2[row] = (float)(table[q][3] * table[q][3]);
         }
     }
-    std::vector<int> as(N);
-    for (int s = 0; s < S; ++s) std::fill(as.begin() + c->sys[s].atom_begin, as.begin() + c->sys[s].atom_begin + c->sys[s].N, s);
-    CTX_HIP(hipMemcpyAsync(c->quad_off.p, off.data(), sizeof(double) * off.size(), hipMemcpyHostToDevice, c->stream));
-    CTX_HIP(hipMemcpyAsync(c->quad_pw.p, pw.data(), sizeof(double) * pw.size(), hipMemcpyHostToDevice, c->stream));
-    CTX_HIP(hipMemcpyAsync(c->quad_r2.p, r2.data(), sizeof(float) * r2.size(), hipMemcpyHostToDevice, c->stream));
-    CTX_HIP(hipMemcpyAsync(c->atom_sys.p, as.data(), sizeof(int) * N, hipMemcpyHostToDevice, c->stream));
-    // binning visits the systems in turn, each with a stride coprime to its size so that
-    // concurrent workgroups hit distant cells of the same system's counters
-    std::vector<int> order(N);
-    for (const SysHost& h : c->sys) {
+    t.atom_sys.assign(c.N, 0);
+    for (int s = 0; s < c.S; ++s)
+        std::fill(t.atom_sys.begin() + c.sys[s].atom_begin, t.atom_sys.begin() + c.sys[s].atom_begin + c.sys[s].N, s);
+    t.bin_order.assign(c.N, 0);
+    for (const SysHost& h : c.sys) {
         long long stride = 1;
         for (int cand : {7919, 7927, 7933, 7937, 7949})
             if (h.N % cand != 0) {
                 stride = cand;
                 break;
             }
-        for (int k = 0; k < h.N; ++k) order[h.atom_begin + k] = h.atom_begin + (int)((k * stride) % h.N);
+        for (int k = 0; k < h.N; ++k) t.bin_order[h.atom_begin + k] = h.atom_begin + (int)((k * stride) % h.N);
     }
-    CTX_HIP(c->bin_order.ensure(N));
-    CTX_HIP(c->pair_atom.ensure(N));
-    {
-        std::vector<int> ident(N);
-        for (size_t k = 0; k < N; ++k) ident[k] = (int)k;
-        CTX_HIP(hipMemcpyAsync(c->pair_atom.p, ident.data(), sizeof(int) * N, hipMemcpyHostToDevice, c->stream));
-        CTX_HIP(hipStreamSynchronize(c->stream));
-        c->pair_order_host = ident;
-    }
-    CTX_HIP(hipMemcpyAsync(c->bin_order.p, order.data(), sizeof(int) * N, hipMemcpyHostToDevice, c->stream));
-    CTX_HIP(hipStreamSynchronize(c->stream));
-    CTX_TRY(upload_atoms(*c));
-    CTX_TRY(build_tiles(*c));
-#undef CTX_TRY
-#undef CTX_HIP
-    *out = c;
+    t.pair_atom.resize(c.N);
+    for (int k = 0; k < c.N; ++k) t.pair_atom[k] = k;
     return GBI_OK;
 }
+
+gbi_status create_batch(int S, const int32_t* n_atoms, const double* charge, const double* radius,
+                        const gbi_params* params, gbi_ctx** out, bool custom = false) {
+    if (!out || !n_atoms || !charge || !radius || !params) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    auto owned = std::make_unique<gbi_ctx>();
+    gbi_ctx* c = owned.get();
+    ST_TRY(setup_host(*c, S, n_atoms, charge, radius, params, custom));
+    HostTables t;
+    ST_TRY(host_tables(*c, t));
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GBI_ERR_NO_DEVICE, "no HIP device");
+    (void)hipGetDevice(&c->device);
+    if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+        c->num_cus <= 0)
+        c->num_cus = 256;
+    HIP_TRY(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+    c->stream = c->own_stream;
+    const size_t N = c->N, NQ = c->NQ, qrow = c->quad_rows;
+    HIP_TRY(c->d_sys.ensure(S));
+    HIP_TRY(c->geom.ensure(S));
+    HIP_TRY(c->d_box.ensure(3 * (size_t)S));
+    HIP_TRY(hipMemsetAsync(c->d_box.p, 0, sizeof(double) * 3 * S, c->stream));
+    HIP_TRY(c->flags_all.ensure(1));
+    HIP_TRY(c->atom_sys.ensure(N));
+    HIP_TRY(c->pos.ensure(3 * N));
+    HIP_TRY(c->d_charge.ensure(N));
+    HIP_TRY(c->d_radius.ensure(N));
+    HIP_TRY(c->d_reach.ensure(N));
+    HIP_TRY(c->aparam.ensure(N));
+    HIP_TRY(c->quad_off.ensure(3 * qrow));
+    HIP_TRY(c->quad_pw.ensure(2 * qrow));
+    HIP_TRY(c->quad_r2.ensure(qrow));
+    HIP_TRY(c->integrals.ensure(2 * N));
+    HIP_TRY(c->born.ensure(N));
+    HIP_TRY(c->dbdi.ensure(2 * N));
+    HIP_TRY(c->dedi.ensure(2 * N));
+    HIP_TRY(c->atom_energy.ensure(N));
+    HIP_TRY(c->energy.ensure(S));
+    HIP_TRY(c->stats.ensure(6));
+    HIP_TRY(c->work.ensure(8));
+    HIP_TRY(c->frec0.ensure(NQ));
+    HIP_TRY(c->frec1.ensure(NQ));
+    HIP_TRY(c->frec2.ensure(c->kinds[gbi::VOL_GBMV2] ? NQ : 1));
+    HIP_TRY(c->vrec.ensure(NQ));
+    HIP_TRY(c->vval.ensure(NQ));
+    HIP_TRY(c->force_fix.ensure(3 * N));
+    HIP_TRY(c->bin_order.ensure(N));
+    HIP_TRY(c->pair_atom.ensure(N));
+    HIP_TRY(hipMemcpyAsync(c->quad_off.p, t.off.data(), sizeof(double) * t.off.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->quad_pw.p, t.pw.data(), sizeof(double) * t.pw.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->quad_r2.p, t.r2.data(), sizeof(float) * t.r2.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->atom_sys.p, t.atom_sys.data(), sizeof(int) * N, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->pair_atom.p, t.pair_atom.data(), sizeof(int) * N, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->bin_order.p, t.bin_order.data(), sizeof(int) * N, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->pair_order_host = t.pair_atom;
+    ST_TRY(upload_atoms(*c));
+    ST_TRY(build_tiles(*c));
+    *out = owned.release();
+    return GBI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// gbi_execute_noalloc: one system evaluated without any device allocation.  Every device
+// buffer is a piece of the library's static scratch array (gbi_noalloc.hip), its capacities
+// computed on the host from the positions: the lookup grid the device will build (the same
+// FP64 expressions as bbox_to_geom, candidate_box, cell_member and point_cell), each cell's
+// raw members and quadrature points, with margins for decisions that could round the other
+// way at a cell face.  A grid that still outgrows them is an error, not a regrowth.
+// ---------------------------------------------------------------------------
+
+struct NoAllocCaps {
+    long long cells = 0, entries = 0;
+    int tmp_cap = 0, pt_cap = 0;
+};
+
+NoAllocCaps noalloc_caps(const SysHost& s, const double* pos, const double* radius, const double* reach) {
+    NoAllocCaps caps;
+    const int N = s.N;
+    if (s.kind == gbi::GRID_BRUTE) {
+        caps.cells = 1;
+        caps.entries = N;
+        caps.tmp_cap = N;
+        caps.pt_cap = N * s.Q;
+        return caps;
+    }
+    const bool faithful = s.kind == gbi::GRID_FAITHFUL;
+    const double g = s.g, inv_g = 1.0 / s.g;
+    const double gpad = faithful ? s.gpad : s.max_reach;
+    double mn[3], mx[3], origin[3], gmax[3];
+    int n[3];
+    for (int d = 0; d < 3; ++d) mn[d] = mx[d] = pos[d];
+    for (int a = 0; a < N; ++a)
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = std::min(mn[d], pos[3 * a + d]);
+            mx[d] = std::max(mx[d], pos[3 * a + d]);
+        }
+    for (int d = 0; d < 3; ++d) {
+        const double lo = mn[d] - gpad, hi = mx[d] + gpad, len = hi - lo;
+        n[d] = len > 1000.0 ? 1 : (faithful ? (int)std::ceil(len / g) + 1 : (int)std::floor(len * inv_g) + 1);
+        origin[d] = lo;
+        gmax[d] = faithful ? lo + (n[d] - 1) * g : lo + n[d] * g;
+    }
+    const long long ncell = (long long)n[0] * n[1] * n[2];
+    caps.cells = (long long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);  // room for one more cell per axis
+    std::vector<int> raw(ncell, 0), pts(ncell, 0);
+    for (int j = 0; j < N; ++j) {
+        const double* x = pos + 3 * j;
+        const double pad = faithful ? (radius[j] + s.halfdiag) + s.P.buffer : reach[j];
+        int b[3], e[3];
+        for (int d = 0; d < 3; ++d) {
+            b[d] = faithful ? (int)std::floor((x[d] - pad - origin[d]) / g) : (int)std::floor((x[d] - pad - origin[d]) * inv_g);
+            e[d] = faithful ? (int)std::ceil((x[d] + pad - origin[d]) / g) : (int)std::floor((x[d] + pad - origin[d]) * inv_g);
+            b[d] = std::max(b[d], 0);
+            e[d] = std::min(e[d], n[d] - 1);
+        }
+        for (int i = b[0]; i <= e[0]; ++i)
+            for (int jj = b[1]; jj <= e[1]; ++jj)
+                for (int k = b[2]; k <= e[2]; ++k) {
+                    const double lx = origin[0] + i * g, ly = origin[1] + jj * g, lz = origin[2] + k * g;
+                    double dx, dy, dz;
+                    if (faithful) {
+                        dx = lx - x[0];
+                        dy = ly - x[1];
+                        dz = lz - x[2];
+                    } else {
+                        dx = std::max(std::max(lx - x[0], x[0] - (lx + g)), 0.0);
+                        dy = std::max(std::max(ly - x[1], x[1] - (ly + g)), 0.0);
+                        dz = std::max(std::max(lz - x[2], x[2] - (lz + g)), 0.0);
+                    }
+                    if (dx * dx + dy * dy + dz * dz < pad * pad) ++raw[((long long)i * n[1] + jj) * n[2] + k];
+                }
+    }
+    std::vector<std::array<double, 5>> table = gbi::quadrature_table(s.P.n_radial, s.P.r0, s.P.r1, s.P.n_lebedev);
+    for (int i = 0; i < N; ++i)
+        for (int q = 0; q < s.Q; ++q) {
+            long long c = 0;
+            for (int d = 0; d < 3; ++d) {
+                const double pt = pos[3 * i + d] + table[q][d];
+                int idx;
+                if (faithful) {
+                    if (pt < origin[d] || pt > gmax[d]) c = -1;
+                    idx = (int)std::floor((pt - origin[d]) / g);
+                } else {
+                    const double t = (pt - origin[d]) * inv_g;
+                    if (!(t >= 0.0) || t >= (double)n[d]) c = -1;
+                    idx = (int)std::floor(t);
+                }
+                if (c >= 0) c = c * n[d] + std::min(std::max(idx, 0), n[d] - 1);
+            }
+            if (c >= 0 && raw[c] > 0) ++pts[c];
+        }
+    int max_raw = 0, max_pts = 0;
+    long long members = 0;
+    const int cap = (s.P.cell_cap > 0) ? s.P.cell_cap : INT_MAX;
+    for (long long c = 0; c < ncell; ++c) {
+        max_raw = std::max(max_raw, raw[c]);
+        max_pts = std::max(max_pts, pts[c]);
+        members += std::min(raw[c], cap);
+    }
+    caps.tmp_cap = max_raw + max_raw / 8 + 16;
+    caps.pt_cap = max_pts + max_pts / 8 + 16;
+    caps.entries = members + members / 8 + 1024;
+    return caps;
+}
+
+// Pieces of the scratch array (256-byte aligned) handed to buffers of a context, which never
+// frees them; without a base it only adds up the bytes.
+struct ScratchCarver {
+    char* base = nullptr;
+    size_t used = 0;
+    template <class T>
+    void take(DevBuf<T>& b, size_t count) {
+        count = std::max<size_t>(count, 1);
+        used = (used + 255) & ~size_t(255);
+        if (base) {
+            b.p = reinterpret_cast<T*>(base + used);
+            b.n = count;
+        }
+        used += count * sizeof(T);
+    }
+};
 
 }  // namespace
 
@@ -1513,6 +1650,157 @@ gbi_status gbi_execute(gbi_ctx* ctx, const double* pos, int32_t include_forces, 
     ST_TRY(run_checked(*ctx, GBI_FORCES | GBI_ENERGY));
     if (include_energy && energy) ST_TRY(gbi_get_energy(ctx, energy));
     if (include_forces) return gbi_get_forces(ctx, forces, 1);
+    return GBI_OK;
+}
+
+gbi_status gbi_execute_noalloc(int32_t n_atoms, const double* charge, const double* radius, const gbi_params* params,
+                               const double* pos, int32_t include_forces, int32_t include_energy, double* energy,
+                               double* forces, void* hip_stream) {
+    if (!charge || !radius || !params || !pos) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
+    if (include_forces && !forces) return fail(GBI_ERR_INVALID_ARGUMENT, "null force array");
+    if (params->nonbonded_method != GBI_NO_CUTOFF)
+        return fail(GBI_ERR_UNSUPPORTED, "gbi_execute_noalloc evaluates NoCutoff systems (use gbi_create / gbi_execute)");
+    for (long long a = 0; a < 3ll * std::max(n_atoms, 0); ++a)
+        if (!std::isfinite(pos[a])) return fail(GBI_ERR_NONFINITE, "non-finite atom position");
+    gbi_ctx c;
+    // the buffers below are pieces of the scratch array: the guard takes them back from the
+    // context before it is destroyed, so nothing is ever freed
+    struct Unborrow {
+        gbi_ctx& c;
+        ~Unborrow() {
+            auto clear = [](auto& b) {
+                b.p = nullptr;
+                b.n = 0;
+            };
+            clear(c.d_sys), clear(c.geom), clear(c.d_box), clear(c.flags_all), clear(c.work), clear(c.atom_sys);
+            clear(c.bin_order), clear(c.pair_atom), clear(c.colblk), clear(c.blk_lo), clear(c.blk_hi);
+            clear(c.pair_tiles), clear(c.pos), clear(c.d_charge), clear(c.d_radius), clear(c.d_reach);
+            clear(c.aparam), clear(c.quad_off), clear(c.quad_pw), clear(c.quad_r2), clear(c.integrals), clear(c.born);
+            clear(c.dbdi), clear(c.dedi), clear(c.atom_energy), clear(c.energy), clear(c.pair_part), clear(c.colpart);
+            clear(c.force_fix), clear(c.cell_fill), clear(c.cell_count_eff), clear(c.cell_pt_count);
+            clear(c.cell_act_count), clear(c.cell_sys), clear(c.cell_start), clear(c.frec_base), clear(c.worklist);
+            clear(c.worklist_sys), clear(c.scan_tmp), clear(c.cell_atoms_tmp), clear(c.cell_atoms), clear(c.ent_pos);
+            clear(c.ent_par), clear(c.prec), clear(c.frec0), clear(c.frec1), clear(c.frec2), clear(c.vval);
+            clear(c.vrec);
+        }
+    } unborrow{c};
+    const int32_t n = n_atoms;
+    ST_TRY(setup_host(c, 1, &n, charge, radius, params, false));
+    HostTables t;
+    ST_TRY(host_tables(c, t));
+    std::vector<float4> ap;
+    std::vector<double> reach;
+    atom_params(c, ap, reach);
+    set_max_reach(c, reach);
+    std::vector<int4> tiles;
+    std::vector<int> colblk;
+    tile_layout(c, tiles, colblk);
+    SysHost& sh = c.sys[0];
+    NoAllocCaps caps;
+    try {
+        caps = noalloc_caps(sh, pos, radius, reach.data());
+    } catch (const std::exception& e) {
+        return fail(GBI_ERR_INVALID_ARGUMENT, e.what());
+    }
+    if (caps.cells > kMaxCells || caps.entries > INT_MAX / 2)
+        return fail(GBI_ERR_GRID_TOO_LARGE, "lookup grid too large for gbi_execute_noalloc");
+    sh.cell_cap = caps.cells;
+    sh.tmp_cap = caps.tmp_cap;
+    sh.pt_cap = caps.pt_cap;
+    c.cell_total = caps.cells;
+    c.entry_cap = (int)caps.entries;
+    c.prec_total = caps.cells * (long long)caps.pt_cap;
+
+    // the buffers, carved once without a base (the size check, before the device is touched)
+    // and once from the scratch array
+    const size_t N = c.N, NQ = c.NQ, Q = c.quad_rows, cells = caps.cells;
+    auto carve = [&](ScratchCarver& sc) {
+    sc.take(c.d_sys, 1), sc.take(c.geom, 1), sc.take(c.d_box, 3), sc.take(c.flags_all, 1), sc.take(c.work, 8);
+    sc.take(c.atom_sys, N), sc.take(c.bin_order, N), sc.take(c.pair_atom, N), sc.take(c.colblk, colblk.size());
+    sc.take(c.blk_lo, colblk.size()), sc.take(c.blk_hi, colblk.size()), sc.take(c.pair_tiles, tiles.size());
+    sc.take(c.pos, 3 * N), sc.take(c.d_charge, N), sc.take(c.d_radius, N), sc.take(c.d_reach, N), sc.take(c.aparam, N);
+    sc.take(c.quad_off, 3 * Q), sc.take(c.quad_pw, 2 * Q), sc.take(c.quad_r2, Q);
+    sc.take(c.integrals, 2 * N), sc.take(c.born, N), sc.take(c.dbdi, 2 * N), sc.take(c.dedi, 2 * N);
+    sc.take(c.atom_energy, N), sc.take(c.energy, 1), sc.take(c.pair_part, (size_t)c.max_splits * 6 * N);
+    sc.take(c.colpart, (size_t)5 * c.colpart_total), sc.take(c.force_fix, 3 * N);
+    sc.take(c.cell_fill, cells), sc.take(c.cell_count_eff, cells), sc.take(c.cell_pt_count, cells);
+    sc.take(c.cell_act_count, cells), sc.take(c.cell_sys, cells), sc.take(c.cell_start, cells + 1);
+    sc.take(c.frec_base, cells + 1), sc.take(c.worklist, gbi::VOL_KINDS * cells);
+    sc.take(c.worklist_sys, gbi::VOL_KINDS * cells), sc.take(c.scan_tmp, (cells + 4095) / 4096 + 16);
+    sc.take(c.cell_atoms_tmp, cells * (size_t)caps.tmp_cap), sc.take(c.cell_atoms, caps.entries);
+    sc.take(c.ent_pos, caps.entries), sc.take(c.ent_par, caps.entries), sc.take(c.prec, c.prec_total);
+    sc.take(c.frec0, NQ), sc.take(c.frec1, NQ), sc.take(c.frec2, c.kinds[gbi::VOL_GBMV2] ? NQ : 1);
+    sc.take(c.vval, NQ), sc.take(c.vrec, NQ);
+    };
+    ScratchCarver dry;
+    carve(dry);
+    if (dry.used > gbi::kNoAllocScratchBytes)
+        return fail(GBI_ERR_INVALID_ARGUMENT,
+                    "system too large for gbi_execute_noalloc: it needs " + std::to_string((dry.used >> 20) + 1) +
+                        " MiB of device scratch, the library holds " +
+                        std::to_string(gbi::kNoAllocScratchBytes >> 20) + " MiB (use gbi_create / gbi_execute)");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GBI_ERR_NO_DEVICE, "no HIP device");
+    (void)hipGetDevice(&c.device);
+    if (hipDeviceGetAttribute(&c.num_cus, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess ||
+        c.num_cus <= 0)
+        c.num_cus = 256;
+    void* base = nullptr;
+    HIP_TRY(gbi::noalloc_scratch(&base));  // this device's array (looked up once per device)
+    ScratchCarver sc;
+    sc.base = static_cast<char*>(base);
+    carve(sc);
+
+    // one call at a time per process: every call on a GPU shares its array
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
+    hipStream_t st = (hipStream_t)hip_stream;
+    c.stream = st;
+    const gbi::SysDesc desc = make_desc(c, sh, 0, 0, 0);
+    const double box[3] = {0.0, 0.0, 0.0};
+    std::vector<int> cell_sys(cells, 0);
+    auto h2d = [st](void* dst, const void* src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+    };
+    HIP_TRY(h2d(c.d_sys.p, &desc, sizeof(desc)));
+    HIP_TRY(h2d(c.d_box.p, box, sizeof(box)));
+    HIP_TRY(h2d(c.atom_sys.p, t.atom_sys.data(), sizeof(int) * N));
+    HIP_TRY(h2d(c.bin_order.p, t.bin_order.data(), sizeof(int) * N));
+    HIP_TRY(h2d(c.pair_atom.p, t.pair_atom.data(), sizeof(int) * N));
+    HIP_TRY(h2d(c.colblk.p, colblk.data(), sizeof(int) * colblk.size()));
+    HIP_TRY(h2d(c.pair_tiles.p, tiles.data(), sizeof(int4) * tiles.size()));
+    HIP_TRY(h2d(c.pos.p, pos, sizeof(double) * 3 * N));
+    HIP_TRY(h2d(c.d_charge.p, c.charge.data(), sizeof(double) * N));
+    HIP_TRY(h2d(c.d_radius.p, c.radius.data(), sizeof(double) * N));
+    HIP_TRY(h2d(c.d_reach.p, reach.data(), sizeof(double) * N));
+    HIP_TRY(h2d(c.aparam.p, ap.data(), sizeof(float4) * N));
+    HIP_TRY(h2d(c.quad_off.p, t.off.data(), sizeof(double) * t.off.size()));
+    HIP_TRY(h2d(c.quad_pw.p, t.pw.data(), sizeof(double) * t.pw.size()));
+    HIP_TRY(h2d(c.quad_r2.p, t.r2.data(), sizeof(float) * t.r2.size()));
+    HIP_TRY(h2d(c.cell_sys.p, cell_sys.data(), sizeof(int) * cells));
+    const gbi::KParams k = make_kparams(c, false);
+    HIP_TRY(gbi::launch_evaluate(k, c.kinds, st, nullptr));
+    gbi::SysGeom G;
+    int all = 0;
+    std::vector<long long> fx(3 * N);
+    double e = 0.0;
+    HIP_TRY(hipMemcpyAsync(&G, c.geom.p, sizeof(G), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&all, c.flags_all.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&e, c.energy.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(fx.data(), c.force_fix.p, sizeof(long long) * 3 * N, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    all |= G.flags;
+    if (all & gbi::FLAG_NONFINITE_POS) return fail(GBI_ERR_NONFINITE, "non-finite atom position");
+    if (all & gbi::FLAG_GRID_TOO_LARGE)
+        return fail(GBI_ERR_GRID_TOO_LARGE,
+                    "CharmmGBMVForce: CustomGBIntegral lookup table dimension is too large, check atom positions!");
+    if (all & (gbi::FLAG_CELL_OVERFLOW | gbi::FLAG_ENTRY_OVERFLOW | gbi::FLAG_POINT_OVERFLOW | gbi::FLAG_LIST_OVERFLOW))
+        return fail(GBI_ERR_CAPACITY, "the lookup grid outgrew the capacities sized for it (use gbi_create / gbi_execute)");
+    if (all & gbi::FLAG_NONFINITE_RESULT) return fail(GBI_ERR_NONFINITE, "non-finite Born radius, energy or force");
+    if (include_energy && energy) *energy = e;
+    if (include_forces)
+        for (size_t a = 0; a < N; ++a)
+            for (int d = 0; d < 3; ++d) forces[3 * a + d] += (double)fx[(size_t)d * N + a] / 4294967296.0;
     return GBI_OK;
 }
 

@@ -237,4 +237,9 @@ constexpr int kPasses = 8;
 hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stream, const hipEvent_t* marks);
 hipError_t launch_stats(const KParams& p, double* out6, hipStream_t stream);
 
+// gbi_execute_noalloc: bytes of the static device scratch array (per GPU), and its device
+// address on the current device (gbi_noalloc.hip).
+constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;
+hipError_t noalloc_scratch(void** ptr);
+
 }  // namespace gbi
