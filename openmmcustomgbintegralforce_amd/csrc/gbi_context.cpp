@@ -539,7 +539,18 @@ void estimate_caps(const gbi_ctx& c, const SysHost& s, const double* pos, long l
     }
     // entries: box of candidate cells per atom (an upper bound of the members)
     double side;
-    if (s.kind == gbi::GRID_FAITHFUL) {
+    if (s.kind == gbi::GRID_FAITHFUL && s.periodic()) {
+        // samples of the periodic walk per axis: 2 (pad + g') / step + 1, pad = R + sqrt3/2 g' + buffer,
+        // g' the largest and step the smallest cell edge L / (ceil(L / g) + 1)
+        double maxR = 0.0, gmax = 0.0, gmin = 1e300;
+        for (int j = s.atom_begin; j < s.atom_begin + s.N; ++j) maxR = std::max(maxR, c.radius[j]);
+        for (int d = 0; d < 3; ++d) {
+            const double st = s.box[d] / (std::ceil(s.box[d] / g) + 1.0);
+            gmax = std::max(gmax, st);
+            gmin = std::min(gmin, st);
+        }
+        side = std::floor(2.0 * (maxR + std::sqrt(3.0) / 2.0 * gmax + s.P.buffer + gmax) / gmin) + 3.0;
+    } else if (s.kind == gbi::GRID_FAITHFUL) {
         double maxR = 0.0;
         for (int j = s.atom_begin; j < s.atom_begin + s.N; ++j) maxR = std::max(maxR, c.radius[j]);
         side = std::floor(2.0 * (maxR + s.halfdiag + s.P.buffer) / g) + 3.0;

@@ -441,9 +441,68 @@ __device__ inline void list_position(const KParams& p, const SysDesc& sd, const 
     for (int d = 0; d < 3; ++d) x[d] = sd.periodic ? wrap_coord(G, d, p.pos[3 * j + d]) : p.pos[3 * j + d];
 }
 
+// Appends key to cell's raw list (fixed capacity per cell; the atomic's count stays exact on
+// overflow, so the host can size the lists from it).
+__device__ inline void append_member(const KParams& p, const SysDesc& sd, int s, int cell, int key) {
+    const int slot = atomicAdd(&p.cell_fill[sd.cell_base + cell], 1);
+    if (slot < sd.tmp_cap)
+        p.cell_atoms_tmp[sd.tmp_base + (long long)cell * sd.tmp_cap + slot] = key;
+    else
+        raise_flag(p, s, FLAG_LIST_OVERFLOW);
+}
+
+constexpr int kMaxWalk = 1024;  // samples per axis of the periodic walk (about 12 at the reference's sizes)
+
+// The reference's periodic membership of CharmmGBMVForce grids (CustomGBIntegral.cpp:114-157):
+// sample points walked from x - pad - g' to x + pad + g' (g' = max L/n) in steps of L/n per
+// axis, each coordinate accumulated in FP64 as the reference's loop does (sample k of an axis
+// = k successive additions); a sample within pad of the atom (x unwrapped) puts the atom in
+// the cell of the sample wrapped into the box, with the image code of that wrap.  One lane
+// per sample.
+__device__ void fill_periodic_samples(const KParams& p, const SysDesc& sd, int s, const SysGeom& G, int j) {
+#pragma clang fp contract(off)
+    const double x[3] = {p.pos[3 * j], p.pos[3 * j + 1], p.pos[3 * j + 2]};
+    const double pad = atom_pad(p, sd, G, j);
+    const double gmax = fmax(fmax(G.step[0], G.step[1]), G.step[2]);
+    double bc[3], ec[3];
+    int cnt[3], kx[3];
+    for (int d = 0; d < 3; ++d) {
+        bc[d] = x[d] - pad - gmax;
+        ec[d] = x[d] + pad + gmax;
+        int k = 0;
+        for (double v = bc[d]; v < ec[d] && k < kMaxWalk; v += G.step[d]) ++k;
+        cnt[d] = k;
+        kx[d] = (int)floor((x[d] - G.origin[d]) / G.L[d]);  // the wrap of the atom (wrap_coord)
+    }
+    if (cnt[0] >= kMaxWalk || cnt[1] >= kMaxWalk || cnt[2] >= kMaxWalk) {  // a radius of many boxes
+        if (threadIdx.x == 0) raise_flag(p, s, FLAG_GRID_TOO_LARGE);
+        return;
+    }
+    const long long total = (long long)cnt[0] * cnt[1] * cnt[2];
+    for (long long t = threadIdx.x; t < total; t += blockDim.x) {
+        const int i3[3] = {(int)(t / (cnt[1] * cnt[2])), (int)((t / cnt[2]) % cnt[1]), (int)(t % cnt[2])};
+        double smp[3];
+        for (int d = 0; d < 3; ++d) {
+            double v = bc[d];
+            for (int k = 0; k < i3[d]; ++k) v += G.step[d];
+            smp[d] = v;
+        }
+        const double dx = smp[0] - x[0], dy = smp[1] - x[1], dz = smp[2] - x[2];
+        if (!(dx * dx + dy * dy + dz * dz < pad * pad)) continue;
+        int idx[3], code = 0;
+        for (int d = 0; d < 3; ++d) {
+            const double ks = floor((smp[d] - G.origin[d]) / G.L[d]);
+            const double w = smp[d] - G.L[d] * ks;
+            int k = (int)floor((w - G.origin[d]) / G.step[d]);
+            idx[d] = k < 0 ? 0 : (k > G.n[d] - 1 ? G.n[d] - 1 : k);
+            code |= (((int)ks - kx[d]) & 3) << (2 * d);  // the atom's image next to the cell
+        }
+        append_member(p, sd, s, (idx[0] * G.n[1] + idx[1]) * G.n[2] + idx[2], (j << kImageBits) | code);
+    }
+}
+
 // One workgroup per atom, lanes over its candidate cells: each member key goes to its cell's
-// raw list (fixed capacity per cell; the atomic's count stays exact on overflow, so the host
-// can size the lists from it).  The CSR layout of the sorted lists is the scan of the counts.
+// raw list.  The CSR layout of the sorted lists is the scan of the counts.
 __global__ void __launch_bounds__(256) k_grid_fill(KParams p) {
     if (batch_fatal(p)) return;
     const int j = blockIdx.x;
@@ -460,19 +519,17 @@ __global__ void __launch_bounds__(256) k_grid_fill(KParams p) {
         return;
     }
     const SysGeom G = p.geom[s];
+    if (sd.periodic && sd.grid_kind == GRID_FAITHFUL) {
+        fill_periodic_samples(p, sd, s, G, j);
+        return;
+    }
     double x[3];
     list_position(p, sd, G, j, x);
     const double pad = atom_pad(p, sd, G, j);
     const CandBox B = candidate_box(sd, G, x, pad);
     for (int t = threadIdx.x; t < B.count; t += blockDim.x) {
         int cell, code;
-        if (candidate_cell(sd, G, B, t, x, pad, cell, code)) {
-            const int slot = atomicAdd(&p.cell_fill[sd.cell_base + cell], 1);
-            if (slot < sd.tmp_cap)
-                p.cell_atoms_tmp[sd.tmp_base + (long long)cell * sd.tmp_cap + slot] = (j << kImageBits) | code;
-            else
-                raise_flag(p, s, FLAG_LIST_OVERFLOW);
-        }
+        if (candidate_cell(sd, G, B, t, x, pad, cell, code)) append_member(p, sd, s, cell, (j << kImageBits) | code);
     }
 }
 
@@ -499,7 +556,9 @@ __device__ inline void write_entry(const KParams& p, const SysDesc& sd, const Sy
 // One wave per cell: order the cell's atoms by index (rank sort), keep the first `cap`
 // (the reference appends in ascending atom order and stops at the cap,
 // CustomGBIntegral.cpp:219-222) and write their corner-relative entries.  (Ordering GBSW
-// lists by distance to the cell, to reach V = 0 sooner, was measured slower.)
+// lists by distance to the cell, to reach V = 0 sooner, was measured slower.)  Only the
+// periodic sample walk can list an atom twice in a cell (two samples of its walk in the
+// cell), as the reference would; equal keys then take consecutive ranks.
 constexpr int kSortLds = 256;  // cells with more raw members sort in global memory
 
 __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
@@ -530,6 +589,7 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
             const int n4 = (n + 3) & ~3;
             for (int k = lane; k < n4; k += kWave) s_in[k] = k < n ? raw[k] : INT_MAX;
             __syncthreads();
+            const bool ties = sd.periodic && sd.grid_kind == GRID_FAITHFUL;
             for (int k = lane; k < n; k += kWave) {
                 const int a = s_in[k];
                 int r = 0;
@@ -537,6 +597,8 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
                     const int4 q = *reinterpret_cast<const int4*>(&s_in[m]);
                     r += (q.x < a) + (q.y < a) + (q.z < a) + (q.w < a);
                 }
+                if (ties)
+                    for (int m = 0; m < k; ++m) r += s_in[m] == a;
                 s_out[r] = a;
             }
             __syncthreads();
@@ -545,7 +607,7 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
             for (int k = lane; k < n; k += kWave) {
                 const int a = raw[k];
                 int r = 0;
-                for (int m = 0; m < n; ++m) r += raw[m] < a;
+                for (int m = 0; m < n; ++m) r += raw[m] < a || (raw[m] == a && m < k);
                 p.cell_atoms[base + r] = a;
             }
             __threadfence_block();

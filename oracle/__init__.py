@@ -25,7 +25,7 @@ _LIB_PATH = os.path.join(_HERE, "libgb_oracle.so")
 MODEL_GBSW_STANDALONE = 0
 MODEL_GBMV_FORM = 1
 INT_GBSW, INT_GBMV1, INT_GBMV2 = 0, 1, 2
-POLICY_GRID, POLICY_BRUTE = 0, 1
+POLICY_GRID, POLICY_BRUTE, POLICY_CORNERS = 0, 1, 2
 
 
 class Params(ctypes.Structure):

@@ -312,15 +312,15 @@ static void grid_free(grid_t* G) {
 
 /* Periodic lookup grid for a rectangular box (CustomGBIntegral.cpp:45-160): origin at the
  * centre of geometry minus half the box (:53-79), n = ceil(L/g) + 1 cells per axis of edge
- * L/n (:87-107), pad R_j + sqrt3/2 max(L/n) + buffer (:114-119).  Membership follows the
- * non-periodic corner rule, taken at the image of each cell nearest the atom: atom j,
- * wrapped into the box, joins the cells of the unwrapped index range around it, each index
- * reduced modulo n.  (The reference tests an atom-anchored sample lattice walked with
- * floating accumulation, :126-157, whose offsets inside the cells are an accident of
- * rounding.)  Atoms in ascending index, cap in index order.  The standalone GBSW model uses
- * its one pad (CharmmReferenceGBSW.cpp:454-455) with the cell edge max(L/n); its own
- * periodic lookup keeps the non-periodic grid and shifts only points outside it
- * (:527-536), which misses the images of atoms across the box faces. */
+ * L/n (:87-107), pad R_j + sqrt3/2 max(L/n) + buffer (:114-119).  GBMV form: the reference's
+ * membership, an atom-anchored sample lattice walked with floating accumulation (:120-157,
+ * below).  ORC_POLICY_CORNERS instead applies the non-periodic corner rule at the image of
+ * each cell nearest the atom (atom j, wrapped into the box, joins the cells of the unwrapped
+ * index range around it, each index reduced modulo n), kept only to measure how much the
+ * choice moves the result.  Atoms in ascending index, cap in index order.  The standalone
+ * GBSW model uses its one pad (CharmmReferenceGBSW.cpp:454-455) with the cell edge max(L/n)
+ * and the corner rule; its own periodic lookup keeps the non-periodic grid and shifts only
+ * points outside it (:527-536), which misses the images of atoms across the box faces. */
 static int grid_build_periodic(grid_t* G, const orc_params* p, int N, const double* pos, const double* R) {
     const int standalone = (p->model == ORC_MODEL_GBSW_STANDALONE);
     const double g = p->grid_length;
@@ -356,6 +356,56 @@ static int grid_build_periodic(grid_t* G, const orc_params* p, int N, const doub
     G->start = calloc((size_t)ncell + 1, sizeof(int));
     if (!count || !G->start) { free(count); return ORC_ERR_MEM; }
     const int nx = G->n[0], ny = G->n[1], nz = G->n[2];
+    if (p->policy != ORC_POLICY_CORNERS && !standalone) {
+        /* The reference's own membership (CustomGBIntegral.cpp:114-157): for each atom in
+         * index order, sample points walked from x - pad - g' to x + pad + g' (g' = max L/n)
+         * in steps of L/n per axis with floating accumulation; each sample is wrapped into the
+         * box for its cell (:135-140) and kept when the SAMPLE lies within pad of the atom
+         * (:143-145), appended while the cell holds fewer than cap atoms (:149-154).  Repeats of
+         * an atom in a cell are kept, as the reference keeps them.  Cell indices are clamped to
+         * [0, n-1] (a wrapped coordinate that rounds to the upper box face would index past the
+         * grid in the reference). */
+        for (int pass = 0; pass < 2; ++pass) {
+            if (pass == 1) {
+                long long s = 0;
+                for (long long c = 0; c < ncell; ++c) { G->start[c] = (int)s; s += count[c]; }
+                G->start[ncell] = (int)s;
+                G->atoms = malloc(sizeof(int) * (size_t)(s > 0 ? s : 1));
+                if (!G->atoms) { free(count); return ORC_ERR_MEM; }
+                memset(count, 0, sizeof(int) * (size_t)ncell);
+            }
+            for (int a = 0; a < N; ++a) {
+                const double* co = pos + 3 * a;
+                const double pad = R[a] + sqrt(3.0) / 2.0 * maxstep + p->buffer;
+                double bc[3], ec[3];
+                for (int d = 0; d < 3; ++d) {
+                    bc[d] = co[d] - pad - maxstep;
+                    ec[d] = co[d] + pad + maxstep;
+                }
+                for (double x = bc[0]; x < ec[0]; x += G->step[0])
+                    for (double y = bc[1]; y < ec[1]; y += G->step[1])
+                        for (double z = bc[2]; z < ec[2]; z += G->step[2]) {
+                            const double smp[3] = {x, y, z};
+                            int idx[3];
+                            for (int d = 0; d < 3; ++d) {
+                                const double w = smp[d] - G->L[d] * floor((smp[d] - G->min[d]) / G->L[d]);
+                                idx[d] = (int)floor((w - G->min[d]) / G->step[d]);
+                                if (idx[d] < 0) idx[d] = 0;
+                                if (idx[d] > G->n[d] - 1) idx[d] = G->n[d] - 1;
+                            }
+                            const double dx = x - co[0], dy = y - co[1], dz = z - co[2];
+                            if (!(dx * dx + dy * dy + dz * dz < pad * pad)) continue;
+                            const long long c = ((long long)idx[0] * ny + idx[1]) * nz + idx[2];
+                            if (count[c] < cap) {
+                                if (pass == 1) G->atoms[G->start[c] + count[c]] = a;
+                                count[c]++;
+                            }
+                        }
+            }
+        }
+        free(count);
+        return 0;
+    }
     for (int pass = 0; pass < 2; ++pass) {
         if (pass == 1) {
             long long s = 0;
@@ -1096,7 +1146,7 @@ int orc_grid_inspect(const orc_params* p, int N, const double* pos, const double
                      int* counts, int* lists, long long counts_cap, long long lists_cap) {
     grid_t G;
     orc_params pp = *p;
-    pp.policy = ORC_POLICY_GRID;
+    if (pp.policy == ORC_POLICY_BRUTE) pp.policy = ORC_POLICY_GRID;
     int rc = grid_build(&G, &pp, N, pos, radius);
     if (rc) { grid_free(&G); return rc; }
     for (int d = 0; d < 3; ++d) {

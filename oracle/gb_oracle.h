@@ -20,8 +20,11 @@ enum { ORC_MODEL_GBSW_STANDALONE = 0, ORC_MODEL_GBMV_FORM = 1 };
 /* CharmmGBMVForce::GBIntegralType (CharmmGBMVForce.h:99-104) */
 enum { ORC_INT_GBSW = 0, ORC_INT_GBMV1 = 1, ORC_INT_GBMV2 = 2 };
 /* neighbour policy */
-enum { ORC_POLICY_GRID = 0,   /* the reference lookup grid of the model */
-       ORC_POLICY_BRUTE = 1 };/* every atom, ascending index (reference computeVolume paths) */
+enum { ORC_POLICY_GRID = 0,    /* the reference lookup grid of the model */
+       ORC_POLICY_BRUTE = 1,   /* every atom, ascending index (reference computeVolume paths) */
+       ORC_POLICY_CORNERS = 2 };/* periodic GBMV form: the non-periodic corner rule on the
+                                 * periodic lattice instead of the reference's sample lattice
+                                 * (only to measure the difference); otherwise GRID */
 
 typedef struct {
     int model;          /* ORC_MODEL_* */
@@ -44,7 +47,7 @@ typedef struct {
     double sw, rscale, roffset;
     /* GBMV-I / GBMV-II constants */
     double gamma0, beta, lambda, P1, P2, S0;
-    double vguard;      /* GBMV-II: |v|^2 below this is treated as S = +inf (V = 1) */
+    double vguard;      /* GBMV-II: |v|^2 below this is treated as S = +inf (V = 1 for beta < 0) */
     /* Born-radius map B = slope / (a0 (1/eta - I4) + a1 (1/(4 eta^4) - I7)^(1/4)) + shift */
     double alpha0, alpha1, eta, slope, shift;
     /* energy prefactor P: E = sum_i P q_i^2 / (2 B_i) + sum_{i<j} P q_i q_j / f_ij */

@@ -1,10 +1,16 @@
-"""Lookup-grid semantics of the oracle against a literal numpy restatement.
+"""Lookup-grid semantics of the oracle.
 
 CustomGBIntegral.cpp:162-228 (GBMV-form build: grid pad 0.03 + sqrt3/2 g + buffer,
 per-atom pad R_j + sqrt3/2 g + buffer tested at each cell's lower corner, atoms in
 ascending index, first `cap` kept) and :310-324 (point -> floor((p - min)/g), outside ->
-empty); CharmmReferenceGBSW.cpp:440-552 (standalone: one pad, sqrt comparison, no cap).
-Cells outside the grid are never written (the reference indexes them out of bounds).
+empty); :114-157 (periodic GBMV form: an atom-anchored sample lattice); CharmmReferenceGBSW
+.cpp:440-552 (standalone: one pad, sqrt comparison, no cap).  Cells outside the grid are
+never written (the reference indexes them out of bounds).
+
+Two kinds of checks: a literal numpy restatement of the build (the first group), and checks
+fixed by the geometry alone (the second): member cells derived by hand for one, two and three
+atoms, a brute force over every (cell, atom) pair, invariance under shifts by whole grid steps,
+and the distances within which a point's list must, and beyond which it must not, hold an atom.
 """
 import numpy as np
 import pytest
@@ -107,3 +113,148 @@ def test_standalone_lists_cover_every_switching_atom():
         need = set(np.nonzero(np.linalg.norm(s.pos - pt, axis=1) < Rp + 0.03)[0])
         have = set() if c < 0 else set(lists[starts[c]:starts[c + 1]])
         assert need <= have
+
+
+# ---------------------------------------------------------------------------------------
+# Checks the geometry fixes (not a second transcription of the build loop)
+# ---------------------------------------------------------------------------------------
+
+def _lists(params, pos, R):
+    geom, counts, lists = O.grid_inspect(params, np.asarray(pos, float), np.asarray(R, float))
+    starts = np.concatenate([[0], np.cumsum(counts)])
+    return geom, {c: list(lists[starts[c]:starts[c + 1]]) for c in range(len(counts)) if counts[c]}
+
+
+def test_single_atom_member_cells_by_hand():
+    """One atom, R = 0.1, g = 0.5, buffer 0: the grid pads the bounding box by
+    0.03 + (sqrt3/2) 0.5 = 0.463 per side, so n = ceil(0.926/0.5) + 1 = 3 per axis and the
+    corners sit at -0.463, +0.037, +0.537 from the atom along each axis.  The atom's pad is
+    0.1 + 0.433 = 0.533 (pad^2 = 0.2843); a corner is a member when the sum of its squared
+    offsets (0.2144, 0.0014, 0.2884) is below that: not with any +0.537 (0.288 alone), not
+    with two -0.463 (0.429), yes otherwise.  Members: (1,1,1), (0,1,1), (1,0,1), (1,1,0)."""
+    p = O.gbmv_params(O.INT_GBMV2, grid_length=0.5, buffer=0.0, cap=0)
+    geom, cells = _lists(p, [[0.0, 0.0, 0.0]], [0.1])
+    assert list(geom[7:10].astype(int)) == [3, 3, 3]
+    flat = {(i * 3 + j) * 3 + k for i, j, k in [(1, 1, 1), (0, 1, 1), (1, 0, 1), (1, 1, 0)]}
+    assert cells == {c: [0] for c in flat}
+
+
+def test_two_atoms_member_cells_by_hand():
+    """Atoms at x = 0 and x = 1 (R = 0.1, g = 0.5, buffer 0): n = (ceil(1.926/0.5) + 1, 3, 3)
+    = (5, 3, 3); atom 1 sits two cells further along x, so its member cells are atom 0's
+    shifted by i + 2: atom 0 (i,j,k) in {(1,1,1), (0,1,1), (1,0,1), (1,1,0)}, atom 1 in
+    {(3,1,1), (2,1,1), (3,0,1), (3,1,0)}; no cell holds both."""
+    p = O.gbmv_params(O.INT_GBMV2, grid_length=0.5, buffer=0.0, cap=0)
+    geom, cells = _lists(p, [[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]], [0.1, 0.1])
+    assert list(geom[7:10].astype(int)) == [5, 3, 3]
+
+    def f(i, j, k):
+        return (i * 3 + j) * 3 + k
+
+    want = {f(1, 1, 1): [0], f(0, 1, 1): [0], f(1, 0, 1): [0], f(1, 1, 0): [0],
+            f(3, 1, 1): [1], f(2, 1, 1): [1], f(3, 0, 1): [1], f(3, 1, 0): [1]}
+    assert cells == want
+
+
+def test_cap_keeps_first_atoms_in_index_order_by_hand():
+    """Three atoms at one point with cap 2: every member cell lists atoms 0 and 1 only
+    (CustomGBIntegral.cpp:219-222 appends in atom order and stops at the cap)."""
+    p = O.gbmv_params(O.INT_GBMV2, grid_length=0.5, buffer=0.0, cap=2)
+    _, cells = _lists(p, [[0.0, 0.0, 0.0]] * 3, [0.1, 0.1, 0.1])
+    assert len(cells) == 4 and all(v == [0, 1] for v in cells.values())
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_grid_equals_brute_force_over_every_cell(seed):
+    """Every (cell, atom) pair of the whole grid tested (not only the index window around each
+    atom), with the grid geometry taken from CustomGBIntegral.cpp:173-187: a window that cut
+    off members, or an origin off by a cell, would show."""
+    s = S.globule(70, 10 + seed)
+    p = O.gbmv_params(O.INT_GBMV2, cap=0)
+    g, buf = p.grid_length, p.buffer
+    geom, cells = _lists(p, s.pos, s.radius)
+    gpad = 0.03 + np.sqrt(3.0) / 2.0 * g + buf
+    lo = s.pos.min(0) - gpad
+    n = np.ceil((s.pos.max(0) + gpad - lo) / g).astype(int) + 1
+    assert np.array_equal(geom[7:10].astype(int), n) and np.array_equal(geom[0:3], lo)
+    ii, jj, kk = np.meshgrid(*[np.arange(m) for m in n], indexing="ij")
+    corners = lo + np.stack([ii.ravel(), jj.ravel(), kk.ravel()], 1) * g
+    pad = s.radius + np.sqrt(3.0) / 2.0 * g + buf
+    d2 = ((corners[:, None, :] - s.pos[None, :, :]) ** 2).sum(-1)
+    member = d2 < pad[None, :] ** 2
+    want = {c: list(np.flatnonzero(member[c])) for c in range(len(corners)) if member[c].any()}
+    assert cells == want
+
+
+def test_lists_unchanged_by_shifts_of_whole_grid_steps():
+    """Moving every atom by an integer number of grid steps moves the grid with them
+    (its origin is the bounding box minus a fixed pad), so every cell keeps its list."""
+    s = S.globule(80, 12)
+    p = O.gbmv_params(O.INT_GBMV1, cap=0)
+    _, base = _lists(p, s.pos, s.radius)
+    for shift in ([1, 0, 0], [7, -3, 2], [-5, 11, -8]):
+        _, moved = _lists(p, s.pos + np.array(shift) * p.grid_length, s.radius)
+        assert moved == base, shift
+
+
+def test_lists_hold_every_atom_near_a_point_and_no_far_one():
+    """A point lies within sqrt3 g of its cell's lower corner, so (uncapped) its list must hold
+    every atom j with |x_j - p| < R_j + buffer - (sqrt3/2) g and no atom with
+    |x_j - p| >= R_j + buffer + (3 sqrt3/2) g."""
+    s = S.globule(150, 13)
+    p = O.gbmv_params(O.INT_GBMV2, cap=0)
+    g, buf = p.grid_length, p.buffer
+    geom, counts, lists = O.grid_inspect(p, s.pos, s.radius)
+    starts = np.concatenate([[0], np.cumsum(counts)])
+    rng = np.random.default_rng(14)
+    inner = s.radius + buf - np.sqrt(3.0) / 2.0 * g
+    outer = s.radius + buf + 1.5 * np.sqrt(3.0) * g
+    for pt in s.pos[rng.integers(0, s.n, 1500)] + rng.normal(scale=0.3, size=(1500, 3)):
+        c = O.grid_query(geom, pt)
+        have = set() if c < 0 else set(lists[starts[c]:starts[c + 1]].tolist())
+        d = np.linalg.norm(s.pos - pt, axis=1)
+        assert set(np.flatnonzero(d < inner - 1e-9).tolist()) <= have
+        assert not (have & set(np.flatnonzero(d >= outer + 1e-9).tolist()))
+
+
+def test_periodic_sample_lattice_covers_its_reach():
+    """The periodic CharmmGBMVForce grid (the reference's sample walk, CustomGBIntegral.cpp:
+    114-157) puts an atom in a cell when a sample of its lattice, spaced L/n per axis, lies in
+    the cell within pad of the atom; every cell of edges L/n holds one sample of the walk.
+    So (uncapped) a cell whose nearest point is within pad - sqrt3 step of the atom's nearest
+    image lists it, and a cell whose nearest point is beyond pad does not."""
+    L = np.array([3.1, 2.9, 3.3])
+    s = S.liquid(L, 15, density=30.0)
+    p = O.gbmv_params(O.INT_GBMV2, cap=0, method=O.CUTOFF_PERIODIC, cutoff=1.2, box=list(L))
+    geom, counts, lists = O.grid_inspect(p, s.pos, s.radius)
+    n = geom[7:10].astype(int)
+    step = L / n
+    lo = geom[0:3]
+    pad = s.radius + np.sqrt(3.0) / 2.0 * step.max() + p.buffer
+    starts = np.concatenate([[0], np.cumsum(counts)])
+    ii, jj, kk = np.meshgrid(*[np.arange(m) for m in n], indexing="ij")
+    cidx = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], 1)
+    for a in range(0, s.n, 7):
+        x = s.pos[a]
+        # nearest point of each cell box to the atom's nearest image
+        cen = lo + (cidx + 0.5) * step
+        dc = cen - x
+        dc -= L * np.round(dc / L)
+        gap = np.maximum(np.abs(dc) - step / 2, 0.0)
+        dist = np.linalg.norm(gap, axis=1)
+        inside = set(np.flatnonzero(dist < pad[a] - np.sqrt(3.0) * step.max() - 1e-9).tolist())
+        outside = set(np.flatnonzero(dist > pad[a] + 1e-9).tolist())
+        holds = {c for c in range(len(cidx)) if a in lists[starts[c]:starts[c + 1]]}
+        assert inside <= holds, a
+        assert not (holds & outside), a
+
+
+def test_periodic_sample_lattice_differs_from_corner_rule():
+    """The two periodic membership rules give different lists (so the choice matters); the
+    measured effect on energies and forces is recorded in the design notes."""
+    L = [2.9, 3.0, 3.1]
+    s = S.liquid(L, 16, density=40.0)
+    kw = dict(cap=0, method=O.CUTOFF_PERIODIC, cutoff=1.2, box=L)
+    _, a = _lists(O.gbmv_params(O.INT_GBMV2, **kw), s.pos, s.radius)
+    _, b = _lists(O.gbmv_params(O.INT_GBMV2, policy=O.POLICY_CORNERS, **kw), s.pos, s.radius)
+    assert a != b
