@@ -307,6 +307,18 @@ gbi_status gbi_get_born_radii(gbi_ctx* ctx, double* born);
 gbi_status gbi_get_integrals(gbi_ctx* ctx, double* integrals); /* 2 x N (integral-major) */
 gbi_status gbi_get_stats(gbi_ctx* ctx, gbi_stats* stats);
 gbi_status gbi_get_device_view(gbi_ctx* ctx, gbi_device_view* view);
+
+/*
+ * The buffers of an OpenMM GPU platform, on the device and asynchronous on the context's
+ * stream, in the platform's atom order: atom_slot (device, N int32, NULL for the identity)
+ * gives the slot of each atom.  gbi_set_positions_posq reads positions from posq, one
+ * (x, y, z, q) per slot as float4 (posq_double = 0) or double4 (1); gbi_add_forces_fixed
+ * ADDS the last evaluation's forces, int64 fixed point with scale 2^32, to force[d *
+ * padded_n + slot] for axis d (the layout into which the CUDA platform's integral kernel
+ * adds its forces, reduceIntegralForce.cu:61-77, and its N^2 kernel, customGBEnergyN2.cu).
+ */
+gbi_status gbi_set_positions_posq(gbi_ctx* ctx, const void* posq, int32_t posq_double, const int32_t* atom_slot);
+gbi_status gbi_add_forces_fixed(gbi_ctx* ctx, int64_t* force, int32_t padded_n, const int32_t* atom_slot);
 gbi_status gbi_set_stream(gbi_ctx* ctx, void* hip_stream); /* NULL: the context's own stream */
 
 /*

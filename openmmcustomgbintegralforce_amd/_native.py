@@ -38,6 +38,7 @@ EXPORTS = [
     "gbi_profile_enable", "gbi_profile_read", "gbi_profile_pass_name", "gbi_set_periodic_box",
     "gbi_set_exclusions", "gbi_create_custom", "gbi_set_custom_globals", "gbi_get_custom_values",
     "gbi_get_custom_param_derivs", "gbi_update_custom_parameters", "gbi_update_constants", "gbi_execute_noalloc",
+    "gbi_set_positions_posq", "gbi_add_forces_fixed",
 ]
 PROFILE_PASSES = ["grid", "bin", "volume", "integrals", "pairs", "chain", "force", "owner"]
 
@@ -189,6 +190,8 @@ def load() -> ctypes.CDLL:
     L.gbi_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     L.gbi_get_device_view.argtypes = [vp, ctypes.POINTER(DeviceView)]
     L.gbi_set_stream.argtypes = [vp, vp]
+    L.gbi_set_positions_posq.argtypes = [vp, vp, ctypes.c_int32, vp]
+    L.gbi_add_forces_fixed.argtypes = [vp, vp, ctypes.c_int32, vp]
     L.gbi_destroy.argtypes = [vp]
     L.gbi_destroy.restype = None
     L.gbi_last_error.restype = ctypes.c_char_p

@@ -60,8 +60,10 @@ def build_openmm_plugin(verbose: bool = False) -> str:
     subprocess.run(cmd, check=True)
     os.replace(tmp, PLUGIN_LIB)
     if not (omm and api):
-        cmd = common + ["-o", DRIVER + ".tmp", os.path.join(PLUGIN, "driver", "gbi_openmm_driver.cpp"), "-L", LIBDIR,
-                        "-lgbi_openmm", "-lgbi", "-Wl,-rpath,$ORIGIN"]
+        # the driver also plays a GPU platform's device buffers, so it uses the HIP runtime
+        cmd = common + ["-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include", "-o", DRIVER + ".tmp",
+                        os.path.join(PLUGIN, "driver", "gbi_openmm_driver.cpp"), "-L", LIBDIR, "-lgbi_openmm", "-lgbi",
+                        "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

@@ -1365,6 +1365,22 @@ void gbi_destroy(gbi_ctx* ctx) {
     delete ctx;
 }
 
+gbi_status gbi_set_positions_posq(gbi_ctx* ctx, const void* posq, int32_t posq_double, const int32_t* atom_slot) {
+    if (!ctx || !posq) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(gbi::launch_gather_posq(ctx->pos.p, posq, posq_double ? 1 : 0, atom_slot, ctx->N, ctx->stream));
+    ctx->have_positions = true;
+    return GBI_OK;
+}
+
+gbi_status gbi_add_forces_fixed(gbi_ctx* ctx, int64_t* force, int32_t padded_n, const int32_t* atom_slot) {
+    if (!ctx || !force) return fail(GBI_ERR_INVALID_ARGUMENT, "null argument");
+    if (padded_n < ctx->N) return fail(GBI_ERR_INVALID_ARGUMENT, "padded_n is below the atom count");
+    if (!ctx->computed) return fail(GBI_ERR_INVALID_ARGUMENT, "nothing computed yet");
+    HIP_TRY(gbi::launch_add_forces_fixed(reinterpret_cast<long long*>(force), padded_n, ctx->force_fix.p, atom_slot,
+                                         ctx->N, ctx->stream));
+    return GBI_OK;
+}
+
 gbi_status gbi_set_stream(gbi_ctx* ctx, void* hip_stream) {
     if (!ctx) return fail(GBI_ERR_INVALID_ARGUMENT, "null context");
     HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -237,6 +237,12 @@ constexpr int kPasses = 8;
 hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stream, const hipEvent_t* marks);
 hipError_t launch_stats(const KParams& p, double* out6, hipStream_t stream);
 
+// OpenMM GPU-platform interop (gbi_set_positions_posq, gbi_add_forces_fixed).
+hipError_t launch_gather_posq(double* pos, const void* posq, int posq_double, const int* atom_slot, int n,
+                              hipStream_t stream);
+hipError_t launch_add_forces_fixed(long long* dst, int padded_n, const long long* force_fix, const int* atom_slot,
+                                   int n, hipStream_t stream);
+
 // gbi_execute_noalloc: bytes of the static device scratch array (per GPU), and its device
 // address on the current device (gbi_noalloc.hip).
 constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;

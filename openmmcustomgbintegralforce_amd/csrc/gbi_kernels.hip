@@ -2070,6 +2070,51 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
     return hipGetLastError();
 }
 
+namespace {
+
+// Positions from an OpenMM posq buffer (float4 or double4 per slot) in the platform's order.
+__global__ void k_gather_posq(double* pos, const void* posq, int posq_double, const int* atom_slot, int n) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const int slot = atom_slot ? atom_slot[a] : a;
+    if (posq_double) {
+        const double4 v = static_cast<const double4*>(posq)[slot];
+        pos[3 * a] = v.x;
+        pos[3 * a + 1] = v.y;
+        pos[3 * a + 2] = v.z;
+    } else {
+        const float4 v = static_cast<const float4*>(posq)[slot];
+        pos[3 * a] = (double)v.x;
+        pos[3 * a + 1] = (double)v.y;
+        pos[3 * a + 2] = (double)v.z;
+    }
+}
+
+// Forces added to an OpenMM fixed-point force buffer (axis-major, padded_n per axis); one
+// thread per atom, so each slot has one writer.
+__global__ void k_add_forces_fixed(long long* dst, int padded_n, const long long* force_fix, const int* atom_slot,
+                                   int n) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const int slot = atom_slot ? atom_slot[a] : a;
+    for (int d = 0; d < 3; ++d) dst[(long long)d * padded_n + slot] += force_fix[(long long)d * n + a];
+}
+
+}  // namespace
+
+hipError_t launch_gather_posq(double* pos, const void* posq, int posq_double, const int* atom_slot, int n,
+                              hipStream_t stream) {
+    hipLaunchKernelGGL(k_gather_posq, dim3((n + 255) / 256), dim3(256), 0, stream, pos, posq, posq_double, atom_slot, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_forces_fixed(long long* dst, int padded_n, const long long* force_fix, const int* atom_slot,
+                                   int n, hipStream_t stream) {
+    hipLaunchKernelGGL(k_add_forces_fixed, dim3((n + 255) / 256), dim3(256), 0, stream, dst, padded_n, force_fix,
+                       atom_slot, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_stats(const KParams& p, double* out6, hipStream_t stream) {
     hipLaunchKernelGGL(k_stats, dim3(256), dim3(256), 0, stream, p, out6);
     return hipGetLastError();

@@ -22,6 +22,8 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_runtime.h>
+
 #include "HipCharmmKernels.h"
 #include "openmm/Platform.h"
 
@@ -105,6 +107,51 @@ void setDefaults(ContextImpl& context, const CharmmGBMVForce& force) {
         context.setParameter(force.getGlobalParameterName(k), force.getGlobalParameterDefaultValue(k));
 }
 
+// The same evaluation with the Context's state on the device, as a GPU platform keeps it:
+// posq (double4) and an int64 fixed-point force buffer, padded, atoms in reversed slots.
+// Prints whether energy and forces equal the host path's bitwise.
+template <class Exec>
+void deviceRun(FILE* out, ContextImpl& context, const std::vector<Vec3>& pos, const std::vector<double>& q,
+               const std::vector<Vec3>& host_forces, double host_energy, Exec exec) {
+    const int n = (int)pos.size(), padded = (n + 31) / 32 * 32;
+    std::vector<int> slot(n);
+    std::vector<double> posq(4 * (size_t)padded, 0.0);
+    for (int a = 0; a < n; ++a) {
+        slot[a] = n - 1 - a;
+        for (int d = 0; d < 3; ++d) posq[4 * (size_t)slot[a] + d] = pos[a][d];
+        posq[4 * (size_t)slot[a] + 3] = q[a];
+    }
+    void *d_posq = nullptr, *d_force = nullptr, *d_slot = nullptr;
+    auto ok = [](hipError_t e) {
+        if (e != hipSuccess) throw std::runtime_error(std::string("HIP: ") + hipGetErrorString(e));
+    };
+    ok(hipMalloc(&d_posq, sizeof(double) * posq.size()));
+    ok(hipMalloc(&d_force, sizeof(long long) * 3 * padded));
+    ok(hipMalloc(&d_slot, sizeof(int) * n));
+    ok(hipMemcpy(d_posq, posq.data(), sizeof(double) * posq.size(), hipMemcpyHostToDevice));
+    ok(hipMemset(d_force, 0, sizeof(long long) * 3 * padded));
+    ok(hipMemcpy(d_slot, slot.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    DeviceBuffers& db = context.deviceBuffers();
+    db.posq = d_posq;
+    db.posq_double = 1;
+    db.force = static_cast<long long*>(d_force);
+    db.padded_n = padded;
+    db.atom_slot = static_cast<const int*>(d_slot);
+    const double e = exec();
+    ok(hipDeviceSynchronize());
+    std::vector<long long> f(3 * (size_t)padded);
+    ok(hipMemcpy(f.data(), d_force, sizeof(long long) * f.size(), hipMemcpyDeviceToHost));
+    int equal = 1;
+    for (int a = 0; a < n; ++a)
+        for (int d = 0; d < 3; ++d)
+            equal &= (double)f[(size_t)d * padded + slot[a]] / 4294967296.0 == host_forces[a][d];
+    fprintf(out, "device_energy_equal %d\ndevice_forces_equal %d\n", (int)(e == host_energy), equal);
+    db = DeviceBuffers();
+    ok(hipFree(d_posq));
+    ok(hipFree(d_force));
+    ok(hipFree(d_slot));
+}
+
 // Which forces the kernel class takes as canonical (host only).
 int canonicalCheck() {
     const std::vector<double> q = {0.5, -0.5}, r = {0.2, 0.17};
@@ -186,6 +233,7 @@ int main(int argc, char** argv) {
             std::vector<Vec3> f = context.getForces();
             fprintf(out, "energy %.17g\n", e);
             for (int i = 0; i < n; ++i) fprintf(out, "%.17g %.17g %.17g\n", f[i][0], f[i][1], f[i][2]);
+            deviceRun(out, context, pos, q, f, e, [&] { return kernel.execute(context, true, true); });
             e_noforce = kernel.execute(context, false, true);
             for (int i = 0; i < n; ++i) force.setParticleParameters(i, qs[i], r[i], 1.0);
             kernel.copyParametersToContext(context, force);
@@ -215,6 +263,7 @@ int main(int argc, char** argv) {
             std::vector<Vec3> f = context.getForces();
             fprintf(out, "energy %.17g\n", e);
             for (int i = 0; i < n; ++i) fprintf(out, "%.17g %.17g %.17g\n", f[i][0], f[i][1], f[i][2]);
+            if (model == 1) deviceRun(out, context, pos, q, f, e, [&] { return kernel.execute(context, true, true); });
             if (model == 3) {
                 // one execute adds dE/dk and dE/dP once; the second (with k = 0.45) adds again
                 auto& d = context.energyParameterDerivatives();

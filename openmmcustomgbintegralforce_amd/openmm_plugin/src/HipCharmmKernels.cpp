@@ -64,7 +64,46 @@ void HipGBKernelBase::update(const std::vector<double>& charge, const std::vecto
     check(gbi_update_parameters(ctx_, (int32_t)charge.size(), charge.data(), radius.data()));
 }
 
+bool HipGBKernelBase::platformBuffers(ContextImpl& context, PlatformBuffers& out) {
+#ifdef GBI_OPENMM_STAND_IN
+    const DeviceBuffers& d = context.deviceBuffers();
+    if (!d.posq) return false;
+    out.posq = d.posq;
+    out.posq_double = d.posq_double;
+    out.force = d.force;
+    out.padded_n = d.padded_n;
+    out.atom_slot = d.atom_slot;
+    out.stream = d.stream;
+    return true;
+#else
+    (void)context;
+    (void)out;
+    return false;
+#endif
+}
+
 double HipGBKernelBase::run(ContextImpl& context, bool includeForces, bool includeEnergy) {
+    PlatformBuffers pb;
+    if (platformBuffers(context, pb)) {
+        if (pb.stream != stream_) {
+            check(gbi_set_stream(ctx_, pb.stream));
+            stream_ = pb.stream;
+        }
+        if (periodic_) {
+            Vec3 a, b, c;
+            context.getPeriodicBoxVectors(a, b, c);
+            const double v[9] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2]};
+            check(gbi_set_periodic_box(ctx_, 0, v));
+        }
+        check(gbi_set_positions_posq(ctx_, pb.posq, pb.posq_double, pb.atom_slot));
+        check(gbi_compute(ctx_, GBI_FORCES | GBI_ENERGY));
+        check(gbi_synchronize(ctx_));  // device flags: errors, or buffer growth and a redo
+        if (includeForces) check(gbi_add_forces_fixed(ctx_, reinterpret_cast<int64_t*>(pb.force), pb.padded_n,
+                                                      pb.atom_slot));
+        double energy = 0.0;
+        if (includeEnergy) check(gbi_get_energy(ctx_, &energy));
+        return energy;
+    }
     std::vector<Vec3> pos;
     context.getPositions(pos);
     if ((int)pos.size() != n_) throw OpenMMException("HIP GB kernel: the Context's particle count changed");

@@ -22,6 +22,21 @@ public:
     gbi_ctx* gbiContext() const { return ctx_; }
 
 protected:
+    // Device buffers of a GPU platform: positions are gathered from its posq and forces added
+    // to its fixed-point buffer on the device (gbi_set_positions_posq, gbi_add_forces_fixed),
+    // with no host round trip, as the CUDA platform's kernels read posq and add into its force
+    // buffer (reduceIntegralForce.cu:61-77).
+    struct PlatformBuffers {
+        const void* posq = nullptr;
+        int posq_double = 0;
+        long long* force = nullptr;
+        int padded_n = 0;
+        const int* atom_slot = nullptr;
+        void* stream = nullptr;
+    };
+    // Whether the Context keeps its state on the device, and its buffers.  The stand-in
+    // ContextImpl reports its DeviceBuffers; a HIP OpenMM platform overrides this with its own.
+    virtual bool platformBuffers(OpenMM::ContextImpl& context, PlatformBuffers& out);
     double run(OpenMM::ContextImpl& context, bool includeForces, bool includeEnergy);
     // Adds N x 3 forces (kJ/mol/nm) to the platform's force accumulator.  The OpenMM-shaped
     // stand-in ContextImpl keeps a host array; a HIP OpenMM platform overrides this and adds
@@ -39,6 +54,7 @@ protected:
     gbi_ctx* ctx_ = nullptr;
     int n_ = 0;
     bool periodic_ = false;  // CutoffPeriodic: the Context's box goes to the engine every evaluation
+    void* stream_ = nullptr; // the platform stream the engine works on
 };
 
 // CalcCharmmGBSWForceKernel on HIP (CharmmReferenceKernels.cpp:380-445 is the Reference one).

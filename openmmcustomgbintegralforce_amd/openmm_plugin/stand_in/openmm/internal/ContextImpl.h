@@ -13,6 +13,17 @@
 
 namespace OpenMM {
 
+// The device buffers of a GPU platform's Context (OpenMM's CUDA and HIP platforms keep posq
+// and an int64 fixed-point force buffer on the device, atoms in the platform's own order).
+struct DeviceBuffers {
+    const void* posq = nullptr;   // (x, y, z, q) per slot, float4 or double4
+    int posq_double = 0;
+    long long* force = nullptr;   // force[d * padded_n + slot], scale 2^32
+    int padded_n = 0;
+    const int* atom_slot = nullptr;  // slot of each atom (device), nullptr: identity
+    void* stream = nullptr;       // the platform's stream (hipStream_t), nullptr: the null stream
+};
+
 class ContextImpl {
 public:
     explicit ContextImpl(int n_particles) : positions_(n_particles), forces_(n_particles) {}
@@ -38,9 +49,12 @@ public:
     void setParameter(const std::string& name, double value) { parameters_[name] = value; }
     std::map<std::string, double>& energyParameterDerivatives() { return param_derivs_; }
     std::vector<Vec3>& getForces() { return forces_; }
+    // set when the Context's state lives on the device (a GPU platform), empty otherwise
+    DeviceBuffers& deviceBuffers() { return device_; }
     void systemChanged() {}
 
 private:
+    DeviceBuffers device_;
     std::vector<Vec3> positions_, forces_;
     std::map<std::string, double> parameters_, param_derivs_;
     Vec3 box_[3] = {Vec3(2.0, 0.0, 0.0), Vec3(0.0, 2.0, 0.0), Vec3(0.0, 0.0, 2.0)};

@@ -49,6 +49,9 @@ def test_cpp_kernel_classes_match_oracle(tmp_path, case):
     assert abs(float(rest["energy_scaled"]) - 2.25 * e) < 1e-6 * abs(e)
     assert rest["illegal_name_throws"] == "1"
     assert rest["count_change_throws"] == "1"
+    # the same evaluation with the Context's state on the device (a GPU platform's posq and
+    # fixed-point force buffer, atoms in reversed slots): bitwise the host path's result
+    assert rest["device_energy_equal"] == "1" and rest["device_forces_equal"] == "1"
     if case == "gbmv2":
         # Context::setParameter("alpha0", -0.2) reaches the next execute (CharmmReferenceKernels.cpp:355-357)
         pa = N.Params.from_buffer_copy(p)
