@@ -101,6 +101,22 @@ struct SysHost {
     bool cutoff() const { return P.nonbonded_method != GBI_NO_CUTOFF; }
 };
 
+// Binning bricks: sk x sk x sk cells of about 0.45 nm reserve their point slots together (an
+// atom's 1,200 GBMV-II points fall into about 200 such bricks against 670 cells of 0.15 nm;
+// 0.17 ms for binning 5,000 atoms against 0.21 by cells).  Grids of cells above 0.18 nm bin by
+// cells (sk = 1): on the 0.20 nm exact GBSW grid, bricks of 2 or 3 cells per edge were slower
+// (0.49-0.52 ms against 0.44 for 25,000 atoms), their second pass costing more than the
+// atomics they save.
+constexpr double kBrickEdgeNm = 0.45;
+int brick_edge(const SysHost& s) {
+    if (s.kind == gbi::GRID_BRUTE) return 1;
+    const int sk = (int)std::lround(kBrickEdgeNm / s.g);
+    return sk >= 3 ? std::min(gbi::kMaxBrickEdge, sk) : 1;
+}
+
+// Cells per axis rounded up to whole bricks.
+long long brick_cells(long long n, int sk) { return (n + sk - 1) / sk * sk; }
+
 }  // namespace
 
 struct gbi_ctx {
@@ -150,6 +166,10 @@ struct gbi_ctx {
     DevBuf<int> cell_fill, cell_start, cell_count_eff, cell_atoms_tmp, cell_atoms, cell_pt_count,
         cell_act_count, frec_base, worklist, worklist_sys, work, scan_tmp;
     DevBuf<float4> ent_pos, ent_par, prec, frec0, frec1, frec2, vrec;
+    DevBuf<int> brick_count;
+    DevBuf<uint2> brick_rec;
+    int brick_total = 0, brick_max = 0;
+    bool any_bricks = false, any_cells = true;
     DevBuf<float> quad_r2, vsig, vval;
     DevBuf<long long> force_fix, dedr_fix, vacc_fix;
 
@@ -208,6 +228,8 @@ struct gbi_ctx {
         for (auto* b : ib) b->release();
         DevBuf<float4>* fb[] = {&ent_pos, &ent_par, &prec, &frec0, &frec1, &frec2, &vrec};
         for (auto* b : fb) b->release();
+        brick_count.release();
+        brick_rec.release();
         quad_r2.release();
         vsig.release();
         vval.release();
@@ -334,6 +356,7 @@ gbi::SysDesc make_desc(const gbi_ctx& c, const SysHost& s, long long cell_base, 
     d.tmp_base = tmp_base;
     d.tmp_cap = s.tmp_cap;
     d.prec_base = prec_base;
+    d.sk = brick_edge(s);
     d.cap = (P.cell_cap > 0 && s.kind != gbi::GRID_BRUTE) ? P.cell_cap : INT_MAX;
     d.pair_splits = s.pair_splits;
     d.pair_k = s.pair_k;
@@ -396,12 +419,21 @@ gbi::SysDesc make_desc(const gbi_ctx& c, const SysHost& s, long long cell_base, 
 // Cell, slot and worklist buffers for the systems' current capacities; uploads the
 // system descriptors and the cell -> system map.
 gbi_status layout(gbi_ctx& c) {
-    long long cells = 0, slots = 0, raw = 0;
+    long long cells = 0, slots = 0, raw = 0, bricks = 0, brick_max = 0, brick_slots = 0;
+    bool any_bricks = false, any_cells = false;
     for (const SysHost& s : c.sys) {
         cells += s.cell_cap;
         slots += s.cell_cap * (long long)s.pt_cap;
         raw += s.cell_cap * (long long)s.tmp_cap;
+        const int sk = brick_edge(s);
+        any_cells |= sk == 1;
+        if (sk == 1) continue;  // binned by cells
+        bricks += s.cell_cap / (sk * sk * sk);
+        brick_max = std::max(brick_max, s.cell_cap / (sk * sk * sk));
+        any_bricks = true;
     }
+    // brick slots share the prec slot layout (a system binning by cells leaves its range unused)
+    brick_slots = any_bricks ? slots : 1;
     if (cells > kMaxCells) return fail(GBI_ERR_GRID_TOO_LARGE, "more than 16M lookup cells in the batch");
     if (slots * (long long)sizeof(float4) > (200ll << 30))
         return fail(GBI_ERR_GRID_TOO_LARGE, "point slots of the lookup grids exceed 200 GB");
@@ -414,10 +446,16 @@ gbi_status layout(gbi_ctx& c) {
     HIP_TRY(c.cell_fill.ensure(cells));
     HIP_TRY(c.cell_start.ensure(cells + 1));
     HIP_TRY(c.cell_count_eff.ensure(cells));
-    // the binning pass's atomic counters get an allocation of their own: sub-allocated beside
+    // the binning pass's atomic counters get allocations of their own: sub-allocated beside
     // other small buffers, that pass was measured 15-20 % slower in some processes (0.26 vs
     // 0.215 ms at 5k atoms, 0.49 vs 0.445 ms at 25k), never with its own allocation
     HIP_TRY(c.cell_pt_count.ensure(cells, kOwnAllocation));
+    HIP_TRY(c.brick_count.ensure(std::max<long long>(1, bricks) * gbi::kBrickStride, kOwnAllocation));
+    HIP_TRY(c.brick_rec.ensure(brick_slots));
+    c.brick_total = (int)bricks;
+    c.brick_max = (int)brick_max;
+    c.any_bricks = any_bricks;
+    c.any_cells = any_cells;
     HIP_TRY(c.cell_act_count.ensure(cells));
     HIP_TRY(c.frec_base.ensure(cells + 1));
     HIP_TRY(c.worklist.ensure(gbi::VOL_KINDS * cells));
@@ -428,9 +466,11 @@ gbi_status layout(gbi_ctx& c) {
     HIP_TRY(c.cell_atoms_tmp.ensure(std::max<long long>(1, raw)));
     std::vector<int> cs(cells);
     std::vector<gbi::SysDesc> desc(c.S);
-    long long cb = 0, pb = 0, tb = 0;
+    long long cb = 0, pb = 0, tb = 0, bb = 0;
     for (int s = 0; s < c.S; ++s) {
         desc[s] = make_desc(c, c.sys[s], cb, pb, tb);
+        desc[s].brick_base = (int)bb;
+        if (desc[s].sk > 1) bb += c.sys[s].cell_cap / (desc[s].sk * desc[s].sk * desc[s].sk);
         std::fill(cs.begin() + cb, cs.begin() + cb + c.sys[s].cell_cap, s);
         cb += c.sys[s].cell_cap;
         pb += c.sys[s].cell_cap * (long long)c.sys[s].pt_cap;
@@ -533,9 +573,10 @@ void estimate_caps(const gbi_ctx& c, const SysHost& s, const double* pos, long l
     const double gridpad = s.kind == gbi::GRID_FAITHFUL ? s.gpad : s.max_reach;
     cells = 1;
     for (int d = 0; d < 3; ++d) {
-        // periodic: the box's cells (ceil(L/g) + 1 per axis), with room for a 5 % larger box
+        // periodic: the box's cells (ceil(L/g) + 1 per axis), with room for a 5 % larger box;
+        // whole bricks of cells
         const double len = s.periodic() ? s.box[d] * 1.05 : (mx[d] - mn[d]) * 1.1 + 2.0 * gridpad + 1.0;
-        cells *= (long long)std::ceil(len / g) + 2;
+        cells *= brick_cells((long long)std::ceil(len / g) + 2, brick_edge(s));
     }
     // entries: box of candidate cells per atom (an upper bound of the members)
     double side;
@@ -782,6 +823,12 @@ gbi::KParams make_kparams(gbi_ctx& c, bool adjoint) {
     k.worklist = c.worklist.p;
     k.worklist_sys = c.worklist_sys.p;
     k.prec = c.prec.p;
+    k.brick_count = c.brick_count.p;
+    k.brick_rec = c.brick_rec.p;
+    k.brick_total = c.brick_total;
+    k.brick_max = c.brick_max;
+    k.any_bricks = c.any_bricks ? 1 : 0;
+    k.any_cells = c.any_cells ? 1 : 0;
     k.frec0 = c.frec0.p;
     k.frec1 = c.frec1.p;
     k.frec2 = c.frec2.p;
@@ -1708,7 +1755,7 @@ gbi_status gbi_execute_noalloc(int32_t n_atoms, const double* charge, const doub
             clear(c.cell_act_count), clear(c.cell_sys), clear(c.cell_start), clear(c.frec_base), clear(c.worklist);
             clear(c.worklist_sys), clear(c.scan_tmp), clear(c.cell_atoms_tmp), clear(c.cell_atoms), clear(c.ent_pos);
             clear(c.ent_par), clear(c.prec), clear(c.frec0), clear(c.frec1), clear(c.frec2), clear(c.vval);
-            clear(c.vrec);
+            clear(c.vrec), clear(c.brick_count), clear(c.brick_rec);
         }
     } unborrow{c};
     const int32_t n = n_atoms;
@@ -1737,6 +1784,11 @@ gbi_status gbi_execute_noalloc(int32_t n_atoms, const double* charge, const doub
     c.cell_total = caps.cells;
     c.entry_cap = (int)caps.entries;
     c.prec_total = caps.cells * (long long)caps.pt_cap;
+    // binned by cells whatever the grid (bricks would need a second record array; the results
+    // do not depend on the binning)
+    c.brick_total = c.brick_max = 0;
+    c.any_bricks = false;
+    c.any_cells = true;
 
     // the buffers, carved once without a base (the size check, before the device is touched)
     // and once from the scratch array
@@ -1758,6 +1810,7 @@ gbi_status gbi_execute_noalloc(int32_t n_atoms, const double* charge, const doub
     sc.take(c.ent_pos, caps.entries), sc.take(c.ent_par, caps.entries), sc.take(c.prec, c.prec_total);
     sc.take(c.frec0, NQ), sc.take(c.frec1, NQ), sc.take(c.frec2, c.kinds[gbi::VOL_GBMV2] ? NQ : 1);
     sc.take(c.vval, NQ), sc.take(c.vrec, NQ);
+    sc.take(c.brick_count, gbi::kBrickStride), sc.take(c.brick_rec, 1);
     };
     ScratchCarver dry;
     carve(dry);
@@ -1783,7 +1836,8 @@ gbi_status gbi_execute_noalloc(int32_t n_atoms, const double* charge, const doub
     std::lock_guard<std::mutex> lock(mu);
     hipStream_t st = (hipStream_t)hip_stream;
     c.stream = st;
-    const gbi::SysDesc desc = make_desc(c, sh, 0, 0, 0);
+    gbi::SysDesc desc = make_desc(c, sh, 0, 0, 0);
+    desc.sk = 1;
     const double box[3] = {0.0, 0.0, 0.0};
     std::vector<int> cell_sys(cells, 0);
     auto h2d = [st](void* dst, const void* src, size_t bytes) {

@@ -39,6 +39,8 @@ constexpr int kMaxAtomChunk = 256;       // atoms staged in LDS per chunk (volum
 constexpr int kQBits = 11;               // packed point = (atom << 11) | q, Q <= 2047
 constexpr int kMaxAtoms = 1 << (32 - kQBits);
 constexpr float kQuantum = 16777216.0f;  // 2^24: chain-rule partials are summed exactly on this grid
+constexpr int kBrickStride = 32;         // ints between brick counters (one 128-byte line each)
+constexpr int kMaxBrickEdge = 4;         // cells per brick edge (<= 64 cells per brick)
 
 // Derivatives of the energy with respect to the constants of a system (adjoint).
 enum Theta : int {
@@ -79,6 +81,8 @@ struct SysDesc {
     int cell_base, cell_cap;   // global cells [cell_base, cell_base + cell_cap)
     int pt_cap;                // point slots per cell
     long long prec_base;       // slot s of local cell c: prec_base + c * pt_cap + s
+    int sk;                    // bricks of sk x sk x sk cells reserve point slots together (binning; 1: cells)
+    int brick_base;            // the system's first brick in KParams::brick_count
     long long tmp_base;        // raw member k of local cell c: cell_atoms_tmp[tmp_base + c * tmp_cap + k]
     int tmp_cap;               // raw member slots per cell (the system's size for the brute cell)
     int cap;                   // atoms kept per cell
@@ -144,7 +148,7 @@ struct SysGeom {
     int n[3];
     int ncell;
     int flags;
-    int pad_[3];
+    int nb[3];         // bricks per axis: ceil(n / sk)
 };
 
 struct KParams {
@@ -199,6 +203,13 @@ struct KParams {
     float4* ent_par;         // entry_cap: aparam of the atom
     // points
     int* cell_pt_count;      // cell_total
+    int* brick_count;        // brick_total x kBrickStride: point slots reserved in each brick (one counter per line)
+    uint2* brick_rec;        // per system: the prec slot range, brick b owning [b sk^3 pt_cap, (b + 1) sk^3 pt_cap):
+                             // (packed point, cell of the point inside the brick)
+    int brick_total;         // sum of the systems' brick ranges
+    int brick_max;           // the largest brick range of a system
+    int any_bricks;          // some system bins by bricks (SysDesc::sk > 1)
+    int any_cells;           // some system bins by cells (SysDesc::sk = 1)
     int* cell_act_count;     // cell_total
     int* frec_base;          // cell_total + 1: exclusive scan of the point counts
     int* worklist;           // VOL_KINDS x cell_total

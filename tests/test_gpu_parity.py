@@ -81,6 +81,28 @@ def test_deterministic_bitwise():
     assert np.array_equal(f1, f2)
 
 
+@pytest.mark.parametrize("kind", ["gbsw", "gbmv2"])
+def test_bitwise_across_launch_shapes(kind):
+    """Energy and forces do not depend on how many workgroups the per-cell passes launch
+    (blocks_per_cu 16 vs 256 regroups which workgroup takes which cell, and the order of the
+    binned points in each cell)."""
+    if kind == "gbsw":
+        p = N.default_params(N.MODEL_GBSW)
+    else:
+        p = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV2)
+        p.n_lebedev = 50
+    s = S.globule(3000, 6)
+    print(f"[parity] {kind} 3000: blocks_per_cu 16 / 256", flush=True)
+    eng = N.Engine(p, s.charge, s.radius)
+    eng.set_tuning(blocks_per_cu=16)
+    e1, f1 = eng.execute(s.pos)
+    eng.set_tuning(blocks_per_cu=256)
+    e2, f2 = eng.execute(s.pos)
+    eng.close()
+    assert e1 == e2
+    assert np.array_equal(f1, f2)
+
+
 @pytest.mark.parametrize("n", [1, 2, 7, 64, 65, 129])
 @pytest.mark.parametrize("kind", ["gbsw", "gbmv2"])
 def test_tiny_and_block_edge_systems(kind, n):
