@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define GBI_ABI_VERSION 4
+#define GBI_ABI_VERSION 5
 
 typedef enum gbi_status {
     GBI_OK = 0,
@@ -269,6 +269,21 @@ typedef struct gbi_custom_spec {
      * parameters whose energy derivative each evaluation also returns (gbi_get_custom_param_derivs) */
     int32_t n_param_derivs;
     const char* const* param_deriv_names;
+    /* CharmmGBMVForce::addTabulatedFunction (CharmmGBMVForce.h:442; the reference evaluates them
+     * with OpenMM's tabulated-function classes, CharmmReferenceKernels.cpp:187-191): functions
+     * of one argument callable by name in every expression.  kinds: 0 Continuous1DFunction (a
+     * cubic spline through sizes[k] values equally spaced on [ranges[2k], ranges[2k + 1]];
+     * natural and zero outside the range, or periodic[k] with period max - min and equal first
+     * and last values), 1 Discrete1DFunction (values[round(x)], 0 <= round(x) < sizes[k]; a
+     * non-finite result otherwise).  values: every function's values, concatenated.  The 2D
+     * and 3D kinds return GBI_ERR_UNSUPPORTED. */
+    int32_t n_functions;
+    const char* const* function_names;
+    const int32_t* function_kinds;
+    const int32_t* function_sizes;
+    const int32_t* function_periodic;
+    const double* function_ranges;
+    const double* function_values;
 } gbi_custom_spec;
 
 /*

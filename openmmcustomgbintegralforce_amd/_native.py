@@ -131,7 +131,28 @@ class CustomSpec(ctypes.Structure):
         ("term_types", ctypes.POINTER(ctypes.c_int32)),
         ("n_param_derivs", ctypes.c_int32),
         ("param_deriv_names", ctypes.POINTER(ctypes.c_char_p)),
+        ("n_functions", ctypes.c_int32),
+        ("function_names", ctypes.POINTER(ctypes.c_char_p)),
+        ("function_kinds", ctypes.POINTER(ctypes.c_int32)),
+        ("function_sizes", ctypes.POINTER(ctypes.c_int32)),
+        ("function_periodic", ctypes.POINTER(ctypes.c_int32)),
+        ("function_ranges", ctypes.POINTER(ctypes.c_double)),
+        ("function_values", ctypes.POINTER(ctypes.c_double)),
     ]
+
+
+TAB_CONTINUOUS_1D, TAB_DISCRETE_1D = 0, 1
+
+
+class Tabulated:
+    """A tabulated function of one argument for general expressions (gbi_custom_spec):
+    kind TAB_CONTINUOUS_1D (cubic spline through values equally spaced on [min, max],
+    natural and zero outside, or periodic) or TAB_DISCRETE_1D (values[round(x)])."""
+
+    def __init__(self, name, kind, values, min=0.0, max=1.0, periodic=False):
+        self.name, self.kind = str(name), int(kind)
+        self.values = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        self.min, self.max, self.periodic = float(min), float(max), bool(periodic)
 
 
 TERM_SINGLE, TERM_PAIR, TERM_PAIR_NO_EXCLUSIONS = 0, 1, 2
@@ -261,13 +282,13 @@ class Engine:
 
     @classmethod
     def custom(cls, params, param_names, param_values, radius_param, integrals, values, terms, globals_=None,
-               param_derivs=()):
+               param_derivs=(), functions=()):
         """A context for general CharmmGBMVForce expressions (gbi_create_custom).
 
         param_values: N x len(param_names); integrals: names (1 or 2; orders from
         params.orders); values: [(name, expression)]; terms: [(expression, TERM_*)];
         globals_: {name: value}; param_derivs: names of globals whose energy derivative each
-        evaluation returns (custom_param_derivs)."""
+        evaluation returns (custom_param_derivs); functions: Tabulated functions."""
         eng = cls.__new__(cls)
         eng._lib = load()
         pv = np.ascontiguousarray(np.asarray(param_values, dtype=np.float64).reshape(-1, len(param_names)))
@@ -279,10 +300,20 @@ class Engine:
         keep = [_strings(list(param_names)), _strings(gnames), _strings(list(integrals)),
                 _strings([v for v, _ in values]), _strings([e for _, e in values]), _strings([e for e, _ in terms]),
                 _strings(list(param_derivs))]
+        functions = list(functions)
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        fk = np.ascontiguousarray([f.kind for f in functions] or [0], dtype=np.int32)
+        fs = np.ascontiguousarray([len(f.values) for f in functions] or [0], dtype=np.int32)
+        fp = np.ascontiguousarray([int(f.periodic) for f in functions] or [0], dtype=np.int32)
+        fr = np.ascontiguousarray([v for f in functions for v in (f.min, f.max)] or [0.0], dtype=np.float64)
+        fv = np.ascontiguousarray(np.concatenate([f.values for f in functions]) if functions else [0.0],
+                                  dtype=np.float64)
+        keep += [_strings([f.name for f in functions]), fk, fs, fp, fr, fv]
         spec = CustomSpec(len(param_names), keep[0], _dp(pv), int(radius_param), len(gnames), keep[1],
                           _dp(gvals) if len(gnames) else None, len(integrals), keep[2], len(values), keep[3], keep[4],
-                          len(terms), keep[5], ttypes.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                          len(param_derivs), keep[6])
+                          len(terms), keep[5], ttypes.ctypes.data_as(i32),
+                          len(param_derivs), keep[6], len(functions), keep[7], fk.ctypes.data_as(i32),
+                          fs.ctypes.data_as(i32), fp.ctypes.data_as(i32), _dp(fr), _dp(fv))
         handle = ctypes.c_void_p()
         _check(eng._lib.gbi_create_custom(n, ctypes.byref(params), ctypes.byref(spec), ctypes.byref(handle)))
         eng._h = handle

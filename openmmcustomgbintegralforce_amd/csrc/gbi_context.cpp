@@ -154,7 +154,8 @@ struct gbi_ctx {
     std::unique_ptr<gbi::CustomProgram> cprog;
     int custom_nparams = 0, custom_radius = 0;
     gbi::CustomLaunch claunch;
-    DevBuf<double> c_prm, c_glob, c_vals, c_dvdi, c_esp, c_despdi, c_part, c_dvdth, c_despdth, c_pdth;
+    DevBuf<double> c_prm, c_glob, c_vals, c_dvdi, c_esp, c_despdi, c_part, c_dvdth, c_despdth, c_pdth, c_dvdx,
+        c_despdx;
     DevBuf<int> c_ex_ptr, c_ex_idx;
     bool any_cut = false, any_trunc = false;
     DevBuf<float> colpart;
@@ -216,7 +217,8 @@ struct gbi_ctx {
         excl_start.release();
         excl_col.release();
         excl_mask.release();
-        DevBuf<double>* cb[] = {&c_prm, &c_glob, &c_vals, &c_dvdi, &c_esp, &c_despdi, &c_part, &c_dvdth, &c_despdth, &c_pdth};
+        DevBuf<double>* cb[] = {&c_prm,     &c_glob,    &c_vals, &c_dvdi, &c_esp,   &c_despdi,
+                                &c_part,    &c_dvdth,   &c_despdth, &c_pdth, &c_dvdx, &c_despdx};
         for (auto* b : cb) b->release();
         c_ex_ptr.release();
         c_ex_idx.release();
@@ -1526,6 +1528,29 @@ gbi_status gbi_create_custom(int32_t n_atoms, const gbi_params* params, const gb
                         "energy parameter derivative of '" + name + "', which is not a global parameter");
         cs.deriv_globals.push_back(g);
     }
+    if (spec->n_functions < 0 || (spec->n_functions > 0 && (!spec->function_names || !spec->function_kinds ||
+                                                            !spec->function_sizes || !spec->function_periodic ||
+                                                            !spec->function_ranges || !spec->function_values)))
+        return fail(GBI_ERR_INVALID_ARGUMENT, "incomplete tabulated function specification");
+    for (int k = 0, off = 0; k < spec->n_functions; ++k) {
+        gbi::TabFunction f;
+        f.name = spec->function_names[k] ? spec->function_names[k] : "";
+        f.kind = spec->function_kinds[k];
+        if (f.kind != gbi::TAB_CONTINUOUS_1D && f.kind != gbi::TAB_DISCRETE_1D)
+            return fail(GBI_ERR_UNSUPPORTED, "tabulated function '" + f.name +
+                                                 "': only Continuous1DFunction and Discrete1DFunction are supported");
+        const int n = spec->function_sizes[k];
+        if (n < (f.kind == gbi::TAB_CONTINUOUS_1D ? 2 : 1))
+            return fail(GBI_ERR_INVALID_ARGUMENT, "tabulated function '" + f.name + "' has too few values");
+        f.periodic = spec->function_periodic[k] != 0;
+        f.min = spec->function_ranges[2 * k];
+        f.max = spec->function_ranges[2 * k + 1];
+        if (f.kind == gbi::TAB_CONTINUOUS_1D && !(f.min < f.max))
+            return fail(GBI_ERR_INVALID_ARGUMENT, "tabulated function '" + f.name + "' needs min < max");
+        f.values.assign(spec->function_values + off, spec->function_values + off + n);
+        off += n;
+        cs.functions.push_back(std::move(f));
+    }
     // the expressions are checked before any device work
     auto prog = std::make_unique<gbi::CustomProgram>();
     try {
@@ -1563,6 +1588,8 @@ gbi_status gbi_create_custom(int32_t n_atoms, const gbi_params* params, const gb
     HIP_TRY(c->c_dvdth.ensure(std::max<size_t>(1, NV) * NDD * N));
     HIP_TRY(c->c_despdth.ensure(NDD * N));
     HIP_TRY(c->c_pdth.ensure(NDD * N));
+    HIP_TRY(c->c_dvdx.ensure(std::max<size_t>(1, NV) * 3 * N));
+    HIP_TRY(c->c_despdx.ensure(3 * N));
     const size_t nb = (N + 63) / 64;
     HIP_TRY(c->c_part.ensure(nb * (nb + 1) / 2 * NPL * 64));
     HIP_TRY(hipMemcpyAsync(c->c_prm.p, spec->param_values, sizeof(double) * N * spec->n_params, hipMemcpyHostToDevice,
@@ -1592,6 +1619,8 @@ gbi_status gbi_create_custom(int32_t n_atoms, const gbi_params* params, const gb
     L.dvdth = c->c_dvdth.p;
     L.despdth = c->c_despdth.p;
     L.pdth = c->c_pdth.p;
+    L.dvdx = c->c_dvdx.p;
+    L.despdx = c->c_despdx.p;
     c->custom_nparams = spec->n_params;
     c->custom_radius = spec->radius_param;
     c->cprog = std::move(prog);

@@ -13,6 +13,7 @@
 
 #include <cctype>
 #include <cmath>
+#include <iomanip>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -82,8 +83,10 @@ const std::map<std::string, std::pair<int, const char*>>& functions() {
 
 class Parser {
 public:
-    Parser(const std::string& expr, const std::map<std::string, std::string>& vars)
-        : src_(expr), toks_(tokenize(expr)), vars_(vars) {}
+    // tabs: tabulated function name -> device function (one argument)
+    Parser(const std::string& expr, const std::map<std::string, std::string>& vars,
+           const std::map<std::string, std::string>& tabs)
+        : src_(expr), toks_(tokenize(expr)), vars_(vars), tabs_(tabs) {}
 
     std::string parse() {
         std::string e = sum();
@@ -161,6 +164,14 @@ private:
         if (t.kind == Token::NAME) {
             ++pos_;
             if (is_op("(")) {
+                auto tab = tabs_.find(t.text);
+                if (tab != tabs_.end()) {
+                    ++pos_;
+                    const std::string arg = sum();
+                    if (is_op(",")) fail("tabulated function '" + t.text + "' takes 1 argument(s)");
+                    expect(")");
+                    return tab->second + "(" + arg + ")";
+                }
                 auto it = functions().find(t.text);
                 if (it == functions().end()) fail("unknown function '" + t.text + "'");
                 ++pos_;
@@ -195,6 +206,7 @@ private:
     std::string src_;
     std::vector<Token> toks_;
     std::map<std::string, std::string> vars_;
+    std::map<std::string, std::string> tabs_;
     size_t pos_ = 0;
 };
 
@@ -275,6 +287,26 @@ DU d_pow(const Du<T, M>& a, const Du<T, M>& b) {
     for (int k = 0; k < M; ++k) r.d[k] = da * a.d[k] + (b.d[k] != T(0) ? f * m_log(a.v) * b.d[k] : T(0));
     return r;
 }
+// cubic spline through y at lo + k h (second derivatives y2), evaluated in FP64: natural and
+// zero outside [lo, hi], or periodic with period hi - lo
+DU d_tabc(const Du<T, M>& a, const double* y, const double* y2, int n, double lo, double hi, int periodic) {
+    double t = (double)a.v;
+    if (periodic) t -= (hi - lo) * floor((t - lo) / (hi - lo));
+    else if (!(t >= lo && t <= hi)) return Du<T, M>::c(T(0));
+    const double h = (hi - lo) / (n - 1);
+    int k = (int)floor((t - lo) / h);
+    k = k < 0 ? 0 : (k > n - 2 ? n - 2 : k);
+    const double A = (lo + (k + 1) * h - t) / h, B = 1.0 - A;
+    const double f = A * y[k] + B * y[k + 1] + ((A * A * A - A) * y2[k] + (B * B * B - B) * y2[k + 1]) * (h * h / 6.0);
+    const double df = (y[k + 1] - y[k]) / h + ((1.0 - 3.0 * A * A) * y2[k] + (3.0 * B * B - 1.0) * y2[k + 1]) * (h / 6.0);
+    return d_chain(a, T(f), T(df));
+}
+// y[round(x)]; NaN outside 0 <= round(x) < n (the evaluation then reports a non-finite result)
+DU d_tabd(const Du<T, M>& a, const double* y, int n) {
+    const double t = round((double)a.v);
+    const T f = (t >= 0.0 && t < (double)n) ? T(y[(int)t]) : T(__builtin_nan(""));
+    return Du<T, M>::c(f);
+}
 )";
 
 void check_hiprtc(hiprtcResult r, const char* what) {
@@ -288,7 +320,8 @@ CustomProgram::~CustomProgram() {
 }
 
 std::string translate_expression(const std::string& expr, const CustomSpec& spec, int context, int index) {
-    std::map<std::string, std::string> vars;
+    std::map<std::string, std::string> vars, tabs;
+    for (size_t k = 0; k < spec.functions.size(); ++k) tabs[spec.functions[k].name] = "d_tab" + std::to_string(k);
     for (size_t k = 0; k < spec.globals.size(); ++k) vars[spec.globals[k]] = "G" + std::to_string(k);
     if (context == 2) {
         vars["r"] = "r_";
@@ -301,12 +334,84 @@ std::string translate_expression(const std::string& expr, const CustomSpec& spec
             vars[spec.value_names[k] + "2"] = "V2_" + std::to_string(k);
         }
     } else {
+        vars["x"] = "X_";
+        vars["y"] = "Y_";
+        vars["z"] = "Z_";
         for (size_t k = 0; k < spec.params.size(); ++k) vars[spec.params[k]] = "P_" + std::to_string(k);
         for (size_t k = 0; k < spec.integrals.size(); ++k) vars[spec.integrals[k]] = "I_" + std::to_string(k);
         const int nv = context == 0 ? index : (int)spec.value_names.size();
         for (int k = 0; k < nv; ++k) vars[spec.value_names[k]] = "V_" + std::to_string(k);
     }
-    return Parser(expr, vars).parse();
+    return Parser(expr, vars, tabs).parse();
+}
+
+std::vector<double> spline_second_derivatives(const std::vector<double>& y, double h, bool periodic) {
+    const int n = (int)y.size();
+    if (n < 2) throw std::runtime_error("a tabulated function needs at least 2 values");
+    if (!(h > 0.0) || !std::isfinite(h)) throw std::runtime_error("a tabulated function needs min < max");
+    std::vector<double> M(n, 0.0);
+    if (!periodic) {
+        // M_0 = M_{n-1} = 0; M_{i-1} + 4 M_i + M_{i+1} = 6 (y_{i+1} - 2 y_i + y_{i-1}) / h^2
+        if (n < 3) return M;
+        const int m = n - 2;
+        std::vector<double> c(m), d(m);
+        for (int i = 0; i < m; ++i) {
+            const double rhs = 6.0 * (y[i + 2] - 2.0 * y[i + 1] + y[i]) / (h * h);
+            const double den = 4.0 - (i > 0 ? c[i - 1] : 0.0);
+            c[i] = 1.0 / den;
+            d[i] = (rhs - (i > 0 ? d[i - 1] : 0.0)) / den;
+        }
+        for (int i = m - 1; i >= 0; --i) M[i + 1] = d[i] - (i + 1 < m ? c[i] * M[i + 2] : 0.0);
+        return M;
+    }
+    if (y.front() != y.back())
+        throw std::runtime_error("a periodic tabulated function needs equal first and last values");
+    // the same equations with indices modulo m = n - 1 (M_{n-1} = M_0): a cyclic tridiagonal
+    // system, solved by Sherman-Morrison from two ordinary tridiagonal solves
+    const int m = n - 1;
+    auto rhs = [&](int i) {
+        const double ym = y[(i - 1 + m) % m], y0 = y[i], yp = y[(i + 1) % m];
+        return 6.0 * (yp - 2.0 * y0 + ym) / (h * h);
+    };
+    if (m <= 2) {  // 1 or 2 unknowns: constant or two-point tables have no curvature to spread
+        if (m == 1) M[0] = rhs(0) / 6.0;  // M_0 (1 + 4 + 1) = rhs
+        if (m == 2) {
+            // [4 2; 2 4] M = rhs
+            const double r0 = rhs(0), r1 = rhs(1);
+            M[0] = (4.0 * r0 - 2.0 * r1) / 12.0;
+            M[1] = (4.0 * r1 - 2.0 * r0) / 12.0;
+        }
+        M[m] = M[0];
+        return M;
+    }
+    // A = T + u v^T with T tridiagonal (diagonal 4, first entry 4 - g, last 4 - 1/g), u = (g, 0, .., 1),
+    // v = (1, 0, .., 1/g), g = -4
+    const double g = -4.0;
+    std::vector<double> diag(m, 4.0), b(m), u(m, 0.0);
+    diag[0] -= g;
+    diag[m - 1] -= 1.0 / g;
+    for (int i = 0; i < m; ++i) b[i] = rhs(i);
+    u[0] = g;
+    u[m - 1] = 1.0;
+    auto solve = [&](std::vector<double> r) {
+        std::vector<double> c(m), x(m);
+        double den = diag[0];
+        c[0] = 1.0 / den;
+        r[0] /= den;
+        for (int i = 1; i < m; ++i) {
+            den = diag[i] - c[i - 1];
+            c[i] = 1.0 / den;
+            r[i] = (r[i] - r[i - 1]) / den;
+        }
+        x[m - 1] = r[m - 1];
+        for (int i = m - 2; i >= 0; --i) x[i] = r[i] - c[i] * x[i + 1];
+        return x;
+    };
+    const std::vector<double> xb = solve(b), zu = solve(u);
+    const double fact = (xb[0] + xb[m - 1] / g) / (1.0 + zu[0] + zu[m - 1] / g);
+    for (int i = 0; i < m; ++i) M[i] = xb[i] - fact * zu[i];
+    M[m] = M[0];
+    return M;
 }
 
 void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
@@ -320,9 +425,19 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
     auto name_ok = [&](const std::string& n, const char* what) {
         if (!valid_name(n)) throw std::runtime_error(std::string("invalid ") + what + " name '" + n + "'");
         if (n == "r") throw std::runtime_error("'r' is reserved for the pair distance");
+        if (n == "x" || n == "y" || n == "z") throw std::runtime_error("'" + n + "' is reserved for the atom's coordinate");
         if (seen.count(n)) throw std::runtime_error("name '" + n + "' defined twice");
         seen[n] = 1;
     };
+    for (const auto& f : spec.functions) {
+        name_ok(f.name, "tabulated function");
+        if (f.kind != TAB_CONTINUOUS_1D && f.kind != TAB_DISCRETE_1D)
+            throw std::runtime_error("tabulated function '" + f.name + "': only Continuous1DFunction and "
+                                     "Discrete1DFunction are supported");
+        if (f.values.empty()) throw std::runtime_error("tabulated function '" + f.name + "' has no values");
+        for (double v : f.values)
+            if (!std::isfinite(v)) throw std::runtime_error("tabulated function '" + f.name + "' has a non-finite value");
+    }
     for (const auto& n : spec.params) name_ok(n, "parameter");
     for (const auto& n : spec.globals) name_ok(n, "global parameter");
     for (const auto& n : spec.integrals) name_ok(n, "integral");
@@ -341,17 +456,44 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
 
     std::ostringstream s;
     s << kDualSource;
+    // the tabulated functions' tables (FP64 literals, exact to the last bit) and callers
+    s << std::setprecision(17);
+    for (size_t k = 0; k < spec.functions.size(); ++k) {
+        const TabFunction& f = spec.functions[k];
+        const int n = (int)f.values.size();
+        auto table = [&](const char* tag, const std::vector<double>& v) {
+            s << "__device__ const double TAB" << k << tag << "[" << n << "] = {";
+            for (int i = 0; i < n; ++i) s << (i ? ", " : "") << v[i];
+            s << "};\n";
+        };
+        table("_Y", f.values);
+        if (f.kind == TAB_CONTINUOUS_1D) {
+            const std::vector<double> y2 =
+                spline_second_derivatives(f.values, (f.max - f.min) / (n - 1), f.periodic);
+            table("_Y2", y2);
+            s << "DU d_tab" << k << "(const Du<T, M>& a) { return d_tabc(a, TAB" << k << "_Y, TAB" << k << "_Y2, " << n
+              << ", " << f.min << ", " << f.max << ", " << (f.periodic ? 1 : 0) << "); }\n";
+        } else {
+            s << "DU d_tab" << k << "(const Du<T, M>& a) { return d_tabd(a, TAB" << k << "_Y, " << n << "); }\n";
+        }
+    }
+    s << std::setprecision(6);
     s << "#define NP " << NP << "\n#define NG " << (NG > 0 ? NG : 1) << "\n#define NV " << NV << "\n#define ND "
       << ND << "\n#define NDD " << (ND > 0 ? ND : 1) << "\n";
     const int NPL = (4 + NV + ND) + (3 + NV);  // row and column planes of a pair tile
 
     // ---- per atom: computed values and single-particle terms, duals over the integrals and
     // the globals whose energy derivative is asked for ----
+    // dual slots: 0, 1 the integrals, 2 .. 2 + ND - 1 the globals of the energy derivatives,
+    // XS .. XS + 2 the atom's x, y, z
+    s << "#define XS (2 + ND)\n";
     s << "extern \"C\" __global__ void cg_values(int N, const double* __restrict__ prm, const double* __restrict__ "
          "integ, const double* __restrict__ glob, double* __restrict__ vals, double* __restrict__ dvdi, double* "
-         "__restrict__ esp, double* __restrict__ despdi, double* __restrict__ dvdth, double* __restrict__ despdth) {\n"
+         "__restrict__ esp, double* __restrict__ despdi, double* __restrict__ dvdth, double* __restrict__ despdth, "
+         "const double* __restrict__ pos, double* __restrict__ dvdx, double* __restrict__ despdx) {\n"
          "    const int i = blockIdx.x * blockDim.x + threadIdx.x;\n    if (i >= N) return;\n"
-         "    typedef Du<double, 2 + ND> D;\n";
+         "    typedef Du<double, XS + 3> D;\n"
+         "    const D X_ = D::var(pos[3 * i], XS), Y_ = D::var(pos[3 * i + 1], XS + 1), Z_ = D::var(pos[3 * i + 2], XS + 2);\n";
     for (int k = 0; k < NG; ++k) {
         if (deriv_slot[k] >= 0)
             s << "    const D G" << k << " = D::var(glob[" << k << "], " << 2 + deriv_slot[k] << ");\n";
@@ -366,13 +508,15 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
         s << "    dvdi[(" << k << " * 2 + 0) * N + i] = V_" << k << ".d[0];\n";
         s << "    dvdi[(" << k << " * 2 + 1) * N + i] = V_" << k << ".d[1];\n";
         s << "    for (int m = 0; m < ND; ++m) dvdth[(" << k << " * NDD + m) * N + i] = V_" << k << ".d[2 + m];\n";
+        s << "    for (int d = 0; d < 3; ++d) dvdx[(" << k << " * 3 + d) * N + i] = V_" << k << ".d[XS + d];\n";
     }
     s << "    D E_ = D::c(0.0);\n";
     for (size_t t = 0; t < spec.term_exprs.size(); ++t)
         if (spec.term_types[t] == TERM_SINGLE)
             s << "    E_ = E_ + (" << translate_expression(spec.term_exprs[t], spec, 1, 0) << ");\n";
     s << "    esp[i] = E_.v;\n    despdi[i] = E_.d[0];\n    despdi[N + i] = E_.d[1];\n"
-         "    for (int m = 0; m < ND; ++m) despdth[m * N + i] = E_.d[2 + m];\n}\n";
+         "    for (int m = 0; m < ND; ++m) despdth[m * N + i] = E_.d[2 + m];\n"
+         "    for (int d = 0; d < 3; ++d) despdx[d * N + i] = E_.d[XS + d];\n}\n";
 
     // ---- pairs: 64 x 64 tiles (row block I, column block J >= I), each pair once with atom1
     // the row atom (atom1 < atom2 as the reference orders a pair, CharmmReferenceGBMV.cpp:
@@ -494,7 +638,8 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
     s << "extern \"C\" __global__ void __launch_bounds__(64 * CG_CHAIN_WAVES) cg_chain(int N, int nb, const double* "
          "__restrict__ part, const double* __restrict__ dvdi, const double* __restrict__ esp, const double* __restrict__ "
          "despdi, double* __restrict__ atom_energy, long long* __restrict__ force_fix, double* __restrict__ dedi, "
-         "const double* __restrict__ dvdth, const double* __restrict__ despdth, double* __restrict__ pdth) {\n"
+         "const double* __restrict__ dvdth, const double* __restrict__ despdth, double* __restrict__ pdth, "
+         "const double* __restrict__ dvdx, const double* __restrict__ despdx) {\n"
          "    const int I = blockIdx.x, l = threadIdx.x & 63, w = threadIdx.x >> 6;\n"
          "    const int a = 64 * I + l;\n"
          "    __shared__ double s_sum[CG_CHAIN_WAVES][NPR + NPC][64];\n"
@@ -514,9 +659,14 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
          "        for (int p = 0; p < NPR; ++p) r[p] += s_sum[v][p][l];\n"
          "        for (int p = 0; p < NPC; ++p) c[p] += s_sum[v][NPR + p][l];\n    }\n"
          "    atom_energy[a] = esp[a] + r[3];\n"
-         "    force_fix[a] = (long long)llrint((r[0] + c[0]) * 4294967296.0);\n"
-         "    force_fix[N + a] = (long long)llrint((r[1] + c[1]) * 4294967296.0);\n"
-         "    force_fix[2 * N + a] = (long long)llrint((r[2] + c[2]) * 4294967296.0);\n"
+         "    // pair forces, minus the gradient of the energy with respect to the atom's own\n"
+         "    // coordinates through the single-particle terms and through the values (dE/dV_k: the\n"
+         "    // pair terms' sums; the single-particle terms' are inside despdx)\n"
+         "    for (int d = 0; d < 3; ++d) {\n"
+         "        double gx = despdx[d * N + a];\n"
+         "        for (int k = 0; k < NV; ++k) gx += (r[4 + k] + c[3 + k]) * dvdx[(k * 3 + d) * N + a];\n"
+         "        force_fix[d * N + a] = (long long)llrint((r[d] + c[d] - gx) * 4294967296.0);\n"
+         "    }\n"
          "    for (int m = 0; m < 2; ++m) {\n"
          "        double g = despdi[m * N + a];\n"
          "        for (int k = 0; k < NV; ++k) g += (r[4 + k] + c[3 + k]) * dvdi[(k * 2 + m) * N + a];\n"

@@ -21,6 +21,22 @@ constexpr int kCustomChainWaves = 4; // waves of a cg_chain workgroup (one 64-at
 
 enum CustomTermType : int { TERM_SINGLE = 0, TERM_PAIR = 1, TERM_PAIR_NO_EXCL = 2 };
 
+// Tabulated functions of one argument (CharmmGBMVForce::addTabulatedFunction with OpenMM's
+// Continuous1DFunction and Discrete1DFunction): a cubic spline through values at equally
+// spaced points on [min, max] (natural, or periodic with period max - min; zero outside the
+// range when not periodic), or values[round(x)] for integer arguments 0 <= round(x) < n.
+enum TabKind : int { TAB_CONTINUOUS_1D = 0, TAB_DISCRETE_1D = 1 };
+struct TabFunction {
+    std::string name;
+    int kind = TAB_CONTINUOUS_1D;
+    bool periodic = false;
+    double min = 0.0, max = 1.0;
+    std::vector<double> values;
+};
+// Second derivatives of the natural (periodic) cubic spline through `values` at equally spaced
+// points h apart; periodic splines need values.front() == values.back().  Throws on bad input.
+std::vector<double> spline_second_derivatives(const std::vector<double>& values, double h, bool periodic);
+
 struct CustomSpec {
     std::vector<std::string> params;     // per-particle parameter names
     std::vector<std::string> globals;    // global parameter names
@@ -29,20 +45,24 @@ struct CustomSpec {
     std::vector<std::string> term_exprs;
     std::vector<int> term_types;
     std::vector<int> deriv_globals;      // globals whose energy derivative is computed (indices)
+    std::vector<TabFunction> functions;  // tabulated functions, callable by name in every expression
 };
 
 // Compiled kernels of one spec.  Kernel signatures (all arrays device, double unless noted;
 // ND = the number of parameter derivatives, arrays sized with max(ND, 1)):
 //   cg_values(int N, prm[N x NP], integ[2 x N], glob[NG], vals[NV x N], dvdi[NV x 2 x N],
-//             esp[N], despdi[2 x N], dvdth[NV x ND x N], despdth[ND x N])
-//                                                                     grid N/256, block 256
+//             esp[N], despdi[2 x N], dvdth[NV x ND x N], despdth[ND x N], pos[N x 3],
+//             dvdx[NV x 3 x N], despdx[3 x N])                        grid N/256, block 256
 //   cg_pairs(int N, int nb, pos[N x 3], prm, vals, glob, int method, double cut2,
 //            const double* box3, const int* ex_ptr, const int* ex_idx,
 //            part[nb (nb + 1) / 2 tiles x NPL x 64])     grid (nb, nb), block 64 kCustomPairWaves
 //   cg_chain(int N, int nb, part, dvdi, esp, despdi, atom_energy[N], long long* force_fix[3 x N],
-//            dedi[2 x N], dvdth, despdth, pdth[ND x N])    grid nb, block 64 kCustomChainWaves
+//            dedi[2 x N], dvdth, despdth, pdth[ND x N], dvdx, despdx)
+//                                                          grid nb, block 64 kCustomChainWaves
 // (nb = ceil(N / 64) atom blocks; NPL = n_planes planes per tile: 4 + NV + ND row, 3 + NV
-// column; pdth holds each atom's share of dE/dtheta, summed over atoms on the host)
+// column; pdth holds each atom's share of dE/dtheta, summed over atoms on the host; dvdx and
+// despdx are the derivatives of the values and single-particle terms with respect to the
+// atom's own coordinates x, y, z, which computed values and single-particle terms may use)
 struct CustomProgram {
     hipModule_t module = nullptr;
     hipFunction_t values = nullptr, pairs = nullptr, chain = nullptr;
@@ -56,9 +76,10 @@ struct CustomProgram {
 };
 
 // Translates one expression to HIP source over the dual type D.  context: 0 computed value
-// number `index` (sees parameters, globals, integrals and earlier values), 1 single-particle
-// energy term, 2 pair energy term (sees r, parameters and values with suffixes 1 and 2,
-// globals).  Throws std::runtime_error naming the problem.
+// number `index` (sees parameters, globals, integrals, earlier values and x, y, z), 1
+// single-particle energy term (also every value), 2 pair energy term (sees r, parameters and
+// values with suffixes 1 and 2, globals); tabulated functions everywhere.  Throws
+// std::runtime_error naming the problem.
 std::string translate_expression(const std::string& expr, const CustomSpec& spec, int context, int index);
 
 // Validates the spec and generates the kernels' source (host only, no device); throws

@@ -134,6 +134,8 @@ struct CustomLaunch {
     double* dvdth = nullptr;   // dV/dtheta per value, derivative and atom
     double* despdth = nullptr; // single-particle dE/dtheta per atom
     double* pdth = nullptr;    // each atom's share of dE/dtheta
+    double* dvdx = nullptr;    // dV/d(x, y, z) per value, axis and atom
+    double* despdx = nullptr;  // d(single-particle energy)/d(x, y, z) per axis and atom
     const int* ex_ptr = nullptr;
     const int* ex_idx = nullptr;
 };

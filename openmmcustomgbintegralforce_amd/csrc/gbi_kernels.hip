@@ -1704,7 +1704,10 @@ __global__ void __launch_bounds__(64, (ADJ || TRUNC) ? 2 : 3) k_force(KParams p)
     __shared__ int q_tf[TRUNC ? kWave : 1];      // TRUNC: 0 every shell atom within c of the owner, 1 test, 2 none
     __shared__ float q_0[kWave], q_1[kWave], q_2[kWave], q_3[kWave], q_4[kWave], q_5[kWave];
     __shared__ int q_own[kWave], q_pid[kWave];
-    __shared__ double s_red[8][kWave];  // one axis of the per-lane atom partials at a time
+    // one axis of the per-lane atom partials at a time; rows padded by 8 doubles so the 8 rows
+    // that one read instruction touches start 64 bytes apart (unpadded, 512-byte rows put 8
+    // lanes on each bank)
+    __shared__ double s_red[8][kWave + 8];
     if (batch_fatal(p)) return;
     const int lane = threadIdx.x, ag = lane & 7, pg = lane >> 3;
     const int nwork = p.work[VK];
@@ -2152,19 +2155,20 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
         const double* integ = p.integrals;
         const double *prm = c.prm, *glob = c.glob, *box = c.box;
         double *vals = c.vals, *dvdi = c.dvdi, *esp = c.esp, *despdi = c.despdi, *part = c.part;
-        double *dvdth = c.dvdth, *despdth = c.despdth, *pdth = c.pdth;
+        double *dvdth = c.dvdth, *despdth = c.despdth, *pdth = c.pdth, *dvdx = c.dvdx, *despdx = c.despdx;
         const int *ex_ptr = c.ex_ptr, *ex_idx = c.ex_idx;
         double* atom_energy = p.atom_energy;
         long long* force_fix = p.force_fix;
         double* dedi = p.dedi;
-        void* va[] = {&N, &prm, &integ, &glob, &vals, &dvdi, &esp, &despdi, &dvdth, &despdth};
+        void* va[] = {&N, &prm, &integ, &glob, &vals, &dvdi, &esp, &despdi, &dvdth, &despdth, &pos, &dvdx, &despdx};
         err = hipModuleLaunchKernel(c.values, (N + 255) / 256, 1, 1, 256, 1, 1, 0, stream, va, nullptr);
         if (err != hipSuccess) return err;
         void* pa[] = {&N, &nb, &pos, &prm, &vals, &glob, &method, &cut2, &box, &ex_ptr, &ex_idx, &part};
         err = hipModuleLaunchKernel(c.pairs, nb, nb, 1, c.pair_threads, 1, 1, 0, stream, pa, nullptr);
         if (err != hipSuccess) return err;
         mark(5);
-        void* ca[] = {&N, &nb, &part, &dvdi, &esp, &despdi, &atom_energy, &force_fix, &dedi, &dvdth, &despdth, &pdth};
+        void* ca[] = {&N,    &nb,        &part, &dvdi,    &esp,    &despdi, &atom_energy,
+                      &force_fix, &dedi, &dvdth, &despdth, &pdth, &dvdx, &despdx};
         err = hipModuleLaunchKernel(c.chain, nb, 1, 1, c.chain_threads, 1, 1, 0, stream, ca, nullptr);
         if (err != hipSuccess) return err;
     } else if (adj) {

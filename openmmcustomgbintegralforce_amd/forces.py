@@ -88,6 +88,50 @@ class CharmmGBSWForce:
 # --------------------------------------------------------------------------------------
 # CharmmGBMVForce (CharmmGBMVForce.h:63-626)
 # --------------------------------------------------------------------------------------
+class TabulatedFunction:
+    """OpenMM's tabulated-function base (openmm/TabulatedFunction.h)."""
+
+    _periodic = False
+
+    def getPeriodic(self) -> bool:
+        return self._periodic
+
+
+class Continuous1DFunction(TabulatedFunction):
+    """A cubic spline through values equally spaced on [min, max] (natural, zero outside the
+    range; or periodic with period max - min)."""
+
+    def __init__(self, values, min: float, max: float, periodic: bool = False):
+        self._values, self._min, self._max, self._periodic = list(values), float(min), float(max), bool(periodic)
+
+    def getFunctionParameters(self):
+        return list(self._values), self._min, self._max
+
+    def setFunctionParameters(self, values, min: float, max: float) -> None:
+        self._values, self._min, self._max = list(values), float(min), float(max)
+
+
+class Discrete1DFunction(TabulatedFunction):
+    """values[round(x)]."""
+
+    def __init__(self, values):
+        self._values = list(values)
+
+    def getFunctionParameters(self):
+        return list(self._values)
+
+    def setFunctionParameters(self, values) -> None:
+        self._values = list(values)
+
+
+class Continuous2DFunction(TabulatedFunction):
+    """Two-dimensional table (declared for completeness; the HIP kernels do not evaluate it)."""
+
+    def __init__(self, xsize, ysize, values, xmin, xmax, ymin, ymax, periodic=False):
+        self._args = (int(xsize), int(ysize), list(values), xmin, xmax, ymin, ymax)
+        self._periodic = bool(periodic)
+
+
 class CharmmGBMVForce:
     NoCutoff, CutoffNonPeriodic, CutoffPeriodic = 0, 1, 2
     SingleParticle, ParticlePair, ParticlePairNoExclusions = 0, 1, 2
@@ -105,6 +149,7 @@ class CharmmGBMVForce:
         self._integrals = []  # (name, ints, reals)
         self._energy = []  # (expression, type)
         self._param_derivs = []
+        self._functions = []  # (name, TabulatedFunction)
 
     # -- bookkeeping (CharmmGBMVForce.h:119-410)
     def getNumParticles(self):
@@ -209,6 +254,25 @@ class CharmmGBMVForce:
     def addExclusion(self, p1: int, p2: int) -> int:
         self._exclusions.append((int(p1), int(p2)))
         return len(self._exclusions) - 1
+
+    # -- tabulated functions (CharmmGBMVForce.h:437-483)
+    def addTabulatedFunction(self, name: str, function: TabulatedFunction) -> int:
+        self._functions.append((name, function))
+        return len(self._functions) - 1
+
+    def getNumTabulatedFunctions(self) -> int:
+        return len(self._functions)
+
+    getNumFunctions = getNumTabulatedFunctions
+
+    def getTabulatedFunction(self, i) -> TabulatedFunction:
+        return self._functions[i][1]
+
+    def getTabulatedFunctionName(self, i) -> str:
+        return self._functions[i][0]
+
+    def addFunction(self, name: str, values, min: float, max: float) -> int:
+        return self.addTabulatedFunction(name, Continuous1DFunction(values, min, max))
 
 
 CustomGBIntegralForce = CharmmGBMVForce
@@ -433,9 +497,19 @@ def gbmv_custom_from_force(force: CharmmGBMVForce) -> dict:
     g = {force.getGlobalParameterName(i): force.getGlobalParameterDefaultValue(i)
          for i in range(force.getNumGlobalParameters())}
     pv = np.array([force.getParticleParameters(i) for i in range(force.getNumParticles())], dtype=np.float64)
+    functions = []
+    for name, fn in force._functions:
+        if isinstance(fn, Continuous1DFunction):
+            v, lo, hi = fn.getFunctionParameters()
+            functions.append(N.Tabulated(name, N.TAB_CONTINUOUS_1D, v, lo, hi, fn.getPeriodic()))
+        elif isinstance(fn, Discrete1DFunction):
+            functions.append(N.Tabulated(name, N.TAB_DISCRETE_1D, fn.getFunctionParameters()))
+        else:
+            raise OpenMMException(f"CharmmGBMVForce: tabulated function '{name}': the HIP platform evaluates "
+                                  "Continuous1DFunction and Discrete1DFunction only")
     return dict(params=p, param_names=names, param_values=pv.reshape(-1, len(names)), radius_param=names.index("radius"),
                 integrals=[n for n, _ in integ], values=values, terms=terms, globals_=g,
-                param_derivs=list(force._param_derivs))
+                param_derivs=list(force._param_derivs), functions=functions)
 
 
 def gbmv_exclusions(force: CharmmGBMVForce) -> np.ndarray:

@@ -4,7 +4,8 @@
 //
 // Input (binary, native endianness): int32 model (0 CharmmGBSWForce, 1 CharmmGBMVForce with
 // the canonical expressions, 2 CharmmGBMVForce with the general ones of generalGBMV, 3 the
-// general ones of derivGBMV, which asks for energy parameter derivatives),
+// general ones of derivGBMV, which asks for energy parameter derivatives, 4 those of tabGBMV,
+// with tabulated functions and the atoms' coordinates),
 // int32 integral type, int32 Lebedev rule (0 = the reference's), int32 N, then N x 3
 // positions, N charges, N radii (doubles), optionally followed by int32 nonbonded method,
 // double cutoff and 3 doubles of a rectangular periodic box, then optionally int32 n and n
@@ -16,6 +17,7 @@
 //
 // `gbi_openmm_driver --canonical-check` (no device needed) prints which CharmmGBMVForce
 // variants the kernel class takes as the canonical forms.
+#include <cmath>
 #include <cstdio>
 #include <cstdint>
 #include <memory>
@@ -97,6 +99,28 @@ CharmmGBMVForce* derivGBMV(const std::vector<double>& q, const std::vector<doubl
     f->addEnergyTerm(gbi_openmm::kCanonicalPairEnergy, CharmmGBMVForce::ParticlePairNoExclusions);
     f->addEnergyParameterDerivative("k");
     f->addEnergyParameterDerivative("P");
+    for (size_t i = 0; i < q.size(); ++i) f->addParticle({q[i], r[i]});
+    return f;
+}
+
+// generalGBMV with a Continuous1DFunction and a Discrete1DFunction, and the coordinates x, y, z
+// in the Born value and in the single-particle term (the same expressions as
+// tests/test_gpu_openmm_plugin.py's oracle call).
+CharmmGBMVForce* tabGBMV(const std::vector<double>& q, const std::vector<double>& r, int type) {
+    auto* f = new CharmmGBMVForce();
+    f->addPerParticleParameter("q");
+    f->addPerParticleParameter("radius");
+    f->addGlobalParameter("P", -138.935456 * (1.0 - 1.0 / 80.0));
+    f->addGBIntegral("I4", std::vector<int>{4});
+    f->setGBIntegralType((CharmmGBMVForce::GBIntegralType)type);
+    std::vector<double> sv(31);
+    for (int k = 0; k < 31; ++k) sv[k] = 1.0 + 0.2 * std::sin(0.3 * k);
+    f->addTabulatedFunction("scale", new Continuous1DFunction(sv, 0.0, 3.0));
+    f->addTabulatedFunction("w", new Discrete1DFunction({0.9, 1.1}));
+    f->addComputedValue("B", "scale(10*radius)*(1+0.002*(x+y-z))/(1/radius-0.5*I4*radius)",
+                        CharmmGBMVForce::SingleParticle);
+    f->addEnergyTerm("0.5*P*q^2/B*w(step(q))+0.01*q^2*(x^2+y^2+z^2)", CharmmGBMVForce::SingleParticle);
+    f->addEnergyTerm(gbi_openmm::kCanonicalPairEnergy, CharmmGBMVForce::ParticlePairNoExclusions);
     for (size_t i = 0; i < q.size(); ++i) f->addParticle({q[i], r[i]});
     return f;
 }
@@ -249,6 +273,7 @@ int main(int argc, char** argv) {
             fprintf(out, "count_change_throws %d\n", count_throws);
         } else {
             auto make = [&](const std::vector<double>& charges) {
+                if (model == 4) return tabGBMV(charges, r, type);
                 return model == 3 ? derivGBMV(charges, r, type)
                                   : (model == 2 ? generalGBMV(charges, r, type) : canonicalGBMV(charges, r, type, rule, excl));
             };

@@ -5,6 +5,7 @@
 #include <cctype>
 
 #include "openmm/OpenMMException.h"
+#include "openmm/TabulatedFunction.h"
 
 using namespace OpenMM;
 
@@ -324,9 +325,41 @@ void HipCalcCharmmGBMVForceKernel::createCustom(const CharmmGBMVForce& force) {
         for (const auto& x : v) out.push_back(x.c_str());
         return out;
     };
+    // tabulated functions: the one-dimensional kinds (OpenMM's Continuous1DFunction, Discrete1DFunction)
+    std::vector<std::string> fnames;
+    std::vector<int32_t> fkinds, fsizes, fperiodic;
+    std::vector<double> franges, fvalues;
+    for (int k = 0; k < force.getNumTabulatedFunctions(); ++k) {
+        const TabulatedFunction& fn = force.getTabulatedFunction(k);
+        fnames.push_back(force.getTabulatedFunctionName(k));
+        std::vector<double> v;
+        double lo = 0.0, hi = 0.0;
+        if (const auto* c1 = dynamic_cast<const Continuous1DFunction*>(&fn)) {
+            c1->getFunctionParameters(v, lo, hi);
+            fkinds.push_back(0);
+        } else if (const auto* d1 = dynamic_cast<const Discrete1DFunction*>(&fn)) {
+            d1->getFunctionParameters(v);
+            fkinds.push_back(1);
+        } else {
+            throw OpenMMException("CharmmGBMVForce: tabulated function '" + fnames.back() +
+                                  "': the HIP platform evaluates Continuous1DFunction and Discrete1DFunction only");
+        }
+        fsizes.push_back((int32_t)v.size());
+        fperiodic.push_back(fn.getPeriodic() ? 1 : 0);
+        franges.push_back(lo);
+        franges.push_back(hi);
+        fvalues.insert(fvalues.end(), v.begin(), v.end());
+    }
     const auto cp = cstr(pnames), cg = cstr(gnames), ci = cstr(inames), cvn = cstr(vnames), cve = cstr(vexprs),
-               ct = cstr(texprs), cd = cstr(derivs_);
+               ct = cstr(texprs), cd = cstr(derivs_), cf = cstr(fnames);
     gbi_custom_spec spec{};
+    spec.n_functions = (int32_t)fnames.size();
+    spec.function_names = cf.data();
+    spec.function_kinds = fkinds.data();
+    spec.function_sizes = fsizes.data();
+    spec.function_periodic = fperiodic.data();
+    spec.function_ranges = franges.data();
+    spec.function_values = fvalues.data();
     spec.n_params = np;
     spec.param_names = cp.data();
     spec.param_values = pvals.data();

@@ -2,15 +2,17 @@
 // openmmapi headers are not on the include path).  Same methods as
 // openmmapi/include/openmm/CharmmGBMVForce.h (per-particle and global parameters, GB
 // integrals with integer/real parameters, computed values, energy terms, exclusions,
-// GBIntegralType); tabulated functions and ForceImpl creation are not part of the stand-in.
+// GBIntegralType, tabulated functions); ForceImpl creation is not part of the stand-in.
 #pragma once
 
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
 
 #include "openmm/Force.h"
 #include "openmm/OpenMMException.h"
+#include "openmm/TabulatedFunction.h"
 
 namespace OpenMM {
 
@@ -116,6 +118,20 @@ public:
     }
     bool usesPeriodicBoundaryConditions() const { return method_ == CutoffPeriodic; }
 
+    // tabulated functions (the force takes ownership, as OpenMM's forces do)
+    int addTabulatedFunction(const std::string& name, TabulatedFunction* function) {
+        functions_.push_back({name, std::shared_ptr<TabulatedFunction>(function)});
+        return (int)functions_.size() - 1;
+    }
+    int getNumTabulatedFunctions() const { return (int)functions_.size(); }
+    int getNumFunctions() const { return getNumTabulatedFunctions(); }
+    const TabulatedFunction& getTabulatedFunction(int index) const { return *functions_.at(index).second; }
+    TabulatedFunction& getTabulatedFunction(int index) { return *functions_.at(index).second; }
+    const std::string& getTabulatedFunctionName(int index) const { return functions_.at(index).first; }
+    int addFunction(const std::string& name, const std::vector<double>& values, double min, double max) {
+        return addTabulatedFunction(name, new Continuous1DFunction(values, min, max));
+    }
+
 private:
     struct Term {
         std::string name, expression;
@@ -132,6 +148,7 @@ private:
     std::vector<Term> values_, energy_;
     std::vector<Integral> integrals_;
     std::vector<std::pair<int, int>> exclusions_;
+    std::vector<std::pair<std::string, std::shared_ptr<TabulatedFunction>>> functions_;
     NonbondedMethod method_ = NoCutoff;
     GBIntegralType integral_type_ = GBSWIntegral;
     double cutoff_ = 1.0;

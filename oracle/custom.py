@@ -6,9 +6,15 @@ follows the Reference platform's CharmmReferenceGBMV::calculateIxn
 orc_integral_chain, CustomGBIntegral / GBSWIntegral / GBMVIntegralType{I,II}), computed
 values as SingleParticle (:173-176, :206-227), single-particle energy terms (:229-266), pair
 terms over i < j with the cutoff rule r >= c skipped and exclusions of ParticlePair terms
-(:268-348), and the chain rule dE/dV -> dE/dI -> forces (:350-446).  The expressions are
-parsed by sympy and differentiated symbolically, independently of the library's dual-number
-code generation.
+(:268-348), and the chain rule dE/dV -> dE/dI -> forces (:350-446), including the atom's own
+coordinates x, y, z in computed values and single-particle terms (the explicit gradients and
+dE/dV_j dV_j/dx, :255-259, :413-435).  The expressions are parsed by sympy and differentiated
+symbolically, independently of the library's dual-number code generation.  Tabulated
+functions (CharmmReferenceKernels.cpp:187-191 hands them to OpenMM's tabulated-function
+classes, which are not in the reference tree) follow OpenMM's documented behaviour:
+Continuous1DFunction a natural (or periodic) cubic spline through values equally spaced on
+[min, max], zero outside it when not periodic, evaluated with scipy's CubicSpline;
+Discrete1DFunction values[round(x)], round half away from zero.
 """
 from __future__ import annotations
 
@@ -17,6 +23,7 @@ import re
 
 import numpy as np
 import sympy as sp
+from scipy.interpolate import CubicSpline
 
 from . import CUTOFF_PERIODIC, NO_CUTOFF, Params, _dp, lib
 
@@ -37,9 +44,59 @@ def _functions():
     }
 
 
-def parse(expr: str, names) -> sp.Expr:
+class Tabulated:
+    """A tabulated function of one argument: kind 0 Continuous1DFunction (values equally
+    spaced on [min, max], periodic or natural spline), kind 1 Discrete1DFunction."""
+
+    def __init__(self, name, kind, values, min=0.0, max=1.0, periodic=False):
+        self.name, self.kind = name, int(kind)
+        self.values = np.asarray(values, dtype=np.float64)
+        self.min, self.max, self.periodic = float(min), float(max), bool(periodic)
+
+    def numeric(self):
+        """(f, f') as numpy callables."""
+        if self.kind == 1:
+            vals = self.values
+
+            def f(t):
+                t = np.asarray(t, dtype=np.float64)
+                k = (np.sign(t) * np.floor(np.abs(t) + 0.5)).astype(np.int64)
+                if np.any((k < 0) | (k >= len(vals))):
+                    raise ValueError(f"{self.name}: argument out of range")
+                return vals[k]
+
+            return f, lambda t: np.zeros_like(np.asarray(t, dtype=np.float64))
+        xs = np.linspace(self.min, self.max, len(self.values))
+        cs = CubicSpline(xs, self.values, bc_type="periodic" if self.periodic else "natural")
+        lo, hi = self.min, self.max
+
+        def make(nu):
+            def g(t):
+                t = np.asarray(t, dtype=np.float64)
+                if self.periodic:
+                    return cs(t - (hi - lo) * np.floor((t - lo) / (hi - lo)), nu)
+                inside = (t >= lo) & (t <= hi)
+                return np.where(inside, cs(np.clip(t, lo, hi), nu), 0.0)
+            return g
+
+        return make(0), make(1)
+
+    def sympy_function(self):
+        f, df = self.numeric()
+        dcls = sp.Function(self.name + "__d")
+        dcls._imp_ = staticmethod(df)
+
+        def fdiff(obj, argindex=1):
+            return dcls(obj.args[0])
+
+        return type(self.name, (sp.Function,), {"fdiff": fdiff, "_imp_": staticmethod(f)})
+
+
+def parse(expr: str, names, functions=()) -> sp.Expr:
     """Lepton syntax -> sympy ('^' is the power operator)."""
     local = dict(_functions())
+    for t in functions:
+        local[t.name] = t.sympy_function()
     for n in names:
         local[n] = sp.Symbol(n)
     text = re.sub(r"\^", "**", expr)
@@ -68,11 +125,12 @@ def integral_chain(params: Params, pos, radius, dEdI=None):
 
 
 def compute(params: Params, pos, param_names, param_values, radius_param, integrals, values, terms, globals_=None,
-            exclusions=None):
+            exclusions=None, functions=()):
     """Energy, forces and computed values of a general CharmmGBMVForce.
 
     params: oracle Params (integral type, quadrature, grid, orders, method, cutoff, box);
-    integrals: names (1 or 2); values: [(name, expr)]; terms: [(expr, type)]."""
+    integrals: names (1 or 2); values: [(name, expr)]; terms: [(expr, type)]; functions:
+    Tabulated functions."""
     pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
     n = pos.shape[0]
     P = np.asarray(param_values, dtype=np.float64).reshape(n, len(param_names))
@@ -86,15 +144,19 @@ def compute(params: Params, pos, param_names, param_values, radius_param, integr
     psym = [sp.Symbol(k) for k in param_names]
     isym = [sp.Symbol(k) for k in integrals]
     vsym = [sp.Symbol(k) for k, _ in values]
+    xyz = list(sp.symbols("x y z"))
+    fn = list(functions)
 
-    # computed values, in order, and their total derivatives dV_k/dI_m
+    # computed values, in order, and their total derivatives dV_k/dI_m and dV_k/dx_d
     V = np.zeros((nV, n))
     dVdI = np.zeros((nV, 2, n))
-    args = psym + isym + vsym
-    cols = [P[:, k] for k in range(len(psym))] + [I[m] for m in range(nI)]
+    dVdX = np.zeros((nV, 3, n))
+    args = xyz + psym + isym + vsym
+    cols = [pos[:, 0], pos[:, 1], pos[:, 2]] + [P[:, k] for k in range(len(psym))] + [I[m] for m in range(nI)]
     for k, (name, expr) in enumerate(values):
-        e = parse(expr, list(param_names) + list(G) + list(integrals) + [v for v, _ in values[:k]]).subs(gsub)
-        a = args[: len(psym) + nI + k]
+        e = parse(expr, ["x", "y", "z"] + list(param_names) + list(G) + list(integrals) + [v for v, _ in values[:k]],
+                  fn).subs(gsub)
+        a = args[: 3 + len(psym) + nI + k]
         vals_in = cols + [V[j] for j in range(k)]
         V[k] = np.broadcast_to(_lam(a, e)(*vals_in), (n,))
         for m in range(nI):
@@ -102,26 +164,37 @@ def compute(params: Params, pos, param_names, param_values, radius_param, integr
             for j in range(k):
                 tot = tot + np.broadcast_to(_lam(a, sp.diff(e, vsym[j]))(*vals_in), (n,)) * dVdI[j, m]
             dVdI[k, m] = tot
+        for d in range(3):
+            tot = np.broadcast_to(_lam(a, sp.diff(e, xyz[d]))(*vals_in), (n,)).astype(float)
+            for j in range(k):
+                tot = tot + np.broadcast_to(_lam(a, sp.diff(e, vsym[j]))(*vals_in), (n,)) * dVdX[j, d]
+            dVdX[k, d] = tot
 
-    # single-particle terms: energy and total dE/dI
+    # single-particle terms: energy, total dE/dI, explicit coordinate gradients (the force's
+    # first part, CharmmReferenceGBMV.cpp:257-259) and dE/dV_j (chained through dV_j/dx below)
     E = 0.0
     dEdI = np.zeros((2, n))
-    sp_args = psym + isym + vsym
+    forces = np.zeros((n, 3))
+    dEdV = np.zeros((nV, n))
+    sp_args = xyz + psym + isym + vsym
     sp_in = cols + [V[j] for j in range(nV)]
     for expr, ttype in terms:
         if ttype != SINGLE:
             continue
-        e = parse(expr, list(param_names) + list(G) + list(integrals) + [v for v, _ in values]).subs(gsub)
+        e = parse(expr, ["x", "y", "z"] + list(param_names) + list(G) + list(integrals) + [v for v, _ in values],
+                  fn).subs(gsub)
         E += float(np.sum(np.broadcast_to(_lam(sp_args, e)(*sp_in), (n,))))
         for m in range(nI):
             tot = np.broadcast_to(_lam(sp_args, sp.diff(e, isym[m]))(*sp_in), (n,)).astype(float)
             for j in range(nV):
                 tot = tot + np.broadcast_to(_lam(sp_args, sp.diff(e, vsym[j]))(*sp_in), (n,)) * dVdI[j, m]
             dEdI[m] += tot
+        for d in range(3):
+            forces[:, d] -= np.broadcast_to(_lam(sp_args, sp.diff(e, xyz[d]))(*sp_in), (n,))
+        for j in range(nV):
+            dEdV[j] += np.broadcast_to(_lam(sp_args, sp.diff(e, vsym[j]))(*sp_in), (n,))
 
     # pair terms over i < j (atom1 = i)
-    forces = np.zeros((n, 3))
-    dEdV = np.zeros((nV, n))
     iu, ju = np.triu_indices(n, 1)
     d = pos[ju] - pos[iu]  # x_2 - x_1
     if params.method == CUTOFF_PERIODIC:
@@ -135,6 +208,7 @@ def compute(params: Params, pos, param_names, param_values, radius_param, integr
         key = set((ex[:, 0] * n + ex[:, 1]).tolist())
         excluded = np.fromiter(((a * n + b) in key for a, b in zip(iu, ju)), dtype=bool, count=len(iu))
     r = np.sqrt(r2)
+    dEdV_pair = np.zeros((nV, n))
     rs = sp.Symbol("r")
     p1 = [sp.Symbol(k + "1") for k in param_names]
     p2 = [sp.Symbol(k + "2") for k in param_names]
@@ -145,7 +219,7 @@ def compute(params: Params, pos, param_names, param_values, radius_param, integr
     for expr, ttype in terms:
         if ttype == SINGLE:
             continue
-        e = parse(expr, pair_names).subs(gsub)
+        e = parse(expr, pair_names, fn).subs(gsub)
         m = keep & (~excluded if ttype == PAIR else True)
         a, b = iu[m], ju[m]
         ins = [r[m]] + [P[a, k] for k in range(len(p1))] + [P[b, k] for k in range(len(p2))] + \
@@ -157,12 +231,16 @@ def compute(params: Params, pos, param_names, param_values, radius_param, integr
         np.add.at(forces, a, g)
         np.add.at(forces, b, -g)
         for k in range(nV):
-            np.add.at(dEdV[k], a, np.broadcast_to(_lam(pair_args, sp.diff(e, v1[k]))(*ins), (cnt,)))
-            np.add.at(dEdV[k], b, np.broadcast_to(_lam(pair_args, sp.diff(e, v2[k]))(*ins), (cnt,)))
+            np.add.at(dEdV_pair[k], a, np.broadcast_to(_lam(pair_args, sp.diff(e, v1[k]))(*ins), (cnt,)))
+            np.add.at(dEdV_pair[k], b, np.broadcast_to(_lam(pair_args, sp.diff(e, v2[k]))(*ins), (cnt,)))
 
+    # dE/dV_k: the pair terms' part feeds dE/dI (the single-particle part is already in the
+    # total dE/dI above); both feed the coordinate chain rule dE/dV_k dV_k/dx
     for m in range(nI):
         for k in range(nV):
-            dEdI[m] += dEdV[k] * dVdI[k, m]
+            dEdI[m] += dEdV_pair[k] * dVdI[k, m]
+    for k in range(nV):
+        forces -= (dEdV[k] + dEdV_pair[k])[:, None] * dVdX[k].T
     _, fchain = integral_chain(params, pos, radius, dEdI)
     forces += fchain
     return float(E), forces, V, I

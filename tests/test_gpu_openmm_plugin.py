@@ -249,3 +249,70 @@ def test_python_kernel_mirror_context_parameters_and_derivatives():
     kg.copyParametersToContext(cg, g)
     e15 = kg.execute(cg, False, True)
     assert abs(e15 - 2.25 * e) < 1e-6 * abs(e)
+
+
+def test_cpp_kernel_classes_tabulated_functions_and_coordinates(tmp_path):
+    """Driver model 4: a CharmmGBMVForce with a Continuous1DFunction, a Discrete1DFunction and
+    the coordinates x, y, z in its Born value and single-particle term, through the C++ kernel
+    class (OpenMM's tabulated-function objects converted to the C ABI's tables)."""
+    from oracle import custom as C
+
+    s = S.globule(300, 45)
+    e, f, rest = _run(tmp_path, 4, 2, 0, s)
+    p = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV2)
+    p.orders[0] = 4
+    P = -138.935456 * (1.0 - 1.0 / 80.0)
+    scale = C.Tabulated("scale", 0, 1.0 + 0.2 * np.sin(0.3 * np.arange(31)), 0.0, 3.0)
+    w = C.Tabulated("w", 1, [0.9, 1.1])
+    E, F, _, _ = C.compute(oracle_params(p), s.pos, ["q", "radius"], np.stack([s.charge, s.radius], 1), 1, ["I4"],
+                           [("B", "scale(10*radius)*(1+0.002*(x+y-z))/(1/radius-0.5*I4*radius)")],
+                           [("0.5*P*q^2/B*w(step(q))+0.01*q^2*(x^2+y^2+z^2)", C.SINGLE),
+                            ("P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))", C.PAIR_NO_EXCLUSIONS)], {"P": P},
+                           functions=[scale, w])
+    print(f"[openmm-plugin] tables + x,y,z N={s.n}: energy {e:.6f} vs {E:.6f} force {rel_rms(f, F):.2e}", flush=True)
+    assert rel_err(e, E) < 1e-5
+    assert rel_rms(f, F) < 1e-5
+
+
+def test_python_kernel_mirror_tabulated_functions():
+    """forces.CharmmGBMVForce.addTabulatedFunction through the Python kernel mirror; a 2D table
+    is refused with OpenMMException."""
+    from oracle import custom as C
+    from openmmcustomgbintegralforce_amd import forces as FM
+
+    s = S.globule(250, 46)
+    P = -138.935456 * (1.0 - 1.0 / 80.0)
+    vals = 1.0 + 0.2 * np.sin(0.3 * np.arange(31))
+
+    def force(fn):
+        f = FM.CharmmGBMVForce()
+        f.addPerParticleParameter("q")
+        f.addPerParticleParameter("radius")
+        f.addGlobalParameter("P", P)
+        f.addGBIntegral("I4", [4])
+        f.setGBIntegralType(FM.CharmmGBMVForce.GBMVIntegralTypeII)
+        f.addTabulatedFunction("scale", fn)
+        f.addComputedValue("B", "scale(10*radius)/(1/radius-0.5*I4*radius)", FM.CharmmGBMVForce.SingleParticle)
+        f.addEnergyTerm("0.5*P*q^2/B+0.01*q^2*z^2", FM.CharmmGBMVForce.SingleParticle)
+        f.addEnergyTerm("P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))", FM.CharmmGBMVForce.ParticlePairNoExclusions)
+        for q, r in zip(s.charge, s.radius):
+            f.addParticle([q, r])
+        return f
+
+    k = FM.HipCalcCharmmGBMVForceKernel()
+    f1 = force(FM.Continuous1DFunction(vals, 0.0, 3.0))
+    k.initialize(FM.System(s.n), f1)
+    ctx = FM.ContextImpl(s.pos, forces=(f1,))
+    e = k.execute(ctx, True, True)
+    p = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV2)
+    p.orders[0] = 4
+    E, F, _, _ = C.compute(oracle_params(p), s.pos, ["q", "radius"], np.stack([s.charge, s.radius], 1), 1, ["I4"],
+                           [("B", "scale(10*radius)/(1/radius-0.5*I4*radius)")],
+                           [("0.5*P*q^2/B+0.01*q^2*z^2", C.SINGLE),
+                            ("P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))", C.PAIR_NO_EXCLUSIONS)], {"P": P},
+                           functions=[C.Tabulated("scale", 0, vals, 0.0, 3.0)])
+    print(f"[openmm-plugin] python mirror tables: energy {e:.6f} vs {E:.6f}", flush=True)
+    assert rel_err(e, E) < 1e-5 and rel_rms(ctx.forces, F) < 1e-5
+    with pytest.raises(FM.OpenMMException):
+        FM.HipCalcCharmmGBMVForceKernel().initialize(
+            FM.System(s.n), force(FM.Continuous2DFunction(2, 2, [1.0] * 4, 0.0, 1.0, 0.0, 1.0)))

@@ -7,6 +7,11 @@
 * forces = -grad E by central differences for non-canonical expressions (two computed
   values, a value depending on an earlier one, single-particle terms using integrals and
   values, a pair term using parameters and values of both atoms, globals, one integral);
+* the tabulated functions reproduce what a spline fixes: a natural spline through samples of
+  a linear function is that function (and zero outside the range), a periodic spline through
+  samples of sin is sin to the sampling error, a discrete table rounds half away from zero;
+* forces = -grad E by central differences with the coordinates x, y, z in computed values and
+  single-particle terms and with tabulated functions in every kind of expression;
 * the library rejects malformed expressions before any device work (status 1).
 """
 import ctypes
@@ -105,3 +110,66 @@ def test_library_rejects_malformed_expressions():
     assert _create(dict(values=[("r", "I4")])) == 1                        # reserved name
     # well formed: only the device is missing here (or the context is created on a GPU)
     assert _create({}) in (0, 8)
+
+
+# expressions using the atom's coordinates and tabulated functions
+SCALE = C.Tabulated("scale", 0, 1.0 + 0.2 * np.sin(0.3 * np.arange(31)), 0.0, 3.0)
+WAVE = C.Tabulated("wave", 0, np.cos(np.linspace(0.0, 2 * np.pi, 41)), 0.0, 2 * np.pi, periodic=True)
+W = C.Tabulated("w", 1, [0.9, 1.1])
+TAB_VALUES = [("B", "scale(10*radius)*(1+0.002*(x+y-z))/(1/radius-0.5*I4*radius)")]
+TAB_TERMS = [("0.5*P*q^2/B*w(step(q))+0.01*q^2*(x^2+y^2+z^2)+0.05*q^2*wave(3*x)", C.SINGLE),
+             ("P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))*(1+0.05*wave(r))", C.PAIR_NO_EXCLUSIONS)]
+
+
+def test_tabulated_functions_pinned_by_closed_forms():
+    lin = C.Tabulated("lin", 0, 0.5 + 2.0 * np.linspace(-1.0, 2.0, 7), -1.0, 2.0)
+    f, df = lin.numeric()
+    t = np.array([-1.0, -0.37, 0.0, 1.234, 2.0])
+    assert np.allclose(f(t), 0.5 + 2.0 * t, atol=1e-13) and np.allclose(df(t), 2.0, atol=1e-12)
+    assert f(np.array([-1.01, 2.01])).tolist() == [0.0, 0.0]  # zero outside [min, max]
+    per = C.Tabulated("per", 0, np.sin(np.linspace(0.0, 2 * np.pi, 201)), 0.0, 2 * np.pi, periodic=True)
+    f, df = per.numeric()
+    t = np.array([-7.0, -0.3, 0.1, 3.0, 9.9])  # wrapped by the period
+    assert np.allclose(f(t), np.sin(t), atol=2e-6) and np.allclose(df(t), np.cos(t), atol=2e-4)
+    f, df = C.Tabulated("d", 1, [10.0, 20.0, 30.0]).numeric()
+    assert f(np.array([0.49, 0.5, 1.5, 2.4])).tolist() == [10.0, 20.0, 30.0, 30.0]
+    with pytest.raises(ValueError):
+        f(np.array([2.5]))
+
+
+def _tabulated(s, pos):
+    pv = np.stack([s.charge, s.radius], 1)
+    p = O.gbmv_params(2)
+    p.orders[0] = 4
+    return C.compute(p, pos, ["q", "radius"], pv, 1, ["I4"], TAB_VALUES, TAB_TERMS, dict(G),
+                     functions=[SCALE, WAVE, W])
+
+
+def test_coordinates_and_tabulated_functions_forces_are_the_gradient():
+    s = S.globule(80, 9)
+    E, F, V, _ = _tabulated(s, s.pos)
+    rng = np.random.default_rng(6)
+    h = 1e-6
+    for _ in range(6):
+        i, d = rng.integers(s.n), rng.integers(3)
+        pp, pm = s.pos.copy(), s.pos.copy()
+        pp[i, d] += h
+        pm[i, d] -= h
+        fd = -(_tabulated(s, pp)[0] - _tabulated(s, pm)[0]) / (2 * h)
+        assert abs(fd - F[i, d]) <= 2e-5 * max(1.0, abs(F[i, d])), (i, d, fd, F[i, d])
+
+
+def test_library_rejects_bad_tables_and_coordinates():
+    tab = N.Tabulated("scale", N.TAB_CONTINUOUS_1D, [1.0, 1.2, 1.1], 0.0, 3.0)
+    val = [("B", "scale(10*radius)/(1/radius-0.5*I4*radius)")]
+    assert _create(dict(values=val, functions=[N.Tabulated("scale", 2, [1.0, 2.0])])) == 2  # 2D/3D: unsupported
+    assert _create(dict(values=val, functions=[N.Tabulated("scale", 0, [1.0, 2.0, 1.5], 0, 3, True)])) == 1
+    assert _create(dict(values=val, functions=[N.Tabulated("scale", 0, [1.0, 2.0], 3.0, 3.0)])) == 1  # min < max
+    assert _create(dict(values=[("B", "scale(I4, I7)")], functions=[tab])) == 1   # one argument
+    assert _create(dict(values=val, functions=[tab, tab])) == 1                    # defined twice
+    assert _create(dict(terms=[SELF, ("P*q1*q2/r*x", C.PAIR)])) == 1               # x is not a pair variable
+    assert _create(dict(param_names=["q", "x"])) == 1                              # reserved coordinate
+    # well formed: only the device is missing here (or the context is created on a GPU)
+    assert _create(dict(values=[("B", "scale(10*radius)*(1+0.01*x)/(1/radius-0.5*I4*radius)")],
+                        terms=[("0.5*P*q^2/B+0.01*z^2", C.SINGLE), (STILL, C.PAIR_NO_EXCLUSIONS)],
+                        functions=[tab])) in (0, 8)
