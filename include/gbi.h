@@ -72,7 +72,8 @@ typedef enum gbi_lookup_policy {
  * reaches only atoms j within the cutoff of i (CharmmReferenceGBSW.cpp:285-296), so its
  * forces are not the exact gradient; GBI_MODEL_GBMV's chain rule is complete.  Periodic
  * systems take minimum-image distances everywhere and tile their rectangular box with the
- * lookup grid (gbi_set_periodic_box).  The adjoint (GBI_ADJOINT) is NoCutoff only.
+ * lookup grid (gbi_set_periodic_box).  The adjoint (GBI_ADJOINT) differentiates the energy
+ * under every method (GBI_MODEL_GBSW's truncation shapes its forces, not dE/dR or dE/dtheta).
  */
 typedef enum gbi_nonbonded_method {
     GBI_NO_CUTOFF = 0,

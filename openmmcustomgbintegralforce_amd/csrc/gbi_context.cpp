@@ -899,7 +899,6 @@ gbi_status run(gbi_ctx& c, uint32_t flags) {
                 return fail(GBI_ERR_INVALID_ARGUMENT, "CutoffPeriodic system without a periodic box (gbi_set_periodic_box)");
             ST_TRY(check_box(c, s, s.box));
         }
-    if (adjoint && c.any_cut) return fail(GBI_ERR_UNSUPPORTED, "the adjoint is implemented for NoCutoff systems only");
     if (adjoint && c.cprog) return fail(GBI_ERR_UNSUPPORTED, "the adjoint covers the canonical GB forms only");
     if (c.cell_total == 0) ST_TRY(size_from_device(c, 0));
     if (adjoint) ST_TRY(ensure_adjoint_buffers(c));

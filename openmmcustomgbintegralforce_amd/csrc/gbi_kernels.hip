@@ -1696,7 +1696,8 @@ __device__ inline float sum8(float v) {
 
 // TRUNC (GBSW): the standalone force with a cutoff applies dB_i/dx_j only to list atoms j
 // within the cutoff of the owner i (minimum image); the owner still takes the whole sum
-// (CharmmReferenceGBSW.cpp:285-296).
+// (CharmmReferenceGBSW.cpp:285-296).  The truncation shapes the forces only: with ADJ the
+// radius and constant derivatives, which differentiate the energy, take every list atom.
 template <int VK, bool ADJ, bool TRUNC>
 __global__ void __launch_bounds__(64, (ADJ || TRUNC) ? 2 : 3) k_force(KParams p) {
     __shared__ float q_x[kWave], q_y[kWave], q_z[kWave];
@@ -2099,8 +2100,8 @@ template <bool ADJ>
 void launch_force(const KParams& p, int vk, int blocks, hipStream_t stream) {
     switch (vk) {
     case VOL_GBSW:
-        if (!ADJ && p.any_trunc)
-            hipLaunchKernelGGL((k_force<VOL_GBSW, false, true>), dim3(blocks), dim3(64), 0, stream, p);
+        if (p.any_trunc)
+            hipLaunchKernelGGL((k_force<VOL_GBSW, ADJ, true>), dim3(blocks), dim3(64), 0, stream, p);
         else
             hipLaunchKernelGGL((k_force<VOL_GBSW, ADJ, false>), dim3(blocks), dim3(64), 0, stream, p);
         break;
@@ -2172,8 +2173,7 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
         err = hipModuleLaunchKernel(c.chain, nb, 1, 1, c.chain_threads, 1, 1, 0, stream, ca, nullptr);
         if (err != hipSuccess) return err;
     } else if (adj) {
-        // the host refuses the adjoint of cutoff systems; exclusions take the general path
-        if (p.any_excl) {
+        if (p.any_cut || p.any_excl) {
             hipLaunchKernelGGL(k_block_box, dim3(p.n_colblocks), dim3(64), 0, stream, p);
             hipLaunchKernelGGL((k_pairs<true, true>), dim3(p.n_tiles), dim3(64), 0, stream, p);
         } else {

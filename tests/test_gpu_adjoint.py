@@ -97,12 +97,17 @@ def _check(tag, gpu, fd, tol=TOL, tol_elem=TOL_ELEM):
     assert elem < max(tol, tol_elem), tag
 
 
-def _run(p, s, model, r_atoms, q_atoms, theta_names, tag):
+def _run(p, s, model, r_atoms, q_atoms, theta_names, tag, box=None):
     print(f"[adjoint] {tag} N={s.n}", flush=True)
     eng = N.Engine(p, s.charge, s.radius)
+    if box is not None:
+        eng.set_periodic_box(box)
     out = eng.execute_adjoint(s.pos)
+    # the adjoint evaluation's forces are the plain evaluation's, bit for bit
+    _, f = eng.execute(s.pos)
     eng.close()
-    op = oracle_params(p)
+    assert np.array_equal(out["forces"], f), tag
+    op = oracle_params(p, box=box)
     ref_e = _energy(op, s.pos, s.charge, s.radius)
     assert abs(out["energy"] - ref_e) < 1e-5 * abs(ref_e)
     # size-independent identities: E is homogeneous of degree 2 in q and linear in P
@@ -199,3 +204,34 @@ def test_adjoint_batch_matches_single():
         assert np.allclose(out["dEdq"][a:b], o1["dEdq"], rtol=1e-9, atol=1e-9 * np.max(np.abs(o1["dEdq"]))), k
         for name in N.THETA:
             assert abs(out["dEdtheta"][name][k] - o1["dEdtheta"][name]) <= 1e-9 * max(1.0, abs(o1["dEdtheta"][name]))
+
+
+def test_adjoint_gbsw_standalone_cutoff():
+    """CharmmGBSWForce with CutoffNonPeriodic: pairs within the cutoff, the chain rule truncated
+    at it (forces only); the energy's derivatives take every list atom."""
+    s = S.globule(200, 32)
+    p = N.default_params(N.MODEL_GBSW)
+    p.nonbonded_method, p.cutoff = N.CUTOFF_NON_PERIODIC, 1.0
+    _run(p, s, "gbsw", np.arange(0, 200, 7), np.arange(0, 200, 10), MODEL_THETA["gbsw"], "gbsw cutoff")
+
+
+def test_adjoint_gbsw_standalone_periodic():
+    s = S.globule(220, 33)
+    L = np.array([3.4, 3.5, 3.6])
+    s.pos = s.pos - s.pos.mean(axis=0) + L / 2
+    p = N.default_params(N.MODEL_GBSW)
+    p.nonbonded_method, p.cutoff = N.CUTOFF_PERIODIC, 1.1
+    _run(p, s, "gbsw", np.arange(0, 220, 8), np.arange(0, 220, 11), ["prefactor", "alpha0", "eta", "sw", "rscale"],
+         "gbsw periodic", box=L)
+
+
+def test_adjoint_gbmv2_periodic_constants():
+    """CharmmGBMVForce GBMV-II in a periodic box on its reference lookup grid: charges and the
+    constants that do not move the grid's membership."""
+    s = S.globule(300, 34)
+    L = np.array([3.8, 3.9, 4.0])
+    s.pos = s.pos - s.pos.mean(axis=0) + L / 2
+    p = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV2)
+    p.n_lebedev = 50
+    p.nonbonded_method, p.cutoff = N.CUTOFF_PERIODIC, 1.2
+    _run(p, s, "gbmv2", None, np.arange(0, 300, 30), MODEL_THETA["gbmv2"], "gbmv2 periodic", box=L)
