@@ -16,7 +16,11 @@ Prints one JSON line (rank 0).  Beside the throughput it reports:
   from HIP events on the library's stream (the binning pass includes its scan and worklist
   launches); ``traffic`` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summaries;
 * ``cpu_baseline``: the oracle in the reference's loop structure (dense, FP64, 1 thread)
-  timed on this host over a bounded sample.
+  timed on this host over a bounded sample;
+* ``extra`` (default workload only; ``--no-extras`` skips it): evals/s of the 25,000-atom
+  GBSW globule, and the 64-system mix batched in one context against the same systems as
+  per-system launches (sequential on one stream, concurrent on 64 streams, and plain
+  gbi_execute calls with positions in and forces out).
 """
 from __future__ import annotations
 
@@ -68,7 +72,8 @@ WORKLOADS = {
                     desc="64 systems in one batched context, N ~ U[300, 6000] (seeded), alternating "
                          "CharmmGBSWForce 24x38 and CharmmGBMVForce GBMVIntegralTypeII 24x50, hipGraph replay; "
                          "beside it the same 64 systems as 64 separate contexts, enqueued back to back on one "
-                         "stream (sequential) and on one stream each (concurrent)"),
+                         "stream (sequential), on one stream each (concurrent) and through gbi_execute one "
+                         "after another (plain)"),
 }
 
 
@@ -83,6 +88,8 @@ def parse():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 accuracy check")
     ap.add_argument("--no-baseline", action="store_true",
                     help="batch64: skip the per-system contexts (for profiling the batched launch alone)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="gbmv2-5000: skip the extra keys (gbsw-25000 evals/s, batch64 against per-system launches)")
     return ap.parse_args()
 
 
@@ -274,6 +281,13 @@ def main() -> int:
                    "n_chain_pairs_visited": stats["n_frc_pairs"], "n_cells": stats["n_cells"]},
     }
     eng.close()
+    if args.workload == "gbmv2-5000" and not args.no_extras:
+        # the other headline configurations, timed in the same run: the 25,000-atom GBSW
+        # globule, and the 64-system mix batched against its per-system launches
+        bres = measure_batch(max(4, args.steps // 2), 2)
+        out["extra"] = {"gbsw-25000": measure_single("gbsw-25000", args.steps, args.warmup),
+                        "batch64": {**batch_summary(bres), "n_atoms": int(np.sum(bres["sizes"])),
+                                    "ms_per_round": 1e3 * bres["batch"]}}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -374,12 +388,16 @@ def run_adjoint(args, w) -> int:
     return 0
 
 
-def run_batch(args, w) -> int:
-    """One step = one evaluation of all 64 systems; value = system evaluations per second."""
+def measure_batch(steps, warmup, baselines=True, accuracy=False, seed=0):
+    """The 64-system mix, batched (one context, graph replay) and, with `baselines`, the same
+    systems as 64 contexts: enqueued back to back on one stream (sequential: device-resident
+    positions, one synchronisation per round), on one stream each (concurrent), and evaluated
+    one after another through gbi_execute (plain: positions in, energy and forces out per
+    system).  Returns seconds per round of 64 system evaluations and the accuracy check."""
     from openmmcustomgbintegralforce_amd import _native as N
     from openmmcustomgbintegralforce_amd import synthetic as S
 
-    sizes = S.batch_sizes(64, 300, 6000, seed=w["seed"])
+    sizes = S.batch_sizes(64, 300, 6000, seed=seed)
     systems = [S.globule(int(n), 1000 + k) for k, n in enumerate(sizes)]
     params = []
     for k in range(64):
@@ -389,61 +407,58 @@ def run_batch(args, w) -> int:
             p = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV2)
             p.n_lebedev = 50
         params.append(p)
-    pos = np.concatenate([s.pos + 40.0 * k for k, s in enumerate(systems)])
+    shifted = [s.pos + 40.0 * k for k, s in enumerate(systems)]
+    pos = np.concatenate(shifted)
 
-    def timed(step_fn, sync_fn):
-        for _ in range(args.warmup):
+    def timed(step_fn, sync_fn, n_steps=steps):
+        for _ in range(warmup):
             step_fn()
         sync_fn()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(n_steps):
             step_fn()
         sync_fn()
-        return time.perf_counter() - t0
+        return (time.perf_counter() - t0) / n_steps
 
-    # batched: one context, every pass one launch over all systems, replayed from a graph
+    res = {"sizes": sizes}
     eng = N.Engine.batch(params, [s.charge for s in systems], [s.radius for s in systems])
     e_b, f_b = eng.execute(pos)
     eng.set_graph(True)
     eng.set_positions(pos)
     eng.compute()
     eng.synchronize()
-    t_batch = timed(eng.compute, eng.synchronize)
+    res["batch"] = timed(eng.compute, eng.synchronize)
     eng.close()
+    if baselines:
+        engines = [N.Engine(p, s.charge, s.radius) for p, s in zip(params, systems)]
+        for e1, x in zip(engines, shifted):
+            e1.execute(x)
+            e1.set_positions(x)
 
-    if args.no_baseline:
-        print(json.dumps({"metric": METRIC, "value": 64 * args.steps / t_batch, "unit": "evals/s", "n_gpus": 1,
-                          "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t_batch / args.steps,
-                          "config": {"workload": args.workload, "systems": 64}}), flush=True)
-        return 0
+        def seq_step():
+            for e1 in engines:
+                e1.compute()
 
-    # the same systems one context each, enqueued back to back, one synchronisation per round
-    engines = [N.Engine(p, s.charge, s.radius) for p, s in zip(params, systems)]
-    for k, (e1, s) in enumerate(zip(engines, systems)):
-        e1.execute(s.pos + 40.0 * k)
-        e1.set_positions(s.pos + 40.0 * k)
+        def seq_sync():
+            for e1 in engines:
+                e1.synchronize()
 
-    def seq_step():
+        res["concurrent"] = timed(seq_step, seq_sync)
+        shared = engines[0].device_view().stream
+        for e1 in engines[1:]:
+            e1.set_stream(shared)
+        res["sequential"] = timed(seq_step, seq_sync)
         for e1 in engines:
-            e1.compute()
+            e1.set_stream(None)
 
-    def seq_sync():
+        def plain_step():
+            for e1, x in zip(engines, shifted):
+                e1.execute(x)
+
+        res["plain"] = timed(plain_step, lambda: None, max(1, steps // 4))
         for e1 in engines:
-            e1.synchronize()
-
-    # 64 streams (one per context): the systems overlap on the device
-    t_conc = timed(seq_step, seq_sync)
-    # one shared stream: 64 sequential per-system evaluations
-    shared = engines[0].device_view().stream
-    for e1 in engines[1:]:
-        e1.set_stream(shared)
-    t_seq = timed(seq_step, seq_sync)
-    for e1 in engines:
-        e1.set_stream(None)
-        e1.close()
-
-    acc = {}
-    if not args.no_accuracy:
+            e1.close()
+    if accuracy:
         import oracle
 
         offs = np.concatenate([[0], np.cumsum(sizes)])
@@ -456,19 +471,35 @@ def run_batch(args, w) -> int:
             worst_f = max(worst_f, float(np.linalg.norm(f - ref.forces) / np.linalg.norm(ref.forces)))
             worst_e = max(worst_e, float(abs(e_b[k] - ref.energy) / abs(ref.energy)))
             checked.append(int(sizes[k]))
-        acc = {"force_rel_rms": worst_f, "energy_rel_err": worst_e, "accuracy_systems": checked}
+        res["accuracy"] = {"force_rel_rms": worst_f, "energy_rel_err": worst_e, "accuracy_systems": checked}
+    return res
 
-    value = 64 * args.steps / t_batch
-    seq_value = 64 * args.steps / t_seq
-    conc_value = 64 * args.steps / t_conc
+
+def batch_summary(res) -> dict:
+    out = {"batch_evals_per_s": 64 / res["batch"]}
+    if "sequential" in res:
+        out.update({"sequential_evals_per_s": 64 / res["sequential"],
+                    "concurrent_evals_per_s": 64 / res["concurrent"],
+                    "plain_evals_per_s": 64 / res["plain"],
+                    "batch_speedup": res["sequential"] / res["batch"],
+                    "batch_speedup_vs_concurrent": res["concurrent"] / res["batch"],
+                    "batch_speedup_vs_plain": res["plain"] / res["batch"]})
+    return out
+
+
+def run_batch(args, w) -> int:
+    """One step = one evaluation of all 64 systems; value = system evaluations per second."""
+    res = measure_batch(args.steps, args.warmup, baselines=not args.no_baseline, accuracy=not args.no_accuracy,
+                        seed=w["seed"])
+    sizes = res["sizes"]
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": 64 / res["batch"],
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * t_batch / args.steps,
+        "ms_per_step": 1e3 * res["batch"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -476,16 +507,37 @@ def run_batch(args, w) -> int:
         "data": "synthetic: 64 seeded globules at 100 atoms/nm^3, sizes synthetic.batch_sizes(64, 300, 6000, seed 0)",
         "config": {"workload": args.workload, "description": w["desc"], "n_atoms": int(np.sum(sizes)),
                    "systems": 64},
-        **acc,
-        "sequential_evals_per_s": seq_value,
-        "batch_speedup": value / seq_value,
-        "concurrent_evals_per_s": conc_value,
-        "batch_speedup_vs_concurrent": value / conc_value,
+        **res.get("accuracy", {}),
+        **batch_summary(res),
         "roofline": None,
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def measure_single(workload, steps, warmup) -> dict:
+    """Evaluations per second of one more single-system workload (positions resident)."""
+    from openmmcustomgbintegralforce_amd import _native as N
+    from openmmcustomgbintegralforce_amd import synthetic as S
+
+    w = WORKLOADS[workload]
+    sysm = S.globule(w["n_atoms"], w["seed"])
+    p = N.default_params(N.MODEL_GBSW) if w["model"] == "gbsw" else N.default_params(N.MODEL_GBMV, w["integral"])
+    p.n_lebedev = w["n_lebedev"]
+    eng = N.Engine(p, sysm.charge, sysm.radius)
+    eng.execute(sysm.pos)
+    eng.set_positions(sysm.pos)
+    for _ in range(warmup):
+        eng.compute()
+    eng.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.compute()
+    eng.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    eng.close()
+    return {"evals_per_s": 1.0 / dt, "ms_per_eval": 1e3 * dt, "n_atoms": sysm.n, "steps": steps}
 
 
 if __name__ == "__main__":
