@@ -24,7 +24,10 @@ import sys
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 COPY_BYTES = 256 << 20
 
-KERNELS = {"volume": "k_volume", "force": "k_force", "pairs": "k_pairs", "bin": "k_point_bin"}
+# the kernels of each pass (the binning pass: the brick or cell binning kernel and, for brick
+# binning, the split that moves the points into their cells)
+KERNELS = {"volume": ("k_volume",), "force": ("k_force",), "pairs": ("k_pairs",),
+           "bin": ("k_point_bin", "k_point_split")}
 
 
 def cmd_run(args) -> int:
@@ -81,11 +84,19 @@ def _average(per, match):
 def cmd_summarize(args) -> int:
     fetch = _counters(args.fetch_dir)
     write = _counters(args.write_dir)
-    kname = KERNELS[args.kernel]
-    is_kernel = lambda n: kname in n  # noqa: E731
     is_copy = lambda n: ("copy" in n.lower() or "elementwise" in n.lower()) and "gbi" not in n  # noqa: E731
-    f_k, n_f = _average(fetch, is_kernel)
-    w_k, n_w = _average(write, is_kernel)
+    # per evaluation: the sum over the pass's kernels of each one's per-dispatch average
+    f_k = w_k = 0.0
+    n_f, n_w, parts = [], [], {}
+    for kname in KERNELS[args.kernel]:
+        if not any(kname in n for n in fetch):
+            continue  # a kernel the workload does not launch (cell binning has no split)
+        fa, nf = _average(fetch, lambda n, kname=kname: kname in n)
+        wa, nw = _average(write, lambda n, kname=kname: kname in n)
+        f_k, w_k = f_k + fa, w_k + wa
+        n_f.append(nf)
+        n_w.append(nw)
+        parts[kname] = {"fetch_kib": fa, "write_kib": wa}
     # the largest copy-like dispatch is the calibration copy
     f_c = max(x for k, v in fetch.items() if is_copy(k) for x in v)
     w_c = max(x for k, v in write.items() if is_copy(k) for x in v)
@@ -103,6 +114,7 @@ def cmd_summarize(args) -> int:
         "write_factor": w_factor,
         "hbm_bytes_per_launch": f_k * kib * f_factor + w_k * kib * w_factor,
         "launches": [n_f, n_w],
+        "kernels": parts,
         "commands": [
             "rocprofv3 --pmc FETCH_SIZE -f csv -d <dir_f> -o run -- python3 -m openmmcustomgbintegralforce_amd.traffic run "
             f"--workload {args.workload}",
