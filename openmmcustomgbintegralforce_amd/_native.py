@@ -252,6 +252,40 @@ def quadrature(n_radial: int = 24, r0: float = 0.05, r1: float = 2.0, n_lebedev:
     return out
 
 
+def tuning_key(params_list, n_atoms) -> str:
+    """Autotuner cache key: each system's model, integral type, quadrature (radial x Lebedev)
+    and nonbonded method, sorted, and the power-of-two bucket of the total atom count."""
+    kinds = sorted({f"m{p.model}t{p.integral_type}q{p.n_radial}x{p.n_lebedev}c{p.nonbonded_method}"
+                    for p in params_list})
+    total = int(np.sum(n_atoms))
+    bucket = 1 << max(0, int(total - 1).bit_length())
+    return f"{'+'.join(kinds)}|N<={bucket}"
+
+
+def load_tuning(path: str) -> dict:
+    """The autotuner cache (JSON: key -> tuning), empty when the file does not exist."""
+    import json
+
+    if not os.path.exists(path):
+        return {}
+    with open(path) as fh:
+        data = json.load(fh)
+    return data if isinstance(data, dict) else {}
+
+
+def store_tuning(path: str, key: str, tuning: dict) -> None:
+    """Adds or replaces one entry of the autotuner cache (written to a temporary file and
+    renamed, so a reader never sees a partial file)."""
+    import json
+
+    data = load_tuning(path)
+    data[key] = {k: (float(v) if isinstance(v, float) else int(v)) for k, v in tuning.items()}
+    tmp = f"{path}.tmp{os.getpid()}"
+    with open(tmp, "w") as fh:
+        json.dump(data, fh, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
 class Engine:
     """One gbi_ctx: the HIP counterpart of a CalcCharmmGB{SW,MV}ForceKernel instance.
 
@@ -467,10 +501,23 @@ class Engine:
     def set_graph(self, enable: bool = True) -> None:
         _check(self._lib.gbi_set_graph(self._h, int(enable)))
 
-    def autotune(self, evals: int = 3) -> dict:
+    def autotune(self, evals: int = 3, cache: str | None = None) -> dict:
+        """Time candidate launch shapes on the current positions and keep the fastest
+        (gbi_autotune).  With `cache` (a JSON file), a result stored for the same key
+        (tuning_key: the systems' models, integral types and quadratures and the total atom
+        count's power-of-two bucket) is applied without timing, and a new result is stored."""
+        key = tuning_key(self.params, self.n_atoms)
+        if cache is not None:
+            hit = load_tuning(cache).get(key)
+            if hit is not None:
+                self.set_tuning(int(hit["blocks_per_cu"]), int(hit["pair_columns"]), float(hit["tight_grid_length"]))
+                return dict(hit, cached=True)
         t = Tuning()
         _check(self._lib.gbi_autotune(self._h, int(evals), ctypes.byref(t)))
-        return {f: getattr(t, f) for f, _ in Tuning._fields_}
+        out = {f: getattr(t, f) for f, _ in Tuning._fields_}
+        if cache is not None:
+            store_tuning(cache, key, out)
+        return out
 
     def set_tuning(self, blocks_per_cu: int = 128, pair_columns: int = 0, tight_grid_length: float = 0.0) -> None:
         t = Tuning(blocks_per_cu, pair_columns, tight_grid_length, 0.0)

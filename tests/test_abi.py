@@ -178,3 +178,26 @@ def test_gbsw_force_api():
     assert k.Name() == "CalcCharmmGBSWForce"
     with pytest.raises(F.OpenMMException):
         k.initialize(F.System(3), f)
+
+
+def test_autotuner_cache_key_and_store(tmp_path):
+    """The autotuner's JSON cache: the key separates models, integral types, quadratures and
+    methods and buckets the atom count by powers of two; entries are added and replaced."""
+    g = N.default_params(N.MODEL_GBSW)
+    m = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV2)
+    m.n_lebedev = 50
+    k1 = N.tuning_key([m], [5000])
+    assert k1 == N.tuning_key([m], [4097]) != N.tuning_key([m], [4096])  # 4097-8192 share a bucket
+    assert N.tuning_key([g, m], [300, 400]) == N.tuning_key([m, g], [400, 300])  # order of systems
+    assert N.tuning_key([g], [5000]) != k1
+    m2 = N.default_params(N.MODEL_GBMV, N.INTEGRAL_GBMV2)
+    assert N.tuning_key([m2], [5000]) != k1  # Lebedev 26 vs 50
+    path = str(tmp_path / "tuning.json")
+    assert N.load_tuning(path) == {}
+    t = {"blocks_per_cu": 64, "pair_columns": 0, "tight_grid_length": 0.2, "ms_per_eval": 0.6}
+    N.store_tuning(path, k1, t)
+    N.store_tuning(path, "other", dict(t, blocks_per_cu=16))
+    N.store_tuning(path, k1, dict(t, blocks_per_cu=128))
+    data = N.load_tuning(path)
+    assert data[k1]["blocks_per_cu"] == 128 and data["other"]["blocks_per_cu"] == 16
+    assert data[k1]["tight_grid_length"] == 0.2

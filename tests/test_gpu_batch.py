@@ -172,18 +172,28 @@ def test_regrow_raw_cell_lists():
     assert rel_err(e, ref.energy) < ENERGY_TOL and rel_rms(f, ref.forces) < FORCE_TOL
 
 
-def test_autotune_keeps_parity():
+def test_autotune_keeps_parity(tmp_path):
+    """Autotuning keeps parity; with a JSON cache the second context of the same kind takes
+    the stored shapes without timing and gives the same result."""
     s = S.globule(900, 8)
+    cache = str(tmp_path / "tuning.json")
     for p in (_params("gbsw"), _params("gbmv2-50")):
         eng = N.Engine(p, s.charge, s.radius)
         eng.set_positions(s.pos)
-        t = eng.autotune(evals=2)
+        t = eng.autotune(evals=2, cache=cache)
         print(f"[autotune] model {p.model}/{p.integral_type}: {t}", flush=True)
-        assert t["ms_per_eval"] > 0.0
+        assert t["ms_per_eval"] > 0.0 and not t.get("cached")
         e, f = eng.execute(s.pos)
         ref = O.compute(oracle_params(p), s.pos, s.charge, s.radius)
         assert rel_err(e, ref.energy) < ENERGY_TOL and rel_rms(f, ref.forces) < FORCE_TOL
         eng.close()
+        again = N.Engine(p, s.charge, s.radius)
+        again.set_positions(s.pos)
+        t2 = again.autotune(evals=2, cache=cache)
+        assert t2.get("cached") and again.tuning()["blocks_per_cu"] == t["blocks_per_cu"]
+        e2, f2 = again.execute(s.pos)
+        again.close()
+        assert e2 == e and np.array_equal(f2, f)
 
 
 def test_batch_of_64_mixed_systems_sampled():
