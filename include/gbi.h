@@ -272,12 +272,13 @@ typedef struct gbi_custom_spec {
     const char* const* param_deriv_names;
     /* CharmmGBMVForce::addTabulatedFunction (CharmmGBMVForce.h:442; the reference evaluates them
      * with OpenMM's tabulated-function classes, CharmmReferenceKernels.cpp:187-191): functions
-     * of one argument callable by name in every expression.  kinds: 0 Continuous1DFunction (a
-     * cubic spline through sizes[k] values equally spaced on [ranges[2k], ranges[2k + 1]];
-     * natural and zero outside the range, or periodic[k] with period max - min and equal first
-     * and last values), 1 Discrete1DFunction (values[round(x)], 0 <= round(x) < sizes[k]; a
-     * non-finite result otherwise).  values: every function's values, concatenated.  The 2D
-     * and 3D kinds return GBI_ERR_UNSUPPORTED. */
+     * callable by name in every expression.  kinds: 0 Continuous1DFunction (a cubic spline
+     * through sizes[k] values equally spaced on [ranges[2k], ranges[2k + 1]]; natural and zero
+     * outside the range, or periodic[k] with period max - min and equal first and last values);
+     * 1, 2, 3 Discrete1D/2D/3DFunction of 1, 2, 3 arguments (values[x + xsize (y + ysize z)]
+     * at the rounded arguments, dims[3k..3k+2] = xsize, ysize, zsize; a non-finite result when
+     * an index leaves its size).  values: every function's values, concatenated.  The
+     * Continuous2D and Continuous3D kinds (4, 5) return GBI_ERR_UNSUPPORTED. */
     int32_t n_functions;
     const char* const* function_names;
     const int32_t* function_kinds;
@@ -285,6 +286,7 @@ typedef struct gbi_custom_spec {
     const int32_t* function_periodic;
     const double* function_ranges;
     const double* function_values;
+    const int32_t* function_dims;
 } gbi_custom_spec;
 
 /*

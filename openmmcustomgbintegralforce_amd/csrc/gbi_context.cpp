@@ -1535,9 +1535,14 @@ gbi_status gbi_create_custom(int32_t n_atoms, const gbi_params* params, const gb
         gbi::TabFunction f;
         f.name = spec->function_names[k] ? spec->function_names[k] : "";
         f.kind = spec->function_kinds[k];
-        if (f.kind != gbi::TAB_CONTINUOUS_1D && f.kind != gbi::TAB_DISCRETE_1D)
+        if (f.kind < gbi::TAB_CONTINUOUS_1D || f.kind > gbi::TAB_DISCRETE_3D)
             return fail(GBI_ERR_UNSUPPORTED, "tabulated function '" + f.name +
-                                                 "': only Continuous1DFunction and Discrete1DFunction are supported");
+                                                 "': only Continuous1DFunction and Discrete1D/2D/3DFunction are "
+                                                 "supported");
+        if (f.kind >= gbi::TAB_DISCRETE_2D) {
+            if (!spec->function_dims) return fail(GBI_ERR_INVALID_ARGUMENT, "multi-dimensional table without sizes");
+            for (int d = 0; d < 3; ++d) f.dims[d] = spec->function_dims[3 * k + d];
+        }
         const int n = spec->function_sizes[k];
         if (n < (f.kind == gbi::TAB_CONTINUOUS_1D ? 2 : 1))
             return fail(GBI_ERR_INVALID_ARGUMENT, "tabulated function '" + f.name + "' has too few values");

@@ -83,9 +83,9 @@ const std::map<std::string, std::pair<int, const char*>>& functions() {
 
 class Parser {
 public:
-    // tabs: tabulated function name -> device function (one argument)
+    // tabs: tabulated function name -> (device function, number of arguments)
     Parser(const std::string& expr, const std::map<std::string, std::string>& vars,
-           const std::map<std::string, std::string>& tabs)
+           const std::map<std::string, std::pair<std::string, int>>& tabs)
         : src_(expr), toks_(tokenize(expr)), vars_(vars), tabs_(tabs) {}
 
     std::string parse() {
@@ -167,10 +167,18 @@ private:
                 auto tab = tabs_.find(t.text);
                 if (tab != tabs_.end()) {
                     ++pos_;
-                    const std::string arg = sum();
-                    if (is_op(",")) fail("tabulated function '" + t.text + "' takes 1 argument(s)");
+                    std::string call = tab->second.first + "(" + sum();
+                    int nargs = 1;
+                    while (is_op(",")) {
+                        ++pos_;
+                        call += ", " + sum();
+                        ++nargs;
+                    }
                     expect(")");
-                    return tab->second + "(" + arg + ")";
+                    if (nargs != tab->second.second)
+                        fail("tabulated function '" + t.text + "' takes " + std::to_string(tab->second.second) +
+                             " argument(s)");
+                    return call + ")";
                 }
                 auto it = functions().find(t.text);
                 if (it == functions().end()) fail("unknown function '" + t.text + "'");
@@ -206,7 +214,7 @@ private:
     std::string src_;
     std::vector<Token> toks_;
     std::map<std::string, std::string> vars_;
-    std::map<std::string, std::string> tabs_;
+    std::map<std::string, std::pair<std::string, int>> tabs_;
     size_t pos_ = 0;
 };
 
@@ -301,11 +309,26 @@ DU d_tabc(const Du<T, M>& a, const double* y, const double* y2, int n, double lo
     const double df = (y[k + 1] - y[k]) / h + ((1.0 - 3.0 * A * A) * y2[k] + (3.0 * B * B - 1.0) * y2[k + 1]) * (h / 6.0);
     return d_chain(a, T(f), T(df));
 }
-// y[round(x)]; NaN outside 0 <= round(x) < n (the evaluation then reports a non-finite result)
+// y[round(x) + nx (round(y) + ny round(z))]; NaN when a rounded index leaves its size (the
+// evaluation then reports a non-finite result); zero derivatives
+__device__ inline bool tab_index(double v, int n, int& k) {
+    const double t = round(v);
+    k = (int)t;
+    return t >= 0.0 && t < (double)n;
+}
 DU d_tabd(const Du<T, M>& a, const double* y, int n) {
-    const double t = round((double)a.v);
-    const T f = (t >= 0.0 && t < (double)n) ? T(y[(int)t]) : T(__builtin_nan(""));
-    return Du<T, M>::c(f);
+    int i;
+    return Du<T, M>::c(tab_index((double)a.v, n, i) ? T(y[i]) : T(__builtin_nan("")));
+}
+DU d_tabd2(const Du<T, M>& a, const Du<T, M>& b, const double* y, int nx, int ny) {
+    int i, j;
+    const bool ok = tab_index((double)a.v, nx, i) & tab_index((double)b.v, ny, j);
+    return Du<T, M>::c(ok ? T(y[i + nx * j]) : T(__builtin_nan("")));
+}
+DU d_tabd3(const Du<T, M>& a, const Du<T, M>& b, const Du<T, M>& c, const double* y, int nx, int ny, int nz) {
+    int i, j, k;
+    const bool ok = tab_index((double)a.v, nx, i) & tab_index((double)b.v, ny, j) & tab_index((double)c.v, nz, k);
+    return Du<T, M>::c(ok ? T(y[i + nx * (j + ny * k)]) : T(__builtin_nan("")));
 }
 )";
 
@@ -320,8 +343,10 @@ CustomProgram::~CustomProgram() {
 }
 
 std::string translate_expression(const std::string& expr, const CustomSpec& spec, int context, int index) {
-    std::map<std::string, std::string> vars, tabs;
-    for (size_t k = 0; k < spec.functions.size(); ++k) tabs[spec.functions[k].name] = "d_tab" + std::to_string(k);
+    std::map<std::string, std::string> vars;
+    std::map<std::string, std::pair<std::string, int>> tabs;
+    for (size_t k = 0; k < spec.functions.size(); ++k)
+        tabs[spec.functions[k].name] = {"d_tab" + std::to_string(k), tab_arity(spec.functions[k].kind)};
     for (size_t k = 0; k < spec.globals.size(); ++k) vars[spec.globals[k]] = "G" + std::to_string(k);
     if (context == 2) {
         vars["r"] = "r_";
@@ -431,10 +456,16 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
     };
     for (const auto& f : spec.functions) {
         name_ok(f.name, "tabulated function");
-        if (f.kind != TAB_CONTINUOUS_1D && f.kind != TAB_DISCRETE_1D)
+        if (f.kind < TAB_CONTINUOUS_1D || f.kind > TAB_DISCRETE_3D)
             throw std::runtime_error("tabulated function '" + f.name + "': only Continuous1DFunction and "
-                                     "Discrete1DFunction are supported");
+                                     "Discrete1D/2D/3DFunction are supported");
         if (f.values.empty()) throw std::runtime_error("tabulated function '" + f.name + "' has no values");
+        if (f.kind == TAB_DISCRETE_2D || f.kind == TAB_DISCRETE_3D) {
+            const long long want = (long long)f.dims[0] * f.dims[1] * (f.kind == TAB_DISCRETE_3D ? f.dims[2] : 1);
+            if (f.dims[0] < 1 || f.dims[1] < 1 || (f.kind == TAB_DISCRETE_3D && f.dims[2] < 1) ||
+                want != (long long)f.values.size())
+                throw std::runtime_error("tabulated function '" + f.name + "': the sizes do not match the values");
+        }
         for (double v : f.values)
             if (!std::isfinite(v)) throw std::runtime_error("tabulated function '" + f.name + "' has a non-finite value");
     }
@@ -473,8 +504,14 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
             table("_Y2", y2);
             s << "DU d_tab" << k << "(const Du<T, M>& a) { return d_tabc(a, TAB" << k << "_Y, TAB" << k << "_Y2, " << n
               << ", " << f.min << ", " << f.max << ", " << (f.periodic ? 1 : 0) << "); }\n";
-        } else {
+        } else if (f.kind == TAB_DISCRETE_1D) {
             s << "DU d_tab" << k << "(const Du<T, M>& a) { return d_tabd(a, TAB" << k << "_Y, " << n << "); }\n";
+        } else if (f.kind == TAB_DISCRETE_2D) {
+            s << "DU d_tab" << k << "(const Du<T, M>& a, const Du<T, M>& b) { return d_tabd2(a, b, TAB" << k << "_Y, "
+              << f.dims[0] << ", " << f.dims[1] << "); }\n";
+        } else {
+            s << "DU d_tab" << k << "(const Du<T, M>& a, const Du<T, M>& b, const Du<T, M>& c) { return d_tabd3(a, b, c, TAB"
+              << k << "_Y, " << f.dims[0] << ", " << f.dims[1] << ", " << f.dims[2] << "); }\n";
         }
     }
     s << std::setprecision(6);

@@ -21,18 +21,21 @@ constexpr int kCustomChainWaves = 4; // waves of a cg_chain workgroup (one 64-at
 
 enum CustomTermType : int { TERM_SINGLE = 0, TERM_PAIR = 1, TERM_PAIR_NO_EXCL = 2 };
 
-// Tabulated functions of one argument (CharmmGBMVForce::addTabulatedFunction with OpenMM's
-// Continuous1DFunction and Discrete1DFunction): a cubic spline through values at equally
-// spaced points on [min, max] (natural, or periodic with period max - min; zero outside the
-// range when not periodic), or values[round(x)] for integer arguments 0 <= round(x) < n.
-enum TabKind : int { TAB_CONTINUOUS_1D = 0, TAB_DISCRETE_1D = 1 };
+// Tabulated functions (CharmmGBMVForce::addTabulatedFunction with OpenMM's
+// Continuous1DFunction and Discrete1D/2D/3DFunction): a cubic spline through values at
+// equally spaced points on [min, max] (natural, or periodic with period max - min; zero
+// outside the range when not periodic), or the value at the rounded arguments
+// (values[x + xsize (y + ysize z)], each rounded index inside its size).
+enum TabKind : int { TAB_CONTINUOUS_1D = 0, TAB_DISCRETE_1D = 1, TAB_DISCRETE_2D = 2, TAB_DISCRETE_3D = 3 };
 struct TabFunction {
     std::string name;
     int kind = TAB_CONTINUOUS_1D;
     bool periodic = false;
     double min = 0.0, max = 1.0;
+    int dims[3] = {0, 1, 1};  // discrete tables: xsize, ysize, zsize
     std::vector<double> values;
 };
+inline int tab_arity(int kind) { return kind == TAB_DISCRETE_2D ? 2 : (kind == TAB_DISCRETE_3D ? 3 : 1); }
 // Second derivatives of the natural (periodic) cubic spline through `values` at equally spaced
 // points h apart; periodic splines need values.front() == values.back().  Throws on bad input.
 std::vector<double> spline_second_derivatives(const std::vector<double>& values, double h, bool periodic);

@@ -124,6 +124,26 @@ class Discrete1DFunction(TabulatedFunction):
         self._values = list(values)
 
 
+class Discrete2DFunction(TabulatedFunction):
+    """values[x + xsize y] at the rounded arguments."""
+
+    def __init__(self, xsize: int, ysize: int, values):
+        self._args = (int(xsize), int(ysize), list(values))
+
+    def getFunctionParameters(self):
+        return self._args
+
+
+class Discrete3DFunction(TabulatedFunction):
+    """values[x + xsize (y + ysize z)] at the rounded arguments."""
+
+    def __init__(self, xsize: int, ysize: int, zsize: int, values):
+        self._args = (int(xsize), int(ysize), int(zsize), list(values))
+
+    def getFunctionParameters(self):
+        return self._args
+
+
 class Continuous2DFunction(TabulatedFunction):
     """Two-dimensional table (declared for completeness; the HIP kernels do not evaluate it)."""
 
@@ -504,9 +524,15 @@ def gbmv_custom_from_force(force: CharmmGBMVForce) -> dict:
             functions.append(N.Tabulated(name, N.TAB_CONTINUOUS_1D, v, lo, hi, fn.getPeriodic()))
         elif isinstance(fn, Discrete1DFunction):
             functions.append(N.Tabulated(name, N.TAB_DISCRETE_1D, fn.getFunctionParameters()))
+        elif isinstance(fn, Discrete2DFunction):
+            nx, ny, v = fn.getFunctionParameters()
+            functions.append(N.Tabulated(name, N.TAB_DISCRETE_2D, v, dims=(nx, ny, 1)))
+        elif isinstance(fn, Discrete3DFunction):
+            nx, ny, nz, v = fn.getFunctionParameters()
+            functions.append(N.Tabulated(name, N.TAB_DISCRETE_3D, v, dims=(nx, ny, nz)))
         else:
             raise OpenMMException(f"CharmmGBMVForce: tabulated function '{name}': the HIP platform evaluates "
-                                  "Continuous1DFunction and Discrete1DFunction only")
+                                  "Continuous1DFunction and Discrete1D/2D/3DFunction only")
     return dict(params=p, param_names=names, param_values=pv.reshape(-1, len(names)), radius_param=names.index("radius"),
                 integrals=[n for n, _ in integ], values=values, terms=terms, globals_=g,
                 param_derivs=list(force._param_derivs), functions=functions)

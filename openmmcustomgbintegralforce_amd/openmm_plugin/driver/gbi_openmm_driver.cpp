@@ -103,7 +103,7 @@ CharmmGBMVForce* derivGBMV(const std::vector<double>& q, const std::vector<doubl
     return f;
 }
 
-// generalGBMV with a Continuous1DFunction and a Discrete1DFunction, and the coordinates x, y, z
+// generalGBMV with a Continuous1DFunction and Discrete1D/2D/3DFunctions, and the coordinates x, y, z
 // in the Born value and in the single-particle term (the same expressions as
 // tests/test_gpu_openmm_plugin.py's oracle call).
 CharmmGBMVForce* tabGBMV(const std::vector<double>& q, const std::vector<double>& r, int type) {
@@ -117,9 +117,14 @@ CharmmGBMVForce* tabGBMV(const std::vector<double>& q, const std::vector<double>
     for (int k = 0; k < 31; ++k) sv[k] = 1.0 + 0.2 * std::sin(0.3 * k);
     f->addTabulatedFunction("scale", new Continuous1DFunction(sv, 0.0, 3.0));
     f->addTabulatedFunction("w", new Discrete1DFunction({0.9, 1.1}));
-    f->addComputedValue("B", "scale(10*radius)*(1+0.002*(x+y-z))/(1/radius-0.5*I4*radius)",
+    f->addTabulatedFunction("lut", new Discrete2DFunction(2, 3, {0.7, 0.8, 0.95, 1.05, 1.1, 1.2}));
+    std::vector<double> cv(12);
+    for (int k = 0; k < 12; ++k) cv[k] = 1.0 + 0.03 * k;
+    f->addTabulatedFunction("tri3", new Discrete3DFunction(2, 2, 3, cv));
+    f->addComputedValue("B", "scale(10*radius)*lut(step(q), 10*radius)*(1+0.002*(x+y-z))/(1/radius-0.5*I4*radius)",
                         CharmmGBMVForce::SingleParticle);
-    f->addEnergyTerm("0.5*P*q^2/B*w(step(q))+0.01*q^2*(x^2+y^2+z^2)", CharmmGBMVForce::SingleParticle);
+    f->addEnergyTerm("0.5*P*q^2/B*w(step(q))*tri3(step(q), step(radius-0.18), 20*radius-2)+0.01*q^2*(x^2+y^2+z^2)",
+                     CharmmGBMVForce::SingleParticle);
     f->addEnergyTerm(gbi_openmm::kCanonicalPairEnergy, CharmmGBMVForce::ParticlePairNoExclusions);
     for (size_t i = 0; i < q.size(); ++i) f->addParticle({q[i], r[i]});
     return f;

@@ -325,25 +325,36 @@ void HipCalcCharmmGBMVForceKernel::createCustom(const CharmmGBMVForce& force) {
         for (const auto& x : v) out.push_back(x.c_str());
         return out;
     };
-    // tabulated functions: the one-dimensional kinds (OpenMM's Continuous1DFunction, Discrete1DFunction)
+    // tabulated functions: OpenMM's Continuous1DFunction and Discrete1D/2D/3DFunction (dims: the
+    // sizes of the discrete tables, values[x + xsize (y + ysize z)])
     std::vector<std::string> fnames;
-    std::vector<int32_t> fkinds, fsizes, fperiodic;
+    std::vector<int32_t> fkinds, fsizes, fperiodic, fdims;
     std::vector<double> franges, fvalues;
     for (int k = 0; k < force.getNumTabulatedFunctions(); ++k) {
         const TabulatedFunction& fn = force.getTabulatedFunction(k);
         fnames.push_back(force.getTabulatedFunctionName(k));
         std::vector<double> v;
         double lo = 0.0, hi = 0.0;
+        int nx = 0, ny = 1, nz = 1;
         if (const auto* c1 = dynamic_cast<const Continuous1DFunction*>(&fn)) {
             c1->getFunctionParameters(v, lo, hi);
             fkinds.push_back(0);
         } else if (const auto* d1 = dynamic_cast<const Discrete1DFunction*>(&fn)) {
             d1->getFunctionParameters(v);
             fkinds.push_back(1);
+        } else if (const auto* d2 = dynamic_cast<const Discrete2DFunction*>(&fn)) {
+            d2->getFunctionParameters(nx, ny, v);
+            fkinds.push_back(2);
+        } else if (const auto* d3 = dynamic_cast<const Discrete3DFunction*>(&fn)) {
+            d3->getFunctionParameters(nx, ny, nz, v);
+            fkinds.push_back(3);
         } else {
             throw OpenMMException("CharmmGBMVForce: tabulated function '" + fnames.back() +
-                                  "': the HIP platform evaluates Continuous1DFunction and Discrete1DFunction only");
+                                  "': the HIP platform evaluates Continuous1DFunction and Discrete1D/2D/3DFunction only");
         }
+        fdims.push_back(nx);
+        fdims.push_back(ny);
+        fdims.push_back(nz);
         fsizes.push_back((int32_t)v.size());
         fperiodic.push_back(fn.getPeriodic() ? 1 : 0);
         franges.push_back(lo);
@@ -360,6 +371,7 @@ void HipCalcCharmmGBMVForceKernel::createCustom(const CharmmGBMVForce& force) {
     spec.function_periodic = fperiodic.data();
     spec.function_ranges = franges.data();
     spec.function_values = fvalues.data();
+    spec.function_dims = fdims.data();
     spec.n_params = np;
     spec.param_names = cp.data();
     spec.param_values = pvals.data();

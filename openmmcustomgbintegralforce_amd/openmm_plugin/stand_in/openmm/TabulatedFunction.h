@@ -1,6 +1,6 @@
 // Stand-in of OpenMM's tabulated-function classes (openmm/TabulatedFunction.h), used only when
-// OpenMM's headers are not on the include path: the one-dimensional kinds the HIP kernels
-// evaluate (Continuous1DFunction, Discrete1DFunction) and a two-dimensional one, which they
+// OpenMM's headers are not on the include path: the kinds the HIP kernels evaluate
+// (Continuous1DFunction, Discrete1D/2D/3DFunction) and Continuous2DFunction, which they
 // reject.  Same constructors and accessors as OpenMM's.
 #pragma once
 
@@ -46,6 +46,37 @@ public:
     void setFunctionParameters(const std::vector<double>& values) { values_ = values; }
 
 private:
+    std::vector<double> values_;
+};
+
+class Discrete2DFunction : public TabulatedFunction {
+public:
+    Discrete2DFunction(int xsize, int ysize, const std::vector<double>& values)
+        : xsize_(xsize), ysize_(ysize), values_(values) {}
+    void getFunctionParameters(int& xsize, int& ysize, std::vector<double>& values) const {
+        xsize = xsize_;
+        ysize = ysize_;
+        values = values_;
+    }
+
+private:
+    int xsize_, ysize_;
+    std::vector<double> values_;
+};
+
+class Discrete3DFunction : public TabulatedFunction {
+public:
+    Discrete3DFunction(int xsize, int ysize, int zsize, const std::vector<double>& values)
+        : xsize_(xsize), ysize_(ysize), zsize_(zsize), values_(values) {}
+    void getFunctionParameters(int& xsize, int& ysize, int& zsize, std::vector<double>& values) const {
+        xsize = xsize_;
+        ysize = ysize_;
+        zsize = zsize_;
+        values = values_;
+    }
+
+private:
+    int xsize_, ysize_, zsize_;
     std::vector<double> values_;
 };
 

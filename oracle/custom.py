@@ -14,7 +14,7 @@ functions (CharmmReferenceKernels.cpp:187-191 hands them to OpenMM's tabulated-f
 classes, which are not in the reference tree) follow OpenMM's documented behaviour:
 Continuous1DFunction a natural (or periodic) cubic spline through values equally spaced on
 [min, max], zero outside it when not periodic, evaluated with scipy's CubicSpline;
-Discrete1DFunction values[round(x)], round half away from zero.
+Discrete1D/2D/3DFunction the value at the rounded arguments, round half away from zero.
 """
 from __future__ import annotations
 
@@ -45,16 +45,36 @@ def _functions():
 
 
 class Tabulated:
-    """A tabulated function of one argument: kind 0 Continuous1DFunction (values equally
-    spaced on [min, max], periodic or natural spline), kind 1 Discrete1DFunction."""
+    """A tabulated function: kind 0 Continuous1DFunction (values equally spaced on [min, max],
+    periodic or natural spline), kinds 1, 2, 3 Discrete1D/2D/3DFunction (dims: xsize, ysize,
+    zsize; values[x + xsize (y + ysize z)] at the rounded arguments)."""
 
-    def __init__(self, name, kind, values, min=0.0, max=1.0, periodic=False):
+    def __init__(self, name, kind, values, min=0.0, max=1.0, periodic=False, dims=(0, 1, 1)):
         self.name, self.kind = name, int(kind)
         self.values = np.asarray(values, dtype=np.float64)
         self.min, self.max, self.periodic = float(min), float(max), bool(periodic)
+        self.dims = tuple(int(d) for d in dims)
 
     def numeric(self):
-        """(f, f') as numpy callables."""
+        """(f, f') as numpy callables (f' of the one-argument kinds; the discrete tables are
+        piecewise constant)."""
+        if self.kind in (2, 3):
+            vals, (nx, ny, nz) = self.values, self.dims
+
+            def idx(t, n):
+                k = (np.sign(t) * np.floor(np.abs(t) + 0.5)).astype(np.int64)
+                if np.any((k < 0) | (k >= n)):
+                    raise ValueError(f"{self.name}: argument out of range")
+                return k
+
+            def f(*a):
+                a = [np.asarray(x, dtype=np.float64) for x in a]
+                k = idx(a[0], nx) + nx * idx(a[1], ny)
+                if self.kind == 3:
+                    k = k + nx * ny * idx(a[2], nz)
+                return vals[k]
+
+            return f, None
         if self.kind == 1:
             vals = self.values
 
@@ -83,6 +103,11 @@ class Tabulated:
 
     def sympy_function(self):
         f, df = self.numeric()
+        if df is None:  # piecewise constant in every argument
+            def fdiff0(obj, argindex=1):
+                return sp.Integer(0)
+
+            return type(self.name, (sp.Function,), {"fdiff": fdiff0, "_imp_": staticmethod(f)})
         dcls = sp.Function(self.name + "__d")
         dcls._imp_ = staticmethod(df)
 

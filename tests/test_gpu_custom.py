@@ -14,7 +14,7 @@ from openmmcustomgbintegralforce_amd import _native as N
 from openmmcustomgbintegralforce_amd import synthetic as S
 
 from common import oracle_params, rel_err, rel_rms
-from test_oracle_custom import BORN, G, SCALE, SELF, STILL, TAB_TERMS, TAB_VALUES, TERMS2, VALUES2, W, WAVE
+from test_oracle_custom import BORN, G, SELF, STILL, TABLES, TAB_TERMS, TAB_VALUES, TERMS2, VALUES2
 
 pytestmark = pytest.mark.gpu
 
@@ -99,14 +99,14 @@ def test_one_integral_and_graph_replay():
 
 
 def _lib_table(t):
-    return N.Tabulated(t.name, t.kind, t.values, t.min, t.max, t.periodic)
+    return N.Tabulated(t.name, t.kind, t.values, t.min, t.max, t.periodic, t.dims)
 
 
 @pytest.mark.parametrize("method", [N.NO_CUTOFF, N.CUTOFF_PERIODIC])
 def test_coordinates_and_tabulated_functions(method):
     """Computed values and single-particle terms using the atom's x, y, z (their gradients are
-    forces on the atom itself), a natural and a periodic Continuous1DFunction and a
-    Discrete1DFunction, against the sympy/scipy oracle."""
+    forces on the atom itself), a natural and a periodic Continuous1DFunction and
+    Discrete1D/2D/3DFunctions, against the sympy/scipy oracle."""
     s = S.globule(500, 77)
     L = np.array([4.0, 4.1, 4.2])
     pos = s.pos - s.pos.mean(axis=0) + (L / 2 if method == N.CUTOFF_PERIODIC else 0.0)
@@ -114,7 +114,7 @@ def test_coordinates_and_tabulated_functions(method):
     p.orders[0] = 4
     p.nonbonded_method, p.cutoff = method, 1.4
     eng = N.Engine.custom(p, ["q", "radius"], np.stack([s.charge, s.radius], 1), 1, ["I4"], TAB_VALUES, TAB_TERMS, G,
-                          functions=[_lib_table(t) for t in (SCALE, WAVE, W)])
+                          functions=[_lib_table(t) for t in TABLES])
     if method == N.CUTOFF_PERIODIC:
         eng.set_periodic_box(L)
     e, f = eng.execute(pos)
@@ -122,7 +122,7 @@ def test_coordinates_and_tabulated_functions(method):
     eng.close()
     E, F, V, _ = C.compute(oracle_params(p, box=L if method == N.CUTOFF_PERIODIC else None), pos, ["q", "radius"],
                            np.stack([s.charge, s.radius], 1), 1, ["I4"], TAB_VALUES, TAB_TERMS, G,
-                           functions=[SCALE, WAVE, W])
+                           functions=TABLES)
     print(f"[custom] x,y,z + tables, method {method}: energy {rel_err(e, E):.2e} force {rel_rms(f, F):.2e} "
           f"values {np.max(np.abs(vals - V) / np.abs(V)):.2e}", flush=True)
     assert rel_err(e, E) < ENERGY_TOL and rel_rms(f, F) < FORCE_TOL

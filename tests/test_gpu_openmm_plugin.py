@@ -252,8 +252,8 @@ def test_python_kernel_mirror_context_parameters_and_derivatives():
 
 
 def test_cpp_kernel_classes_tabulated_functions_and_coordinates(tmp_path):
-    """Driver model 4: a CharmmGBMVForce with a Continuous1DFunction, a Discrete1DFunction and
-    the coordinates x, y, z in its Born value and single-particle term, through the C++ kernel
+    """Driver model 4: a CharmmGBMVForce with a Continuous1DFunction, Discrete1D/2D/3DFunctions
+    and the coordinates x, y, z in its Born value and single-particle term, through the C++ kernel
     class (OpenMM's tabulated-function objects converted to the C ABI's tables)."""
     from oracle import custom as C
 
@@ -264,19 +264,23 @@ def test_cpp_kernel_classes_tabulated_functions_and_coordinates(tmp_path):
     P = -138.935456 * (1.0 - 1.0 / 80.0)
     scale = C.Tabulated("scale", 0, 1.0 + 0.2 * np.sin(0.3 * np.arange(31)), 0.0, 3.0)
     w = C.Tabulated("w", 1, [0.9, 1.1])
+    lut = C.Tabulated("lut", 2, [0.7, 0.8, 0.95, 1.05, 1.1, 1.2], dims=(2, 3, 1))
+    tri3 = C.Tabulated("tri3", 3, 1.0 + 0.03 * np.arange(12), dims=(2, 2, 3))
     E, F, _, _ = C.compute(oracle_params(p), s.pos, ["q", "radius"], np.stack([s.charge, s.radius], 1), 1, ["I4"],
-                           [("B", "scale(10*radius)*(1+0.002*(x+y-z))/(1/radius-0.5*I4*radius)")],
-                           [("0.5*P*q^2/B*w(step(q))+0.01*q^2*(x^2+y^2+z^2)", C.SINGLE),
+                           [("B", "scale(10*radius)*lut(step(q), 10*radius)*(1+0.002*(x+y-z))/(1/radius-0.5*I4*radius)")],
+                           [("0.5*P*q^2/B*w(step(q))*tri3(step(q), step(radius-0.18), 20*radius-2)"
+                             "+0.01*q^2*(x^2+y^2+z^2)", C.SINGLE),
                             ("P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))", C.PAIR_NO_EXCLUSIONS)], {"P": P},
-                           functions=[scale, w])
+                           functions=[scale, w, lut, tri3])
     print(f"[openmm-plugin] tables + x,y,z N={s.n}: energy {e:.6f} vs {E:.6f} force {rel_rms(f, F):.2e}", flush=True)
     assert rel_err(e, E) < 1e-5
     assert rel_rms(f, F) < 1e-5
 
 
 def test_python_kernel_mirror_tabulated_functions():
-    """forces.CharmmGBMVForce.addTabulatedFunction through the Python kernel mirror; a 2D table
-    is refused with OpenMMException."""
+    """forces.CharmmGBMVForce.addTabulatedFunction through the Python kernel mirror (a
+    Continuous1DFunction, then a Discrete2DFunction); a Continuous2DFunction is refused with
+    OpenMMException."""
     from oracle import custom as C
     from openmmcustomgbintegralforce_amd import forces as FM
 
@@ -284,7 +288,7 @@ def test_python_kernel_mirror_tabulated_functions():
     P = -138.935456 * (1.0 - 1.0 / 80.0)
     vals = 1.0 + 0.2 * np.sin(0.3 * np.arange(31))
 
-    def force(fn):
+    def force(fn, arg="10*radius"):
         f = FM.CharmmGBMVForce()
         f.addPerParticleParameter("q")
         f.addPerParticleParameter("radius")
@@ -292,7 +296,7 @@ def test_python_kernel_mirror_tabulated_functions():
         f.addGBIntegral("I4", [4])
         f.setGBIntegralType(FM.CharmmGBMVForce.GBMVIntegralTypeII)
         f.addTabulatedFunction("scale", fn)
-        f.addComputedValue("B", "scale(10*radius)/(1/radius-0.5*I4*radius)", FM.CharmmGBMVForce.SingleParticle)
+        f.addComputedValue("B", f"scale({arg})/(1/radius-0.5*I4*radius)", FM.CharmmGBMVForce.SingleParticle)
         f.addEnergyTerm("0.5*P*q^2/B+0.01*q^2*z^2", FM.CharmmGBMVForce.SingleParticle)
         f.addEnergyTerm("P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))", FM.CharmmGBMVForce.ParticlePairNoExclusions)
         for q, r in zip(s.charge, s.radius):
@@ -313,6 +317,19 @@ def test_python_kernel_mirror_tabulated_functions():
                            functions=[C.Tabulated("scale", 0, vals, 0.0, 3.0)])
     print(f"[openmm-plugin] python mirror tables: energy {e:.6f} vs {E:.6f}", flush=True)
     assert rel_err(e, E) < 1e-5 and rel_rms(ctx.forces, F) < 1e-5
+    d2 = [0.8, 0.9, 1.0, 1.1, 1.15, 1.25]
+    f2 = force(FM.Discrete2DFunction(2, 3, d2), "step(q), 10*radius")
+    k2 = FM.HipCalcCharmmGBMVForceKernel()
+    k2.initialize(FM.System(s.n), f2)
+    ctx2 = FM.ContextImpl(s.pos, forces=(f2,))
+    e2 = k2.execute(ctx2, True, True)
+    E2, F2, _, _ = C.compute(oracle_params(p), s.pos, ["q", "radius"], np.stack([s.charge, s.radius], 1), 1, ["I4"],
+                             [("B", "scale(step(q), 10*radius)/(1/radius-0.5*I4*radius)")],
+                             [("0.5*P*q^2/B+0.01*q^2*z^2", C.SINGLE),
+                              ("P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))", C.PAIR_NO_EXCLUSIONS)], {"P": P},
+                             functions=[C.Tabulated("scale", 2, d2, dims=(2, 3, 1))])
+    print(f"[openmm-plugin] python mirror Discrete2D: energy {e2:.6f} vs {E2:.6f}", flush=True)
+    assert rel_err(e2, E2) < 1e-5 and rel_rms(ctx2.forces, F2) < 1e-5
     with pytest.raises(FM.OpenMMException):
         FM.HipCalcCharmmGBMVForceKernel().initialize(
             FM.System(s.n), force(FM.Continuous2DFunction(2, 2, [1.0] * 4, 0.0, 1.0, 0.0, 1.0)))

@@ -116,8 +116,14 @@ def test_library_rejects_malformed_expressions():
 SCALE = C.Tabulated("scale", 0, 1.0 + 0.2 * np.sin(0.3 * np.arange(31)), 0.0, 3.0)
 WAVE = C.Tabulated("wave", 0, np.cos(np.linspace(0.0, 2 * np.pi, 41)), 0.0, 2 * np.pi, periodic=True)
 W = C.Tabulated("w", 1, [0.9, 1.1])
-TAB_VALUES = [("B", "scale(10*radius)*(1+0.002*(x+y-z))/(1/radius-0.5*I4*radius)")]
-TAB_TERMS = [("0.5*P*q^2/B*w(step(q))+0.01*q^2*(x^2+y^2+z^2)+0.05*q^2*wave(3*x)", C.SINGLE),
+# values[x + xsize (y + ysize z)] at the rounded arguments; the synthetic radii are 0.1, 0.17,
+# 0.185 and 0.2 nm, so 10*radius rounds to 1 or 2 and 20*radius-2 to 0, 1 or 2
+LUT = C.Tabulated("lut", 2, [0.7, 0.8, 0.95, 1.05, 1.1, 1.2], dims=(2, 3, 1))
+TRI3 = C.Tabulated("tri3", 3, 1.0 + 0.03 * np.arange(12), dims=(2, 2, 3))
+TABLES = [SCALE, WAVE, W, LUT, TRI3]
+TAB_VALUES = [("B", "scale(10*radius)*lut(step(q), 10*radius)*(1+0.002*(x+y-z))/(1/radius-0.5*I4*radius)")]
+TAB_TERMS = [("0.5*P*q^2/B*w(step(q))*tri3(step(q), step(radius-0.18), 20*radius-2)"
+              "+0.01*q^2*(x^2+y^2+z^2)+0.05*q^2*wave(3*x)", C.SINGLE),
              ("P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))*(1+0.05*wave(r))", C.PAIR_NO_EXCLUSIONS)]
 
 
@@ -135,14 +141,22 @@ def test_tabulated_functions_pinned_by_closed_forms():
     assert f(np.array([0.49, 0.5, 1.5, 2.4])).tolist() == [10.0, 20.0, 30.0, 30.0]
     with pytest.raises(ValueError):
         f(np.array([2.5]))
+    # 2D and 3D tables: x fastest, then y, then z (a row-major array read as [z][y][x])
+    grid = np.arange(24.0).reshape(4, 3, 2)  # [z][y][x] with xsize 2, ysize 3, zsize 4
+    f3, _ = C.Tabulated("c", 3, grid.ravel(), dims=(2, 3, 4)).numeric()
+    x, y, z = np.array([0.4, 1.0, 0.6]), np.array([2.2, 0.5, 1.49]), np.array([3.0, 0.0, 2.5])
+    assert f3(x, y, z).tolist() == [grid[3, 2, 0], grid[0, 1, 1], grid[3, 1, 1]]
+    f2, _ = C.Tabulated("s", 2, grid[0].ravel(), dims=(2, 3, 1)).numeric()
+    assert f2(x, y).tolist() == [grid[0, 2, 0], grid[0, 1, 1], grid[0, 1, 1]]
+    with pytest.raises(ValueError):
+        f2(np.array([0.0]), np.array([2.6]))
 
 
 def _tabulated(s, pos):
     pv = np.stack([s.charge, s.radius], 1)
     p = O.gbmv_params(2)
     p.orders[0] = 4
-    return C.compute(p, pos, ["q", "radius"], pv, 1, ["I4"], TAB_VALUES, TAB_TERMS, dict(G),
-                     functions=[SCALE, WAVE, W])
+    return C.compute(p, pos, ["q", "radius"], pv, 1, ["I4"], TAB_VALUES, TAB_TERMS, dict(G), functions=TABLES)
 
 
 def test_coordinates_and_tabulated_functions_forces_are_the_gradient():
@@ -162,14 +176,25 @@ def test_coordinates_and_tabulated_functions_forces_are_the_gradient():
 def test_library_rejects_bad_tables_and_coordinates():
     tab = N.Tabulated("scale", N.TAB_CONTINUOUS_1D, [1.0, 1.2, 1.1], 0.0, 3.0)
     val = [("B", "scale(10*radius)/(1/radius-0.5*I4*radius)")]
-    assert _create(dict(values=val, functions=[N.Tabulated("scale", 2, [1.0, 2.0])])) == 2  # 2D/3D: unsupported
+    assert _create(dict(values=val, functions=[N.Tabulated("scale", 4, [1.0, 2.0])])) == 2  # Continuous2D/3D
+    d2 = N.Tabulated("lut", N.TAB_DISCRETE_2D, [1.0, 2.0, 3.0, 4.0], dims=(2, 2, 1))
+    d3 = N.Tabulated("lut", N.TAB_DISCRETE_3D, [1.0] * 8, dims=(2, 2, 2))
+    val2 = [("B", "lut(1, 0)/(1/radius-0.5*I4*radius)")]
+    assert _create(dict(values=val2, functions=[N.Tabulated("lut", 2, [1.0, 2.0, 3.0], dims=(2, 2, 1))])) == 1
+    assert _create(dict(values=val2, functions=[N.Tabulated("lut", 2, [1.0, 2.0], dims=(0, 2, 1))])) == 1
+    assert _create(dict(values=[("B", "lut(1)/(1/radius-0.5*I4*radius)")], functions=[d2])) == 1   # two arguments
+    assert _create(dict(values=val2, functions=[d3])) == 1                                        # three arguments
     assert _create(dict(values=val, functions=[N.Tabulated("scale", 0, [1.0, 2.0, 1.5], 0, 3, True)])) == 1
     assert _create(dict(values=val, functions=[N.Tabulated("scale", 0, [1.0, 2.0], 3.0, 3.0)])) == 1  # min < max
     assert _create(dict(values=[("B", "scale(I4, I7)")], functions=[tab])) == 1   # one argument
     assert _create(dict(values=val, functions=[tab, tab])) == 1                    # defined twice
+    assert _create(dict(values=[("B", "cube(1, 0, 0)/(1/radius-0.5*I4*radius)")],
+                        functions=[N.Tabulated("cube", 3, [1.0] * 8, dims=(2, 2, 2))])) == 1  # a built-in's name
     assert _create(dict(terms=[SELF, ("P*q1*q2/r*x", C.PAIR)])) == 1               # x is not a pair variable
     assert _create(dict(param_names=["q", "x"])) == 1                              # reserved coordinate
     # well formed: only the device is missing here (or the context is created on a GPU)
     assert _create(dict(values=[("B", "scale(10*radius)*(1+0.01*x)/(1/radius-0.5*I4*radius)")],
                         terms=[("0.5*P*q^2/B+0.01*z^2", C.SINGLE), (STILL, C.PAIR_NO_EXCLUSIONS)],
                         functions=[tab])) in (0, 8)
+    assert _create(dict(values=val2, functions=[d2])) in (0, 8)
+    assert _create(dict(values=[("B", "lut(1, 0, step(q))/(1/radius-0.5*I4*radius)")], functions=[d3])) in (0, 8)
