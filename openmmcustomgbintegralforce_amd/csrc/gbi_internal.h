@@ -216,8 +216,9 @@ struct KParams {
     int* frec_base;          // cell_total + 1: exclusive scan of the point counts
     int* worklist;           // VOL_KINDS x cell_total
     int* worklist_sys;       // VOL_KINDS x cell_total: system of each worklist cell
-    float4* prec;            // per system: cell_cap x pt_cap slots
-    float4* frec0;           // NQ: active points, grouped by cell (copy of prec)
+    float4* prec;            // per system: cell_cap x pt_cap slots; records (FP32 point - corner, packed point),
+                             // or, binning by bricks, packed points (atom << kQBits | q) in the range's first quarter
+    float4* frec0;           // NQ: active points, grouped by cell (corner-relative FP32 point, packed point)
     float4* frec1;           // NQ: GBSW (V, 0, 0, 0) / GBMV1 (dV/dS, V, 0, 0) / GBMV2 (c1, c2, c3, vx)
     float4* frec2;           // NQ: GBMV2 (vy, vz, V, 0)
     float* vval;             // NQ by pid: V, sign bit set on the points with a chain-rule factor

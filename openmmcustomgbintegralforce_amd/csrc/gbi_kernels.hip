@@ -725,32 +725,45 @@ __device__ inline void quad_point(const KParams& p, double xi, double yi, double
 // Two ways, per system:
 //   bricks (sk > 1): the key is the brick of sk x sk x sk cells holding the point's cell, its
 //     counter on a 128-byte line of its own; the workgroup files (packed point, cell in brick)
-//     in the brick's slots and k_point_split moves them into the cells.  An atom's 1,200
-//     GBMV-II points fall into about 200 bricks of 0.45 nm against 670 cells of 0.15 nm: the
-//     two passes together take 0.17 ms against 0.21 for one pass on cells.
+//     in the brick's slots and k_point_split moves the packed points into the cells.  An atom's 1,200 GBMV-II
+//     points fall into about 200 bricks of 0.45 nm against 670 cells of 0.15 nm.
 //   cells (sk = 1, the exact GBSW grid of 0.20 nm, where bricks of 0.40 nm save too few
 //     atomics to pay for the second pass): the key is the cell and the workgroup writes the
-//     final records.
-// A record is the point relative to its cell's corner in FP32 and the packed point.  Points
-// outside the grid, in an empty cell, or (GBSW) inside the owner's own burial radius get their
-// volume directly.
+//     cell's slots directly.
+// What a cell slot holds depends on the way (pt_slot / point_record):
+//   bricks: the packed point (atom << kQBits | q), 4 bytes; the volume pass recomputes the
+//     point from the atom's FP64 position and the quadrature row.  Against a 16-byte record
+//     this halves k_point_split (0.062 -> 0.030 ms at 5,000 GBMV-II atoms) for 0.005 ms more
+//     in the volume pass.
+//   cells: the record (the point relative to its cell's corner in FP32, the packed point),
+//     16 bytes.  There the binning pass is bound by its atomics, not by its writes, and the
+//     GBSW volume pass, whose points mostly leave after a few atoms on burial, took 20 % longer
+//     recomputing the points.
+// Points outside the grid, in an empty cell, or (GBSW) inside the owner's own burial radius get
+// their volume directly.
 constexpr int kBinHash = 2048;                      // > Q: the table never fills
 constexpr int kBinPer = kBinHash / 256;             // points per thread
 
-// Corner-relative FP32 record of point row of atom i in cell (ix, iy, iz): the FP64 point as
-// binned (r_q = x_i + off_q, wrapped into a periodic box) minus the corner.
+// Corner-relative FP32 record of point row of atom i in the cell with minimum corner `corner`:
+// the FP64 point as binned (r_q = x_i + off_q, wrapped into a periodic box) minus the corner,
+// and the packed point.
 __device__ inline float4 point_record(const KParams& p, const SysDesc& sd, const SysGeom& G, int i, int row,
-                                      uint32_t packed, int ix, int iy, int iz) {
+                                      uint32_t packed, const double* corner) {
+#pragma clang fp contract(off)
     double pt[3];
     quad_point(p, p.pos[3 * i], p.pos[3 * i + 1], p.pos[3 * i + 2], row, pt);
     if (sd.periodic) {
         int idx[3];
         (void)point_cell_periodic(G, pt, idx);
     }
-    double corner[3];
-    cell_corner_idx(sd, G, ix, iy, iz, corner);
     return make_float4((float)(pt[0] - corner[0]), (float)(pt[1] - corner[1]), (float)(pt[2] - corner[2]),
                        __uint_as_float(packed));
+}
+
+// Slot s of local cell c of a system binning by bricks: the packed points live in the first
+// quarter of the system's own float4 slot range.
+__device__ inline uint32_t* pt_slot(const KParams& p, const SysDesc& sd, int c, int s) {
+    return reinterpret_cast<uint32_t*>(p.prec) + 4 * sd.prec_base + (long long)c * sd.pt_cap + s;
 }
 
 // BRICKS: the launch for the systems binning by bricks (the other systems' workgroups return).
@@ -855,11 +868,11 @@ __global__ void __launch_bounds__(256) k_point_bin(KParams p) {
     }
 }
 
-// k_point_split moves each brick's points into their cells (point_record).  A workgroup takes
-// one chunk of kSplitChunk points of a brick at a time (blockIdx.x: brick, blockIdx.z: chunk,
-// blockIdx.y: system; the fullest bricks hold 4x the mean, so whole bricks per workgroup would
-// leave a long tail), sorts the chunk's packed points by cell in LDS, reserves each cell's
-// slots with one global atomic (at most sk^3 per chunk) and writes the records in that order.
+// k_point_split moves each brick's points into their cells.  A workgroup takes one chunk of
+// kSplitChunk points of a brick at a time (blockIdx.x: brick, blockIdx.z: chunk, blockIdx.y:
+// system; the fullest bricks hold 4x the mean, so whole bricks per workgroup would leave a long
+// tail), sorts the chunk's packed points by cell in LDS, reserves each cell's slots with one
+// global atomic (at most sk^3 per chunk) and writes the packed points in that order.
 constexpr int kSplitChunk = 2048;
 constexpr int kSplitPer = kSplitChunk / 256;
 constexpr int kSplitParts = 4;  // workgroups per brick (each takes every 4th chunk)
@@ -872,7 +885,7 @@ __global__ void __launch_bounds__(256) k_point_split(KParams p) {
     const int s = blockIdx.y, tid = threadIdx.x;
     const SysDesc& sd = p.sys[s];
     const SysGeom& G = p.geom[s];
-    const int sk = sd.sk, sk3 = sk * sk * sk, cb = sd.cell_base, qb = sd.quad_base, pt_cap = sd.pt_cap;
+    const int sk = sd.sk, sk3 = sk * sk * sk, cb = sd.cell_base, pt_cap = sd.pt_cap;
     const int nbrick = sk > 1 ? G.nb[0] * G.nb[1] * G.nb[2] : 0;  // cells binned directly: nothing to move
     const long long brick_slots = (long long)sk3 * pt_cap;
     for (int b = blockIdx.x; b < nbrick; b += gridDim.x) {
@@ -885,7 +898,7 @@ __global__ void __launch_bounds__(256) k_point_split(KParams p) {
             __syncthreads();
             if (tid < sk3) s_cnt[tid] = 0;
             __syncthreads();
-            uint2 r[kSplitPer];
+            uint2 r[kSplitPer];  // (packed point, its cell in the brick)
             int rank[kSplitPer];
 #pragma unroll
             for (int j = 0; j < kSplitPer; ++j) {
@@ -924,8 +937,7 @@ __global__ void __launch_bounds__(256) k_point_split(KParams p) {
                 }
                 const int ix = bx * sk + u / (sk * sk), iy = by * sk + (u / sk) % sk, iz = bz * sk + u % sk;
                 const int c = (ix * G.n[1] + iy) * G.n[2] + iz;
-                const int i = (int)(e.x >> kQBits), row = qb + (int)(e.x & ((1u << kQBits) - 1));
-                p.prec[sd.prec_base + (long long)c * pt_cap + slot] = point_record(p, sd, G, i, row, e.x, ix, iy, iz);
+                *pt_slot(p, sd, c, slot) = e.x;
             }
         }
     }
@@ -992,7 +1004,7 @@ __device__ inline float mark_active(float v, bool active) {
 __device__ inline float volume_of(float vv) { return __int_as_float(__float_as_int(vv) & 0x7fffffff); }
 
 template <int VK>
-__global__ void __launch_bounds__(64) k_volume(KParams p) {
+__global__ void __launch_bounds__(64, 7) k_volume(KParams p) {
     __shared__ float s_x[kMaxAtomChunk], s_y[kMaxAtomChunk], s_z[kMaxAtomChunk];
     __shared__ float s_a[kMaxAtomChunk], s_b[kMaxAtomChunk], s_c[kMaxAtomChunk];
     if (batch_fatal(p)) return;
@@ -1002,7 +1014,9 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
     const int* worklist_sys = p.worklist_sys + VK * p.cell_total;
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         const int c = worklist[w];
-        const SysDesc& sd = p.sys[worklist_sys[w]];
+        const int sys = worklist_sys[w];
+        const SysDesc& sd = p.sys[sys];
+        const SysGeom& G = p.geom[sys];
         const int pt_cap = sd.pt_cap;
         const int np = min(p.cell_pt_count[c], pt_cap);
         const int na = p.cell_count_eff[c];
@@ -1010,7 +1024,10 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
         const int as = p.cell_start[c];
         const bool single = na <= kMaxAtomChunk;
         const int fb = p.frec_base[c];  // exclusive scan of the point counts
-        const int pt_base = sd.pt_base, abeg = sd.atom_begin, Q = sd.Q;
+        const int pt_base = sd.pt_base, abeg = sd.atom_begin, Q = sd.Q, qb = sd.quad_base;
+        const bool ids = sd.sk > 1;  // the slots hold packed points (bricks), else records
+        double corner[3];
+        cell_corner(sd, G, c - sd.cell_base, corner);
         const float k1 = sd.k1, k3 = sd.k3, fclamp = sd.fclamp, S0 = sd.S0, vguard = sd.vguard;
         const float beta = sd.beta, lambda = sd.lambda;
         const bool acc64 = VK != VOL_GBSW && sd.acc64;
@@ -1039,8 +1056,17 @@ __global__ void __launch_bounds__(64) k_volume(KParams p) {
         for (int pb = 0; pb < np; pb += kWave) {
             const int k = pb + lane;
             const bool valid = k < np;
-            const float4 pr = valid ? p.prec[ps + k] : make_float4(0.f, 0.f, 0.f, 0.f);
-            const uint32_t pk = __float_as_uint(pr.w);
+            float4 pr = make_float4(0.f, 0.f, 0.f, 0.f);
+            uint32_t pk = 0u;
+            if (valid) {
+                if (ids) {
+                    pk = *pt_slot(p, sd, c - sd.cell_base, k);
+                    pr = point_record(p, sd, G, (int)(pk >> kQBits), qb + (int)(pk & ((1u << kQBits) - 1)), pk, corner);
+                } else {
+                    pr = p.prec[ps + k];
+                    pk = __float_as_uint(pr.w);
+                }
+            }
             const int i = (int)(pk >> kQBits), q = (int)(pk & ((1u << kQBits) - 1));
             const float prx = pr.x, pry = pr.y, prz = pr.z;
             float V = 1.f;
