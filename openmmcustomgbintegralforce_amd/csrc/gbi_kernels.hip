@@ -741,8 +741,13 @@ __device__ inline void quad_point(const KParams& p, double xi, double yi, double
 //     recomputing the points.
 // Points outside the grid, in an empty cell, or (GBSW) inside the owner's own burial radius get
 // their volume directly.
-constexpr int kBinHash = 2048;                      // > Q: the table never fills
-constexpr int kBinPer = kBinHash / 256;             // points per thread
+constexpr int kBinHash = 2048;  // > Q: the table never fills
+// Threads per workgroup: bricks 512 (4 points per thread, 56 VGPRs, 8 waves per SIMD: 0.088 ->
+// 0.075 ms at 5,000 GBMV-II atoms against 256 threads); cells 256 (8 points per thread: the
+// 512-thread form took 0.62 ms against 0.42 at 25,000 GBSW atoms, whose pass is bound by its
+// returning atomics).
+template <bool BRICKS>
+constexpr int bin_threads() { return BRICKS ? 512 : 256; }
 
 // Corner-relative FP32 record of point row of atom i in the cell with minimum corner `corner`:
 // the FP64 point as binned (r_q = x_i + off_q, wrapped into a periodic box) minus the corner,
@@ -768,8 +773,11 @@ __device__ inline uint32_t* pt_slot(const KParams& p, const SysDesc& sd, int c, 
 
 // BRICKS: the launch for the systems binning by bricks (the other systems' workgroups return).
 template <bool BRICKS>
-__global__ void __launch_bounds__(256) k_point_bin(KParams p) {
-    __shared__ int t_key[kBinHash], t_cnt[kBinHash], t_base[kBinHash];
+__global__ void __launch_bounds__(bin_threads<BRICKS>()) k_point_bin(KParams p) {
+    constexpr int kBinThreads = bin_threads<BRICKS>();
+    constexpr int kBinPer = kBinHash / kBinThreads;  // points per thread
+    // t_cnt: the points per key, then the key's first reserved slot
+    __shared__ int t_key[kBinHash], t_cnt[kBinHash];
     if (batch_fatal(p)) return;
     const int tid = threadIdx.x;
     const int i = p.bin_order[blockIdx.x];
@@ -789,7 +797,7 @@ __global__ void __launch_bounds__(256) k_point_bin(KParams p) {
     const float sig_empty = sd.vol_kind == VOL_GBMV1 ? 0.0f : __int_as_float(0x7fc00000);
     const double xi = p.pos[3 * i], yi = p.pos[3 * i + 1], zi = p.pos[3 * i + 2];
     const float own_lo2 = p.aparam[i].y;
-    for (int k = tid; k < kBinHash; k += 256) {
+    for (int k = tid; k < kBinHash; k += kBinThreads) {
         t_key[k] = -1;
         t_cnt[k] = 0;
     }
@@ -802,7 +810,7 @@ __global__ void __launch_bounds__(256) k_point_bin(KParams p) {
         rank[u] = 0;
         cell[u] = 0;
         rx[u] = ry[u] = rz[u] = 0.f;
-        const int q = tid + 256 * u;
+        const int q = tid + kBinThreads * u;
         if (q >= Q) continue;
         const int row = qb + q;
         int c = -1, idx[3] = {0, 0, 0};
@@ -841,21 +849,21 @@ __global__ void __launch_bounds__(256) k_point_bin(KParams p) {
         rank[u] = atomicAdd(&t_cnt[e], 1);
     }
     __syncthreads();
-    for (int k = tid; k < kBinHash; k += 256) {
+    for (int k = tid; k < kBinHash; k += kBinThreads) {
         const int n = t_cnt[k];
         if (n > 0) {
             if constexpr (bricks)
-                t_base[k] = atomicAdd(&p.brick_count[(long long)(sd.brick_base + t_key[k]) * kBrickStride], n);
+                t_cnt[k] = atomicAdd(&p.brick_count[(long long)(sd.brick_base + t_key[k]) * kBrickStride], n);
             else
-                t_base[k] = atomicAdd(&p.cell_pt_count[cb + t_key[k]], n);
+                t_cnt[k] = atomicAdd(&p.cell_pt_count[cb + t_key[k]], n);
         }
     }
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < kBinPer; ++u) {
         if (h[u] < 0) continue;
-        const int q = tid + 256 * u;
-        const long long slot = (long long)t_base[h[u]] + rank[u];
+        const int q = tid + kBinThreads * u;
+        const long long slot = (long long)t_cnt[h[u]] + rank[u];
         const int key = t_key[h[u]];
         const uint32_t packed = ((uint32_t)i << kQBits) | (uint32_t)q;
         if (slot >= key_slots) {
@@ -2154,8 +2162,8 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(k_grid_sort, dim3(std::max(1, std::min(p.cell_total, 65536))), dim3(64), 0, stream, p);
     mark(1);
-    if (p.any_cells) hipLaunchKernelGGL(k_point_bin<false>, dim3(p.N), dim3(256), 0, stream, p);
-    if (p.any_bricks) hipLaunchKernelGGL(k_point_bin<true>, dim3(p.N), dim3(256), 0, stream, p);
+    if (p.any_cells) hipLaunchKernelGGL(k_point_bin<false>, dim3(p.N), dim3(bin_threads<false>()), 0, stream, p);
+    if (p.any_bricks) hipLaunchKernelGGL(k_point_bin<true>, dim3(p.N), dim3(bin_threads<true>()), 0, stream, p);
     if (p.any_bricks)
         hipLaunchKernelGGL(k_point_split, dim3(std::max(1, std::min(p.brick_max, 4096)), p.S, kSplitParts), dim3(256),
                            0, stream, p);
