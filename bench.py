@@ -207,10 +207,11 @@ def main() -> int:
                           "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP32_TFLOPS,
                           "traffic": traffic, "alg_flops_per_launch": kernel_flops[k], "avg_ms": passes[k]})
     if passes.get("bin", 0.0) > 0:
-        # binning moves bytes: per quadrature point a 16 B record (binned) or a 4 B volume
-        # (decided directly), plus the atom positions; it is bound by its atomics' latency
+        # binning moves bytes: per quadrature point its cell slot (a 4 B packed point, or a 16 B
+        # record on grids binned by cells) or a 4 B volume (decided directly), plus the atom
+        # positions; it is bound by its atomics' latency
         nq = stats["n_quad"] * n_atoms
-        bbytes = FL.bin_bytes(stats["n_points"], nq, n_atoms)
+        bbytes = FL.bin_bytes(stats["n_points"], nq, n_atoms, stats["slot_bytes"])
         ach = bbytes / (passes["bin"] * 1e-3) / 1e9
         traffic = None
         tpath = os.path.join(ROOT, "profiles", f"traffic_bin_{args.workload}.json")

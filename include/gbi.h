@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define GBI_ABI_VERSION 5
+#define GBI_ABI_VERSION 6
 
 typedef enum gbi_status {
     GBI_OK = 0,
@@ -124,6 +124,8 @@ typedef struct gbi_stats {
     double n_quad;        /* quadrature points per atom */
     double n_active;      /* points with a chain-rule term (0 < V < 1 for GBSW) */
     double n_frc_pairs;   /* point/atom pairs the chain-rule pass visits (active points x their lists) */
+    double slot_bytes;    /* bytes a binned point occupies in its cell's slots, averaged over the atoms:
+                             4 (packed point; systems binning by bricks) or 16 (FP32 record and packed point) */
 } gbi_stats;
 
 /* Device view of a context's buffers (for callers that keep data on the GPU). */

@@ -88,7 +88,8 @@ class Params(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
-                ("n_points", "n_vol_pairs", "n_cells", "n_entries", "n_atoms", "n_quad", "n_active", "n_frc_pairs")]
+                ("n_points", "n_vol_pairs", "n_cells", "n_entries", "n_atoms", "n_quad", "n_active", "n_frc_pairs",
+                 "slot_bytes")]
 
 
 class Tuning(ctypes.Structure):

@@ -2005,6 +2005,9 @@ gbi_status gbi_get_stats(gbi_ctx* ctx, gbi_stats* stats) {
     stats->n_quad = (double)ctx->NQ / ctx->N;
     stats->n_active = h[4];
     stats->n_frc_pairs = h[5];
+    double slot = 0.0;
+    for (const SysHost& s : ctx->sys) slot += (brick_edge(s) > 1 ? 4.0 : 16.0) * s.N;
+    stats->slot_bytes = slot / ctx->N;
     return GBI_OK;
 }
 

@@ -23,8 +23,10 @@ def pair_flops(n_atoms: int) -> float:
     return C_PAIR * n_atoms * (n_atoms - 1) / 2.0
 
 
-def bin_bytes(n_binned: float, n_quad_points: float, n_atoms: int) -> float:
-    """Algorithmic bytes of the binning pass: a 16 B record per binned point, a 4 B volume per
-    point decided directly (outside the grid, empty cell, buried in its owner), 24 B of
-    position per atom."""
-    return 16.0 * n_binned + 4.0 * (n_quad_points - n_binned) + 24.0 * n_atoms
+def bin_bytes(n_binned: float, n_quad_points: float, n_atoms: int, slot_bytes: float = 16.0) -> float:
+    """Algorithmic bytes of the binning pass: what a binned point occupies in its cell's slots
+    (slot_bytes: 4 for a packed point, 16 for an FP32 record and packed point), a 4 B volume
+    per point decided directly (outside the grid, empty cell, buried in its owner), 24 B of
+    position per atom.  The brick records that binning by bricks writes and reads again are
+    the method's own traffic, not algorithmic bytes."""
+    return slot_bytes * n_binned + 4.0 * (n_quad_points - n_binned) + 24.0 * n_atoms

@@ -116,12 +116,12 @@ def cmd_summarize(args) -> int:
         "launches": [n_f, n_w],
         "kernels": parts,
         "commands": [
-            "rocprofv3 --pmc FETCH_SIZE -f csv -d <dir_f> -o run -- python3 -m openmmcustomgbintegralforce_amd.traffic run "
-            f"--workload {args.workload}",
-            "rocprofv3 --pmc WRITE_SIZE -f csv -d <dir_w> -o run -- python3 -m openmmcustomgbintegralforce_amd.traffic run "
-            f"--workload {args.workload}",
-            f"python3 -m openmmcustomgbintegralforce_amd.traffic summarize <dir_f> <dir_w> --kernel {args.kernel} "
-            f"--workload {args.workload}",
+            f"rocprofv3 --pmc FETCH_SIZE -f csv -d {args.fetch_dir} -o run -- python3 -m "
+            f"openmmcustomgbintegralforce_amd.traffic run --workload {args.workload}",
+            f"rocprofv3 --pmc WRITE_SIZE -f csv -d {args.write_dir} -o run -- python3 -m "
+            f"openmmcustomgbintegralforce_amd.traffic run --workload {args.workload}",
+            f"python3 -m openmmcustomgbintegralforce_amd.traffic summarize {args.fetch_dir} {args.write_dir} "
+            f"--kernel {args.kernel} --workload {args.workload}",
         ],
     }
     print(json.dumps(out, indent=1))

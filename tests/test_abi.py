@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     for s in syms:
         assert hasattr(lib, s), s
     assert sorted(N.EXPORTS) == syms
-    assert lib.gbi_abi_version() == 5
+    assert lib.gbi_abi_version() == 6
 
 
 def test_defaults_are_reference_constants():
