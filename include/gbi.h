@@ -279,8 +279,12 @@ typedef struct gbi_custom_spec {
      * outside the range, or periodic[k] with period max - min and equal first and last values);
      * 1, 2, 3 Discrete1D/2D/3DFunction of 1, 2, 3 arguments (values[x + xsize (y + ysize z)]
      * at the rounded arguments, dims[3k..3k+2] = xsize, ysize, zsize; a non-finite result when
-     * an index leaves its size).  values: every function's values, concatenated.  The
-     * Continuous2D and Continuous3D kinds (4, 5) return GBI_ERR_UNSUPPORTED. */
+     * an index leaves its size); 4, 5 Continuous2D/3DFunction (values[x + xsize (y + ysize z)]
+     * at equally spaced points of [ranges[2k], ranges[2k + 1]] x [ranges_yz[4k], ranges_yz[4k + 1]]
+     * (x [ranges_yz[4k + 2], ranges_yz[4k + 3]]), at least 2 per axis; the bicubic (tricubic)
+     * Hermite patch whose knot derivatives come from cubic splines along each axis, natural and
+     * zero outside the ranges, or periodic on every axis).  values: every function's values,
+     * concatenated. */
     int32_t n_functions;
     const char* const* function_names;
     const int32_t* function_kinds;
@@ -289,6 +293,7 @@ typedef struct gbi_custom_spec {
     const double* function_ranges;
     const double* function_values;
     const int32_t* function_dims;
+    const double* function_ranges_yz;  /* 4 per function; read for the Continuous2D/3D kinds only */
 } gbi_custom_spec;
 
 /*

@@ -140,23 +140,29 @@ class CustomSpec(ctypes.Structure):
         ("function_ranges", ctypes.POINTER(ctypes.c_double)),
         ("function_values", ctypes.POINTER(ctypes.c_double)),
         ("function_dims", ctypes.POINTER(ctypes.c_int32)),
+        ("function_ranges_yz", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
-TAB_CONTINUOUS_1D, TAB_DISCRETE_1D, TAB_DISCRETE_2D, TAB_DISCRETE_3D = 0, 1, 2, 3
+TAB_CONTINUOUS_1D, TAB_DISCRETE_1D, TAB_DISCRETE_2D, TAB_DISCRETE_3D, TAB_CONTINUOUS_2D, TAB_CONTINUOUS_3D = 0, 1, 2, 3, 4, 5
 
 
 class Tabulated:
     """A tabulated function for general expressions (gbi_custom_spec): kind
     TAB_CONTINUOUS_1D (cubic spline through values equally spaced on [min, max], natural and
-    zero outside, or periodic) or TAB_DISCRETE_1D/2D/3D (values[x + xsize (y + ysize z)] at
-    the rounded arguments; dims = (xsize, ysize, zsize))."""
+    zero outside, or periodic), TAB_DISCRETE_1D/2D/3D (values[x + xsize (y + ysize z)] at the
+    rounded arguments; dims = (xsize, ysize, zsize)) or TAB_CONTINUOUS_2D/3D (values[x + xsize
+    (y + ysize z)] equally spaced on [min, max] x yrange (x zrange): the bicubic / tricubic
+    Hermite patch with spline knot derivatives)."""
 
-    def __init__(self, name, kind, values, min=0.0, max=1.0, periodic=False, dims=(0, 1, 1)):
+    def __init__(self, name, kind, values, min=0.0, max=1.0, periodic=False, dims=(0, 1, 1), yrange=(0.0, 1.0),
+                 zrange=(0.0, 1.0)):
         self.name, self.kind = str(name), int(kind)
         self.values = np.ascontiguousarray(values, dtype=np.float64).ravel()
         self.min, self.max, self.periodic = float(min), float(max), bool(periodic)
         self.dims = tuple(int(d) for d in dims)
+        self.yrange = (float(yrange[0]), float(yrange[1]))
+        self.zrange = (float(zrange[0]), float(zrange[1]))
 
 
 TERM_SINGLE, TERM_PAIR, TERM_PAIR_NO_EXCLUSIONS = 0, 1, 2
@@ -347,12 +353,15 @@ class Engine:
         fv = np.ascontiguousarray(np.concatenate([f.values for f in functions]) if functions else [0.0],
                                   dtype=np.float64)
         fdm = np.ascontiguousarray([d for f in functions for d in f.dims] or [0, 0, 0], dtype=np.int32)
-        keep += [_strings([f.name for f in functions]), fk, fs, fp, fr, fv, fdm]
+        fyz = np.ascontiguousarray([v for f in functions for v in (*f.yrange, *f.zrange)] or [0.0] * 4,
+                                   dtype=np.float64)
+        keep += [_strings([f.name for f in functions]), fk, fs, fp, fr, fv, fdm, fyz]
         spec = CustomSpec(len(param_names), keep[0], _dp(pv), int(radius_param), len(gnames), keep[1],
                           _dp(gvals) if len(gnames) else None, len(integrals), keep[2], len(values), keep[3], keep[4],
                           len(terms), keep[5], ttypes.ctypes.data_as(i32),
                           len(param_derivs), keep[6], len(functions), keep[7], fk.ctypes.data_as(i32),
-                          fs.ctypes.data_as(i32), fp.ctypes.data_as(i32), _dp(fr), _dp(fv), fdm.ctypes.data_as(i32))
+                          fs.ctypes.data_as(i32), fp.ctypes.data_as(i32), _dp(fr), _dp(fv), fdm.ctypes.data_as(i32),
+                          _dp(fyz))
         handle = ctypes.c_void_p()
         _check(eng._lib.gbi_create_custom(n, ctypes.byref(params), ctypes.byref(spec), ctypes.byref(handle)))
         eng._h = handle

@@ -1535,13 +1535,20 @@ gbi_status gbi_create_custom(int32_t n_atoms, const gbi_params* params, const gb
         gbi::TabFunction f;
         f.name = spec->function_names[k] ? spec->function_names[k] : "";
         f.kind = spec->function_kinds[k];
-        if (f.kind < gbi::TAB_CONTINUOUS_1D || f.kind > gbi::TAB_DISCRETE_3D)
-            return fail(GBI_ERR_UNSUPPORTED, "tabulated function '" + f.name +
-                                                 "': only Continuous1DFunction and Discrete1D/2D/3DFunction are "
-                                                 "supported");
-        if (f.kind >= gbi::TAB_DISCRETE_2D) {
+        if (f.kind < gbi::TAB_CONTINUOUS_1D || f.kind > gbi::TAB_CONTINUOUS_3D)
+            return fail(GBI_ERR_UNSUPPORTED, "tabulated function '" + f.name + "': unknown kind " +
+                                                 std::to_string(f.kind));
+        if (gbi::tab_arity(f.kind) > 1) {
             if (!spec->function_dims) return fail(GBI_ERR_INVALID_ARGUMENT, "multi-dimensional table without sizes");
             for (int d = 0; d < 3; ++d) f.dims[d] = spec->function_dims[3 * k + d];
+        }
+        if (f.kind == gbi::TAB_CONTINUOUS_2D || f.kind == gbi::TAB_CONTINUOUS_3D) {
+            if (!spec->function_ranges_yz)
+                return fail(GBI_ERR_INVALID_ARGUMENT, "Continuous2D/3D table without y and z ranges");
+            f.ymin = spec->function_ranges_yz[4 * k];
+            f.ymax = spec->function_ranges_yz[4 * k + 1];
+            f.zmin = spec->function_ranges_yz[4 * k + 2];
+            f.zmax = spec->function_ranges_yz[4 * k + 3];
         }
         const int n = spec->function_sizes[k];
         if (n < (f.kind == gbi::TAB_CONTINUOUS_1D ? 2 : 1))
@@ -1549,7 +1556,8 @@ gbi_status gbi_create_custom(int32_t n_atoms, const gbi_params* params, const gb
         f.periodic = spec->function_periodic[k] != 0;
         f.min = spec->function_ranges[2 * k];
         f.max = spec->function_ranges[2 * k + 1];
-        if (f.kind == gbi::TAB_CONTINUOUS_1D && !(f.min < f.max))
+        if ((f.kind == gbi::TAB_CONTINUOUS_1D || f.kind == gbi::TAB_CONTINUOUS_2D || f.kind == gbi::TAB_CONTINUOUS_3D) &&
+            !(f.min < f.max))
             return fail(GBI_ERR_INVALID_ARGUMENT, "tabulated function '" + f.name + "' needs min < max");
         f.values.assign(spec->function_values + off, spec->function_values + off + n);
         off += n;

@@ -330,6 +330,95 @@ DU d_tabd3(const Du<T, M>& a, const Du<T, M>& b, const Du<T, M>& c, const double
     const bool ok = tab_index((double)a.v, nx, i) & tab_index((double)b.v, ny, j) & tab_index((double)c.v, nz, k);
     return Du<T, M>::c(ok ? T(y[i + nx * (j + ny * k)]) : T(__builtin_nan("")));
 }
+// Continuous2D/3D: the knot interval of v on one axis (wrapped when periodic; false outside
+// the range otherwise), its width h and the local coordinate t in [0, 1]
+__device__ inline bool tab_cell(double v, double lo, double hi, int n, int periodic, int& k, double& t, double& h) {
+    if (periodic) v -= (hi - lo) * floor((v - lo) / (hi - lo));
+    else if (!(v >= lo && v <= hi)) return false;
+    h = (hi - lo) / (n - 1);
+    k = (int)floor((v - lo) / h);
+    k = k < 0 ? 0 : (k > n - 2 ? n - 2 : k);
+    t = (v - (lo + k * h)) / h;
+    return true;
+}
+// cubic Hermite weights on [0, 1]: w[0], w[1] of the values at 0 and 1, w[2], w[3] of the
+// slopes at 0 and 1 (per unit t); dw their derivatives in t
+__device__ inline void hermite_w(double t, double* w, double* dw) {
+    const double t2 = t * t, t3 = t2 * t;
+    w[0] = 2.0 * t3 - 3.0 * t2 + 1.0;
+    w[1] = 3.0 * t2 - 2.0 * t3;
+    w[2] = t3 - 2.0 * t2 + t;
+    w[3] = t3 - t2;
+    dw[0] = 6.0 * t2 - 6.0 * t;
+    dw[1] = 6.0 * t - 6.0 * t2;
+    dw[2] = 3.0 * t2 - 4.0 * t + 1.0;
+    dw[3] = 3.0 * t2 - 2.0 * t;
+}
+// bicubic patch from the knot tables f, f_x, f_y, f_xy
+DU d_tabc2(const Du<T, M>& a, const Du<T, M>& b, const double* f, const double* fx, const double* fy, const double* fxy,
+           int nx, int ny, double x0, double x1, double y0, double y1, int periodic) {
+    int i, j;
+    double t, u, hx, hy;
+    if (!tab_cell((double)a.v, x0, x1, nx, periodic, i, t, hx) || !tab_cell((double)b.v, y0, y1, ny, periodic, j, u, hy))
+        return Du<T, M>::c(T(0));
+    double wx[4], dwx[4], wy[4], dwy[4];
+    hermite_w(t, wx, dwx);
+    hermite_w(u, wy, dwy);
+    double v = 0.0, gx = 0.0, gy = 0.0;
+    for (int cb = 0; cb < 2; ++cb)
+        for (int ca = 0; ca < 2; ++ca) {
+            const int e = (i + ca) + nx * (j + cb);
+            const double c00 = f[e], c10 = fx[e] * hx, c01 = fy[e] * hy, c11 = fxy[e] * hx * hy;
+            const double X0 = wx[ca], X1 = wx[2 + ca], Y0 = wy[cb], Y1 = wy[2 + cb];
+            v += c00 * X0 * Y0 + c10 * X1 * Y0 + c01 * X0 * Y1 + c11 * X1 * Y1;
+            gx += c00 * dwx[ca] * Y0 + c10 * dwx[2 + ca] * Y0 + c01 * dwx[ca] * Y1 + c11 * dwx[2 + ca] * Y1;
+            gy += c00 * X0 * dwy[cb] + c10 * X1 * dwy[cb] + c01 * X0 * dwy[2 + cb] + c11 * X1 * dwy[2 + cb];
+        }
+    gx /= hx;
+    gy /= hy;
+    Du<T, M> r;
+    r.v = T(v);
+    for (int k = 0; k < M; ++k) r.d[k] = T(gx) * a.d[k] + T(gy) * b.d[k];
+    return r;
+}
+// tricubic patch from the knot tables g[0..7] = f, f_x, f_y, f_z, f_xy, f_xz, f_yz, f_xyz
+DU d_tabc3(const Du<T, M>& a, const Du<T, M>& b, const Du<T, M>& c, const double* f, const double* fx,
+           const double* fy, const double* fz, const double* fxy, const double* fxz, const double* fyz,
+           const double* fxyz, int nx, int ny, int nz, double x0, double x1, double y0, double y1, double z0,
+           double z1, int periodic) {
+    int i, j, l;
+    double t, u, s, hx, hy, hz;
+    if (!tab_cell((double)a.v, x0, x1, nx, periodic, i, t, hx) || !tab_cell((double)b.v, y0, y1, ny, periodic, j, u, hy) ||
+        !tab_cell((double)c.v, z0, z1, nz, periodic, l, s, hz))
+        return Du<T, M>::c(T(0));
+    double wx[4], dwx[4], wy[4], dwy[4], wz[4], dwz[4];
+    hermite_w(t, wx, dwx);
+    hermite_w(u, wy, dwy);
+    hermite_w(s, wz, dwz);
+    const double* g[8] = {f, fx, fy, fz, fxy, fxz, fyz, fxyz};
+    const int px[8] = {0, 1, 0, 0, 1, 1, 0, 1}, py[8] = {0, 0, 1, 0, 1, 0, 1, 1}, pz[8] = {0, 0, 0, 1, 0, 1, 1, 1};
+    double v = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+    for (int cc = 0; cc < 2; ++cc)
+        for (int cb = 0; cb < 2; ++cb)
+            for (int ca = 0; ca < 2; ++ca) {
+                const int e = (i + ca) + nx * ((j + cb) + ny * (l + cc));
+                for (int m = 0; m < 8; ++m) {
+                    const double k = g[m][e] * (px[m] ? hx : 1.0) * (py[m] ? hy : 1.0) * (pz[m] ? hz : 1.0);
+                    const double X = wx[2 * px[m] + ca], Y = wy[2 * py[m] + cb], Z = wz[2 * pz[m] + cc];
+                    v += k * X * Y * Z;
+                    gx += k * dwx[2 * px[m] + ca] * Y * Z;
+                    gy += k * X * dwy[2 * py[m] + cb] * Z;
+                    gz += k * X * Y * dwz[2 * pz[m] + cc];
+                }
+            }
+    gx /= hx;
+    gy /= hy;
+    gz /= hz;
+    Du<T, M> r;
+    r.v = T(v);
+    for (int k = 0; k < M; ++k) r.d[k] = T(gx) * a.d[k] + T(gy) * b.d[k] + T(gz) * c.d[k];
+    return r;
+}
 )";
 
 void check_hiprtc(hiprtcResult r, const char* what) {
@@ -439,6 +528,47 @@ std::vector<double> spline_second_derivatives(const std::vector<double>& y, doub
     return M;
 }
 
+std::vector<double> spline_knot_derivatives(const std::vector<double>& y, double h, bool periodic) {
+    const std::vector<double> M = spline_second_derivatives(y, h, periodic);
+    const int n = (int)y.size();
+    std::vector<double> d(n);
+    // S'(x_i) from the piece to the right of knot i (the last knot: the piece to its left)
+    for (int i = 0; i + 1 < n; ++i) d[i] = (y[i + 1] - y[i]) / h - h * (2.0 * M[i] + M[i + 1]) / 6.0;
+    d[n - 1] = (y[n - 1] - y[n - 2]) / h + h * (M[n - 2] + 2.0 * M[n - 1]) / 6.0;
+    return d;
+}
+
+std::vector<std::vector<double>> hermite_tables(const TabFunction& f) {
+    const bool three = f.kind == TAB_CONTINUOUS_3D;
+    const int n[3] = {f.dims[0], f.dims[1], three ? f.dims[2] : 1};
+    const double lo[3] = {f.min, f.ymin, f.zmin}, hi[3] = {f.max, f.ymax, f.zmax};
+    double h[3];
+    for (int a = 0; a < (three ? 3 : 2); ++a) {
+        if (n[a] < 2) throw std::runtime_error("a continuous tabulated function needs at least 2 values per axis");
+        h[a] = (hi[a] - lo[a]) / (n[a] - 1);
+    }
+    // derivative along `axis` at the knots of every line of the table
+    auto along = [&](const std::vector<double>& in, int axis) {
+        std::vector<double> out(in.size()), line(n[axis]);
+        const int stride = axis == 0 ? 1 : (axis == 1 ? n[0] : n[0] * n[1]);
+        for (int z = 0; z < n[2]; ++z)
+            for (int y = 0; y < n[1]; ++y)
+                for (int x = 0; x < n[0]; ++x) {
+                    const int first = x + n[0] * (y + n[1] * z);
+                    if ((axis == 0 && x) || (axis == 1 && y) || (axis == 2 && z)) continue;  // one pass per line
+                    for (int k = 0; k < n[axis]; ++k) line[k] = in[first + k * stride];
+                    const std::vector<double> d = spline_knot_derivatives(line, h[axis], f.periodic);
+                    for (int k = 0; k < n[axis]; ++k) out[first + k * stride] = d[k];
+                }
+        return out;
+    };
+    const std::vector<double>& v = f.values;
+    const std::vector<double> fx = along(v, 0), fy = along(v, 1), fxy = along(fx, 1);
+    if (!three) return {v, fx, fy, fxy};
+    const std::vector<double> fz = along(v, 2), fxz = along(fx, 2), fyz = along(fy, 2), fxyz = along(fxy, 2);
+    return {v, fx, fy, fz, fxy, fxz, fyz, fxyz};
+}
+
 void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
     const int NP = (int)spec.params.size(), NG = (int)spec.globals.size(), NI = (int)spec.integrals.size();
     const int NV = (int)spec.value_names.size();
@@ -456,15 +586,20 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
     };
     for (const auto& f : spec.functions) {
         name_ok(f.name, "tabulated function");
-        if (f.kind < TAB_CONTINUOUS_1D || f.kind > TAB_DISCRETE_3D)
-            throw std::runtime_error("tabulated function '" + f.name + "': only Continuous1DFunction and "
-                                     "Discrete1D/2D/3DFunction are supported");
+        if (f.kind < TAB_CONTINUOUS_1D || f.kind > TAB_CONTINUOUS_3D)
+            throw std::runtime_error("tabulated function '" + f.name + "': unknown kind");
         if (f.values.empty()) throw std::runtime_error("tabulated function '" + f.name + "' has no values");
-        if (f.kind == TAB_DISCRETE_2D || f.kind == TAB_DISCRETE_3D) {
-            const long long want = (long long)f.dims[0] * f.dims[1] * (f.kind == TAB_DISCRETE_3D ? f.dims[2] : 1);
-            if (f.dims[0] < 1 || f.dims[1] < 1 || (f.kind == TAB_DISCRETE_3D && f.dims[2] < 1) ||
+        const int ar = tab_arity(f.kind);
+        if (ar > 1) {
+            const long long want = (long long)f.dims[0] * f.dims[1] * (ar == 3 ? f.dims[2] : 1);
+            const int least = f.kind == TAB_CONTINUOUS_2D || f.kind == TAB_CONTINUOUS_3D ? 2 : 1;
+            if (f.dims[0] < least || f.dims[1] < least || (ar == 3 && f.dims[2] < least) ||
                 want != (long long)f.values.size())
                 throw std::runtime_error("tabulated function '" + f.name + "': the sizes do not match the values");
+        }
+        if (f.kind == TAB_CONTINUOUS_2D || f.kind == TAB_CONTINUOUS_3D) {
+            if (!(f.min < f.max) || !(f.ymin < f.ymax) || (ar == 3 && !(f.zmin < f.zmax)))
+                throw std::runtime_error("tabulated function '" + f.name + "' needs min < max on every axis");
         }
         for (double v : f.values)
             if (!std::isfinite(v)) throw std::runtime_error("tabulated function '" + f.name + "' has a non-finite value");
@@ -509,9 +644,30 @@ void generate_custom_source(const CustomSpec& spec, CustomProgram& out) {
         } else if (f.kind == TAB_DISCRETE_2D) {
             s << "DU d_tab" << k << "(const Du<T, M>& a, const Du<T, M>& b) { return d_tabd2(a, b, TAB" << k << "_Y, "
               << f.dims[0] << ", " << f.dims[1] << "); }\n";
-        } else {
+        } else if (f.kind == TAB_DISCRETE_3D) {
             s << "DU d_tab" << k << "(const Du<T, M>& a, const Du<T, M>& b, const Du<T, M>& c) { return d_tabd3(a, b, c, TAB"
               << k << "_Y, " << f.dims[0] << ", " << f.dims[1] << ", " << f.dims[2] << "); }\n";
+        } else {
+            // knot tables of the bicubic / tricubic patch (the values are TABk_Y)
+            const std::vector<std::vector<double>> ht = hermite_tables(f);
+            static const char* tags[8] = {"_Y", "_X", "_YD", "_Z", "_XY", "_XZ", "_YZ", "_XYZ"};
+            const bool three = f.kind == TAB_CONTINUOUS_3D;
+            // 2D order f, f_x, f_y, f_xy; 3D order f, f_x, f_y, f_z, f_xy, f_xz, f_yz, f_xyz
+            const int tag_of[8] = {0, 1, 2, three ? 3 : 4, 4, 5, 6, 7};
+            for (size_t m = 1; m < ht.size(); ++m) table(tags[tag_of[m]], ht[m]);
+            std::ostringstream arg;
+            arg << std::setprecision(17);
+            for (size_t m = 0; m < ht.size(); ++m) arg << "TAB" << k << tags[tag_of[m]] << ", ";
+            if (!three) {
+                s << "DU d_tab" << k << "(const Du<T, M>& a, const Du<T, M>& b) { return d_tabc2(a, b, " << arg.str()
+                  << f.dims[0] << ", " << f.dims[1] << ", " << f.min << ", " << f.max << ", " << f.ymin << ", "
+                  << f.ymax << ", " << (f.periodic ? 1 : 0) << "); }\n";
+            } else {
+                s << "DU d_tab" << k << "(const Du<T, M>& a, const Du<T, M>& b, const Du<T, M>& c) { return d_tabc3(a, b, c, "
+                  << arg.str() << f.dims[0] << ", " << f.dims[1] << ", " << f.dims[2] << ", " << f.min << ", "
+                  << f.max << ", " << f.ymin << ", " << f.ymax << ", " << f.zmin << ", " << f.zmax << ", "
+                  << (f.periodic ? 1 : 0) << "); }\n";
+            }
         }
     }
     s << std::setprecision(6);

@@ -22,23 +22,42 @@ constexpr int kCustomChainWaves = 4; // waves of a cg_chain workgroup (one 64-at
 enum CustomTermType : int { TERM_SINGLE = 0, TERM_PAIR = 1, TERM_PAIR_NO_EXCL = 2 };
 
 // Tabulated functions (CharmmGBMVForce::addTabulatedFunction with OpenMM's
-// Continuous1DFunction and Discrete1D/2D/3DFunction): a cubic spline through values at
-// equally spaced points on [min, max] (natural, or periodic with period max - min; zero
-// outside the range when not periodic), or the value at the rounded arguments
-// (values[x + xsize (y + ysize z)], each rounded index inside its size).
-enum TabKind : int { TAB_CONTINUOUS_1D = 0, TAB_DISCRETE_1D = 1, TAB_DISCRETE_2D = 2, TAB_DISCRETE_3D = 3 };
+// Continuous1D/2D/3DFunction and Discrete1D/2D/3DFunction).  Continuous: values at equally
+// spaced points of [min, max] per axis (values[x + xsize (y + ysize z)]); one argument, the
+// cubic spline through them; two or three, the bicubic (tricubic) Hermite patch whose knot
+// derivatives d/dx, d/dy, d2/dxdy, ... come from cubic splines along each axis.  Natural
+// splines and zero outside the ranges, or periodic with period max - min per axis.
+// Discrete: the value at the rounded arguments, each rounded index inside its size.
+enum TabKind : int {
+    TAB_CONTINUOUS_1D = 0,
+    TAB_DISCRETE_1D = 1,
+    TAB_DISCRETE_2D = 2,
+    TAB_DISCRETE_3D = 3,
+    TAB_CONTINUOUS_2D = 4,
+    TAB_CONTINUOUS_3D = 5
+};
 struct TabFunction {
     std::string name;
     int kind = TAB_CONTINUOUS_1D;
     bool periodic = false;
-    double min = 0.0, max = 1.0;
-    int dims[3] = {0, 1, 1};  // discrete tables: xsize, ysize, zsize
+    double min = 0.0, max = 1.0;                          // x range of the continuous kinds
+    double ymin = 0.0, ymax = 1.0, zmin = 0.0, zmax = 1.0;  // Continuous2D/3D: y and z ranges
+    int dims[3] = {0, 1, 1};  // tables of 2 or 3 arguments: xsize, ysize, zsize
     std::vector<double> values;
 };
-inline int tab_arity(int kind) { return kind == TAB_DISCRETE_2D ? 2 : (kind == TAB_DISCRETE_3D ? 3 : 1); }
+inline int tab_arity(int kind) {
+    if (kind == TAB_DISCRETE_2D || kind == TAB_CONTINUOUS_2D) return 2;
+    if (kind == TAB_DISCRETE_3D || kind == TAB_CONTINUOUS_3D) return 3;
+    return 1;
+}
 // Second derivatives of the natural (periodic) cubic spline through `values` at equally spaced
 // points h apart; periodic splines need values.front() == values.back().  Throws on bad input.
 std::vector<double> spline_second_derivatives(const std::vector<double>& values, double h, bool periodic);
+// First derivatives of that spline at its knots.
+std::vector<double> spline_knot_derivatives(const std::vector<double>& values, double h, bool periodic);
+// The knot tables of a Continuous2D/3DFunction: {f, f_x, f_y, f_xy} or
+// {f, f_x, f_y, f_z, f_xy, f_xz, f_yz, f_xyz}, each indexed like the values.  Throws on bad input.
+std::vector<std::vector<double>> hermite_tables(const TabFunction& f);
 
 struct CustomSpec {
     std::vector<std::string> params;     // per-particle parameter names

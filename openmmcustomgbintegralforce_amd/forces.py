@@ -145,11 +145,28 @@ class Discrete3DFunction(TabulatedFunction):
 
 
 class Continuous2DFunction(TabulatedFunction):
-    """Two-dimensional table (declared for completeness; the HIP kernels do not evaluate it)."""
+    """values[x + xsize y] equally spaced on [xmin, xmax] x [ymin, ymax]: the bicubic patch
+    with spline knot derivatives (natural and zero outside, or periodic)."""
 
     def __init__(self, xsize, ysize, values, xmin, xmax, ymin, ymax, periodic=False):
-        self._args = (int(xsize), int(ysize), list(values), xmin, xmax, ymin, ymax)
+        self._args = (int(xsize), int(ysize), list(values), float(xmin), float(xmax), float(ymin), float(ymax))
         self._periodic = bool(periodic)
+
+    def getFunctionParameters(self):
+        return self._args
+
+
+class Continuous3DFunction(TabulatedFunction):
+    """values[x + xsize (y + ysize z)] equally spaced on [xmin, xmax] x [ymin, ymax] x
+    [zmin, zmax]: the tricubic patch with spline knot derivatives."""
+
+    def __init__(self, xsize, ysize, zsize, values, xmin, xmax, ymin, ymax, zmin, zmax, periodic=False):
+        self._args = (int(xsize), int(ysize), int(zsize), list(values), float(xmin), float(xmax), float(ymin),
+                      float(ymax), float(zmin), float(zmax))
+        self._periodic = bool(periodic)
+
+    def getFunctionParameters(self):
+        return self._args
 
 
 class CharmmGBMVForce:
@@ -530,9 +547,16 @@ def gbmv_custom_from_force(force: CharmmGBMVForce) -> dict:
         elif isinstance(fn, Discrete3DFunction):
             nx, ny, nz, v = fn.getFunctionParameters()
             functions.append(N.Tabulated(name, N.TAB_DISCRETE_3D, v, dims=(nx, ny, nz)))
+        elif isinstance(fn, Continuous2DFunction):
+            nx, ny, v, x0, x1, y0, y1 = fn.getFunctionParameters()
+            functions.append(N.Tabulated(name, N.TAB_CONTINUOUS_2D, v, x0, x1, fn.getPeriodic(), dims=(nx, ny, 1),
+                                         yrange=(y0, y1)))
+        elif isinstance(fn, Continuous3DFunction):
+            nx, ny, nz, v, x0, x1, y0, y1, z0, z1 = fn.getFunctionParameters()
+            functions.append(N.Tabulated(name, N.TAB_CONTINUOUS_3D, v, x0, x1, fn.getPeriodic(), dims=(nx, ny, nz),
+                                         yrange=(y0, y1), zrange=(z0, z1)))
         else:
-            raise OpenMMException(f"CharmmGBMVForce: tabulated function '{name}': the HIP platform evaluates "
-                                  "Continuous1DFunction and Discrete1D/2D/3DFunction only")
+            raise OpenMMException(f"CharmmGBMVForce: tabulated function '{name}': unknown kind")
     return dict(params=p, param_names=names, param_values=pv.reshape(-1, len(names)), radius_param=names.index("radius"),
                 integrals=[n for n, _ in integ], values=values, terms=terms, globals_=g,
                 param_derivs=list(force._param_derivs), functions=functions)

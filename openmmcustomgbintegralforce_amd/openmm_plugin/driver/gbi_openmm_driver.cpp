@@ -103,7 +103,7 @@ CharmmGBMVForce* derivGBMV(const std::vector<double>& q, const std::vector<doubl
     return f;
 }
 
-// generalGBMV with a Continuous1DFunction and Discrete1D/2D/3DFunctions, and the coordinates x, y, z
+// generalGBMV with Continuous1D/2D/3DFunctions and Discrete1D/2D/3DFunctions, and the coordinates x, y, z
 // in the Born value and in the single-particle term (the same expressions as
 // tests/test_gpu_openmm_plugin.py's oracle call).
 CharmmGBMVForce* tabGBMV(const std::vector<double>& q, const std::vector<double>& r, int type) {
@@ -121,9 +121,18 @@ CharmmGBMVForce* tabGBMV(const std::vector<double>& q, const std::vector<double>
     std::vector<double> cv(12);
     for (int k = 0; k < 12; ++k) cv[k] = 1.0 + 0.03 * k;
     f->addTabulatedFunction("tri3", new Discrete3DFunction(2, 2, 3, cv));
+    std::vector<double> s2(5 * 4), v3(4 * 3 * 3);
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 5; ++i) s2[i + 5 * j] = 0.2 * std::sin(0.7 * i) + 0.1 * j;
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 4; ++i) v3[i + 4 * (j + 3 * k)] = 1.0 + 0.05 * i - 0.03 * j + 0.02 * k * i;
+    f->addTabulatedFunction("surf", new Continuous2DFunction(5, 4, s2, -1.5, 1.5, -1.5, 1.5));
+    f->addTabulatedFunction("vol3", new Continuous3DFunction(4, 3, 3, v3, -1.5, 1.5, -1.5, 1.5, -1.5, 1.5));
     f->addComputedValue("B", "scale(10*radius)*lut(step(q), 10*radius)*(1+0.002*(x+y-z))/(1/radius-0.5*I4*radius)",
                         CharmmGBMVForce::SingleParticle);
-    f->addEnergyTerm("0.5*P*q^2/B*w(step(q))*tri3(step(q), step(radius-0.18), 20*radius-2)+0.01*q^2*(x^2+y^2+z^2)",
+    f->addEnergyTerm("0.5*P*q^2/B*w(step(q))*tri3(step(q), step(radius-0.18), 20*radius-2)+0.01*q^2*(x^2+y^2+z^2)"
+                     "+0.2*q^2*surf(x, y)+0.3*q^2*vol3(x, z, y)",
                      CharmmGBMVForce::SingleParticle);
     f->addEnergyTerm(gbi_openmm::kCanonicalPairEnergy, CharmmGBMVForce::ParticlePairNoExclusions);
     for (size_t i = 0; i < q.size(); ++i) f->addParticle({q[i], r[i]});

@@ -325,16 +325,17 @@ void HipCalcCharmmGBMVForceKernel::createCustom(const CharmmGBMVForce& force) {
         for (const auto& x : v) out.push_back(x.c_str());
         return out;
     };
-    // tabulated functions: OpenMM's Continuous1DFunction and Discrete1D/2D/3DFunction (dims: the
-    // sizes of the discrete tables, values[x + xsize (y + ysize z)])
+    // tabulated functions: OpenMM's Continuous1D/2D/3DFunction and Discrete1D/2D/3DFunction
+    // (dims: the sizes of the tables of 2 or 3 arguments, values[x + xsize (y + ysize z)];
+    // ranges_yz: the y and z ranges of the continuous ones)
     std::vector<std::string> fnames;
     std::vector<int32_t> fkinds, fsizes, fperiodic, fdims;
-    std::vector<double> franges, fvalues;
+    std::vector<double> franges, fvalues, franges_yz;
     for (int k = 0; k < force.getNumTabulatedFunctions(); ++k) {
         const TabulatedFunction& fn = force.getTabulatedFunction(k);
         fnames.push_back(force.getTabulatedFunctionName(k));
         std::vector<double> v;
-        double lo = 0.0, hi = 0.0;
+        double lo = 0.0, hi = 0.0, ylo = 0.0, yhi = 1.0, zlo = 0.0, zhi = 1.0;
         int nx = 0, ny = 1, nz = 1;
         if (const auto* c1 = dynamic_cast<const Continuous1DFunction*>(&fn)) {
             c1->getFunctionParameters(v, lo, hi);
@@ -348,10 +349,19 @@ void HipCalcCharmmGBMVForceKernel::createCustom(const CharmmGBMVForce& force) {
         } else if (const auto* d3 = dynamic_cast<const Discrete3DFunction*>(&fn)) {
             d3->getFunctionParameters(nx, ny, nz, v);
             fkinds.push_back(3);
+        } else if (const auto* c2 = dynamic_cast<const Continuous2DFunction*>(&fn)) {
+            c2->getFunctionParameters(nx, ny, v, lo, hi, ylo, yhi);
+            fkinds.push_back(4);
+        } else if (const auto* c3 = dynamic_cast<const Continuous3DFunction*>(&fn)) {
+            c3->getFunctionParameters(nx, ny, nz, v, lo, hi, ylo, yhi, zlo, zhi);
+            fkinds.push_back(5);
         } else {
-            throw OpenMMException("CharmmGBMVForce: tabulated function '" + fnames.back() +
-                                  "': the HIP platform evaluates Continuous1DFunction and Discrete1D/2D/3DFunction only");
+            throw OpenMMException("CharmmGBMVForce: tabulated function '" + fnames.back() + "': unknown kind");
         }
+        franges_yz.push_back(ylo);
+        franges_yz.push_back(yhi);
+        franges_yz.push_back(zlo);
+        franges_yz.push_back(zhi);
         fdims.push_back(nx);
         fdims.push_back(ny);
         fdims.push_back(nz);
@@ -372,6 +382,7 @@ void HipCalcCharmmGBMVForceKernel::createCustom(const CharmmGBMVForce& force) {
     spec.function_ranges = franges.data();
     spec.function_values = fvalues.data();
     spec.function_dims = fdims.data();
+    spec.function_ranges_yz = franges_yz.data();
     spec.n_params = np;
     spec.param_names = cp.data();
     spec.param_values = pvals.data();

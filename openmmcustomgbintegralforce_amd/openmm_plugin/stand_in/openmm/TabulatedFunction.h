@@ -1,7 +1,6 @@
 // Stand-in of OpenMM's tabulated-function classes (openmm/TabulatedFunction.h), used only when
-// OpenMM's headers are not on the include path: the kinds the HIP kernels evaluate
-// (Continuous1DFunction, Discrete1D/2D/3DFunction) and Continuous2DFunction, which they
-// reject.  Same constructors and accessors as OpenMM's.
+// OpenMM's headers are not on the include path: Continuous1D/2D/3DFunction and
+// Discrete1D/2D/3DFunction.  Same constructors and accessors as OpenMM's.
 #pragma once
 
 #include <vector>
@@ -87,11 +86,49 @@ public:
         : xsize_(xsize), ysize_(ysize), values_(values), xmin_(xmin), xmax_(xmax), ymin_(ymin), ymax_(ymax) {
         periodic_ = periodic;
     }
+    void getFunctionParameters(int& xsize, int& ysize, std::vector<double>& values, double& xmin, double& xmax,
+                               double& ymin, double& ymax) const {
+        xsize = xsize_;
+        ysize = ysize_;
+        values = values_;
+        xmin = xmin_;
+        xmax = xmax_;
+        ymin = ymin_;
+        ymax = ymax_;
+    }
 
 private:
     int xsize_, ysize_;
     std::vector<double> values_;
     double xmin_, xmax_, ymin_, ymax_;
+};
+
+class Continuous3DFunction : public TabulatedFunction {
+public:
+    Continuous3DFunction(int xsize, int ysize, int zsize, const std::vector<double>& values, double xmin, double xmax,
+                         double ymin, double ymax, double zmin, double zmax, bool periodic = false)
+        : xsize_(xsize), ysize_(ysize), zsize_(zsize), values_(values), xmin_(xmin), xmax_(xmax), ymin_(ymin),
+          ymax_(ymax), zmin_(zmin), zmax_(zmax) {
+        periodic_ = periodic;
+    }
+    void getFunctionParameters(int& xsize, int& ysize, int& zsize, std::vector<double>& values, double& xmin,
+                               double& xmax, double& ymin, double& ymax, double& zmin, double& zmax) const {
+        xsize = xsize_;
+        ysize = ysize_;
+        zsize = zsize_;
+        values = values_;
+        xmin = xmin_;
+        xmax = xmax_;
+        ymin = ymin_;
+        ymax = ymax_;
+        zmin = zmin_;
+        zmax = zmax_;
+    }
+
+private:
+    int xsize_, ysize_, zsize_;
+    std::vector<double> values_;
+    double xmin_, xmax_, ymin_, ymax_, zmin_, zmax_;
 };
 
 }  // namespace OpenMM

@@ -19,6 +19,7 @@ Discrete1D/2D/3DFunction the value at the rounded arguments, round half away fro
 from __future__ import annotations
 
 import ctypes
+import itertools
 import re
 
 import numpy as np
@@ -47,13 +48,79 @@ def _functions():
 class Tabulated:
     """A tabulated function: kind 0 Continuous1DFunction (values equally spaced on [min, max],
     periodic or natural spline), kinds 1, 2, 3 Discrete1D/2D/3DFunction (dims: xsize, ysize,
-    zsize; values[x + xsize (y + ysize z)] at the rounded arguments)."""
+    zsize; values[x + xsize (y + ysize z)] at the rounded arguments), kinds 4, 5
+    Continuous2D/3DFunction (values[x + xsize (y + ysize z)] equally spaced on [min, max] x
+    yrange (x zrange)).
 
-    def __init__(self, name, kind, values, min=0.0, max=1.0, periodic=False, dims=(0, 1, 1)):
+    Continuous2D/3D follow the published method of OpenMM's classes (third party, absent here):
+    at every knot the derivatives d/dx, d/dy (d/dz) come from cubic splines through the table's
+    lines along each axis (natural, or periodic), the mixed ones from splines of those along the
+    other axes, and between knots the function is the bicubic (tricubic) Hermite patch that
+    matches them, the unique one of its degree; zero outside the ranges unless periodic."""
+
+    def __init__(self, name, kind, values, min=0.0, max=1.0, periodic=False, dims=(0, 1, 1), yrange=(0.0, 1.0),
+                 zrange=(0.0, 1.0)):
         self.name, self.kind = name, int(kind)
         self.values = np.asarray(values, dtype=np.float64)
         self.min, self.max, self.periodic = float(min), float(max), bool(periodic)
         self.dims = tuple(int(d) for d in dims)
+        self.yrange = (float(yrange[0]), float(yrange[1]))
+        self.zrange = (float(zrange[0]), float(zrange[1]))
+
+    def _patch(self):
+        """(value, [d/dx, d/dy(, d/dz)]) callables of a Continuous2D/3DFunction."""
+        nd = 2 if self.kind == 4 else 3
+        n = self.dims[:nd]
+        lo = [self.min, self.yrange[0], self.zrange[0]][:nd]
+        hi = [self.max, self.yrange[1], self.zrange[1]][:nd]
+        h = [(hi[a] - lo[a]) / (n[a] - 1) for a in range(nd)]
+        V = self.values.reshape(n[::-1]).transpose()  # V[x, y(, z)]
+        bc = "periodic" if self.periodic else "natural"
+
+        def along(A, a):
+            xs = lo[a] + np.arange(n[a]) * h[a]
+            return CubicSpline(xs, A, axis=a, bc_type=bc)(xs, 1)
+
+        tables = {(0,) * nd: V}
+        for p in sorted(itertools.product((0, 1), repeat=nd), key=sum):
+            if sum(p) == 0:
+                continue
+            a = max(i for i in range(nd) if p[i])  # differentiate the table without axis a along a
+            q = tuple(p[i] if i != a else 0 for i in range(nd))
+            tables[p] = along(tables[q], a)
+
+        def hermite(t):
+            t2, t3 = t * t, t * t * t
+            return ([2 * t3 - 3 * t2 + 1, 3 * t2 - 2 * t3, t3 - 2 * t2 + t, t3 - t2],
+                    [6 * t2 - 6 * t, 6 * t - 6 * t2, 3 * t2 - 4 * t + 1, 3 * t2 - 2 * t])
+
+        def evaluate(which, *args):
+            args = np.broadcast_arrays(*[np.asarray(x, dtype=np.float64) for x in args])
+            inside = np.ones(args[0].shape, dtype=bool)
+            ks, ws = [], []
+            for a in range(nd):
+                v = args[a]
+                if self.periodic:
+                    v = v - (hi[a] - lo[a]) * np.floor((v - lo[a]) / (hi[a] - lo[a]))
+                else:
+                    inside &= (v >= lo[a]) & (v <= hi[a])
+                    v = np.clip(v, lo[a], hi[a])
+                k = np.clip(np.floor((v - lo[a]) / h[a]).astype(np.int64), 0, n[a] - 2)
+                ks.append(k)
+                ws.append(hermite((v - (lo[a] + k * h[a])) / h[a]))
+            out = np.zeros(args[0].shape)
+            for corner in itertools.product((0, 1), repeat=nd):
+                idx = tuple(ks[a] + corner[a] for a in range(nd))
+                for p, T in tables.items():
+                    term = T[idx] * np.prod([h[a] for a in range(nd) if p[a]])
+                    for a in range(nd):
+                        term = term * ws[a][1 if a == which else 0][2 * p[a] + corner[a]]
+                    out = out + term
+            if which is not None:
+                out = out / h[which]
+            return np.where(inside, out, 0.0)
+
+        return (lambda *a: evaluate(None, *a)), [(lambda *a, w=w: evaluate(w, *a)) for w in range(nd)]
 
     def numeric(self):
         """(f, f') as numpy callables (f' of the one-argument kinds; the discrete tables are
@@ -102,6 +169,18 @@ class Tabulated:
         return make(0), make(1)
 
     def sympy_function(self):
+        if self.kind in (4, 5):
+            f, parts = self._patch()
+            pcls = []
+            for w, pf in enumerate(parts):
+                c = sp.Function(f"{self.name}__d{w}")
+                c._imp_ = staticmethod(pf)
+                pcls.append(c)
+
+            def fdiffn(obj, argindex=1):
+                return pcls[argindex - 1](*obj.args)
+
+            return type(self.name, (sp.Function,), {"fdiff": fdiffn, "_imp_": staticmethod(f)})
         f, df = self.numeric()
         if df is None:  # piecewise constant in every argument
             def fdiff0(obj, argindex=1):

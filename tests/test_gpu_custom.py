@@ -99,13 +99,14 @@ def test_one_integral_and_graph_replay():
 
 
 def _lib_table(t):
-    return N.Tabulated(t.name, t.kind, t.values, t.min, t.max, t.periodic, t.dims)
+    return N.Tabulated(t.name, t.kind, t.values, t.min, t.max, t.periodic, t.dims, t.yrange, t.zrange)
 
 
 @pytest.mark.parametrize("method", [N.NO_CUTOFF, N.CUTOFF_PERIODIC])
 def test_coordinates_and_tabulated_functions(method):
     """Computed values and single-particle terms using the atom's x, y, z (their gradients are
-    forces on the atom itself), a natural and a periodic Continuous1DFunction and
+    forces on the atom itself), a natural and a periodic Continuous1DFunction, a periodic
+    Continuous2DFunction and a natural Continuous3DFunction of the coordinates, and
     Discrete1D/2D/3DFunctions, against the sympy/scipy oracle."""
     s = S.globule(500, 77)
     L = np.array([4.0, 4.1, 4.2])

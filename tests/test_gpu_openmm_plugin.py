@@ -252,7 +252,7 @@ def test_python_kernel_mirror_context_parameters_and_derivatives():
 
 
 def test_cpp_kernel_classes_tabulated_functions_and_coordinates(tmp_path):
-    """Driver model 4: a CharmmGBMVForce with a Continuous1DFunction, Discrete1D/2D/3DFunctions
+    """Driver model 4: a CharmmGBMVForce with Continuous1D/2D/3DFunctions, Discrete1D/2D/3DFunctions
     and the coordinates x, y, z in its Born value and single-particle term, through the C++ kernel
     class (OpenMM's tabulated-function objects converted to the C ABI's tables)."""
     from oracle import custom as C
@@ -266,12 +266,18 @@ def test_cpp_kernel_classes_tabulated_functions_and_coordinates(tmp_path):
     w = C.Tabulated("w", 1, [0.9, 1.1])
     lut = C.Tabulated("lut", 2, [0.7, 0.8, 0.95, 1.05, 1.1, 1.2], dims=(2, 3, 1))
     tri3 = C.Tabulated("tri3", 3, 1.0 + 0.03 * np.arange(12), dims=(2, 2, 3))
+    i2, j2 = np.meshgrid(np.arange(5), np.arange(4), indexing="ij")
+    surf = C.Tabulated("surf", 4, (0.2 * np.sin(0.7 * i2) + 0.1 * j2).T.ravel(), -1.5, 1.5, dims=(5, 4, 1),
+                       yrange=(-1.5, 1.5))
+    i3, j3, k3 = np.meshgrid(np.arange(4), np.arange(3), np.arange(3), indexing="ij")
+    vol3 = C.Tabulated("vol3", 5, (1.0 + 0.05 * i3 - 0.03 * j3 + 0.02 * k3 * i3).transpose(2, 1, 0).ravel(), -1.5, 1.5,
+                       dims=(4, 3, 3), yrange=(-1.5, 1.5), zrange=(-1.5, 1.5))
     E, F, _, _ = C.compute(oracle_params(p), s.pos, ["q", "radius"], np.stack([s.charge, s.radius], 1), 1, ["I4"],
                            [("B", "scale(10*radius)*lut(step(q), 10*radius)*(1+0.002*(x+y-z))/(1/radius-0.5*I4*radius)")],
                            [("0.5*P*q^2/B*w(step(q))*tri3(step(q), step(radius-0.18), 20*radius-2)"
-                             "+0.01*q^2*(x^2+y^2+z^2)", C.SINGLE),
+                             "+0.01*q^2*(x^2+y^2+z^2)+0.2*q^2*surf(x, y)+0.3*q^2*vol3(x, z, y)", C.SINGLE),
                             ("P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))", C.PAIR_NO_EXCLUSIONS)], {"P": P},
-                           functions=[scale, w, lut, tri3])
+                           functions=[scale, w, lut, tri3, surf, vol3])
     print(f"[openmm-plugin] tables + x,y,z N={s.n}: energy {e:.6f} vs {E:.6f} force {rel_rms(f, F):.2e}", flush=True)
     assert rel_err(e, E) < 1e-5
     assert rel_rms(f, F) < 1e-5
@@ -279,8 +285,8 @@ def test_cpp_kernel_classes_tabulated_functions_and_coordinates(tmp_path):
 
 def test_python_kernel_mirror_tabulated_functions():
     """forces.CharmmGBMVForce.addTabulatedFunction through the Python kernel mirror (a
-    Continuous1DFunction, then a Discrete2DFunction); a Continuous2DFunction is refused with
-    OpenMMException."""
+    Continuous1DFunction, a Discrete2DFunction, a Continuous2DFunction); a table class the
+    platform does not know is refused with OpenMMException."""
     from oracle import custom as C
     from openmmcustomgbintegralforce_amd import forces as FM
 
@@ -330,6 +336,24 @@ def test_python_kernel_mirror_tabulated_functions():
                              functions=[C.Tabulated("scale", 2, d2, dims=(2, 3, 1))])
     print(f"[openmm-plugin] python mirror Discrete2D: energy {e2:.6f} vs {E2:.6f}", flush=True)
     assert rel_err(e2, E2) < 1e-5 and rel_rms(ctx2.forces, F2) < 1e-5
+    # Continuous2DFunction of (10 radius, 10 B-free argument) on [0.5, 2.5] x [0, 1]
+    gx, gy = np.meshgrid(np.linspace(0.5, 2.5, 6), np.linspace(0.0, 1.0, 4), indexing="ij")
+    c2 = (1.0 + 0.1 * np.sin(gx) * np.cos(2 * gy)).T.ravel()
+    f3 = force(FM.Continuous2DFunction(6, 4, c2, 0.5, 2.5, 0.0, 1.0), "10*radius, 0.4")
+    k3 = FM.HipCalcCharmmGBMVForceKernel()
+    k3.initialize(FM.System(s.n), f3)
+    ctx3 = FM.ContextImpl(s.pos, forces=(f3,))
+    e3 = k3.execute(ctx3, True, True)
+    E3, F3, _, _ = C.compute(oracle_params(p), s.pos, ["q", "radius"], np.stack([s.charge, s.radius], 1), 1, ["I4"],
+                             [("B", "scale(10*radius, 0.4)/(1/radius-0.5*I4*radius)")],
+                             [("0.5*P*q^2/B+0.01*q^2*z^2", C.SINGLE),
+                              ("P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))", C.PAIR_NO_EXCLUSIONS)], {"P": P},
+                             functions=[C.Tabulated("scale", 4, c2, 0.5, 2.5, dims=(6, 4, 1), yrange=(0.0, 1.0))])
+    print(f"[openmm-plugin] python mirror Continuous2D: energy {e3:.6f} vs {E3:.6f}", flush=True)
+    assert rel_err(e3, E3) < 1e-5 and rel_rms(ctx3.forces, F3) < 1e-5
+
+    class Unknown(FM.TabulatedFunction):
+        pass
+
     with pytest.raises(FM.OpenMMException):
-        FM.HipCalcCharmmGBMVForceKernel().initialize(
-            FM.System(s.n), force(FM.Continuous2DFunction(2, 2, [1.0] * 4, 0.0, 1.0, 0.0, 1.0)))
+        FM.HipCalcCharmmGBMVForceKernel().initialize(FM.System(s.n), force(Unknown()))

@@ -17,6 +17,7 @@
 import ctypes
 
 import numpy as np
+from scipy.interpolate import CubicSpline
 import pytest
 
 import oracle as O
@@ -120,11 +121,28 @@ W = C.Tabulated("w", 1, [0.9, 1.1])
 # 0.185 and 0.2 nm, so 10*radius rounds to 1 or 2 and 20*radius-2 to 0, 1 or 2
 LUT = C.Tabulated("lut", 2, [0.7, 0.8, 0.95, 1.05, 1.1, 1.2], dims=(2, 3, 1))
 TRI3 = C.Tabulated("tri3", 3, 1.0 + 0.03 * np.arange(12), dims=(2, 2, 3))
-TABLES = [SCALE, WAVE, W, LUT, TRI3]
+
+
+def _grid(n, lo, hi):
+    return np.linspace(lo, hi, n)
+
+
+# Continuous2D, periodic with period 2 on both axes (first and last values of every line equal),
+# and Continuous3D, natural on [-1.5, 1.5]^3 (values[x + xsize (y + ysize z)])
+_SX, _SY = np.meshgrid(_grid(9, 0.0, 2.0), _grid(7, 0.0, 2.0), indexing="ij")
+_SURF = np.sin(np.pi * _SX) * np.cos(np.pi * _SY) + 0.5 * np.cos(np.pi * _SX)
+_SURF[-1, :], _SURF[:, -1] = _SURF[0, :], _SURF[:, 0]
+SURF = C.Tabulated("surf", 4, _SURF.T.ravel(), 0.0, 2.0, periodic=True, dims=(9, 7, 1), yrange=(0.0, 2.0))
+_VX, _VY, _VZ = np.meshgrid(_grid(6, -1.5, 1.5), _grid(5, -1.5, 1.5), _grid(7, -1.5, 1.5), indexing="ij")
+_VOL = np.exp(-0.5 * (_VX ** 2 + 0.7 * _VY ** 2)) * np.cos(_VZ) + 0.1 * _VX * _VZ
+VOL3 = C.Tabulated("vol3", 5, _VOL.transpose(2, 1, 0).ravel(), -1.5, 1.5, dims=(6, 5, 7), yrange=(-1.5, 1.5),
+                   zrange=(-1.5, 1.5))
+TABLES = [SCALE, WAVE, W, LUT, TRI3, SURF, VOL3]
 TAB_VALUES = [("B", "scale(10*radius)*lut(step(q), 10*radius)*(1+0.002*(x+y-z))/(1/radius-0.5*I4*radius)")]
 TAB_TERMS = [("0.5*P*q^2/B*w(step(q))*tri3(step(q), step(radius-0.18), 20*radius-2)"
-              "+0.01*q^2*(x^2+y^2+z^2)+0.05*q^2*wave(3*x)", C.SINGLE),
-             ("P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))*(1+0.05*wave(r))", C.PAIR_NO_EXCLUSIONS)]
+              "+0.01*q^2*(x^2+y^2+z^2)+0.05*q^2*wave(3*x)+0.2*q^2*surf(x, y)+0.3*q^2*vol3(x, z, y)", C.SINGLE),
+             ("P*q1*q2/sqrt(r^2+B1*B2*exp(-r^2/(4*B1*B2)))*(1+0.05*wave(r))*(1+0.02*surf(r, 10*B1))",
+              C.PAIR_NO_EXCLUSIONS)]
 
 
 def test_tabulated_functions_pinned_by_closed_forms():
@@ -152,6 +170,48 @@ def test_tabulated_functions_pinned_by_closed_forms():
         f2(np.array([0.0]), np.array([2.6]))
 
 
+def test_continuous_2d_3d_tables_pinned_by_closed_forms():
+    """Continuous2D/3DFunction: bilinear and trilinear data are reproduced exactly with their
+    gradients (natural splines reproduce linear lines, and a Hermite patch with exact knot
+    derivatives reproduces polynomials of degree <= 3 per axis); along a grid line the patch is
+    the 1-D spline of that line; zero outside the ranges; a periodic table of sin x cos y
+    converges to it."""
+    xs, ys, zs = np.linspace(-1.0, 2.0, 7), np.linspace(0.5, 1.5, 5), np.linspace(-0.5, 0.5, 4)
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    f, (fx, fy) = C.Tabulated("b", 4, (0.3 + 1.2 * X - 0.7 * Y + 0.4 * X * Y).T.ravel(), -1.0, 2.0, dims=(7, 5, 1),
+                              yrange=(0.5, 1.5))._patch()
+    px, py, pz = np.array([-1, -0.37, 0.3, 1.99, 2.0]), np.array([0.5, 0.77, 1.2, 1.5, 0.9]), np.array(
+        [-0.5, 0.1, 0.33, 0.5, -0.2])
+    assert np.allclose(f(px, py), 0.3 + 1.2 * px - 0.7 * py + 0.4 * px * py, atol=1e-13)
+    assert np.allclose(fx(px, py), 1.2 + 0.4 * py, atol=1e-12) and np.allclose(fy(px, py), -0.7 + 0.4 * px, atol=1e-12)
+    W = np.sin(1.3 * X) * np.cos(2 * Y) + 0.1 * X ** 3
+    f2, (f2x, _) = C.Tabulated("w", 4, W.T.ravel(), -1.0, 2.0, dims=(7, 5, 1), yrange=(0.5, 1.5))._patch()
+    cs = CubicSpline(xs, W[:, 2], bc_type="natural")
+    q = np.linspace(-1.0, 2.0, 13)
+    assert np.allclose(f2(q, np.full_like(q, ys[2])), cs(q), atol=1e-13)
+    assert np.allclose(f2x(q, np.full_like(q, ys[2])), cs(q, 1), atol=1e-12)
+    assert f2(np.array([2.01]), np.array([1.0]))[0] == 0.0 and f2(np.array([0.0]), np.array([0.49]))[0] == 0.0
+    X3, Y3, Z3 = np.meshgrid(xs, ys, zs, indexing="ij")
+    V3 = 1 + X3 - 2 * Y3 + 0.5 * Z3 + 0.3 * X3 * Y3 * Z3 - 0.2 * X3 * Z3
+    f3, (g1, g2, g3) = C.Tabulated("c", 5, V3.transpose(2, 1, 0).ravel(), -1.0, 2.0, dims=(7, 5, 4),
+                                   yrange=(0.5, 1.5), zrange=(-0.5, 0.5))._patch()
+    assert np.allclose(f3(px, py, pz), 1 + px - 2 * py + 0.5 * pz + 0.3 * px * py * pz - 0.2 * px * pz, atol=1e-13)
+    assert np.allclose(g1(px, py, pz), 1 + 0.3 * py * pz - 0.2 * pz, atol=1e-12)
+    assert np.allclose(g2(px, py, pz), -2 + 0.3 * px * pz, atol=1e-12)
+    assert np.allclose(g3(px, py, pz), 0.5 + 0.3 * px * py - 0.2 * px, atol=1e-12)
+    n = 41
+    g = np.linspace(0.0, 2 * np.pi, n)
+    PX, PY = np.meshgrid(g, g, indexing="ij")
+    P = np.sin(PX) * np.cos(PY)
+    P[-1, :], P[:, -1] = P[0, :], P[:, 0]
+    fp, (fpx, fpy) = C.Tabulated("p", 4, P.T.ravel(), 0.0, 2 * np.pi, periodic=True, dims=(n, n, 1),
+                                 yrange=(0.0, 2 * np.pi))._patch()
+    a, b = np.array([-7.0, 0.3, 3.0, 9.9]), np.array([1.0, -2.0, 7.5, 0.2])  # wrapped by the period
+    assert np.allclose(fp(a, b), np.sin(a) * np.cos(b), atol=2e-6)
+    assert np.allclose(fpx(a, b), np.cos(a) * np.cos(b), atol=2e-5)
+    assert np.allclose(fpy(a, b), -np.sin(a) * np.sin(b), atol=2e-5)
+
+
 def _tabulated(s, pos):
     pv = np.stack([s.charge, s.radius], 1)
     p = O.gbmv_params(2)
@@ -176,7 +236,16 @@ def test_coordinates_and_tabulated_functions_forces_are_the_gradient():
 def test_library_rejects_bad_tables_and_coordinates():
     tab = N.Tabulated("scale", N.TAB_CONTINUOUS_1D, [1.0, 1.2, 1.1], 0.0, 3.0)
     val = [("B", "scale(10*radius)/(1/radius-0.5*I4*radius)")]
-    assert _create(dict(values=val, functions=[N.Tabulated("scale", 4, [1.0, 2.0])])) == 2  # Continuous2D/3D
+    assert _create(dict(values=val, functions=[N.Tabulated("scale", 6, [1.0, 2.0])])) == 2  # unknown kind
+    c2 = dict(values=[("B", "surf(1, 0.5)/(1/radius-0.5*I4*radius)")])
+    assert _create(dict(c2, functions=[N.Tabulated("surf", 4, [1.0] * 6, 0, 1, dims=(3, 3, 1))])) == 1  # sizes
+    assert _create(dict(c2, functions=[N.Tabulated("surf", 4, [1.0] * 3, 0, 1, dims=(1, 3, 1))])) == 1  # >= 2 per axis
+    assert _create(dict(c2, functions=[N.Tabulated("surf", 4, [1.0] * 4, 0, 1, dims=(2, 2, 1),
+                                                   yrange=(1.0, 1.0))])) == 1                        # ymin < ymax
+    assert _create(dict(c2, functions=[N.Tabulated("surf", 4, [1.0, 2.0, 1.5, 2.5], 0, 1, True, dims=(2, 2, 1))])) == 1
+    assert _create(dict(c2, functions=[N.Tabulated("surf", 4, [1.0, 2.0, 1.5, 2.5], 0, 1, dims=(2, 2, 1))])) in (0, 8)
+    assert _create(dict(values=[("B", "vol(1, 0.5, 0.5)/(1/radius-0.5*I4*radius)")],
+                        functions=[N.Tabulated("vol", 5, np.arange(8.0), 0, 1, dims=(2, 2, 2))])) in (0, 8)
     d2 = N.Tabulated("lut", N.TAB_DISCRETE_2D, [1.0, 2.0, 3.0, 4.0], dims=(2, 2, 1))
     d3 = N.Tabulated("lut", N.TAB_DISCRETE_3D, [1.0] * 8, dims=(2, 2, 2))
     val2 = [("B", "lut(1, 0)/(1/radius-0.5*I4*radius)")]
