@@ -139,7 +139,8 @@ struct gbi_ctx {
     // device buffers
     DevBuf<gbi::SysDesc> d_sys;
     DevBuf<gbi::SysGeom> geom;
-    DevBuf<int> flags_all, atom_sys, cell_sys, bin_order;
+    DevBuf<int> flags_all, atom_sys, cell_sys, bin_order, bbox_done;
+    DevBuf<double> bbox_part;
     DevBuf<int4> pair_tiles;
     DevBuf<int> colblk;
     DevBuf<double> d_box;
@@ -224,6 +225,8 @@ struct gbi_ctx {
         c_ex_idx.release();
         cprog.reset();
         colpart.release();
+        bbox_part.release();
+        bbox_done.release();
         DevBuf<int>* ib[] = {&flags_all,      &atom_sys,     &cell_sys,       &bin_order,       &cell_fill,
                              &cell_start,     &cell_count_eff, &cell_atoms_tmp, &cell_atoms, &cell_pt_count,
                              &cell_act_count, &frec_base,    &worklist,       &worklist_sys, &work, &scan_tmp};
@@ -799,6 +802,10 @@ gbi::KParams make_kparams(gbi_ctx& c, bool adjoint) {
     k.excl_col = c.excl_col.p;
     k.excl_mask = c.excl_mask.p;
     k.flags_all = c.flags_all.p;
+    k.bbox_part = c.bbox_part.p;
+    k.bbox_done = c.bbox_done.p;
+    k.bbox_slices = 1;
+    for (const SysHost& s : c.sys) k.bbox_slices = std::max(k.bbox_slices, gbi::bbox_slices(s.N));
     k.atom_sys = c.atom_sys.p;
     k.cell_sys = c.cell_sys.p;
     k.pos = c.pos.p;
@@ -1148,6 +1155,8 @@ gbi_status create_batch(int S, const int32_t* n_atoms, const double* charge, con
     HIP_TRY(c->d_box.ensure(3 * (size_t)S));
     HIP_TRY(hipMemsetAsync(c->d_box.p, 0, sizeof(double) * 3 * S, c->stream));
     HIP_TRY(c->flags_all.ensure(1));
+    HIP_TRY(c->bbox_part.ensure((size_t)S * gbi::kBboxMaxSlices * gbi::kBboxPart));
+    HIP_TRY(c->bbox_done.ensure(S));
     HIP_TRY(c->atom_sys.ensure(N));
     HIP_TRY(c->pos.ensure(3 * N));
     HIP_TRY(c->d_charge.ensure(N));
@@ -1852,6 +1861,7 @@ gbi_status gbi_execute_noalloc(int32_t n_atoms, const double* charge, const doub
     sc.take(c.frec0, NQ), sc.take(c.frec1, NQ), sc.take(c.frec2, c.kinds[gbi::VOL_GBMV2] ? NQ : 1);
     sc.take(c.vval, NQ), sc.take(c.vrec, NQ);
     sc.take(c.brick_count, gbi::kBrickStride), sc.take(c.brick_rec, 1);
+    sc.take(c.bbox_part, (size_t)gbi::kBboxMaxSlices * gbi::kBboxPart), sc.take(c.bbox_done, 1);
     };
     ScratchCarver dry;
     carve(dry);

@@ -41,6 +41,15 @@ constexpr int kMaxAtoms = 1 << (32 - kQBits);
 constexpr float kQuantum = 16777216.0f;  // 2^24: chain-rule partials are summed exactly on this grid
 constexpr int kBrickStride = 32;         // ints between brick counters (one 128-byte line each)
 constexpr int kMaxBrickEdge = 4;         // cells per brick edge (<= 64 cells per brick)
+// Bounding box: a system's atoms in up to kBboxMaxSlices slices of about kBboxSlice atoms,
+// one workgroup each (the count depends on the system's size only).
+constexpr int kBboxSlice = 1024;
+constexpr int kBboxMaxSlices = 64;
+constexpr int kBboxPart = 10;            // per slice: min xyz, max xyz, sum xyz, any non-finite
+__host__ __device__ inline int bbox_slices(int n_atoms) {
+    const int b = (n_atoms + kBboxSlice - 1) / kBboxSlice;
+    return b < 1 ? 1 : (b > kBboxMaxSlices ? kBboxMaxSlices : b);
+}
 
 // Derivatives of the energy with respect to the constants of a system (adjoint).
 enum Theta : int {
@@ -178,6 +187,9 @@ struct KParams {
     const int* excl_col;     // per entry: its column block (>= the row block)
     const unsigned long long* excl_mask;  // per entry, 64 rows: bit c = pair (row, column c) excluded
     int* flags_all;          // OR of every system's flags
+    double* bbox_part;       // S x kBboxMaxSlices x 10: per slice of a system's atoms min, max, sum (xyz), bad
+    int* bbox_done;          // S: slices of the system whose bounding box is in bbox_part
+    int bbox_slices;         // the largest slice count of a system (bbox_slices())
     const int* atom_sys;     // N
     const int* cell_sys;     // cell_total
     // inputs
