@@ -1035,7 +1035,8 @@ gbi_status setup_host(gbi_ctx& c, int S, const int32_t* n_atoms, const double* c
     for (int s = 0; s < S; ++s) {
         ST_TRY(validate_params(params[s], n_atoms[s], custom));
         Ntot += n_atoms[s];
-        NQtot += (long long)n_atoms[s] * params[s].n_radial * params[s].n_lebedev;
+        // each system's points start on a multiple of 4 (16-byte vector loads of its V)
+        NQtot += ((long long)n_atoms[s] * params[s].n_radial * params[s].n_lebedev + 3) & ~3ll;
     }
     if (Ntot >= gbi::kMaxAtoms) return fail(GBI_ERR_INVALID_ARGUMENT, "too many atoms (at most 2^21 - 1)");
     if (NQtot >= (1ll << 31)) return fail(GBI_ERR_INVALID_ARGUMENT, "N * Q must be below 2^31");
@@ -1077,7 +1078,7 @@ gbi_status setup_host(gbi_ctx& c, int S, const int32_t* n_atoms, const double* c
         c.any_trunc = c.any_trunc || (h.cutoff() && h.P.model == GBI_MODEL_GBSW);
         atom += h.N;
         qrow += h.Q;
-        pt += h.N * h.Q;
+        pt += (h.N * h.Q + 3) & ~3;  // 4-aligned point blocks (NQ counts the padding)
         c.sys.push_back(h);
     }
     c.quad_rows = qrow;
@@ -2020,7 +2021,9 @@ gbi_status gbi_get_stats(gbi_ctx* ctx, gbi_stats* stats) {
     stats->n_cells = h[2];
     stats->n_entries = h[3];
     stats->n_atoms = ctx->N;
-    stats->n_quad = (double)ctx->NQ / ctx->N;
+    double nq = 0.0;  // quadrature points (NQ also counts each system's alignment padding)
+    for (const SysHost& s : ctx->sys) nq += (double)s.N * s.Q;
+    stats->n_quad = nq / ctx->N;
     stats->n_active = h[4];
     stats->n_frc_pairs = h[5];
     double slot = 0.0;

@@ -1272,12 +1272,30 @@ __global__ void __launch_bounds__(64) k_integrals(KParams p) {
     const int pbase = sd.pt_base + (i - sd.atom_begin) * Q;
     double a0 = 0.0, a1 = 0.0;
     const bool gbsw = sd.vol_kind == VOL_GBSW;
+    // Q a multiple of 4 (every system's points start on a multiple of 4): 4 points per lane
+    // and load (0.045 -> 0.037 ms at 25k GBSW atoms), in a fixed order either way
+    if ((Q & 3) == 0) {
+        const float4* v4 = reinterpret_cast<const float4*>(p.vval + pbase);
+        for (int q4 = lane; q4 < (Q >> 2); q4 += kWave) {
+            const float4 w = v4[q4];
+            const float vv[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int q = 4 * q4 + j;
+                const double v = (double)volume_of(vv[j]);
+                const double f = gbsw ? 1.0 - v : v;
+                a0 += p.quad_pw[qb + q] * f;
+                a1 += p.quad_pw[p.quad_rows + qb + q] * f;
+            }
+        }
+    } else {
 #pragma unroll 4
-    for (int q = lane; q < Q; q += kWave) {
-        const double v = (double)volume_of(p.vval[pbase + q]);
-        const double f = gbsw ? 1.0 - v : v;
-        a0 += p.quad_pw[qb + q] * f;
-        a1 += p.quad_pw[p.quad_rows + qb + q] * f;
+        for (int q = lane; q < Q; q += kWave) {
+            const double v = (double)volume_of(p.vval[pbase + q]);
+            const double f = gbsw ? 1.0 - v : v;
+            a0 += p.quad_pw[qb + q] * f;
+            a1 += p.quad_pw[p.quad_rows + qb + q] * f;
+        }
     }
     a0 = wave_sum(a0);
     a1 = wave_sum(a1);
@@ -2027,11 +2045,8 @@ __global__ void __launch_bounds__(64) k_owner_reduce(KParams p) {
     const double beta = sd.beta, lambda = sd.lambda;
     double a[3] = {0.0, 0.0, 0.0};
     double tb = 0.0, tl = 0.0, ts = 0.0;
-#pragma unroll 4
-    for (int q = lane; q < Q; q += kWave) {
+    auto point = [&](int q, float vv) {
         const int pid = pbase + q;
-        const double facs = sc * (e4 * p.quad_pw[qb + q] + e7 * p.quad_pw[p.quad_rows + qb + q]);
-        const float vv = p.vval[pid];
         const bool marked = __float_as_int(vv) < 0;
         const float4 v = p.vrec[marked ? pid : pbase];
         a[0] += marked ? (double)v.x : 0.0;
@@ -2040,6 +2055,7 @@ __global__ void __launch_bounds__(64) k_owner_reduce(KParams p) {
         if (ADJ && !gbsw) {
             const float S = p.vsig[pid];
             if (!isnan(S)) {
+                const double facs = sc * (e4 * p.quad_pw[qb + q] + e7 * p.quad_pw[p.quad_rows + qb + q]);
                 const double V = volume_of(vv);
                 const double u = V * (1.0 - V);
                 tb += facs * u * ((double)S - lambda);
@@ -2047,6 +2063,19 @@ __global__ void __launch_bounds__(64) k_owner_reduce(KParams p) {
                 ts += facs * u * beta * (double)S;
             }
         }
+    };
+    if ((Q & 3) == 0) {  // 4 points per lane and load, as k_integrals
+        const float4* v4 = reinterpret_cast<const float4*>(p.vval + pbase);
+        for (int q4 = lane; q4 < (Q >> 2); q4 += kWave) {
+            const float4 w = v4[q4];
+            point(4 * q4, w.x);
+            point(4 * q4 + 1, w.y);
+            point(4 * q4 + 2, w.z);
+            point(4 * q4 + 3, w.w);
+        }
+    } else {
+#pragma unroll 4
+        for (int q = lane; q < Q; q += kWave) point(q, p.vval[pbase + q]);
     }
     for (int d = 0; d < 3; ++d) a[d] = wave_sum(a[d]);
     if (ADJ) {
