@@ -979,10 +979,17 @@ __global__ void __launch_bounds__(256) k_point_split(KParams p) {
 }
 
 // Cells holding points, one list per volume kind (order irrelevant: every later sum is
-// order independent).  One atomic per wave and kind.
-__global__ void k_worklist(KParams p) {
+// order independent).  One atomic per workgroup and kind: all of them go to one counter
+// per kind, which serialises them (one per wave took 0.009 ms for 770 waves at 5,000 atoms,
+// one per 1,024-thread workgroup 0.002 ms; 0.014 -> 0.005 ms at 25,000).
+constexpr int kWorkThreads = 1024;
+
+__global__ void __launch_bounds__(kWorkThreads) k_worklist(KParams p) {
+    constexpr int kWaves = kWorkThreads / kWave;
+    __shared__ int s_cnt[VOL_KINDS][kWaves];
+    __shared__ int s_base[VOL_KINDS];
     if (batch_fatal(p)) return;
-    const int lane = threadIdx.x & (kWave - 1);
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     const int stride = gridDim.x * blockDim.x;
     for (int base = blockIdx.x * blockDim.x; base < p.cell_total; base += stride) {
         const int c = base + (int)threadIdx.x;
@@ -991,19 +998,29 @@ __global__ void k_worklist(KParams p) {
             s = p.cell_sys[c];
             vk = p.sys[s].vol_kind;
         }
+        unsigned long long m[VOL_KINDS];
+#pragma unroll
         for (int k = 0; k < VOL_KINDS; ++k) {
-            const unsigned long long m = __ballot(vk == k);
-            if (m == 0ull) continue;
-            const int leader = __ffsll((long long)m) - 1;
-            int off = 0;
-            if (lane == leader) off = atomicAdd(&p.work[k], __popcll(m));
-            off = __shfl(off, leader, kWave);
-            if (vk == k) {
-                const int slot = k * p.cell_total + off + __popcll(m & ((1ull << lane) - 1ull));
-                p.worklist[slot] = c;
-                p.worklist_sys[slot] = s;
-            }
+            m[k] = __ballot(vk == k);
+            if (lane == 0) s_cnt[k][wid] = __popcll(m[k]);
         }
+        __syncthreads();
+        if (threadIdx.x < VOL_KINDS) {
+            int total = 0;
+            for (int w = 0; w < kWaves; ++w) total += s_cnt[threadIdx.x][w];
+            s_base[threadIdx.x] = total > 0 ? atomicAdd(&p.work[threadIdx.x], total) : 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < VOL_KINDS; ++k) {
+            if (vk != k) continue;
+            int off = s_base[k];
+            for (int w = 0; w < wid; ++w) off += s_cnt[k][w];
+            const int slot = k * p.cell_total + off + __popcll(m[k] & ((1ull << lane) - 1ull));
+            p.worklist[slot] = c;
+            p.worklist_sys[slot] = s;
+        }
+        __syncthreads();  // s_cnt and s_base are reused by the next iteration
     }
 }
 
@@ -2227,8 +2244,8 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
     err = scan_exclusive(p.cell_pt_count, p.frec_base, p.cell_total, p.scan_tmp, p.NQ, p.flags_all,
                          FLAG_POINT_OVERFLOW, stream);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k_worklist, dim3(std::max(1, std::min((p.cell_total + 255) / 256, 4096))), dim3(256), 0,
-                       stream, p);
+    hipLaunchKernelGGL(k_worklist, dim3(std::max(1, std::min((p.cell_total + kWorkThreads - 1) / kWorkThreads, 4096))),
+                       dim3(kWorkThreads), 0, stream, p);
     const int pb = std::max(1, std::min(p.cell_total, p.persist_blocks));
     mark(2);
     if (kinds[VOL_GBSW]) hipLaunchKernelGGL(k_volume<VOL_GBSW>, dim3(pb), dim3(64), 0, stream, p);
