@@ -1118,16 +1118,29 @@ gbi_status host_tables(const gbi_ctx& c, HostTables& t) {
     t.atom_sys.assign(c.N, 0);
     for (int s = 0; s < c.S; ++s)
         std::fill(t.atom_sys.begin() + c.sys[s].atom_begin, t.atom_sys.begin() + c.sys[s].atom_begin + c.sys[s].N, s);
+    // binning order: each system's atoms with a stride coprime to its size, the systems
+    // interleaved round-robin, so the workgroups in flight at once spread over every system's
+    // counters.  System after system, most of them belonged to one small system and contended
+    // on its counters: 8 conformations of 2,000 atoms took 0.41 ms to bin, about what one
+    // 25,000-atom system takes; interleaved, the adjoint of that batch runs at 8,550 against
+    // 7,309 conformation evaluations per second, the 64-system mix at 4,294 against 4,120.
     t.bin_order.assign(c.N, 0);
-    for (const SysHost& h : c.sys) {
+    std::vector<std::vector<int>> seq(c.S);
+    for (int s = 0; s < c.S; ++s) {
+        const SysHost& h = c.sys[s];
         long long stride = 1;
         for (int cand : {7919, 7927, 7933, 7937, 7949})
             if (h.N % cand != 0) {
                 stride = cand;
                 break;
             }
-        for (int k = 0; k < h.N; ++k) t.bin_order[h.atom_begin + k] = h.atom_begin + (int)((k * stride) % h.N);
+        seq[s].resize(h.N);
+        for (int k = 0; k < h.N; ++k) seq[s][k] = h.atom_begin + (int)((k * stride) % h.N);
     }
+    std::vector<int> next(c.S, 0);
+    for (int w = 0; w < c.N;)
+        for (int s = 0; s < c.S; ++s)
+            if (next[s] < c.sys[s].N) t.bin_order[w++] = seq[s][next[s]++];
     t.pair_atom.resize(c.N);
     for (int k = 0; k < c.N; ++k) t.pair_atom[k] = k;
     return GBI_OK;
