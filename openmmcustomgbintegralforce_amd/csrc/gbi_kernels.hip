@@ -544,10 +544,12 @@ __device__ void fill_periodic_samples(const KParams& p, const SysDesc& sd, int s
 }
 
 // One workgroup per atom, lanes over its candidate cells: each member key goes to its cell's
-// raw list.  The CSR layout of the sorted lists is the scan of the counts.
+// raw list.  The CSR layout of the sorted lists is the scan of the counts.  Atoms in the
+// binning order (systems interleaved, each strided), so the workgroups in flight append to
+// different systems' and distant cells' counters.
 __global__ void __launch_bounds__(256) k_grid_fill(KParams p) {
     if (batch_fatal(p)) return;
-    const int j = blockIdx.x;
+    const int j = p.bin_order[blockIdx.x];
     const int s = p.atom_sys[j];
     const SysDesc& sd = p.sys[s];
     if (sd.grid_kind == GRID_BRUTE) {
