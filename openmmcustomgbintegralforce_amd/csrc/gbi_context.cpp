@@ -872,7 +872,7 @@ gbi_status ensure_adjoint_buffers(gbi_ctx& c) {
     HIP_TRY(c.dedq.ensure(c.N));
     HIP_TRY(c.dedr.ensure(c.N));
     HIP_TRY(c.dedr_fix.ensure(c.N));
-    HIP_TRY(c.vacc_fix.ensure((size_t)c.S * gbi::VA_COUNT));
+    HIP_TRY(c.vacc_fix.ensure((size_t)c.S * gbi::kVaccCopies * gbi::kVaccStride));
     HIP_TRY(c.theta_atom.ensure((size_t)gbi::TH_COUNT * c.N));
     HIP_TRY(c.theta.ensure((size_t)c.S * gbi::TH_COUNT));
     return GBI_OK;

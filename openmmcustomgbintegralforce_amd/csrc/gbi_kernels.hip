@@ -185,12 +185,13 @@ __global__ void k_clear(KParams p) {
     }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < p.brick_total; i += stride)
         p.brick_count[(long long)i * kBrickStride] = 0;
+    if (p.adjoint)
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < p.S * kVaccCopies * kVaccStride; i += stride)
+            p.vacc_fix[i] = 0;
     if (blockIdx.x == 0) {
         if (threadIdx.x < 8) p.work[threadIdx.x] = 0;
         if (threadIdx.x == 0) *p.flags_all = 0;
         for (int k = threadIdx.x; k < p.S; k += blockDim.x) p.bbox_done[k] = 0;
-        if (p.adjoint)
-            for (int k = threadIdx.x; k < p.S * VA_COUNT; k += blockDim.x) p.vacc_fix[k] = 0;
     }
 }
 
@@ -2036,7 +2037,7 @@ __global__ void __launch_bounds__(64, (ADJ || TRUNC) ? 2 : 3) k_force(KParams p)
             pacc0 = wave_sum(pacc0);
             pacc1 = wave_sum(pacc1);
             if (lane == 0) {
-                long long* va = p.vacc_fix + s * VA_COUNT;
+                long long* va = p.vacc_fix + ((long long)s * kVaccCopies + (w & (kVaccCopies - 1))) * kVaccStride;
                 const int i0 = VK == VOL_GBSW ? VA_W : VK == VOL_GBMV2 ? VA_A : VA_GAMMA;
                 if (pacc0 != 0.0) add_fix(&va[i0], pacc0);
                 if (VK == VOL_GBMV2 && pacc1 != 0.0) add_fix(&va[VA_AR], pacc1);
@@ -2157,7 +2158,11 @@ __global__ void __launch_bounds__(256) k_theta_reduce(KParams p) {
         if (tid == 0) p.theta[s * TH_COUNT + atom_rows[r]] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
     }
     if (tid == 0) {
-        const long long* va = p.vacc_fix + s * VA_COUNT;
+        long long va[VA_COUNT];
+        for (int k = 0; k < VA_COUNT; ++k) {
+            va[k] = 0;
+            for (int c = 0; c < kVaccCopies; ++c) va[k] += p.vacc_fix[((long long)s * kVaccCopies + c) * kVaccStride + k];
+        }
         double* th = p.theta + s * TH_COUNT;
         th[TH_SW] = th[TH_GAMMA0] = th[TH_P1] = th[TH_P2] = 0.0;
         if (sd.vol_kind == VOL_GBSW) {
