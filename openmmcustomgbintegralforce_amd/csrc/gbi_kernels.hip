@@ -2135,9 +2135,17 @@ __global__ void k_adjoint_atoms(KParams p) {
     }
 }
 
-// Adjoint, per system: fixed-order sums of the per-atom parts plus the force-pass sums.
-__global__ void __launch_bounds__(256) k_theta_reduce(KParams p) {
-    __shared__ double s_w[4];
+// Adjoint, per system: fixed-order sums of the per-atom parts plus the force-pass sums.  One
+// wave per sum (11 per-atom rows, then the VA_COUNT accumulators, lane c reading copy c), all
+// in flight together; thread 0 combines them.
+constexpr int kThetaRows = 11;
+constexpr int kThetaThreads = 1024;
+static_assert(kVaccCopies == kWave, "one lane per accumulator copy");
+static_assert((kThetaRows + VA_COUNT) * kWave <= kThetaThreads, "one wave per sum");
+
+__global__ void __launch_bounds__(kThetaThreads) k_theta_reduce(KParams p) {
+    __shared__ double s_row[kThetaRows];
+    __shared__ long long s_va[VA_COUNT];
     const int s = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const SysDesc& sd = p.sys[s];
@@ -2145,33 +2153,35 @@ __global__ void __launch_bounds__(256) k_theta_reduce(KParams p) {
         if (tid < TH_COUNT) p.theta[s * TH_COUNT + tid] = 0.0;
         return;
     }
-    const int atom_rows[] = {TH_PREFACTOR, TH_ALPHA0, TH_ALPHA1, TH_ETA, TH_SLOPE, TH_SHIFT,
-                             TH_RSCALE,    TH_ROFFSET, TH_BETA,  TH_LAMBDA, TH_S0};
-    for (int r = 0; r < (int)(sizeof(atom_rows) / sizeof(int)); ++r) {
-        const double* row = p.theta_atom + (size_t)atom_rows[r] * p.N;
-        double acc = 0.0;
-        for (int i = sd.atom_begin + tid; i < sd.atom_end; i += blockDim.x) acc += row[i];
-        acc = wave_sum(acc);
-        __syncthreads();
-        if (lane == 0) s_w[wid] = acc;
-        __syncthreads();
-        if (tid == 0) p.theta[s * TH_COUNT + atom_rows[r]] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    const int atom_rows[kThetaRows] = {TH_PREFACTOR, TH_ALPHA0, TH_ALPHA1, TH_ETA,   TH_SLOPE, TH_SHIFT,
+                                       TH_RSCALE,    TH_ROFFSET, TH_BETA,  TH_LAMBDA, TH_S0};
+    if (wid < kThetaRows) {
+        const double* row = p.theta_atom + (size_t)atom_rows[wid] * p.N;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};  // four loads in flight per lane
+        int i = sd.atom_begin + lane;
+        for (; i + 3 * kWave < sd.atom_end; i += 4 * kWave)
+            for (int u = 0; u < 4; ++u) acc[u] += row[i + u * kWave];
+        for (int u = 0; i < sd.atom_end; i += kWave, ++u) acc[u] += row[i];
+        const double sum = wave_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
+        if (lane == 0) s_row[wid] = sum;
+    } else if (wid < kThetaRows + VA_COUNT) {
+        const int k = wid - kThetaRows;
+        long long v = p.vacc_fix[((long long)s * kVaccCopies + lane) * kVaccStride + k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+        if (lane == 0) s_va[k] = v;
     }
+    __syncthreads();
     if (tid == 0) {
-        long long va[VA_COUNT];
-        for (int k = 0; k < VA_COUNT; ++k) {
-            va[k] = 0;
-            for (int c = 0; c < kVaccCopies; ++c) va[k] += p.vacc_fix[((long long)s * kVaccCopies + c) * kVaccStride + k];
-        }
         double* th = p.theta + s * TH_COUNT;
+        for (int r = 0; r < kThetaRows; ++r) th[atom_rows[r]] = s_row[r];
         th[TH_SW] = th[TH_GAMMA0] = th[TH_P1] = th[TH_P2] = 0.0;
         if (sd.vol_kind == VOL_GBSW) {
-            th[TH_SW] = -(double)va[VA_W] / kFix32;
+            th[TH_SW] = -(double)s_va[VA_W] / kFix32;
         } else if (sd.vol_kind == VOL_GBMV2) {
-            th[TH_P1] = -(double)va[VA_AR] / kFix32;
-            th[TH_P2] = -(double)va[VA_A] / kFix32;
+            th[TH_P1] = -(double)s_va[VA_AR] / kFix32;
+            th[TH_P2] = -(double)s_va[VA_A] / kFix32;
         } else {
-            const double gsum = -(double)va[VA_GAMMA] / kFix32;
+            const double gsum = -(double)s_va[VA_GAMMA] / kFix32;
             th[TH_GAMMA0] = gsum / (double)sd.gamma0;
             th[TH_LAMBDA] += gsum / ((double)sd.lambda * log((double)sd.lambda));
         }
@@ -2319,7 +2329,7 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
     if (adj) {
         hipLaunchKernelGGL(k_owner_reduce<true>, dim3(p.N), dim3(64), 0, stream, p);
         hipLaunchKernelGGL(k_adjoint_atoms, dim3((p.N + 255) / 256), dim3(256), 0, stream, p);
-        hipLaunchKernelGGL(k_theta_reduce, dim3(p.S), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL(k_theta_reduce, dim3(p.S), dim3(kThetaThreads), 0, stream, p);
     } else {
         hipLaunchKernelGGL(k_owner_reduce<false>, dim3(p.N), dim3(64), 0, stream, p);
     }
