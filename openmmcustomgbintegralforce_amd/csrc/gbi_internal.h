@@ -223,7 +223,11 @@ struct KParams {
     float4* ent_pos;         // entry_cap: position relative to the cell corner, .w = atom index bits
     float4* ent_par;         // entry_cap: aparam of the atom
     // points
-    int* cell_pt_count;      // cell_total
+    int* cell_pt_count;      // cell_total: points per cell (gathered from cell_ctr by the scan)
+    int* cell_ctr;           // ctr_len: the binning passes' point counters, cell c's at ctr_slot(c)
+                             // (ctr_perm = 1), or cell_pt_count itself (ctr_perm = 0)
+    int ctr_perm;
+    int ctr_len;             // cell_total rounded up to whole tiles of 4,096 (ctr_perm = 1)
     int* brick_count;        // brick_total x kBrickStride: point slots reserved in each brick (one counter per line)
     uint2* brick_rec;        // per system: the prec slot range, brick b owning [b sk^3 pt_cap, (b + 1) sk^3 pt_cap):
                              // (packed point, cell of the point inside the brick)

@@ -181,8 +181,9 @@ __global__ void k_clear(KParams p) {
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < p.cell_total; i += stride) {
         p.cell_fill[i] = 0;
-        p.cell_pt_count[i] = 0;
+        p.cell_ctr[i] = 0;
     }
+    for (int i = p.cell_total + blockIdx.x * blockDim.x + threadIdx.x; i < p.ctr_len; i += stride) p.cell_ctr[i] = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < p.brick_total; i += stride)
         p.brick_count[(long long)i * kBrickStride] = 0;
     if (p.adjoint)
@@ -668,7 +669,18 @@ __global__ void __launch_bounds__(64) k_grid_sort(KParams p) {
 // Exclusive scan of int arrays (two passes, deterministic)
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int* in, int* out, int n, int* block_sums) {
+// Slot of cell c's binning counter: within each tile of 4,096 counters, cell r of the tile at
+// (r mod 128) x 32 + r / 128, so the 32 counters of a 128-byte line belong to cells 128 apart
+// and the cells of one atom's points, mostly neighbours, reserve on different lines.  Dense:
+// the clear and the scan's gather read whole lines.
+__host__ __device__ inline long long ctr_slot(long long c) {
+    return (c & ~4095ll) | ((c & 127) << 5) | ((c >> 7) & 31);
+}
+__device__ inline long long cell_ctr_slot(const KParams& p, long long c) { return p.ctr_perm ? ctr_slot(c) : c; }
+
+// perm: the inputs are the binning counters in[ctr_slot(idx)], copied to compact[idx] on the way.
+__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int* in, int* out, int n, int* block_sums,
+                                                             int perm, int* compact) {
     __shared__ int s_wave[kScanThreads / kWave];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const long long base = (long long)blockIdx.x * kScanTile + (long long)tid * kScanItems;
@@ -676,7 +688,8 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int* in, int*
     int tsum = 0;
     for (int k = 0; k < kScanItems; ++k) {
         const long long idx = base + k;
-        v[k] = idx < n ? in[idx] : 0;
+        v[k] = idx < n ? in[perm ? ctr_slot(idx) : idx] : 0;
+        if (perm && idx < n) compact[idx] = v[k];
         tsum += v[k];
     }
     const int incl = wave_scan_incl(tsum, lane);
@@ -725,11 +738,12 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_add(int* out, int n, cons
 }
 
 // out[0..n] = exclusive scan of in[0..n-1] (out[n] = total); flags |= flag when total > limit.
+// perm: the inputs are in[ctr_slot(k)], also copied to compact[k].
 hipError_t scan_exclusive(const int* in, int* out, int n, int* tmp, int limit, int* flags, int flag,
-                          hipStream_t stream) {
+                          hipStream_t stream, int perm = 0, int* compact = nullptr) {
     const int nblocks = (n + kScanTile - 1) / kScanTile;
     if (nblocks > kScanTile || nblocks < 1) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_scan_tiles, dim3(nblocks), dim3(kScanThreads), 0, stream, in, out, n, tmp);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nblocks), dim3(kScanThreads), 0, stream, in, out, n, tmp, perm, compact);
     hipLaunchKernelGGL(k_scan_add, dim3(nblocks), dim3(kScanThreads), 0, stream, out, n, (const int*)tmp, limit, flags,
                        flag);
     return hipGetLastError();
@@ -885,7 +899,7 @@ __global__ void __launch_bounds__(bin_threads<BRICKS>()) k_point_bin(KParams p) 
             if constexpr (bricks)
                 t_cnt[k] = atomicAdd(&p.brick_count[(long long)(sd.brick_base + t_key[k]) * kBrickStride], n);
             else
-                t_cnt[k] = atomicAdd(&p.cell_pt_count[cb + t_key[k]], n);
+                t_cnt[k] = atomicAdd(&p.cell_ctr[cell_ctr_slot(p, cb + t_key[k])], n);
         }
     }
     __syncthreads();
@@ -957,7 +971,8 @@ __global__ void __launch_bounds__(256) k_point_split(KParams p) {
                 if (tid < sk3) s_off[tid] = incl - cnt;
                 if (cnt > 0) {
                     const int ix = bx * sk + tid / (sk * sk), iy = by * sk + (tid / sk) % sk, iz = bz * sk + tid % sk;
-                    s_base[tid] = atomicAdd(&p.cell_pt_count[cb + (ix * G.n[1] + iy) * G.n[2] + iz], cnt);
+                    const int c = cb + (ix * G.n[1] + iy) * G.n[2] + iz;
+                    s_base[tid] = atomicAdd(&p.cell_ctr[cell_ctr_slot(p, c)], cnt);
                 }
             }
             __syncthreads();
@@ -2258,8 +2273,9 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
         hipLaunchKernelGGL(k_point_split, dim3(std::max(1, std::min(p.brick_max, 4096)), p.S, kSplitParts), dim3(256),
                            0, stream, p);
     // records of the active points of cell c start at the exclusive prefix sum of the point counts
-    err = scan_exclusive(p.cell_pt_count, p.frec_base, p.cell_total, p.scan_tmp, p.NQ, p.flags_all,
-                         FLAG_POINT_OVERFLOW, stream);
+    // (and the per-line counters gathered into cell_pt_count)
+    err = scan_exclusive(p.cell_ctr, p.frec_base, p.cell_total, p.scan_tmp, p.NQ, p.flags_all, FLAG_POINT_OVERFLOW,
+                         stream, p.ctr_perm, p.cell_pt_count);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(k_worklist, dim3(std::max(1, std::min((p.cell_total + kWorkThreads - 1) / kWorkThreads, 4096))),
                        dim3(kWorkThreads), 0, stream, p);
