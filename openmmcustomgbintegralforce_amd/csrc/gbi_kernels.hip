@@ -38,6 +38,9 @@ constexpr int kScanThreads = 1024;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr double kFix32 = 4294967296.0;  // 2^32
+// s_waitcnt immediate for vmcnt(0) alone (gfx9 encoding: vmcnt in bits 3:0 and 15:14, expcnt
+// 6:4 and lgkmcnt 11:8 at their maxima, i.e. not waited for)
+constexpr int kWaitVmem = 0x0F70;
 
 __device__ inline double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
@@ -1871,6 +1874,10 @@ __global__ void __launch_bounds__(64, (ADJ || TRUNC) ? 2 : 3) k_force(KParams p)
                 acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
                 accr[k] = 0.0;
             }
+            // the atoms' registers complete here: otherwise every (point row, atom) block waits
+            // for them, and that wait (vmcnt counts stores too) also waits out the previous
+            // row's owner-force store
+            __builtin_amdgcn_s_waitcnt(kWaitVmem);
             for (int pb = 0; pb < nact; pb += kWave) {
                 const int cntp = min(kWave, nact - pb);
                 __syncthreads();
