@@ -819,8 +819,10 @@ __device__ inline uint32_t* pt_slot(const KParams& p, const SysDesc& sd, int c, 
 }
 
 // BRICKS: the launch for the systems binning by bricks (the other systems' workgroups return).
+// Bricks: 8 waves per SIMD (the system's fields held in scalar registers would otherwise
+// cost one); cells: register bound at 5.
 template <bool BRICKS>
-__global__ void __launch_bounds__(bin_threads<BRICKS>()) k_point_bin(KParams p) {
+__global__ void __launch_bounds__(bin_threads<BRICKS>(), BRICKS ? 8 : 1) k_point_bin(KParams p) {
     constexpr int kBinThreads = bin_threads<BRICKS>();
     constexpr int kBinPer = kBinHash / kBinThreads;  // points per thread
     // t_cnt: the points per key, then the key's first reserved slot
@@ -829,8 +831,10 @@ __global__ void __launch_bounds__(bin_threads<BRICKS>()) k_point_bin(KParams p) 
     const int tid = threadIdx.x;
     const int i = p.bin_order[blockIdx.x];
     const int s = p.atom_sys[i];
-    const SysDesc& sd = p.sys[s];
-    const SysGeom& G = p.geom[s];
+    // copies: the fields are read once, not again per point (the LDS atomics between points
+    // keep the compiler from reusing values loaded through a reference)
+    const SysDesc sd = p.sys[s];
+    const SysGeom G = p.geom[s];
     const int Q = sd.Q, qb = sd.quad_base, cb = sd.cell_base, sk = sd.sk, pt_cap = sd.pt_cap;
     constexpr bool bricks = BRICKS;
     if ((sk > 1) != BRICKS) return;
@@ -938,8 +942,8 @@ __global__ void __launch_bounds__(256) k_point_split(KParams p) {
     __shared__ int s_cnt[kCells], s_off[kCells], s_base[kCells];
     if (batch_fatal(p)) return;
     const int s = blockIdx.y, tid = threadIdx.x;
-    const SysDesc& sd = p.sys[s];
-    const SysGeom& G = p.geom[s];
+    const SysDesc sd = p.sys[s];  // copies, read once (as in k_point_bin)
+    const SysGeom G = p.geom[s];
     const int sk = sd.sk, sk3 = sk * sk * sk, cb = sd.cell_base, pt_cap = sd.pt_cap;
     const int nbrick = sk > 1 ? G.nb[0] * G.nb[1] * G.nb[2] : 0;  // cells binned directly: nothing to move
     const long long brick_slots = (long long)sk3 * pt_cap;
