@@ -716,11 +716,10 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int* in, int*
 
 // Adds to tile t the sum of the tile sums before it (each workgroup sums them itself: at
 // most kScanTile ints, exact in any order); the last tile also writes the total.
-__global__ void __launch_bounds__(kScanThreads) k_scan_add(int* out, int n, const int* block_sums, int limit,
-                                                           int* flags, int flag) {
-    __shared__ int s_wave[kScanThreads / kWave];
+// Sum of the tile sums before tile t, by a whole workgroup of kScanThreads threads (every
+// thread gets it; s_wave is reused on return).
+__device__ inline int tiles_before(const int* block_sums, int t, int* s_wave) {
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
-    const int t = blockIdx.x;
     int part = 0;
     for (int k = tid; k < t; k += kScanThreads) part += block_sums[k];
     const int wsum = wave_scan_incl(part, lane);
@@ -728,6 +727,16 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_add(int* out, int n, cons
     __syncthreads();
     int add = 0;
     for (int w = 0; w < kScanThreads / kWave; ++w) add += s_wave[w];
+    __syncthreads();
+    return add;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_add(int* out, int n, const int* block_sums, int limit,
+                                                           int* flags, int flag) {
+    __shared__ int s_wave[kScanThreads / kWave];
+    const int tid = threadIdx.x;
+    const int t = blockIdx.x;
+    const int add = tiles_before(block_sums, t, s_wave);
     if (t == (int)gridDim.x - 1 && tid == 0) {
         const int total = add + block_sums[t];
         out[n] = total;
@@ -1007,15 +1016,32 @@ __global__ void __launch_bounds__(256) k_point_split(KParams p) {
 // order independent).  One atomic per workgroup and kind: all of them go to one counter
 // per kind, which serialises them (one per wave took 0.009 ms for 770 waves at 5,000 atoms,
 // one per 1,024-thread workgroup 0.002 ms; 0.014 -> 0.005 ms at 25,000).
+// It also finishes the exclusive scan of the point counts (frec_base), whose tile pass ran
+// before it: each chunk of kWorkThreads cells lies in one scan tile and adds the sum of the
+// tile sums before it (the work of k_scan_add, without a launch of its own).
 constexpr int kWorkThreads = 1024;
+static_assert(kWorkThreads == kScanThreads && kScanTile % kWorkThreads == 0, "a chunk lies in one scan tile");
 
 __global__ void __launch_bounds__(kWorkThreads) k_worklist(KParams p) {
     constexpr int kWaves = kWorkThreads / kWave;
     __shared__ int s_cnt[VOL_KINDS][kWaves];
     __shared__ int s_base[VOL_KINDS];
-    if (batch_fatal(p)) return;
+    __shared__ int s_scan[kScanThreads / kWave];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     const int stride = gridDim.x * blockDim.x;
+    const int ntiles = (p.cell_total + kScanTile - 1) / kScanTile;
+    for (int base = blockIdx.x * blockDim.x; base < p.cell_total; base += stride) {
+        const int t = base / kScanTile;
+        const int add = tiles_before(p.scan_tmp, t, s_scan);
+        const int c = base + (int)threadIdx.x;
+        if (c < p.cell_total) p.frec_base[c] += add;
+        if (c == p.cell_total - 1) {  // the last cell: the total and its limit
+            const int total = add + p.scan_tmp[ntiles - 1];
+            p.frec_base[p.cell_total] = total;
+            if (total > p.NQ) atomicOr(p.flags_all, FLAG_POINT_OVERFLOW);
+        }
+    }
+    if (batch_fatal(p)) return;
     for (int base = blockIdx.x * blockDim.x; base < p.cell_total; base += stride) {
         const int c = base + (int)threadIdx.x;
         int vk = -1, s = 0;
@@ -2286,10 +2312,13 @@ hipError_t launch_evaluate(const KParams& p, const bool* kinds, hipStream_t stre
         hipLaunchKernelGGL(k_point_split, dim3(std::max(1, std::min(p.brick_max, 4096)), p.S, kSplitParts), dim3(256),
                            0, stream, p);
     // records of the active points of cell c start at the exclusive prefix sum of the point counts
-    // (and the per-line counters gathered into cell_pt_count)
-    err = scan_exclusive(p.cell_ctr, p.frec_base, p.cell_total, p.scan_tmp, p.NQ, p.flags_all, FLAG_POINT_OVERFLOW,
-                         stream, p.ctr_perm, p.cell_pt_count);
-    if (err != hipSuccess) return err;
+    // (the counters gathered into cell_pt_count); k_worklist adds the tile offsets
+    {
+        const int nblocks = (p.cell_total + kScanTile - 1) / kScanTile;
+        if (nblocks > kScanTile || nblocks < 1) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_scan_tiles, dim3(nblocks), dim3(kScanThreads), 0, stream, p.cell_ctr, p.frec_base,
+                           p.cell_total, p.scan_tmp, p.ctr_perm, p.cell_pt_count);
+    }
     hipLaunchKernelGGL(k_worklist, dim3(std::max(1, std::min((p.cell_total + kWorkThreads - 1) / kWorkThreads, 4096))),
                        dim3(kWorkThreads), 0, stream, p);
     const int pb = std::max(1, std::min(p.cell_total, p.persist_blocks));
